@@ -1,0 +1,235 @@
+/*
+ * dpd_hip.h -- C-ABI of the MI355X-native DPD tensor-contraction engine.
+ *
+ * This is the drop-in boundary for the reference's lib_cc contract/sort call
+ * surface and the libsrint engines it runs on.  Every entry point takes plain
+ * pointers and sizes; tensor data pointers are DEVICE pointers (HBM, FP64) on
+ * the context's device, exactly as the reference's engines take device
+ * pointers when igpu=1 (libsrint/vv2v_dpd_444.f90:45-46).  Host-side inputs
+ * are marked as such.
+ *
+ * Conventions (same as the reference):
+ *   irreps are 1-based; population arrays have 8 entries; integers are int64
+ *   (the reference builds with -i8); tensors are column-major DPD, blocks
+ *   ordered by right-pair irrep, left pair fastest; pair packing iop:
+ *   0 = p<q (antisymmetric), 1 = full, 2 = p<=q (symmetric).
+ *   Label strings are NOT NUL-terminated: 4 characters for a 4-index tensor,
+ *   2 for a 2-index one.
+ * Errors: every function returns 0 on success or a negative DPD_E* code; the
+ * reference's `write(6,*) ...; stop` becomes an error return with the message
+ * available from dpd_last_error() (per host thread).
+ * Threading: a context is not re-entrant; use one context per host thread.
+ * All device work is enqueued on the context's stream (dpd_ctx_set_stream);
+ * calls return when the work is enqueued unless stated otherwise.
+ */
+#ifndef DPD_HIP_H
+#define DPD_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DPD_OK = 0,
+  DPD_EINVAL = -1,     /* illegal argument (iop, anti, label string, sizes) */
+  DPD_ESYM = -2,       /* symmetry mismatch (e.g. Sym1*Sym2 != Sym3) */
+  DPD_EPOP = -3,       /* contracted-index populations differ */
+  DPD_EIOP = -4,       /* compressed pair used where it must be full (IOP barrier) */
+  DPD_ENOMEM = -5,     /* device allocation failed */
+  DPD_EHIP = -6,       /* HIP runtime error */
+  DPD_ENODEV = -7,     /* no usable gfx950 device */
+  DPD_ENOTCONV = -8    /* Davidson: not converged within the cycle limit (results still written) */
+};
+
+typedef struct dpd_ctx dpd_ctx;
+
+/* Descriptor of one DPD tensor: restates `type opt` (lib_cc/mod_operator.f90:8-25)
+ * with the storage pointer (`arr`) as a device pointer.  For 2-index operands
+ * (vv2v_424's v2, vv2v_442's v3) only pop[0..1], iop[0] and sym are read. */
+typedef struct {
+  int64_t pop[4][8]; /* popp, popq, popr, pops */
+  int64_t iop[2];    /* ioppq, ioprs */
+  int64_t sym;       /* 1-based irrep of the tensor */
+  double *data;      /* device pointer, FP64, length dpd_tensor_len() */
+} dpd_tensor;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the global mod_symm state (lib_mod/mod_symm.f): nirrep and the
+ * 8x8 direct-product table (host, row-major dirprd[i*8+j] = dirprd(i+1,j+1)).
+ * device = -1 creates a host-only context: metadata calls work, every call that
+ * needs the GPU returns DPD_ENODEV. */
+int dpd_ctx_create(dpd_ctx **ctx, int64_t nirrep, const int64_t *dirprd64, int device);
+int dpd_ctx_destroy(dpd_ctx *ctx);
+/* hipStream_t as void*; NULL = the default stream */
+int dpd_ctx_set_stream(dpd_ctx *ctx, void *stream);
+int dpd_ctx_synchronize(dpd_ctx *ctx);
+const char *dpd_last_error(void);
+/* bytes of device scratch the context currently holds (grows on demand) */
+size_t dpd_ctx_workspace_bytes(const dpd_ctx *ctx);
+
+/* ---- metadata (host, no device work) ---------------------------------- */
+/* libsrint/irpdso.f:1-51 */
+int64_t dpd_irpdso(const dpd_ctx *ctx, int64_t sym, const int64_t *popp, const int64_t *popq,
+                   int64_t iop);
+/* libsrint/idsymsoc.f:1-29 */
+int64_t dpd_idsymsoc(const dpd_ctx *ctx, int64_t irrepx, const int64_t *popp,
+                     const int64_t *popq, int64_t ioppq, const int64_t *popr,
+                     const int64_t *pops, int64_t ioprs);
+/* libsrint/isymoffso.f:1-53 (1-based offset) */
+int64_t dpd_isymoffso(const dpd_ctx *ctx, int64_t irrepq1, int64_t irrep, const int64_t *popp,
+                      const int64_t *popq, int64_t iop);
+/* length of a descriptor (lib_cc/mod_operator.f90:93, init_opt) */
+int64_t dpd_tensor_len(const dpd_ctx *ctx, const dpd_tensor *t);
+/* lib_cc/getidx.f90: 1-based linear index -> info8 = (irrp,irrq,irrr,irrs,p,q,r,s) */
+int dpd_getidx(const dpd_ctx *ctx, int64_t idx, int64_t *info8, const dpd_tensor *t);
+/* lib_cc/putidx.f90: info8 -> 1-based linear index (0 if not stored) */
+int64_t dpd_putidx(const dpd_ctx *ctx, const int64_t *info8, const dpd_tensor *t);
+
+/* ---- engines: v3 = fact1 * (v1 (x) v2) + fact2 * v3 -------------------- */
+/* libsrint/vv2v_dpd_444.f90:55-232 (routes to the fast path or the fallback) */
+int dpd_vv2v_dpd_444(dpd_ctx *ctx, const dpd_tensor *v1, const char *type1,
+                     const dpd_tensor *v2, const char *type2, dpd_tensor *v3,
+                     const char *type3, double fact1, double fact2);
+/* libsrint/vv2v_dpd_424.f90:21-  (v2 is 2-index) */
+int dpd_vv2v_dpd_424(dpd_ctx *ctx, const dpd_tensor *v1, const char *type1,
+                     const dpd_tensor *v2, const char *type2, dpd_tensor *v3,
+                     const char *type3, double fact1, double fact2);
+/* libsrint/vv2v_dpd_442.f90:22-  (v3 is 2-index) */
+int dpd_vv2v_dpd_442(dpd_ctx *ctx, const dpd_tensor *v1, const char *type1,
+                     const dpd_tensor *v2, const char *type2, dpd_tensor *v3,
+                     const char *type3, double fact1, double fact2);
+
+/* ---- contract surface: v3 += fact * P{v1 (x) v2} (lib_cc/mod_contract.f90) */
+/* Scratch (the reference's iCore/maxCore) comes from the context's workspace. */
+/* mod_contract.f90:10-162, anti = 0..3 antisymmetrises none/pq/rs/both of v3 */
+int dpd_vv2v_444(dpd_ctx *ctx, const dpd_tensor *v1, const dpd_tensor *v2, dpd_tensor *v3,
+                 double fact, const char *type1, const char *type2, const char *type3,
+                 int64_t anti);
+/* mod_contract.f90:166-180 (labels of 2 characters each) */
+int dpd_vv2v_222(dpd_ctx *ctx, const dpd_tensor *v1, const dpd_tensor *v2, dpd_tensor *v3,
+                 double fact, const char *type1, const char *type2, const char *type3);
+/* mod_contract.f90:184-199 (type1 4 chars, type2/type3 2 chars) */
+int dpd_vv2v_422(dpd_ctx *ctx, const dpd_tensor *v1, const dpd_tensor *v2, dpd_tensor *v3,
+                 double fact, const char *type1, const char *type2, const char *type3);
+/* mod_contract.f90:203-219 (type1/type2 2 chars, type3 4 chars) */
+int dpd_vv2v_224(dpd_ctx *ctx, const dpd_tensor *v1, const dpd_tensor *v2, dpd_tensor *v3,
+                 double fact, const char *type1, const char *type2, const char *type3,
+                 int64_t anti);
+/* mod_contract.f90:223-368, anti = 0..2 */
+int dpd_vv2v_424(dpd_ctx *ctx, const dpd_tensor *v1, const dpd_tensor *v2, dpd_tensor *v3,
+                 double fact, const char *type1, const char *type2, const char *type3,
+                 int64_t anti);
+/* mod_contract.f90:372-455 */
+int dpd_vv2v_442(dpd_ctx *ctx, const dpd_tensor *v1, const dpd_tensor *v2, dpd_tensor *v3,
+                 double fact, const char *type1, const char *type2, const char *type3);
+
+/* ---- sort kernels (device in/out, FP64) -------------------------------- */
+/* libsrint/iexpso.f90:24-120 */
+int dpd_iexpso(dpd_ctx *ctx, const double *in, double *out, const int64_t *popp,
+               const int64_t *popq, int64_t ioppq, int64_t iexppq, const int64_t *popr,
+               const int64_t *pops, int64_t ioprs, int64_t iexprs, int64_t sym);
+/* libsrint/icompso.f:1-93 */
+int dpd_icompso(dpd_ctx *ctx, const double *in, double *out, const int64_t *popp,
+                const int64_t *popq, int64_t ioppq, int64_t icpq, const int64_t *popr,
+                const int64_t *pops, int64_t ioprs, int64_t icrs, int64_t sym);
+/* libsrint/isymtrso.f:1-120 */
+int dpd_isymtrso(dpd_ctx *ctx, const double *in, double *out, const int64_t *popp,
+                 const int64_t *popq, int64_t ioppq, int64_t icpq, const int64_t *popr,
+                 const int64_t *pops, int64_t ioprs, int64_t icrs, int64_t sym);
+/* libsrint/itranspso.f:1-53 */
+int dpd_itranspso(dpd_ctx *ctx, const double *in, double *out, const int64_t *popp,
+                  const int64_t *popq, int64_t ioppq, const int64_t *popr, const int64_t *pops,
+                  int64_t ioprs, int64_t sym);
+/* libsrint/sstgenso.f90:22-45 (sstgenso1), type4 e.g. "3142" */
+int dpd_sstgenso(dpd_ctx *ctx, const double *win, double *wout, const int64_t *pop1,
+                 const int64_t *pop2, const int64_t *pop3, const int64_t *pop4, int64_t irrepx,
+                 const char *type4);
+
+/* ---- dense GEMM: libsrint/xgemmso.f:1-96 (XGEMMD / XGEMMDG / XGEMMSG) ---- */
+/* column-major C = alpha*op(A)*op(B) + beta*C on device FP64 arrays; prec 0 runs
+ * FP64 MFMA, prec 1 FP32 MFMA (operands rounded to FP32, FP32 accumulation);
+ * synchronises the stream before returning. */
+int dpd_xgemm(dpd_ctx *ctx, const char *transa, const char *transb, int64_t m, int64_t n,
+              int64_t k, double alpha, const double *A, int64_t lda, const double *B,
+              int64_t ldb, double beta, double *C, int64_t ldc, int prec);
+
+/* ---- Davidson vector primitives (device vectors, FP64) ----------------- */
+/* result written to a host double (synchronises the stream) */
+int dpd_dot(dpd_ctx *ctx, int64_t n, const double *x, const double *y, double *result);
+int dpd_nrm2(dpd_ctx *ctx, int64_t n, const double *x, double *result);
+/* idamax: 1-based index of the first max |x_i| and the signed value there */
+int dpd_amax(dpd_ctx *ctx, int64_t n, const double *x, int64_t *index, double *value);
+int dpd_axpy(dpd_ctx *ctx, int64_t n, double a, const double *x, double *y);
+int dpd_scal(dpd_ctx *ctx, int64_t n, double a, double *x);
+/* Davidson preconditioner (lib_cc/mod_davidson.f90:417-428):
+ * p_i = e_i / (w - diag_i) with |w - diag_i| clamped to 1e-4 (sign kept) */
+int dpd_precondition(dpd_ctx *ctx, int64_t n, double w, const double *diag, const double *e,
+                     double *p);
+
+/* ---- synthetic Hamiltonians (host; the pyscf2cc/getHF data path) -------- */
+/* Seeded synthetic spin-free DPD Hamiltonian: see dpd_synth_* in the source. */
+typedef struct {
+  int64_t nirrep;
+  int64_t nso[8];    /* spatial orbitals per irrep */
+  int64_t nalpha, nbeta;
+  uint64_t seed;
+  /* outputs (filled by dpd_synth_layout) */
+  int64_t popa[8], popb[8], vrta[8], vrtb[8];
+  int64_t popo[8], popv[8]; /* spin-free occupied / virtual per irrep (alpha then beta) */
+} dpd_synth_spec;
+int dpd_synth_layout(dpd_synth_spec *spec);
+/* Fill one integral tensor in host memory.  which: "fij","fab","fia","ijkl","ijka",
+ * "abij","iabj","abci","abcd" (layouts of CISdoo.SD.SF/allocate_mem.f90:17-54 with
+ * iop as given); eorb (host) receives the spin-free orbital energies when
+ * which == "eorb" (length sum(popo)+sum(popv), occupied irreps then virtual). */
+int dpd_synth_fill(const dpd_synth_spec *spec, const char *which, int64_t iop, double *host_out,
+                   int64_t len);
+
+/* ---- sigma builders and Davidson (device-resident) --------------------- */
+typedef struct dpd_problem dpd_problem;
+/* method: 0 = SF-CIS (EE-CIS.SF/calAR.f90), 1 = CIS(Doo) (CISdoo.SD.SF/calAR.f90),
+ *         2 = SF-CPSd = CIS(Doo) + doubles-doubles first-order coupling
+ *             (EE-CPSd.SF/calC2_VR2.f90)
+ * precision: 0 = FP64 MFMA, 1 = FP32 MFMA (FP32 operands, FP32 accumulate),
+ *            2 = BF16x3 split MFMA (FP32 accumulate) */
+int dpd_problem_create(dpd_ctx *ctx, const dpd_synth_spec *spec, int method, int64_t symR,
+                       dpd_problem **prob);
+int dpd_problem_destroy(dpd_problem *prob);
+int64_t dpd_problem_dim(const dpd_problem *prob);
+int dpd_problem_set_precision(dpd_problem *prob, int precision);
+/* sigma = J * R for nvec vectors stored contiguously (device, stride dim) */
+int dpd_problem_sigma(dpd_problem *prob, const double *R, double *sigma, int64_t nvec);
+/* diagonal preconditioner (device, length dim): (OED1, OED2) */
+int dpd_problem_diag(dpd_problem *prob, double *diag);
+/* spin-flip alpha->beta initial guesses (lib_cc/preDavidson.f90:120-223);
+ * writes nroot unit vectors (device, contiguous) and their OED energies (host) */
+int dpd_problem_guess(dpd_problem *prob, int64_t nroot, double *R, double *energies);
+
+typedef struct {
+  int64_t nroot;
+  int64_t max_dim;   /* davidson_dim (default 20) */
+  int64_t max_cyc;   /* davidson_cyc (default 50) */
+  double conv;       /* max-abs residual tolerance, 10^-davidson_conv */
+  int lowest;        /* 0: root tracking by overlap (default), 1: lowest */
+  int final_check;   /* 1: recompute |A R - w R|_inf for every root at the end */
+  /* outputs */
+  int64_t iterations;
+  int64_t sigma_calls;
+  double residual[64];
+  double final_residual[64];
+} dpd_davidson_opts;
+/* Subspace eigensolve (the dgeev call of lib_cc/mod_davidson.f90:186), host FP64:
+ * eigenvalues (wr, wi) and unit-norm right eigenvectors (columns of V, column
+ * major) of a real nonsymmetric n x n matrix given row-major. */
+int dpd_small_eig(int64_t n, const double *A_rowmajor, double *wr, double *wi, double *V_colmajor);
+/* Non-Hermitian Davidson (lib_cc/mod_davidson.f90:40-561) on the problem's
+ * operator; R (device, nroot*dim) holds the guesses on entry and the
+ * eigenvectors on exit; w (host, nroot) receives the eigenvalues. */
+int dpd_davidson(dpd_problem *prob, dpd_davidson_opts *opts, double *R, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

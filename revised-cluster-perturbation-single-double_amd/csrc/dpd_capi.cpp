@@ -1,0 +1,474 @@
+// dpd_capi.cpp -- extern "C" entry points of include/dpd_hip.h (context,
+// metadata, engines, contract surface, sort kernels, vector primitives).
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "dpd_engine.hpp"
+
+using namespace dpd;
+
+namespace dpd {
+const char* last_error();
+}
+
+#define CHECK_CTX(c) \
+  if (!(c)) return fail(DPD_EINVAL, "null context")
+
+static int set_device(dpd_ctx* c) {
+  if (c->device < 0) return fail(DPD_ENODEV, "host-only context (device -1) cannot run device work");
+  DPD_HIP_TRY(hipSetDevice(c->device));
+  return DPD_OK;
+}
+
+extern "C" {
+
+const char* dpd_last_error(void) { return dpd::last_error(); }
+
+int dpd_ctx_create(dpd_ctx** out, int64_t nirrep, const int64_t* dirprd64, int device) {
+  if (!out || !dirprd64 || nirrep < 1 || nirrep > 8) return fail(DPD_EINVAL, "dpd_ctx_create: bad arguments");
+  if (device == -1) {
+    // host-only context: metadata functions work, device work fails with DPD_ENODEV
+    dpd_ctx* c = new dpd_ctx();
+    c->sym.set((int)nirrep, dirprd64);
+    c->device = -1;
+    *out = c;
+    return DPD_OK;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+    return fail(DPD_ENODEV, "no HIP device %d available", device);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(DPD_ENODEV, "hipGetDeviceProperties failed");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(DPD_ENODEV, "device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
+  for (int i = 0; i < 8; i++)
+    for (int j = 0; j < 8; j++) {
+      int64_t v = dirprd64[i * 8 + j];
+      if ((i < nirrep && j < nirrep) && (v < 1 || v > nirrep))
+        return fail(DPD_EINVAL, "dirprd(%d,%d)=%lld out of range", i + 1, j + 1, (long long)v);
+    }
+  dpd_ctx* c = new dpd_ctx();
+  c->sym.set((int)nirrep, dirprd64);
+  c->device = device;
+  if (set_device(c)) {
+    delete c;
+    return DPD_EHIP;
+  }
+  int rc = c->red.alloc(sizeof(double) * (kRedBlocks * 72 + 64) + sizeof(int64_t) * (kRedBlocks + 8));
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return DPD_OK;
+}
+
+int dpd_ctx_destroy(dpd_ctx* c) {
+  if (!c) return DPD_OK;
+  if (c->device >= 0) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  delete c;
+  return DPD_OK;
+}
+
+int dpd_ctx_set_stream(dpd_ctx* c, void* stream) {
+  CHECK_CTX(c);
+  c->stream = (hipStream_t)stream;
+  return DPD_OK;
+}
+
+int dpd_ctx_synchronize(dpd_ctx* c) {
+  CHECK_CTX(c);
+  if (set_device(c)) return DPD_ENODEV;
+  DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  return DPD_OK;
+}
+
+size_t dpd_ctx_workspace_bytes(const dpd_ctx* c) { return c ? c->ws.bytes : 0; }
+
+// ---- metadata ---------------------------------------------------------------
+int64_t dpd_irpdso(const dpd_ctx* c, int64_t sym, const int64_t* popp, const int64_t* popq, int64_t iop) {
+  if (!c || sym < 1 || sym > c->sym.nirrep) return 0;
+  return c->sym.irpdso((int)sym, popp, popq, (int)iop);
+}
+
+int64_t dpd_idsymsoc(const dpd_ctx* c, int64_t irrepx, const int64_t* popp, const int64_t* popq, int64_t ioppq,
+                     const int64_t* popr, const int64_t* pops, int64_t ioprs) {
+  if (!c || irrepx < 1 || irrepx > c->sym.nirrep) return 0;
+  return c->sym.idsymsoc((int)irrepx, popp, popq, (int)ioppq, popr, pops, (int)ioprs);
+}
+
+int64_t dpd_isymoffso(const dpd_ctx* c, int64_t irrepq1, int64_t irrep, const int64_t* popp, const int64_t* popq,
+                      int64_t iop) {
+  if (!c) return 0;
+  return c->sym.isymoffso((int)irrepq1, (int)irrep, popp, popq, (int)iop);
+}
+
+int64_t dpd_tensor_len(const dpd_ctx* c, const dpd_tensor* t) {
+  if (!c || !t) return 0;
+  return c->sym.idsymsoc((int)t->sym, t->pop[0], t->pop[1], (int)t->iop[0], t->pop[2], t->pop[3], (int)t->iop[1]);
+}
+
+int dpd_getidx(const dpd_ctx* c, int64_t idx, int64_t* info8, const dpd_tensor* t) {
+  if (!c || !t || !info8) return fail(DPD_EINVAL, "dpd_getidx: null argument");
+  DevLayout L = make_layout(c->sym, Desc::from(t, 4));
+  if (idx < 1 || idx > L.len) return fail(DPD_EINVAL, "getidx: index %lld out of range", (long long)idx);
+  int h[4];
+  int64_t x[4];
+  decode_elem(L, idx - 1, h, x);
+  for (int i = 0; i < 4; i++) {
+    info8[i] = h[i] + 1;
+    info8[4 + i] = x[i] + 1;
+  }
+  return DPD_OK;
+}
+
+int64_t dpd_putidx(const dpd_ctx* c, const int64_t* info8, const dpd_tensor* t) {
+  if (!c || !t || !info8) return 0;
+  DevLayout L = make_layout(c->sym, Desc::from(t, 4));
+  double sg;
+  for (int i = 0; i < 4; i++)
+    if (info8[i] < 1 || info8[i] > c->sym.nirrep) return 0;
+  int64_t ad = fetch_addr(L, (int)info8[0] - 1, (int)info8[1] - 1, (int)info8[2] - 1, (int)info8[3] - 1,
+                          info8[4] - 1, info8[5] - 1, info8[6] - 1, info8[7] - 1, &sg);
+  // putidx addresses stored elements only (no mirroring)
+  if (ad < 0) return 0;
+  int h[4];
+  int64_t x[4];
+  decode_elem(L, ad, h, x);
+  for (int i = 0; i < 4; i++)
+    if (h[i] != info8[i] - 1 || x[i] != info8[4 + i] - 1) return 0;
+  return ad + 1;
+}
+
+// ---- engines ------------------------------------------------------------------
+static int check_iop(const dpd_tensor* t, int rank) {
+  for (int s = 0; s < (rank == 4 ? 2 : 1); s++)
+    if (t->iop[s] < 0 || t->iop[s] > 2) return fail(DPD_EINVAL, "iop must be 0, 1 or 2");
+  return DPD_OK;
+}
+
+static int make_operand(dpd_ctx* c, const dpd_tensor* t, const char* lab, int rank, Operand& o) {
+  if (!t) return fail(DPD_EINVAL, "null tensor");
+  int rc = check_iop(t, rank);
+  if (rc) return rc;
+  if (t->sym < 1 || t->sym > c->sym.nirrep) return fail(DPD_EINVAL, "tensor symmetry out of range");
+  if (!lab) return fail(DPD_EINVAL, "null label string");
+  rc = check_labels(lab, rank);
+  if (rc) return rc;
+  o.d = Desc::from(t, rank);
+  o.rank = rank;
+  std::memset(o.lab, 0, 4);
+  std::memcpy(o.lab, lab, rank);
+  o.data = t->data;
+  return DPD_OK;
+}
+
+static int engine_entry(dpd_ctx* c, int kind, const dpd_tensor* v1, const char* t1, const dpd_tensor* v2,
+                        const char* t2, dpd_tensor* v3, const char* t3, double f1, double f2) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  Operand o1, o2, o3;
+  int r2 = (kind == 424) ? 2 : 4, r3 = (kind == 442) ? 2 : 4;
+  int rc = make_operand(c, v1, t1, 4, o1);
+  if (!rc) rc = make_operand(c, v2, t2, r2, o2);
+  if (!rc) rc = make_operand(c, v3, t3, r3, o3);
+  if (rc) return rc;
+  return engine_call(c, kind, o1, o2, o3, f1, f2, 1);
+}
+
+int dpd_vv2v_dpd_444(dpd_ctx* c, const dpd_tensor* v1, const char* t1, const dpd_tensor* v2, const char* t2,
+                     dpd_tensor* v3, const char* t3, double f1, double f2) {
+  return engine_entry(c, 444, v1, t1, v2, t2, v3, t3, f1, f2);
+}
+int dpd_vv2v_dpd_424(dpd_ctx* c, const dpd_tensor* v1, const char* t1, const dpd_tensor* v2, const char* t2,
+                     dpd_tensor* v3, const char* t3, double f1, double f2) {
+  return engine_entry(c, 424, v1, t1, v2, t2, v3, t3, f1, f2);
+}
+int dpd_vv2v_dpd_442(dpd_ctx* c, const dpd_tensor* v1, const char* t1, const dpd_tensor* v2, const char* t2,
+                     dpd_tensor* v3, const char* t3, double f1, double f2) {
+  return engine_entry(c, 442, v1, t1, v2, t2, v3, t3, f1, f2);
+}
+
+// ---- contract surface ---------------------------------------------------------
+static int contract_entry(dpd_ctx* c, int kind, const dpd_tensor* v1, const char* t1, int r1,
+                          const dpd_tensor* v2, const char* t2, int r2, dpd_tensor* v3, const char* t3,
+                          int r3, double fact, int64_t anti) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  int maxanti = (kind == 444) ? 3 : (kind == 424 ? 2 : 0);
+  if (anti < 0 || anti > maxanti)
+    return fail(DPD_EINVAL, "ERROR: vv2v_%d error in antisymmetrize. %lld", kind, (long long)anti);
+  Operand o1, o2, o3;
+  int rc = make_operand(c, v1, t1, r1, o1);
+  if (!rc) rc = make_operand(c, v2, t2, r2, o2);
+  if (!rc) rc = make_operand(c, v3, t3, r3, o3);
+  if (rc) return rc;
+  return contract_call(c, kind, o1, o2, o3, fact, (int)anti, 1, PREC_F64);
+}
+
+int dpd_vv2v_444(dpd_ctx* c, const dpd_tensor* v1, const dpd_tensor* v2, dpd_tensor* v3, double fact,
+                 const char* t1, const char* t2, const char* t3, int64_t anti) {
+  return contract_entry(c, 444, v1, t1, 4, v2, t2, 4, v3, t3, 4, fact, anti);
+}
+
+// mod_contract.f90:166-219: the rank-2 wrappers pad with '=','+','-'
+int dpd_vv2v_222(dpd_ctx* c, const dpd_tensor* v1, const dpd_tensor* v2, dpd_tensor* v3, double fact,
+                 const char* t1, const char* t2, const char* t3) {
+  if (!t1 || !t2 || !t3) return fail(DPD_EINVAL, "null label string");
+  char a[4] = {t1[0], t1[1], '=', '+'}, b[4] = {t2[0], t2[1], '=', '-'}, d[4] = {t3[0], t3[1], '+', '-'};
+  return contract_entry(c, 444, v1, a, 4, v2, b, 4, v3, d, 4, fact, 0);
+}
+int dpd_vv2v_422(dpd_ctx* c, const dpd_tensor* v1, const dpd_tensor* v2, dpd_tensor* v3, double fact,
+                 const char* t1, const char* t2, const char* t3) {
+  if (!t1 || !t2 || !t3) return fail(DPD_EINVAL, "null label string");
+  char b[4] = {t2[0], t2[1], '+', '-'}, d[4] = {t3[0], t3[1], '+', '-'};
+  return contract_entry(c, 444, v1, t1, 4, v2, b, 4, v3, d, 4, fact, 0);
+}
+int dpd_vv2v_224(dpd_ctx* c, const dpd_tensor* v1, const dpd_tensor* v2, dpd_tensor* v3, double fact,
+                 const char* t1, const char* t2, const char* t3, int64_t anti) {
+  if (!t1 || !t2 || !t3) return fail(DPD_EINVAL, "null label string");
+  char a[4] = {t1[0], t1[1], '+', '-'}, b[4] = {t2[0], t2[1], '+', '-'};
+  return contract_entry(c, 444, v1, a, 4, v2, b, 4, v3, t3, 4, fact, anti);
+}
+int dpd_vv2v_424(dpd_ctx* c, const dpd_tensor* v1, const dpd_tensor* v2, dpd_tensor* v3, double fact,
+                 const char* t1, const char* t2, const char* t3, int64_t anti) {
+  return contract_entry(c, 424, v1, t1, 4, v2, t2, 2, v3, t3, 4, fact, anti);
+}
+int dpd_vv2v_442(dpd_ctx* c, const dpd_tensor* v1, const dpd_tensor* v2, dpd_tensor* v3, double fact,
+                 const char* t1, const char* t2, const char* t3) {
+  return contract_entry(c, 442, v1, t1, 4, v2, t2, 4, v3, t3, 2, fact, 0);
+}
+
+// ---- sort kernels -------------------------------------------------------------
+static Desc mk(const int64_t* p, const int64_t* q, const int64_t* r, const int64_t* s, int i1, int i2, int sym) {
+  Desc d;
+  std::memcpy(d.pop[0], p, 64);
+  std::memcpy(d.pop[1], q, 64);
+  std::memcpy(d.pop[2], r, 64);
+  std::memcpy(d.pop[3], s, 64);
+  d.iop[0] = i1;
+  d.iop[1] = i2;
+  d.sym = sym;
+  return d;
+}
+
+static bool same8(const int64_t* a, const int64_t* b) { return std::memcmp(a, b, 64) == 0; }
+
+static int run_sort(dpd_ctx* c, const Desc& dout, const Desc& din, const int8_t* perm, const double* in,
+                    double* out) {
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  if (din.sym < 1 || din.sym > c->sym.nirrep) return fail(DPD_EINVAL, "symmetry out of range");
+  MapArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.out = make_layout(c->sym, dout);
+  a.in = make_layout(c->sym, din);
+  a.nterm = 1;
+  for (int i = 0; i < 4; i++) a.term[0].perm[i] = perm[i];
+  a.term[0].coef = 1.0;
+  a.alpha = 1.0;
+  a.beta = 0.0;
+  a.nbatch = 1;
+  return gather(c, a, in, out);
+}
+
+int dpd_iexpso(dpd_ctx* c, const double* in, double* out, const int64_t* popp, const int64_t* popq, int64_t ioppq,
+               int64_t iexppq, const int64_t* popr, const int64_t* pops, int64_t ioprs, int64_t iexprs, int64_t sym) {
+  CHECK_CTX(c);
+  if (ioppq < 0 || ioppq > 2) return fail(DPD_EINVAL, "fatal error: ioppq");
+  if (ioprs < 0 || ioprs > 2) return fail(DPD_EINVAL, "fatal error: ioprs");
+  if (iexppq == 1 && ioppq != 1 && !same8(popp, popq)) return fail(DPD_EINVAL, "fatal error: popp /= popq");
+  if (iexprs == 1 && ioprs != 1 && !same8(popr, pops)) return fail(DPD_EINVAL, "fatal error: popr /= pops");
+  bool ok = (iexppq == 1 && ioppq == 0 && iexprs == 0) || (ioprs == 0 && iexprs == 1 && iexppq == 0) ||
+            (iexppq == 1 && ioppq == 2 && iexprs == 0) || (ioprs == 2 && iexprs == 1 && iexppq == 0) ||
+            (iexppq == 1 && ioppq != 1 && iexprs == 1 && ioprs != 1);
+  if (!ok) return fail(DPD_EINVAL, "fatal error: iexpso must expand exactly one side, or both packed sides");
+  Desc din = mk(popp, popq, popr, pops, (int)ioppq, (int)ioprs, (int)sym);
+  Desc dout = mk(popp, popq, popr, pops, iexppq ? 1 : (int)ioppq, iexprs ? 1 : (int)ioprs, (int)sym);
+  const int8_t id[4] = {0, 1, 2, 3};
+  return run_sort(c, dout, din, id, in, out);
+}
+
+int dpd_icompso(dpd_ctx* c, const double* in, double* out, const int64_t* popp, const int64_t* popq,
+                int64_t ioppq, int64_t icpq, const int64_t* popr, const int64_t* pops, int64_t ioprs, int64_t icrs,
+                int64_t sym) {
+  CHECK_CTX(c);
+  if (ioppq < 0 || ioppq > 2 || ioprs < 0 || ioprs > 2) return fail(DPD_EINVAL, "error of iop in icompso");
+  bool ok = (icpq == 0 && icrs == 1) || (icpq == 1 && icrs == 0) || (icpq == 2 && icrs == 1) ||
+            (icpq == 1 && icrs == 2);
+  if (!ok) return fail(DPD_EINVAL, "attention!!!, error in icompso");
+  if ((icpq != 1 && ioppq != 1) || (icrs != 1 && ioprs != 1))
+    return fail(DPD_EINVAL, "icompso: the side being compressed must be full (iop=1)");
+  Desc din = mk(popp, popq, popr, pops, (int)ioppq, (int)ioprs, (int)sym);
+  Desc dout = mk(popp, popq, popr, pops, icpq == 1 ? (int)ioppq : (int)icpq, icrs == 1 ? (int)ioprs : (int)icrs,
+                 (int)sym);
+  const int8_t id[4] = {0, 1, 2, 3};
+  return run_sort(c, dout, din, id, in, out);
+}
+
+int dpd_isymtrso(dpd_ctx* c, const double* in, double* out, const int64_t* popp, const int64_t* popq,
+                 int64_t ioppq, int64_t icpq, const int64_t* popr, const int64_t* pops, int64_t ioprs, int64_t icrs,
+                 int64_t sym) {
+  CHECK_CTX(c);
+  if (ioppq < 0 || ioppq > 2 || ioprs < 0 || ioprs > 2) return fail(DPD_EINVAL, "error of iop in isymtrso");
+  if ((icpq == 1 && ioppq != 1) || (icrs == 1 && ioprs != 1))
+    return fail(DPD_EINVAL, "isymtrso: a swapped pair must be full (iop=1)");
+  Desc din = mk(popp, popq, popr, pops, (int)ioppq, (int)ioprs, (int)sym);
+  Desc dout = mk(icpq == 1 ? popq : popp, icpq == 1 ? popp : popq, icrs == 1 ? pops : popr,
+                 icrs == 1 ? popr : pops, (int)ioppq, (int)ioprs, (int)sym);
+  const int8_t perm[4] = {(int8_t)(icpq == 1 ? 1 : 0), (int8_t)(icpq == 1 ? 0 : 1), (int8_t)(icrs == 1 ? 3 : 2),
+                          (int8_t)(icrs == 1 ? 2 : 3)};
+  return run_sort(c, dout, din, perm, in, out);
+}
+
+int dpd_itranspso(dpd_ctx* c, const double* in, double* out, const int64_t* popp, const int64_t* popq,
+                  int64_t ioppq, const int64_t* popr, const int64_t* pops, int64_t ioprs, int64_t sym) {
+  CHECK_CTX(c);
+  if (ioppq < 0 || ioppq > 2 || ioprs < 0 || ioprs > 2) return fail(DPD_EINVAL, "error of iop in itranspso");
+  Desc din = mk(popp, popq, popr, pops, (int)ioppq, (int)ioprs, (int)sym);
+  Desc dout = mk(popr, pops, popp, popq, (int)ioprs, (int)ioppq, (int)sym);
+  const int8_t perm[4] = {2, 3, 0, 1};
+  return run_sort(c, dout, din, perm, in, out);
+}
+
+int dpd_sstgenso(dpd_ctx* c, const double* win, double* wout, const int64_t* pop1, const int64_t* pop2,
+                 const int64_t* pop3, const int64_t* pop4, int64_t irrepx, const char* type4) {
+  CHECK_CTX(c);
+  if (!type4) return fail(DPD_EINVAL, "null type");
+  int8_t perm[4];
+  const int64_t* pin[4] = {pop1, pop2, pop3, pop4};
+  int seen = 0;
+  for (int i = 0; i < 4; i++) {
+    int p = type4[i] - '1';
+    if (p < 0 || p > 3 || (seen & (1 << p)))
+      return fail(DPD_EINVAL, "fatal error in sstgenso: invalid type received: %.4s", type4);
+    seen |= 1 << p;
+    perm[i] = (int8_t)p;
+  }
+  Desc din = mk(pop1, pop2, pop3, pop4, 1, 1, (int)irrepx);
+  Desc dout = mk(pin[perm[0]], pin[perm[1]], pin[perm[2]], pin[perm[3]], 1, 1, (int)irrepx);
+  return run_sort(c, dout, din, perm, win, wout);
+}
+
+// ---- dense GEMM (libsrint/xgemmso.f:1-96 router: XGEMMD/XGEMMDG/XGEMMSG) ----------
+int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int64_t n, int64_t k, double alpha,
+              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+              int prec) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  if (!transa || !transb) return fail(DPD_EINVAL, "null trans flag");
+  char ta = transa[0], tb = transb[0];
+  if ((ta != 'N' && ta != 'T') || (tb != 'N' && tb != 'T')) return fail(DPD_EINVAL, "trans must be 'N' or 'T'");
+  if (m < 0 || n < 0 || k < 0 || m > INT32_MAX || n > INT32_MAX || k > INT32_MAX)
+    return fail(DPD_EINVAL, "bad GEMM sizes");
+  if (ldc < std::max<int64_t>(1, m) || lda < std::max<int64_t>(1, ta == 'N' ? m : k) ||
+      ldb < std::max<int64_t>(1, tb == 'N' ? k : n))
+    return fail(DPD_EINVAL, "bad leading dimension");
+  if (prec != PREC_F64 && prec != PREC_F32) return fail(DPD_EINVAL, "precision %d is not available", prec);
+  if (m == 0 || n == 0) return DPD_OK;
+  GemmPlan p;
+  GemmTask tk;
+  std::memset(&tk, 0, sizeof tk);
+  tk.M = (int32_t)m;
+  tk.N = (int32_t)n;
+  tk.md[0] = (int32_t)m;
+  tk.md[1] = 1;
+  tk.nd[0] = (int32_t)n;
+  tk.nd[1] = 1;
+  tk.sc_m[0] = 1;
+  tk.sc_n[0] = ldc;
+  GemmSeg sg;
+  std::memset(&sg, 0, sizeof sg);
+  sg.K = (int32_t)k;
+  sg.kd[0] = (int32_t)std::max<int64_t>(k, 1);
+  sg.kd[1] = 1;
+  sg.sa_m[0] = ta == 'N' ? 1 : lda;
+  sg.sa_k[0] = ta == 'N' ? lda : 1;
+  sg.sb_k[0] = tb == 'N' ? 1 : ldb;
+  sg.sb_n[0] = tb == 'N' ? ldb : 1;
+  sg.ktiles = (int32_t)((k + 15) / 16);
+  tk.nseg = 1;
+  tk.ktiles_total = sg.ktiles;
+  tk.tiles_m = (int32_t)((m + 63) / 64);
+  tk.tiles_n = (int32_t)((n + 63) / 64);
+  tk.ksplit = 1;
+  p.tasks.push_back(tk);
+  p.segs.push_back(sg);
+  p.nwg = tk.tiles_m * tk.tiles_n;
+  int rc = p.upload();
+  if (rc) return rc;
+  rc = run_gemm(c, p, A, 0, B, 0, C, alpha, beta, prec);
+  if (rc) return rc;
+  // the plan's device arrays are freed on return: wait for the launch
+  DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  return DPD_OK;
+}
+
+// ---- vector primitives ----------------------------------------------------------
+static double* red_d(dpd_ctx* c) { return (double*)c->red.p; }
+
+int dpd_dot(dpd_ctx* c, int64_t n, const double* x, const double* y, double* result) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  double* part = red_d(c);
+  double* out = part + kRedBlocks * 64;
+  DPD_HIP_TRY(launch_multidot(n, x, y, 0, 1, part, out, c->stream));
+  DPD_HIP_TRY(hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  return DPD_OK;
+}
+
+int dpd_nrm2(dpd_ctx* c, int64_t n, const double* x, double* result) {
+  double s;
+  int rc = dpd_dot(c, n, x, x, &s);
+  if (rc) return rc;
+  *result = std::sqrt(s);
+  return DPD_OK;
+}
+
+int dpd_amax(dpd_ctx* c, int64_t n, const double* x, int64_t* index, double* value) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  if (n <= 0) {
+    *index = 0;
+    *value = 0.0;
+    return DPD_OK;
+  }
+  double* pv = red_d(c);
+  int64_t* pi = (int64_t*)(pv + kRedBlocks * 72 + 64);
+  double* ov = pv + kRedBlocks * 64 + 8;
+  int64_t* oi = pi + kRedBlocks;
+  DPD_HIP_TRY(launch_amax(n, x, pv, pi, ov, oi, c->stream));
+  DPD_HIP_TRY(hipMemcpyAsync(value, ov, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DPD_HIP_TRY(hipMemcpyAsync(index, oi, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  *index += 1;
+  return DPD_OK;
+}
+
+int dpd_axpy(dpd_ctx* c, int64_t n, double a, const double* x, double* y) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  DPD_HIP_TRY(launch_axpy(n, a, x, y, c->stream));
+  return DPD_OK;
+}
+
+int dpd_scal(dpd_ctx* c, int64_t n, double a, double* x) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  DPD_HIP_TRY(launch_scal(n, a, x, c->stream));
+  return DPD_OK;
+}
+
+int dpd_precondition(dpd_ctx* c, int64_t n, double w, const double* diag, const double* e, double* p) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  DPD_HIP_TRY(launch_precondition(n, w, diag, e, p, c->stream));
+  return DPD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,917 @@
+// dpd_engine.cpp -- contraction planner and mod_contract surface on the GPU.
+//
+// A contraction v3 = f1 * v1 (x) v2 + f2 * v3 over DPD tensors is turned into
+// one grouped-GEMM launch:
+//   * every tensor dimension becomes a "mode": a single orbital index, or a
+//     packed pair (iop 0/2) that travels intact between two tensors (the
+//     reference's fast paths and macro pairs, vv2v_dpd_444_d.f90:120-235,
+//     vv2v_dpd_442.f90:150-200); a batch of Davidson vectors is one more mode;
+//   * modes shared by v1 and v3 form the GEMM rows (M), v2 and v3 the columns
+//     (N), v1 and v2 the reduction (K);
+//   * for every irrep assignment of the M and N modes (one output block) the
+//     planner lists every compatible irrep assignment of the K modes as a K
+//     segment; inside a segment each tensor is affine in every index (the
+//     reference fallback's "task" decomposition, vv2v_dpd_444_d.f90:256-517),
+//     so the kernel addresses A, B, C with mixed-radix strides and never packs.
+// Plans hold no data pointers and are cached per signature.
+#include "dpd_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+namespace dpd {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+const char* last_error() { return g_err.c_str(); }
+
+int DevBuf::alloc(size_t b) {
+  reset();
+  if (b == 0) return DPD_OK;
+  if (hipMalloc(&p, b) != hipSuccess) {
+    p = nullptr;
+    return fail(DPD_ENOMEM, "hipMalloc of %zu bytes failed", b);
+  }
+  bytes = b;
+  return DPD_OK;
+}
+
+int ws_reserve(dpd_ctx* c, size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  if (c->ws.bytes >= c->ws_top + bytes) return DPD_OK;
+  if (c->ws_top != 0)
+    return fail(DPD_ENOMEM, "workspace must be reserved before it is used (internal)");
+  DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  size_t nb = std::max(bytes, c->ws.bytes * 2);
+  return c->ws.alloc(nb);
+}
+
+void* ws_take(dpd_ctx* c, size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  void* p = (char*)c->ws.p + c->ws_top;
+  c->ws_top += bytes;
+  return p;
+}
+
+int check_labels(const char* t, int n) {
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++)
+      if (t[i] == t[j]) return fail(DPD_EINVAL, "label string '%.*s' repeats an index", n, t);
+  return DPD_OK;
+}
+
+// label mapper of libsrint/vv2v_dpd_444.f90:88-133: outputs -> 1,2,5,6,
+// contracted -> 3,4 in order of appearance in type1
+int map444(const char* type1, const char* type2, const char* type3, char* t1, char* t2, char* t3) {
+  char from[6], to[6];
+  int nmap = 4;
+  const char outs[4] = {'1', '2', '5', '6'};
+  for (int i = 0; i < 4; i++) {
+    from[i] = type3[i];
+    to[i] = outs[i];
+  }
+  for (int i = 0; i < 4; i++) {
+    bool contracted = true;
+    for (int j = 0; j < 4; j++)
+      if (type1[i] == type3[j]) contracted = false;
+    if (contracted) {
+      if (nmap >= 6) return fail(DPD_EINVAL, "vv2v_dpd_444: more than two contracted indices");
+      from[nmap] = type1[i];
+      to[nmap] = nmap == 4 ? '3' : '4';
+      nmap++;
+    }
+  }
+  if (nmap != 6) return fail(DPD_EINVAL, "vv2v_dpd_444: exactly two contracted indices are required");
+  std::memcpy(t1, type1, 4);
+  std::memcpy(t2, type2, 4);
+  std::memcpy(t3, type3, 4);
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < nmap; j++) {
+      if (type1[i] == from[j]) t1[i] = to[j];
+      if (type2[i] == from[j]) t2[i] = to[j];
+      if (type3[i] == from[j]) t3[i] = to[j];
+    }
+  return DPD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// planner
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kMaxMode = 14;
+
+struct Mode {
+  char lab[2];
+  bool pair = false;
+  bool batch = false;
+  int64_t size[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int cls = -1;  // 0 M, 1 N, 2 K, 3 dummy (1-sized)
+  int64_t total() const {
+    int64_t t = 0;
+    for (int h = 0; h < 8; h++) t += size[h];
+    return t;
+  }
+};
+
+struct TV {
+  Desc d;
+  char lab[4];
+  int mode_of_dim[4];
+  int pair_mode[2];
+  int batch_mode = -1;
+  int64_t bstride = 0;
+  DevLayout L;
+};
+
+int find_mode(const std::vector<Mode>& m, const char* lab, bool pair) {
+  for (size_t i = 0; i < m.size(); i++) {
+    if (m[i].pair != pair || m[i].batch) continue;
+    if (m[i].lab[0] == lab[0] && (!pair || m[i].lab[1] == lab[1])) return (int)i;
+  }
+  return -1;
+}
+
+void pad_labels(TV& t, const Operand& o, char d0, char d1) {
+  t.d = o.d;
+  if (o.rank == 2) {
+    for (int k = 2; k < 4; k++)
+      for (int h = 0; h < 8; h++) t.d.pop[k][h] = (h == 0);
+    t.d.iop[1] = 1;
+    t.lab[0] = o.lab[0];
+    t.lab[1] = o.lab[1];
+    t.lab[2] = d0;
+    t.lab[3] = d1;
+  } else {
+    std::memcpy(t.lab, o.lab, 4);
+  }
+}
+
+struct Affine {
+  int64_t base = 0;
+  int64_t stride[kMaxMode];
+  bool valid = false;
+};
+
+bool tensor_sym_ok(const Sym& S, const TV& t, const int* mirr) {
+  int x = 0;
+  for (int s = 0; s < 2; s++) {
+    if (t.pair_mode[s] >= 0) x = S.dp[x][mirr[t.pair_mode[s]]];
+    else x = S.dp[S.dp[x][mirr[t.mode_of_dim[2 * s]]]][mirr[t.mode_of_dim[2 * s + 1]]];
+  }
+  return x == t.d.sym - 1;
+}
+
+Affine tensor_affine(const Sym& S, const TV& t, const int* mirr, int nmode) {
+  Affine a;
+  for (int i = 0; i < nmode; i++) a.stride[i] = 0;
+  int h[4] = {0, 0, 0, 0};
+  int xl, xr;
+  if (t.pair_mode[0] >= 0) {
+    xl = mirr[t.pair_mode[0]];
+  } else {
+    h[0] = mirr[t.mode_of_dim[0]];
+    h[1] = mirr[t.mode_of_dim[1]];
+    xl = S.dp[h[0]][h[1]];
+  }
+  if (t.pair_mode[1] >= 0) {
+    xr = mirr[t.pair_mode[1]];
+  } else {
+    h[2] = mirr[t.mode_of_dim[2]];
+    h[3] = mirr[t.mode_of_dim[3]];
+    xr = S.dp[h[2]][h[3]];
+  }
+  if (S.dp[xl][xr] != t.d.sym - 1) return a;
+  const DevLayout& L = t.L;
+  const int64_t nl = L.nL[xl];
+  a.base = L.blk[xr];
+  if (t.pair_mode[0] >= 0) {
+    a.stride[t.pair_mode[0]] += 1;
+  } else {
+    a.base += L.offL[xl][h[1]];
+    a.stride[t.mode_of_dim[0]] += 1;
+    a.stride[t.mode_of_dim[1]] += L.pop[0][h[0]];
+  }
+  if (t.pair_mode[1] >= 0) {
+    a.stride[t.pair_mode[1]] += nl;
+  } else {
+    a.base += nl * (int64_t)L.offR[xr][h[3]];
+    a.stride[t.mode_of_dim[2]] += nl;
+    a.stride[t.mode_of_dim[3]] += nl * (int64_t)L.pop[2][h[2]];
+  }
+  if (t.batch_mode >= 0) a.stride[t.batch_mode] += t.bstride;
+  a.valid = true;
+  return a;
+}
+
+}  // namespace
+
+int GemmPlan::upload() {
+  size_t bt = tasks.size() * sizeof(GemmTask);
+  size_t bs = segs.size() * sizeof(GemmSeg);
+  size_t bp = split.size() * sizeof(int32_t);
+  size_t tot = ((bt + 255) & ~size_t(255)) + ((bs + 255) & ~size_t(255)) + bp + 256;
+  int rc = dev.alloc(tot);
+  if (rc) return rc;
+  char* base = (char*)dev.p;
+  d_tasks = (GemmTask*)base;
+  d_segs = (GemmSeg*)(base + ((bt + 255) & ~size_t(255)));
+  d_split = (int32_t*)((char*)d_segs + ((bs + 255) & ~size_t(255)));
+  if (bt) DPD_HIP_TRY(hipMemcpy(d_tasks, tasks.data(), bt, hipMemcpyHostToDevice));
+  if (bs) DPD_HIP_TRY(hipMemcpy(d_segs, segs.data(), bs, hipMemcpyHostToDevice));
+  if (bp) DPD_HIP_TRY(hipMemcpy(d_split, split.data(), bp, hipMemcpyHostToDevice));
+  return DPD_OK;
+}
+
+int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Operand& o3,
+                    int nbatch, GemmPlan& plan) {
+  TV T[3];
+  pad_labels(T[0], o1, '\x01', '\x02');
+  pad_labels(T[1], o2, '\x03', '\x04');
+  pad_labels(T[2], o3, '\x05', '\x06');
+  std::vector<Mode> modes;
+  for (int t = 0; t < 3; t++) {
+    TV& tv = T[t];
+    for (int s = 0; s < 2; s++) {
+      tv.pair_mode[s] = -1;
+      if (tv.d.iop[s] != 1) {
+        char lab[2] = {tv.lab[2 * s], tv.lab[2 * s + 1]};
+        int id = find_mode(modes, lab, true);
+        int64_t sz[8];
+        for (int x = 1; x <= 8; x++)
+          sz[x - 1] = (x <= S.nirrep) ? S.irpdso(x, tv.d.pop[2 * s], tv.d.pop[2 * s + 1], tv.d.iop[s]) : 0;
+        if (id < 0) {
+          Mode m;
+          m.lab[0] = lab[0];
+          m.lab[1] = lab[1];
+          m.pair = true;
+          std::memcpy(m.size, sz, sizeof sz);
+          modes.push_back(m);
+          id = (int)modes.size() - 1;
+        } else {
+          for (int h = 0; h < 8; h++)
+            if (modes[id].size[h] != sz[h])
+              return fail(DPD_EPOP, "packed pair '%c%c' has mismatched populations or iop", lab[0], lab[1]);
+        }
+        tv.pair_mode[s] = id;
+        tv.mode_of_dim[2 * s] = tv.mode_of_dim[2 * s + 1] = id;
+      } else {
+        for (int k = 0; k < 2; k++) {
+          char lab[2] = {tv.lab[2 * s + k], 0};
+          int id = find_mode(modes, lab, false);
+          if (id < 0) {
+            Mode m;
+            m.lab[0] = lab[0];
+            m.lab[1] = 0;
+            for (int h = 0; h < 8; h++) m.size[h] = (h < S.nirrep) ? tv.d.pop[2 * s + k][h] : 0;
+            modes.push_back(m);
+            id = (int)modes.size() - 1;
+          } else {
+            for (int h = 0; h < S.nirrep; h++)
+              if (modes[id].size[h] != tv.d.pop[2 * s + k][h])
+                return fail(DPD_EPOP, "index '%c' has mismatched populations in two tensors", lab[0]);
+          }
+          tv.mode_of_dim[2 * s + k] = id;
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < modes.size(); i++)
+    if (modes[i].pair)
+      for (size_t j = 0; j < modes.size(); j++)
+        if (!modes[j].pair && (modes[j].lab[0] == modes[i].lab[0] || modes[j].lab[0] == modes[i].lab[1]))
+          return fail(DPD_EIOP,
+                      "IOP barrier: packed pair '%c%c' is split or reordered; expand it to iop=1 first",
+                      modes[i].lab[0], modes[i].lab[1]);
+  // batch mode
+  int bm = -1;
+  if (nbatch > 1) {
+    Mode m;
+    m.lab[0] = '#';
+    m.lab[1] = 0;
+    m.batch = true;
+    m.size[0] = nbatch;
+    modes.push_back(m);
+    bm = (int)modes.size() - 1;
+    int nb = 0;
+    for (int t = 0; t < 2; t++) {
+      const Operand& o = t == 0 ? o1 : o2;
+      if (o.batched) {
+        T[t].batch_mode = bm;
+        T[t].bstride = o.bstride;
+        nb++;
+      }
+    }
+    if (nb != 1) return fail(DPD_EINVAL, "exactly one input operand must be batched");
+    T[2].batch_mode = bm;
+    T[2].bstride = o3.bstride;
+  }
+  const int nm = (int)modes.size();
+  if (nm > kMaxMode) return fail(DPD_EINVAL, "too many indices in contraction");
+  for (int i = 0; i < nm; i++) {
+    bool in[3] = {false, false, false};
+    for (int t = 0; t < 3; t++) {
+      for (int d = 0; d < 4; d++)
+        if (T[t].mode_of_dim[d] == i) in[t] = true;
+      if (T[t].batch_mode == i) in[t] = true;
+    }
+    Mode& m = modes[i];
+    if (!m.batch && m.total() == 1 && m.size[0] == 1) m.cls = 3;
+    else if (in[0] && in[2] && !in[1]) m.cls = 0;
+    else if (in[1] && in[2] && !in[0]) m.cls = 1;
+    else if (in[0] && in[1] && !in[2]) m.cls = 2;
+    else if (in[0] && in[1] && in[2])
+      return fail(DPD_EINVAL, "index '%c' appears in all three tensors", m.lab[0]);
+    else
+      return fail(DPD_EINVAL, "index '%c' appears in one tensor only", m.lab[0]);
+  }
+  for (int t = 0; t < 3; t++) T[t].L = make_layout(S, T[t].d);
+
+  // digit order: M and N in order of appearance in v3 (coalesced C), batch last
+  std::vector<int> idM, idN, idK, idDum;
+  for (int d = 0; d < 4; d++) {
+    int id = T[2].mode_of_dim[d];
+    if (modes[id].cls == 0 && std::find(idM.begin(), idM.end(), id) == idM.end()) idM.push_back(id);
+    if (modes[id].cls == 1 && std::find(idN.begin(), idN.end(), id) == idN.end()) idN.push_back(id);
+  }
+  for (int d = 0; d < 4; d++) {
+    int id = T[0].mode_of_dim[d];
+    if (modes[id].cls == 2 && std::find(idK.begin(), idK.end(), id) == idK.end()) idK.push_back(id);
+  }
+  for (int i = 0; i < nm; i++) {
+    if (modes[i].cls == 3) idDum.push_back(i);
+    if (modes[i].batch) (modes[i].cls == 0 ? idM : idN).push_back(i);
+  }
+  if (idM.size() > 3 || idN.size() > 3 || idK.size() > 3)
+    return fail(DPD_EINVAL, "contraction needs more than three digits per GEMM dimension");
+
+  // enumerate
+  std::vector<int> ext;
+  for (int id : idM)
+    if (!modes[id].batch) ext.push_back(id);
+  for (int id : idN)
+    if (!modes[id].batch) ext.push_back(id);
+  const int nir = S.nirrep;
+  int64_t ecount = 1, kcount = 1;
+  for (size_t i = 0; i < ext.size(); i++) ecount *= nir;
+  for (size_t i = 0; i < idK.size(); i++) kcount *= nir;
+  int mirr[kMaxMode];
+  for (int i = 0; i < nm; i++) mirr[i] = 0;
+
+  auto size_of = [&](int id) -> int64_t { return modes[id].batch ? nbatch : modes[id].size[mirr[id]]; };
+
+  plan.tasks.clear();
+  plan.segs.clear();
+  plan.split.clear();
+  plan.flops = 0;
+  for (int64_t e = 0; e < ecount; e++) {
+    int64_t r = e;
+    bool skip = false;
+    for (int id : ext) {
+      mirr[id] = (int)(r % nir);
+      r /= nir;
+      if (modes[id].size[mirr[id]] == 0) skip = true;
+    }
+    if (skip) continue;
+    for (int id : idK) mirr[id] = 0;
+    if (!tensor_sym_ok(S, T[2], mirr)) continue;
+    Affine a3 = tensor_affine(S, T[2], mirr, nm);
+    if (!a3.valid) continue;
+    GemmTask tk;
+    std::memset(&tk, 0, sizeof tk);
+    int64_t M = 1, N = 1;
+    for (size_t i = 0; i < 3; i++) {
+      int64_t szm = i < idM.size() ? size_of(idM[i]) : 1;
+      int64_t szn = i < idN.size() ? size_of(idN[i]) : 1;
+      if (i < 2) {
+        tk.md[i] = (int32_t)szm;
+        tk.nd[i] = (int32_t)szn;
+      }
+      tk.sc_m[i] = i < idM.size() ? a3.stride[idM[i]] : 0;
+      tk.sc_n[i] = i < idN.size() ? a3.stride[idN[i]] : 0;
+      M *= szm;
+      N *= szn;
+    }
+    if (M == 0 || N == 0) continue;
+    if (M > INT32_MAX || N > INT32_MAX) return fail(DPD_EINVAL, "GEMM block too large");
+    tk.M = (int32_t)M;
+    tk.N = (int32_t)N;
+    tk.c_off = a3.base;
+    tk.seg_begin = (int32_t)plan.segs.size();
+    int ktile = 0;
+    for (int64_t kk = 0; kk < kcount; kk++) {
+      int64_t rr = kk;
+      bool sk = false;
+      for (int id : idK) {
+        mirr[id] = (int)(rr % nir);
+        rr /= nir;
+        if (modes[id].size[mirr[id]] == 0) sk = true;
+      }
+      if (sk) continue;
+      if (!tensor_sym_ok(S, T[0], mirr) || !tensor_sym_ok(S, T[1], mirr)) continue;
+      Affine a1 = tensor_affine(S, T[0], mirr, nm), a2 = tensor_affine(S, T[1], mirr, nm);
+      if (!a1.valid || !a2.valid) continue;
+      GemmSeg sg;
+      std::memset(&sg, 0, sizeof sg);
+      int64_t K = 1;
+      for (size_t i = 0; i < 3; i++) {
+        int64_t szk = i < idK.size() ? modes[idK[i]].size[mirr[idK[i]]] : 1;
+        if (i < 2) sg.kd[i] = (int32_t)szk;
+        sg.sa_k[i] = i < idK.size() ? a1.stride[idK[i]] : 0;
+        sg.sb_k[i] = i < idK.size() ? a2.stride[idK[i]] : 0;
+        sg.sa_m[i] = i < idM.size() ? a1.stride[idM[i]] : 0;
+        sg.sb_n[i] = i < idN.size() ? a2.stride[idN[i]] : 0;
+        K *= szk;
+      }
+      if (K == 0) continue;
+      if (K > INT32_MAX) return fail(DPD_EINVAL, "GEMM reduction too large");
+      sg.K = (int32_t)K;
+      sg.a_off = a1.base;
+      sg.b_off = a2.base;
+      sg.ktiles = (int32_t)((K + 15) / 16);
+      sg.ktile_begin = ktile;
+      ktile += sg.ktiles;
+      plan.segs.push_back(sg);
+      plan.flops += 2 * M * N * K;
+    }
+    tk.nseg = (int32_t)plan.segs.size() - tk.seg_begin;
+    tk.ktiles_total = ktile;
+    tk.tiles_m = (int32_t)((M + 63) / 64);
+    tk.tiles_n = (int32_t)((N + 63) / 64);
+    tk.ksplit = 1;
+    plan.tasks.push_back(tk);
+  }
+  // tasks without any K segment still apply beta (C = f2*C): give them one empty
+  // segment so the kernel writes alpha*0 + beta*C.
+  for (auto& tk : plan.tasks) {
+    if (tk.nseg == 0) {
+      GemmSeg sg;
+      std::memset(&sg, 0, sizeof sg);
+      sg.kd[0] = sg.kd[1] = 1;
+      sg.K = 0;
+      sg.ktiles = 0;
+      tk.seg_begin = (int32_t)plan.segs.size();
+      tk.nseg = 1;
+      plan.segs.push_back(sg);
+    }
+  }
+  // split-K: balance K tiles so the launch fills the chip
+  int64_t work = 0, tiles = 0;
+  for (auto& tk : plan.tasks) {
+    work += (int64_t)tk.tiles_m * tk.tiles_n * std::max(tk.ktiles_total, 1);
+    tiles += (int64_t)tk.tiles_m * tk.tiles_n;
+  }
+  const int64_t target = 1024;
+  int64_t per_wg = std::max<int64_t>(8, (work + target - 1) / target);
+  int64_t partial = 0;
+  int max_mn = 0;
+  int wg = 0;
+  for (size_t i = 0; i < plan.tasks.size(); i++) {
+    GemmTask& tk = plan.tasks[i];
+    int ks = 1;
+    if (tiles < target && tk.ktiles_total > per_wg) {
+      ks = (int)std::min<int64_t>(64, (tk.ktiles_total + per_wg - 1) / per_wg);
+      if (ks < 1) ks = 1;
+    }
+    tk.ksplit = ks;
+    if (ks > 1) {
+      tk.part_off = partial;
+      partial += (int64_t)ks * tk.M * tk.N;
+      plan.split.push_back((int32_t)i);
+      max_mn = std::max<int64_t>(max_mn, (int64_t)tk.M * tk.N);
+    }
+    tk.wg_begin = wg;
+    wg += tk.tiles_m * tk.tiles_n * ks;
+  }
+  plan.nwg = wg;
+  plan.partial_elems = partial;
+  plan.max_split_mn = max_mn;
+  (void)idDum;
+  // host-side bounds proof: every address a task or segment can form lies
+  // inside its tensor (all strides are non-negative)
+  int64_t lim[3];
+  for (int t = 0; t < 3; t++) lim[t] = T[t].L.len + (T[t].batch_mode >= 0 ? (int64_t)(nbatch - 1) * T[t].bstride : 0);
+  auto span = [](int64_t n, const int32_t* d, const int64_t* s) {
+    int64_t d2 = n / ((int64_t)d[0] * d[1]);
+    return (d[0] - 1) * s[0] + (d[1] - 1) * s[1] + (d2 - 1) * s[2];
+  };
+  for (const auto& tk : plan.tasks) {
+    if ((int64_t)tk.md[0] * tk.md[1] * (tk.M / ((int64_t)tk.md[0] * tk.md[1])) != tk.M ||
+        (int64_t)tk.nd[0] * tk.nd[1] * (tk.N / ((int64_t)tk.nd[0] * tk.nd[1])) != tk.N)
+      return fail(DPD_EINVAL, "internal: GEMM digit sizes do not factor M/N");
+    if (tk.c_off < 0 || tk.c_off + span(tk.M, tk.md, tk.sc_m) + span(tk.N, tk.nd, tk.sc_n) >= lim[2])
+      return fail(DPD_EINVAL, "internal: output block out of bounds");
+    for (int si = tk.seg_begin; si < tk.seg_begin + tk.nseg; si++) {
+      const GemmSeg& sg = plan.segs[si];
+      if (sg.K == 0) continue;
+      if ((int64_t)sg.kd[0] * sg.kd[1] * (sg.K / ((int64_t)sg.kd[0] * sg.kd[1])) != sg.K)
+        return fail(DPD_EINVAL, "internal: GEMM digit sizes do not factor K");
+      if (sg.a_off < 0 || sg.a_off + span(tk.M, tk.md, sg.sa_m) + span(sg.K, sg.kd, sg.sa_k) >= lim[0])
+        return fail(DPD_EINVAL, "internal: A segment out of bounds");
+      if (sg.b_off < 0 || sg.b_off + span(sg.K, sg.kd, sg.sb_k) + span(tk.N, tk.nd, sg.sb_n) >= lim[1])
+        return fail(DPD_EINVAL, "internal: B segment out of bounds");
+    }
+  }
+  return plan.upload();
+}
+
+int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
+             double f1, double f2, int prec) {
+  if (p.nwg == 0) return DPD_OK;
+  GemmLaunch g;
+  std::memset(&g, 0, sizeof g);
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.tasks = p.d_tasks;
+  g.segs = p.d_segs;
+  g.ntask = (int)p.tasks.size();
+  g.nwg = p.nwg;
+  g.alpha = f1;
+  g.beta = f2;
+  g.prec = prec;
+  g.a_f32 = a_f32;
+  g.b_f32 = b_f32;
+  g.split_tasks = p.d_split;
+  g.nsplit = (int)p.split.size();
+  g.max_split_mn = p.max_split_mn;
+  if (p.partial_elems > 0) g.partial = (double*)ws_take(c, sizeof(double) * p.partial_elems);
+  DPD_HIP_TRY(launch_gemm(g, c->stream));
+  return DPD_OK;
+}
+
+int gather(dpd_ctx* c, MapArgs a, const double* in, double* out) {
+  DPD_HIP_TRY(launch_gather_map(a, in, out, c->stream));
+  return DPD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// engine level: libsrint/vv2v_dpd_{444,424,442}.f90 dispatcher checks
+// ---------------------------------------------------------------------------
+static bool pops_eq(const int64_t* a, const int64_t* b) {
+  for (int i = 0; i < 8; i++)
+    if (a[i] != b[i]) return false;
+  return true;
+}
+
+static int check_engine(const Sym& S, int kind, const Operand& o1, const Operand& o2,
+                        const Operand& o3, Operand& m1, Operand& m2, Operand& m3) {
+  m1 = o1;
+  m2 = o2;
+  m3 = o3;
+  if (kind == 444) {
+    int rc = map444(o1.lab, o2.lab, o3.lab, m1.lab, m2.lab, m3.lab);
+    if (rc) return rc;
+    // contracted populations (vv2v_dpd_444.f90:152-174)
+    for (char cc : {'3', '4'}) {
+      int p1 = -1, p2 = -1;
+      for (int i = 0; i < 4; i++) {
+        if (m1.lab[i] == cc) p1 = i;
+        if (m2.lab[i] == cc) p2 = i;
+      }
+      if (p1 >= 0 && p2 >= 0 && !pops_eq(o1.d.pop[p1], o2.d.pop[p2]))
+        return fail(DPD_EPOP, "FATAL ERROR in vv2v_dpd_444: contracted index populations differ (%.4s * %.4s -> %.4s)",
+                    o1.lab, o2.lab, o3.lab);
+    }
+    // fast-path detection (vv2v_dpd_444.f90:177-187) and the IOP wall (:189-201)
+    const char* t1 = m1.lab;
+    const char* t2 = m2.lab;
+    const char* t3 = m3.lab;
+    const int* i1 = o1.d.iop;
+    const int* i2 = o2.d.iop;
+    const int* i3 = o3.d.iop;
+    bool fast = false;
+    if (!std::memcmp(t3, "1256", 4)) {
+      if (!std::memcmp(t1, "3412", 4) && !std::memcmp(t2, "3456", 4))
+        fast = i1[0] == i2[0] && i1[1] == i3[0] && i2[1] == i3[1];
+      else if (!std::memcmp(t1, "1234", 4) && !std::memcmp(t2, "3456", 4))
+        fast = i1[1] == i2[0] && i1[0] == i3[0] && i2[1] == i3[1];
+      else if (!std::memcmp(t1, "1234", 4) && !std::memcmp(t2, "5634", 4))
+        fast = i1[1] == i2[1] && i1[0] == i3[0] && i2[0] == i3[1];
+      else if (!std::memcmp(t1, "3412", 4) && !std::memcmp(t2, "5634", 4))
+        fast = i1[0] == i2[1] && i1[1] == i3[0] && i2[0] == i3[1];
+    }
+    if (!fast && (i1[0] != 1 || i1[1] != 1 || i2[0] != 1 || i2[1] != 1 || i3[0] != 1 || i3[1] != 1))
+      return fail(DPD_EIOP, "FATAL ERROR in vv2v_dpd_444: the fallback path forbids compressed IOP (%.4s * %.4s -> %.4s)",
+                  o1.lab, o2.lab, o3.lab);
+    return DPD_OK;
+  }
+  if (S.D(o1.d.sym, o2.d.sym) != o3.d.sym)
+    return fail(DPD_ESYM, "FATAL ERROR in vv2v_dpd_%d: Sym1*Sym2 != Sym3", kind);
+  if (kind == 424) {
+    int ncon = 0;
+    char cc = 0;
+    for (int i = 0; i < 4; i++) {
+      bool in3 = false;
+      for (int j = 0; j < 4; j++)
+        if (o1.lab[i] == o3.lab[j]) in3 = true;
+      if (!in3) {
+        ncon++;
+        cc = o1.lab[i];
+      }
+    }
+    if (ncon != 1) return fail(DPD_EINVAL, "FATAL ERROR: vv2v_dpd_424 requires EXACTLY 1 contracted index.");
+    if (o2.lab[0] != cc && o2.lab[1] != cc)
+      return fail(DPD_EINVAL, "FATAL ERROR: contracted index '%c' missing in V2.", cc);
+    // IOP barrier (vv2v_dpd_424.f90:164-196)
+    for (int s = 0; s < 2; s++) {
+      if (o1.d.iop[s] != 1) {
+        bool ok = false;
+        for (int u = 0; u < 2; u++)
+          if (!std::memcmp(o1.lab + 2 * s, o3.lab + 2 * u, 2) && o1.d.iop[s] == o3.d.iop[u]) ok = true;
+        if (!ok) return fail(DPD_EIOP, "FATAL ERROR in vv2v_dpd_424: IOP barrier");
+      }
+      if (o3.d.iop[s] != 1) {
+        bool ok = false;
+        for (int u = 0; u < 2; u++)
+          if (!std::memcmp(o3.lab + 2 * s, o1.lab + 2 * u, 2) && o3.d.iop[s] == o1.d.iop[u]) ok = true;
+        if (!std::memcmp(o3.lab + 2 * s, o2.lab, 2) && o3.d.iop[s] == o2.d.iop[0]) ok = true;
+        if (!ok) return fail(DPD_EIOP, "FATAL ERROR in vv2v_dpd_424: IOP barrier");
+      }
+    }
+    if (o2.d.iop[0] != 1) {
+      bool ok = false;
+      for (int u = 0; u < 2; u++)
+        if (!std::memcmp(o2.lab, o3.lab + 2 * u, 2) && o2.d.iop[0] == o3.d.iop[u]) ok = true;
+      if (!ok) return fail(DPD_EIOP, "FATAL ERROR in vv2v_dpd_424: IOP barrier");
+    }
+    return DPD_OK;
+  }
+  // 442
+  int ncon = 0;
+  for (int i = 0; i < 4; i++) {
+    if (o1.lab[i] == o3.lab[0] || o1.lab[i] == o3.lab[1]) continue;
+    ncon++;
+    bool in2 = false;
+    for (int j = 0; j < 4; j++)
+      if (o2.lab[j] == o1.lab[i]) in2 = true;
+    if (!in2) return fail(DPD_EINVAL, "FATAL ERROR: Contracted index missing in V2.");
+  }
+  if (ncon != 3) return fail(DPD_EINVAL, "FATAL ERROR: vv2v_dpd_442 requires EXACTLY 3 contracted indices.");
+  return DPD_OK;
+}
+
+static std::string sig_of(int kind, const Operand& o1, const Operand& o2, const Operand& o3,
+                          int nbatch, int anti, int extra) {
+  std::string s;
+  auto put = [&](const void* p, size_t n) { s.append((const char*)p, n); };
+  put(&kind, sizeof kind);
+  for (const Operand* o : {&o1, &o2, &o3}) {
+    put(o->d.pop, sizeof o->d.pop);
+    put(o->d.iop, sizeof o->d.iop);
+    put(&o->d.sym, sizeof o->d.sym);
+    put(o->lab, 4);
+    put(&o->rank, sizeof o->rank);
+    put(&o->bstride, sizeof o->bstride);
+    bool b = o->batched;
+    put(&b, 1);
+  }
+  put(&nbatch, sizeof nbatch);
+  put(&anti, sizeof anti);
+  put(&extra, sizeof extra);
+  return s;
+}
+
+int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, const Operand& o3,
+                double f1, double f2, int nbatch) {
+  Operand m1, m2, m3;
+  int rc = check_engine(c->sym, kind, o1, o2, o3, m1, m2, m3);
+  if (rc) return rc;
+  std::string key = sig_of(kind, m1, m2, m3, nbatch, -1, 0);
+  auto it = c->engine_cache.find(key);
+  GemmPlan* p;
+  if (it == c->engine_cache.end()) {
+    auto np = std::make_unique<GemmPlan>();
+    rc = build_gemm_plan(c->sym, m1, m2, m3, nbatch, *np);
+    if (rc) return rc;
+    p = np.get();
+    c->engine_cache.emplace(key, std::move(np));
+  } else {
+    p = it->second.get();
+  }
+  ws_reset(c);
+  rc = ws_reserve(c, sizeof(double) * (p->partial_elems + 32));
+  if (rc) return rc;
+  return run_gemm(c, *p, o1.data, 0, o2.data, 0, const_cast<double*>(o3.data), f1, f2, PREC_F64);
+}
+
+// ---------------------------------------------------------------------------
+// contract surface: lib_cc/mod_contract.f90
+// ---------------------------------------------------------------------------
+static bool flat_pair(const char (*pairs)[2], int n, int self) {
+  for (int i = 0; i < n; i++) {
+    if (i == self) continue;
+    if (pairs[i][0] == pairs[self][0] && pairs[i][1] == pairs[self][1]) return false;
+  }
+  return true;
+}
+
+static MapArgs identity_map(const Sym& S, const Desc& dout, const Desc& din) {
+  MapArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.out = make_layout(S, dout);
+  a.in = make_layout(S, din);
+  a.nterm = 1;
+  a.term[0].perm[0] = 0;
+  a.term[0].perm[1] = 1;
+  a.term[0].perm[2] = 2;
+  a.term[0].perm[3] = 3;
+  a.term[0].coef = 1.0;
+  a.alpha = 1.0;
+  a.beta = 0.0;
+  a.nbatch = 1;
+  return a;
+}
+
+// mod_contract.f90:111-160 fused: v3 += X - P(pq)X - P(rs)X + P(pq)P(rs)X,
+// gathered straight into v3's (possibly packed) layout
+static MapArgs anti_map(const Sym& S, const Desc& d3, const Desc& dx, bool apq, bool ars) {
+  MapArgs a = identity_map(S, d3, dx);
+  a.nterm = 0;
+  for (int i = 0; i < (apq ? 2 : 1); i++)
+    for (int j = 0; j < (ars ? 2 : 1); j++) {
+      MapTerm& t = a.term[a.nterm++];
+      t.perm[0] = i ? 1 : 0;
+      t.perm[1] = i ? 0 : 1;
+      t.perm[2] = j ? 3 : 2;
+      t.perm[3] = j ? 2 : 3;
+      t.coef = ((i + j) & 1) ? -1.0 : 1.0;
+    }
+  a.beta = 1.0;
+  return a;
+}
+
+static int build_contract_plan(const Sym& S, int kind, const Operand& o1, const Operand& o2,
+                               const Operand& o3, int anti, int nbatch, ContractPlan& P) {
+  P.kind = kind;
+  P.anti = anti;
+  P.nbatch = nbatch;
+  // pairs whose labels must be flat (check_flat, mod_contract.f90:493-510)
+  int iexp1[2] = {0, 0}, iexp2[2] = {0, 0}, iexp3[2] = {0, 0};
+  if (kind == 444) {
+    char pairs[6][2] = {{o1.lab[0], o1.lab[1]}, {o1.lab[2], o1.lab[3]}, {o2.lab[0], o2.lab[1]},
+                        {o2.lab[2], o2.lab[3]}, {o3.lab[0], o3.lab[1]}, {o3.lab[2], o3.lab[3]}};
+    int iop[6] = {o1.d.iop[0], o1.d.iop[1], o2.d.iop[0], o2.d.iop[1], o3.d.iop[0], o3.d.iop[1]};
+    int* ex[6] = {&iexp1[0], &iexp1[1], &iexp2[0], &iexp2[1], &iexp3[0], &iexp3[1]};
+    for (int i = 0; i < 6; i++)
+      if (flat_pair(pairs, 6, i) && iop[i] != 1) *ex[i] = 1;
+    if (anti == 0 && (iexp3[0] || iexp3[1]))
+      return fail(DPD_EINVAL, "vv2v_444: an output pair must be expanded but anti=0");
+  } else if (kind == 424) {
+    char pairs[4][2] = {{o1.lab[0], o1.lab[1]}, {o1.lab[2], o1.lab[3]}, {o3.lab[0], o3.lab[1]},
+                        {o3.lab[2], o3.lab[3]}};
+    int iop[4] = {o1.d.iop[0], o1.d.iop[1], o3.d.iop[0], o3.d.iop[1]};
+    int* ex[4] = {&iexp1[0], &iexp1[1], &iexp3[0], &iexp3[1]};
+    for (int i = 0; i < 4; i++)
+      if (flat_pair(pairs, 4, i) && iop[i] != 1) *ex[i] = 1;
+    if (iexp3[0] && anti != 1)
+      return fail(DPD_EINVAL, "ERROR: vv2v_424 error for the 1st pair not antisymmertize but be expand.");
+    if (iexp3[1] && anti != 2)
+      return fail(DPD_EINVAL, "ERROR: vv2v_424 error for the 2nd pair not antisymmertize but be expand.");
+  } else {
+    char pairs[4][2] = {{o1.lab[0], o1.lab[1]}, {o1.lab[2], o1.lab[3]}, {o2.lab[0], o2.lab[1]},
+                        {o2.lab[2], o2.lab[3]}};
+    int iop[4] = {o1.d.iop[0], o1.d.iop[1], o2.d.iop[0], o2.d.iop[1]};
+    int* ex[4] = {&iexp1[0], &iexp1[1], &iexp2[0], &iexp2[1]};
+    for (int i = 0; i < 4; i++)
+      if (flat_pair(pairs, 4, i) && iop[i] != 1) *ex[i] = 1;
+  }
+  // expansions (iexpso semantics, popp==popq required)
+  auto expand = [&](const Operand& o, int* ie, bool& flag, Desc& dx, MapArgs& m, int64_t& len) -> int {
+    dx = o.d;
+    flag = ie[0] || ie[1];
+    if (!flag) return DPD_OK;
+    if (ie[0] && !pops_eq(o.d.pop[0], o.d.pop[1])) return fail(DPD_EINVAL, "fatal error: popp /= popq");
+    if (ie[1] && !pops_eq(o.d.pop[2], o.d.pop[3])) return fail(DPD_EINVAL, "fatal error: popr /= pops");
+    if (ie[0]) dx.iop[0] = 1;
+    if (ie[1]) dx.iop[1] = 1;
+    m = identity_map(S, dx, o.d);
+    len = m.out.len;
+    return DPD_OK;
+  };
+  int rc = expand(o1, iexp1, P.exp1, P.d1x, P.map1, P.len1x);
+  if (rc) return rc;
+  if (kind != 424) {
+    rc = expand(o2, iexp2, P.exp2, P.d2x, P.map2, P.len2x);
+    if (rc) return rc;
+  } else {
+    P.d2x = o2.d;
+  }
+  P.d3x = o3.d;
+  if (iexp3[0]) P.d3x.iop[0] = 1;
+  if (iexp3[1]) P.d3x.iop[1] = 1;
+  bool apq = (kind == 444) ? (anti == 1 || anti == 3) : (kind == 424 && anti == 1);
+  bool ars = (kind == 444) ? (anti == 2 || anti == 3) : (kind == 424 && anti == 2);
+  if (apq && (P.d3x.iop[0] != 1 || !pops_eq(o3.d.pop[0], o3.d.pop[1])))
+    return fail(DPD_EINVAL, "ERROR: vv2v_%d anti 1st pair error.", kind);
+  if (ars && (P.d3x.iop[1] != 1 || !pops_eq(o3.d.pop[2], o3.d.pop[3])))
+    return fail(DPD_EINVAL, "ERROR: vv2v_%d anti 2nd pair error.", kind);
+  if (anti != 0) {
+    P.epi = anti_map(S, o3.d, P.d3x, apq, ars);
+    P.len3x = P.epi.in.len;
+  }
+  // batch strides of the scratch copies
+  P.batch1 = o1.batched;
+  P.batch2 = o2.batched;
+  P.bstride1 = P.exp1 ? P.len1x : o1.bstride;
+  P.bstride2 = P.exp2 ? P.len2x : o2.bstride;
+  P.bstride3 = anti != 0 ? P.len3x : o3.bstride;
+  Operand e1 = o1, e2 = o2, e3 = o3;
+  e1.d = P.d1x;
+  e2.d = P.d2x;
+  e3.d = P.d3x;
+  e1.bstride = P.bstride1;
+  e2.bstride = P.bstride2;
+  e3.bstride = P.bstride3;
+  Operand m1, m2, m3;
+  rc = check_engine(S, kind, e1, e2, e3, m1, m2, m3);
+  if (rc) return rc;
+  return build_gemm_plan(S, m1, m2, m3, nbatch, P.gemm);
+}
+
+int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
+                  double fact, int anti, int nbatch, int prec, int64_t* flops) {
+  std::string key = sig_of(kind, o1, o2, o3, nbatch, anti, 0);
+  auto it = c->contract_cache.find(key);
+  ContractPlan* P;
+  if (it == c->contract_cache.end()) {
+    auto np = std::make_unique<ContractPlan>();
+    int rc = build_contract_plan(c->sym, kind, o1, o2, o3, anti, nbatch, *np);
+    if (rc) return rc;
+    P = np.get();
+    c->contract_cache.emplace(key, std::move(np));
+  } else {
+    P = it->second.get();
+  }
+  if (flops) *flops += P->gemm.flops;
+  const int nb1 = o1.batched ? nbatch : 1, nb2 = o2.batched ? nbatch : 1;
+  size_t need = 0;
+  if (P->exp1) need += sizeof(double) * P->len1x * nb1 + 256;
+  if (P->exp2) need += sizeof(double) * P->len2x * nb2 + 256;
+  if (anti) need += sizeof(double) * P->len3x * nbatch + 256;
+  need += sizeof(double) * P->gemm.partial_elems + 256;
+  ws_reset(c);
+  int rc = ws_reserve(c, need);
+  if (rc) return rc;
+  const double* p1 = o1.data;
+  const double* p2 = o2.data;
+  const float* p1f = o1.data_f32;
+  const float* p2f = o2.data_f32;
+  if (P->exp1) {
+    double* x = (double*)ws_take(c, sizeof(double) * P->len1x * nb1);
+    MapArgs m = P->map1;
+    m.nbatch = nb1;
+    m.in_bstride = o1.bstride;
+    m.out_bstride = P->len1x;
+    rc = gather(c, m, o1.data, x);
+    if (rc) return rc;
+    p1 = x;
+    p1f = nullptr;
+  }
+  if (P->exp2) {
+    double* x = (double*)ws_take(c, sizeof(double) * P->len2x * nb2);
+    MapArgs m = P->map2;
+    m.nbatch = nb2;
+    m.in_bstride = o2.bstride;
+    m.out_bstride = P->len2x;
+    rc = gather(c, m, o2.data, x);
+    if (rc) return rc;
+    p2 = x;
+    p2f = nullptr;
+  }
+  double* p3 = const_cast<double*>(o3.data);
+  if (anti) {
+    p3 = (double*)ws_take(c, sizeof(double) * P->len3x * nbatch);
+    DPD_HIP_TRY(launch_fill(P->len3x * nbatch, 0.0, p3, c->stream));
+  }
+  int use_f32 = (prec != PREC_F64);
+  const void* A = (use_f32 && p1f) ? (const void*)p1f : (const void*)p1;
+  const void* B = (use_f32 && p2f) ? (const void*)p2f : (const void*)p2;
+  rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, 1.0,
+                use_f32 ? PREC_F32 : PREC_F64);
+  if (rc) return rc;
+  if (anti) {
+    MapArgs m = P->epi;
+    m.nbatch = nbatch;
+    m.in_bstride = P->len3x;
+    m.out_bstride = o3.bstride;
+    rc = gather(c, m, p3, const_cast<double*>(o3.data));
+    if (rc) return rc;
+  }
+  return DPD_OK;
+}
+
+}  // namespace dpd

@@ -1,0 +1,125 @@
+// dpd_engine.hpp -- context, device memory and the contraction planner.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "dpd_common.hpp"
+#include "dpd_kernels.hpp"
+
+struct dpd_ctx;
+
+namespace dpd {
+
+#define DPD_HIP_TRY(expr)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return fail(DPD_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// device buffer owned by RAII
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t b);
+};
+
+// One GEMM stage of a contraction, independent of the data pointers.
+struct GemmPlan {
+  std::vector<GemmTask> tasks;
+  std::vector<GemmSeg> segs;
+  std::vector<int32_t> split;
+  int nwg = 0;
+  int max_split_mn = 0;
+  int64_t partial_elems = 0;
+  int64_t flops = 0;  // 2*M*N*K summed (algorithmic, per launch)
+  DevBuf dev;         // tasks | segs | split
+  GemmTask* d_tasks = nullptr;
+  GemmSeg* d_segs = nullptr;
+  int32_t* d_split = nullptr;
+  int upload();
+};
+
+// Operand of a contraction: descriptor (rank 2 or 4), labels, data, and an
+// optional batch of `nbatch` tensors `bstride` elements apart.
+struct Operand {
+  Desc d;
+  int rank = 4;
+  char lab[4];
+  const double* data = nullptr;
+  const float* data_f32 = nullptr;  // optional FP32 copy (mixed precision)
+  int64_t bstride = 0;
+  bool batched = false;
+};
+
+// Build the grouped-GEMM plan of v3 = f1*(v1 x v2) + f2*v3 (labels already checked).
+int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Operand& o3,
+                    int nbatch, GemmPlan& plan);
+
+// Full description of a mod_contract call (vv2v_444 / 424 / 442), reusable.
+struct ContractPlan {
+  int kind = 444;  // 444, 424, 442
+  int anti = 0;
+  // expansions of operands (iexpso semantics)
+  bool exp1 = false, exp2 = false;
+  Desc d1x, d2x;    // descriptors after expansion
+  Desc d3x;         // descriptor of the scratch result (anti != 0)
+  MapArgs map1, map2;  // expansion gathers
+  MapArgs epi;         // antisymmetrise + compress + accumulate into v3
+  GemmPlan gemm;
+  int64_t len1x = 0, len2x = 0, len3x = 0;
+  int nbatch = 1;
+  bool batch1 = false, batch2 = false;
+  int64_t bstride1 = 0, bstride2 = 0, bstride3 = 0;
+};
+
+}  // namespace dpd
+
+// the opaque context of the C-ABI
+struct dpd_ctx {
+  dpd::Sym sym;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t max_threads_block = 256;
+  // scratch: grow-only device workspace (bump allocated per top-level call)
+  dpd::DevBuf ws;
+  size_t ws_top = 0;
+  // small device scratch for reductions
+  dpd::DevBuf red;
+  // plan caches
+  std::unordered_map<std::string, std::unique_ptr<dpd::ContractPlan>> contract_cache;
+  std::unordered_map<std::string, std::unique_ptr<dpd::GemmPlan>> engine_cache;
+  std::vector<char> pinned_scratch;
+};
+
+namespace dpd {
+// workspace helpers: reserve before use (may synchronise + reallocate)
+int ws_reserve(dpd_ctx* c, size_t bytes);
+void* ws_take(dpd_ctx* c, size_t bytes);
+inline void ws_reset(dpd_ctx* c) { c->ws_top = 0; }
+
+// engine level (libsrint vv2v_dpd_* semantics)
+int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, const Operand& o3,
+                double f1, double f2, int nbatch);
+// contract surface (mod_contract semantics) with optional batching of one operand
+int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
+                  double fact, int anti, int nbatch, int prec, int64_t* flops = nullptr);
+int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
+             double f1, double f2, int prec);
+int gather(dpd_ctx* c, MapArgs a, const double* in, double* out);
+// labels / checks shared by the C-ABI and the sigma builders
+int check_labels(const char* t, int n);
+int map444(const char* type1, const char* type2, const char* type3, char* t1, char* t2, char* t3);
+}  // namespace dpd

@@ -1,0 +1,589 @@
+// dpd_kernels.hip -- hand-written gfx950 (CDNA4) kernels of the DPD engine.
+//
+//  * k_gather_map: one generic, element-parallel DPD sort kernel.  Each thread
+//    owns one OUTPUT element (coalesced writes), decodes its irreps/indices from
+//    the output layout, permutes them and gathers up to four signed terms from
+//    the input layout.  It implements iexpso/icompso/isymtrso/itranspso/sstgenso
+//    and the fused antisymmetrise+compress+accumulate epilogue of mod_contract.
+//  * k_gemm: grouped GEMM over DPD irrep tasks on MFMA (FP64
+//    v_mfma_f64_16x16x4f64 or FP32 v_mfma_f32_16x16x4f32), 64x64x16 LDS tiles,
+//    4 waves of 32x32, K segments per task (several irrep tuples accumulate into
+//    one output block) and deterministic split-K.
+//  * reductions for the Davidson vector algebra (FP64): multi-dot, max-abs,
+//    axpy/scal/lincomb, the clamped preconditioner.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "dpd_kernels.hpp"
+
+namespace dpd {
+
+// ===========================================================================
+// gather-map
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_gather_map(MapArgs a, const double* __restrict__ in,
+                                                    double* __restrict__ out) {
+  const int64_t n = a.out.len;
+  const int b = blockIdx.y;
+  const double* inb = in + (int64_t)b * a.in_bstride;
+  double* outb = out + (int64_t)b * a.out_bstride;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+       o += (int64_t)gridDim.x * blockDim.x) {
+    int h[4];
+    int64_t x[4];
+    decode_elem(a.out, o, h, x);
+    double acc = 0.0;
+    for (int t = 0; t < a.nterm; t++) {
+      int hi[4];
+      int64_t xi[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        hi[a.term[t].perm[i]] = h[i];
+        xi[a.term[t].perm[i]] = x[i];
+      }
+      double sg;
+      int64_t ad = fetch_addr(a.in, hi[0], hi[1], hi[2], hi[3], xi[0], xi[1], xi[2], xi[3], &sg);
+      if (ad >= 0) acc += a.term[t].coef * sg * inb[ad];
+    }
+    double v = a.alpha * acc;
+    if (a.beta != 0.0) v = a.beta * outb[o] + v;
+    outb[o] = v;
+  }
+}
+
+hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hipStream_t s) {
+  if (a.out.len == 0 || a.nbatch == 0) return hipSuccess;
+  int64_t blocks = (a.out.len + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_gather_map, dim3((unsigned)blocks, (unsigned)a.nbatch), dim3(256), 0, s, a,
+                     in, out);
+  return hipGetLastError();
+}
+
+// ===========================================================================
+// grouped GEMM
+// ===========================================================================
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int PAD = 2;
+
+template <typename CT>
+struct Acc;
+// Result layout of the 16x16 output tile: lane l, register r holds row
+// row(l,r), column l%16.  A and B operands: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] for both instructions.
+template <>
+struct Acc<double> {
+  typedef double4_t T;
+  static __device__ inline T mfma(double a, double b, T c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Acc<float> {
+  typedef float4_t T;
+  static __device__ inline T mfma(float a, float b, T c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+__device__ inline int64_t digits3(int64_t v, const int32_t* d, const int64_t* s) {
+  int64_t d0 = v % d[0];
+  int64_t t = v / d[0];
+  int64_t d1 = t % d[1];
+  int64_t d2 = t / d[1];
+  return d0 * s[0] + d1 * s[1] + d2 * s[2];
+}
+
+template <typename T>
+__device__ inline double ldg_as_double(const T* p, int64_t i);
+template <>
+__device__ inline double ldg_as_double<double>(const double* p, int64_t i) {
+  return p[i];
+}
+template <>
+__device__ inline double ldg_as_double<float>(const float* p, int64_t i) {
+  return (double)p[i];
+}
+
+template <typename CT, typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_gemm(GemmLaunch g) {
+  __shared__ CT As[BK][BM + PAD];
+  __shared__ CT Bs[BK][BN + PAD];
+  __shared__ int64_t moffA[BM], noffB[BN], koffA[BK], koffB[BK];
+  __shared__ int s_mfast, s_nfast;
+
+  const int tid = threadIdx.x;
+  const int wg = blockIdx.x;
+  // locate the task (binary search on wg_begin)
+  int lo = 0, hi = g.ntask - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (g.tasks[mid].wg_begin <= wg) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmTask& T = g.tasks[lo];
+  int local = wg - T.wg_begin;
+  const int kp = local % T.ksplit;
+  local /= T.ksplit;
+  const int tm = local % T.tiles_m;
+  const int tn = local / T.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  const TA* A = (const TA*)g.A;
+  const TB* B = (const TB*)g.B;
+
+  typedef typename Acc<CT>::T AT;
+  AT acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = AT{0, 0, 0, 0};
+
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+
+  const int64_t kt0 = (int64_t)kp * T.ktiles_total / T.ksplit;
+  const int64_t kt1 = (int64_t)(kp + 1) * T.ktiles_total / T.ksplit;
+  int s = T.seg_begin;
+  int cur_seg = -1;
+  for (int64_t kt = kt0; kt < kt1; kt++) {
+    while (kt >= g.segs[s].ktile_begin + g.segs[s].ktiles) s++;
+    const GemmSeg& S = g.segs[s];
+    if (s != cur_seg) {
+      __syncthreads();
+      if (tid < BM) {
+        int m = m0 + tid;
+        moffA[tid] = (m < T.M) ? digits3(m, T.md, S.sa_m) : -1;
+      } else if (tid < BM + BN) {
+        int n = n0 + tid - BM;
+        noffB[tid - BM] = (n < T.N) ? digits3(n, T.nd, S.sb_n) : -1;
+      }
+      if (tid == 0) {
+        s_mfast = (S.sa_m[0] == 1) || (S.sa_k[0] != 1);
+        s_nfast = (S.sb_n[0] == 1) || (S.sb_k[0] != 1);
+      }
+      cur_seg = s;
+    }
+    const int k0 = (int)(kt - S.ktile_begin) * BK;
+    __syncthreads();
+    if (tid < BK) {
+      int k = k0 + tid;
+      koffA[tid] = (k < S.K) ? digits3(k, S.kd, S.sa_k) : -1;
+    } else if (tid < 2 * BK) {
+      int k = k0 + tid - BK;
+      koffB[tid - BK] = (k < S.K) ? digits3(k, S.kd, S.sb_k) : -1;
+    }
+    __syncthreads();
+    // A tile
+    if (s_mfast) {
+      const int mm = tid & 63, kb = tid >> 6;
+      const int64_t mo = moffA[mm];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int kk = kb + 4 * j;
+        const int64_t ko = koffA[kk];
+        double v = (mo >= 0 && ko >= 0) ? ldg_as_double(A, S.a_off + mo + ko) : 0.0;
+        As[kk][mm] = (CT)v;
+      }
+    } else {
+      const int kk = tid & 15, mb = tid >> 4;
+      const int64_t ko = koffA[kk];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int mm = mb + 16 * j;
+        const int64_t mo = moffA[mm];
+        double v = (mo >= 0 && ko >= 0) ? ldg_as_double(A, S.a_off + mo + ko) : 0.0;
+        As[kk][mm] = (CT)v;
+      }
+    }
+    // B tile
+    if (s_nfast) {
+      const int nn = tid & 63, kb = tid >> 6;
+      const int64_t no = noffB[nn];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int kk = kb + 4 * j;
+        const int64_t ko = koffB[kk];
+        double v = (no >= 0 && ko >= 0) ? ldg_as_double(B, S.b_off + no + ko) : 0.0;
+        Bs[kk][nn] = (CT)v;
+      }
+    } else {
+      const int kk = tid & 15, nb = tid >> 4;
+      const int64_t ko = koffB[kk];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int nn = nb + 16 * j;
+        const int64_t no = noffB[nn];
+        double v = (no >= 0 && ko >= 0) ? ldg_as_double(B, S.b_off + no + ko) : 0.0;
+        Bs[kk][nn] = (CT)v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ks++) {
+      const int kr = ks * 4 + (lane >> 4);
+      CT a0 = As[kr][wm + (lane & 15)];
+      CT a1 = As[kr][wm + 16 + (lane & 15)];
+      CT b0 = Bs[kr][wn + (lane & 15)];
+      CT b1 = Bs[kr][wn + 16 + (lane & 15)];
+      acc[0][0] = Acc<CT>::mfma(a0, b0, acc[0][0]);
+      acc[0][1] = Acc<CT>::mfma(a0, b1, acc[0][1]);
+      acc[1][0] = Acc<CT>::mfma(a1, b0, acc[1][0]);
+      acc[1][1] = Acc<CT>::mfma(a1, b1, acc[1][1]);
+    }
+  }
+
+  // epilogue: lane holds D[row(lane,r)][lane%16] of each 16x16 sub-tile
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm + 16 * i + Acc<CT>::row(lane, r);
+        const int n = n0 + wn + 16 * j + (lane & 15);
+        if (m >= T.M || n >= T.N) continue;
+        const double v = g.alpha * (double)acc[i][j][r];
+        if (T.ksplit > 1) {
+          g.partial[T.part_off + (int64_t)kp * T.M * T.N + m + (int64_t)n * T.M] = v;
+        } else {
+          const int64_t ad = T.c_off + digits3(m, T.md, T.sc_m) + digits3(n, T.nd, T.sc_n);
+          g.C[ad] = (g.beta == 0.0) ? v : g.beta * g.C[ad] + v;
+        }
+      }
+}
+
+__global__ __launch_bounds__(256) void k_splitk_reduce(GemmLaunch g) {
+  const GemmTask& T = g.tasks[g.split_tasks[blockIdx.y]];
+  const int64_t mn = (int64_t)T.M * T.N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < mn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double sum = 0.0;
+    for (int kp = 0; kp < T.ksplit; kp++) sum += g.partial[T.part_off + kp * mn + e];
+    const int m = (int)(e % T.M), n = (int)(e / T.M);
+    const int64_t ad = T.c_off + digits3(m, T.md, T.sc_m) + digits3(n, T.nd, T.sc_n);
+    g.C[ad] = (g.beta == 0.0) ? sum : g.beta * g.C[ad] + sum;
+  }
+}
+
+hipError_t launch_gemm(const GemmLaunch& g, hipStream_t s) {
+  if (g.nwg == 0) return hipSuccess;
+  if (g.prec == PREC_F64) {
+    if (g.a_f32 || g.b_f32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_gemm<double, double, double>), dim3(g.nwg), dim3(256), 0, s, g);
+  } else {
+    if (g.a_f32 && g.b_f32)
+      hipLaunchKernelGGL((k_gemm<float, float, float>), dim3(g.nwg), dim3(256), 0, s, g);
+    else if (g.a_f32)
+      hipLaunchKernelGGL((k_gemm<float, float, double>), dim3(g.nwg), dim3(256), 0, s, g);
+    else if (g.b_f32)
+      hipLaunchKernelGGL((k_gemm<float, double, float>), dim3(g.nwg), dim3(256), 0, s, g);
+    else
+      hipLaunchKernelGGL((k_gemm<float, double, double>), dim3(g.nwg), dim3(256), 0, s, g);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (g.nsplit > 0) {
+    int bx = (g.max_split_mn + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(k_splitk_reduce, dim3(bx, g.nsplit), dim3(256), 0, s, g);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+// ===========================================================================
+// vector kernels
+// ===========================================================================
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// dot products of x with nv vectors; grid (kRedBlocks, nv); fixed reduction order
+__global__ __launch_bounds__(256) void k_multidot_partial(int64_t n, const double* __restrict__ x,
+                                                          const double* __restrict__ y,
+                                                          int64_t ystride, double* partial) {
+  __shared__ double red[4];
+  const int v = blockIdx.y;
+  const double* yv = y + (int64_t)v * ystride;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i] * yv[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(int64_t)v * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_rows(const double* partial, int nb, double* out) {
+  __shared__ double red[4];
+  const int v = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[(int64_t)v * nb + i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[v] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+hipError_t launch_multidot(int64_t n, const double* x, const double* y, int64_t ystride, int nv,
+                           double* partial, double* out, hipStream_t s) {
+  if (nv == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_multidot_partial, dim3(kRedBlocks, nv), dim3(256), 0, s, n, x, y, ystride,
+                     partial);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nv), dim3(256), 0, s, partial, kRedBlocks, out);
+  return hipGetLastError();
+}
+
+// max |x| with the FIRST index attaining it (BLAS idamax semantics)
+__device__ inline void amax_merge(double& v, int64_t& i, double v2, int64_t i2) {
+  double a = fabs(v), b = fabs(v2);
+  if (b > a || (b == a && i2 < i)) {
+    v = v2;
+    i = i2;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_amax_partial(int64_t n, const double* __restrict__ x,
+                                                      double* pv, int64_t* pi) {
+  __shared__ double sv[256];
+  __shared__ int64_t si[256];
+  double v = 0.0;
+  int64_t idx = INT64_MAX;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    amax_merge(v, idx, x[i], i);
+  sv[threadIdx.x] = v;
+  si[threadIdx.x] = idx;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      double v2 = sv[threadIdx.x + off];
+      int64_t i2 = si[threadIdx.x + off];
+      double v1 = sv[threadIdx.x];
+      int64_t i1 = si[threadIdx.x];
+      amax_merge(v1, i1, v2, i2);
+      sv[threadIdx.x] = v1;
+      si[threadIdx.x] = i1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = sv[0];
+    pi[blockIdx.x] = si[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_amax_final(int nb, const double* pv, const int64_t* pi,
+                                                    double* ov, int64_t* oi) {
+  __shared__ double sv[256];
+  __shared__ int64_t si[256];
+  double v = 0.0;
+  int64_t idx = INT64_MAX;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) amax_merge(v, idx, pv[i], pi[i]);
+  sv[threadIdx.x] = v;
+  si[threadIdx.x] = idx;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      double v1 = sv[threadIdx.x];
+      int64_t i1 = si[threadIdx.x];
+      amax_merge(v1, i1, sv[threadIdx.x + off], si[threadIdx.x + off]);
+      sv[threadIdx.x] = v1;
+      si[threadIdx.x] = i1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *ov = sv[0];
+    *oi = si[0];
+  }
+}
+
+hipError_t launch_amax(int64_t n, const double* x, double* pv, int64_t* pi, double* ov,
+                       int64_t* oi, hipStream_t s) {
+  hipLaunchKernelGGL(k_amax_partial, dim3(kRedBlocks), dim3(256), 0, s, n, x, pv, pi);
+  hipLaunchKernelGGL(k_amax_final, dim3(1), dim3(256), 0, s, kRedBlocks, pv, pi, ov, oi);
+  return hipGetLastError();
+}
+
+__global__ void k_axpy(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+__global__ void k_axpy_dev(int64_t n, const double* a, double mul, const double* __restrict__ x,
+                           double* __restrict__ y) {
+  const double c = mul * a[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += c * x[i];
+}
+__global__ void k_scal(int64_t n, double a, double* x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+__global__ void k_scal_dev(int64_t n, const double* a, int invert_sqrt, double* x) {
+  const double c = invert_sqrt ? 1.0 / sqrt(a[0]) : a[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= c;
+}
+__global__ void k_fill(int64_t n, double v, double* x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+__global__ void k_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+// out_j = sum_k coef[k + j*ldc] X_k  (in the reference's order: k ascending)
+__global__ void k_lincomb(int64_t n, const double* __restrict__ X, int64_t xs, int nk,
+                          const double* __restrict__ coef, int ldc, double* __restrict__ out) {
+  const int j = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = 0; k < nk; k++) s += coef[k + (int64_t)j * ldc] * X[(int64_t)k * xs + i];
+    out[(int64_t)j * n + i] = s;
+  }
+}
+// lib_cc/mod_davidson.f90:417-428
+__global__ void k_precond(int64_t n, double w, const double* __restrict__ d,
+                          const double* __restrict__ e, double* __restrict__ p) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double t = w - d[i];
+    if (fabs(t) < 1.0e-4) t = (t < 0.0) ? -1.0e-4 : 1.0e-4;
+    p[i] = e[i] / t;
+  }
+}
+__global__ void k_hadamard_acc(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                               double* __restrict__ c, int64_t bsb, int64_t bsc) {
+  const int v = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    c[(int64_t)v * bsc + i] += a[i] * b[(int64_t)v * bsb + i];
+}
+__global__ void k_convert_f32(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
+static inline unsigned grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+hipError_t launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, s, n, a, x, y);
+  return hipGetLastError();
+}
+hipError_t launch_axpy_dev(int64_t n, const double* a, double mul, const double* x, double* y,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy_dev, dim3(grid_for(n)), dim3(256), 0, s, n, a, mul, x, y);
+  return hipGetLastError();
+}
+hipError_t launch_scal(int64_t n, double a, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_scal, dim3(grid_for(n)), dim3(256), 0, s, n, a, x);
+  return hipGetLastError();
+}
+hipError_t launch_scal_dev(int64_t n, const double* a, int inv, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_scal_dev, dim3(grid_for(n)), dim3(256), 0, s, n, a, inv, x);
+  return hipGetLastError();
+}
+hipError_t launch_fill(int64_t n, double v, double* x, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, n, v, x);
+  return hipGetLastError();
+}
+hipError_t launch_copy(int64_t n, const double* x, double* y, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, s, n, x, y);
+  return hipGetLastError();
+}
+hipError_t launch_lincomb(int64_t n, const double* X, int64_t xs, int nk, const double* coef,
+                          int ldc, int nout, double* out, hipStream_t s) {
+  if (nout == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n), nout), dim3(256), 0, s, n, X, xs, nk, coef, ldc,
+                     out);
+  return hipGetLastError();
+}
+hipError_t launch_precondition(int64_t n, double w, const double* d, const double* e, double* p,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_precond, dim3(grid_for(n)), dim3(256), 0, s, n, w, d, e, p);
+  return hipGetLastError();
+}
+hipError_t launch_hadamard_acc(int64_t n, const double* a, const double* b, double* c, int nbatch,
+                               int64_t bsb, int64_t bsc, hipStream_t s) {
+  if (n <= 0 || nbatch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hadamard_acc, dim3(grid_for(n), nbatch), dim3(256), 0, s, n, a, b, c, bsb,
+                     bsc);
+  return hipGetLastError();
+}
+hipError_t launch_convert_f32(int64_t n, const double* x, float* y, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_convert_f32, dim3(grid_for(n)), dim3(256), 0, s, n, x, y);
+  return hipGetLastError();
+}
+
+}  // namespace dpd
+
+namespace dpd {
+
+__global__ __launch_bounds__(256) void k_orbital_energy(DevLayout L, const double* __restrict__ e,
+                                                        EorbOff off, int mode, double* __restrict__ out) {
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < L.len;
+       o += (int64_t)gridDim.x * blockDim.x) {
+    int h[4];
+    int64_t x[4];
+    decode_elem(L, o, h, x);
+    double v;
+    if (mode == 1) {
+      v = e[off.offv[h[0]] + x[0]] - e[off.offo[h[1]] + x[1]];
+    } else {
+      const double ea = e[off.offv[h[0]] + x[0]], eb = e[off.offv[h[1]] + x[1]];
+      const double ei = e[off.offo[h[2]] + x[2]], ej = e[off.offo[h[3]] + x[3]];
+      v = (mode == 2) ? ea + eb - ei - ej : ei + ej - ea - eb;
+    }
+    out[o] = v;
+  }
+}
+
+__global__ void k_div(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                      double* __restrict__ c) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    c[i] = a[i] / b[i];
+}
+
+hipError_t launch_orbital_energy(const DevLayout& L, const double* eorb, const EorbOff& off, int mode,
+                                 double* out, hipStream_t s) {
+  if (L.len == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_orbital_energy, dim3(grid_for(L.len)), dim3(256), 0, s, L, eorb, off, mode, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_div(int64_t n, const double* a, const double* b, double* c, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_div, dim3(grid_for(n)), dim3(256), 0, s, n, a, b, c);
+  return hipGetLastError();
+}
+
+}  // namespace dpd

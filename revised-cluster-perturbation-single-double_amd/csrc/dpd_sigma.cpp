@@ -1,0 +1,364 @@
+// dpd_sigma.cpp -- device-resident problem setup and the batched sigma builders.
+//
+// Operators (J R for a batch of R = (R1(a,i), R2(a<b,i<j)) vectors):
+//   method 0, SF-CIS  (EE-CIS.SF/calAR.f90:6-34):
+//       c(a,i) = e(a,i) r(a,i) + <fn||ai> r(f,n)                 [c1_r1iabj]
+//   method 1, CIS(Doo) (CISdoo.SD.SF/calAR.f90:9-79):
+//       c(a,i)  = Fac r(c,i) - Fki r(a,k) + Wkaci r(c,k)
+//               + 1/2 <fe||am> r(fe,im) - 1/2 r(ae,mn) <mn||ie>
+//       c(ab,ij)= e(ab,ij) r(ab,ij) + P(ij) <ab||ej> r(e,i) - P(ab) <ij||mb> r(a,m)
+//     with the MP2 intermediates of CISdoo.SD.SF/run.f90:31-49
+//       t(ab,ij) = <ab||ij> / (e_i+e_j-e_a-e_b),   Fac = f_ac - 1/2 t(af,mn)<ef||mn>,
+//       Fki = f_ki + 1/2 <ef||mn> t(ef,in),        Wkaci = <ka||ci> + <fe||nm> t(bf,jn)
+//   method 2, SF-CPSd = method 1 + the doubles-doubles first-order coupling of
+//     EE-CPSd.SF/calC2_VR2.f90:4-47 (ring P(ab)P(ij) r(ae,im) <mb||ej>,
+//     hh-ladder 1/2 r(ab,mn) <mn||ij>, pp-ladder 1/2 <ab||ef> r(ef,ij)).
+// Every term is one mod_contract-style call on the engine with the vector batch
+// folded into the GEMM, in the order of the reference's calAR.
+#include "dpd_sigma.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace dpd {
+
+static Desc mkdesc(const int64_t* a, const int64_t* b, const int64_t* c, const int64_t* d, int i1, int i2,
+                   int sym) {
+  Desc x;
+  std::memcpy(x.pop[0], a, 64);
+  std::memcpy(x.pop[1], b, 64);
+  std::memcpy(x.pop[2], c, 64);
+  std::memcpy(x.pop[3], d, 64);
+  x.iop[0] = i1;
+  x.iop[1] = i2;
+  x.sym = sym;
+  return x;
+}
+
+static Operand op(const Desc& d, const char* lab, int rank, const double* data, const float* f32 = nullptr) {
+  Operand o;
+  o.d = d;
+  o.rank = rank;
+  std::memset(o.lab, 0, 4);
+  std::memcpy(o.lab, lab, std::min<size_t>((size_t)rank, std::strlen(lab)));
+  o.data = data;
+  o.data_f32 = f32;
+  return o;
+}
+
+static Operand batched(Operand o, int64_t stride) {
+  o.batched = true;
+  o.bstride = stride;
+  return o;
+}
+
+static EorbOff eorb_off(const dpd_problem* p) {
+  EorbOff e;
+  int n = p->ctx->sym.nirrep;
+  int64_t k = 0;
+  for (int h = 0; h < 8; h++) {
+    e.offo[h] = k;
+    if (h < n) k += p->popo[h];
+  }
+  for (int h = 0; h < 8; h++) {
+    e.offv[h] = k;
+    if (h < n) k += p->popv[h];
+  }
+  return e;
+}
+
+int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) {
+  dpd_ctx* c = p->ctx;
+  const int64_t *v = p->popv, *o = p->popo, *z = p->zorb;
+  const int S = p->symR;
+  const int nb = (int)nvec;
+  const bool f32 = p->prec != PREC_F64;
+  auto F = [&](const DevBuf& b) -> const float* { return f32 ? (const float*)b.p : nullptr; };
+  DPD_HIP_TRY(launch_fill(p->dim * nvec, 0.0, sigma, c->stream));
+  Desc dR1 = mkdesc(v, o, z, z, 1, 1, S);
+  Operand R1 = batched(op(dR1, "", 4, R), p->dim);
+  Operand C1 = batched(op(dR1, "", 4, sigma), p->dim);
+  int rc;
+  auto lab = [](Operand x, const char* l, int rank) {
+    x.rank = rank;
+    std::memset(x.lab, 0, 4);
+    std::memcpy(x.lab, l, rank);
+    return x;
+  };
+  if (p->method == 0) {
+    // e(a,i) r(a,i)
+    DPD_HIP_TRY(launch_hadamard_acc(p->l1, (const double*)p->diag.p, R, sigma, nb, p->dim, p->dim, c->stream));
+    Operand W = op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->iabj.p, F(p->iabj32));
+    return contract_call(c, 444, W, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec);
+  }
+  Desc dR2 = mkdesc(v, v, o, o, 0, 0, S);
+  Operand R2 = batched(op(dR2, "", 4, R + p->l1), p->dim);
+  Operand C2 = batched(op(dR2, "", 4, sigma + p->l1), p->dim);
+  Operand Fac = op(mkdesc(v, v, z, z, 1, 1, 1), "ae=+", 4, (const double*)p->fac.p, F(p->fac32));
+  Operand Fki = op(mkdesc(o, o, z, z, 1, 1, 1), "mi=-", 4, (const double*)p->fki.p, F(p->fki32));
+  Operand Wk = op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->wkaci.p, F(p->wkaci32));
+  Operand abci = op(mkdesc(v, v, v, o, 0, 1, 1), "feam", 4, (const double*)p->abci.p, F(p->abci32));
+  Operand ijka = op(mkdesc(o, o, o, v, 0, 1, 1), "mnie", 4, (const double*)p->ijka.p, F(p->ijka32));
+  // ---- singles (CISdoo.SD.SF/calAR.f90:41-56)
+  // c1_r1Fae: vv2v_222(Fac, r1, c1, 'ae','ei','ai')
+  rc = contract_call(c, 444, Fac, lab(R1, "ei=-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec);
+  if (rc) return rc;
+  // c1_r1Fmi: vv2v_222(r1, Fki, c1, -1, 'am','mi','ai')
+  rc = contract_call(c, 444, lab(R1, "am=+", 4), Fki, lab(C1, "ai+-", 4), -1.0, 0, nb, p->prec);
+  if (rc) return rc;
+  // c1_r1iabj: vv2v_422(Wkaci, r1, c1, 'fnai','fn','ai')
+  rc = contract_call(c, 444, Wk, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec);
+  if (rc) return rc;
+  // c1_r2abci: vv2v_442(abci, r2, c1, 'feam','feim','ai')  (r2 iop 0: fact)
+  rc = contract_call(c, 442, abci, lab(R2, "feim", 4), lab(C1, "ai", 2), 1.0, 0, nb, p->prec);
+  if (rc) return rc;
+  // c1_r2ijka: vv2v_442(r2, ijka, c1, -1, 'aemn','mnie','ai')
+  rc = contract_call(c, 442, lab(R2, "aemn", 4), ijka, lab(C1, "ai", 2), -1.0, 0, nb, p->prec);
+  if (rc) return rc;
+  // ---- doubles (CISdoo.SD.SF/calAR.f90:63-77)
+  DPD_HIP_TRY(launch_hadamard_acc(p->l2, (const double*)p->diag.p + p->l1, R + p->l1, sigma + p->l1, nb, p->dim,
+                                  p->dim, c->stream));
+  // c2_r1abci: vv2v_424(abci, r1, c2, 'abej','ei','abij', anti=2)
+  rc = contract_call(c, 424, lab(abci, "abej", 4), lab(R1, "ei", 2), lab(C2, "abij", 4), 1.0, 2, nb, p->prec);
+  if (rc) return rc;
+  // c2_r1ijka: vv2v_424(ijka, r1, c2, -1, 'ijmb','am','abij', anti=1)
+  rc = contract_call(c, 424, lab(ijka, "ijmb", 4), lab(R1, "am", 2), lab(C2, "abij", 4), -1.0, 1, nb, p->prec);
+  if (rc) return rc;
+  if (p->method == 2) {
+    Operand W = op(mkdesc(v, o, v, o, 1, 1, 1), "embj", 4, (const double*)p->iabj.p, F(p->iabj32));
+    Operand ijkl = op(mkdesc(o, o, o, o, 0, 0, 1), "mnij", 4, (const double*)p->ijkl.p, F(p->ijkl32));
+    Operand abcd = op(mkdesc(v, v, v, v, 0, 0, 1), "abef", 4, (const double*)p->abcd.p, F(p->abcd32));
+    // c2_r2Wmbej: vv2v_444(r2, W, c2, 'aeim','embj','abij', anti=3)
+    rc = contract_call(c, 444, lab(R2, "aeim", 4), W, lab(C2, "abij", 4), 1.0, 3, nb, p->prec);
+    if (rc) return rc;
+    // c2_r2Wmnij: vv2v_444(r2, ijkl, c2, 'abmn','mnij','abij')
+    rc = contract_call(c, 444, lab(R2, "abmn", 4), ijkl, lab(C2, "abij", 4), 1.0, 0, nb, p->prec);
+    if (rc) return rc;
+    // c2_r2abcd: vv2v_444(abcd, r2, c2, 'abef','efij','abij')
+    rc = contract_call(c, 444, abcd, lab(R2, "efij", 4), lab(C2, "abij", 4), 1.0, 0, nb, p->prec);
+    if (rc) return rc;
+  }
+  return DPD_OK;
+}
+
+static int upload(DevBuf& b, const std::vector<double>& h, hipStream_t s) {
+  int rc = b.alloc(std::max<size_t>(8, h.size() * sizeof(double)));
+  if (rc) return rc;
+  if (!h.empty()) DPD_HIP_TRY(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  DPD_HIP_TRY(hipStreamSynchronize(s));
+  return DPD_OK;
+}
+
+static int make_f32(DevBuf& dst, const DevBuf& src, int64_t n, hipStream_t s) {
+  int rc = dst.alloc(std::max<size_t>(8, n * sizeof(float)));
+  if (rc) return rc;
+  DPD_HIP_TRY(launch_convert_f32(n, (const double*)src.p, (float*)dst.p, s));
+  return DPD_OK;
+}
+
+}  // namespace dpd
+
+using namespace dpd;
+
+extern "C" {
+
+int dpd_synth_fill(const dpd_synth_spec* spec, const char* which, int64_t iop, double* out, int64_t len);
+
+int dpd_problem_create(dpd_ctx* c, const dpd_synth_spec* spec_in, int method, int64_t symR, dpd_problem** out) {
+  if (!c || !spec_in || !out) return fail(DPD_EINVAL, "null argument");
+  if (method < 0 || method > 2) return fail(DPD_EINVAL, "method must be 0 (SF-CIS), 1 (CIS(Doo)) or 2 (SF-CPSd)");
+  DPD_HIP_TRY(hipSetDevice(c->device));
+  dpd_synth_spec spec = *spec_in;
+  int rc = dpd_synth_layout(&spec);
+  if (rc) return rc;
+  if (spec.nirrep != c->sym.nirrep) return fail(DPD_EINVAL, "spec nirrep differs from the context");
+  if (symR < 1 || symR > spec.nirrep) return fail(DPD_EINVAL, "symR out of range");
+  auto p = std::make_unique<dpd_problem>();
+  p->ctx = c;
+  p->spec = spec;
+  p->method = method;
+  p->symR = (int)symR;
+  const Sym& S = c->sym;
+  for (int h = 0; h < 8; h++) {
+    p->popo[h] = h < S.nirrep ? spec.popo[h] : 0;
+    p->popv[h] = h < S.nirrep ? spec.popv[h] : 0;
+    p->zorb[h] = h == 0;
+  }
+  const int64_t *v = p->popv, *o = p->popo, *z = p->zorb;
+  p->l1 = S.irpdso((int)symR, v, o, 1);
+  p->l2 = method == 0 ? 0 : S.idsymsoc((int)symR, v, v, 0, o, o, 0);
+  p->dim = p->l1 + p->l2;
+  hipStream_t st = c->stream;
+  // orbital energies
+  int64_t ne = 0;
+  for (int h = 0; h < S.nirrep; h++) ne += p->popo[h] + p->popv[h];
+  p->eorb_host.assign(ne, 0.0);
+  rc = dpd_synth_fill(&spec, "eorb", 0, p->eorb_host.data(), ne);
+  if (rc) return rc;
+  if ((rc = upload(p->eorb, p->eorb_host, st))) return rc;
+  auto gen = [&](const char* which, int iop, const Desc& d, DevBuf& dst, int64_t& len) -> int {
+    len = make_layout(S, d).len;
+    std::vector<double> h(len);
+    int r = dpd_synth_fill(&spec, which, iop, h.data(), len);
+    if (r) return r;
+    return upload(dst, h, st);
+  };
+  if ((rc = gen("iabj", 1, mkdesc(v, o, v, o, 1, 1, 1), p->iabj, p->len_iabj))) return rc;
+  // diagonal (OED1, OED2): lib_cc/Eorb_diff.f90
+  if ((rc = p->diag.alloc(sizeof(double) * std::max<int64_t>(1, p->dim)))) return rc;
+  EorbOff eo = eorb_off(p.get());
+  DPD_HIP_TRY(launch_orbital_energy(make_layout(S, mkdesc(v, o, z, z, 1, 1, (int)symR)), (const double*)p->eorb.p,
+                                    eo, 1, (double*)p->diag.p, st));
+  if (method >= 1) {
+    DPD_HIP_TRY(launch_orbital_energy(make_layout(S, mkdesc(v, v, o, o, 0, 0, (int)symR)),
+                                      (const double*)p->eorb.p, eo, 2, (double*)p->diag.p + p->l1, st));
+    if ((rc = gen("ijka", 0, mkdesc(o, o, o, v, 0, 1, 1), p->ijka, p->len_ijka))) return rc;
+    if ((rc = gen("abij", 0, mkdesc(v, v, o, o, 0, 0, 1), p->abij, p->len_abij))) return rc;
+    if ((rc = gen("abci", 0, mkdesc(v, v, v, o, 0, 1, 1), p->abci, p->len_abci))) return rc;
+    if (method == 2) {
+      if ((rc = gen("ijkl", 0, mkdesc(o, o, o, o, 0, 0, 1), p->ijkl, p->len_ijkl))) return rc;
+      if ((rc = gen("abcd", 0, mkdesc(v, v, v, v, 0, 0, 1), p->abcd, p->len_abcd))) return rc;
+    }
+    // MP2 amplitudes t = <ab||ij> / (e_i+e_j-e_a-e_b)   (denomT.f90:94-249)
+    Desc dT = mkdesc(v, v, o, o, 0, 0, 1);
+    if ((rc = p->mp2t.alloc(sizeof(double) * std::max<int64_t>(1, p->len_abij)))) return rc;
+    DPD_HIP_TRY(launch_orbital_energy(make_layout(S, dT), (const double*)p->eorb.p, eo, 3, (double*)p->mp2t.p, st));
+    DPD_HIP_TRY(launch_div(p->len_abij, (const double*)p->abij.p, (const double*)p->mp2t.p, (double*)p->mp2t.p, st));
+    // Fac = f_vv (putDiagFock 'V') + Fae_r2abij(MP2t, abij)
+    Desc dFac = mkdesc(v, v, z, z, 1, 1, 1), dFki = mkdesc(o, o, z, z, 1, 1, 1);
+    p->len_fac = make_layout(S, dFac).len;
+    p->len_fki = make_layout(S, dFki).len;
+    std::vector<double> hf(p->len_fac, 0.0), hk(p->len_fki, 0.0);
+    {
+      int64_t offv = 0, offo = 0, ev = 0;
+      for (int h = 0; h < S.nirrep; h++) ev += p->popo[h];
+      for (int h = 0; h < S.nirrep; h++) {
+        int64_t nv = p->popv[h], no = p->popo[h];
+        int64_t bv = S.isymoffso(h + 1, 1, v, v, 1) - 1, bo = S.isymoffso(h + 1, 1, o, o, 1) - 1;
+        for (int64_t a = 0; a < nv; a++) hf[bv + a * nv + a] = p->eorb_host[ev + offv + a];
+        for (int64_t i = 0; i < no; i++) hk[bo + i * no + i] = p->eorb_host[offo + i];
+        offv += nv;
+        offo += no;
+      }
+    }
+    if ((rc = upload(p->fac, hf, st))) return rc;
+    if ((rc = upload(p->fki, hk, st))) return rc;
+    Operand T = op(dT, "afmn", 4, (const double*)p->mp2t.p);
+    Operand Vabij = op(dT, "efmn", 4, (const double*)p->abij.p);
+    // Fae_r2abij.f90:16: vv2v_442(r2, abij, Fae, -fact, 'afmn','efmn','ae')
+    rc = contract_call(c, 442, T, Vabij, op(dFac, "ae", 2, (const double*)p->fac.p), -1.0, 0, 1, PREC_F64);
+    if (rc) return rc;
+    // Fmi_r2abij.f90:18: vv2v_442(abij, r2, Fmi, fact, 'efmn','efin','mi')
+    T = op(dT, "efin", 4, (const double*)p->mp2t.p);
+    rc = contract_call(c, 442, Vabij, T, op(dFki, "mi", 2, (const double*)p->fki.p), 1.0, 0, 1, PREC_F64);
+    if (rc) return rc;
+    // Wkaci = iabj + Wmbej_r2abij(MP2t, abij, 2.0): vv2v_444(abij, r2, W, 0.5*2, 'fenm','bfjn','embj')
+    if ((rc = p->wkaci.alloc(sizeof(double) * std::max<int64_t>(1, p->len_iabj)))) return rc;
+    DPD_HIP_TRY(launch_copy(p->len_iabj, (const double*)p->iabj.p, (double*)p->wkaci.p, st));
+    rc = contract_call(c, 444, op(dT, "fenm", 4, (const double*)p->abij.p), op(dT, "bfjn", 4, (const double*)p->mp2t.p),
+                       op(mkdesc(v, o, v, o, 1, 1, 1), "embj", 4, (const double*)p->wkaci.p), 1.0, 0, 1, PREC_F64);
+    if (rc) return rc;
+  }
+  DPD_HIP_TRY(hipStreamSynchronize(st));
+  *out = p.release();
+  return DPD_OK;
+}
+
+int dpd_problem_destroy(dpd_problem* p) {
+  if (!p) return DPD_OK;
+  (void)hipStreamSynchronize(p->ctx->stream);
+  delete p;
+  return DPD_OK;
+}
+
+int64_t dpd_problem_dim(const dpd_problem* p) { return p ? p->dim : 0; }
+
+int dpd_problem_set_precision(dpd_problem* p, int prec) {
+  if (!p) return fail(DPD_EINVAL, "null problem");
+  if (prec != PREC_F64 && prec != PREC_F32) return fail(DPD_EINVAL, "precision %d is not available", prec);
+  hipStream_t s = p->ctx->stream;
+  if (prec == PREC_F32 && !p->iabj32.p) {
+    int rc;
+    if ((rc = make_f32(p->iabj32, p->iabj, p->len_iabj, s))) return rc;
+    if (p->method >= 1) {
+      if ((rc = make_f32(p->ijka32, p->ijka, p->len_ijka, s))) return rc;
+      if ((rc = make_f32(p->abci32, p->abci, p->len_abci, s))) return rc;
+      if ((rc = make_f32(p->fac32, p->fac, p->len_fac, s))) return rc;
+      if ((rc = make_f32(p->fki32, p->fki, p->len_fki, s))) return rc;
+      if ((rc = make_f32(p->wkaci32, p->wkaci, p->len_iabj, s))) return rc;
+    }
+    if (p->method == 2) {
+      if ((rc = make_f32(p->ijkl32, p->ijkl, p->len_ijkl, s))) return rc;
+      if ((rc = make_f32(p->abcd32, p->abcd, p->len_abcd, s))) return rc;
+    }
+    DPD_HIP_TRY(hipStreamSynchronize(s));
+  }
+  p->prec = prec;
+  return DPD_OK;
+}
+
+int dpd_problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) {
+  if (!p || !R || !sigma || nvec < 1) return fail(DPD_EINVAL, "dpd_problem_sigma: bad arguments");
+  DPD_HIP_TRY(hipSetDevice(p->ctx->device));
+  return problem_sigma(p, R, sigma, nvec);
+}
+
+int dpd_problem_diag(dpd_problem* p, double* diag) {
+  if (!p || !diag) return fail(DPD_EINVAL, "null argument");
+  DPD_HIP_TRY(hipMemcpyAsync(diag, p->diag.p, sizeof(double) * p->dim, hipMemcpyDeviceToDevice, p->ctx->stream));
+  return DPD_OK;
+}
+
+// lib_cc/preDavidson.f90:120-223 (spin_flip=ab): unit vectors at the nroot lowest
+// e(a_beta) - e(i_alpha) of the excited-state symmetry, a placed after the alpha
+// virtuals of its irrep.
+int dpd_problem_guess(dpd_problem* p, int64_t nroot, double* R, double* energies) {
+  if (!p || !R || nroot < 1) return fail(DPD_EINVAL, "bad arguments");
+  const Sym& S = p->ctx->sym;
+  const dpd_synth_spec& sp = p->spec;
+  const int sym = p->symR;
+  // energies of alpha occupied / beta virtual spin orbitals per irrep
+  int64_t offo[8], offv[8], k = 0;
+  for (int h = 0; h < S.nirrep; h++) {
+    offo[h] = k;
+    k += p->popo[h];
+  }
+  for (int h = 0; h < S.nirrep; h++) {
+    offv[h] = k;
+    k += p->popv[h];
+  }
+  struct Cand {
+    double e;
+    int64_t lin;
+    int ha, hi;
+    int64_t a, i;
+  };
+  std::vector<Cand> cand;
+  // (vrtb, popa) pair block of irrep sym: irrep of i ascending, a fastest
+  for (int hi = 0; hi < S.nirrep; hi++) {
+    int ha = S.dp[hi][sym - 1];
+    for (int64_t i = 0; i < sp.popa[hi]; i++)
+      for (int64_t a = 0; a < sp.vrtb[ha]; a++) {
+        double e = p->eorb_host[offv[ha] + sp.vrta[ha] + a] - p->eorb_host[offo[hi] + i];
+        cand.push_back({e, (int64_t)cand.size(), ha, hi, a, i});
+      }
+  }
+  if ((int64_t)cand.size() < nroot) return fail(DPD_EINVAL, "ERROR: nSize < nlowest in getLowest");
+  std::stable_sort(cand.begin(), cand.end(), [](const Cand& x, const Cand& y) { return x.e < y.e; });
+  DevLayout L = make_layout(S, mkdesc(p->popv, p->popo, p->zorb, p->zorb, 1, 1, sym));
+  std::vector<double> h((size_t)(nroot * p->dim), 0.0);
+  for (int64_t r = 0; r < nroot; r++) {
+    const Cand& c = cand[r];
+    double sg;
+    int64_t ad = fetch_addr(L, c.ha, c.hi, 0, 0, sp.vrta[c.ha] + c.a, c.i, 0, 0, &sg);
+    if (ad < 0) return fail(DPD_EINVAL, "guess index not stored (internal)");
+    h[r * p->dim + ad] = 1.0;
+    if (energies) energies[r] = c.e;
+  }
+  DPD_HIP_TRY(hipMemcpyAsync(R, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, p->ctx->stream));
+  DPD_HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+  return DPD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,28 @@
+// dpd_sigma.hpp -- device-resident problem (integrals, intermediates, diagonal)
+// and the batched sigma builders.
+#pragma once
+#include <vector>
+
+#include "dpd_engine.hpp"
+
+struct dpd_problem {
+  dpd_ctx* ctx = nullptr;
+  dpd_synth_spec spec;
+  int method = 2;
+  int symR = 1;
+  int prec = dpd::PREC_F64;
+  int64_t popo[8], popv[8], zorb[8];
+  int64_t l1 = 0, l2 = 0, dim = 0;
+  // device tensors (FP64)
+  dpd::DevBuf eorb, iabj, ijka, ijkl, abij, abci, abcd, fac, fki, wkaci, mp2t, diag;
+  // FP32 copies used by the mixed-precision contractions
+  dpd::DevBuf iabj32, ijka32, ijkl32, abci32, abcd32, fac32, fki32, wkaci32;
+  std::vector<double> eorb_host;
+  int64_t len_iabj = 0, len_ijka = 0, len_ijkl = 0, len_abij = 0, len_abci = 0, len_abcd = 0;
+  int64_t len_fac = 0, len_fki = 0;
+  int64_t sigma_flops = 0;  // algorithmic GEMM flops of one sigma (one vector)
+};
+
+namespace dpd {
+int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec);
+}

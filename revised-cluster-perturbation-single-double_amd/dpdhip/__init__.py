@@ -1,0 +1,371 @@
+"""dpdhip -- Python front end of the MI355X DPD contraction engine.
+
+Mirrors the reference's lib_cc interface (lib_cc/mod_operator.f90 `type opt`,
+lib_cc/mod_contract.f90 vv2v_444/222/422/224/424/442, the libsrint sort kernels)
+over the C-ABI in include/dpd_hip.h.  Device memory comes from torch (HIP);
+every computation runs in the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import DpdError, check, lib
+
+__all__ = ["Context", "Opt", "Spec", "Problem", "DpdError", "xor_dirprd", "ZORB"]
+
+ZORB = np.array([1, 0, 0, 0, 0, 0, 0, 0], dtype=np.int64)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def xor_dirprd(nirrep):
+    """Abelian direct-product table (XOR of 0-based irrep ids), 1-based values."""
+    d = np.ones((8, 8), dtype=np.int64)
+    for i in range(nirrep):
+        for j in range(nirrep):
+            d[i, j] = (i ^ j) + 1
+    return d
+
+
+def _i64(a):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+    return a, a.ctypes.data_as(_lib.c_i64_p)
+
+
+class Context:
+    """Replaces the reference's global symmetry/IOP state (mod_symm, mod_iop)."""
+
+    def __init__(self, nirrep, device=0, dirprd=None):
+        torch = _torch()
+        self.nirrep = nirrep
+        self.device = device
+        self.dirprd = xor_dirprd(nirrep) if dirprd is None else np.asarray(dirprd, dtype=np.int64)
+        torch.cuda.init()
+        h = ctypes.c_void_p()
+        d, dp = _i64(self.dirprd.reshape(-1))
+        check(lib().dpd_ctx_create(ctypes.byref(h), nirrep, dp, device))
+        self.h = h
+        self.use_current_stream()
+
+    def use_current_stream(self):
+        torch = _torch()
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        check(lib().dpd_ctx_set_stream(self.h, ctypes.c_void_p(s)))
+
+    def synchronize(self):
+        check(lib().dpd_ctx_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dpd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # metadata ---------------------------------------------------------------
+    def irpdso(self, sym, popp, popq, iop):
+        a, ap = _i64(popp)
+        b, bp = _i64(popq)
+        return int(lib().dpd_irpdso(self.h, sym, ap, bp, iop))
+
+    def idsymsoc(self, irrepx, popp, popq, ioppq, popr, pops, ioprs):
+        ps = [_i64(x) for x in (popp, popq, popr, pops)]
+        return int(lib().dpd_idsymsoc(self.h, irrepx, ps[0][1], ps[1][1], ioppq, ps[2][1], ps[3][1], ioprs))
+
+    def isymoffso(self, irrepq1, irrep, popp, popq, iop):
+        a, ap = _i64(popp)
+        b, bp = _i64(popq)
+        return int(lib().dpd_isymoffso(self.h, irrepq1, irrep, ap, bp, iop))
+
+
+class Opt:
+    """Mirror of `type opt` (lib_cc/mod_operator.f90:8-25): populations of the
+    four indices, pair packing (ioppq, ioprs), symmetry and an FP64 device array."""
+
+    def __init__(self, ctx: Context, popp, popq, popr=ZORB, pops=ZORB, ioppq=1, ioprs=1, sym=1, arr=None):
+        torch = _torch()
+        self.ctx = ctx
+        self.pops = [np.asarray(p, dtype=np.int64) for p in (popp, popq, popr, pops)]
+        self.ioppq, self.ioprs, self.sym = int(ioppq), int(ioprs), int(sym)
+        t = self._struct()
+        self.len = int(lib().dpd_tensor_len(ctx.h, ctypes.byref(t)))
+        if arr is None:
+            arr = torch.zeros(max(self.len, 1), dtype=torch.float64, device=f"cuda:{ctx.device}")
+        self.arr = arr
+
+    def _struct(self, arr=None):
+        t = _lib.Tensor_()
+        for k in range(4):
+            for h in range(8):
+                t.pop[k][h] = int(self.pops[k][h])
+        t.iop[0], t.iop[1], t.sym = self.ioppq, self.ioprs, self.sym
+        a = arr if arr is not None else getattr(self, "arr", None)
+        t.data = a.data_ptr() if a is not None else None
+        return t
+
+    def norm(self):
+        return float(self.arr[: self.len].norm())
+
+    def numpy(self):
+        return self.arr[: self.len].cpu().numpy()
+
+
+def _lab(s):
+    return s.encode()
+
+
+def vv2v_dpd_444(v1: Opt, t1, v2: Opt, t2, v3: Opt, t3, fact1, fact2):
+    """libsrint/vv2v_dpd_444.f90: v3 = fact1*v1(x)v2 + fact2*v3"""
+    a, b, c = v1._struct(), v2._struct(), v3._struct()
+    check(lib().dpd_vv2v_dpd_444(v1.ctx.h, ctypes.byref(a), _lab(t1), ctypes.byref(b), _lab(t2), ctypes.byref(c),
+                                 _lab(t3), fact1, fact2))
+
+
+def vv2v_dpd_424(v1: Opt, t1, v2: Opt, t2, v3: Opt, t3, fact1, fact2):
+    a, b, c = v1._struct(), v2._struct(), v3._struct()
+    check(lib().dpd_vv2v_dpd_424(v1.ctx.h, ctypes.byref(a), _lab(t1), ctypes.byref(b), _lab(t2), ctypes.byref(c),
+                                 _lab(t3), fact1, fact2))
+
+
+def vv2v_dpd_442(v1: Opt, t1, v2: Opt, t2, v3: Opt, t3, fact1, fact2):
+    a, b, c = v1._struct(), v2._struct(), v3._struct()
+    check(lib().dpd_vv2v_dpd_442(v1.ctx.h, ctypes.byref(a), _lab(t1), ctypes.byref(b), _lab(t2), ctypes.byref(c),
+                                 _lab(t3), fact1, fact2))
+
+
+def _contract(fn, v1, v2, v3, fact, t1, t2, t3, *extra):
+    a, b, c = v1._struct(), v2._struct(), v3._struct()
+    check(fn(v1.ctx.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), fact, _lab(t1), _lab(t2), _lab(t3),
+             *extra))
+
+
+def vv2v_444(v1, v2, v3, fact, t1, t2, t3, anti=0):
+    """lib_cc/mod_contract.f90:10-162: v3 += fact * P{v1 (x) v2}"""
+    _contract(lib().dpd_vv2v_444, v1, v2, v3, fact, t1, t2, t3, anti)
+
+
+def vv2v_222(v1, v2, v3, fact, t1, t2, t3):
+    _contract(lib().dpd_vv2v_222, v1, v2, v3, fact, t1, t2, t3)
+
+
+def vv2v_422(v1, v2, v3, fact, t1, t2, t3):
+    _contract(lib().dpd_vv2v_422, v1, v2, v3, fact, t1, t2, t3)
+
+
+def vv2v_224(v1, v2, v3, fact, t1, t2, t3, anti=0):
+    _contract(lib().dpd_vv2v_224, v1, v2, v3, fact, t1, t2, t3, anti)
+
+
+def vv2v_424(v1, v2, v3, fact, t1, t2, t3, anti=0):
+    _contract(lib().dpd_vv2v_424, v1, v2, v3, fact, t1, t2, t3, anti)
+
+
+def vv2v_442(v1, v2, v3, fact, t1, t2, t3):
+    _contract(lib().dpd_vv2v_442, v1, v2, v3, fact, t1, t2, t3)
+
+
+# ---- sort kernels (device tensors in and out) ---------------------------------
+def iexpso(ctx, x, out, popp, popq, ioppq, iexppq, popr, pops, ioprs, iexprs, sym):
+    ps = [_i64(p) for p in (popp, popq, popr, pops)]
+    check(lib().dpd_iexpso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ioppq, iexppq, ps[2][1],
+                           ps[3][1], ioprs, iexprs, sym))
+
+
+def icompso(ctx, x, out, popp, popq, ioppq, icpq, popr, pops, ioprs, icrs, sym):
+    ps = [_i64(p) for p in (popp, popq, popr, pops)]
+    check(lib().dpd_icompso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ioppq, icpq, ps[2][1],
+                            ps[3][1], ioprs, icrs, sym))
+
+
+def isymtrso(ctx, x, out, popp, popq, ioppq, icpq, popr, pops, ioprs, icrs, sym):
+    ps = [_i64(p) for p in (popp, popq, popr, pops)]
+    check(lib().dpd_isymtrso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ioppq, icpq, ps[2][1],
+                             ps[3][1], ioprs, icrs, sym))
+
+
+def itranspso(ctx, x, out, popp, popq, ioppq, popr, pops, ioprs, sym):
+    ps = [_i64(p) for p in (popp, popq, popr, pops)]
+    check(lib().dpd_itranspso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ioppq, ps[2][1], ps[3][1],
+                              ioprs, sym))
+
+
+def sstgenso(ctx, x, out, pop1, pop2, pop3, pop4, irrepx, type4):
+    ps = [_i64(p) for p in (pop1, pop2, pop3, pop4)]
+    check(lib().dpd_sstgenso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ps[2][1], ps[3][1], irrepx,
+                             type4.encode()))
+
+
+# ---- dense GEMM (libsrint/xgemmso.f router) ----------------------------------
+def xgemm(ctx, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec="fp64"):
+    check(lib().dpd_xgemm(ctx.h, transa.encode(), transb.encode(), m, n, k, alpha, A.data_ptr(), lda,
+                          B.data_ptr(), ldb, beta, C.data_ptr(), ldc, PRECISIONS[prec]))
+
+
+# ---- vector primitives ------------------------------------------------------
+def dot(ctx, x, y):
+    r = ctypes.c_double()
+    check(lib().dpd_dot(ctx.h, x.numel(), x.data_ptr(), y.data_ptr(), ctypes.byref(r)))
+    return r.value
+
+
+def nrm2(ctx, x):
+    r = ctypes.c_double()
+    check(lib().dpd_nrm2(ctx.h, x.numel(), x.data_ptr(), ctypes.byref(r)))
+    return r.value
+
+
+def amax(ctx, x):
+    i = ctypes.c_int64()
+    v = ctypes.c_double()
+    check(lib().dpd_amax(ctx.h, x.numel(), x.data_ptr(), ctypes.byref(i), ctypes.byref(v)))
+    return i.value, v.value
+
+
+def axpy(ctx, a, x, y):
+    check(lib().dpd_axpy(ctx.h, x.numel(), a, x.data_ptr(), y.data_ptr()))
+
+
+def scal(ctx, a, x):
+    check(lib().dpd_scal(ctx.h, x.numel(), a, x.data_ptr()))
+
+
+def precondition(ctx, w, diag, e, p):
+    check(lib().dpd_precondition(ctx.h, e.numel(), w, diag.data_ptr(), e.data_ptr(), p.data_ptr()))
+
+
+# ---- synthetic Hamiltonians ---------------------------------------------------
+@dataclass
+class Spec:
+    """Seeded synthetic spin-free Hamiltonian (see csrc/dpd_synth.cpp)."""
+    nso: tuple
+    nalpha: int
+    nbeta: int
+    seed: int = 0
+
+    @property
+    def nirrep(self):
+        return len(self.nso)
+
+    def c(self):
+        s = _lib.SynthSpec_()
+        s.nirrep = len(self.nso)
+        for h, n in enumerate(self.nso):
+            s.nso[h] = n
+        s.nalpha, s.nbeta, s.seed = self.nalpha, self.nbeta, self.seed
+        check(lib().dpd_synth_layout(ctypes.byref(s)))
+        return s
+
+    def layout(self):
+        s = self.c()
+        g = lambda f: np.array(list(getattr(s, f)), dtype=np.int64)
+        return {k: g(k) for k in ("popa", "popb", "vrta", "vrtb", "popo", "popv")}
+
+    def fill(self, which, iop, length):
+        s = self.c()
+        out = np.zeros(max(int(length), 1))
+        check(lib().dpd_synth_fill(ctypes.byref(s), which.encode(), iop, out.ctypes.data_as(_lib.c_double_p),
+                                   int(length)))
+        return out[: int(length)]
+
+    def eri(self, idx4):
+        s = self.c()
+        idx = np.ascontiguousarray(np.asarray(idx4, dtype=np.int64).reshape(-1, 4))
+        out = np.zeros(len(idx))
+        check(lib().dpd_synth_eri(ctypes.byref(s), len(idx), idx.ctypes.data_as(_lib.c_i64_p),
+                                  out.ctypes.data_as(_lib.c_double_p)))
+        return out
+
+
+def small_eig(A):
+    """Subspace eigensolve (host FP64): eigenvalues and unit-norm right
+    eigenvectors of a real nonsymmetric matrix."""
+    A = np.ascontiguousarray(np.asarray(A, dtype=np.float64))
+    n = A.shape[0]
+    wr, wi = np.zeros(n), np.zeros(n)
+    V = np.zeros(n * n)
+    P = lambda a: a.ctypes.data_as(_lib.c_double_p)
+    check(lib().dpd_small_eig(n, P(A), P(wr), P(wi), P(V)))
+    return wr, wi, V.reshape(n, n, order="F")
+
+
+METHODS = {"sf-cis": 0, "cisdoo": 1, "sf-cpsd": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
+class Problem:
+    """Device-resident operator (integrals + intermediates) of one method and
+    excited-state symmetry; sigma = J R for batches of vectors."""
+
+    def __init__(self, ctx: Context, spec: Spec, method="sf-cpsd", symR=1):
+        self.ctx = ctx
+        self.spec = spec
+        self.method = method
+        self.symR = symR
+        s = spec.c()
+        h = ctypes.c_void_p()
+        check(lib().dpd_problem_create(ctx.h, ctypes.byref(s), METHODS[method], symR, ctypes.byref(h)))
+        self.h = h
+        self.dim = int(lib().dpd_problem_dim(h))
+        self.precision = "fp64"
+
+    def set_precision(self, prec):
+        check(lib().dpd_problem_set_precision(self.h, PRECISIONS[prec]))
+        self.precision = prec
+
+    def sigma(self, R, out=None):
+        torch = _torch()
+        nvec = R.numel() // self.dim
+        if out is None:
+            out = torch.empty_like(R)
+        check(lib().dpd_problem_sigma(self.h, R.data_ptr(), out.data_ptr(), nvec))
+        return out
+
+    def diag(self):
+        torch = _torch()
+        d = torch.empty(self.dim, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        check(lib().dpd_problem_diag(self.h, d.data_ptr()))
+        return d
+
+    def guess(self, nroot):
+        torch = _torch()
+        R = torch.zeros(nroot * self.dim, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        e = np.zeros(nroot)
+        check(lib().dpd_problem_guess(self.h, nroot, R.data_ptr(), e.ctypes.data_as(_lib.c_double_p)))
+        return R, e
+
+    def davidson(self, nroot, conv=1e-8, max_dim=20, max_cyc=50, lowest=False, final_check=True, R=None):
+        """Non-Hermitian Davidson (lib_cc/mod_davidson.f90:40-561)."""
+        if R is None:
+            R, _ = self.guess(nroot)
+        o = _lib.DavidsonOpts_()
+        o.nroot, o.max_dim, o.max_cyc, o.conv = nroot, max_dim, max_cyc, conv
+        o.lowest, o.final_check = int(lowest), int(final_check)
+        w = np.zeros(nroot)
+        rc = lib().dpd_davidson(self.h, ctypes.byref(o), R.data_ptr(), w.ctypes.data_as(_lib.c_double_p))
+        if rc not in (0, _lib.DPD_ENOTCONV):
+            check(rc)
+        info = {"converged": rc == 0, "iterations": o.iterations, "sigma_calls": o.sigma_calls,
+                "residual": list(o.residual[:nroot]), "final_residual": list(o.final_residual[:nroot])}
+        return w, R, info
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dpd_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

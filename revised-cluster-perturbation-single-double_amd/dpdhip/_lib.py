@@ -1,0 +1,151 @@
+"""ctypes binding of libdpdhip.so (include/dpd_hip.h).
+
+The product path: every call goes to the HIP library.  There is no CPU
+fallback; if the shared object is missing the import fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdpdhip.so")
+
+i64 = ctypes.c_int64
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_i64_p = ctypes.POINTER(ctypes.c_int64)
+c_void_p = ctypes.c_void_p
+
+DPD_OK = 0
+DPD_ENOTCONV = -8
+
+
+class DpdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Tensor_(ctypes.Structure):
+    """dpd_tensor"""
+    _fields_ = [("pop", (i64 * 8) * 4), ("iop", i64 * 2), ("sym", i64), ("data", c_void_p)]
+
+
+class SynthSpec_(ctypes.Structure):
+    """dpd_synth_spec"""
+    _fields_ = [("nirrep", i64), ("nso", i64 * 8), ("nalpha", i64), ("nbeta", i64), ("seed", ctypes.c_uint64),
+                ("popa", i64 * 8), ("popb", i64 * 8), ("vrta", i64 * 8), ("vrtb", i64 * 8), ("popo", i64 * 8),
+                ("popv", i64 * 8)]
+
+
+class DavidsonOpts_(ctypes.Structure):
+    """dpd_davidson_opts"""
+    _fields_ = [("nroot", i64), ("max_dim", i64), ("max_cyc", i64), ("conv", ctypes.c_double),
+                ("lowest", ctypes.c_int), ("final_check", ctypes.c_int), ("iterations", i64),
+                ("sigma_calls", i64), ("residual", ctypes.c_double * 64), ("final_residual", ctypes.c_double * 64)]
+
+
+_lib = None
+
+# exported symbol -> (restype, argtypes)
+_SIGS = {
+    "dpd_last_error": (ctypes.c_char_p, []),
+    "dpd_ctx_create": (ctypes.c_int, [ctypes.POINTER(c_void_p), i64, c_i64_p, ctypes.c_int]),
+    "dpd_ctx_destroy": (ctypes.c_int, [c_void_p]),
+    "dpd_ctx_set_stream": (ctypes.c_int, [c_void_p, c_void_p]),
+    "dpd_ctx_synchronize": (ctypes.c_int, [c_void_p]),
+    "dpd_ctx_workspace_bytes": (ctypes.c_size_t, [c_void_p]),
+    "dpd_irpdso": (i64, [c_void_p, i64, c_i64_p, c_i64_p, i64]),
+    "dpd_idsymsoc": (i64, [c_void_p, i64, c_i64_p, c_i64_p, i64, c_i64_p, c_i64_p, i64]),
+    "dpd_isymoffso": (i64, [c_void_p, i64, i64, c_i64_p, c_i64_p, i64]),
+    "dpd_tensor_len": (i64, [c_void_p, ctypes.POINTER(Tensor_)]),
+    "dpd_getidx": (ctypes.c_int, [c_void_p, i64, c_i64_p, ctypes.POINTER(Tensor_)]),
+    "dpd_putidx": (i64, [c_void_p, c_i64_p, ctypes.POINTER(Tensor_)]),
+    "dpd_vv2v_dpd_444": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.POINTER(Tensor_),
+                                        ctypes.c_char_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.c_double,
+                                        ctypes.c_double]),
+    "dpd_vv2v_dpd_424": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.POINTER(Tensor_),
+                                        ctypes.c_char_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.c_double,
+                                        ctypes.c_double]),
+    "dpd_vv2v_dpd_442": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.POINTER(Tensor_),
+                                        ctypes.c_char_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.c_double,
+                                        ctypes.c_double]),
+    "dpd_vv2v_444": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.POINTER(Tensor_),
+                                    ctypes.POINTER(Tensor_), ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
+                                    ctypes.c_char_p, i64]),
+    "dpd_vv2v_222": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.POINTER(Tensor_),
+                                    ctypes.POINTER(Tensor_), ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
+                                    ctypes.c_char_p]),
+    "dpd_vv2v_422": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.POINTER(Tensor_),
+                                    ctypes.POINTER(Tensor_), ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
+                                    ctypes.c_char_p]),
+    "dpd_vv2v_224": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.POINTER(Tensor_),
+                                    ctypes.POINTER(Tensor_), ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
+                                    ctypes.c_char_p, i64]),
+    "dpd_vv2v_424": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.POINTER(Tensor_),
+                                    ctypes.POINTER(Tensor_), ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
+                                    ctypes.c_char_p, i64]),
+    "dpd_vv2v_442": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.POINTER(Tensor_),
+                                    ctypes.POINTER(Tensor_), ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
+                                    ctypes.c_char_p]),
+    "dpd_iexpso": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_i64_p, c_i64_p, i64, i64, c_i64_p, c_i64_p, i64,
+                                  i64, i64]),
+    "dpd_icompso": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_i64_p, c_i64_p, i64, i64, c_i64_p, c_i64_p, i64,
+                                   i64, i64]),
+    "dpd_isymtrso": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_i64_p, c_i64_p, i64, i64, c_i64_p, c_i64_p, i64,
+                                    i64, i64]),
+    "dpd_itranspso": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_i64_p, c_i64_p, i64, c_i64_p, c_i64_p, i64,
+                                     i64]),
+    "dpd_sstgenso": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_i64_p, c_i64_p, c_i64_p, c_i64_p, i64,
+                                    ctypes.c_char_p]),
+    "dpd_xgemm": (ctypes.c_int, [c_void_p, ctypes.c_char_p, ctypes.c_char_p, i64, i64, i64, ctypes.c_double,
+                                 c_void_p, i64, c_void_p, i64, ctypes.c_double, c_void_p, i64, ctypes.c_int]),
+    "dpd_dot": (ctypes.c_int, [c_void_p, i64, c_void_p, c_void_p, c_double_p]),
+    "dpd_nrm2": (ctypes.c_int, [c_void_p, i64, c_void_p, c_double_p]),
+    "dpd_amax": (ctypes.c_int, [c_void_p, i64, c_void_p, c_i64_p, c_double_p]),
+    "dpd_axpy": (ctypes.c_int, [c_void_p, i64, ctypes.c_double, c_void_p, c_void_p]),
+    "dpd_scal": (ctypes.c_int, [c_void_p, i64, ctypes.c_double, c_void_p]),
+    "dpd_precondition": (ctypes.c_int, [c_void_p, i64, ctypes.c_double, c_void_p, c_void_p, c_void_p]),
+    "dpd_synth_layout": (ctypes.c_int, [ctypes.POINTER(SynthSpec_)]),
+    "dpd_synth_fill": (ctypes.c_int, [ctypes.POINTER(SynthSpec_), ctypes.c_char_p, i64, c_double_p, i64]),
+    "dpd_problem_create": (ctypes.c_int, [c_void_p, ctypes.POINTER(SynthSpec_), ctypes.c_int, i64,
+                                          ctypes.POINTER(c_void_p)]),
+    "dpd_problem_destroy": (ctypes.c_int, [c_void_p]),
+    "dpd_problem_dim": (i64, [c_void_p]),
+    "dpd_problem_set_precision": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "dpd_problem_sigma": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i64]),
+    "dpd_problem_diag": (ctypes.c_int, [c_void_p, c_void_p]),
+    "dpd_problem_guess": (ctypes.c_int, [c_void_p, i64, c_void_p, c_double_p]),
+    "dpd_small_eig": (ctypes.c_int, [i64, c_double_p, c_double_p, c_double_p, c_double_p]),
+    "dpd_davidson": (ctypes.c_int, [c_void_p, ctypes.POINTER(DavidsonOpts_), c_void_p, c_double_p]),
+}
+
+# helpers exported for tests but not part of the public header
+_EXTRA = {
+    "dpd_synth_eri": (ctypes.c_int, [ctypes.POINTER(SynthSpec_), i64, c_i64_p, c_double_p]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libdpdhip.so not built at {LIB_PATH}: run __graft_entry__.build() "
+                              "(make -C revised-cluster-perturbation-single-double_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in {**_SIGS, **_EXTRA}.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise DpdError(rc, lib().dpd_last_error().decode())
+    return rc
+
+
+def public_symbols():
+    return [k for k in _SIGS]
