@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark: Davidson time to converge 5 SF-CPSd roots on one MI355X,
+mixed-precision MFMA vs FP64 MFMA, on the seeded synthetic D2h Hamiltonian with
+120 spatial orbitals [30,8,14,18,26,6,10,8], n_alpha=12, n_beta=10
+(22 occupied / 218 virtual spin orbitals, |R1|+|R2| = 699 455).
+
+One "step" is one complete Davidson solve from the spin-flip guesses with all
+integrals already resident in HBM.  The FP64 run uses FP64 MFMA for every
+contraction; the mixed run uses FP32 MFMA (FP32 operands, FP32 accumulation)
+for every contraction while the Davidson vectors, orthogonalisation, residuals,
+preconditioner and subspace eigensolve stay FP64, and finishes with one batched
+FP64 sigma of the converged vectors (FP64 Rayleigh quotients).  `value` is the
+mixed-precision seconds per solve; the FP64 time, the speedup and the per-root
+|w_mixed - w_fp64| are reported beside it.
+
+Prints ONE JSON line.  `--gpus N` with N > 1 exits: this benchmark is one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "revised-cluster-perturbation-single-double_amd"), os.path.join(ROOT, "oracle"),
+           os.path.join(ROOT, "tests", "golden")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+WORKLOADS = {
+    "cfg1": dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1, nroot=3),
+    "cfg2": dict(nso=(30, 8, 14, 18, 26, 6, 10, 8), nalpha=12, nbeta=10, seed=2, nroot=5),
+}
+PEAK_TFLOPS = {0: 78.6, 1: 157.3}  # MI355X dense MFMA: FP64, FP32
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--conv", type=float, default=1e-7, help="FP64 max-abs residual tolerance")
+    ap.add_argument("--conv-mixed", type=float, default=1e-5, help="mixed-precision residual tolerance")
+    ap.add_argument("--max-dim", type=int, default=40)
+    ap.add_argument("--max-cyc", type=int, default=200)
+    ap.add_argument("--tracking", choices=["lowest", "overlap"], default="lowest")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
+              "benchmark", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import dpdhip
+
+    cfg = WORKLOADS[args.workload]
+    nroot = cfg["nroot"]
+    lowest = args.tracking == "lowest"
+    t0 = time.time()
+    ctx = dpdhip.Context(len(cfg["nso"]), 0)
+    spec = dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"])
+    prob = dpdhip.Problem(ctx, spec, "sf-cpsd", 1)
+    guess, _ = prob.guess(nroot)
+    log(f"problem ready: dim={prob.dim} setup {time.time() - t0:.1f}s")
+
+    def solve(prec, conv, refine):
+        prob.set_precision(prec)
+        R = guess.clone()
+        w, R, info = prob.davidson(nroot, conv=conv, max_dim=args.max_dim, max_cyc=args.max_cyc, lowest=lowest,
+                                   final_check=False, R=R)
+        if refine:
+            # FP64 Rayleigh quotients of the converged vectors: one batched FP64 sigma
+            prob.set_precision("fp64")
+            S = prob.sigma(R)
+            Rm = R.view(nroot, -1)
+            w = (Rm * S.view(nroot, -1)).sum(1) / (Rm * Rm).sum(1)
+            w = w.cpu().numpy()
+        return np.asarray(w), info
+
+    def timed(prec, conv, refine, steps, warmup):
+        for _ in range(warmup):
+            solve(prec, conv, refine)
+        torch.cuda.synchronize()
+        ts = []
+        out = None
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            out = solve(prec, conv, refine)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - a)
+        return ts, out
+
+    ts64, (w64, info64) = timed("fp64", args.conv, False, args.steps, args.warmup)
+    log(f"fp64: {np.median(ts64):.3f}s/solve iterations={info64['iterations']} conv={info64['converged']} w={w64}")
+    # the same code in FP64 at the mixed run's tolerance (equal-work comparison)
+    ts64b, (w64b, info64b) = timed("fp64", args.conv_mixed, True, args.steps, args.warmup)
+    log(f"fp64@conv_mixed: {np.median(ts64b):.3f}s/solve iterations={info64b['iterations']} w={w64b}")
+    # the timed mixed-precision region also measures every GEMM launch with HIP
+    # events on the stream it runs on
+    for _ in range(args.warmup):
+        solve("fp32", args.conv_mixed, True)
+    ctx.profile(True, 200000)
+    tsm, (wm, infom) = timed("fp32", args.conv_mixed, True, args.steps, 0)
+    prof = ctx.profile_report()
+    ctx.profile(False)
+    log(f"mixed: {np.median(tsm):.3f}s/solve iterations={infom['iterations']} conv={infom['converged']} w={wm}")
+    dw = np.abs(wm - w64)
+    t_fp64 = float(np.median(ts64))
+    t_mix = float(np.median(tsm))
+
+    # dominant kernel of the measured (mixed) run
+    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    tag, st = dom
+    avg_ms = st["ms"] / st["count"]
+    flops_per_launch = st["flops_total"] / st["count"]
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[st["prec"]]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(f"{args.workload}:{tag}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, args, infom, lowest)
+
+    line = {
+        "metric": "Davidson time to 5 roots at <=1e-6 Eh: mixed-prec MFMA vs FP64 GPU",
+        "value": round(t_mix, 6),
+        "unit": "s/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_mix, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "fp32-mfma/fp64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": f"{args.workload}: SF-CPSd D2h 120 spatial orbitals [30,8,14,18,26,6,10,8] "
+                               f"n_alpha=12 n_beta=10, {nroot} roots, dim {prob.dim}",
+                   "davidson": {"tracking": args.tracking, "max_dim": args.max_dim, "max_cyc": args.max_cyc,
+                                "conv_fp64": args.conv, "conv_mixed": args.conv_mixed}},
+        "fp64_s_per_solve": round(t_fp64, 6),
+        "fp64_same_tol_s_per_solve": round(float(np.median(ts64b)), 6),
+        "speedup_mixed_vs_fp64": round(t_fp64 / t_mix, 3),
+        "speedup_mixed_vs_fp64_same_tol": round(float(np.median(ts64b)) / t_mix, 3),
+        "fp64_iterations": info64["iterations"],
+        "mixed_iterations": infom["iterations"],
+        "converged": bool(info64["converged"] and infom["converged"]),
+        "w_fp64": [round(float(x), 10) for x in w64],
+        "max_abs_dw_mixed_vs_fp64": float(dw.max()),
+        "roofline": {"bound": "mfma", "kernel": f"k_gemm<float> ({tag})", "achieved": round(achieved, 3),
+                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "flops_per_launch": int(flops_per_launch), "avg_launch_ms": round(avg_ms, 5),
+                     "launches": st["count"]},
+        "kernels": {k: {"launches": v["count"], "ms_total": round(v["ms"], 3),
+                        "tflops": round(v["flops_total"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] > 0 else None}
+                    for k, v in prof.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(cfg, args, info_gpu, lowest):
+    """The FP64 oracle (C restatement of the reference engines + OpenBLAS +
+    numpy restatement of mod_davidson.f90) on the host cores, on a bounded
+    sample: Davidson iterations of the same problem until ~cpu-seconds of work,
+    scaled by the full solve's sigma-call count."""
+    import numpy as np
+
+    import oracle as O
+    import oracle_sigma as OS
+    from make_oracle_references import oracle_problem
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    O.set_threads(cores)
+    op = oracle_problem(cfg, "sf-cpsd", 1)
+    g, _ = op.guess(cfg["nroot"])
+    calls = {"n": 0, "t": 0.0}
+
+    def sig(x):
+        a = time.perf_counter()
+        y = op.sigma(x)
+        calls["t"] += time.perf_counter() - a
+        calls["n"] += 1
+        return y
+
+    t_start = time.perf_counter()
+    budget = args.cpu_seconds
+    its = {"n": 0}
+
+    class Stop(Exception):
+        pass
+
+    def logcb(it, w, res):
+        its["n"] = it
+        if time.perf_counter() - t_start > budget:
+            raise Stop()
+
+    try:
+        OS.davidson(sig, op.diag(), g, cfg["nroot"], conv=args.conv, max_dim=args.max_dim, max_cyc=args.max_cyc,
+                    lowest=lowest, final_check=False, log=logcb)
+    except Stop:
+        pass
+    wall = time.perf_counter() - t_start
+    full_calls = info_gpu["sigma_calls"]
+    per_call = wall / max(calls["n"], 1)
+    return {"value": round(per_call * full_calls, 3), "unit": "s/solve", "cores": cores, "kind": "port",
+            "sample": f"first {its['n']} Davidson iterations ({calls['n']} FP64 sigma calls, {wall:.1f} s wall) of "
+                      f"the same SF-CPSd solve on the oracle (C engines + {O.blas_name()} BLAS, {cores} threads), "
+                      f"scaled by the {full_calls} sigma calls of the full solve"}
+
+
+if __name__ == "__main__":
+    main()

@@ -68,6 +68,12 @@ int dpd_ctx_synchronize(dpd_ctx *ctx);
 const char *dpd_last_error(void);
 /* bytes of device scratch the context currently holds (grows on demand) */
 size_t dpd_ctx_workspace_bytes(const dpd_ctx *ctx);
+/* Kernel timing (the reference's cpu_time prints, on the device): when enabled,
+ * every tagged GEMM stage of the sigma builders is bracketed by HIP events on
+ * the context stream (up to max_records launches).  The report aggregates per
+ * tag {count, ms, flops_total, prec} as JSON and clears the records. */
+int dpd_profile_enable(dpd_ctx *ctx, int on, int max_records);
+int dpd_profile_report(dpd_ctx *ctx, char *json, int64_t len);
 
 /* ---- metadata (host, no device work) ---------------------------------- */
 /* libsrint/irpdso.f:1-51 */

@@ -52,6 +52,9 @@ typedef void (*dgemm_fn)(const char *, const char *, const i64 *, const i64 *, c
                          const double *, const double *, const i64 *, const double *,
                          const i64 *, const double *, double *, const i64 *, size_t, size_t);
 static dgemm_fn g_dgemm = NULL;
+typedef void (*blas_threads_fn)(int);
+static blas_threads_fn g_blas_threads = NULL;
+static int g_threads = 0; /* 0: OpenMP default */
 
 static int set_err(int code, const char *fmt, ...) {
   va_list ap;
@@ -70,12 +73,19 @@ void orc_set_symmetry(i64 nirrep, const i64 *dirprd64) {
 }
 
 void orc_set_dgemm(void *fn) { g_dgemm = (dgemm_fn)fn; }
+void orc_set_blas_threads_fn(void *fn) { g_blas_threads = (blas_threads_fn)fn; }
+static int max_threads(void) {
+#ifdef _OPENMP
+  return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 int orc_has_blas(void) { return g_dgemm != NULL; }
 void orc_set_threads(int n) {
+  g_threads = n;
 #ifdef _OPENMP
   if (n > 0) omp_set_num_threads(n);
-#else
-  (void)n;
 #endif
 }
 
@@ -312,13 +322,55 @@ static void make_desc(orc_desc *d, const i64 *p1, const i64 *p2, const i64 *p3, 
   d->iop[0] = iop12; d->iop[1] = iop34; d->sym = sym;
 }
 
+/* decode a linear storage index into irreps/indices (inverse of lay_addr) */
+static void pair_decode(const i64 *popp, i64 iop, const i64 (*off)[9], i64 X, i64 l, i64 *hp, i64 *p, i64 *hq,
+                        i64 *q) {
+  i64 h = g_nirrep;
+  while (h > 1 && off[X][h] > l) h--;
+  i64 a = DP(h, X);
+  i64 l2 = l - off[X][h];
+  *hq = h;
+  *hp = a;
+  if (iop == 1 || a < h) {
+    *p = l2 % popp[a - 1];
+    *q = l2 / popp[a - 1];
+  } else if (iop == 0) {
+    i64 qq = 1;
+    while ((qq + 1) * qq / 2 <= l2) qq++;
+    *q = qq;
+    *p = l2 - qq * (qq - 1) / 2;
+  } else {
+    i64 qq = 0;
+    while ((qq + 1) * (qq + 2) / 2 <= l2) qq++;
+    *q = qq;
+    *p = l2 - qq * (qq + 1) / 2;
+  }
+}
+
+static void decode_elem(const orc_lay *L, i64 o, i64 *h, i64 *x) {
+  const orc_desc *d = L->d;
+  i64 xr = g_nirrep;
+  while (xr > 1 && L->blk[xr] > o) xr--;
+  i64 xl = DP(xr, d->sym);
+  i64 loc = o - L->blk[xr];
+  i64 l = loc % L->nL[xl], r = loc / L->nL[xl];
+  pair_decode(d->pop[0], d->iop[0], (const i64(*)[9])L->offL, xl, l, &h[0], &x[0], &h[1], &x[1]);
+  pair_decode(d->pop[2], d->iop[1], (const i64(*)[9])L->offR, xr, r, &h[2], &x[2], &h[3], &x[3]);
+}
+
 static void run_sort(const orc_desc *din, const double *in, const orc_desc *dout, double *out,
                      const int *perm) {
   orc_lay Li, Lo;
   lay_init(&Li, din);
   lay_init(&Lo, dout);
-  sort_u u = {&Li, in, out, {perm[0], perm[1], perm[2], perm[3]}, 0};
-  for_each_elem(&Lo, sort_cb, &u);
+  i64 n = orc_desc_len(dout);
+#pragma omp parallel for schedule(static) num_threads(max_threads())
+  for (i64 o = 0; o < n; o++) {
+    i64 h[4], x[4], hi[4], xi[4];
+    decode_elem(&Lo, o, h, x);
+    for (int i = 0; i < 4; i++) { hi[perm[i]] = h[i]; xi[perm[i]] = x[i]; }
+    out[o] = fetch_packed(&Li, in, hi, xi);
+  }
 }
 
 /* iexpso.f90:24-120: expand packed pairs (iop 0/2) of one or both sides to iop=1 */
@@ -415,7 +467,7 @@ static void gemm(char ta, char tb, i64 m, i64 n, i64 k, double alpha, const doub
     g_dgemm(&ta, &tb, &m, &n, &k, &alpha, a, &lda, b, &ldb, &beta, c, &ldc, 1, 1);
     return;
   }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (!omp_in_parallel())
   for (i64 j = 0; j < n; j++) {
     for (i64 i = 0; i < m; i++) {
       double s = 0.0;
@@ -639,31 +691,46 @@ int orc_contract_general(const double *v1, const orc_desc *d1, const char *type1
   }
   (void)ids3;
 
-  /* enumerate output tasks: irrep assignment of M and N modes */
+  /* enumerate output tasks: irrep assignment of M and N modes
+   * (the task list of vv2v_dpd_444_d.f90:299-318) */
   int ext[MAXMODE], next = 0;
   for (int i = 0; i < nM; i++) ext[next++] = idsM[i];
   for (int i = 0; i < nN; i++) ext[next++] = idsN[i];
-  i64 mirr[MAXMODE];
   i64 ecount = 1;
   for (int i = 0; i < next; i++) ecount *= g_nirrep;
   i64 kcount = 1;
   for (int i = 0; i < nK; i++) kcount *= g_nirrep;
-
+  i64 *tasks = (i64 *)malloc(sizeof(i64) * (size_t)(ecount > 0 ? ecount : 1));
+  i64 ntask = 0;
   for (i64 e = 0; e < ecount; e++) {
-    i64 r = e;
+    i64 mirr[MAXMODE] = {0}, r = e;
     int skip = 0;
     for (int i = 0; i < next; i++) {
       mirr[ext[i]] = 1 + r % g_nirrep;
       r /= g_nirrep;
       if (modes[ext[i]].size[mirr[ext[i]]] == 0) skip = 1;
     }
-    if (skip) continue;
-    for (int i = 0; i < nK; i++) mirr[idsK[i]] = 1;
-    if (!sym_ok(&T[2], mirr)) continue;
+    for (int i = 0; i < nm; i++) if (mirr[i] == 0) mirr[i] = 1;
+    if (skip || !sym_ok(&T[2], mirr)) continue;
+    tasks[ntask++] = e;
+  }
+  /* many tasks: one task per thread with single-threaded BLAS
+   * (vv2v_dpd_444_d.f90:322-363); few tasks: threaded BLAS inside each */
+  int nthr = max_threads();
+  int par = ntask >= 2 * nthr;
+  if (par && g_blas_threads) g_blas_threads(1);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(par ? nthr : 1)
+  for (i64 t = 0; t < ntask; t++) {
+    i64 mirr[MAXMODE] = {0}, r = tasks[t];
+    for (int i = 0; i < next; i++) {
+      mirr[ext[i]] = 1 + r % g_nirrep;
+      r /= g_nirrep;
+    }
+    for (int i = 0; i < nm; i++) if (mirr[i] == 0) mirr[i] = 1;
     i64 M = prod_sizes(modes, idsM, nM, mirr), N = prod_sizes(modes, idsN, nN, mirr);
     aff_t a3;
     tensor_affine(&T[2], modes, mirr, &a3);
-    if (!a3.valid) continue;
+    if (!a3.valid || M == 0 || N == 0) continue;
     i64 *o3m = (i64 *)malloc(sizeof(i64) * M), *o3n = (i64 *)malloc(sizeof(i64) * N);
     offsets_for(modes, idsM, nM, mirr, a3.stride, M, o3m);
     offsets_for(modes, idsN, nN, mirr, a3.stride, N, o3n);
@@ -692,17 +759,16 @@ int orc_contract_general(const double *v1, const orc_desc *d1, const char *type1
       offsets_for(modes, idsN, nN, mirr, a2.stride, N, o2n);
       double *A = (double *)malloc(sizeof(double) * (size_t)(M * K));
       double *B = (double *)malloc(sizeof(double) * (size_t)(K * N));
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (!par) num_threads(nthr)
       for (i64 k = 0; k < K; k++)
         for (i64 m = 0; m < M; m++) A[m + M * k] = v1[a1.base + o1m[m] + o1k[k]];
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (!par) num_threads(nthr)
       for (i64 n = 0; n < N; n++)
         for (i64 k = 0; k < K; k++) B[k + K * n] = v2[a2.base + o2k[k] + o2n[n]];
       gemm('N', 'N', M, N, K, 1.0, A, M, B, K, 1.0, C, M);
       free(A); free(B); free(o1m); free(o1k); free(o2k); free(o2n);
     }
-    /* scatter: v3 = fact2*v3 + fact1*C  (vv2v_dpd_444_d.f90 scatter) */
-#pragma omp parallel for schedule(static)
+    /* scatter: v3 = fact2*v3 + fact1*C  (vv2v_dpd_444_d.f90:486-512) */
     for (i64 n = 0; n < N; n++)
       for (i64 m = 0; m < M; m++) {
         i64 ad = a3.base + o3m[m] + o3n[n];
@@ -711,6 +777,8 @@ int orc_contract_general(const double *v1, const orc_desc *d1, const char *type1
       }
     free(C); free(o3m); free(o3n);
   }
+  if (par && g_blas_threads) g_blas_threads(nthr);
+  free(tasks);
   return 0;
 }
 

@@ -41,6 +41,8 @@ void orc_set_symmetry(i64 nirrep, const i64 *dirprd64);
 /* optional BLAS: Fortran ILP64 dgemm_ pointer (e.g. OpenBLAS scipy_dgemm_64_) */
 void orc_set_dgemm(void *fn);
 int orc_has_blas(void);
+/* optional: openblas_set_num_threads-style setter, used around task-parallel loops */
+void orc_set_blas_threads_fn(void *fn);
 void orc_set_threads(int n);
 
 /* metadata: irpdso.f, idsymsoc.f, isymoffso.f, getidx.f90, putidx.f90 */

@@ -77,6 +77,10 @@ def _try_blas(L):
     global _openblas
     _openblas = ob
     L.orc_set_dgemm(ctypes.cast(fn, ctypes.c_void_p))
+    try:
+        L.orc_set_blas_threads_fn(ctypes.cast(ob.scipy_openblas_set_num_threads64_, ctypes.c_void_p))
+    except AttributeError:
+        pass
     return True
 
 

@@ -2,6 +2,7 @@
 // metadata, engines, contract surface, sort kernels, vector primitives).
 #include <algorithm>
 #include <climits>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -405,6 +406,69 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
   if (rc) return rc;
   // the plan's device arrays are freed on return: wait for the launch
   DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  return DPD_OK;
+}
+
+// ---- profiler ------------------------------------------------------------------
+int dpd_profile_enable(dpd_ctx* c, int on, int max_records) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  for (auto e : c->prof.pool) (void)hipEventDestroy(e);
+  c->prof.pool.clear();
+  c->prof.recs.clear();
+  c->prof.used = 0;
+  c->prof.on = on != 0;
+  if (c->prof.on) {
+    int n = 2 * std::max(1, max_records);
+    c->prof.pool.resize(n);
+    for (int i = 0; i < n; i++) DPD_HIP_TRY(hipEventCreate(&c->prof.pool[i]));
+    c->prof.recs.reserve(max_records);
+  }
+  return DPD_OK;
+}
+
+// aggregate per tag: count, total milliseconds, flops per launch (of the last
+// record), precision; JSON written into buf
+int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
+  CHECK_CTX(c);
+  if (!buf || len < 3) return fail(DPD_EINVAL, "bad report buffer");
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<std::string> tags;
+  std::vector<double> ms;
+  std::vector<int64_t> cnt, fl, fltot;
+  std::vector<int> pr;
+  for (auto& r : c->prof.recs) {
+    float t = 0.f;
+    DPD_HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
+    size_t k = std::find(tags.begin(), tags.end(), r.tag) - tags.begin();
+    if (k == tags.size()) {
+      tags.push_back(r.tag);
+      ms.push_back(0);
+      cnt.push_back(0);
+      fl.push_back(0);
+      fltot.push_back(0);
+      pr.push_back(r.prec);
+    }
+    ms[k] += t;
+    cnt[k] += 1;
+    fl[k] = r.flops;
+    fltot[k] += r.flops;
+    pr[k] = r.prec;
+  }
+  std::string js = "{";
+  for (size_t k = 0; k < tags.size(); k++) {
+    char item[256];
+    snprintf(item, sizeof item, "%s\"%s\": {\"count\": %lld, \"ms\": %.6f, \"flops_total\": %lld, \"prec\": %d}",
+             k ? ", " : "", tags[k].c_str(), (long long)cnt[k], ms[k], (long long)fltot[k], pr[k]);
+    js += item;
+  }
+  js += "}";
+  if ((int64_t)js.size() + 1 > len) return fail(DPD_EINVAL, "report buffer too small (%zu)", js.size() + 1);
+  std::memcpy(buf, js.c_str(), js.size() + 1);
+  c->prof.recs.clear();
+  c->prof.used = 0;
   return DPD_OK;
 }
 

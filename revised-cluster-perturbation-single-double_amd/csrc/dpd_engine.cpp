@@ -843,7 +843,7 @@ static int build_contract_plan(const Sym& S, int kind, const Operand& o1, const 
 }
 
 int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
-                  double fact, int anti, int nbatch, int prec, int64_t* flops) {
+                  double fact, int anti, int nbatch, int prec, int64_t* flops, const char* tag) {
   std::string key = sig_of(kind, o1, o2, o3, nbatch, anti, 0);
   auto it = c->contract_cache.find(key);
   ContractPlan* P;
@@ -900,9 +900,18 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   int use_f32 = (prec != PREC_F64);
   const void* A = (use_f32 && p1f) ? (const void*)p1f : (const void*)p1;
   const void* B = (use_f32 && p2f) ? (const void*)p2f : (const void*)p2;
+  ProfRec* pr = nullptr;
+  if (c->prof.on && tag && c->prof.used + 2 <= c->prof.pool.size()) {
+    c->prof.recs.push_back({tag, c->prof.pool[c->prof.used], c->prof.pool[c->prof.used + 1], P->gemm.flops,
+                            use_f32 ? PREC_F32 : PREC_F64});
+    c->prof.used += 2;
+    pr = &c->prof.recs.back();
+    DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
+  }
   rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, 1.0,
                 use_f32 ? PREC_F32 : PREC_F64);
   if (rc) return rc;
+  if (pr) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
   if (anti) {
     MapArgs m = P->epi;
     m.nbatch = nbatch;

@@ -87,8 +87,25 @@ struct ContractPlan {
 
 }  // namespace dpd
 
+namespace dpd {
+// per-launch timing of tagged GEMM stages (HIP events on the context stream)
+struct ProfRec {
+  std::string tag;
+  hipEvent_t a, b;
+  int64_t flops;
+  int prec;
+};
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<ProfRec> recs;
+};
+}  // namespace dpd
+
 // the opaque context of the C-ABI
 struct dpd_ctx {
+  dpd::Profiler prof;
   dpd::Sym sym;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -115,7 +132,8 @@ int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, cons
                 double f1, double f2, int nbatch);
 // contract surface (mod_contract semantics) with optional batching of one operand
 int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
-                  double fact, int anti, int nbatch, int prec, int64_t* flops = nullptr);
+                  double fact, int anti, int nbatch, int prec, int64_t* flops = nullptr,
+                  const char* tag = nullptr);
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
              double f1, double f2, int prec);
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out);

@@ -90,7 +90,7 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
     // e(a,i) r(a,i)
     DPD_HIP_TRY(launch_hadamard_acc(p->l1, (const double*)p->diag.p, R, sigma, nb, p->dim, p->dim, c->stream));
     Operand W = op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->iabj.p, F(p->iabj32));
-    return contract_call(c, 444, W, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec);
+    return contract_call(c, 444, W, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec, nullptr, "c1_r1iabj");
   }
   Desc dR2 = mkdesc(v, v, o, o, 0, 0, S);
   Operand R2 = batched(op(dR2, "", 4, R + p->l1), p->dim);
@@ -102,41 +102,41 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
   Operand ijka = op(mkdesc(o, o, o, v, 0, 1, 1), "mnie", 4, (const double*)p->ijka.p, F(p->ijka32));
   // ---- singles (CISdoo.SD.SF/calAR.f90:41-56)
   // c1_r1Fae: vv2v_222(Fac, r1, c1, 'ae','ei','ai')
-  rc = contract_call(c, 444, Fac, lab(R1, "ei=-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec);
+  rc = contract_call(c, 444, Fac, lab(R1, "ei=-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec, nullptr, "c1_r1Fae");
   if (rc) return rc;
   // c1_r1Fmi: vv2v_222(r1, Fki, c1, -1, 'am','mi','ai')
-  rc = contract_call(c, 444, lab(R1, "am=+", 4), Fki, lab(C1, "ai+-", 4), -1.0, 0, nb, p->prec);
+  rc = contract_call(c, 444, lab(R1, "am=+", 4), Fki, lab(C1, "ai+-", 4), -1.0, 0, nb, p->prec, nullptr, "c1_r1Fmi");
   if (rc) return rc;
   // c1_r1iabj: vv2v_422(Wkaci, r1, c1, 'fnai','fn','ai')
-  rc = contract_call(c, 444, Wk, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec);
+  rc = contract_call(c, 444, Wk, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec, nullptr, "c1_r1iabj");
   if (rc) return rc;
   // c1_r2abci: vv2v_442(abci, r2, c1, 'feam','feim','ai')  (r2 iop 0: fact)
-  rc = contract_call(c, 442, abci, lab(R2, "feim", 4), lab(C1, "ai", 2), 1.0, 0, nb, p->prec);
+  rc = contract_call(c, 442, abci, lab(R2, "feim", 4), lab(C1, "ai", 2), 1.0, 0, nb, p->prec, nullptr, "c1_r2abci");
   if (rc) return rc;
   // c1_r2ijka: vv2v_442(r2, ijka, c1, -1, 'aemn','mnie','ai')
-  rc = contract_call(c, 442, lab(R2, "aemn", 4), ijka, lab(C1, "ai", 2), -1.0, 0, nb, p->prec);
+  rc = contract_call(c, 442, lab(R2, "aemn", 4), ijka, lab(C1, "ai", 2), -1.0, 0, nb, p->prec, nullptr, "c1_r2ijka");
   if (rc) return rc;
   // ---- doubles (CISdoo.SD.SF/calAR.f90:63-77)
   DPD_HIP_TRY(launch_hadamard_acc(p->l2, (const double*)p->diag.p + p->l1, R + p->l1, sigma + p->l1, nb, p->dim,
                                   p->dim, c->stream));
   // c2_r1abci: vv2v_424(abci, r1, c2, 'abej','ei','abij', anti=2)
-  rc = contract_call(c, 424, lab(abci, "abej", 4), lab(R1, "ei", 2), lab(C2, "abij", 4), 1.0, 2, nb, p->prec);
+  rc = contract_call(c, 424, lab(abci, "abej", 4), lab(R1, "ei", 2), lab(C2, "abij", 4), 1.0, 2, nb, p->prec, nullptr, "c2_r1abci");
   if (rc) return rc;
   // c2_r1ijka: vv2v_424(ijka, r1, c2, -1, 'ijmb','am','abij', anti=1)
-  rc = contract_call(c, 424, lab(ijka, "ijmb", 4), lab(R1, "am", 2), lab(C2, "abij", 4), -1.0, 1, nb, p->prec);
+  rc = contract_call(c, 424, lab(ijka, "ijmb", 4), lab(R1, "am", 2), lab(C2, "abij", 4), -1.0, 1, nb, p->prec, nullptr, "c2_r1ijka");
   if (rc) return rc;
   if (p->method == 2) {
     Operand W = op(mkdesc(v, o, v, o, 1, 1, 1), "embj", 4, (const double*)p->iabj.p, F(p->iabj32));
     Operand ijkl = op(mkdesc(o, o, o, o, 0, 0, 1), "mnij", 4, (const double*)p->ijkl.p, F(p->ijkl32));
     Operand abcd = op(mkdesc(v, v, v, v, 0, 0, 1), "abef", 4, (const double*)p->abcd.p, F(p->abcd32));
     // c2_r2Wmbej: vv2v_444(r2, W, c2, 'aeim','embj','abij', anti=3)
-    rc = contract_call(c, 444, lab(R2, "aeim", 4), W, lab(C2, "abij", 4), 1.0, 3, nb, p->prec);
+    rc = contract_call(c, 444, lab(R2, "aeim", 4), W, lab(C2, "abij", 4), 1.0, 3, nb, p->prec, nullptr, "c2_r2Wmbej");
     if (rc) return rc;
     // c2_r2Wmnij: vv2v_444(r2, ijkl, c2, 'abmn','mnij','abij')
-    rc = contract_call(c, 444, lab(R2, "abmn", 4), ijkl, lab(C2, "abij", 4), 1.0, 0, nb, p->prec);
+    rc = contract_call(c, 444, lab(R2, "abmn", 4), ijkl, lab(C2, "abij", 4), 1.0, 0, nb, p->prec, nullptr, "c2_r2Wmnij");
     if (rc) return rc;
     // c2_r2abcd: vv2v_444(abcd, r2, c2, 'abef','efij','abij')
-    rc = contract_call(c, 444, abcd, lab(R2, "efij", 4), lab(C2, "abij", 4), 1.0, 0, nb, p->prec);
+    rc = contract_call(c, 444, abcd, lab(R2, "efij", 4), lab(C2, "abij", 4), 1.0, 0, nb, p->prec, nullptr, "c2_r2abcd");
     if (rc) return rc;
   }
   return DPD_OK;

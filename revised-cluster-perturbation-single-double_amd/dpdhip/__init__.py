@@ -73,6 +73,16 @@ class Context:
         except Exception:
             pass
 
+    def profile(self, on=True, max_records=100000):
+        """Enable/disable per-launch HIP-event timing of the tagged GEMM stages."""
+        check(lib().dpd_profile_enable(self.h, int(on), int(max_records)))
+
+    def profile_report(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(lib().dpd_profile_report(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     # metadata ---------------------------------------------------------------
     def irpdso(self, sym, popp, popq, iop):
         a, ap = _i64(popp)

@@ -55,6 +55,8 @@ _SIGS = {
     "dpd_ctx_set_stream": (ctypes.c_int, [c_void_p, c_void_p]),
     "dpd_ctx_synchronize": (ctypes.c_int, [c_void_p]),
     "dpd_ctx_workspace_bytes": (ctypes.c_size_t, [c_void_p]),
+    "dpd_profile_enable": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int]),
+    "dpd_profile_report": (ctypes.c_int, [c_void_p, ctypes.c_char_p, i64]),
     "dpd_irpdso": (i64, [c_void_p, i64, c_i64_p, c_i64_p, i64]),
     "dpd_idsymsoc": (i64, [c_void_p, i64, c_i64_p, c_i64_p, i64, c_i64_p, c_i64_p, i64]),
     "dpd_isymoffso": (i64, [c_void_p, i64, i64, c_i64_p, c_i64_p, i64]),
