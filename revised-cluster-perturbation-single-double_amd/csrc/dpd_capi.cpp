@@ -442,9 +442,10 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
   for (auto& r : c->prof.recs) {
     float t = 0.f;
     DPD_HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
-    size_t k = std::find(tags.begin(), tags.end(), r.tag) - tags.begin();
+    std::string key = r.tag + (r.prec == PREC_F64 ? ":fp64" : ":fp32");
+    size_t k = std::find(tags.begin(), tags.end(), key) - tags.begin();
     if (k == tags.size()) {
-      tags.push_back(r.tag);
+      tags.push_back(key);
       ms.push_back(0);
       cnt.push_back(0);
       fl.push_back(0);
