@@ -65,6 +65,11 @@ int dpd_ctx_destroy(dpd_ctx *ctx);
 /* hipStream_t as void*; NULL = the default stream */
 int dpd_ctx_set_stream(dpd_ctx *ctx, void *stream);
 int dpd_ctx_synchronize(dpd_ctx *ctx);
+/* GEMM tile configuration of every later plan: -1 automatic (default), 0 the
+ * first-generation 64x64 kernel, 1..4 the pipelined kernel with 64x64, 128x64,
+ * 64x128, 128x128 tiles.  Clears the context's plan caches.  No counterpart in
+ * the reference (its GEMMs are cuBLAS/BLAS calls); used for tuning and tests. */
+int dpd_ctx_set_gemm_config(dpd_ctx *ctx, int cfg);
 const char *dpd_last_error(void);
 /* bytes of device scratch the context currently holds (grows on demand) */
 size_t dpd_ctx_workspace_bytes(const dpd_ctx *ctx);

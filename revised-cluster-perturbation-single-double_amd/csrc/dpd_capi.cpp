@@ -394,8 +394,9 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
   sg.ktiles = (int32_t)((k + 15) / 16);
   tk.nseg = 1;
   tk.ktiles_total = sg.ktiles;
-  tk.tiles_m = (int32_t)((m + 63) / 64);
-  tk.tiles_n = (int32_t)((n + 63) / 64);
+  p.cfg = choose_gemm_cfg({tk}, c->gemm_cfg);
+  tk.tiles_m = (int32_t)((m + gemm_bm(p.cfg) - 1) / gemm_bm(p.cfg));
+  tk.tiles_n = (int32_t)((n + gemm_bn(p.cfg) - 1) / gemm_bn(p.cfg));
   tk.ksplit = 1;
   p.tasks.push_back(tk);
   p.segs.push_back(sg);
@@ -406,6 +407,16 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
   if (rc) return rc;
   // the plan's device arrays are freed on return: wait for the launch
   DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  return DPD_OK;
+}
+
+int dpd_ctx_set_gemm_config(dpd_ctx* c, int cfg) {
+  CHECK_CTX(c);
+  if (cfg < -1 || cfg > GEMM_128x128) return fail(DPD_EINVAL, "GEMM configuration must be -1..4");
+  if (c->device >= 0) DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->gemm_cfg = cfg;
+  c->contract_cache.clear();
+  c->engine_cache.clear();
   return DPD_OK;
 }
 

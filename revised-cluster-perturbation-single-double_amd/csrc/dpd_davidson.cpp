@@ -252,7 +252,7 @@ int small_eig(int n, const double* Arow, double* wr, double* wi, double* V) {
 // Davidson
 // ---------------------------------------------------------------------------
 struct DavWork {
-  DevBuf B, AB, NB, NAB, q, aq, e, p, coef, part, dots, amaxv, amaxi, scal;
+  DevBuf B, AB, NB, NAB, AQ, E, P, coef, part, partA, partB, dots, amaxp, amaxpi, amaxv, amaxi, wdev, idx;
 };
 
 static int dsync(dpd_ctx* c) {
@@ -280,66 +280,72 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
   const double maxError = o->conv > 0 ? o->conv : 1e-8;
   if (nroot < 1 || nroot > 64) return fail(DPD_EINVAL, "nroot must be 1..64");
   if (nroot >= dimAbar) return fail(DPD_EINVAL, "ERROR: too much roots required %d %d", nroot, dimAbar);
+  if (dimAbar > 1024) return fail(DPD_EINVAL, "max_dim must be <= 1024");
   const int dimDel = dimAbar - nroot - 5;
   hipStream_t st = c->stream;
   DavWork W;
   int rc;
   const size_t vb = sizeof(double) * (size_t)n;
+  const size_t nd2 = (size_t)dimAbar * dimAbar;
   if ((rc = W.B.alloc(vb * dimAbar)) || (rc = W.AB.alloc(vb * dimAbar)) || (rc = W.NB.alloc(vb * dimAbar)) ||
-      (rc = W.NAB.alloc(vb * dimAbar)) || (rc = W.q.alloc(vb)) || (rc = W.aq.alloc(vb)) || (rc = W.e.alloc(vb)) ||
-      (rc = W.p.alloc(vb)) || (rc = W.coef.alloc(sizeof(double) * dimAbar * dimAbar)) ||
-      (rc = W.part.alloc(sizeof(double) * kRedBlocks * dimAbar)) || (rc = W.dots.alloc(sizeof(double) * 2 * dimAbar)) ||
-      (rc = W.amaxv.alloc(sizeof(double) * (kRedBlocks + 8))) || (rc = W.amaxi.alloc(sizeof(int64_t) * (kRedBlocks + 8))))
+      (rc = W.NAB.alloc(vb * dimAbar)) || (rc = W.AQ.alloc(vb * nroot)) || (rc = W.E.alloc(vb * nroot)) ||
+      (rc = W.P.alloc(vb * nroot)) || (rc = W.coef.alloc(sizeof(double) * nd2)) ||
+      (rc = W.part.alloc(sizeof(double) * kRedBlocks * nd2)) || (rc = W.partA.alloc(sizeof(double) * kRedBlocks)) ||
+      (rc = W.partB.alloc(sizeof(double) * kRedBlocks)) || (rc = W.dots.alloc(sizeof(double) * 2 * nd2)) ||
+      (rc = W.amaxp.alloc(sizeof(double) * kRedBlocks * nroot)) ||
+      (rc = W.amaxpi.alloc(sizeof(int64_t) * kRedBlocks * nroot)) || (rc = W.amaxv.alloc(sizeof(double) * nroot)) ||
+      (rc = W.amaxi.alloc(sizeof(int64_t) * nroot)) || (rc = W.wdev.alloc(sizeof(double) * nroot)) ||
+      (rc = W.idx.alloc(sizeof(int32_t) * nroot)))
     return rc;
   double* B = (double*)W.B.p;
   double* AB = (double*)W.AB.p;
-  double* q = (double*)W.q.p;
-  double* aq = (double*)W.aq.p;
-  double* e = (double*)W.e.p;
-  double* p = (double*)W.p.p;
+  double* AQ = (double*)W.AQ.p;
+  double* E = (double*)W.E.p;
+  double* PV = (double*)W.P.p;
   double* dots = (double*)W.dots.p;
   double* part = (double*)W.part.p;
+  double* wdev = (double*)W.wdev.p;
   const double* diag = (const double*)P->diag.p;
   auto bv = [&](int i) { return B + (size_t)i * n; };
   auto abv = [&](int i) { return AB + (size_t)i * n; };
-  // host helpers
-  auto dot_many = [&](const double* x, const double* y0, int nv, double* host) -> int {
-    DPD_HIP_TRY(launch_multidot(n, x, y0, n, nv, part, dots, st));
-    DPD_HIP_TRY(hipMemcpyAsync(host, dots, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
-    return dsync(c);
-  };
-  auto nrm2 = [&](const double* x, double* out) -> int {
-    double s;
-    int r = dot_many(x, x, 1, &s);
-    *out = std::sqrt(s);
-    return r;
-  };
   int64_t nsig = 0;
   auto sigma = [&](const double* x, double* y, int nv) -> int {
     nsig += nv;
     return problem_sigma(P, x, y, nv);
   };
+  // all dots X_i . Y_j (i < nx, j < ny) -> host[i + j*nx], one synchronisation
+  auto block_dots = [&](const double* X, int nx, const double* Y, int ny, double* host) -> int {
+    DPD_HIP_TRY(launch_block_dots(n, X, n, nx, Y, n, ny, part, dots, st));
+    DPD_HIP_TRY(hipMemcpyAsync(host, dots, sizeof(double) * nx * ny, hipMemcpyDeviceToHost, st));
+    return dsync(c);
+  };
+  // e_v = A r_v - w_v r_v and max|e_v| for v < nroot (w already on the device)
+  auto residuals = [&](const double* AR, double* res_host) -> int {
+    DPD_HIP_TRY(launch_resid_amax(n, AR, R, wdev, E, nroot, (double*)W.amaxp.p, (int64_t*)W.amaxpi.p,
+                                  (double*)W.amaxv.p, (int64_t*)W.amaxi.p, st));
+    DPD_HIP_TRY(hipMemcpyAsync(res_host, W.amaxv.p, sizeof(double) * nroot, hipMemcpyDeviceToHost, st));
+    return dsync(c);
+  };
 
-  std::vector<double> Abar((size_t)dimAbar * dimAbar, 0.0);  // column-major Abar(i,j) = [i + j*dimAbar]
+  std::vector<double> Abar(nd2, 0.0);  // column-major Abar(i,j) = [i + j*dimAbar]
   auto A_ = [&](int i, int j) -> double& { return Abar[i + (size_t)j * dimAbar]; };
   std::vector<double> vecNeed((size_t)dimAbar * nroot, 0.0), eigenVal(nroot, 0.0), res(nroot, 0.0);
+  std::vector<double> hd(2 * nd2);
   // initDavidson (:563-610): b_i = guess_i, Ab_i = sigma(b_i), Abar(j,i) = g_j . Ab_i
   DPD_HIP_TRY(hipMemcpyAsync(B, R, vb * nroot, hipMemcpyDeviceToDevice, st));
   if ((rc = sigma(B, AB, nroot))) return rc;
-  {
-    std::vector<double> h(nroot);
-    for (int i = 0; i < nroot; i++) {
-      if ((rc = dot_many(abv(i), R, nroot, h.data()))) return rc;
-      for (int j = 0; j < nroot; j++) A_(j, i) = h[j];
-      vecNeed[i + (size_t)i * dimAbar] = 1.0;
-    }
+  if ((rc = block_dots(AB, nroot, B, nroot, hd.data()))) return rc;
+  for (int i = 0; i < nroot; i++) {
+    for (int j = 0; j < nroot; j++) A_(j, i) = hd[i + (size_t)j * nroot];
+    vecNeed[i + (size_t)i * dimAbar] = 1.0;
   }
   int iterations = 1;
   int dimIter = nroot, dimFree = dimAbar - dimIter;
   bool dim_exceed = false, converged = false;
   int ndimIter = 0, ndimFree = 0;
-  std::vector<double> wr(dimAbar), wi(dimAbar), V((size_t)dimAbar * dimAbar), sub((size_t)dimAbar * dimAbar);
+  std::vector<double> wr(dimAbar), wi(dimAbar), V(nd2), sub(nd2), nrm(nroot), mv(nroot);
   std::vector<int> lowestE(dimAbar);
+  std::vector<int32_t> cand;
   while (true) {
     if (iterations > maxcyc) break;
     // eigenproblem of Abar(1:dimIter,1:dimIter) (:185-219)
@@ -396,66 +402,72 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
       std::swap(W.AB.p, W.NAB.p);
       B = (double*)W.B.p;
       AB = (double*)W.AB.p;
-      if ((rc = dsync(c))) return rc;
       dimFree = ndimFree;
       dimIter = ndimIter;
-      std::vector<double> h(dimIter);
-      for (int i = 0; i < dimIter; i++) {
-        if ((rc = dot_many(abv(i), B, dimIter, h.data()))) return rc;
-        for (int j = 0; j < dimIter; j++) A_(j, i) = h[j];
+      if ((rc = block_dots(AB, dimIter, B, dimIter, hd.data()))) return rc;
+      for (int i = 0; i < dimIter; i++)
+        for (int j = 0; j < dimIter; j++) A_(j, i) = hd[i + (size_t)j * dimIter];
+    }
+    // q and Aq of every root (:352-377); q is written straight into R, the
+    // eigenvector output the reference rewrites every iteration (:388-390)
+    if (dim_exceed) {
+      DPD_HIP_TRY(hipMemcpyAsync(R, B, vb * nroot, hipMemcpyDeviceToDevice, st));
+      DPD_HIP_TRY(hipMemcpyAsync(AQ, AB, vb * nroot, hipMemcpyDeviceToDevice, st));
+    } else {
+      DPD_HIP_TRY(hipMemcpyAsync(W.coef.p, vecNeed.data(), sizeof(double) * dimAbar * nroot, hipMemcpyHostToDevice,
+                                 st));
+      DPD_HIP_TRY(launch_lincomb(n, AB, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, AQ, st));
+      DPD_HIP_TRY(launch_lincomb(n, B, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, R, st));
+    }
+    DPD_HIP_TRY(launch_pair_dots(n, R, n, R, n, nroot, part, dots, st));
+    DPD_HIP_TRY(hipMemcpyAsync(nrm.data(), dots, sizeof(double) * nroot, hipMemcpyDeviceToHost, st));
+    if ((rc = dsync(c))) return rc;
+    for (int iroot = 0; iroot < nroot; iroot++) {
+      const double t = std::sqrt(nrm[iroot]);
+      if (std::fabs(t - 1.0) > 1e-10) {
+        DPD_HIP_TRY(launch_scal(n, 1.0 / t, R + (size_t)iroot * n, st));
+        DPD_HIP_TRY(launch_scal(n, 1.0 / t, AQ + (size_t)iroot * n, st));
       }
     }
-    int nNewBase = 0;
+    // residuals (:396-406)
+    DPD_HIP_TRY(hipMemcpyAsync(wdev, eigenVal.data(), sizeof(double) * nroot, hipMemcpyHostToDevice, st));
+    if ((rc = residuals(AQ, mv.data()))) return rc;
     int nconv = 0;
+    cand.clear();
     for (int iroot = 0; iroot < nroot; iroot++) {
-      // q and Aq (:352-377)
-      if (dim_exceed) {
-        DPD_HIP_TRY(launch_copy(n, abv(iroot), aq, st));
-        DPD_HIP_TRY(launch_copy(n, bv(iroot), q, st));
-      } else {
-        DPD_HIP_TRY(hipMemcpyAsync(W.coef.p, &vecNeed[(size_t)iroot * dimAbar], sizeof(double) * dimIter,
-                                   hipMemcpyHostToDevice, st));
-        DPD_HIP_TRY(launch_lincomb(n, AB, n, dimIter, (const double*)W.coef.p, dimIter, 1, aq, st));
-        DPD_HIP_TRY(launch_lincomb(n, B, n, dimIter, (const double*)W.coef.p, dimIter, 1, q, st));
-      }
-      double t;
-      if ((rc = nrm2(q, &t))) return rc;
-      if (std::fabs(t - 1.0) > 1e-10) {
-        DPD_HIP_TRY(launch_scal(n, 1.0 / t, q, st));
-        DPD_HIP_TRY(launch_scal(n, 1.0 / t, aq, st));
-      }
-      // eigenvector written out every iteration (:388-390)
-      DPD_HIP_TRY(launch_copy(n, q, R + (size_t)iroot * n, st));
-      // residual (:396-406)
-      DPD_HIP_TRY(launch_copy(n, aq, e, st));
-      DPD_HIP_TRY(launch_axpy(n, -eigenVal[iroot], q, e, st));
-      double mv;
-      int64_t mi;
-      DPD_HIP_TRY(launch_amax(n, e, (double*)W.amaxv.p, (int64_t*)W.amaxi.p, (double*)W.amaxv.p + kRedBlocks,
-                              (int64_t*)W.amaxi.p + kRedBlocks, st));
-      DPD_HIP_TRY(hipMemcpyAsync(&mv, (double*)W.amaxv.p + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, st));
-      DPD_HIP_TRY(hipMemcpyAsync(&mi, (int64_t*)W.amaxi.p + kRedBlocks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-      if ((rc = dsync(c))) return rc;
-      res[iroot] = std::fabs(mv);
-      if (std::fabs(mv) < maxError) {
-        nconv++;
-        continue;
-      }
+      res[iroot] = std::fabs(mv[iroot]);
+      if (std::fabs(mv[iroot]) < maxError) nconv++;
+      else cand.push_back(iroot);
+    }
+    // preconditioned directions of every unconverged root (:417-428)
+    int nNewBase = 0;
+    if (dimFree > 0 && !cand.empty()) {
+      DPD_HIP_TRY(hipMemcpyAsync(W.idx.p, cand.data(), sizeof(int32_t) * cand.size(), hipMemcpyHostToDevice, st));
+      DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p, (int)cand.size(), diag, E, PV, st));
+    }
+    // SchmidtS (:687-726) root by root: normalise, modified Gram-Schmidt against
+    // every basis vector (the new ones of this iteration included), normalise;
+    // skipped when the remaining norm is below 1e-6.  Each projection step is one
+    // launch that finishes the previous reduction, updates p and starts the next.
+    for (size_t k = 0; k < cand.size(); k++) {
       if (dimFree == 0) continue;
-      // preconditioner (:417-428)
-      DPD_HIP_TRY(launch_precondition(n, eigenVal[iroot], diag, e, p, st));
-      // SchmidtS (:687-726): normalise, modified Gram-Schmidt, normalise
-      double nrm;
-      if ((rc = nrm2(p, &nrm))) return rc;
-      DPD_HIP_TRY(launch_scal(n, 1.0 / nrm, p, st));
-      for (int i = 0; i < dimIter + nNewBase; i++) {
-        DPD_HIP_TRY(launch_multidot(n, p, bv(i), n, 1, part, dots, st));
-        DPD_HIP_TRY(launch_axpy_dev(n, dots, -1.0, bv(i), p, st));
+      double* p = PV + k * n;
+      double* pin = (double*)W.partA.p;
+      double* pout = (double*)W.partB.p;
+      DPD_HIP_TRY(launch_pair_dots_partial(n, p, p, pin, st));
+      const int nb = dimIter + nNewBase;
+      for (int i = 0; i <= nb; i++) {
+        const double* bprev = i > 0 ? bv(i - 1) : nullptr;
+        const double* bnext = i < nb ? bv(i) : nullptr;
+        DPD_HIP_TRY(launch_mgs_step(n, p, bprev, bnext, pin, i == 0 ? 0 : 1, pout, st));
+        std::swap(pin, pout);
       }
-      if ((rc = nrm2(p, &nrm))) return rc;
-      if (nrm < 1e-6) continue;
-      DPD_HIP_TRY(launch_scal(n, 1.0 / nrm, p, st));
-      DPD_HIP_TRY(launch_copy(n, p, bv(dimIter + nNewBase), st));
+      double s2;
+      DPD_HIP_TRY(launch_reduce_partials(pin, 1, dots, st));
+      DPD_HIP_TRY(hipMemcpyAsync(&s2, dots, sizeof(double), hipMemcpyDeviceToHost, st));
+      if ((rc = dsync(c))) return rc;
+      if (std::sqrt(s2) < 1e-6) continue;
+      DPD_HIP_TRY(launch_scale_by_norm(n, p, pin, bv(dimIter + nNewBase), st));
       dimFree--;
       nNewBase++;
     }
@@ -477,17 +489,18 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
         ndimFree = dimDel;
       }
     }
-    // Abar update (:484-509)
+    // Abar update (:484-509): new rows against the old columns, then the new
+    // columns against every basis vector
     if (nNewBase > 0) {
-      std::vector<double> h(dimIter + nNewBase);
-      for (int j = 0; j < dimIter; j++) {
-        if ((rc = dot_many(abv(j), bv(dimIter), nNewBase, h.data()))) return rc;
-        for (int i = 0; i < nNewBase; i++) A_(dimIter + i, j) = h[i];
-      }
-      for (int j = dimIter; j < dimIter + nNewBase; j++) {
-        if ((rc = dot_many(abv(j), B, dimIter + nNewBase, h.data()))) return rc;
-        for (int i = 0; i < dimIter + nNewBase; i++) A_(i, j) = h[i];
-      }
+      const int nt = dimIter + nNewBase;
+      DPD_HIP_TRY(launch_block_dots(n, AB, n, dimIter, bv(dimIter), n, nNewBase, part, dots, st));
+      DPD_HIP_TRY(launch_block_dots(n, B, n, nt, abv(dimIter), n, nNewBase, part, dots + nd2, st));
+      DPD_HIP_TRY(hipMemcpyAsync(hd.data(), dots, sizeof(double) * 2 * nd2, hipMemcpyDeviceToHost, st));
+      if ((rc = dsync(c))) return rc;
+      for (int j = 0; j < dimIter; j++)
+        for (int i = 0; i < nNewBase; i++) A_(dimIter + i, j) = hd[j + (size_t)i * dimIter];
+      for (int j = 0; j < nNewBase; j++)
+        for (int i = 0; i < nt; i++) A_(i, dimIter + j) = hd[nd2 + i + (size_t)j * nt];
     }
     dimIter += nNewBase;
     iterations++;
@@ -498,16 +511,9 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
   // final check (:527-542): e = A R - w R
   if (o->final_check) {
     if ((rc = sigma(R, (double*)W.NAB.p, nroot))) return rc;
-    for (int i = 0; i < nroot; i++) {
-      double* ev = (double*)W.NAB.p + (size_t)i * n;
-      DPD_HIP_TRY(launch_axpy(n, -eigenVal[i], R + (size_t)i * n, ev, st));
-      double mv;
-      DPD_HIP_TRY(launch_amax(n, ev, (double*)W.amaxv.p, (int64_t*)W.amaxi.p, (double*)W.amaxv.p + kRedBlocks,
-                              (int64_t*)W.amaxi.p + kRedBlocks, st));
-      DPD_HIP_TRY(hipMemcpyAsync(&mv, (double*)W.amaxv.p + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, st));
-      if ((rc = dsync(c))) return rc;
-      if (i < 64) o->final_residual[i] = std::fabs(mv);
-    }
+    DPD_HIP_TRY(hipMemcpyAsync(wdev, eigenVal.data(), sizeof(double) * nroot, hipMemcpyHostToDevice, st));
+    if ((rc = residuals((const double*)W.NAB.p, mv.data()))) return rc;
+    for (int i = 0; i < nroot && i < 64; i++) o->final_residual[i] = std::fabs(mv[i]);
   }
   o->sigma_calls = nsig;
   if ((rc = dsync(c))) return rc;

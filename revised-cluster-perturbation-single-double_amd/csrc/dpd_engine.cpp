@@ -21,6 +21,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 
 namespace dpd {
 
@@ -449,8 +450,6 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
     }
     tk.nseg = (int32_t)plan.segs.size() - tk.seg_begin;
     tk.ktiles_total = ktile;
-    tk.tiles_m = (int32_t)((M + 63) / 64);
-    tk.tiles_n = (int32_t)((N + 63) / 64);
     tk.ksplit = 1;
     plan.tasks.push_back(tk);
   }
@@ -467,6 +466,15 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
       tk.nseg = 1;
       plan.segs.push_back(sg);
     }
+  }
+  // tile shape: the largest tiles whose padding waste stays small; the
+  // pipelined kernel handles every task shape, the first-generation kernel is
+  // kept selectable (dpd_ctx_set_gemm_config) for comparison
+  plan.cfg = choose_gemm_cfg(plan.tasks, plan.cfg_req);
+  const int BM = gemm_bm(plan.cfg), BN = gemm_bn(plan.cfg);
+  for (auto& tk : plan.tasks) {
+    tk.tiles_m = (tk.M + BM - 1) / BM;
+    tk.tiles_n = (tk.N + BN - 1) / BN;
   }
   // split-K: balance K tiles so the launch fills the chip
   int64_t work = 0, tiles = 0;
@@ -528,6 +536,32 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
   return plan.upload();
 }
 
+int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested) {
+  if (requested >= GEMM_V1 && requested <= GEMM_128x128) return requested;
+  // cost ~ padded MFMA work / relative tile efficiency, plus a penalty when the
+  // tile count cannot fill 256 CUs (split-K recovers part of that)
+  static const int cand[4] = {GEMM_128x128, GEMM_128x64, GEMM_64x128, GEMM_64x64};
+  static const double eff[4] = {1.0, 0.85, 0.85, 0.7};
+  int best = GEMM_64x64;
+  double best_cost = 0.0;
+  for (int ci = 0; ci < 4; ci++) {
+    const int bm = gemm_bm(cand[ci]), bn = gemm_bn(cand[ci]);
+    double padded = 0.0, ntile = 0.0;
+    for (const auto& tk : tasks) {
+      const double tmn = (double)((tk.M + bm - 1) / bm) * ((tk.N + bn - 1) / bn);
+      padded += tmn * bm * bn * std::max(tk.ktiles_total, 1);
+      ntile += tmn;
+    }
+    double cost = padded / eff[ci];
+    if (ntile < 512) cost *= 1.0 + 0.25 * (512 - ntile) / 512;
+    if (ci == 0 || cost < best_cost) {
+      best = cand[ci];
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
              double f1, double f2, int prec) {
   if (p.nwg == 0) return DPD_OK;
@@ -548,6 +582,7 @@ int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, i
   g.split_tasks = p.d_split;
   g.nsplit = (int)p.split.size();
   g.max_split_mn = p.max_split_mn;
+  g.cfg = p.cfg;
   if (p.partial_elems > 0) g.partial = (double*)ws_take(c, sizeof(double) * p.partial_elems);
   DPD_HIP_TRY(launch_gemm(g, c->stream));
   return DPD_OK;
@@ -695,6 +730,7 @@ int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, cons
   GemmPlan* p;
   if (it == c->engine_cache.end()) {
     auto np = std::make_unique<GemmPlan>();
+    np->cfg_req = c->gemm_cfg;
     rc = build_gemm_plan(c->sym, m1, m2, m3, nbatch, *np);
     if (rc) return rc;
     p = np.get();
@@ -849,6 +885,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   ContractPlan* P;
   if (it == c->contract_cache.end()) {
     auto np = std::make_unique<ContractPlan>();
+    np->gemm.cfg_req = c->gemm_cfg;
     int rc = build_contract_plan(c->sym, kind, o1, o2, o3, anti, nbatch, *np);
     if (rc) return rc;
     P = np.get();

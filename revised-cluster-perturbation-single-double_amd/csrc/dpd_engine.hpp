@@ -42,6 +42,8 @@ struct GemmPlan {
   std::vector<GemmSeg> segs;
   std::vector<int32_t> split;
   int nwg = 0;
+  int cfg = 0;       // GemmCfg tile configuration chosen by the planner
+  int cfg_req = -1;  // requested configuration (-1: planner's choice)
   int max_split_mn = 0;
   int64_t partial_elems = 0;
   int64_t flops = 0;  // 2*M*N*K summed (algorithmic, per launch)
@@ -110,6 +112,7 @@ struct dpd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int64_t max_threads_block = 256;
+  int gemm_cfg = -1;  // GEMM tile configuration override (-1: automatic)
   // scratch: grow-only device workspace (bump allocated per top-level call)
   dpd::DevBuf ws;
   size_t ws_top = 0;
@@ -136,6 +139,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
                   const char* tag = nullptr);
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
              double f1, double f2, int prec);
+int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested);
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out);
 // labels / checks shared by the C-ABI and the sigma builders
 int check_labels(const char* t, int n);

@@ -275,7 +275,10 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(GemmLaunch g) {
 
 hipError_t launch_gemm(const GemmLaunch& g, hipStream_t s) {
   if (g.nwg == 0) return hipSuccess;
-  if (g.prec == PREC_F64) {
+  if (g.cfg != GEMM_V1) {
+    hipError_t e2 = launch_gemm2(g, s);
+    if (e2 != hipSuccess) return e2;
+  } else if (g.prec == PREC_F64) {
     if (g.a_f32 || g.b_f32) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_gemm<double, double, double>), dim3(g.nwg), dim3(256), 0, s, g);
   } else {
@@ -450,15 +453,27 @@ __global__ void k_copy(int64_t n, const double* __restrict__ x, double* __restri
        i += (int64_t)gridDim.x * blockDim.x)
     y[i] = x[i];
 }
-// out_j = sum_k coef[k + j*ldc] X_k  (in the reference's order: k ascending)
+// out_j = sum_k coef[k + j*ldc] X_k  (in the reference's order: k ascending); every thread
+// forms up to kLinOut outputs so each X element is read once per grid row
+constexpr int kLinOut = 8;
 __global__ void k_lincomb(int64_t n, const double* __restrict__ X, int64_t xs, int nk,
-                          const double* __restrict__ coef, int ldc, double* __restrict__ out) {
-  const int j = blockIdx.y;
+                          const double* __restrict__ coef, int ldc, int nout, double* __restrict__ out) {
+  const int j0 = blockIdx.y * kLinOut;
+  const int nj = min(kLinOut, nout - j0);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int k = 0; k < nk; k++) s += coef[k + (int64_t)j * ldc] * X[(int64_t)k * xs + i];
-    out[(int64_t)j * n + i] = s;
+    double s[kLinOut];
+#pragma unroll
+    for (int j = 0; j < kLinOut; j++) s[j] = 0.0;
+    for (int k = 0; k < nk; k++) {
+      const double x = X[(int64_t)k * xs + i];
+#pragma unroll
+      for (int j = 0; j < kLinOut; j++)
+        if (j < nj) s[j] += coef[k + (int64_t)(j0 + j) * ldc] * x;
+    }
+#pragma unroll
+    for (int j = 0; j < kLinOut; j++)
+      if (j < nj) out[(int64_t)(j0 + j) * n + i] = s[j];
   }
 }
 // lib_cc/mod_davidson.f90:417-428
@@ -521,8 +536,8 @@ hipError_t launch_copy(int64_t n, const double* x, double* y, hipStream_t s) {
 hipError_t launch_lincomb(int64_t n, const double* X, int64_t xs, int nk, const double* coef,
                           int ldc, int nout, double* out, hipStream_t s) {
   if (nout == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n), nout), dim3(256), 0, s, n, X, xs, nk, coef, ldc,
-                     out);
+  hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n), (nout + kLinOut - 1) / kLinOut), dim3(256), 0, s, n, X, xs,
+                     nk, coef, ldc, nout, out);
   return hipGetLastError();
 }
 hipError_t launch_precondition(int64_t n, double w, const double* d, const double* e, double* p,
@@ -583,6 +598,210 @@ hipError_t launch_orbital_energy(const DevLayout& L, const double* eorb, const E
 hipError_t launch_div(int64_t n, const double* a, const double* b, double* c, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_div, dim3(grid_for(n)), dim3(256), 0, s, n, a, b, c);
+  return hipGetLastError();
+}
+
+}  // namespace dpd
+
+namespace dpd {
+
+__device__ inline double block_sum_256(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return s;
+}
+
+// every block reduces the same kRedBlocks partials in the same order
+__device__ inline double reduce_parts(const double* part, double* red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) s += part[i];
+  return block_sum_256(s, red);
+}
+
+__global__ __launch_bounds__(256) void k_pair_dots(int64_t n, const double* __restrict__ x, int64_t xs,
+                                                   const double* __restrict__ y, int64_t ys, double* partial) {
+  __shared__ double red[4];
+  const int v = blockIdx.y;
+  const double* xv = x + (int64_t)v * xs;
+  const double* yv = y + (int64_t)v * ys;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += xv[i] * yv[i];
+  s = block_sum_256(s, red);
+  if (threadIdx.x == 0) partial[(int64_t)v * gridDim.x + blockIdx.x] = s;
+}
+
+// out[xi + yj*nx] = X_xi . Y_yj; grid (kRedBlocks, ceil(nx/4), ceil(ny/4)), 4x4 dots per thread
+__global__ __launch_bounds__(256) void k_block_dots(int64_t n, const double* __restrict__ X, int64_t xs, int nx,
+                                                    const double* __restrict__ Y, int64_t ys, int ny,
+                                                    double* partial) {
+  __shared__ double red[4][16];
+  const int x0 = blockIdx.y * 4, y0 = blockIdx.z * 4;
+  const int nxl = min(4, nx - x0), nyl = min(4, ny - y0);
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double xv[4], yv[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) xv[a] = a < nxl ? X[(int64_t)(x0 + a) * xs + i] : 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) yv[b] = b < nyl ? Y[(int64_t)(y0 + b) * ys + i] : 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) acc[a][b] += xv[a] * yv[b];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      double v = wave_sum(acc[a][b]);
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][a * 4 + b] = v;
+    }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int a = threadIdx.x >> 2, b = threadIdx.x & 3;
+    if (a < nxl && b < nyl) {
+      const double v = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+      partial[((int64_t)(x0 + a) + (int64_t)(y0 + b) * nx) * gridDim.x + blockIdx.x] = v;
+    }
+  }
+}
+
+hipError_t launch_block_dots(int64_t n, const double* X, int64_t xs, int nx, const double* Y, int64_t ys, int ny,
+                             double* partial, double* out, hipStream_t s) {
+  if (nx <= 0 || ny <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_block_dots, dim3(kRedBlocks, (nx + 3) / 4, (ny + 3) / 4), dim3(256), 0, s, n, X, xs, nx, Y,
+                     ys, ny, partial);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nx * ny), dim3(256), 0, s, partial, kRedBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_dots(int64_t n, const double* x, int64_t xs, const double* y, int64_t ys, int nv,
+                            double* partial, double* out, hipStream_t s) {
+  if (nv == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pair_dots, dim3(kRedBlocks, nv), dim3(256), 0, s, n, x, xs, y, ys, partial);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nv), dim3(256), 0, s, partial, kRedBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_dots_partial(int64_t n, const double* x, const double* y, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_dots, dim3(kRedBlocks, 1), dim3(256), 0, s, n, x, (int64_t)0, y, (int64_t)0, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(const double* partial, int nrows, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(256), 0, s, partial, kRedBlocks, out);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_resid_amax_partial(int64_t n, const double* __restrict__ aq,
+                                                            const double* __restrict__ q, const double* w,
+                                                            double* __restrict__ e, double* pv, int64_t* pi) {
+  __shared__ double sv[256];
+  __shared__ int64_t si[256];
+  const int v = blockIdx.y;
+  const double wv = w[v];
+  const double* aqv = aq + (int64_t)v * n;
+  const double* qv = q + (int64_t)v * n;
+  double* ev = e + (int64_t)v * n;
+  double best = 0.0;
+  int64_t bi = INT64_MAX;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double r = aqv[i] + (-wv) * qv[i];
+    ev[i] = r;
+    amax_merge(best, bi, r, i);
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      double v1 = sv[threadIdx.x];
+      int64_t i1 = si[threadIdx.x];
+      amax_merge(v1, i1, sv[threadIdx.x + off], si[threadIdx.x + off]);
+      sv[threadIdx.x] = v1;
+      si[threadIdx.x] = i1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[(int64_t)v * gridDim.x + blockIdx.x] = sv[0];
+    pi[(int64_t)v * gridDim.x + blockIdx.x] = si[0];
+  }
+}
+
+hipError_t launch_resid_amax(int64_t n, const double* aq, const double* q, const double* w, double* e, int nv,
+                             double* pv, int64_t* pi, double* ov, int64_t* oi, hipStream_t s) {
+  hipLaunchKernelGGL(k_resid_amax_partial, dim3(kRedBlocks, nv), dim3(256), 0, s, n, aq, q, w, e, pv, pi);
+  for (int v = 0; v < nv; v++)
+    hipLaunchKernelGGL(k_amax_final, dim3(1), dim3(256), 0, s, kRedBlocks, pv + (int64_t)v * kRedBlocks,
+                       pi + (int64_t)v * kRedBlocks, ov + v, oi + v);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_mgs_step(int64_t n, double* __restrict__ p,
+                                                  const double* __restrict__ bprev,
+                                                  const double* __restrict__ bnext, const double* part_in,
+                                                  int mode, double* part_out) {
+  __shared__ double red[4];
+  const double c = reduce_parts(part_in, red);
+  const double f = (mode == 0) ? 1.0 / sqrt(c) : c;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double x = p[i];
+    if (mode == 0) x = x * f;
+    else x = x + (-f) * bprev[i];
+    p[i] = x;
+    s += x * (bnext ? bnext[i] : x);
+  }
+  s = block_sum_256(s, red);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = s;
+}
+
+hipError_t launch_mgs_step(int64_t n, double* p, const double* bprev, const double* bnext, const double* part_in,
+                           int mode, double* part_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_mgs_step, dim3(kRedBlocks), dim3(256), 0, s, n, p, bprev, bnext, part_in, mode, part_out);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_scale_by_norm(int64_t n, const double* __restrict__ p, const double* part,
+                                                       double* __restrict__ dst) {
+  __shared__ double red[4];
+  const double c = reduce_parts(part, red);
+  const double f = 1.0 / sqrt(c);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = p[i] * f;
+}
+
+hipError_t launch_scale_by_norm(int64_t n, const double* p, const double* part, double* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_by_norm, dim3(kRedBlocks), dim3(256), 0, s, n, p, part, dst);
+  return hipGetLastError();
+}
+
+__global__ void k_precond_batched(int64_t n, const double* w, const int32_t* idx, const double* __restrict__ d,
+                                  const double* __restrict__ e, double* __restrict__ p) {
+  const int v = idx[blockIdx.y];
+  const double wv = w[v];
+  const double* ev = e + (int64_t)v * n;
+  double* pv = p + (int64_t)blockIdx.y * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double t = wv - d[i];
+    if (fabs(t) < 1.0e-4) t = (t < 0.0) ? -1.0e-4 : 1.0e-4;
+    pv[i] = ev[i] / t;
+  }
+}
+
+hipError_t launch_precond_batched(int64_t n, const double* w, const int32_t* idx, int nv, const double* d,
+                                  const double* e, double* p, hipStream_t s) {
+  if (nv == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_precond_batched, dim3(grid_for(n), nv), dim3(256), 0, s, n, w, idx, d, e, p);
   return hipGetLastError();
 }
 

@@ -56,6 +56,11 @@ struct GemmTask {
 };
 
 enum Prec { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2 };
+// tile configurations: GEMM_V1 is the first-generation 64x64 kernel, the others
+// the pipelined kernel of dpd_gemm.hip with BM x BN tiles
+enum GemmCfg { GEMM_V1 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_64x128 = 3, GEMM_128x128 = 4 };
+inline int gemm_bm(int cfg) { return (cfg == GEMM_128x64 || cfg == GEMM_128x128) ? 128 : 64; }
+inline int gemm_bn(int cfg) { return (cfg == GEMM_64x128 || cfg == GEMM_128x128) ? 128 : 64; }
 
 struct GemmLaunch {
   const void* A;  // storage of operand 1 (double, or float when a_f32)
@@ -73,8 +78,10 @@ struct GemmLaunch {
   const int32_t* split_tasks;  // device list of task ids with ksplit > 1
   int nsplit;
   int max_split_mn;
+  int cfg;  // GemmCfg
 };
 hipError_t launch_gemm(const GemmLaunch& g, hipStream_t s);
+hipError_t launch_gemm2(const GemmLaunch& g, hipStream_t s);
 
 // ---- vector kernels (FP64) ------------------------------------------------
 constexpr int kRedBlocks = 512;
@@ -114,4 +121,30 @@ struct EorbOff {
 hipError_t launch_orbital_energy(const DevLayout& L, const double* eorb, const EorbOff& off, int mode,
                                  double* out, hipStream_t s);
 hipError_t launch_div(int64_t n, const double* num, const double* den, double* out, hipStream_t s);
+}  // namespace dpd
+
+namespace dpd {
+// ---- batched Davidson kernels --------------------------------------------
+// out[v] = dot(x + v*xs, y + v*ys) for v < nv   (partial: kRedBlocks*nv)
+hipError_t launch_pair_dots(int64_t n, const double* x, int64_t xs, const double* y, int64_t ys, int nv,
+                            double* partial, double* out, hipStream_t s);
+// out[xi + yj*nx] = X_xi . Y_yj  (partial: kRedBlocks*nx*ny)
+hipError_t launch_block_dots(int64_t n, const double* X, int64_t xs, int nx, const double* Y, int64_t ys, int ny,
+                             double* partial, double* out, hipStream_t s);
+// partial[0..kRedBlocks) of dot(x, y); out[r] = sum of row r of kRedBlocks partials
+hipError_t launch_pair_dots_partial(int64_t n, const double* x, const double* y, double* partial, hipStream_t s);
+hipError_t launch_reduce_partials(const double* partial, int nrows, double* out, hipStream_t s);
+// e_v = aq_v - w_v q_v and max|e_v| (first index) for v < nv; w on device
+hipError_t launch_resid_amax(int64_t n, const double* aq, const double* q, const double* w, double* e, int nv,
+                             double* pv, int64_t* pi, double* ov, int64_t* oi, hipStream_t s);
+// one fused modified Gram-Schmidt step on p:
+//   c = sum(part_in[0..nb)), mode 0: p *= 1/sqrt(c); mode 1: p -= c*b_prev;
+//   then part_out[block] = dot(p, b_next) (b_next == nullptr: dot(p, p))
+hipError_t launch_mgs_step(int64_t n, double* p, const double* b_prev, const double* b_next, const double* part_in,
+                           int mode, double* part_out, hipStream_t s);
+// dst = p * (1/sqrt(sum(part))) ; returns nothing (norm read by the host separately)
+hipError_t launch_scale_by_norm(int64_t n, const double* p, const double* part, double* dst, hipStream_t s);
+// p = e / clamp(w - d) for the roots listed in idx (w on device)
+hipError_t launch_precond_batched(int64_t n, const double* w, const int32_t* idx, int nv, const double* d,
+                                  const double* e, double* p, hipStream_t s);
 }  // namespace dpd

@@ -73,6 +73,11 @@ class Context:
         except Exception:
             pass
 
+    def set_gemm_config(self, cfg=-1):
+        """GEMM tiles of later plans: -1 automatic, 0 first-generation kernel,
+        1..4 pipelined kernel with 64x64, 128x64, 64x128, 128x128 tiles."""
+        check(lib().dpd_ctx_set_gemm_config(self.h, int(cfg)))
+
     def profile(self, on=True, max_records=100000):
         """Enable/disable per-launch HIP-event timing of the tagged GEMM stages."""
         check(lib().dpd_profile_enable(self.h, int(on), int(max_records)))

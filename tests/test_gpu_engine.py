@@ -40,6 +40,21 @@ def ctx_for(n):
     return _ctxs[n]
 
 
+# GEMM tile configurations: automatic, first-generation kernel, pipelined
+# kernel with 64x64 / 128x64 / 64x128 / 128x128 tiles
+GEMM_CFGS = [-1, 0, 1, 2, 3, 4]
+
+
+@pytest.fixture(params=GEMM_CFGS, ids=["auto", "v1", "t64x64", "t128x64", "t64x128", "t128x128"])
+def gemm_cfg(request, torch):
+    cfg = request.param
+    for n in (4, 8):
+        ctx_for(n).set_gemm_config(cfg)
+    yield cfg
+    for n in (4, 8):
+        ctx_for(n).set_gemm_config(-1)
+
+
 @pytest.fixture(params=[4, 8], ids=["c2v", "d2h"])
 def sym(request, torch):
     s = O.Sym(request.param)
@@ -86,8 +101,9 @@ def test_mfma_lane_layout_single_tile(torch, sym):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("shape", [(16, 16, 4, "N", "N"), (70, 45, 37, "N", "N"), (65, 130, 100, "T", "N"),
-                                   (33, 64, 17, "N", "T"), (128, 20, 300, "T", "T")])
-def test_xgemm_layout_and_transposes(torch, prec, shape):
+                                   (33, 64, 17, "N", "T"), (128, 20, 300, "T", "T"),
+                                   (257, 190, 129, "T", "N")])
+def test_xgemm_layout_and_transposes(torch, prec, shape, gemm_cfg):
     """dense GEMM through the same MFMA kernel: pins the FP64 and FP32 MFMA operand
     and result layouts directly (one 16x16x4 tile first, then ragged tiles)."""
     import dpdhip
@@ -127,7 +143,7 @@ def _engine_case(torch, sym, kind, pops, iops, syms, t1, t2, t3, f1=1.0, f2=0.0,
     np.testing.assert_allclose(g3.numpy(), ref, rtol=RTOL, atol=ATOL)
 
 
-def test_engine_fast_paths(torch, sym):
+def test_engine_fast_paths(torch, sym, gemm_cfg):
     for large in (False, True):
         if large and sym.nirrep == 8:
             continue
@@ -142,7 +158,7 @@ def test_engine_fast_paths(torch, sym):
                      "mnab", "cdmn", "abcd", f2=0.5)
 
 
-def test_engine_fallback_and_scrambles(torch, sym):
+def test_engine_fallback_and_scrambles(torch, sym, gemm_cfg):
     V, Oc = spaces(sym)
     full = [(1, 1)] * 3
     _engine_case(torch, sym, 444, [[V, V, Oc, Oc]] * 3, full, (1, 2, 2), "aeim", "ebmj", "abij")
@@ -190,7 +206,7 @@ def _surface_case(torch, sym, kind, ops, t, fact, anti, seed=1):
 
 
 @pytest.mark.parametrize("large", [False, True], ids=["small", "large"])
-def test_contract_surface_sigma_calls(torch, sym, large):
+def test_contract_surface_sigma_calls(torch, sym, large, gemm_cfg):
     if large and sym.nirrep == 8:
         pytest.skip("large spaces are C2v only")
     V, Oc = spaces(sym, large)
