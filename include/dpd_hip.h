@@ -65,11 +65,13 @@ int dpd_ctx_destroy(dpd_ctx *ctx);
 /* hipStream_t as void*; NULL = the default stream */
 int dpd_ctx_set_stream(dpd_ctx *ctx, void *stream);
 int dpd_ctx_synchronize(dpd_ctx *ctx);
-/* GEMM tile configuration of every later plan: -1 automatic (default), 0 the
- * first-generation 64x64 kernel, 1..4 the pipelined kernel with 64x64, 128x64,
- * 64x128, 128x128 tiles.  Clears the context's plan caches.  No counterpart in
- * the reference (its GEMMs are cuBLAS/BLAS calls); used for tuning and tests. */
-int dpd_ctx_set_gemm_config(dpd_ctx *ctx, int cfg);
+/* GEMM kernels of every later plan.  cfg: -1 automatic (default), 0 the
+ * first-generation 64x64 kernel, 1..4 64x64, 128x64, 64x128, 128x128 tiles.
+ * packing: 0 operands gathered inside the GEMM, 1 (default) packed panels when
+ * an operand is a problem's constant tensor (packed once and kept), 2 packed
+ * panels always (BF16x3 always uses panels).  Clears the plan caches.  No
+ * counterpart in the reference (its GEMMs are cuBLAS/BLAS calls). */
+int dpd_ctx_set_gemm_config(dpd_ctx *ctx, int cfg, int packing);
 const char *dpd_last_error(void);
 /* bytes of device scratch the context currently holds (grows on demand) */
 size_t dpd_ctx_workspace_bytes(const dpd_ctx *ctx);
@@ -160,8 +162,9 @@ int dpd_sstgenso(dpd_ctx *ctx, const double *win, double *wout, const int64_t *p
 
 /* ---- dense GEMM: libsrint/xgemmso.f:1-96 (XGEMMD / XGEMMDG / XGEMMSG) ---- */
 /* column-major C = alpha*op(A)*op(B) + beta*C on device FP64 arrays; prec 0 runs
- * FP64 MFMA, prec 1 FP32 MFMA (operands rounded to FP32, FP32 accumulation);
- * synchronises the stream before returning. */
+ * FP64 MFMA, prec 1 FP32 MFMA (operands rounded to FP32, FP32 accumulation),
+ * prec 2 BF16x3 (operands split into bf16 hi + lo, three BF16 MFMAs, FP32
+ * accumulation); synchronises the stream before returning. */
 int dpd_xgemm(dpd_ctx *ctx, const char *transa, const char *transb, int64_t m, int64_t n,
               int64_t k, double alpha, const double *A, int64_t lda, const double *B,
               int64_t ldb, double beta, double *C, int64_t ldc, int prec);

@@ -369,7 +369,8 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
   if (ldc < std::max<int64_t>(1, m) || lda < std::max<int64_t>(1, ta == 'N' ? m : k) ||
       ldb < std::max<int64_t>(1, tb == 'N' ? k : n))
     return fail(DPD_EINVAL, "bad leading dimension");
-  if (prec != PREC_F64 && prec != PREC_F32) return fail(DPD_EINVAL, "precision %d is not available", prec);
+  if (prec != PREC_F64 && prec != PREC_F32 && prec != PREC_BF16X3)
+    return fail(DPD_EINVAL, "precision %d is not available", prec);
   if (m == 0 || n == 0) return DPD_OK;
   GemmPlan p;
   GemmTask tk;
@@ -403,18 +404,29 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
   p.nwg = tk.tiles_m * tk.tiles_n;
   int rc = p.upload();
   if (rc) return rc;
-  rc = run_gemm(c, p, A, 0, B, 0, C, alpha, beta, prec);
+  if (prec == PREC_BF16X3 || c->packing == 2) {
+    if ((rc = build_variant(p, prec))) return rc;
+    const GemmVariant& V = p.pk[prec];
+    ws_reset(c);
+    rc = ws_reserve(c, pk_esize(prec) * (V.pa_elems + V.pb_elems) + sizeof(double) * V.partial_elems + 1024);
+    if (rc) return rc;
+    rc = run_gemm_packed(c, p, prec, A, 0, B, 0, C, alpha, beta);
+  } else {
+    rc = run_gemm(c, p, A, 0, B, 0, C, alpha, beta, prec);
+  }
   if (rc) return rc;
   // the plan's device arrays are freed on return: wait for the launch
   DPD_HIP_TRY(hipStreamSynchronize(c->stream));
   return DPD_OK;
 }
 
-int dpd_ctx_set_gemm_config(dpd_ctx* c, int cfg) {
+int dpd_ctx_set_gemm_config(dpd_ctx* c, int cfg, int packing) {
   CHECK_CTX(c);
   if (cfg < -1 || cfg > GEMM_128x128) return fail(DPD_EINVAL, "GEMM configuration must be -1..4");
+  if (packing < 0 || packing > 2) return fail(DPD_EINVAL, "packing must be 0, 1 or 2");
   if (c->device >= 0) DPD_HIP_TRY(hipStreamSynchronize(c->stream));
   c->gemm_cfg = cfg;
+  c->packing = packing;
   c->contract_cache.clear();
   c->engine_cache.clear();
   return DPD_OK;
@@ -453,7 +465,7 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
   for (auto& r : c->prof.recs) {
     float t = 0.f;
     DPD_HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
-    std::string key = r.tag + (r.prec == PREC_F64 ? ":fp64" : ":fp32");
+    std::string key = r.tag + (r.prec == PREC_F64 ? ":fp64" : r.prec == PREC_F32 ? ":fp32" : ":bf16x3");
     size_t k = std::find(tags.begin(), tags.end(), key) - tags.begin();
     if (k == tags.size()) {
       tags.push_back(key);

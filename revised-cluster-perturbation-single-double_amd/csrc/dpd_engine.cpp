@@ -62,6 +62,10 @@ int ws_reserve(dpd_ctx* c, size_t bytes) {
 
 void* ws_take(dpd_ctx* c, size_t bytes) {
   bytes = (bytes + 255) & ~size_t(255);
+  if (c->ws_top + bytes > c->ws.bytes) {
+    fail(DPD_ENOMEM, "workspace overrun (internal): %zu + %zu > %zu", c->ws_top, bytes, c->ws.bytes);
+    return nullptr;
+  }
   void* p = (char*)c->ws.p + c->ws_top;
   c->ws_top += bytes;
   return p;
@@ -562,6 +566,145 @@ int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested) {
   return best;
 }
 
+int build_variant(GemmPlan& p, int mode) {
+  GemmVariant& V = p.pk[mode];
+  if (V.built) return DPD_OK;
+  V.mode = mode;
+  V.tasks = p.tasks;
+  V.split.clear();
+  V.cfg = choose_gemm_cfg(p.tasks, p.cfg_req >= GEMM_64x64 ? p.cfg_req : -1);
+  if (V.cfg == GEMM_V1) V.cfg = GEMM_64x64;
+  const int BM = gemm_bm(V.cfg), BN = gemm_bn(V.cfg), KS = pk_kstep(mode), PL = pk_planes(mode);
+  for (auto& tk : V.tasks) {
+    tk.steps = (int32_t)(((int64_t)tk.ktiles_total * 16 + KS - 1) / KS);
+    tk.tiles_m = (tk.M + BM - 1) / BM;
+    tk.tiles_n = (tk.N + BN - 1) / BN;
+  }
+  // split-K over K steps so that the launch fills the chip
+  int64_t work = 0, tiles = 0;
+  for (auto& tk : V.tasks) {
+    work += (int64_t)tk.tiles_m * tk.tiles_n * std::max(tk.steps, 1);
+    tiles += (int64_t)tk.tiles_m * tk.tiles_n;
+  }
+  const int64_t target = 1024;
+  const int64_t min_steps = mode == PREC_F64 ? 8 : 4;
+  int64_t per_wg = std::max<int64_t>(min_steps, (work + target - 1) / target);
+  int64_t partial = 0, pa = 0, pb = 0;
+  int max_mn = 0, wg = 0, pan_a = 0, pan_b = 0;
+  for (size_t i = 0; i < V.tasks.size(); i++) {
+    GemmTask& tk = V.tasks[i];
+    int ks = 1;
+    if (tiles < target && tk.steps > per_wg) ks = (int)std::min<int64_t>(64, (tk.steps + per_wg - 1) / per_wg);
+    tk.ksplit = ks;
+    tk.part_off = 0;
+    if (ks > 1) {
+      tk.part_off = partial;
+      partial += (int64_t)ks * tk.M * tk.N;
+      V.split.push_back((int32_t)i);
+      max_mn = std::max<int64_t>(max_mn, (int64_t)tk.M * tk.N);
+    }
+    tk.wg_begin = wg;
+    wg += tk.tiles_m * tk.tiles_n * ks;
+    tk.pan_a = pan_a;
+    tk.pan_b = pan_b;
+    tk.pa_off = pa;
+    tk.pb_off = pb;
+    pan_a += tk.tiles_m * tk.steps;
+    pan_b += tk.tiles_n * tk.steps;
+    pa += (int64_t)tk.tiles_m * tk.steps * BM * KS * PL;
+    pb += (int64_t)tk.tiles_n * tk.steps * BN * KS * PL;
+    if (wg < 0 || pan_a < 0 || pan_b < 0) return fail(DPD_EINVAL, "GEMM plan too large");
+  }
+  V.nwg = wg;
+  V.npan_a = pan_a;
+  V.npan_b = pan_b;
+  V.pa_elems = pa;
+  V.pb_elems = pb;
+  V.partial_elems = partial;
+  V.max_split_mn = max_mn;
+  size_t bt = V.tasks.size() * sizeof(GemmTask), bp = V.split.size() * sizeof(int32_t);
+  int rc = V.dev.alloc(((bt + 255) & ~size_t(255)) + bp + 256);
+  if (rc) return rc;
+  V.d_tasks = (GemmTask*)V.dev.p;
+  V.d_split = (int32_t*)((char*)V.dev.p + ((bt + 255) & ~size_t(255)));
+  if (bt) DPD_HIP_TRY(hipMemcpy(V.d_tasks, V.tasks.data(), bt, hipMemcpyHostToDevice));
+  if (bp) DPD_HIP_TRY(hipMemcpy(V.d_split, V.split.data(), bp, hipMemcpyHostToDevice));
+  V.built = true;
+  return DPD_OK;
+}
+
+// pack one operand side of a variant into dst
+static int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const double* src, void* dst) {
+  PackArgs a;
+  a.src = src;
+  a.dst = dst;
+  a.tasks = V.d_tasks;
+  a.segs = p.d_segs;
+  a.ntask = (int)V.tasks.size();
+  a.npanel = side == 0 ? V.npan_a : V.npan_b;
+  a.side = side;
+  a.mode = V.mode;
+  a.tile = side == 0 ? gemm_bm(V.cfg) : gemm_bn(V.cfg);
+  DPD_HIP_TRY(launch_pack(a, c->stream));
+  return DPD_OK;
+}
+
+// packed-panel GEMM: static operands come from the variant's cache (packed on
+// first use), the others are packed into the workspace reserved by the caller
+int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const double* A, uint64_t a_id, const double* B,
+                           uint64_t b_id, double* C, double f1, double f2) {
+  GemmVariant& V = p.pk[mode];
+  const size_t es = pk_esize(mode);
+  const void* side_ptr[2];
+  const double* src[2] = {A, B};
+  const uint64_t ids[2] = {a_id, b_id};
+  const int64_t elems[2] = {V.pa_elems, V.pb_elems};
+  for (int sd = 0; sd < 2; sd++) {
+    if (elems[sd] == 0) {
+      side_ptr[sd] = nullptr;
+      continue;
+    }
+    if (ids[sd]) {
+      PackCache& pc = V.cache[sd];
+      if (pc.src != src[sd] || pc.id != ids[sd] || !pc.buf.p) {
+        if (!pc.buf.p) {
+          int rc = pc.buf.alloc(es * elems[sd]);
+          if (rc) return rc;
+        }
+        int rc = pack_side(c, p, V, sd, src[sd], pc.buf.p);
+        if (rc) return rc;
+        pc.src = src[sd];
+        pc.id = ids[sd];
+      }
+      side_ptr[sd] = pc.buf.p;
+    } else {
+      void* w = ws_take(c, es * elems[sd]);
+      if (!w) return DPD_ENOMEM;
+      int rc = pack_side(c, p, V, sd, src[sd], w);
+      if (rc) return rc;
+      side_ptr[sd] = w;
+    }
+  }
+  if (V.nwg == 0) return DPD_OK;
+  GemmLaunch g;
+  std::memset(&g, 0, sizeof g);
+  g.C = C;
+  g.tasks = V.d_tasks;
+  g.segs = p.d_segs;
+  g.ntask = (int)V.tasks.size();
+  g.nwg = V.nwg;
+  g.alpha = f1;
+  g.beta = f2;
+  g.prec = mode;
+  g.split_tasks = V.d_split;
+  g.nsplit = (int)V.split.size();
+  g.max_split_mn = V.max_split_mn;
+  g.cfg = V.cfg;
+  if (V.partial_elems > 0 && !(g.partial = (double*)ws_take(c, sizeof(double) * V.partial_elems))) return DPD_ENOMEM;
+  DPD_HIP_TRY(launch_gemm_pk(g, side_ptr[0], side_ptr[1], c->stream));
+  return DPD_OK;
+}
+
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
              double f1, double f2, int prec) {
   if (p.nwg == 0) return DPD_OK;
@@ -583,7 +726,7 @@ int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, i
   g.nsplit = (int)p.split.size();
   g.max_split_mn = p.max_split_mn;
   g.cfg = p.cfg;
-  if (p.partial_elems > 0) g.partial = (double*)ws_take(c, sizeof(double) * p.partial_elems);
+  if (p.partial_elems > 0 && !(g.partial = (double*)ws_take(c, sizeof(double) * p.partial_elems))) return DPD_ENOMEM;
   DPD_HIP_TRY(launch_gemm(g, c->stream));
   return DPD_OK;
 }
@@ -739,6 +882,13 @@ int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, cons
     p = it->second.get();
   }
   ws_reset(c);
+  if (c->packing == 2) {
+    if ((rc = build_variant(*p, PREC_F64))) return rc;
+    const GemmVariant& V = p->pk[PREC_F64];
+    rc = ws_reserve(c, pk_esize(PREC_F64) * (V.pa_elems + V.pb_elems) + sizeof(double) * V.partial_elems + 1024);
+    if (rc) return rc;
+    return run_gemm_packed(c, *p, PREC_F64, o1.data, 0, o2.data, 0, const_cast<double*>(o3.data), f1, f2);
+  }
   rc = ws_reserve(c, sizeof(double) * (p->partial_elems + 32));
   if (rc) return rc;
   return run_gemm(c, *p, o1.data, 0, o2.data, 0, const_cast<double*>(o3.data), f1, f2, PREC_F64);
@@ -895,11 +1045,24 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   }
   if (flops) *flops += P->gemm.flops;
   const int nb1 = o1.batched ? nbatch : 1, nb2 = o2.batched ? nbatch : 1;
+  // packed panels: always for BF16x3, otherwise when an operand is static
+  const bool packed = prec == PREC_BF16X3 || c->packing == 2 ||
+                      (c->packing == 1 && ((o1.static_id && !P->exp1) || (o2.static_id && !P->exp2)));
+  if (prec == PREC_BF16X3 && c->packing == 0) return fail(DPD_EINVAL, "BF16x3 needs packed panels (packing 0)");
+  const uint64_t id1 = P->exp1 ? 0 : o1.static_id, id2 = P->exp2 ? 0 : o2.static_id;
   size_t need = 0;
+  if (packed) {
+    int rc0 = build_variant(P->gemm, prec);
+    if (rc0) return rc0;
+    const GemmVariant& V = P->gemm.pk[prec];
+    if (!id1) need += pk_esize(prec) * V.pa_elems + 256;
+    if (!id2) need += pk_esize(prec) * V.pb_elems + 256;
+    need += sizeof(double) * V.partial_elems + 256;
+  }
   if (P->exp1) need += sizeof(double) * P->len1x * nb1 + 256;
   if (P->exp2) need += sizeof(double) * P->len2x * nb2 + 256;
   if (anti) need += sizeof(double) * P->len3x * nbatch + 256;
-  need += sizeof(double) * P->gemm.partial_elems + 256;
+  if (!packed) need += sizeof(double) * P->gemm.partial_elems + 256;
   ws_reset(c);
   int rc = ws_reserve(c, need);
   if (rc) return rc;
@@ -909,6 +1072,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   const float* p2f = o2.data_f32;
   if (P->exp1) {
     double* x = (double*)ws_take(c, sizeof(double) * P->len1x * nb1);
+    if (!x) return DPD_ENOMEM;
     MapArgs m = P->map1;
     m.nbatch = nb1;
     m.in_bstride = o1.bstride;
@@ -920,6 +1084,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   }
   if (P->exp2) {
     double* x = (double*)ws_take(c, sizeof(double) * P->len2x * nb2);
+    if (!x) return DPD_ENOMEM;
     MapArgs m = P->map2;
     m.nbatch = nb2;
     m.in_bstride = o2.bstride;
@@ -932,6 +1097,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   double* p3 = const_cast<double*>(o3.data);
   if (anti) {
     p3 = (double*)ws_take(c, sizeof(double) * P->len3x * nbatch);
+    if (!p3) return DPD_ENOMEM;
     DPD_HIP_TRY(launch_fill(P->len3x * nbatch, 0.0, p3, c->stream));
   }
   int use_f32 = (prec != PREC_F64);
@@ -940,13 +1106,16 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   ProfRec* pr = nullptr;
   if (c->prof.on && tag && c->prof.used + 2 <= c->prof.pool.size()) {
     c->prof.recs.push_back({tag, c->prof.pool[c->prof.used], c->prof.pool[c->prof.used + 1], P->gemm.flops,
-                            use_f32 ? PREC_F32 : PREC_F64});
+                            prec});
     c->prof.used += 2;
     pr = &c->prof.recs.back();
     DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
   }
-  rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, 1.0,
-                use_f32 ? PREC_F32 : PREC_F64);
+  if (packed)
+    rc = run_gemm_packed(c, P->gemm, prec, p1, id1, p2, id2, p3, fact, 1.0);
+  else
+    rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, 1.0,
+                  use_f32 ? PREC_F32 : PREC_F64);
   if (rc) return rc;
   if (pr) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
   if (anti) {

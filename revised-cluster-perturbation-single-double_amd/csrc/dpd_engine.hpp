@@ -37,6 +37,27 @@ struct DevBuf {
 };
 
 // One GEMM stage of a contraction, independent of the data pointers.
+// packed copy of one operand side, kept for operands that do not change
+// between calls (identified by data pointer and a caller-given id)
+struct PackCache {
+  const void* src = nullptr;
+  uint64_t id = 0;
+  DevBuf buf;
+};
+// the tiling of a plan for the packed-panel kernel in one precision
+struct GemmVariant {
+  bool built = false;
+  int mode = 0, cfg = 0;
+  std::vector<GemmTask> tasks;
+  std::vector<int32_t> split;
+  int nwg = 0, max_split_mn = 0, npan_a = 0, npan_b = 0;
+  int64_t partial_elems = 0, pa_elems = 0, pb_elems = 0;  // panel elements (BF16x3: both planes)
+  DevBuf dev;
+  GemmTask* d_tasks = nullptr;
+  int32_t* d_split = nullptr;
+  PackCache cache[2];
+};
+
 struct GemmPlan {
   std::vector<GemmTask> tasks;
   std::vector<GemmSeg> segs;
@@ -51,6 +72,7 @@ struct GemmPlan {
   GemmTask* d_tasks = nullptr;
   GemmSeg* d_segs = nullptr;
   int32_t* d_split = nullptr;
+  GemmVariant pk[3];  // packed-panel tilings per precision (built on first use)
   int upload();
 };
 
@@ -62,6 +84,7 @@ struct Operand {
   char lab[4];
   const double* data = nullptr;
   const float* data_f32 = nullptr;  // optional FP32 copy (mixed precision)
+  uint64_t static_id = 0;  // nonzero: contents never change under this id (packed once)
   int64_t bstride = 0;
   bool batched = false;
 };
@@ -113,6 +136,7 @@ struct dpd_ctx {
   hipStream_t stream = nullptr;
   int64_t max_threads_block = 256;
   int gemm_cfg = -1;  // GEMM tile configuration override (-1: automatic)
+  int packing = 1;    // 0 gather kernels only, 1 packed panels when an operand is static, 2 always
   // scratch: grow-only device workspace (bump allocated per top-level call)
   dpd::DevBuf ws;
   size_t ws_top = 0;
@@ -140,6 +164,11 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
              double f1, double f2, int prec);
 int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested);
+int build_variant(GemmPlan& p, int mode);
+// packed-panel GEMM; operands with a nonzero id are packed once and cached,
+// the others are packed into workspace the caller reserved
+int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const double* A, uint64_t a_id, const double* B,
+                    uint64_t b_id, double* C, double f1, double f2);
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out);
 // labels / checks shared by the C-ABI and the sigma builders
 int check_labels(const char* t, int n);

@@ -1,0 +1,382 @@
+// dpd_gemm_pk.hip -- grouped DPD GEMM on pre-packed operand panels (gfx950).
+//
+// The DPD operands of a contraction are irrep blocks addressed through
+// mixed-radix digits; gathering them inside the GEMM costs address arithmetic
+// and narrow loads on every K tile.  Here a pack kernel first copies each
+// operand into panels laid out exactly as the GEMM stages them in LDS:
+//
+//   A panel (task, m tile, K step):  BM x KS elements,  B panel: BN x KS
+//   FP64 (KS = 16) and FP32 (KS = 32): row-major [k][m]  (m contiguous)
+//   BF16x3 (KS = 32): hi plane [m][k] then lo plane [m][k] (k contiguous),
+//                     x = hi + lo with hi = bf16(x), lo = bf16(x - hi)
+//
+// Panels pad M/N to the tile and every K segment to 16 with zeros, so the GEMM
+// main loop is 16-byte loads of contiguous memory, LDS double buffering with a
+// register prefetch and one barrier per K step.  Operands that do not change
+// between calls (integrals, intermediates) are packed once and kept.
+//
+// MFMA: FP64 v_mfma_f64_16x16x4f64, FP32 v_mfma_f32_16x16x4f32, BF16x3
+// v_mfma_f32_16x16x32_bf16 three times (hi*hi + hi*lo + lo*hi, FP32
+// accumulation).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dpd_kernels.hpp"
+
+namespace dpd {
+
+namespace {
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u4_t __attribute__((ext_vector_type(4)));  // one 16-byte chunk
+
+__device__ inline int64_t digit_off(int v, const int32_t* d, const int64_t* s) {
+  const int d0 = v % d[0];
+  const int t = v / d[0];
+  const int d1 = t % d[1];
+  const int d2 = t / d[1];
+  return (int64_t)d0 * s[0] + (int64_t)d1 * s[1] + (int64_t)d2 * s[2];
+}
+
+// round-to-nearest-even FP32 -> BF16 (finite inputs)
+__device__ inline uint16_t bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+constexpr int kThreads = 256;
+
+__device__ inline int find_task(const GemmTask* tasks, int ntask, int key, int which) {
+  int lo = 0, hi = ntask - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    int b = which == 0 ? tasks[mid].wg_begin : which == 1 ? tasks[mid].pan_a : tasks[mid].pan_b;
+    if (b <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// pack: one block per panel
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_pack(PackArgs p) {
+  constexpr int KS = MODE == PREC_F64 ? 16 : 32;
+  const int pid = blockIdx.x;
+  const int ti = find_task(p.tasks, p.ntask, pid, 1 + p.side);
+  const GemmTask& T = p.tasks[ti];
+  const int local = pid - (p.side == 0 ? T.pan_a : T.pan_b);
+  const int tile = local / T.steps, step = local % T.steps;
+  const int R = p.side == 0 ? T.M : T.N;
+  const int r0 = tile * p.tile;
+  const int64_t pan_elems = (int64_t)p.tile * KS;
+  const int64_t base = (p.side == 0 ? T.pa_off : T.pb_off) + (int64_t)local * pan_elems * (MODE == PREC_BF16X3 ? 2 : 1);
+  int sg = T.seg_begin;
+  for (int e = threadIdx.x; e < pan_elems; e += kThreads) {
+    int r, k;
+    if (MODE == PREC_BF16X3) {
+      r = e / KS;
+      k = e % KS;
+    } else {
+      k = e / p.tile;
+      r = e % p.tile;
+    }
+    const int kg = step * KS + k;
+    const int gran = kg >> 4;
+    double v = 0.0;
+    if (r0 + r < R && gran < T.ktiles_total) {
+      // segments are visited in increasing order by each thread's elements
+      // only within one panel row; restart the scan when the granule moved back
+      if (gran < p.segs[sg].ktile_begin) sg = T.seg_begin;
+      while (gran >= p.segs[sg].ktile_begin + p.segs[sg].ktiles) sg++;
+      const GemmSeg& S = p.segs[sg];
+      const int ks = (gran - S.ktile_begin) * 16 + (kg & 15);
+      if (ks < S.K) {
+        int64_t ad;
+        if (p.side == 0) ad = S.a_off + digit_off(r0 + r, T.md, S.sa_m) + digit_off(ks, S.kd, S.sa_k);
+        else ad = S.b_off + digit_off(ks, S.kd, S.sb_k) + digit_off(r0 + r, T.nd, S.sb_n);
+        v = p.src[ad];
+      }
+    }
+    if (MODE == PREC_F64) {
+      ((double*)p.dst)[base + e] = v;
+    } else if (MODE == PREC_F32) {
+      ((float*)p.dst)[base + e] = (float)v;
+    } else {
+      const float f = (float)v;
+      const uint16_t h = bf16_bits(f);
+      const uint16_t l = bf16_bits((float)(v - (double)bf16_float(h)));
+      ((uint16_t*)p.dst)[base + e] = h;
+      ((uint16_t*)p.dst)[base + pan_elems + e] = l;
+    }
+  }
+}
+
+hipError_t launch_pack(const PackArgs& p, hipStream_t s) {
+  if (p.npanel == 0) return hipSuccess;
+  switch (p.mode) {
+    case PREC_F64:
+      hipLaunchKernelGGL(k_pack<PREC_F64>, dim3(p.npanel), dim3(256), 0, s, p);
+      break;
+    case PREC_F32:
+      hipLaunchKernelGGL(k_pack<PREC_F32>, dim3(p.npanel), dim3(256), 0, s, p);
+      break;
+    case PREC_BF16X3:
+      hipLaunchKernelGGL(k_pack<PREC_BF16X3>, dim3(p.npanel), dim3(256), 0, s, p);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// packed GEMM
+// ---------------------------------------------------------------------------
+template <int MODE>
+struct PkTraits;
+template <>
+struct PkTraits<PREC_F64> {
+  typedef double E;  // element type in LDS
+  typedef double CT;
+  typedef d4_t AT;
+  static constexpr int KS = 16;
+  static constexpr int PAD = 16;  // elements; rows land 32 banks apart
+  static constexpr int PLANES = 1;
+};
+template <>
+struct PkTraits<PREC_F32> {
+  typedef float E;
+  typedef float CT;
+  typedef f4_t AT;
+  static constexpr int KS = 32;
+  static constexpr int PAD = 16;
+  static constexpr int PLANES = 1;
+};
+template <>
+struct PkTraits<PREC_BF16X3> {
+  typedef uint16_t E;
+  typedef float CT;
+  typedef f4_t AT;
+  static constexpr int KS = 32;
+  static constexpr int PAD = 8;  // row = 40 bf16 = 80 bytes: 16 rows of a b128 read hit distinct banks
+  static constexpr int PLANES = 2;
+};
+
+template <int MODE, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_pk(GemmLaunch g, const void* __restrict__ PA,
+                                                 const void* __restrict__ PB) {
+  typedef PkTraits<MODE> Tr;
+  typedef typename Tr::E E;
+  typedef typename Tr::AT AT;
+  constexpr int KS = Tr::KS;
+  constexpr bool BF = MODE == PREC_BF16X3;
+  // LDS tiles: FP64/FP32 [k][m] rows of BM(+PAD); BF16 [plane][m][k] rows of KS(+PAD)
+  constexpr int A_ELEMS = BF ? 2 * BM * (KS + Tr::PAD) : KS * (BM + Tr::PAD);
+  constexpr int B_ELEMS = BF ? 2 * BN * (KS + Tr::PAD) : KS * (BN + Tr::PAD);
+  constexpr int A_PANEL = BM * KS * Tr::PLANES;  // elements per packed panel
+  constexpr int B_PANEL = BN * KS * Tr::PLANES;
+  constexpr int EPC = 16 / sizeof(E);  // elements per 16-byte chunk
+  constexpr int CA = A_PANEL / EPC / kThreads, CB = B_PANEL / EPC / kThreads;
+  static_assert(CA * EPC * kThreads == A_PANEL && CB * EPC * kThreads == B_PANEL, "panel/threads mismatch");
+  constexpr int TM = BM / 32, TN = BN / 32;
+  __shared__ __attribute__((aligned(16))) E As[2][A_ELEMS];
+  __shared__ __attribute__((aligned(16))) E Bs[2][B_ELEMS];
+  __shared__ int64_t cm[BM], cn[BN];
+
+  const int tid = threadIdx.x;
+  int wg;
+  {
+    const int x = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int q = g.nwg >> 3, r = g.nwg & 7;
+    wg = x * q + (x < r ? x : r) + slot;
+  }
+  const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
+  int local = wg - T.wg_begin;
+  const int tn = local % T.tiles_n;
+  local /= T.tiles_n;
+  const int tm = local % T.tiles_m;
+  const int kp = local / T.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int s0 = (int)((int64_t)kp * T.steps / T.ksplit);
+  const int s1 = (int)((int64_t)(kp + 1) * T.steps / T.ksplit);
+  const u4_t* pa = (const u4_t*)((const E*)PA + T.pa_off + ((int64_t)tm * T.steps + s0) * A_PANEL);
+  const u4_t* pb = (const u4_t*)((const E*)PB + T.pb_off + ((int64_t)tn * T.steps + s0) * B_PANEL);
+
+  AT acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) acc[i][j] = AT{0, 0, 0, 0};
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
+
+  // LDS position of element e of a panel with `rows` rows (FP64/FP32 [k][r],
+  // BF16x3 [plane][r][k])
+#define PK_LDS_POS(e, rows)                                                                                  \
+  (BF ? ((e) / ((rows) * KS)) * (rows) * (KS + Tr::PAD) + (((e) % ((rows) * KS)) / KS) * (KS + Tr::PAD) +    \
+            ((e) % KS)                                                                                       \
+      : ((e) / (rows)) * ((rows) + Tr::PAD) + ((e) % (rows)))
+  u4_t ra[CA], rb[CB];
+  const int nst = s1 - s0;
+  if (nst > 0) {
+#pragma unroll
+    for (int j = 0; j < CA; j++) ra[j] = pa[tid + j * kThreads];
+#pragma unroll
+    for (int j = 0; j < CB; j++) rb[j] = pb[tid + j * kThreads];
+#pragma unroll
+    for (int j = 0; j < CA; j++) *(u4_t*)&As[0][PK_LDS_POS((tid + j * kThreads) * EPC, BM)] = ra[j];
+#pragma unroll
+    for (int j = 0; j < CB; j++) *(u4_t*)&Bs[0][PK_LDS_POS((tid + j * kThreads) * EPC, BN)] = rb[j];
+    __syncthreads();
+    for (int i = 0; i < nst; i++) {
+      const int buf = i & 1;
+      const bool more = i + 1 < nst;
+      if (more) {
+        const u4_t* qa = pa + (int64_t)(i + 1) * (A_PANEL / EPC);
+        const u4_t* qb = pb + (int64_t)(i + 1) * (B_PANEL / EPC);
+#pragma unroll
+        for (int j = 0; j < CA; j++) ra[j] = qa[tid + j * kThreads];
+#pragma unroll
+        for (int j = 0; j < CB; j++) rb[j] = qb[tid + j * kThreads];
+      }
+      if constexpr (!BF) {
+        typedef typename Tr::CT CT;
+        const CT* as = (const CT*)As[buf];
+        const CT* bs = (const CT*)Bs[buf];
+#pragma unroll
+        for (int ks = 0; ks < KS / 4; ks++) {
+          const int kr = ks * 4 + (lane >> 4);
+          CT a[TM], b[TN];
+#pragma unroll
+          for (int x = 0; x < TM; x++) a[x] = as[kr * (BM + Tr::PAD) + wm + 16 * x + (lane & 15)];
+#pragma unroll
+          for (int y = 0; y < TN; y++) b[y] = bs[kr * (BN + Tr::PAD) + wn + 16 * y + (lane & 15)];
+#pragma unroll
+          for (int x = 0; x < TM; x++)
+#pragma unroll
+            for (int y = 0; y < TN; y++) {
+              if constexpr (MODE == PREC_F64)
+                acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+              else
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[x], b[y], acc[x][y], 0, 0, 0);
+            }
+        }
+      } else {
+        const int kq = 8 * (lane >> 4);
+        bf8_t ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+        for (int x = 0; x < TM; x++) {
+          const int row = wm + 16 * x + (lane & 15);
+          ah[x] = *(const bf8_t*)&As[buf][row * (KS + Tr::PAD) + kq];
+          al[x] = *(const bf8_t*)&As[buf][BM * (KS + Tr::PAD) + row * (KS + Tr::PAD) + kq];
+        }
+#pragma unroll
+        for (int y = 0; y < TN; y++) {
+          const int row = wn + 16 * y + (lane & 15);
+          bh[y] = *(const bf8_t*)&Bs[buf][row * (KS + Tr::PAD) + kq];
+          bl[y] = *(const bf8_t*)&Bs[buf][BN * (KS + Tr::PAD) + row * (KS + Tr::PAD) + kq];
+        }
+#pragma unroll
+        for (int x = 0; x < TM; x++)
+#pragma unroll
+          for (int y = 0; y < TN; y++) {
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+          }
+      }
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < CA; j++) *(u4_t*)&As[buf ^ 1][PK_LDS_POS((tid + j * kThreads) * EPC, BM)] = ra[j];
+#pragma unroll
+        for (int j = 0; j < CB; j++) *(u4_t*)&Bs[buf ^ 1][PK_LDS_POS((tid + j * kThreads) * EPC, BN)] = rb[j];
+      }
+      __syncthreads();
+    }
+  }
+#undef PK_LDS_POS
+
+  const bool split = T.ksplit > 1;
+  if (!split) {
+    for (int i = tid; i < BM + BN; i += kThreads) {
+      if (i < BM) {
+        const int m = m0 + i;
+        cm[i] = m < T.M ? digit_off(m, T.md, T.sc_m) : -1;
+      } else {
+        const int n = n0 + i - BM;
+        cn[i - BM] = n < T.N ? digit_off(n, T.nd, T.sc_n) : -1;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < TM; x++)
+#pragma unroll
+    for (int y = 0; y < TN; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        // result register r of lane l: row (l>>4)+4r for FP64, 4(l>>4)+r for FP32 accumulators
+        const int row = MODE == PREC_F64 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+        const int ml = wm + 16 * x + row;
+        const int nl = wn + 16 * y + (lane & 15);
+        const int m = m0 + ml, n = n0 + nl;
+        if (m >= T.M || n >= T.N) continue;
+        const double v = g.alpha * (double)acc[x][y][r];
+        if (split) {
+          g.partial[T.part_off + (int64_t)kp * T.M * T.N + m + (int64_t)n * T.M] = v;
+        } else {
+          const int64_t ad = T.c_off + cm[ml] + cn[nl];
+          g.C[ad] = (g.beta == 0.0) ? v : g.beta * g.C[ad] + v;
+        }
+      }
+}
+
+template <int MODE>
+static hipError_t launch_pk_mode(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s) {
+  switch (g.cfg) {
+    case GEMM_64x64:
+      hipLaunchKernelGGL((k_gemm_pk<MODE, 64, 64>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
+      break;
+    case GEMM_128x64:
+      hipLaunchKernelGGL((k_gemm_pk<MODE, 128, 64>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
+      break;
+    case GEMM_64x128:
+      hipLaunchKernelGGL((k_gemm_pk<MODE, 64, 128>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
+      break;
+    case GEMM_128x128:
+      hipLaunchKernelGGL((k_gemm_pk<MODE, 128, 128>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_pk(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s) {
+  if (g.nwg == 0) return hipSuccess;
+  hipError_t e;
+  switch (g.prec) {
+    case PREC_F64:
+      e = launch_pk_mode<PREC_F64>(g, PA, PB, s);
+      break;
+    case PREC_F32:
+      e = launch_pk_mode<PREC_F32>(g, PA, PB, s);
+      break;
+    case PREC_BF16X3:
+      e = launch_pk_mode<PREC_BF16X3>(g, PA, PB, s);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  return launch_splitk_reduce(g, s);
+}
+
+}  // namespace dpd

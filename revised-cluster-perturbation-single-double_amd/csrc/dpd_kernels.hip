@@ -293,14 +293,16 @@ hipError_t launch_gemm(const GemmLaunch& g, hipStream_t s) {
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (g.nsplit > 0) {
-    int bx = (g.max_split_mn + 255) / 256;
-    if (bx > 1024) bx = 1024;
-    if (bx < 1) bx = 1;
-    hipLaunchKernelGGL(k_splitk_reduce, dim3(bx, g.nsplit), dim3(256), 0, s, g);
-    e = hipGetLastError();
-  }
-  return e;
+  return launch_splitk_reduce(g, s);
+}
+
+hipError_t launch_splitk_reduce(const GemmLaunch& g, hipStream_t s) {
+  if (g.nsplit == 0) return hipSuccess;
+  int bx = (g.max_split_mn + 255) / 256;
+  if (bx > 1024) bx = 1024;
+  if (bx < 1) bx = 1;
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(bx, g.nsplit), dim3(256), 0, s, g);
+  return hipGetLastError();
 }
 
 // ===========================================================================

@@ -53,6 +53,11 @@ struct GemmTask {
   int32_t wg_begin;
   int32_t pad_;
   int64_t part_off;  // split-K partial buffer offset (elements) if ksplit > 1
+  // packed-panel kernel: K steps, first panel ids and panel offsets (elements)
+  int32_t steps;
+  int32_t pan_a, pan_b;
+  int32_t pad2_;
+  int64_t pa_off, pb_off;
 };
 
 enum Prec { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2 };
@@ -82,6 +87,27 @@ struct GemmLaunch {
 };
 hipError_t launch_gemm(const GemmLaunch& g, hipStream_t s);
 hipError_t launch_gemm2(const GemmLaunch& g, hipStream_t s);
+hipError_t launch_splitk_reduce(const GemmLaunch& g, hipStream_t s);
+
+// ---- packed-panel GEMM (dpd_gemm_pk.hip) ----------------------------------
+// copies one operand side of every task into GEMM panels of precision `mode`
+struct PackArgs {
+  const double* src;
+  void* dst;
+  const GemmTask* tasks;  // device
+  const GemmSeg* segs;    // device
+  int ntask;
+  int npanel;
+  int side;  // 0: A (rows = M), 1: B (rows = N)
+  int mode;  // Prec
+  int tile;  // BM or BN
+};
+hipError_t launch_pack(const PackArgs& p, hipStream_t s);
+// K step of a packed panel and its element count factor (BF16x3 keeps two planes)
+inline int pk_kstep(int mode) { return mode == PREC_F64 ? 16 : 32; }
+inline int pk_planes(int mode) { return mode == PREC_BF16X3 ? 2 : 1; }
+inline int pk_esize(int mode) { return mode == PREC_F64 ? 8 : mode == PREC_F32 ? 4 : 2; }
+hipError_t launch_gemm_pk(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s);
 
 // ---- vector kernels (FP64) ------------------------------------------------
 constexpr int kRedBlocks = 512;

@@ -18,6 +18,7 @@
 #include "dpd_sigma.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <numeric>
 
@@ -47,6 +48,12 @@ static Operand op(const Desc& d, const char* lab, int rank, const double* data, 
   return o;
 }
 
+// a constant tensor of the problem: packed GEMM panels of it are built once
+static Operand konst(Operand o, const dpd_problem* p, int which) {
+  o.static_id = p->uid * 64 + (uint64_t)which + 1;
+  return o;
+}
+
 static Operand batched(Operand o, int64_t stride) {
   o.batched = true;
   o.bstride = stride;
@@ -73,7 +80,12 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
   const int64_t *v = p->popv, *o = p->popo, *z = p->zorb;
   const int S = p->symR;
   const int nb = (int)nvec;
-  const bool f32 = p->prec != PREC_F64;
+  // FP32 copies feed the gather kernels; the packed-panel kernels convert on packing
+  const bool f32 = p->prec == PREC_F32 && c->packing == 0;
+  if (f32 && !p->iabj32.p) {
+    int rc0 = problem_make_f32(p);
+    if (rc0) return rc0;
+  }
   auto F = [&](const DevBuf& b) -> const float* { return f32 ? (const float*)b.p : nullptr; };
   DPD_HIP_TRY(launch_fill(p->dim * nvec, 0.0, sigma, c->stream));
   Desc dR1 = mkdesc(v, o, z, z, 1, 1, S);
@@ -89,17 +101,17 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
   if (p->method == 0) {
     // e(a,i) r(a,i)
     DPD_HIP_TRY(launch_hadamard_acc(p->l1, (const double*)p->diag.p, R, sigma, nb, p->dim, p->dim, c->stream));
-    Operand W = op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->iabj.p, F(p->iabj32));
+    Operand W = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->iabj.p, F(p->iabj32)), p, 1);
     return contract_call(c, 444, W, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec, nullptr, "c1_r1iabj");
   }
   Desc dR2 = mkdesc(v, v, o, o, 0, 0, S);
   Operand R2 = batched(op(dR2, "", 4, R + p->l1), p->dim);
   Operand C2 = batched(op(dR2, "", 4, sigma + p->l1), p->dim);
-  Operand Fac = op(mkdesc(v, v, z, z, 1, 1, 1), "ae=+", 4, (const double*)p->fac.p, F(p->fac32));
-  Operand Fki = op(mkdesc(o, o, z, z, 1, 1, 1), "mi=-", 4, (const double*)p->fki.p, F(p->fki32));
-  Operand Wk = op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->wkaci.p, F(p->wkaci32));
-  Operand abci = op(mkdesc(v, v, v, o, 0, 1, 1), "feam", 4, (const double*)p->abci.p, F(p->abci32));
-  Operand ijka = op(mkdesc(o, o, o, v, 0, 1, 1), "mnie", 4, (const double*)p->ijka.p, F(p->ijka32));
+  Operand Fac = konst(op(mkdesc(v, v, z, z, 1, 1, 1), "ae=+", 4, (const double*)p->fac.p, F(p->fac32)), p, 2);
+  Operand Fki = konst(op(mkdesc(o, o, z, z, 1, 1, 1), "mi=-", 4, (const double*)p->fki.p, F(p->fki32)), p, 3);
+  Operand Wk = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->wkaci.p, F(p->wkaci32)), p, 4);
+  Operand abci = konst(op(mkdesc(v, v, v, o, 0, 1, 1), "feam", 4, (const double*)p->abci.p, F(p->abci32)), p, 5);
+  Operand ijka = konst(op(mkdesc(o, o, o, v, 0, 1, 1), "mnie", 4, (const double*)p->ijka.p, F(p->ijka32)), p, 6);
   // ---- singles (CISdoo.SD.SF/calAR.f90:41-56)
   // c1_r1Fae: vv2v_222(Fac, r1, c1, 'ae','ei','ai')
   rc = contract_call(c, 444, Fac, lab(R1, "ei=-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec, nullptr, "c1_r1Fae");
@@ -126,9 +138,9 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
   rc = contract_call(c, 424, lab(ijka, "ijmb", 4), lab(R1, "am", 2), lab(C2, "abij", 4), -1.0, 1, nb, p->prec, nullptr, "c2_r1ijka");
   if (rc) return rc;
   if (p->method == 2) {
-    Operand W = op(mkdesc(v, o, v, o, 1, 1, 1), "embj", 4, (const double*)p->iabj.p, F(p->iabj32));
-    Operand ijkl = op(mkdesc(o, o, o, o, 0, 0, 1), "mnij", 4, (const double*)p->ijkl.p, F(p->ijkl32));
-    Operand abcd = op(mkdesc(v, v, v, v, 0, 0, 1), "abef", 4, (const double*)p->abcd.p, F(p->abcd32));
+    Operand W = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "embj", 4, (const double*)p->iabj.p, F(p->iabj32)), p, 7);
+    Operand ijkl = konst(op(mkdesc(o, o, o, o, 0, 0, 1), "mnij", 4, (const double*)p->ijkl.p, F(p->ijkl32)), p, 8);
+    Operand abcd = konst(op(mkdesc(v, v, v, v, 0, 0, 1), "abef", 4, (const double*)p->abcd.p, F(p->abcd32)), p, 9);
     // c2_r2Wmbej: vv2v_444(r2, W, c2, 'aeim','embj','abij', anti=3)
     rc = contract_call(c, 444, lab(R2, "aeim", 4), W, lab(C2, "abij", 4), 1.0, 3, nb, p->prec, nullptr, "c2_r2Wmbej");
     if (rc) return rc;
@@ -157,6 +169,24 @@ static int make_f32(DevBuf& dst, const DevBuf& src, int64_t n, hipStream_t s) {
   return DPD_OK;
 }
 
+int problem_make_f32(dpd_problem* p) {
+  hipStream_t s = p->ctx->stream;
+  int rc;
+  if ((rc = make_f32(p->iabj32, p->iabj, p->len_iabj, s))) return rc;
+  if (p->method >= 1) {
+    if ((rc = make_f32(p->ijka32, p->ijka, p->len_ijka, s))) return rc;
+    if ((rc = make_f32(p->abci32, p->abci, p->len_abci, s))) return rc;
+    if ((rc = make_f32(p->fac32, p->fac, p->len_fac, s))) return rc;
+    if ((rc = make_f32(p->fki32, p->fki, p->len_fki, s))) return rc;
+    if ((rc = make_f32(p->wkaci32, p->wkaci, p->len_iabj, s))) return rc;
+  }
+  if (p->method == 2) {
+    if ((rc = make_f32(p->ijkl32, p->ijkl, p->len_ijkl, s))) return rc;
+    if ((rc = make_f32(p->abcd32, p->abcd, p->len_abcd, s))) return rc;
+  }
+  DPD_HIP_TRY(hipStreamSynchronize(s));
+  return DPD_OK;
+}
 }  // namespace dpd
 
 using namespace dpd;
@@ -175,6 +205,8 @@ int dpd_problem_create(dpd_ctx* c, const dpd_synth_spec* spec_in, int method, in
   if (spec.nirrep != c->sym.nirrep) return fail(DPD_EINVAL, "spec nirrep differs from the context");
   if (symR < 1 || symR > spec.nirrep) return fail(DPD_EINVAL, "symR out of range");
   auto p = std::make_unique<dpd_problem>();
+  static std::atomic<uint64_t> next_uid{1};
+  p->uid = next_uid++;
   p->ctx = c;
   p->spec = spec;
   p->method = method;
@@ -274,26 +306,11 @@ int dpd_problem_destroy(dpd_problem* p) {
 
 int64_t dpd_problem_dim(const dpd_problem* p) { return p ? p->dim : 0; }
 
+
 int dpd_problem_set_precision(dpd_problem* p, int prec) {
   if (!p) return fail(DPD_EINVAL, "null problem");
-  if (prec != PREC_F64 && prec != PREC_F32) return fail(DPD_EINVAL, "precision %d is not available", prec);
-  hipStream_t s = p->ctx->stream;
-  if (prec == PREC_F32 && !p->iabj32.p) {
-    int rc;
-    if ((rc = make_f32(p->iabj32, p->iabj, p->len_iabj, s))) return rc;
-    if (p->method >= 1) {
-      if ((rc = make_f32(p->ijka32, p->ijka, p->len_ijka, s))) return rc;
-      if ((rc = make_f32(p->abci32, p->abci, p->len_abci, s))) return rc;
-      if ((rc = make_f32(p->fac32, p->fac, p->len_fac, s))) return rc;
-      if ((rc = make_f32(p->fki32, p->fki, p->len_fki, s))) return rc;
-      if ((rc = make_f32(p->wkaci32, p->wkaci, p->len_iabj, s))) return rc;
-    }
-    if (p->method == 2) {
-      if ((rc = make_f32(p->ijkl32, p->ijkl, p->len_ijkl, s))) return rc;
-      if ((rc = make_f32(p->abcd32, p->abcd, p->len_abcd, s))) return rc;
-    }
-    DPD_HIP_TRY(hipStreamSynchronize(s));
-  }
+  if (prec != PREC_F64 && prec != PREC_F32 && prec != PREC_BF16X3)
+    return fail(DPD_EINVAL, "precision %d is not available", prec);
   p->prec = prec;
   return DPD_OK;
 }

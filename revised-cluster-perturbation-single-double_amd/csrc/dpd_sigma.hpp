@@ -21,8 +21,10 @@ struct dpd_problem {
   int64_t len_iabj = 0, len_ijka = 0, len_ijkl = 0, len_abij = 0, len_abci = 0, len_abcd = 0;
   int64_t len_fac = 0, len_fki = 0;
   int64_t sigma_flops = 0;  // algorithmic GEMM flops of one sigma (one vector)
+  uint64_t uid = 0;         // process-unique id: constant tensors are packed once under it
 };
 
 namespace dpd {
 int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec);
+int problem_make_f32(dpd_problem* p);
 }

@@ -73,10 +73,12 @@ class Context:
         except Exception:
             pass
 
-    def set_gemm_config(self, cfg=-1):
-        """GEMM tiles of later plans: -1 automatic, 0 first-generation kernel,
-        1..4 pipelined kernel with 64x64, 128x64, 64x128, 128x128 tiles."""
-        check(lib().dpd_ctx_set_gemm_config(self.h, int(cfg)))
+    def set_gemm_config(self, cfg=-1, packing=1):
+        """GEMM kernels of later plans.  cfg: -1 automatic, 0 first-generation
+        kernel, 1..4 tiles 64x64, 128x64, 64x128, 128x128.  packing: 0 operands
+        gathered inside the GEMM, 1 packed panels when an operand is a problem
+        constant (packed once), 2 packed panels always."""
+        check(lib().dpd_ctx_set_gemm_config(self.h, int(cfg), int(packing)))
 
     def profile(self, on=True, max_records=100000):
         """Enable/disable per-launch HIP-event timing of the tagged GEMM stages."""

@@ -86,3 +86,23 @@ def test_davidson_not_converged_path(torch):
     assert not ref["converged"] and not info["converged"]
     assert info["iterations"] == ref["iterations"] == 61
     assert np.max(np.abs(w - ref["w"])) <= 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_davidson_energies(torch, prec):
+    """mixed-precision contractions (FP64 vectors, orthogonalisation and
+    subspace) converged to 1e-5, then FP64 Rayleigh quotients of the vectors:
+    energies within 1e-6 Eh of the FP64 Davidson at 1e-7"""
+    sp, op, gp = make(CFG1, "sf-cpsd", 1)
+    nroot = 3
+    w64, _, info64 = gp.davidson(nroot, conv=1e-7, max_dim=20, max_cyc=60)
+    gp.set_precision(prec)
+    try:
+        wm, R, infom = gp.davidson(nroot, conv=1e-5, max_dim=20, max_cyc=60)
+    finally:
+        gp.set_precision("fp64")
+    assert info64["converged"] and infom["converged"]
+    S = gp.sigma(R).view(nroot, -1)
+    Rm = R.view(nroot, -1)
+    wr = ((Rm * S).sum(1) / (Rm * Rm).sum(1)).cpu().numpy()
+    assert np.max(np.abs(wr - w64)) < 1e-6, (wr, w64)

@@ -40,19 +40,22 @@ def ctx_for(n):
     return _ctxs[n]
 
 
-# GEMM tile configurations: automatic, first-generation kernel, pipelined
-# kernel with 64x64 / 128x64 / 64x128 / 128x128 tiles
-GEMM_CFGS = [-1, 0, 1, 2, 3, 4]
+# GEMM kernels (tile configuration, packing): automatic; first-generation
+# kernel; gathering kernel with 64x64 / 128x64 / 64x128 / 128x128 tiles; the
+# packed-panel kernel with automatic and with each fixed tile shape
+GEMM_CFGS = [(-1, 1), (0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (-1, 2), (1, 2), (2, 2), (3, 2), (4, 2)]
+GEMM_IDS = ["auto", "v1", "g64x64", "g128x64", "g64x128", "g128x128", "pk_auto", "pk64x64", "pk128x64",
+            "pk64x128", "pk128x128"]
 
 
-@pytest.fixture(params=GEMM_CFGS, ids=["auto", "v1", "t64x64", "t128x64", "t64x128", "t128x128"])
+@pytest.fixture(params=GEMM_CFGS, ids=GEMM_IDS)
 def gemm_cfg(request, torch):
-    cfg = request.param
+    cfg, packing = request.param
     for n in (4, 8):
-        ctx_for(n).set_gemm_config(cfg)
-    yield cfg
+        ctx_for(n).set_gemm_config(cfg, packing)
+    yield request.param
     for n in (4, 8):
-        ctx_for(n).set_gemm_config(-1)
+        ctx_for(n).set_gemm_config(-1, 1)
 
 
 @pytest.fixture(params=[4, 8], ids=["c2v", "d2h"])
@@ -126,6 +129,37 @@ def test_xgemm_layout_and_transposes(torch, prec, shape, gemm_cfg):
     dpdhip.xgemm(ctx_for(4), ta, tb, m, n, k, 1.5, gA, A.shape[0], gB, B.shape[0], -0.5, gC, m, prec)
     got = gC.cpu().numpy().reshape(m, n, order="F")
     np.testing.assert_allclose(got, exp, **tol)
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 32, "N", "N"), (70, 45, 37, "N", "N"), (65, 130, 100, "T", "N"),
+                                   (257, 190, 129, "T", "N"), (128, 20, 300, "T", "T")])
+def test_xgemm_bf16x3(torch, shape, gemm_cfg):
+    """BF16x3 MFMA path: matches hi*hi + hi*lo + lo*hi of the emulated split to
+    FP32-accumulation accuracy, and the exact product to the split's accuracy."""
+    import dpdhip
+    if gemm_cfg[1] != 2 and gemm_cfg != (-1, 1):
+        pytest.skip("BF16x3 always runs the packed kernel")
+    m, n, k, ta, tb = shape
+    rng = np.random.default_rng(m * 11 + k)
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    C = rng.standard_normal((m, n))
+    opA = A if ta == "N" else A.T
+    opB = B if tb == "N" else B.T
+    from bf16_emulation import bf16x3_split
+    ah, al = bf16x3_split(opA)
+    bh, bl = bf16x3_split(opB)
+    emu = 1.5 * (ah @ bh + ah @ bl + al @ bh) - 0.5 * C
+    gA = dev(torch, A.flatten(order="F"))
+    gB = dev(torch, B.flatten(order="F"))
+    gC = dev(torch, C.flatten(order="F"))
+    dpdhip.xgemm(ctx_for(4), ta, tb, m, n, k, 1.5, gA, A.shape[0], gB, B.shape[0], -0.5, gC, m, "bf16x3")
+    got = gC.cpu().numpy().reshape(m, n, order="F")
+    mag = 1.5 * (np.abs(opA) @ np.abs(opB))
+    # FP32 accumulation of the three products: at most k roundings of 2^-24
+    assert np.all(np.abs(got - emu) <= max(k, 32) * 2.0 ** -24 * mag + 1e-12), np.abs(got - emu).max()
+    exact = 1.5 * (opA @ opB) - 0.5 * C
+    assert np.all(np.abs(got - exact) <= 3 * 2.0 ** -16 * mag + 1e-12)
 
 
 def _engine_case(torch, sym, kind, pops, iops, syms, t1, t2, t3, f1=1.0, f2=0.0, seed=0):

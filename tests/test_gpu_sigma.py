@@ -74,14 +74,34 @@ def test_sigma_matches_oracle(torch, cfg, symR, method):
     np.testing.assert_allclose(ge, oe, rtol=0, atol=0)
 
 
-def test_sigma_fp32_close_to_fp64(torch):
+@pytest.mark.parametrize("prec,bound", [("fp32", 1e-5), ("bf16x3", 2e-4)])
+def test_sigma_mixed_close_to_fp64(torch, prec, bound):
     import dpdhip
     sp = dpdhip.Spec(CFG1["nso"], CFG1["nalpha"], CFG1["nbeta"], CFG1["seed"])
     gp = dpdhip.Problem(ctx(4), sp, "sf-cpsd", 1)
     rng = np.random.default_rng(3)
     R = torch.from_numpy(rng.standard_normal(2 * gp.dim)).to("cuda:0")
     s64 = gp.sigma(R).cpu().numpy()
-    gp.set_precision("fp32")
+    gp.set_precision(prec)
     s32 = gp.sigma(R).cpu().numpy()
     err = np.abs(s32 - s64).max() / np.abs(s64).max()
-    assert 1e-12 < err < 1e-5, err
+    assert 1e-12 < err < bound, err
+
+
+@pytest.mark.parametrize("packing", [0, 2])
+def test_sigma_kernel_paths_agree(torch, packing):
+    """the same FP64 sigma through the gathering kernel (packing 0) and with
+    every operand packed (packing 2) as through the default path"""
+    import dpdhip
+    sp = dpdhip.Spec(CFG1["nso"], CFG1["nalpha"], CFG1["nbeta"], CFG1["seed"])
+    c = ctx(4)
+    gp = dpdhip.Problem(c, sp, "sf-cpsd", 1)
+    rng = np.random.default_rng(4)
+    R = torch.from_numpy(rng.standard_normal(3 * gp.dim)).to("cuda:0")
+    ref = gp.sigma(R).cpu().numpy()
+    try:
+        c.set_gemm_config(-1, packing)
+        got = gp.sigma(R).cpu().numpy()
+    finally:
+        c.set_gemm_config(-1, 1)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
