@@ -31,7 +31,9 @@ WORKLOADS = {
     "cfg1": dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1, nroot=3),
     "cfg2": dict(nso=(30, 8, 14, 18, 26, 6, 10, 8), nalpha=12, nbeta=10, seed=2, nroot=5),
 }
-PEAK_TFLOPS = {0: 78.6, 1: 157.3}  # MI355X dense MFMA: FP64, FP32
+# MI355X dense MFMA peaks per ALGORITHMIC flop: FP64, FP32, and BF16x3 (three
+# BF16 MFMAs of the 2.5 PFLOP/s dense BF16 peak per algorithmic product)
+PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0 / 3}
 
 
 def log(*a):
@@ -105,20 +107,28 @@ def main():
     # the same code in FP64 at the mixed run's tolerance (equal-work comparison)
     ts64b, (w64b, info64b) = timed("fp64", args.conv_mixed, True, args.steps, args.warmup)
     log(f"fp64@conv_mixed: {np.median(ts64b):.3f}s/solve iterations={info64b['iterations']} w={w64b}")
-    # the timed mixed-precision region also measures every GEMM launch with HIP
-    # events on the stream it runs on
-    for _ in range(args.warmup):
-        solve("fp32", args.conv_mixed, True)
-    ctx.profile(True, 200000)
-    tsm, (wm, infom) = timed("fp32", args.conv_mixed, True, args.steps, 0)
-    prof = ctx.profile_report()
-    ctx.profile(False)
-    log(f"mixed: {np.median(tsm):.3f}s/solve iterations={infom['iterations']} conv={infom['converged']} w={wm}")
-    dw = np.abs(wm - w64)
+    # mixed precision: FP32 MFMA and BF16x3 MFMA contractions; the timed region
+    # also measures every GEMM launch with HIP events on the stream it runs on
+    mixed = {}
+    for prec in ("fp32", "bf16x3"):
+        for _ in range(args.warmup):
+            solve(prec, args.conv_mixed, True)
+        ctx.profile(True, 200000)
+        ts, (wm, infom) = timed(prec, args.conv_mixed, True, args.steps, 0)
+        prof = ctx.profile_report()
+        ctx.profile(False)
+        dw = float(np.abs(wm - w64).max())
+        mixed[prec] = dict(t=float(np.median(ts)), w=wm, info=infom, prof=prof, dw=dw)
+        log(f"{prec}: {np.median(ts):.3f}s/solve iterations={infom['iterations']} conv={infom['converged']} "
+            f"max|dw|={dw:.2e}")
+    ok = {k: v for k, v in mixed.items() if v["info"]["converged"] and v["dw"] <= 1e-6}
+    best = min(ok or mixed, key=lambda k: mixed[k]["t"])
+    M = mixed[best]
     t_fp64 = float(np.median(ts64))
-    t_mix = float(np.median(tsm))
+    t_mix = M["t"]
+    infom, prof = M["info"], M["prof"]
 
-    # dominant kernel of the measured (mixed) run
+    # dominant GEMM kernel of the measured mixed run
     dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
     tag, st = dom
     avg_ms = st["ms"] / st["count"]
@@ -134,6 +144,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, args, infom, lowest)
 
+    kname = {0: "k_gemm_pk<FP64>", 1: "k_gemm_pk<FP32>", 2: "k_gemm_pk<BF16x3>"}[st["prec"]]
     line = {
         "metric": "Davidson time to 5 roots at <=1e-6 Eh: mixed-prec MFMA vs FP64 GPU",
         "value": round(t_mix, 6),
@@ -145,12 +156,13 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "fp32-mfma/fp64",
+        "dtype": f"{best}-mfma/fp64",
         "data": "synthetic (seeded)",
         "config": {"workload": f"{args.workload}: SF-CPSd D2h 120 spatial orbitals [30,8,14,18,26,6,10,8] "
                                f"n_alpha=12 n_beta=10, {nroot} roots, dim {prob.dim}",
                    "davidson": {"tracking": args.tracking, "max_dim": args.max_dim, "max_cyc": args.max_cyc,
                                 "conv_fp64": args.conv, "conv_mixed": args.conv_mixed}},
+        "mixed_precision": best,
         "fp64_s_per_solve": round(t_fp64, 6),
         "fp64_same_tol_s_per_solve": round(float(np.median(ts64b)), 6),
         "speedup_mixed_vs_fp64": round(t_fp64 / t_mix, 3),
@@ -159,8 +171,10 @@ def main():
         "mixed_iterations": infom["iterations"],
         "converged": bool(info64["converged"] and infom["converged"]),
         "w_fp64": [round(float(x), 10) for x in w64],
-        "max_abs_dw_mixed_vs_fp64": float(dw.max()),
-        "roofline": {"bound": "mfma", "kernel": f"k_gemm<float> ({tag})", "achieved": round(achieved, 3),
+        "max_abs_dw_mixed_vs_fp64": M["dw"],
+        "mixed_all": {k: {"s_per_solve": round(v["t"], 6), "iterations": v["info"]["iterations"],
+                          "max_abs_dw": v["dw"]} for k, v in mixed.items()},
+        "roofline": {"bound": "mfma", "kernel": f"{kname} ({tag})", "achieved": round(achieved, 3),
                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                      "flops_per_launch": int(flops_per_launch), "avg_launch_ms": round(avg_ms, 5),
                      "launches": st["count"]},

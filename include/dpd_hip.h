@@ -231,6 +231,7 @@ typedef struct {
   /* outputs */
   int64_t iterations;
   int64_t sigma_calls;
+  int64_t skipped;   /* new directions dropped by the norm < 1e-6 test (:437) */
   double residual[64];
   double final_residual[64];
 } dpd_davidson_opts;

@@ -189,6 +189,7 @@ def davidson(sigma_fn, diagA, guesses, nroot, conv=1e-8, max_dim=20, max_cyc=50,
         raise ValueError("too much roots required")
     maxError = conv
     nsig = 0
+    nskip = 0
     b = np.zeros((dimAbar, dimA))
     Ab = np.zeros((dimAbar, dimA))
     Abar = np.zeros((dimAbar, dimAbar))
@@ -227,6 +228,10 @@ def davidson(sigma_fn, diagA, guesses, nroot, conv=1e-8, max_dim=20, max_cyc=50,
             eigenVal[iroot] = valR[lowestE[iroot]]
         history.append(eigenVal.copy())
         if dim_exceed:
+            # ndimIter = nroot + 5: the reference reads past the subspace when
+            # davidson_dim < nroot + 5; treated as an input error here
+            if ndimIter > dimIter:
+                raise ValueError("subspace collapse needs davidson_dim >= nroot + 5")
             old = np.stack([vecR[:, lowestE[i]] for i in range(ndimIter)], axis=1)
             new = _schmidt_2d(old)
             nb = new.T @ b[:dimIter]
@@ -281,6 +286,7 @@ def davidson(sigma_fn, diagA, guesses, nroot, conv=1e-8, max_dim=20, max_cyc=50,
                 p = p - ov * b[i]
             nrm = np.linalg.norm(p)
             if nrm < 1e-6:
+                nskip += 1
                 continue
             p = p / nrm
             b[dimIter + nNewBase] = p
@@ -318,4 +324,4 @@ def davidson(sigma_fn, diagA, guesses, nroot, conv=1e-8, max_dim=20, max_cyc=50,
             nsig += 1
             final[iroot] = np.max(np.abs(e))
     return {"w": eigenVal.copy(), "R": R, "iterations": iterations, "converged": converged, "sigma_calls": nsig,
-            "residual": res, "final_residual": final, "history": history}
+            "skipped": nskip, "residual": res, "final_residual": final, "history": history}

@@ -33,24 +33,28 @@ namespace dpd {
 
 // ---------------------------------------------------------------------------
 // small dense nonsymmetric eigenproblem (FP64, row-major n x n)
-// Householder reduction to Hessenberg form, Francis double-shift QR for the
-// eigenvalues, inverse iteration on the original matrix for the eigenvectors
-// of real eigenvalues; vectors normalised to unit 2-norm (as dgeev does).
+// Householder reduction A = Q H Q^T to upper Hessenberg form (Q accumulated),
+// Francis double-shift QR for the eigenvalues, and for the requested columns
+// inverse iteration on H (O(n^2) per solve) mapped back by Q; vectors have
+// unit 2-norm (as dgeev's).
 // ---------------------------------------------------------------------------
-static void hessenberg(std::vector<double>& a, int n) {
+static void hessenberg(std::vector<double>& a, int n, std::vector<double>& q) {
+  q.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) q[(size_t)i * n + i] = 1.0;
+  std::vector<double> v(n);
   for (int k = 0; k < n - 2; k++) {
     double alpha = 0.0;
     for (int i = k + 1; i < n; i++) alpha += a[i * n + k] * a[i * n + k];
     alpha = std::sqrt(alpha);
     if (alpha == 0.0) continue;
     if (a[(k + 1) * n + k] > 0) alpha = -alpha;
-    std::vector<double> v(n, 0.0);
+    std::fill(v.begin(), v.end(), 0.0);
     v[k + 1] = a[(k + 1) * n + k] - alpha;
     for (int i = k + 2; i < n; i++) v[i] = a[i * n + k];
     double vv = 0.0;
     for (int i = k + 1; i < n; i++) vv += v[i] * v[i];
     if (vv == 0.0) continue;
-    // A = H A H with H = I - 2 v v^T / (v^T v)
+    // A = H A H with H = I - 2 v v^T / (v^T v); Q = Q H
     for (int j = 0; j < n; j++) {
       double s = 0.0;
       for (int i = k + 1; i < n; i++) s += v[i] * a[i * n + j];
@@ -58,10 +62,17 @@ static void hessenberg(std::vector<double>& a, int n) {
       for (int i = k + 1; i < n; i++) a[i * n + j] -= s * v[i];
     }
     for (int i = 0; i < n; i++) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; j++) s += a[i * n + j] * v[j];
+      double s = 0.0, t = 0.0;
+      for (int j = k + 1; j < n; j++) {
+        s += a[i * n + j] * v[j];
+        t += q[(size_t)i * n + j] * v[j];
+      }
       s = 2.0 * s / vv;
-      for (int j = k + 1; j < n; j++) a[i * n + j] -= s * v[j];
+      t = 2.0 * t / vv;
+      for (int j = k + 1; j < n; j++) {
+        a[i * n + j] -= s * v[j];
+        q[(size_t)i * n + j] -= t * v[j];
+      }
     }
     for (int i = k + 2; i < n; i++) a[i * n + k] = 0.0;
   }
@@ -193,58 +204,84 @@ static int hqr(std::vector<double>& hm, int n, double* wr, double* wi) {
   return 0;
 }
 
-// solve (A - mu I) x = b by LU with partial pivoting (in place on a copy)
-static void shifted_solve(const std::vector<double>& A, int n, double mu, std::vector<double>& b, double tiny) {
-  std::vector<double> M(A);
-  std::vector<int> piv(n);
-  for (int i = 0; i < n; i++) M[i * n + i] -= mu;
-  for (int k = 0; k < n; k++) {
-    int pr = k;
-    for (int i = k + 1; i < n; i++)
-      if (std::fabs(M[i * n + k]) > std::fabs(M[pr * n + k])) pr = i;
-    piv[k] = pr;
-    if (pr != k) {
-      for (int j = 0; j < n; j++) std::swap(M[k * n + j], M[pr * n + j]);
-      std::swap(b[k], b[pr]);
+// solve (H - mu I) y = b for upper Hessenberg H: LU with partial pivoting
+// between neighbouring rows (O(n^2)); tiny pivots are replaced by +-tiny
+static void hess_solve(const std::vector<double>& H, int n, double mu, double* b, double tiny,
+                       std::vector<double>& M) {
+  M = H;
+  for (int i = 0; i < n; i++) M[(size_t)i * n + i] -= mu;
+  for (int k = 0; k < n - 1; k++) {
+    double* rk = &M[(size_t)k * n];
+    double* r1 = &M[(size_t)(k + 1) * n];
+    if (std::fabs(r1[k]) > std::fabs(rk[k])) {
+      for (int j = k; j < n; j++) std::swap(rk[j], r1[j]);
+      std::swap(b[k], b[k + 1]);
     }
-    if (std::fabs(M[k * n + k]) < tiny) M[k * n + k] = M[k * n + k] < 0 ? -tiny : tiny;
-    for (int i = k + 1; i < n; i++) {
-      double f = M[i * n + k] / M[k * n + k];
-      M[i * n + k] = 0.0;
-      for (int j = k + 1; j < n; j++) M[i * n + j] -= f * M[k * n + j];
-      b[i] -= f * b[k];
-    }
+    if (std::fabs(rk[k]) < tiny) rk[k] = rk[k] < 0 ? -tiny : tiny;
+    const double f = r1[k] / rk[k];
+    r1[k] = 0.0;
+    for (int j = k + 1; j < n; j++) r1[j] -= f * rk[j];
+    b[k + 1] -= f * b[k];
   }
+  double* rl = &M[(size_t)(n - 1) * n];
+  if (std::fabs(rl[n - 1]) < tiny) rl[n - 1] = rl[n - 1] < 0 ? -tiny : tiny;
   for (int i = n - 1; i >= 0; i--) {
     double s = b[i];
-    for (int j = i + 1; j < n; j++) s -= M[i * n + j] * b[j];
-    b[i] = s / M[i * n + i];
+    const double* ri = &M[(size_t)i * n];
+    for (int j = i + 1; j < n; j++) s -= ri[j] * b[j];
+    b[i] = s / ri[i];
   }
 }
 
-// eigenvalues (wr, wi) and right eigenvectors (columns of V, column-major n x n)
-int small_eig(int n, const double* Arow, double* wr, double* wi, double* V) {
-  std::vector<double> A(Arow, Arow + (size_t)n * n), H(A);
-  if (n == 0) return 0;
-  hessenberg(H, n);
-  if (hqr(H, n, wr, wi) != 0) return fail(DPD_EINVAL, "subspace eigensolver did not converge");
-  double anorm = 0.0;
-  for (int i = 0; i < n * n; i++) anorm = std::max(anorm, std::fabs(A[i]));
-  if (anorm == 0.0) anorm = 1.0;
-  const double eps = 2.220446049250313e-16;
-  for (int k = 0; k < n; k++) {
-    std::vector<double> x(n);
-    for (int i = 0; i < n; i++) x[i] = 1.0 + 0.01 * ((i * 7 + 3) % 11);
-    double mu = wr[k] + eps * anorm * 4.0;
-    for (int it = 0; it < 3; it++) {
-      shifted_solve(A, n, mu, x, eps * anorm);
-      double nrm = 0.0;
-      for (int i = 0; i < n; i++) nrm += x[i] * x[i];
-      nrm = std::sqrt(nrm);
-      for (int i = 0; i < n; i++) x[i] /= nrm;
-    }
-    for (int i = 0; i < n; i++) V[i + (size_t)k * n] = x[i];
+// two-phase solver: eigenvalues first, then eigenvectors of chosen real eigenvalues
+struct SmallEig {
+  int n = 0;
+  std::vector<double> H, Q, T, y, M;
+  double anorm = 1.0;
+  const double* wr = nullptr;
+  int values(int n_, const double* Arow, double* wr_, double* wi_) {
+    n = n_;
+    wr = wr_;
+    if (n == 0) return DPD_OK;
+    H.assign(Arow, Arow + (size_t)n * n);
+    hessenberg(H, n, Q);
+    T = H;
+    if (hqr(T, n, wr_, wi_) != 0) return fail(DPD_EINVAL, "subspace eigensolver did not converge");
+    anorm = 0.0;
+    for (int i = 0; i < n * n; i++) anorm = std::max(anorm, std::fabs(Arow[i]));
+    if (anorm == 0.0) anorm = 1.0;
+    return DPD_OK;
   }
+  // unit right eigenvector of eigenvalue k into x[0..n)
+  void vector(int k, double* x) {
+    const double eps = 2.220446049250313e-16;
+    y.resize(n);
+    for (int i = 0; i < n; i++) y[i] = 1.0 + 0.01 * ((i * 7 + 3) % 11);
+    const double mu = wr[k] + eps * anorm * 4.0;
+    for (int it = 0; it < 3; it++) {
+      hess_solve(H, n, mu, y.data(), eps * anorm, M);
+      double nrm = 0.0;
+      for (int i = 0; i < n; i++) nrm += y[i] * y[i];
+      nrm = std::sqrt(nrm);
+      for (int i = 0; i < n; i++) y[i] /= nrm;
+    }
+    // x = Q y (Q orthogonal: unit norm kept)
+    for (int i = 0; i < n; i++) {
+      double s = 0.0;
+      const double* qi = &Q[(size_t)i * n];
+      for (int j = 0; j < n; j++) s += qi[j] * y[j];
+      x[i] = s;
+    }
+  }
+};
+
+// eigenvalues (wr, wi) of the row-major matrix and right eigenvectors of every
+// eigenvalue (columns of V, column-major n x n)
+int small_eig(int n, const double* Arow, double* wr, double* wi, double* V) {
+  SmallEig E;
+  int rc = E.values(n, Arow, wr, wi);
+  if (rc) return rc;
+  for (int k = 0; k < n; k++) E.vector(k, V + (size_t)k * n);
   return DPD_OK;
 }
 
@@ -308,7 +345,7 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
   const double* diag = (const double*)P->diag.p;
   auto bv = [&](int i) { return B + (size_t)i * n; };
   auto abv = [&](int i) { return AB + (size_t)i * n; };
-  int64_t nsig = 0;
+  int64_t nsig = 0, nskip = 0;
   auto sigma = [&](const double* x, double* y, int nv) -> int {
     nsig += nv;
     return problem_sigma(P, x, y, nv);
@@ -343,21 +380,28 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
   int dimIter = nroot, dimFree = dimAbar - dimIter;
   bool dim_exceed = false, converged = false;
   int ndimIter = 0, ndimFree = 0;
-  std::vector<double> wr(dimAbar), wi(dimAbar), V(nd2), sub(nd2), nrm(nroot), mv(nroot);
+  std::vector<double> wr(dimAbar), wi(dimAbar), V(nd2), sub(nd2), mv(nroot);
   std::vector<int> lowestE(dimAbar);
   std::vector<int32_t> cand;
+  SmallEig eig;
   while (true) {
     if (iterations > maxcyc) break;
     // eigenproblem of Abar(1:dimIter,1:dimIter) (:185-219)
     for (int i = 0; i < dimIter; i++)
       for (int j = 0; j < dimIter; j++) sub[(size_t)i * dimIter + j] = A_(i, j);
-    if ((rc = small_eig(dimIter, sub.data(), wr.data(), wi.data(), V.data()))) return rc;
+    if ((rc = eig.values(dimIter, sub.data(), wr.data(), wi.data()))) return rc;
     auto vec = [&](int row, int col) { return V[row + (size_t)col * dimIter]; };
     // sort (:614-635) and root selection (:225-251)
     for (int i = 0; i < dimIter; i++) lowestE[i] = i;
     for (int i = 0; i < dimIter - 1; i++)
       for (int j = i + 1; j < dimIter; j++)
         if (wr[lowestE[j]] < wr[lowestE[i]]) std::swap(lowestE[i], lowestE[j]);
+    // eigenvectors: every one when roots are tracked by overlap, else only the
+    // lowest nroot (nroot + 5 when the subspace collapses)
+    {
+      const int need = o->lowest ? std::min(dimIter, dim_exceed ? std::max(nroot, ndimIter) : nroot) : dimIter;
+      for (int j = 0; j < need; j++) eig.vector(lowestE[j], V.data() + (size_t)lowestE[j] * dimIter);
+    }
     if (!o->lowest) {
       for (int iroot = 0; iroot < nroot; iroot++) {
         double t1 = 0.0;
@@ -378,7 +422,10 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
       for (int i = 0; i < dimIter; i++) vecNeed[i + (size_t)iroot * dimAbar] = vec(i, lowestE[iroot]);
       eigenVal[iroot] = wr[lowestE[iroot]];
     }
-    // collapse (:268-341)
+    // collapse (:268-341); ndimIter = nroot + 5 and the reference reads past
+    // the subspace when davidson_dim < nroot + 5: an input error here
+    if (dim_exceed && ndimIter > dimIter)
+      return fail(DPD_EINVAL, "subspace collapse needs davidson_dim >= nroot + 5 (%d < %d)", dimAbar, ndimIter);
     if (dim_exceed) {
       std::vector<double> nw((size_t)dimIter * ndimIter);
       for (int k = 0; k < ndimIter; k++)
@@ -420,15 +467,7 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
       DPD_HIP_TRY(launch_lincomb(n, B, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, R, st));
     }
     DPD_HIP_TRY(launch_pair_dots(n, R, n, R, n, nroot, part, dots, st));
-    DPD_HIP_TRY(hipMemcpyAsync(nrm.data(), dots, sizeof(double) * nroot, hipMemcpyDeviceToHost, st));
-    if ((rc = dsync(c))) return rc;
-    for (int iroot = 0; iroot < nroot; iroot++) {
-      const double t = std::sqrt(nrm[iroot]);
-      if (std::fabs(t - 1.0) > 1e-10) {
-        DPD_HIP_TRY(launch_scal(n, 1.0 / t, R + (size_t)iroot * n, st));
-        DPD_HIP_TRY(launch_scal(n, 1.0 / t, AQ + (size_t)iroot * n, st));
-      }
-    }
+    DPD_HIP_TRY(launch_renorm_pair(n, dots, R, AQ, nroot, st));
     // residuals (:396-406)
     DPD_HIP_TRY(hipMemcpyAsync(wdev, eigenVal.data(), sizeof(double) * nroot, hipMemcpyHostToDevice, st));
     if ((rc = residuals(AQ, mv.data()))) return rc;
@@ -449,27 +488,57 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
     // every basis vector (the new ones of this iteration included), normalise;
     // skipped when the remaining norm is below 1e-6.  Each projection step is one
     // launch that finishes the previous reduction, updates p and starts the next.
-    for (size_t k = 0; k < cand.size(); k++) {
-      if (dimFree == 0) continue;
+    // The roots are orthogonalised back to back assuming none is skipped; the
+    // norms are checked once at the end and, in the rare case that one fell
+    // below 1e-6, the roots from there on are redone one at a time.
+    auto schmidt = [&](size_t k, int slot, double* nrm_dev) -> int {
       double* p = PV + k * n;
       double* pin = (double*)W.partA.p;
       double* pout = (double*)W.partB.p;
       DPD_HIP_TRY(launch_pair_dots_partial(n, p, p, pin, st));
-      const int nb = dimIter + nNewBase;
+      const int nb = dimIter + slot;
       for (int i = 0; i <= nb; i++) {
         const double* bprev = i > 0 ? bv(i - 1) : nullptr;
         const double* bnext = i < nb ? bv(i) : nullptr;
         DPD_HIP_TRY(launch_mgs_step(n, p, bprev, bnext, pin, i == 0 ? 0 : 1, pout, st));
         std::swap(pin, pout);
       }
-      double s2;
-      DPD_HIP_TRY(launch_reduce_partials(pin, 1, dots, st));
-      DPD_HIP_TRY(hipMemcpyAsync(&s2, dots, sizeof(double), hipMemcpyDeviceToHost, st));
-      if ((rc = dsync(c))) return rc;
-      if (std::sqrt(s2) < 1e-6) continue;
-      DPD_HIP_TRY(launch_scale_by_norm(n, p, pin, bv(dimIter + nNewBase), st));
-      dimFree--;
-      nNewBase++;
+      DPD_HIP_TRY(launch_reduce_partials(pin, 1, nrm_dev, st));
+      DPD_HIP_TRY(launch_scale_by_norm(n, p, pin, bv(dimIter + slot), st));
+      return DPD_OK;
+    };
+    {
+      const size_t nopt = std::min<size_t>(cand.size(), (size_t)std::max(dimFree, 0));
+      for (size_t k = 0; k < nopt; k++)
+        if ((rc = schmidt(k, (int)k, dots + k))) return rc;
+      std::vector<double> s2(nopt);
+      if (nopt) {
+        DPD_HIP_TRY(hipMemcpyAsync(s2.data(), dots, sizeof(double) * nopt, hipMemcpyDeviceToHost, st));
+        if ((rc = dsync(c))) return rc;
+      }
+      size_t k = 0;
+      for (; k < nopt && std::sqrt(s2[k]) >= 1e-6; k++) {
+        dimFree--;
+        nNewBase++;
+      }
+      if (k < nopt) {
+        nskip++;
+        // root k is skipped: the roots after it were projected against it, redo them
+        for (k = k + 1; k < cand.size(); k++) {
+          if (dimFree == 0) continue;
+          DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p + k, 1, diag, E, PV + k * n, st));
+          if ((rc = schmidt(k, nNewBase, dots))) return rc;
+          double t2;
+          DPD_HIP_TRY(hipMemcpyAsync(&t2, dots, sizeof(double), hipMemcpyDeviceToHost, st));
+          if ((rc = dsync(c))) return rc;
+          if (std::sqrt(t2) < 1e-6) {
+            nskip++;
+            continue;
+          }
+          dimFree--;
+          nNewBase++;
+        }
+      }
     }
     // sigma of this iteration's new vectors, one batched evaluation (:442-444)
     if (nNewBase > 0)
@@ -516,6 +585,7 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
     for (int i = 0; i < nroot && i < 64; i++) o->final_residual[i] = std::fabs(mv[i]);
   }
   o->sigma_calls = nsig;
+  o->skipped = nskip;
   if ((rc = dsync(c))) return rc;
   return converged ? DPD_OK : DPD_ENOTCONV;
 }

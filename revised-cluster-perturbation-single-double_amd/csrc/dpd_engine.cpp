@@ -652,7 +652,7 @@ static int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const do
 // packed-panel GEMM: static operands come from the variant's cache (packed on
 // first use), the others are packed into the workspace reserved by the caller
 int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const double* A, uint64_t a_id, const double* B,
-                           uint64_t b_id, double* C, double f1, double f2) {
+                    uint64_t b_id, double* C, double f1, double f2, hipEvent_t ev0, hipEvent_t ev1) {
   GemmVariant& V = p.pk[mode];
   const size_t es = pk_esize(mode);
   const void* side_ptr[2];
@@ -685,7 +685,11 @@ int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const double* A, uint64_t
       side_ptr[sd] = w;
     }
   }
-  if (V.nwg == 0) return DPD_OK;
+  if (V.nwg == 0) {
+    if (ev0) DPD_HIP_TRY(hipEventRecord(ev0, c->stream));
+    if (ev1) DPD_HIP_TRY(hipEventRecord(ev1, c->stream));
+    return DPD_OK;
+  }
   GemmLaunch g;
   std::memset(&g, 0, sizeof g);
   g.C = C;
@@ -701,7 +705,9 @@ int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const double* A, uint64_t
   g.max_split_mn = V.max_split_mn;
   g.cfg = V.cfg;
   if (V.partial_elems > 0 && !(g.partial = (double*)ws_take(c, sizeof(double) * V.partial_elems))) return DPD_ENOMEM;
+  if (ev0) DPD_HIP_TRY(hipEventRecord(ev0, c->stream));
   DPD_HIP_TRY(launch_gemm_pk(g, side_ptr[0], side_ptr[1], c->stream));
+  if (ev1) DPD_HIP_TRY(hipEventRecord(ev1, c->stream));
   return DPD_OK;
 }
 
@@ -1109,15 +1115,17 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
                             prec});
     c->prof.used += 2;
     pr = &c->prof.recs.back();
-    DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
+    // the packed path records the events around the GEMM kernel itself
+    if (!packed) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
   }
   if (packed)
-    rc = run_gemm_packed(c, P->gemm, prec, p1, id1, p2, id2, p3, fact, 1.0);
+    rc = run_gemm_packed(c, P->gemm, prec, p1, id1, p2, id2, p3, fact, 1.0, pr ? pr->a : nullptr,
+                         pr ? pr->b : nullptr);
   else
     rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, 1.0,
                   use_f32 ? PREC_F32 : PREC_F64);
   if (rc) return rc;
-  if (pr) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
+  if (pr && !packed) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
   if (anti) {
     MapArgs m = P->epi;
     m.nbatch = nbatch;

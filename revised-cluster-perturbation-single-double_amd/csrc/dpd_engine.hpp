@@ -168,7 +168,8 @@ int build_variant(GemmPlan& p, int mode);
 // packed-panel GEMM; operands with a nonzero id are packed once and cached,
 // the others are packed into workspace the caller reserved
 int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const double* A, uint64_t a_id, const double* B,
-                    uint64_t b_id, double* C, double f1, double f2);
+                    uint64_t b_id, double* C, double f1, double f2, hipEvent_t ev0 = nullptr,
+                    hipEvent_t ev1 = nullptr);
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out);
 // labels / checks shared by the C-ABI and the sigma builders
 int check_labels(const char* t, int n);

@@ -693,6 +693,27 @@ hipError_t launch_pair_dots(int64_t n, const double* x, int64_t xs, const double
   return hipGetLastError();
 }
 
+// x_v *= 1/t and y_v *= 1/t with t = sqrt(nrm2[v]) when |t - 1| > 1e-10
+// (lib_cc/mod_davidson.f90:378-386: the Ritz vector is renormalised only then)
+__global__ void k_renorm_pair(int64_t n, const double* nrm2, double* __restrict__ x, double* __restrict__ y) {
+  const int v = blockIdx.y;
+  const double t = sqrt(nrm2[v]);
+  if (!(fabs(t - 1.0) > 1e-10)) return;
+  const double f = 1.0 / t;
+  double* xv = x + (int64_t)v * n;
+  double* yv = y + (int64_t)v * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    xv[i] *= f;
+    yv[i] *= f;
+  }
+}
+
+hipError_t launch_renorm_pair(int64_t n, const double* nrm2, double* x, double* y, int nv, hipStream_t s) {
+  if (nv == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_renorm_pair, dim3(grid_for(n), nv), dim3(256), 0, s, n, nrm2, x, y);
+  return hipGetLastError();
+}
+
 hipError_t launch_pair_dots_partial(int64_t n, const double* x, const double* y, double* partial, hipStream_t s) {
   hipLaunchKernelGGL(k_pair_dots, dim3(kRedBlocks, 1), dim3(256), 0, s, n, x, (int64_t)0, y, (int64_t)0, partial);
   return hipGetLastError();

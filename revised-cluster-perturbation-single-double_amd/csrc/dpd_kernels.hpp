@@ -157,6 +157,8 @@ hipError_t launch_pair_dots(int64_t n, const double* x, int64_t xs, const double
 // out[xi + yj*nx] = X_xi . Y_yj  (partial: kRedBlocks*nx*ny)
 hipError_t launch_block_dots(int64_t n, const double* X, int64_t xs, int nx, const double* Y, int64_t ys, int ny,
                              double* partial, double* out, hipStream_t s);
+// x_v, y_v scaled by 1/sqrt(nrm2[v]) unless that norm is 1 within 1e-10
+hipError_t launch_renorm_pair(int64_t n, const double* nrm2, double* x, double* y, int nv, hipStream_t s);
 // partial[0..kRedBlocks) of dot(x, y); out[r] = sum of row r of kRedBlocks partials
 hipError_t launch_pair_dots_partial(int64_t n, const double* x, const double* y, double* partial, hipStream_t s);
 hipError_t launch_reduce_partials(const double* partial, int nrows, double* out, hipStream_t s);

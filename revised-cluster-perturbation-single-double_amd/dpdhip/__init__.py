@@ -373,6 +373,7 @@ class Problem:
         if rc not in (0, _lib.DPD_ENOTCONV):
             check(rc)
         info = {"converged": rc == 0, "iterations": o.iterations, "sigma_calls": o.sigma_calls,
+                "skipped": o.skipped,
                 "residual": list(o.residual[:nroot]), "final_residual": list(o.final_residual[:nroot])}
         return w, R, info
 

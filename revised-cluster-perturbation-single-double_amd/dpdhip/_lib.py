@@ -42,7 +42,7 @@ class DavidsonOpts_(ctypes.Structure):
     """dpd_davidson_opts"""
     _fields_ = [("nroot", i64), ("max_dim", i64), ("max_cyc", i64), ("conv", ctypes.c_double),
                 ("lowest", ctypes.c_int), ("final_check", ctypes.c_int), ("iterations", i64),
-                ("sigma_calls", i64), ("residual", ctypes.c_double * 64), ("final_residual", ctypes.c_double * 64)]
+                ("sigma_calls", i64), ("skipped", i64), ("residual", ctypes.c_double * 64), ("final_residual", ctypes.c_double * 64)]
 
 
 _lib = None

@@ -106,3 +106,34 @@ def test_mixed_precision_davidson_energies(torch, prec):
     Rm = R.view(nroot, -1)
     wr = ((Rm * S).sum(1) / (Rm * Rm).sum(1)).cpu().numpy()
     assert np.max(np.abs(wr - w64)) < 1e-6, (wr, w64)
+
+
+@pytest.mark.parametrize("cfg,method,nroot,max_dim", [
+    (dict(nso=(4, 1, 2, 3), nalpha=3, nbeta=1, seed=3), "sf-cis", 3, 12),
+    (dict(nso=(3, 1, 1, 2), nalpha=2, nbeta=1, seed=5), "sf-cis", 3, 12),
+    (dict(nso=(3, 1, 1, 2), nalpha=2, nbeta=1, seed=5), "sf-cpsd", 4, 20)], ids=["tiny-cis", "small-cis", "small-cpsd"])
+def test_davidson_skipped_directions(torch, cfg, method, nroot, max_dim):
+    """subspaces that fill the space: directions with remaining norm < 1e-6
+    are dropped (:437) -- the batched orthogonalisation redoes the roots after a
+    dropped one; the same directions are dropped as in the oracle"""
+    sp, op, gp = make(cfg, method, 1)
+    guesses, _ = op.guess(nroot)
+    ref = OS.davidson(op.sigma, op.diag(), guesses, nroot, conv=1e-9, max_dim=max_dim, max_cyc=60)
+    w, R, info = gp.davidson(nroot, conv=1e-9, max_dim=max_dim, max_cyc=60)
+    assert ref["skipped"] > 0
+    assert info["skipped"] == ref["skipped"], (info["skipped"], ref["skipped"])
+    assert info["iterations"] == ref["iterations"]
+    assert info["converged"] == ref["converged"]
+    assert np.max(np.abs(w - ref["w"])) <= 1e-8, (w, ref["w"])
+
+
+def test_davidson_collapse_needs_room(torch):
+    """a collapse keeps nroot + 5 vectors: davidson_dim below that is rejected
+    (the reference reads past its subspace there)"""
+    import dpdhip
+    cfg = dict(nso=(4, 1, 2, 3), nalpha=3, nbeta=1, seed=3)
+    sp, op, gp = make(cfg, "sf-cis", 1)
+    with pytest.raises(ValueError):
+        OS.davidson(op.sigma, op.diag(), op.guess(4)[0], 4, conv=1e-9, max_dim=8, max_cyc=60)
+    with pytest.raises(dpdhip._lib.DpdError, match="nroot \\+ 5"):
+        gp.davidson(4, conv=1e-9, max_dim=8, max_cyc=60)
