@@ -289,7 +289,8 @@ int small_eig(int n, const double* Arow, double* wr, double* wi, double* V) {
 // Davidson
 // ---------------------------------------------------------------------------
 struct DavWork {
-  DevBuf B, AB, NB, NAB, AQ, E, P, coef, part, partA, partB, dots, amaxp, amaxpi, amaxv, amaxi, wdev, idx;
+  DevBuf B, AB, NB, NAB, AQ, E, P, coef, part, partA, partB, partC, partD, dots, amaxp, amaxpi, amaxv, amaxi, wdev,
+      idx;
 };
 
 static int dsync(dpd_ctx* c) {
@@ -327,8 +328,10 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
   if ((rc = W.B.alloc(vb * dimAbar)) || (rc = W.AB.alloc(vb * dimAbar)) || (rc = W.NB.alloc(vb * dimAbar)) ||
       (rc = W.NAB.alloc(vb * dimAbar)) || (rc = W.AQ.alloc(vb * nroot)) || (rc = W.E.alloc(vb * nroot)) ||
       (rc = W.P.alloc(vb * nroot)) || (rc = W.coef.alloc(sizeof(double) * nd2)) ||
-      (rc = W.part.alloc(sizeof(double) * kRedBlocks * nd2)) || (rc = W.partA.alloc(sizeof(double) * kRedBlocks)) ||
-      (rc = W.partB.alloc(sizeof(double) * kRedBlocks)) || (rc = W.dots.alloc(sizeof(double) * 2 * nd2)) ||
+      (rc = W.part.alloc(sizeof(double) * kRedBlocks * nd2)) ||
+      (rc = W.partA.alloc(sizeof(double) * kRedBlocks * kMgsMulti)) ||
+      (rc = W.partB.alloc(sizeof(double) * kRedBlocks * kMgsMulti)) ||
+      (rc = W.partC.alloc(sizeof(double) * kRedBlocks)) || (rc = W.partD.alloc(sizeof(double) * kRedBlocks)) || (rc = W.dots.alloc(sizeof(double) * 2 * nd2)) ||
       (rc = W.amaxp.alloc(sizeof(double) * kRedBlocks * nroot)) ||
       (rc = W.amaxpi.alloc(sizeof(int64_t) * kRedBlocks * nroot)) || (rc = W.amaxv.alloc(sizeof(double) * nroot)) ||
       (rc = W.amaxi.alloc(sizeof(int64_t) * nroot)) || (rc = W.wdev.alloc(sizeof(double) * nroot)) ||
@@ -509,8 +512,40 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
     };
     {
       const size_t nopt = std::min<size_t>(cand.size(), (size_t)std::max(dimFree, 0));
-      for (size_t k = 0; k < nopt; k++)
-        if ((rc = schmidt(k, (int)k, dots + k))) return rc;
+      // phase 1: up to kMgsMulti directions at a time are normalised and
+      // projected on the old basis together (each basis vector read once);
+      // phase 2: each is projected on this iteration's earlier new vectors
+      for (size_t c0 = 0; c0 < nopt; c0 += kMgsMulti) {
+        const int nv = (int)std::min<size_t>(kMgsMulti, nopt - c0);
+        double* P0 = PV + c0 * n;
+        double* pin = (double*)W.partA.p;
+        double* pout = (double*)W.partB.p;
+        DPD_HIP_TRY(launch_pair_dots(n, P0, n, P0, n, nv, pin, nullptr, st));
+        for (int i = 0; i <= dimIter; i++) {
+          const double* bprev = i > 0 ? bv(i - 1) : nullptr;
+          const double* bnext = i < dimIter ? bv(i) : nullptr;
+          DPD_HIP_TRY(launch_mgs_multi(n, P0, nv, bprev, bnext, pin, i == 0 ? 0 : 1, pout, st));
+          std::swap(pin, pout);
+        }
+        for (int j = 0; j < nv; j++) {
+          const size_t k = c0 + j;
+          double* pk = PV + k * n;
+          const double* fin = pin + (size_t)j * kRedBlocks;  // p_k . p_k after phase 1
+          if (k > 0) {
+            double* qa = (double*)W.partC.p;
+            double* qb = (double*)W.partD.p;
+            DPD_HIP_TRY(launch_pair_dots_partial(n, pk, bv(dimIter), qa, st));
+            for (size_t j2 = 0; j2 < k; j2++) {
+              const double* bnext = j2 + 1 < k ? bv(dimIter + (int)j2 + 1) : nullptr;
+              DPD_HIP_TRY(launch_mgs_step(n, pk, bv(dimIter + (int)j2), bnext, qa, 1, qb, st));
+              std::swap(qa, qb);
+            }
+            fin = qa;
+          }
+          DPD_HIP_TRY(launch_reduce_partials(fin, 1, dots + k, st));
+          DPD_HIP_TRY(launch_scale_by_norm(n, pk, fin, bv(dimIter + (int)k), st));
+        }
+      }
       std::vector<double> s2(nopt);
       if (nopt) {
         DPD_HIP_TRY(hipMemcpyAsync(s2.data(), dots, sizeof(double) * nopt, hipMemcpyDeviceToHost, st));

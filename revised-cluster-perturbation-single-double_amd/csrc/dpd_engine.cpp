@@ -248,11 +248,28 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
   pad_labels(T[1], o2, '\x03', '\x04');
   pad_labels(T[2], o3, '\x05', '\x06');
   std::vector<Mode> modes;
+  // a full (iop=1) pair whose two labels appear only as that same ordered pair
+  // in every tensor is one GEMM digit (the pair index of its irrep block): the
+  // GEMM then runs per pair irrep instead of per irrep of each index
+  auto intact_full = [&](int t, int s) -> bool {
+    const char p0 = T[t].lab[2 * s], q0 = T[t].lab[2 * s + 1];
+    for (int t2 = 0; t2 < 3; t2++)
+      for (int s2 = 0; s2 < 2; s2++) {
+        const char a = T[t2].lab[2 * s2], b = T[t2].lab[2 * s2 + 1];
+        const bool has = a == p0 || a == q0 || b == p0 || b == q0;
+        if (has && !(a == p0 && b == q0 && T[t2].d.iop[s2] == 1)) return false;
+        // populations must agree index by index (else the per-index modes report it)
+        if (has && (std::memcmp(T[t2].d.pop[2 * s2], T[t].d.pop[2 * s], sizeof(int64_t) * 8) != 0 ||
+                    std::memcmp(T[t2].d.pop[2 * s2 + 1], T[t].d.pop[2 * s + 1], sizeof(int64_t) * 8) != 0))
+          return false;
+      }
+    return true;
+  };
   for (int t = 0; t < 3; t++) {
     TV& tv = T[t];
     for (int s = 0; s < 2; s++) {
       tv.pair_mode[s] = -1;
-      if (tv.d.iop[s] != 1) {
+      if (tv.d.iop[s] != 1 || intact_full(t, s)) {
         char lab[2] = {tv.lab[2 * s], tv.lab[2 * s + 1]};
         int id = find_mode(modes, lab, true);
         int64_t sz[8];
@@ -930,6 +947,29 @@ static MapArgs identity_map(const Sym& S, const Desc& dout, const Desc& din) {
 
 // mod_contract.f90:111-160 fused: v3 += X - P(pq)X - P(rs)X + P(pq)P(rs)X,
 // gathered straight into v3's (possibly packed) layout
+// gather of din into dout where output dim i comes from input dim perm[i]
+static MapArgs perm_map(const Sym& S, const Desc& dout, const Desc& din, const int* perm) {
+  MapArgs a = identity_map(S, dout, din);
+  for (int i = 0; i < 4; i++) a.term[0].perm[i] = (int8_t)perm[i];
+  return a;
+}
+
+// v3 += X - P(pq)X - P(rs)X + P(pq)P(rs)X (the swaps as requested) where the
+// scratch X holds v3's indices in another order: v3 dim i is X dim pos[i]
+static MapArgs anti_map_perm(const Sym& S, const Desc& d3, const Desc& dx, const int* pos, bool apq, bool ars) {
+  MapArgs a = identity_map(S, d3, dx);
+  a.nterm = 0;
+  for (int i = 0; i < (apq ? 2 : 1); i++)
+    for (int j = 0; j < (ars ? 2 : 1); j++) {
+      MapTerm& t = a.term[a.nterm++];
+      const int src[4] = {i ? 1 : 0, i ? 0 : 1, j ? 3 : 2, j ? 2 : 3};
+      for (int d = 0; d < 4; d++) t.perm[d] = (int8_t)pos[src[d]];
+      t.coef = ((i + j) & 1) ? -1.0 : 1.0;
+    }
+  a.beta = 1.0;
+  return a;
+}
+
 static MapArgs anti_map(const Sym& S, const Desc& d3, const Desc& dx, bool apq, bool ars) {
   MapArgs a = identity_map(S, d3, dx);
   a.nterm = 0;
@@ -944,6 +984,118 @@ static MapArgs anti_map(const Sym& S, const Desc& d3, const Desc& dx, bool apq, 
     }
   a.beta = 1.0;
   return a;
+}
+
+// Regrouping of a 444 contraction whose operands split pairs between GEMM
+// dimensions (e.g. the ring term R(ae,im) W(em,bj) -> X(ab,ij)): the operands
+// are gathered into full-pair layouts (M M | K K), (K K | N N) and the GEMM
+// writes X(M M | N N), so that every pair is one GEMM digit and the GEMM runs
+// per pair irrep (few large blocks) instead of per irrep of every index.  X is
+// then mapped into v3 with the requested antisymmetrisation.  The result is the
+// same contraction; only the GEMM blocking differs.
+static int regroup_444(const Sym& S, const Operand& o1, const Operand& o2, const Operand& o3, bool apq, bool ars,
+                       ContractPlan& P) {
+  const char* L[3] = {o1.lab, o2.lab, o3.lab};
+  const Desc* D[3] = {&o1.d, &o2.d, &o3.d};
+  auto find = [&](int t, char c) -> int {
+    for (int i = 0; i < 4; i++)
+      if (L[t][i] == c) return i;
+    return -1;
+  };
+  // classes: 0 = M (v1,v3), 1 = N (v2,v3), 2 = K (v1,v2)
+  auto cls = [&](char c) -> int {
+    const bool a = find(0, c) >= 0, b = find(1, c) >= 0, d = find(2, c) >= 0;
+    if (a && d && !b) return 0;
+    if (b && d && !a) return 1;
+    if (a && b && !d) return 2;
+    return -1;
+  };
+  int cnt[3][3] = {{0}};
+  for (int t = 0; t < 3; t++)
+    for (int i = 0; i < 4; i++) {
+      const int k = cls(L[t][i]);
+      if (k < 0) return DPD_OK;
+      cnt[t][k]++;
+    }
+  if (cnt[0][0] != 2 || cnt[0][2] != 2 || cnt[1][2] != 2 || cnt[1][1] != 2 || cnt[2][0] != 2 || cnt[2][1] != 2)
+    return DPD_OK;
+  bool mixed = false;
+  for (int t = 0; t < 3; t++)
+    for (int s2 = 0; s2 < 2; s2++)
+      if (cls(L[t][2 * s2]) != cls(L[t][2 * s2 + 1])) mixed = true;
+  if (!mixed) return DPD_OK;
+  // the ordered pair of class k: taken from a tensor that already holds it as a pair
+  auto pair_of = [&](int k, const int* pref, int npref, char* out) {
+    for (int pi = 0; pi < npref; pi++) {
+      const int t = pref[pi];
+      for (int s2 = 0; s2 < 2; s2++)
+        if (cls(L[t][2 * s2]) == k && cls(L[t][2 * s2 + 1]) == k) {
+          out[0] = L[t][2 * s2];
+          out[1] = L[t][2 * s2 + 1];
+          return;
+        }
+    }
+    int n = 0;
+    for (int i = 0; i < 4 && n < 2; i++)
+      if (cls(L[pref[0]][i]) == k) out[n++] = L[pref[0]][i];
+  };
+  char pm[2], pn[2], pk[2];
+  const int prefM[2] = {0, 2}, prefN[2] = {1, 2}, prefK[2] = {1, 0};
+  pair_of(0, prefM, 2, pm);
+  pair_of(1, prefN, 2, pn);
+  pair_of(2, prefK, 2, pk);
+  // target layouts; an operand already in one of its target forms stays as is
+  auto build = [&](int t, const char* x, const char* y, char* lab, Desc& dx, bool& moved, MapArgs& m,
+                   int64_t& len) {
+    const char* cur = L[t];
+    const bool xy = cur[0] == x[0] && cur[1] == x[1] && cur[2] == y[0] && cur[3] == y[1];
+    const bool yx = cur[0] == y[0] && cur[1] == y[1] && cur[2] == x[0] && cur[3] == x[1];
+    if ((xy || yx) && D[t]->iop[0] == 1 && D[t]->iop[1] == 1) {
+      std::memcpy(lab, cur, 4);
+      dx = *D[t];
+      moved = false;
+      return;
+    }
+    const char want[4] = {x[0], x[1], y[0], y[1]};
+    int perm[4];
+    dx = *D[t];
+    for (int i = 0; i < 4; i++) {
+      perm[i] = find(t, want[i]);
+      std::memcpy(dx.pop[i], D[t]->pop[perm[i]], sizeof dx.pop[i]);
+      lab[i] = want[i];
+    }
+    dx.iop[0] = dx.iop[1] = 1;
+    m = perm_map(S, dx, *D[t], perm);
+    len = m.out.len;
+    moved = true;
+  };
+  build(0, pm, pk, P.lab1x, P.d1x, P.exp1, P.map1, P.len1x);
+  build(1, pk, pn, P.lab2x, P.d2x, P.exp2, P.map2, P.len2x);
+  // result: v3 itself when it already is (M M | N N) full and needs no antisymmetrisation
+  const char want3[4] = {pm[0], pm[1], pn[0], pn[1]};
+  const bool same3 = std::memcmp(o3.lab, want3, 4) == 0 && o3.d.iop[0] == 1 && o3.d.iop[1] == 1;
+  if (same3 && !apq && !ars) {
+    P.epi_on = false;
+    std::memcpy(P.lab3x, o3.lab, 4);
+    P.d3x = o3.d;
+    return DPD_OK;
+  }
+  Desc dx = o3.d;
+  int pos[4];
+  for (int i = 0; i < 4; i++) {
+    const int src = find(2, want3[i]);
+    std::memcpy(dx.pop[i], o3.d.pop[src], sizeof dx.pop[i]);
+    P.lab3x[i] = want3[i];
+  }
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      if (want3[j] == o3.lab[i]) pos[i] = j;
+  dx.iop[0] = dx.iop[1] = 1;
+  P.d3x = dx;
+  P.epi = anti_map_perm(S, o3.d, dx, pos, apq, ars);
+  P.len3x = P.epi.in.len;
+  P.epi_on = true;
+  return DPD_OK;
 }
 
 static int build_contract_plan(const Sym& S, int kind, const Operand& o1, const Operand& o2,
@@ -1011,20 +1163,31 @@ static int build_contract_plan(const Sym& S, int kind, const Operand& o1, const 
     return fail(DPD_EINVAL, "ERROR: vv2v_%d anti 1st pair error.", kind);
   if (ars && (P.d3x.iop[1] != 1 || !pops_eq(o3.d.pop[2], o3.d.pop[3])))
     return fail(DPD_EINVAL, "ERROR: vv2v_%d anti 2nd pair error.", kind);
+  std::memcpy(P.lab1x, o1.lab, 4);
+  std::memcpy(P.lab2x, o2.lab, 4);
+  std::memcpy(P.lab3x, o3.lab, 4);
+  P.epi_on = anti != 0;
   if (anti != 0) {
     P.epi = anti_map(S, o3.d, P.d3x, apq, ars);
     P.len3x = P.epi.in.len;
+  }
+  if (kind == 444) {
+    rc = regroup_444(S, o1, o2, o3, apq, ars, P);
+    if (rc) return rc;
   }
   // batch strides of the scratch copies
   P.batch1 = o1.batched;
   P.batch2 = o2.batched;
   P.bstride1 = P.exp1 ? P.len1x : o1.bstride;
   P.bstride2 = P.exp2 ? P.len2x : o2.bstride;
-  P.bstride3 = anti != 0 ? P.len3x : o3.bstride;
+  P.bstride3 = P.epi_on ? P.len3x : o3.bstride;
   Operand e1 = o1, e2 = o2, e3 = o3;
   e1.d = P.d1x;
   e2.d = P.d2x;
   e3.d = P.d3x;
+  std::memcpy(e1.lab, P.lab1x, 4);
+  std::memcpy(e2.lab, P.lab2x, 4);
+  std::memcpy(e3.lab, P.lab3x, 4);
   e1.bstride = P.bstride1;
   e2.bstride = P.bstride2;
   e3.bstride = P.bstride3;
@@ -1067,7 +1230,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   }
   if (P->exp1) need += sizeof(double) * P->len1x * nb1 + 256;
   if (P->exp2) need += sizeof(double) * P->len2x * nb2 + 256;
-  if (anti) need += sizeof(double) * P->len3x * nbatch + 256;
+  if (P->epi_on) need += sizeof(double) * P->len3x * nbatch + 256;
   if (!packed) need += sizeof(double) * P->gemm.partial_elems + 256;
   ws_reset(c);
   int rc = ws_reserve(c, need);
@@ -1100,11 +1263,15 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
     p2 = x;
     p2f = nullptr;
   }
+  // scratch result: every element the epilogue reads lies in a block some GEMM
+  // task writes (tasks without K segments write zeros), so it is written with
+  // beta = 0 and needs no clearing
   double* p3 = const_cast<double*>(o3.data);
-  if (anti) {
+  double beta = 1.0;
+  if (P->epi_on) {
     p3 = (double*)ws_take(c, sizeof(double) * P->len3x * nbatch);
     if (!p3) return DPD_ENOMEM;
-    DPD_HIP_TRY(launch_fill(P->len3x * nbatch, 0.0, p3, c->stream));
+    beta = 0.0;
   }
   int use_f32 = (prec != PREC_F64);
   const void* A = (use_f32 && p1f) ? (const void*)p1f : (const void*)p1;
@@ -1119,14 +1286,14 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
     if (!packed) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
   }
   if (packed)
-    rc = run_gemm_packed(c, P->gemm, prec, p1, id1, p2, id2, p3, fact, 1.0, pr ? pr->a : nullptr,
+    rc = run_gemm_packed(c, P->gemm, prec, p1, id1, p2, id2, p3, fact, beta, pr ? pr->a : nullptr,
                          pr ? pr->b : nullptr);
   else
-    rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, 1.0,
+    rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, beta,
                   use_f32 ? PREC_F32 : PREC_F64);
   if (rc) return rc;
   if (pr && !packed) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
-  if (anti) {
+  if (P->epi_on) {
     MapArgs m = P->epi;
     m.nbatch = nbatch;
     m.in_bstride = P->len3x;

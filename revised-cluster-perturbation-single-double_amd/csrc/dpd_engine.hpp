@@ -103,6 +103,9 @@ struct ContractPlan {
   Desc d3x;         // descriptor of the scratch result (anti != 0)
   MapArgs map1, map2;  // expansion gathers
   MapArgs epi;         // antisymmetrise + compress + accumulate into v3
+  bool epi_on = false;  // the GEMM writes a scratch result that epi maps into v3
+  // labels of the GEMM operands (differ from the caller's when regrouped)
+  char lab1x[4], lab2x[4], lab3x[4];
   GemmPlan gemm;
   int64_t len1x = 0, len2x = 0, len3x = 0;
   int nbatch = 1;

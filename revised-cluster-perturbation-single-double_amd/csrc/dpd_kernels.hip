@@ -689,7 +689,7 @@ hipError_t launch_pair_dots(int64_t n, const double* x, int64_t xs, const double
                             double* partial, double* out, hipStream_t s) {
   if (nv == 0) return hipSuccess;
   hipLaunchKernelGGL(k_pair_dots, dim3(kRedBlocks, nv), dim3(256), 0, s, n, x, xs, y, ys, partial);
-  hipLaunchKernelGGL(k_reduce_rows, dim3(nv), dim3(256), 0, s, partial, kRedBlocks, out);
+  if (out) hipLaunchKernelGGL(k_reduce_rows, dim3(nv), dim3(256), 0, s, partial, kRedBlocks, out);
   return hipGetLastError();
 }
 
@@ -791,6 +791,57 @@ __global__ __launch_bounds__(256) void k_mgs_step(int64_t n, double* __restrict_
 hipError_t launch_mgs_step(int64_t n, double* p, const double* bprev, const double* bnext, const double* part_in,
                            int mode, double* part_out, hipStream_t s) {
   hipLaunchKernelGGL(k_mgs_step, dim3(kRedBlocks), dim3(256), 0, s, n, p, bprev, bnext, part_in, mode, part_out);
+  return hipGetLastError();
+}
+
+// the same step for nv <= kMgsMulti vectors p_j = P + j*n against one basis
+// vector: every block reduces all nv partial rows, updates its chunk of every
+// p_j and writes the partials of p_j . b_next (or p_j . p_j)
+__global__ __launch_bounds__(256) void k_mgs_multi(int64_t n, double* __restrict__ P, int nv,
+                                                   const double* __restrict__ bprev,
+                                                   const double* __restrict__ bnext, const double* part_in,
+                                                   int mode, double* part_out) {
+  __shared__ double red[4];
+  double f[kMgsMulti];
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++) {
+    f[j] = 0.0;
+    if (j < nv) {
+      const double c = reduce_parts(part_in + (int64_t)j * kRedBlocks, red);
+      f[j] = (mode == 0) ? 1.0 / sqrt(c) : c;
+    }
+  }
+  double s[kMgsMulti];
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++) s[j] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double bp = mode == 0 ? 0.0 : bprev[i];
+    const double bn = bnext ? bnext[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kMgsMulti; j++) {
+      if (j < nv) {
+        double x = P[(int64_t)j * n + i];
+        if (mode == 0) x = x * f[j];
+        else x = x + (-f[j]) * bp;
+        P[(int64_t)j * n + i] = x;
+        s[j] += x * (bnext ? bn : x);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++) {
+    if (j < nv) {
+      const double v = block_sum_256(s[j], red);
+      if (threadIdx.x == 0) part_out[(int64_t)j * kRedBlocks + blockIdx.x] = v;
+    }
+  }
+}
+
+hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, const double* bnext,
+                            const double* part_in, int mode, double* part_out, hipStream_t s) {
+  if (nv <= 0 || nv > kMgsMulti) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mgs_multi, dim3(kRedBlocks), dim3(256), 0, s, n, P, nv, bprev, bnext, part_in, mode,
+                     part_out);
   return hipGetLastError();
 }
 

@@ -151,7 +151,8 @@ hipError_t launch_div(int64_t n, const double* num, const double* den, double* o
 
 namespace dpd {
 // ---- batched Davidson kernels --------------------------------------------
-// out[v] = dot(x + v*xs, y + v*ys) for v < nv   (partial: kRedBlocks*nv)
+// out[v] = dot(x + v*xs, y + v*ys) for v < nv   (partial: kRedBlocks*nv; out may be
+// null: partials only)
 hipError_t launch_pair_dots(int64_t n, const double* x, int64_t xs, const double* y, int64_t ys, int nv,
                             double* partial, double* out, hipStream_t s);
 // out[xi + yj*nx] = X_xi . Y_yj  (partial: kRedBlocks*nx*ny)
@@ -170,6 +171,10 @@ hipError_t launch_resid_amax(int64_t n, const double* aq, const double* q, const
 //   then part_out[block] = dot(p, b_next) (b_next == nullptr: dot(p, p))
 hipError_t launch_mgs_step(int64_t n, double* p, const double* b_prev, const double* b_next, const double* part_in,
                            int mode, double* part_out, hipStream_t s);
+// the same step for nv <= kMgsMulti vectors P + j*n at once (partials: kRedBlocks per vector)
+constexpr int kMgsMulti = 8;
+hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, const double* bnext,
+                            const double* part_in, int mode, double* part_out, hipStream_t s);
 // dst = p * (1/sqrt(sum(part))) ; returns nothing (norm read by the host separately)
 hipError_t launch_scale_by_norm(int64_t n, const double* p, const double* part, double* dst, hipStream_t s);
 // p = e / clamp(w - d) for the roots listed in idx (w on device)

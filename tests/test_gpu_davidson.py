@@ -137,3 +137,19 @@ def test_davidson_collapse_needs_room(torch):
         OS.davidson(op.sigma, op.diag(), op.guess(4)[0], 4, conv=1e-9, max_dim=8, max_cyc=60)
     with pytest.raises(dpdhip._lib.DpdError, match="nroot \\+ 5"):
         gp.davidson(4, conv=1e-9, max_dim=8, max_cyc=60)
+
+
+def test_block_davidson_16_roots(torch):
+    """block Davidson with 16 roots (the 16 lowest, davidson_dim 40): all new
+    directions of an iteration go through one batched sigma; iteration count
+    and roots as in the oracle"""
+    cfg = dict(nso=(8, 2, 3, 5), nalpha=5, nbeta=3, seed=11)  # dim 2674
+    sp, op, gp = make(cfg, "sf-cpsd", 1)
+    nroot = 16
+    guesses, _ = op.guess(nroot)
+    ref = OS.davidson(op.sigma, op.diag(), guesses, nroot, conv=1e-7, max_dim=40, max_cyc=150, lowest=True)
+    w, R, info = gp.davidson(nroot, conv=1e-7, max_dim=40, max_cyc=150, lowest=True)
+    assert ref["converged"] and info["converged"]
+    assert info["iterations"] == ref["iterations"]
+    assert np.max(np.abs(w - ref["w"])) <= 1e-8
+    assert max(info["final_residual"]) < 1e-6
