@@ -410,7 +410,10 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
     ws_reset(c);
     rc = ws_reserve(c, pk_esize(prec) * (V.pa_elems + V.pb_elems) + sizeof(double) * V.partial_elems + 1024);
     if (rc) return rc;
-    rc = run_gemm_packed(c, p, prec, A, 0, B, 0, C, alpha, beta);
+    PackSrc a, b;
+    a.data = A;
+    b.data = B;
+    rc = run_gemm_packed(c, p, prec, a, b, C, alpha, beta);
   } else {
     rc = run_gemm(c, p, A, 0, B, 0, C, alpha, beta, prec);
   }
@@ -422,7 +425,7 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
 
 int dpd_ctx_set_gemm_config(dpd_ctx* c, int cfg, int packing) {
   CHECK_CTX(c);
-  if (cfg < -1 || cfg > GEMM_128x128) return fail(DPD_EINVAL, "GEMM configuration must be -1..4");
+  if (cfg < -1 || cfg > kGemmCfgMax) return fail(DPD_EINVAL, "GEMM configuration must be -1..%d", kGemmCfgMax);
   if (packing < 0 || packing > 2) return fail(DPD_EINVAL, "packing must be 0, 1 or 2");
   if (c->device >= 0) DPD_HIP_TRY(hipStreamSynchronize(c->stream));
   c->gemm_cfg = cfg;

@@ -191,9 +191,15 @@ __host__ __device__ inline void decode_pair(const int32_t* popp, const int32_t* 
   *hq = h;
   *hp = a;
   if (iop == 1 || a < h) {
-    int64_t np = popp[a];
-    *p = l2 % np;
-    *q = l2 / np;
+    const int64_t np = popp[a];
+    if (l2 < 0x7fffffff) {  // 32-bit division where it fits
+      const int32_t l32 = (int32_t)l2, n32 = (int32_t)np;
+      *p = l32 % n32;
+      *q = l32 / n32;
+    } else {
+      *p = l2 % np;
+      *q = l2 / np;
+    }
   } else if (iop == 0) {
     int64_t qq = (int64_t)((1.0 + sqrt(1.0 + 8.0 * (double)l2)) * 0.5);
     while (qq * (qq - 1) / 2 > l2) qq--;

@@ -327,8 +327,9 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
   const size_t nd2 = (size_t)dimAbar * dimAbar;
   if ((rc = W.B.alloc(vb * dimAbar)) || (rc = W.AB.alloc(vb * dimAbar)) || (rc = W.NB.alloc(vb * dimAbar)) ||
       (rc = W.NAB.alloc(vb * dimAbar)) || (rc = W.AQ.alloc(vb * nroot)) || (rc = W.E.alloc(vb * nroot)) ||
-      (rc = W.P.alloc(vb * nroot)) || (rc = W.coef.alloc(sizeof(double) * nd2)) ||
-      (rc = W.part.alloc(sizeof(double) * kRedBlocks * nd2)) ||
+      (rc = W.P.alloc(vb * nroot)) ||
+      (rc = W.coef.alloc(sizeof(double) * std::max<size_t>(nd2, (size_t)kMgsMulti * dimAbar))) ||
+      (rc = W.part.alloc(sizeof(double) * kRedBlocks * std::max<size_t>(nd2, (size_t)kMgsMulti * dimAbar))) ||
       (rc = W.partA.alloc(sizeof(double) * kRedBlocks * kMgsMulti)) ||
       (rc = W.partB.alloc(sizeof(double) * kRedBlocks * kMgsMulti)) ||
       (rc = W.partC.alloc(sizeof(double) * kRedBlocks)) || (rc = W.partD.alloc(sizeof(double) * kRedBlocks)) || (rc = W.dots.alloc(sizeof(double) * 2 * nd2)) ||
@@ -487,13 +488,16 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
       DPD_HIP_TRY(hipMemcpyAsync(W.idx.p, cand.data(), sizeof(int32_t) * cand.size(), hipMemcpyHostToDevice, st));
       DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p, (int)cand.size(), diag, E, PV, st));
     }
-    // SchmidtS (:687-726) root by root: normalise, modified Gram-Schmidt against
+    // SchmidtS (:687-726) root by root: normalise, remove the components along
     // every basis vector (the new ones of this iteration included), normalise;
-    // skipped when the remaining norm is below 1e-6.  Each projection step is one
-    // launch that finishes the previous reduction, updates p and starts the next.
-    // The roots are orthogonalised back to back assuming none is skipped; the
-    // norms are checked once at the end and, in the rare case that one fell
-    // below 1e-6, the roots from there on are redone one at a time.
+    // skipped when the remaining norm is below 1e-6.  Against the old basis the
+    // directions are projected together with block operations (classical
+    // Gram-Schmidt applied twice); against this iteration's new vectors one
+    // modified Gram-Schmidt step per vector (one launch that finishes the
+    // previous reduction, updates p and starts the next).  The roots are
+    // orthogonalised back to back assuming none is skipped; the norms are
+    // checked once at the end and, in the rare case that one fell below 1e-6,
+    // the roots from there on are redone one at a time (sequential MGS).
     auto schmidt = [&](size_t k, int slot, double* nrm_dev) -> int {
       double* p = PV + k * n;
       double* pin = (double*)W.partA.p;
@@ -520,13 +524,16 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
         double* P0 = PV + c0 * n;
         double* pin = (double*)W.partA.p;
         double* pout = (double*)W.partB.p;
+        // normalise, then project on the old (orthonormal) basis twice with
+        // block operations (classical Gram-Schmidt with one re-orthogonalisation:
+        // in exact arithmetic the same vector as the sequential projections)
         DPD_HIP_TRY(launch_pair_dots(n, P0, n, P0, n, nv, pin, nullptr, st));
-        for (int i = 0; i <= dimIter; i++) {
-          const double* bprev = i > 0 ? bv(i - 1) : nullptr;
-          const double* bnext = i < dimIter ? bv(i) : nullptr;
-          DPD_HIP_TRY(launch_mgs_multi(n, P0, nv, bprev, bnext, pin, i == 0 ? 0 : 1, pout, st));
-          std::swap(pin, pout);
+        DPD_HIP_TRY(launch_mgs_multi(n, P0, nv, nullptr, nullptr, pin, 0, pout, st));
+        for (int pass = 0; pass < 2; pass++) {
+          DPD_HIP_TRY(launch_dots_multi(n, P0, nv, B, dimIter, part, (double*)W.coef.p, st));
+          DPD_HIP_TRY(launch_lincomb_sub(n, B, dimIter, (const double*)W.coef.p, nv, P0, st));
         }
+        DPD_HIP_TRY(launch_pair_dots(n, P0, n, P0, n, nv, pin, nullptr, st));
         for (int j = 0; j < nv; j++) {
           const size_t k = c0 + j;
           double* pk = PV + k * n;
@@ -596,15 +603,31 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
     // Abar update (:484-509): new rows against the old columns, then the new
     // columns against every basis vector
     if (nNewBase > 0) {
+      // every basis vector is read once per group of up to kMgsMulti new vectors
       const int nt = dimIter + nNewBase;
-      DPD_HIP_TRY(launch_block_dots(n, AB, n, dimIter, bv(dimIter), n, nNewBase, part, dots, st));
-      DPD_HIP_TRY(launch_block_dots(n, B, n, nt, abv(dimIter), n, nNewBase, part, dots + nd2, st));
-      DPD_HIP_TRY(hipMemcpyAsync(hd.data(), dots, sizeof(double) * 2 * nd2, hipMemcpyDeviceToHost, st));
+      size_t off = 0;
+      std::vector<size_t> off1, off2;
+      for (int c0 = 0; c0 < nNewBase; c0 += kMgsMulti) {
+        const int nv = std::min(kMgsMulti, nNewBase - c0);
+        DPD_HIP_TRY(launch_dots_multi(n, bv(dimIter + c0), nv, AB, dimIter, part, dots + off, st));
+        off1.push_back(off);
+        off += (size_t)nv * dimIter;
+      }
+      for (int c0 = 0; c0 < nNewBase; c0 += kMgsMulti) {
+        const int nv = std::min(kMgsMulti, nNewBase - c0);
+        DPD_HIP_TRY(launch_dots_multi(n, abv(dimIter + c0), nv, B, nt, part, dots + off, st));
+        off2.push_back(off);
+        off += (size_t)nv * nt;
+      }
+      DPD_HIP_TRY(hipMemcpyAsync(hd.data(), dots, sizeof(double) * off, hipMemcpyDeviceToHost, st));
       if ((rc = dsync(c))) return rc;
-      for (int j = 0; j < dimIter; j++)
-        for (int i = 0; i < nNewBase; i++) A_(dimIter + i, j) = hd[j + (size_t)i * dimIter];
-      for (int j = 0; j < nNewBase; j++)
-        for (int i = 0; i < nt; i++) A_(i, dimIter + j) = hd[nd2 + i + (size_t)j * nt];
+      for (int c0 = 0, ci = 0; c0 < nNewBase; c0 += kMgsMulti, ci++) {
+        const int nv = std::min(kMgsMulti, nNewBase - c0);
+        for (int j = 0; j < dimIter; j++)
+          for (int i = 0; i < nv; i++) A_(dimIter + c0 + i, j) = hd[off1[ci] + i + (size_t)j * nv];
+        for (int j = 0; j < nv; j++)
+          for (int i = 0; i < nt; i++) A_(i, dimIter + c0 + j) = hd[off2[ci] + j + (size_t)i * nv];
+      }
     }
     dimIter += nNewBase;
     iterations++;

@@ -558,14 +558,14 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
 }
 
 int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested) {
-  if (requested >= GEMM_V1 && requested <= GEMM_128x128) return requested;
+  if (requested >= GEMM_V1 && requested <= kGemmCfgMax) return requested;
   // cost ~ padded MFMA work / relative tile efficiency, plus a penalty when the
   // tile count cannot fill 256 CUs (split-K recovers part of that)
-  static const int cand[4] = {GEMM_128x128, GEMM_128x64, GEMM_64x128, GEMM_64x64};
-  static const double eff[4] = {1.0, 0.85, 0.85, 0.7};
+  static const int cand[5] = {GEMM_128x128, GEMM_128x64, GEMM_64x128, GEMM_64x64, GEMM_32x32};
+  static const double eff[5] = {1.0, 0.85, 0.85, 0.7, 0.4};
   int best = GEMM_64x64;
   double best_cost = 0.0;
-  for (int ci = 0; ci < 4; ci++) {
+  for (int ci = 0; ci < 5; ci++) {
     const int bm = gemm_bm(cand[ci]), bn = gemm_bn(cand[ci]);
     double padded = 0.0, ntile = 0.0;
     for (const auto& tk : tasks) {
@@ -651,9 +651,25 @@ int build_variant(GemmPlan& p, int mode) {
 }
 
 // pack one operand side of a variant into dst
-static int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const double* src, void* dst) {
+static int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const PackSrc& src, void* dst) {
   PackArgs a;
-  a.src = src;
+  std::memset(&a, 0, sizeof a);
+  a.src = src.data;
+  // read order: the source's unit stride decides (rows or k's fastest)
+  if (!V.tasks.empty()) {
+    const GemmSeg& s0 = p.segs[V.tasks[0].seg_begin];
+    a.rfast = side == 0 ? (s0.sa_m[0] == 1) : (s0.sb_n[0] == 1);
+  }
+  if (src.map) {
+    a.use_map = 1;
+    a.rowmask = src.rowmask;
+    a.map = *src.map;
+    // the unsorted tensor's dim 0 is its fastest: is it a row or a k index?
+    for (int d = 0; d < 4; d++)
+      if (src.map->term[0].perm[d] == 0) a.rfast = (src.rowmask >> d) & 1;
+    a.xlen = src.xlen;
+    a.src_bstride = src.bstride;
+  }
   a.dst = dst;
   a.tasks = V.d_tasks;
   a.segs = p.d_segs;
@@ -668,36 +684,35 @@ static int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const do
 
 // packed-panel GEMM: static operands come from the variant's cache (packed on
 // first use), the others are packed into the workspace reserved by the caller
-int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const double* A, uint64_t a_id, const double* B,
-                    uint64_t b_id, double* C, double f1, double f2, hipEvent_t ev0, hipEvent_t ev1) {
+int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double* C, double f1,
+                    double f2, hipEvent_t ev0, hipEvent_t ev1) {
   GemmVariant& V = p.pk[mode];
   const size_t es = pk_esize(mode);
   const void* side_ptr[2];
-  const double* src[2] = {A, B};
-  const uint64_t ids[2] = {a_id, b_id};
+  const PackSrc* src[2] = {&A, &B};
   const int64_t elems[2] = {V.pa_elems, V.pb_elems};
   for (int sd = 0; sd < 2; sd++) {
     if (elems[sd] == 0) {
       side_ptr[sd] = nullptr;
       continue;
     }
-    if (ids[sd]) {
+    if (src[sd]->id) {
       PackCache& pc = V.cache[sd];
-      if (pc.src != src[sd] || pc.id != ids[sd] || !pc.buf.p) {
+      if (pc.src != src[sd]->data || pc.id != src[sd]->id || !pc.buf.p) {
         if (!pc.buf.p) {
           int rc = pc.buf.alloc(es * elems[sd]);
           if (rc) return rc;
         }
-        int rc = pack_side(c, p, V, sd, src[sd], pc.buf.p);
+        int rc = pack_side(c, p, V, sd, *src[sd], pc.buf.p);
         if (rc) return rc;
-        pc.src = src[sd];
-        pc.id = ids[sd];
+        pc.src = src[sd]->data;
+        pc.id = src[sd]->id;
       }
       side_ptr[sd] = pc.buf.p;
     } else {
       void* w = ws_take(c, es * elems[sd]);
       if (!w) return DPD_ENOMEM;
-      int rc = pack_side(c, p, V, sd, src[sd], w);
+      int rc = pack_side(c, p, V, sd, *src[sd], w);
       if (rc) return rc;
       side_ptr[sd] = w;
     }
@@ -910,7 +925,10 @@ int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, cons
     const GemmVariant& V = p->pk[PREC_F64];
     rc = ws_reserve(c, pk_esize(PREC_F64) * (V.pa_elems + V.pb_elems) + sizeof(double) * V.partial_elems + 1024);
     if (rc) return rc;
-    return run_gemm_packed(c, *p, PREC_F64, o1.data, 0, o2.data, 0, const_cast<double*>(o3.data), f1, f2);
+    PackSrc a, b;
+    a.data = o1.data;
+    b.data = o2.data;
+    return run_gemm_packed(c, *p, PREC_F64, a, b, const_cast<double*>(o3.data), f1, f2);
   }
   rc = ws_reserve(c, sizeof(double) * (p->partial_elems + 32));
   if (rc) return rc;
@@ -1215,10 +1233,19 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   if (flops) *flops += P->gemm.flops;
   const int nb1 = o1.batched ? nbatch : 1, nb2 = o2.batched ? nbatch : 1;
   // packed panels: always for BF16x3, otherwise when an operand is static
-  const bool packed = prec == PREC_BF16X3 || c->packing == 2 ||
-                      (c->packing == 1 && ((o1.static_id && !P->exp1) || (o2.static_id && !P->exp2)));
+  const bool packed = prec == PREC_BF16X3 || c->packing == 2 || (c->packing == 1 && (o1.static_id || o2.static_id));
   if (prec == PREC_BF16X3 && c->packing == 0) return fail(DPD_EINVAL, "BF16x3 needs packed panels (packing 0)");
-  const uint64_t id1 = P->exp1 ? 0 : o1.static_id, id2 = P->exp2 ? 0 : o2.static_id;
+  // packed panels read sorted constant operands straight from the unsorted
+  // tensors (packed once); sorted per-call operands are expanded first unless
+  // fuse_sorts is set
+  const uint64_t id1 = o1.static_id, id2 = o2.static_id;
+  static const int fuse_env = [] {
+    const char* e = getenv("DPD_FUSE_SORTS");
+    return e ? atoi(e) : 0;
+  }();
+  const bool fuse1 = packed && P->exp1 && (id1 || fuse_env);
+  const bool fuse2 = packed && P->exp2 && (id2 || fuse_env);
+  const bool gx1 = P->exp1 && !fuse1, gx2 = P->exp2 && !fuse2;
   size_t need = 0;
   if (packed) {
     int rc0 = build_variant(P->gemm, prec);
@@ -1228,8 +1255,8 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
     if (!id2) need += pk_esize(prec) * V.pb_elems + 256;
     need += sizeof(double) * V.partial_elems + 256;
   }
-  if (P->exp1) need += sizeof(double) * P->len1x * nb1 + 256;
-  if (P->exp2) need += sizeof(double) * P->len2x * nb2 + 256;
+  if (gx1) need += sizeof(double) * P->len1x * nb1 + 256;
+  if (gx2) need += sizeof(double) * P->len2x * nb2 + 256;
   if (P->epi_on) need += sizeof(double) * P->len3x * nbatch + 256;
   if (!packed) need += sizeof(double) * P->gemm.partial_elems + 256;
   ws_reset(c);
@@ -1239,7 +1266,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   const double* p2 = o2.data;
   const float* p1f = o1.data_f32;
   const float* p2f = o2.data_f32;
-  if (P->exp1) {
+  if (gx1) {
     double* x = (double*)ws_take(c, sizeof(double) * P->len1x * nb1);
     if (!x) return DPD_ENOMEM;
     MapArgs m = P->map1;
@@ -1251,7 +1278,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
     p1 = x;
     p1f = nullptr;
   }
-  if (P->exp2) {
+  if (gx2) {
     double* x = (double*)ws_take(c, sizeof(double) * P->len2x * nb2);
     if (!x) return DPD_ENOMEM;
     MapArgs m = P->map2;
@@ -1285,9 +1312,33 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
     // the packed path records the events around the GEMM kernel itself
     if (!packed) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
   }
-  if (packed)
-    rc = run_gemm_packed(c, P->gemm, prec, p1, id1, p2, id2, p3, fact, beta, pr ? pr->a : nullptr,
-                         pr ? pr->b : nullptr);
+  if (packed) {
+    PackSrc a, b;
+    a.data = fuse1 ? o1.data : p1;
+    a.id = id1;
+    // dims of a sorted operand that are GEMM rows: the ones it shares with the result
+    auto rows_of = [&](const char* lab) {
+      int m = 0;
+      for (int d = 0; d < 4; d++)
+        if (std::memchr(P->lab3x, lab[d], 4)) m |= 1 << d;
+      return m;
+    };
+    if (fuse1) {
+      a.map = &P->map1;
+      a.rowmask = rows_of(P->lab1x);
+      a.xlen = P->len1x;
+      a.bstride = o1.bstride;
+    }
+    b.data = fuse2 ? o2.data : p2;
+    b.id = id2;
+    if (fuse2) {
+      b.map = &P->map2;
+      b.rowmask = rows_of(P->lab2x);
+      b.xlen = P->len2x;
+      b.bstride = o2.bstride;
+    }
+    rc = run_gemm_packed(c, P->gemm, prec, a, b, p3, fact, beta, pr ? pr->a : nullptr, pr ? pr->b : nullptr);
+  }
   else
     rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, beta,
                   use_f32 ? PREC_F32 : PREC_F64);

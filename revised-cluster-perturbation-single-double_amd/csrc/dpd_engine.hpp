@@ -168,11 +168,20 @@ int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, i
              double f1, double f2, int prec);
 int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested);
 int build_variant(GemmPlan& p, int mode);
-// packed-panel GEMM; operands with a nonzero id are packed once and cached,
-// the others are packed into workspace the caller reserved
-int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const double* A, uint64_t a_id, const double* B,
-                    uint64_t b_id, double* C, double f1, double f2, hipEvent_t ev0 = nullptr,
-                    hipEvent_t ev1 = nullptr);
+// one operand of the packed-panel GEMM: its data, a nonzero id when the contents
+// never change (packed once and cached), and optionally the sort that turns it
+// into the GEMM operand (fused into the pack: `data` is then the unsorted tensor,
+// xlen the sorted length per batch member, bstride the unsorted batch stride)
+struct PackSrc {
+  const double* data = nullptr;
+  uint64_t id = 0;
+  const MapArgs* map = nullptr;
+  int rowmask = 0;  // dims of the sorted operand that are GEMM rows (M or N)
+  int64_t xlen = 0, bstride = 0;
+};
+// packed-panel GEMM; uncached operands are packed into workspace the caller reserved
+int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double* C, double f1,
+                    double f2, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out);
 // labels / checks shared by the C-ABI and the sigma builders
 int check_labels(const char* t, int n);

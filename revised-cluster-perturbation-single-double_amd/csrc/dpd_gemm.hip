@@ -286,6 +286,9 @@ static hipError_t launch_cfg(const GemmLaunch& g, hipStream_t s) {
     case GEMM_128x128:
       hipLaunchKernelGGL((k_gemm2<CT, TA, TB, 128, 128>), dim3(g.nwg), dim3(256), 0, s, g);
       break;
+    case GEMM_32x32:
+      hipLaunchKernelGGL((k_gemm2<CT, TA, TB, 32, 32>), dim3(g.nwg), dim3(256), 0, s, g);
+      break;
     default:
       return hipErrorInvalidValue;
   }

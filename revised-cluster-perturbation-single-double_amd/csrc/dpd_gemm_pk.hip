@@ -63,11 +63,23 @@ __device__ inline int find_task(const GemmTask* tasks, int ntask, int key, int w
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// pack: one block per panel
+// pack: one block per panel.  The block first tabulates, per K granule of the
+// panel (16 k's share a segment), the offset of each of its rows and of each of
+// its k's (one mixed-radix decode per row and per k instead of one per element);
+// every element is then one add, or, for a fused sort, one fetch from the
+// unsorted tensor with the row's and the k's indices.
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
   constexpr int KS = MODE == PREC_F64 ? 16 : 32;
+  constexpr int NG = KS / 16;    // granules per panel
+  constexpr int MAXT = 128;      // largest tile
+  __shared__ int64_t rowoff[NG][MAXT];  // row part of the address (-1: padding row)
+  __shared__ int64_t koff[KS];          // k part incl. the segment base (-1: padding k)
+  __shared__ int8_t rh[NG][MAXT][4], kh[KS][4];
+  __shared__ int32_t rx[NG][MAXT][4], kx[KS][4], rb[NG][MAXT];
+  __shared__ int sgi[NG];
+  __shared__ double tv[KS][MAXT + 1];  // the panel, [k][row]
   const int pid = blockIdx.x;
   const int ti = find_task(p.tasks, p.ntask, pid, 1 + p.side);
   const GemmTask& T = p.tasks[ti];
@@ -77,7 +89,107 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
   const int r0 = tile * p.tile;
   const int64_t pan_elems = (int64_t)p.tile * KS;
   const int64_t base = (p.side == 0 ? T.pa_off : T.pb_off) + (int64_t)local * pan_elems * (MODE == PREC_BF16X3 ? 2 : 1);
-  int sg = T.seg_begin;
+  const int32_t* rd = p.side == 0 ? T.md : T.nd;
+  if (threadIdx.x < NG) {
+    const int gran = step * NG + threadIdx.x;
+    int sg = -1;
+    if (gran < T.ktiles_total) {
+      sg = T.seg_begin;
+      while (gran >= p.segs[sg].ktile_begin + p.segs[sg].ktiles) sg++;
+    }
+    sgi[threadIdx.x] = sg;
+  }
+  __syncthreads();
+  // rows (one thread per granule and row)
+  for (int t = threadIdx.x; t < NG * p.tile; t += kThreads) {
+    const int g = t / p.tile, r = t % p.tile;
+    const int sg = sgi[g];
+    int64_t ro = -1;
+    if (sg >= 0 && r0 + r < R) {
+      const GemmSeg& S = p.segs[sg];
+      ro = digit_off(r0 + r, rd, p.side == 0 ? S.sa_m : S.sb_n);
+      if (p.use_map) {
+        // indices of the row: decode the element (row, first k of the granule)
+        const int gran = step * NG + g;
+        const int k0 = (gran - S.ktile_begin) * 16;
+        const int64_t ad = (p.side == 0 ? S.a_off : S.b_off) + ro +
+                           digit_off(k0, S.kd, p.side == 0 ? S.sa_k : S.sb_k);
+        const int64_t b = ad / p.xlen;
+        int h[4];
+        int64_t x[4];
+        decode_elem(p.map.out, ad - b * p.xlen, h, x);
+        for (int d = 0; d < 4; d++) {
+          rh[g][r][d] = (int8_t)h[d];
+          rx[g][r][d] = (int32_t)x[d];
+        }
+        rb[g][r] = (int32_t)b;
+      }
+    }
+    rowoff[g][r] = ro;
+  }
+  // k's (one thread per k of the panel)
+  if (threadIdx.x < KS) {
+    const int k = threadIdx.x;
+    const int g = k >> 4;
+    const int sg = sgi[g];
+    int64_t ko = -1;
+    if (sg >= 0) {
+      const GemmSeg& S = p.segs[sg];
+      const int gran = step * NG + g;
+      const int ks = (gran - S.ktile_begin) * 16 + (k & 15);
+      if (ks < S.K) {
+        ko = (p.side == 0 ? S.a_off : S.b_off) + digit_off(ks, S.kd, p.side == 0 ? S.sa_k : S.sb_k);
+        if (p.use_map) {
+          // indices of the k: decode the element (first row of the task, k)
+          const int64_t ad = ko + digit_off(0, rd, p.side == 0 ? S.sa_m : S.sb_n);
+          const int64_t b = ad / p.xlen;
+          int h[4];
+          int64_t x[4];
+          decode_elem(p.map.out, ad - b * p.xlen, h, x);
+          for (int d = 0; d < 4; d++) {
+            kh[k][d] = (int8_t)h[d];
+            kx[k][d] = (int32_t)x[d];
+          }
+        }
+      }
+    }
+    koff[k] = ko;
+  }
+  __syncthreads();
+  // read the panel in the order in which the source is contiguous (rows or
+  // k's fastest) into LDS, then write it in panel order
+  for (int idx = threadIdx.x; idx < pan_elems; idx += kThreads) {
+    int r, k;
+    if (p.rfast) {
+      r = idx % p.tile;
+      k = idx / p.tile;
+    } else {
+      k = idx % KS;
+      r = idx / KS;
+    }
+    const int g = k >> 4;
+    double v = 0.0;
+    const int64_t ro = rowoff[g][r], ko = koff[k];
+    if (ro >= 0 && ko >= 0) {
+      if (!p.use_map) {
+        v = p.src[ro + ko];
+      } else {
+        int hi[4];
+        int64_t xi[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+          const bool row = (p.rowmask >> d) & 1;
+          hi[p.map.term[0].perm[d]] = row ? rh[g][r][d] : kh[k][d];
+          xi[p.map.term[0].perm[d]] = row ? rx[g][r][d] : kx[k][d];
+        }
+        double sg;
+        const int64_t ia = fetch_addr(p.map.in, hi[0], hi[1], hi[2], hi[3], xi[0], xi[1], xi[2], xi[3], &sg);
+        if (ia >= 0) v = p.map.alpha * (p.map.term[0].coef * sg * p.src[(int64_t)rb[g][r] * p.src_bstride + ia]);
+      }
+    }
+    tv[k][r] = v;
+  }
+  __syncthreads();
   for (int e = threadIdx.x; e < pan_elems; e += kThreads) {
     int r, k;
     if (MODE == PREC_BF16X3) {
@@ -87,23 +199,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
       k = e / p.tile;
       r = e % p.tile;
     }
-    const int kg = step * KS + k;
-    const int gran = kg >> 4;
-    double v = 0.0;
-    if (r0 + r < R && gran < T.ktiles_total) {
-      // segments are visited in increasing order by each thread's elements
-      // only within one panel row; restart the scan when the granule moved back
-      if (gran < p.segs[sg].ktile_begin) sg = T.seg_begin;
-      while (gran >= p.segs[sg].ktile_begin + p.segs[sg].ktiles) sg++;
-      const GemmSeg& S = p.segs[sg];
-      const int ks = (gran - S.ktile_begin) * 16 + (kg & 15);
-      if (ks < S.K) {
-        int64_t ad;
-        if (p.side == 0) ad = S.a_off + digit_off(r0 + r, T.md, S.sa_m) + digit_off(ks, S.kd, S.sa_k);
-        else ad = S.b_off + digit_off(ks, S.kd, S.sb_k) + digit_off(r0 + r, T.nd, S.sb_n);
-        v = p.src[ad];
-      }
-    }
+    const double v = tv[k][r];
     if (MODE == PREC_F64) {
       ((double*)p.dst)[base + e] = v;
     } else if (MODE == PREC_F32) {
@@ -352,6 +448,9 @@ static hipError_t launch_pk_mode(const GemmLaunch& g, const void* PA, const void
       break;
     case GEMM_128x128:
       hipLaunchKernelGGL((k_gemm_pk<MODE, 128, 128>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
+      break;
+    case GEMM_32x32:
+      hipLaunchKernelGGL((k_gemm_pk<MODE, 32, 32>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
       break;
     default:
       return hipErrorInvalidValue;

@@ -53,12 +53,78 @@ __global__ __launch_bounds__(256) void k_gather_map(MapArgs a, const double* __r
   }
 }
 
+// Column-tiled variant: every block owns a chunk of kGatherChunk consecutive
+// elements of one column (one right pair of one irrep block) of the output, so
+// the right pair is decoded once per block and every element only decodes its
+// left pair (no search over blocks, no 64-bit division).
+constexpr int kGatherChunk = 1024;
+
+__device__ inline int64_t gather_chunks(const DevLayout& L, int xr) {
+  const int xl = L.dp[xr][L.sym];
+  return L.nR[xr] * ((L.nL[xl] + kGatherChunk - 1) / kGatherChunk);
+}
+
+__global__ __launch_bounds__(256) void k_gather_cols(MapArgs a, const double* __restrict__ in,
+                                                     double* __restrict__ out) {
+  const DevLayout& L = a.out;
+  int64_t b = blockIdx.x;
+  int xr = 0;
+  for (; xr < L.nirrep; xr++) {
+    const int64_t c = gather_chunks(L, xr);
+    if (b < c) break;
+    b -= c;
+  }
+  if (xr >= L.nirrep) return;
+  const int xl = L.dp[xr][L.sym];
+  const int64_t nl = L.nL[xl];
+  const int64_t per = (nl + kGatherChunk - 1) / kGatherChunk;
+  const int64_t r = b / per, l0 = (b % per) * kGatherChunk;
+  int h[4];
+  int64_t x[4];
+  decode_pair(L.pop[2], L.pop[3], L.iop[1], L.dp, L.offR[xr], L.nirrep, xr, r, &h[2], &x[2], &h[3], &x[3]);
+  const int bt = blockIdx.y;
+  const double* inb = in + (int64_t)bt * a.in_bstride;
+  double* outb = out + (int64_t)bt * a.out_bstride;
+  const int64_t col = L.blk[xr] + r * nl;
+  const int64_t l1 = l0 + kGatherChunk < nl ? l0 + kGatherChunk : nl;
+  for (int64_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+    decode_pair(L.pop[0], L.pop[1], L.iop[0], L.dp, L.offL[xl], L.nirrep, xl, l, &h[0], &x[0], &h[1], &x[1]);
+    double acc = 0.0;
+    for (int t = 0; t < a.nterm; t++) {
+      int hi[4];
+      int64_t xi[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        hi[a.term[t].perm[i]] = h[i];
+        xi[a.term[t].perm[i]] = x[i];
+      }
+      double sg;
+      const int64_t ad = fetch_addr(a.in, hi[0], hi[1], hi[2], hi[3], xi[0], xi[1], xi[2], xi[3], &sg);
+      if (ad >= 0) acc += a.term[t].coef * sg * inb[ad];
+    }
+    const int64_t o = col + l;
+    double v = a.alpha * acc;
+    if (a.beta != 0.0) v = a.beta * outb[o] + v;
+    outb[o] = v;
+  }
+}
+
 hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hipStream_t s) {
   if (a.out.len == 0 || a.nbatch == 0) return hipSuccess;
-  int64_t blocks = (a.out.len + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_gather_map, dim3((unsigned)blocks, (unsigned)a.nbatch), dim3(256), 0, s, a,
-                     in, out);
+  // one block per (column, chunk): the same count the kernel derives
+  int64_t blocks = 0;
+  for (int xr = 0; xr < a.out.nirrep; xr++) {
+    const int xl = a.out.dp[xr][a.out.sym];
+    blocks += a.out.nR[xr] * ((a.out.nL[xl] + kGatherChunk - 1) / kGatherChunk);
+  }
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > 0x7fffffff) {
+    int64_t b2 = (a.out.len + 255) / 256;
+    if (b2 > 8192) b2 = 8192;
+    hipLaunchKernelGGL(k_gather_map, dim3((unsigned)b2, (unsigned)a.nbatch), dim3(256), 0, s, a, in, out);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)blocks, (unsigned)a.nbatch), dim3(256), 0, s, a, in, out);
   return hipGetLastError();
 }
 
@@ -842,6 +908,77 @@ hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, c
   if (nv <= 0 || nv > kMgsMulti) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_mgs_multi, dim3(kRedBlocks), dim3(256), 0, s, n, P, nv, bprev, bnext, part_in, mode,
                      part_out);
+  return hipGetLastError();
+}
+
+// C[j + k*nv] = P_j . B_k for nv <= kMgsMulti vectors P (stride n) and nb basis
+// vectors B (stride n); grid (kRedBlocks, ceil(nb/4)); each basis vector is read
+// once, partials reduced in a fixed order
+__global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __restrict__ P, int nv,
+                                                    const double* __restrict__ B, int nb, double* partial) {
+  __shared__ double red[4];
+  const int k0 = blockIdx.y * 4;
+  const int nk = min(4, nb - k0);
+  double acc[kMgsMulti][4];
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[j][q] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double b[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) b[q] = q < nk ? B[(int64_t)(k0 + q) * n + i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kMgsMulti; j++) {
+      if (j < nv) {
+        const double x = P[(int64_t)j * n + i];
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[j][q] += x * b[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++)
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (j < nv && q < nk) {
+        const double v = block_sum_256(acc[j][q], red);
+        if (threadIdx.x == 0) partial[((int64_t)j + (int64_t)(k0 + q) * nv) * kRedBlocks + blockIdx.x] = v;
+      }
+}
+
+hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
+                             double* out, hipStream_t s) {
+  if (nv <= 0 || nb <= 0) return hipSuccess;
+  if (nv > kMgsMulti) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_dots_multi, dim3(kRedBlocks, (nb + 3) / 4), dim3(256), 0, s, n, P, nv, B, nb, partial);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nv * nb), dim3(256), 0, s, partial, kRedBlocks, out);
+  return hipGetLastError();
+}
+
+// P_j -= sum_k C[j + k*nv] B_k for j < nv <= kMgsMulti (k ascending)
+__global__ void k_lincomb_sub(int64_t n, const double* __restrict__ B, int nb, const double* __restrict__ C, int nv,
+                              double* __restrict__ P) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s[kMgsMulti];
+#pragma unroll
+    for (int j = 0; j < kMgsMulti; j++) s[j] = 0.0;
+    for (int k = 0; k < nb; k++) {
+      const double b = B[(int64_t)k * n + i];
+#pragma unroll
+      for (int j = 0; j < kMgsMulti; j++)
+        if (j < nv) s[j] += C[j + (int64_t)k * nv] * b;
+    }
+#pragma unroll
+    for (int j = 0; j < kMgsMulti; j++)
+      if (j < nv) P[(int64_t)j * n + i] -= s[j];
+  }
+}
+
+hipError_t launch_lincomb_sub(int64_t n, const double* B, int nb, const double* C, int nv, double* P, hipStream_t s) {
+  if (nv <= 0 || nb <= 0) return hipSuccess;
+  if (nv > kMgsMulti) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_lincomb_sub, dim3(grid_for(n)), dim3(256), 0, s, n, B, nb, C, nv, P);
   return hipGetLastError();
 }
 

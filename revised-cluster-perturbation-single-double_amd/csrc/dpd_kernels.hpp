@@ -63,9 +63,14 @@ struct GemmTask {
 enum Prec { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2 };
 // tile configurations: GEMM_V1 is the first-generation 64x64 kernel, the others
 // the pipelined kernel of dpd_gemm.hip with BM x BN tiles
-enum GemmCfg { GEMM_V1 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_64x128 = 3, GEMM_128x128 = 4 };
-inline int gemm_bm(int cfg) { return (cfg == GEMM_128x64 || cfg == GEMM_128x128) ? 128 : 64; }
-inline int gemm_bn(int cfg) { return (cfg == GEMM_64x128 || cfg == GEMM_128x128) ? 128 : 64; }
+enum GemmCfg { GEMM_V1 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_64x128 = 3, GEMM_128x128 = 4, GEMM_32x32 = 5 };
+constexpr int kGemmCfgMax = GEMM_32x32;
+inline int gemm_bm(int cfg) {
+  return (cfg == GEMM_128x64 || cfg == GEMM_128x128) ? 128 : cfg == GEMM_32x32 ? 32 : 64;
+}
+inline int gemm_bn(int cfg) {
+  return (cfg == GEMM_64x128 || cfg == GEMM_128x128) ? 128 : cfg == GEMM_32x32 ? 32 : 64;
+}
 
 struct GemmLaunch {
   const void* A;  // storage of operand 1 (double, or float when a_f32)
@@ -101,6 +106,13 @@ struct PackArgs {
   int side;  // 0: A (rows = M), 1: B (rows = N)
   int mode;  // Prec
   int tile;  // BM or BN
+  // optional fused sort: the GEMM operand is map.out (an expanded/permuted copy
+  // of map.in, xlen elements per batch member); the pack reads map.in directly
+  int use_map;
+  int rowmask;  // bit d: dim d of map.out is a row (M or N) index, else a K index
+  int rfast;    // read the source with rows (1) or k's (0) varying fastest
+  int64_t xlen, src_bstride;
+  MapArgs map;
 };
 hipError_t launch_pack(const PackArgs& p, hipStream_t s);
 // K step of a packed panel and its element count factor (BF16x3 keeps two planes)
@@ -175,6 +187,11 @@ hipError_t launch_mgs_step(int64_t n, double* p, const double* b_prev, const dou
 constexpr int kMgsMulti = 8;
 hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, const double* bnext,
                             const double* part_in, int mode, double* part_out, hipStream_t s);
+// out[j + k*nv] = P_j . B_k (partial: kRedBlocks * nv * nb), nv <= kMgsMulti
+hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
+                             double* out, hipStream_t s);
+// P_j -= sum_k C[j + k*nv] B_k, nv <= kMgsMulti
+hipError_t launch_lincomb_sub(int64_t n, const double* B, int nb, const double* C, int nv, double* P, hipStream_t s);
 // dst = p * (1/sqrt(sum(part))) ; returns nothing (norm read by the host separately)
 hipError_t launch_scale_by_norm(int64_t n, const double* p, const double* part, double* dst, hipStream_t s);
 // p = e / clamp(w - d) for the roots listed in idx (w on device)

@@ -43,9 +43,10 @@ def ctx_for(n):
 # GEMM kernels (tile configuration, packing): automatic; first-generation
 # kernel; gathering kernel with 64x64 / 128x64 / 64x128 / 128x128 tiles; the
 # packed-panel kernel with automatic and with each fixed tile shape
-GEMM_CFGS = [(-1, 1), (0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (-1, 2), (1, 2), (2, 2), (3, 2), (4, 2)]
-GEMM_IDS = ["auto", "v1", "g64x64", "g128x64", "g64x128", "g128x128", "pk_auto", "pk64x64", "pk128x64",
-            "pk64x128", "pk128x128"]
+GEMM_CFGS = [(-1, 1), (0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (-1, 2), (1, 2), (2, 2), (3, 2), (4, 2),
+             (5, 2)]
+GEMM_IDS = ["auto", "v1", "g64x64", "g128x64", "g64x128", "g128x128", "g32x32", "pk_auto", "pk64x64", "pk128x64",
+            "pk64x128", "pk128x128", "pk32x32"]
 
 
 @pytest.fixture(params=GEMM_CFGS, ids=GEMM_IDS)
