@@ -71,6 +71,12 @@ __device__ inline int find_task(const GemmTask* tasks, int ntask, int key, int w
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
+  // the sort's layouts are indexed by per-lane irreps: keep them in LDS
+  __shared__ MapArgs smap;
+  if (p.use_map) {
+    for (int i = threadIdx.x; i < (int)(sizeof(MapArgs) / 4); i += blockDim.x)
+      ((int32_t*)&smap)[i] = ((const int32_t*)&p.map)[i];
+  }
   constexpr int KS = MODE == PREC_F64 ? 16 : 32;
   constexpr int NG = KS / 16;    // granules per panel
   constexpr int MAXT = 128;      // largest tile
@@ -117,7 +123,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
         const int64_t b = ad / p.xlen;
         int h[4];
         int64_t x[4];
-        decode_elem(p.map.out, ad - b * p.xlen, h, x);
+        decode_elem(smap.out, ad - b * p.xlen, h, x);
         for (int d = 0; d < 4; d++) {
           rh[g][r][d] = (int8_t)h[d];
           rx[g][r][d] = (int32_t)x[d];
@@ -145,7 +151,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
           const int64_t b = ad / p.xlen;
           int h[4];
           int64_t x[4];
-          decode_elem(p.map.out, ad - b * p.xlen, h, x);
+          decode_elem(smap.out, ad - b * p.xlen, h, x);
           for (int d = 0; d < 4; d++) {
             kh[k][d] = (int8_t)h[d];
             kx[k][d] = (int32_t)x[d];
@@ -179,12 +185,12 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
 #pragma unroll
         for (int d = 0; d < 4; d++) {
           const bool row = (p.rowmask >> d) & 1;
-          hi[p.map.term[0].perm[d]] = row ? rh[g][r][d] : kh[k][d];
-          xi[p.map.term[0].perm[d]] = row ? rx[g][r][d] : kx[k][d];
+          hi[smap.term[0].perm[d]] = row ? rh[g][r][d] : kh[k][d];
+          xi[smap.term[0].perm[d]] = row ? rx[g][r][d] : kx[k][d];
         }
         double sg;
-        const int64_t ia = fetch_addr(p.map.in, hi[0], hi[1], hi[2], hi[3], xi[0], xi[1], xi[2], xi[3], &sg);
-        if (ia >= 0) v = p.map.alpha * (p.map.term[0].coef * sg * p.src[(int64_t)rb[g][r] * p.src_bstride + ia]);
+        const int64_t ia = fetch_addr(smap.in, hi[0], hi[1], hi[2], hi[3], xi[0], xi[1], xi[2], xi[3], &sg);
+        if (ia >= 0) v = smap.alpha * (smap.term[0].coef * sg * p.src[(int64_t)rb[g][r] * p.src_bstride + ia]);
       }
     }
     tv[k][r] = v;

@@ -23,8 +23,12 @@ namespace dpd {
 // ===========================================================================
 // gather-map
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_gather_map(MapArgs a, const double* __restrict__ in,
+__device__ inline void stage_map(const MapArgs& a, MapArgs& sa);
+
+__global__ __launch_bounds__(256) void k_gather_map(MapArgs a_arg, const double* __restrict__ in,
                                                     double* __restrict__ out) {
+  __shared__ MapArgs a;
+  stage_map(a_arg, a);
   const int64_t n = a.out.len;
   const int b = blockIdx.y;
   const double* inb = in + (int64_t)b * a.in_bstride;
@@ -64,8 +68,19 @@ __device__ inline int64_t gather_chunks(const DevLayout& L, int xr) {
   return L.nR[xr] * ((L.nL[xl] + kGatherChunk - 1) / kGatherChunk);
 }
 
-__global__ __launch_bounds__(256) void k_gather_cols(MapArgs a, const double* __restrict__ in,
+// the layouts are indexed by per-lane irreps: staged in LDS so that those
+// lookups are LDS reads rather than dependent loads from the argument segment
+__device__ inline void stage_map(const MapArgs& a, MapArgs& sa) {
+  static_assert(sizeof(MapArgs) % 4 == 0, "MapArgs size");
+  for (int i = threadIdx.x; i < (int)(sizeof(MapArgs) / 4); i += blockDim.x)
+    ((int32_t*)&sa)[i] = ((const int32_t*)&a)[i];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const double* __restrict__ in,
                                                      double* __restrict__ out) {
+  __shared__ MapArgs a;
+  stage_map(a_arg, a);
   const DevLayout& L = a.out;
   int64_t b = blockIdx.x;
   int xr = 0;
