@@ -65,8 +65,11 @@ int dpd_ctx_destroy(dpd_ctx *ctx);
 /* hipStream_t as void*; NULL = the default stream */
 int dpd_ctx_set_stream(dpd_ctx *ctx, void *stream);
 int dpd_ctx_synchronize(dpd_ctx *ctx);
-/* GEMM kernels of every later plan.  cfg: -1 automatic (default), 0 the
- * first-generation 64x64 kernel, 1..5 64x64, 128x64, 64x128, 128x128, 32x32 tiles.
+/* GEMM kernels of every later plan.  cfg: -1 automatic (default: tile shape of
+ * the packed-panel kernel picked by timing every shape on a plan's first call),
+ * -2 automatic by a padding cost model without timing, 0 the first-generation
+ * 64x64 kernel, 1..6 64x64, 128x64, 64x128, 128x128, 32x32, 64x192 tiles (64x192
+ * packed-panel kernel only).
  * packing: 0 operands gathered inside the GEMM, 1 (default) packed panels when
  * an operand is a problem's constant tensor (packed once and kept), 2 packed
  * panels always (BF16x3 always uses panels).  Clears the plan caches.  No

@@ -425,7 +425,7 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
 
 int dpd_ctx_set_gemm_config(dpd_ctx* c, int cfg, int packing) {
   CHECK_CTX(c);
-  if (cfg < -1 || cfg > kGemmCfgMax) return fail(DPD_EINVAL, "GEMM configuration must be -1..%d", kGemmCfgMax);
+  if (cfg < -2 || cfg > kGemmCfgMax) return fail(DPD_EINVAL, "GEMM configuration must be -2..%d", kGemmCfgMax);
   if (packing < 0 || packing > 2) return fail(DPD_EINVAL, "packing must be 0, 1 or 2");
   if (c->device >= 0) DPD_HIP_TRY(hipStreamSynchronize(c->stream));
   c->gemm_cfg = cfg;

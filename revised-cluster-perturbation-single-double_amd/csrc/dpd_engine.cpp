@@ -492,6 +492,7 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
   // pipelined kernel handles every task shape, the first-generation kernel is
   // kept selectable (dpd_ctx_set_gemm_config) for comparison
   plan.cfg = choose_gemm_cfg(plan.tasks, plan.cfg_req);
+  if (plan.cfg == GEMM_64x192) plan.cfg = GEMM_64x128;  // the gathering kernel has no 192-wide tile
   const int BM = gemm_bm(plan.cfg), BN = gemm_bn(plan.cfg);
   for (auto& tk : plan.tasks) {
     tk.tiles_m = (tk.M + BM - 1) / BM;
@@ -583,14 +584,11 @@ int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested) {
   return best;
 }
 
-int build_variant(GemmPlan& p, int mode) {
-  GemmVariant& V = p.pk[mode];
-  if (V.built) return DPD_OK;
+static int build_variant_cfg(GemmPlan& p, GemmVariant& V, int mode, int cfg) {
   V.mode = mode;
   V.tasks = p.tasks;
   V.split.clear();
-  V.cfg = choose_gemm_cfg(p.tasks, p.cfg_req >= GEMM_64x64 ? p.cfg_req : -1);
-  if (V.cfg == GEMM_V1) V.cfg = GEMM_64x64;
+  V.cfg = cfg;
   const int BM = gemm_bm(V.cfg), BN = gemm_bn(V.cfg), KS = pk_kstep(mode), PL = pk_planes(mode);
   for (auto& tk : V.tasks) {
     tk.steps = (int32_t)(((int64_t)tk.ktiles_total * 16 + KS - 1) / KS);
@@ -648,6 +646,25 @@ int build_variant(GemmPlan& p, int mode) {
   if (bp) DPD_HIP_TRY(hipMemcpy(V.d_split, V.split.data(), bp, hipMemcpyHostToDevice));
   V.built = true;
   return DPD_OK;
+}
+
+int build_variant(GemmPlan& p, int mode) {
+  GemmVariant& V = p.pk[mode];
+  if (V.built) return DPD_OK;
+  int cfg = choose_gemm_cfg(p.tasks, p.cfg_req >= GEMM_64x64 ? p.cfg_req : -1);
+  if (cfg == GEMM_V1) cfg = GEMM_64x64;
+  return build_variant_cfg(p, V, mode, cfg);
+}
+
+// one past the largest element of C a plan can write
+static int64_t c_span(const GemmPlan& p) {
+  auto span = [](int64_t n, const int32_t* d, const int64_t* s) {
+    int64_t d2 = n / ((int64_t)d[0] * d[1]);
+    return (d[0] - 1) * s[0] + (d[1] - 1) * s[1] + (d2 - 1) * s[2];
+  };
+  int64_t m = 0;
+  for (const auto& tk : p.tasks) m = std::max(m, tk.c_off + span(tk.M, tk.md, tk.sc_m) + span(tk.N, tk.nd, tk.sc_n) + 1);
+  return m;
 }
 
 // pack one operand side of a variant into dst
@@ -741,6 +758,71 @@ int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const P
   DPD_HIP_TRY(launch_gemm_pk(g, side_ptr[0], side_ptr[1], c->stream));
   if (ev1) DPD_HIP_TRY(hipEventRecord(ev1, c->stream));
   return DPD_OK;
+}
+
+// Autotuner: times every tile shape of the packed-panel kernel on the call's
+// own operands (packed into temporary buffers, results into a scratch C) and
+// keeps the fastest for this plan and precision.  Runs once per plan.
+int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double f1, double f2) {
+  static const int cand[] = {GEMM_32x32, GEMM_64x64, GEMM_128x64, GEMM_64x128, GEMM_128x128, GEMM_64x192};
+  const size_t es = pk_esize(mode);
+  DevBuf cs;
+  int rc = cs.alloc(sizeof(double) * std::max<int64_t>(c_span(p), 1));
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  DPD_HIP_TRY(hipEventCreate(&e0));
+  DPD_HIP_TRY(hipEventCreate(&e1));
+  int best = -1;
+  float best_ms = 0.0f;
+  for (int cfg : cand) {
+    GemmVariant T;
+    if ((rc = build_variant_cfg(p, T, mode, cfg))) break;
+    DevBuf ta, tb, tp;
+    if ((T.pa_elems && (rc = ta.alloc(es * T.pa_elems))) || (T.pb_elems && (rc = tb.alloc(es * T.pb_elems))) ||
+        (T.partial_elems && (rc = tp.alloc(sizeof(double) * T.partial_elems))))
+      break;
+    if ((T.pa_elems && (rc = pack_side(c, p, T, 0, A, ta.p))) || (T.pb_elems && (rc = pack_side(c, p, T, 1, B, tb.p))))
+      break;
+    GemmLaunch g;
+    std::memset(&g, 0, sizeof g);
+    g.C = (double*)cs.p;
+    g.tasks = T.d_tasks;
+    g.segs = p.d_segs;
+    g.ntask = (int)T.tasks.size();
+    g.nwg = T.nwg;
+    g.alpha = f1;
+    g.beta = f2;
+    g.prec = mode;
+    g.split_tasks = T.d_split;
+    g.nsplit = (int)T.split.size();
+    g.max_split_mn = T.max_split_mn;
+    g.cfg = cfg;
+    g.partial = (double*)tp.p;
+    DPD_HIP_TRY(launch_gemm_pk(g, ta.p, tb.p, c->stream));
+    DPD_HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < 3; r++) DPD_HIP_TRY(launch_gemm_pk(g, ta.p, tb.p, c->stream));
+    DPD_HIP_TRY(hipEventRecord(e1, c->stream));
+    DPD_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    DPD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (best < 0 || ms < best_ms) {
+      best = cfg;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (best < 0) best = GEMM_64x64;
+  GemmVariant& V = p.pk[mode];
+  if (V.built && V.cfg == best) return DPD_OK;
+  // panels cached for the old tile shape are rebuilt on the next call
+  for (auto& pc : V.cache) {
+    pc.src = nullptr;
+    pc.id = 0;
+    pc.buf.reset();
+  }
+  return build_variant_cfg(p, V, mode, best);
 }
 
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
@@ -1338,6 +1420,12 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
       b.bstride = o2.bstride;
     }
     rc = run_gemm_packed(c, P->gemm, prec, a, b, p3, fact, beta, pr ? pr->a : nullptr, pr ? pr->b : nullptr);
+    // first call of this plan in this precision: time every tile shape on these
+    // operands (results go to a scratch C) and keep the fastest for later calls
+    if (!rc && c->gemm_cfg == -1 && !P->gemm.pk[prec].tuned) {
+      rc = tune_variant(c, P->gemm, prec, a, b, fact, beta);
+      P->gemm.pk[prec].tuned = true;
+    }
   }
   else
     rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, beta,

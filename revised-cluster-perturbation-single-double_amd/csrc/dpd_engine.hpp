@@ -47,6 +47,7 @@ struct PackCache {
 // the tiling of a plan for the packed-panel kernel in one precision
 struct GemmVariant {
   bool built = false;
+  bool tuned = false;  // tile shape chosen by timing (autotuner)
   int mode = 0, cfg = 0;
   std::vector<GemmTask> tasks;
   std::vector<int32_t> split;
@@ -168,6 +169,8 @@ int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, i
              double f1, double f2, int prec);
 int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested);
 int build_variant(GemmPlan& p, int mode);
+struct PackSrc;
+int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double f1, double f2);
 // one operand of the packed-panel GEMM: its data, a nonzero id when the contents
 // never change (packed once and cached), and optionally the sort that turns it
 // into the GEMM operand (fused into the pack: `data` is then the unsorted tensor,

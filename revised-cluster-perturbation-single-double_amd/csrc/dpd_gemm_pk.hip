@@ -69,7 +69,7 @@ __device__ inline int find_task(const GemmTask* tasks, int ntask, int key, int w
 // every element is then one add, or, for a fused sort, one fetch from the
 // unsorted tensor with the row's and the k's indices.
 // ---------------------------------------------------------------------------
-template <int MODE>
+template <int MODE, int MAXT>
 __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
   // the sort's layouts are indexed by per-lane irreps: keep them in LDS
   __shared__ MapArgs smap;
@@ -79,7 +79,6 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
   }
   constexpr int KS = MODE == PREC_F64 ? 16 : 32;
   constexpr int NG = KS / 16;    // granules per panel
-  constexpr int MAXT = 128;      // largest tile
   __shared__ int64_t rowoff[NG][MAXT];  // row part of the address (-1: padding row)
   __shared__ int64_t koff[KS];          // k part incl. the segment base (-1: padding k)
   __shared__ int8_t rh[NG][MAXT][4], kh[KS][4];
@@ -220,22 +219,28 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
   }
 }
 
+template <int MODE>
+static hipError_t launch_pack_mode(const PackArgs& p, hipStream_t s) {
+  if (p.tile <= 32) hipLaunchKernelGGL((k_pack<MODE, 32>), dim3(p.npanel), dim3(256), 0, s, p);
+  else if (p.tile <= 64) hipLaunchKernelGGL((k_pack<MODE, 64>), dim3(p.npanel), dim3(256), 0, s, p);
+  else if (p.tile <= 128) hipLaunchKernelGGL((k_pack<MODE, 128>), dim3(p.npanel), dim3(256), 0, s, p);
+  else if (p.tile <= 192) hipLaunchKernelGGL((k_pack<MODE, 192>), dim3(p.npanel), dim3(256), 0, s, p);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 hipError_t launch_pack(const PackArgs& p, hipStream_t s) {
   if (p.npanel == 0) return hipSuccess;
   switch (p.mode) {
     case PREC_F64:
-      hipLaunchKernelGGL(k_pack<PREC_F64>, dim3(p.npanel), dim3(256), 0, s, p);
-      break;
+      return launch_pack_mode<PREC_F64>(p, s);
     case PREC_F32:
-      hipLaunchKernelGGL(k_pack<PREC_F32>, dim3(p.npanel), dim3(256), 0, s, p);
-      break;
+      return launch_pack_mode<PREC_F32>(p, s);
     case PREC_BF16X3:
-      hipLaunchKernelGGL(k_pack<PREC_BF16X3>, dim3(p.npanel), dim3(256), 0, s, p);
-      break;
+      return launch_pack_mode<PREC_BF16X3>(p, s);
     default:
       return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -457,6 +462,9 @@ static hipError_t launch_pk_mode(const GemmLaunch& g, const void* PA, const void
       break;
     case GEMM_32x32:
       hipLaunchKernelGGL((k_gemm_pk<MODE, 32, 32>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
+      break;
+    case GEMM_64x192:
+      hipLaunchKernelGGL((k_gemm_pk<MODE, 64, 192>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
       break;
     default:
       return hipErrorInvalidValue;

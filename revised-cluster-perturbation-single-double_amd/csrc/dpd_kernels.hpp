@@ -63,13 +63,26 @@ struct GemmTask {
 enum Prec { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2 };
 // tile configurations: GEMM_V1 is the first-generation 64x64 kernel, the others
 // the pipelined kernel of dpd_gemm.hip with BM x BN tiles
-enum GemmCfg { GEMM_V1 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_64x128 = 3, GEMM_128x128 = 4, GEMM_32x32 = 5 };
-constexpr int kGemmCfgMax = GEMM_32x32;
+// (GEMM_64x192, the wide-N tile that covers a whole narrow N in one tile, exists
+// for the packed-panel kernel only)
+enum GemmCfg {
+  GEMM_V1 = 0,
+  GEMM_64x64 = 1,
+  GEMM_128x64 = 2,
+  GEMM_64x128 = 3,
+  GEMM_128x128 = 4,
+  GEMM_32x32 = 5,
+  GEMM_64x192 = 6
+};
+constexpr int kGemmCfgMax = GEMM_64x192;
 inline int gemm_bm(int cfg) {
   return (cfg == GEMM_128x64 || cfg == GEMM_128x128) ? 128 : cfg == GEMM_32x32 ? 32 : 64;
 }
 inline int gemm_bn(int cfg) {
-  return (cfg == GEMM_64x128 || cfg == GEMM_128x128) ? 128 : cfg == GEMM_32x32 ? 32 : 64;
+  return (cfg == GEMM_64x128 || cfg == GEMM_128x128) ? 128
+         : cfg == GEMM_32x32                         ? 32
+         : cfg == GEMM_64x192                        ? 192
+                                                     : 64;
 }
 
 struct GemmLaunch {

@@ -74,10 +74,11 @@ class Context:
             pass
 
     def set_gemm_config(self, cfg=-1, packing=1):
-        """GEMM kernels of later plans.  cfg: -1 automatic, 0 first-generation
-        kernel, 1..4 tiles 64x64, 128x64, 64x128, 128x128.  packing: 0 operands
-        gathered inside the GEMM, 1 packed panels when an operand is a problem
-        constant (packed once), 2 packed panels always."""
+        """GEMM kernels of later plans.  cfg: -1 automatic (tile shape timed on a
+        plan's first call), -2 automatic by cost model, 0 first-generation kernel,
+        1..6 tiles 64x64, 128x64, 64x128, 128x128, 32x32, 64x192 (packed kernel).
+        packing: 0 operands gathered inside the GEMM, 1 packed panels when an
+        operand is a problem constant (packed once), 2 packed panels always."""
         check(lib().dpd_ctx_set_gemm_config(self.h, int(cfg), int(packing)))
 
     def profile(self, on=True, max_records=100000):

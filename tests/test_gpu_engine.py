@@ -40,13 +40,14 @@ def ctx_for(n):
     return _ctxs[n]
 
 
-# GEMM kernels (tile configuration, packing): automatic; first-generation
+# GEMM kernels (tile configuration, packing): automatic (timed) and automatic
+# (cost model); first-generation
 # kernel; gathering kernel with 64x64 / 128x64 / 64x128 / 128x128 tiles; the
 # packed-panel kernel with automatic and with each fixed tile shape
-GEMM_CFGS = [(-1, 1), (0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (-1, 2), (1, 2), (2, 2), (3, 2), (4, 2),
-             (5, 2)]
-GEMM_IDS = ["auto", "v1", "g64x64", "g128x64", "g64x128", "g128x128", "g32x32", "pk_auto", "pk64x64", "pk128x64",
-            "pk64x128", "pk128x128", "pk32x32"]
+GEMM_CFGS = [(-1, 1), (-2, 1), (0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (-1, 2), (1, 2), (2, 2), (3, 2),
+             (4, 2), (5, 2), (6, 2)]
+GEMM_IDS = ["auto", "model", "v1", "g64x64", "g128x64", "g64x128", "g128x128", "g32x32", "pk_auto", "pk64x64",
+            "pk128x64", "pk64x128", "pk128x128", "pk32x32", "pk64x192"]
 
 
 @pytest.fixture(params=GEMM_CFGS, ids=GEMM_IDS)
