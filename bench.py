@@ -38,6 +38,7 @@ WORKLOADS = {
 # MI355X dense MFMA peaks per ALGORITHMIC flop: FP64, FP32, and BF16x3 (three
 # BF16 MFMAs of the 2.5 PFLOP/s dense BF16 peak per algorithmic product)
 PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0 / 3}
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 
 
 def log(*a):
@@ -57,6 +58,10 @@ def main():
     ap.add_argument("--tracking", choices=["lowest", "overlap"], default="lowest")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", default=None, choices=["fp64", "fp32", "bf16x3"],
+                    help="counter-run mode: a few batched sigma calls in this precision (tile shape --tile), "
+                         "then a plain 512 MiB device copy for calibrating FETCH_SIZE/WRITE_SIZE; no JSON line")
+    ap.add_argument("--tile", type=int, default=-1, help="GEMM tile configuration for --probe (-1: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
@@ -70,6 +75,9 @@ def main():
 
     cfg = WORKLOADS[args.workload]
     nroot = cfg["nroot"]
+    if args.probe:
+        probe(cfg, args)
+        return
     lowest = args.tracking == "lowest"
     t0 = time.time()
     ctx = dpdhip.Context(len(cfg["nso"]), 0)
@@ -132,24 +140,37 @@ def main():
     t_mix = M["t"]
     infom, prof = M["info"], M["prof"]
 
-    # dominant GEMM kernel of the measured mixed run
+    # dominant GEMM kernel of the measured mixed run; it is MFMA-bound when its
+    # algorithmic intensity (flops per algorithmic byte) exceeds the ridge point
+    # peak_flops / 8 TB/s, else HBM-bound
     dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
     tag, st = dom
     avg_ms = st["ms"] / st["count"]
     flops_per_launch = st["flops_total"] / st["count"]
-    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS[st["prec"]]
+    bytes_per_launch = st.get("bytes_total", 0) / st["count"]
+    intensity = flops_per_launch / max(bytes_per_launch, 1.0)
+    ridge = PEAK_TFLOPS[st["prec"]] * 1e12 / (PEAK_HBM_GBS * 1e9)
+    if intensity >= ridge:
+        bound, unit, peak = "mfma", "TFLOP/s", PEAK_TFLOPS[st["prec"]]
+        achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    else:
+        bound, unit, peak = "hbm", "GB/s", PEAK_HBM_GBS
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(f"{args.workload}:{tag}")
+        ent = json.load(open(tpath)).get(f"{args.workload}:{tag.rsplit(':t', 1)[0]}:cfg{st.get('cfg')}")
+        traffic = ent["traffic_bytes"] if ent else None
 
     cpu = None
     if not args.no_cpu and args.workload != "cfg3":
         # (at C2v/180 one oracle sigma alone exceeds the bounded sample)
         cpu = cpu_baseline(cfg, args, infom, lowest)
 
-    kname = {0: "k_gemm_pk<FP64>", 1: "k_gemm_pk<FP32>", 2: "k_gemm_pk<BF16x3>"}[st["prec"]]
+    tiles = {1: (64, 64), 2: (128, 64), 3: (64, 128), 4: (128, 128), 5: (32, 32), 6: (64, 192)}
+    bm, bn = tiles.get(st.get("cfg", 1), (64, 64))
+    mode = {0: 0, 1: 1, 2: 2}[st["prec"]]
+    kname = f"k_gemm_pk<{mode}, {bm}, {bn}>" if st.get("packed", 1) else f"k_gemm2<{bm}, {bn}>"
     line = {
         "metric": f"Davidson time to {nroot} roots at <=1e-6 Eh: mixed-prec MFMA vs FP64 GPU"
         if nroot != 5 else "Davidson time to 5 roots at <=1e-6 Eh: mixed-prec MFMA vs FP64 GPU",
@@ -182,16 +203,42 @@ def main():
         "max_abs_dw_mixed_vs_fp64": M["dw"],
         "mixed_all": {k: {"s_per_solve": round(v["t"], 6), "iterations": v["info"]["iterations"],
                           "max_abs_dw": v["dw"]} for k, v in mixed.items()},
-        "roofline": {"bound": "mfma", "kernel": f"{kname} ({tag})", "achieved": round(achieved, 3),
-                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-                     "flops_per_launch": int(flops_per_launch), "avg_launch_ms": round(avg_ms, 5),
-                     "launches": st["count"]},
+        "roofline": {"bound": bound, "kernel": f"{kname} ({tag})", "tile_cfg": st.get("cfg"),
+                     "achieved": round(achieved, 3), "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
+                     "traffic": traffic, "flops_per_launch": int(flops_per_launch),
+                     "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                     "intensity_flop_per_byte": round(intensity, 2), "ridge_flop_per_byte": round(ridge, 2),
+                     "tflops": round(flops_per_launch / (avg_ms * 1e-3) / 1e12, 3),
+                     "avg_launch_ms": round(avg_ms, 5), "launches": st["count"]},
         "kernels": {k: {"launches": v["count"], "ms_total": round(v["ms"], 3),
                         "tflops": round(v["flops_total"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] > 0 else None}
                     for k, v in prof.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def probe(cfg, args):
+    """The program profiled by the rocprofv3 --pmc runs: batched sigma calls of
+    the benchmark problem (5 vectors) with a fixed GEMM tile shape, then a plain
+    copy of 2^26 doubles whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    import torch
+
+    import dpdhip
+    ctx = dpdhip.Context(len(cfg["nso"]), 0)
+    ctx.set_gemm_config(args.tile, 1)
+    prob = dpdhip.Problem(ctx, dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"]), "sf-cpsd", 1)
+    prob.set_precision(args.probe)
+    R = torch.randn(cfg["nroot"], prob.dim, device="cuda:0", dtype=torch.float64)
+    for _ in range(3):
+        prob.sigma(R)
+    torch.cuda.synchronize()
+    a = torch.empty(1 << 26, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("probe done")
 
 
 def cpu_baseline(cfg, args, info_gpu, lowest):

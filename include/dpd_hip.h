@@ -80,8 +80,10 @@ const char *dpd_last_error(void);
 size_t dpd_ctx_workspace_bytes(const dpd_ctx *ctx);
 /* Kernel timing (the reference's cpu_time prints, on the device): when enabled,
  * every tagged GEMM stage of the sigma builders is bracketed by HIP events on
- * the context stream (up to max_records launches).  The report aggregates per
- * tag {count, ms, flops_total, prec} as JSON and clears the records. */
+ * the context stream (up to max_records launches; packed-panel GEMMs: the GEMM
+ * kernel alone, without the packing).  The report aggregates per
+ * "tag:precision:t<tile shape>" {count, ms, flops_total, bytes_total, prec, cfg,
+ * packed} as JSON and clears the records. */
 int dpd_profile_enable(dpd_ctx *ctx, int on, int max_records);
 int dpd_profile_report(dpd_ctx *ctx, char *json, int64_t len);
 

@@ -463,12 +463,15 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
   DPD_HIP_TRY(hipStreamSynchronize(c->stream));
   std::vector<std::string> tags;
   std::vector<double> ms;
-  std::vector<int64_t> cnt, fl, fltot;
-  std::vector<int> pr;
+  std::vector<int64_t> cnt, fl, fltot, by;
+  std::vector<int> pr, cf, pk;
   for (auto& r : c->prof.recs) {
     float t = 0.f;
     DPD_HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
-    std::string key = r.tag + (r.prec == PREC_F64 ? ":fp64" : r.prec == PREC_F32 ? ":fp32" : ":bf16x3");
+    // one entry per tag, precision and tile shape (a tag's plans for different
+    // vector counts may be tuned to different tiles)
+    std::string key = r.tag + (r.prec == PREC_F64 ? ":fp64" : r.prec == PREC_F32 ? ":fp32" : ":bf16x3") + ":t" +
+                      std::to_string(r.cfg);
     size_t k = std::find(tags.begin(), tags.end(), key) - tags.begin();
     if (k == tags.size()) {
       tags.push_back(key);
@@ -476,19 +479,28 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
       cnt.push_back(0);
       fl.push_back(0);
       fltot.push_back(0);
+      by.push_back(0);
       pr.push_back(r.prec);
+      cf.push_back(r.cfg);
+      pk.push_back(r.packed);
     }
     ms[k] += t;
     cnt[k] += 1;
     fl[k] = r.flops;
     fltot[k] += r.flops;
+    by[k] += r.bytes;
     pr[k] = r.prec;
+    cf[k] = r.cfg;
+    pk[k] = r.packed;
   }
   std::string js = "{";
   for (size_t k = 0; k < tags.size(); k++) {
-    char item[256];
-    snprintf(item, sizeof item, "%s\"%s\": {\"count\": %lld, \"ms\": %.6f, \"flops_total\": %lld, \"prec\": %d}",
-             k ? ", " : "", tags[k].c_str(), (long long)cnt[k], ms[k], (long long)fltot[k], pr[k]);
+    char item[512];
+    snprintf(item, sizeof item,
+             "%s\"%s\": {\"count\": %lld, \"ms\": %.6f, \"flops_total\": %lld, \"bytes_total\": %lld, "
+             "\"prec\": %d, \"cfg\": %d, \"packed\": %d}",
+             k ? ", " : "", tags[k].c_str(), (long long)cnt[k], ms[k], (long long)fltot[k], (long long)by[k], pr[k],
+             cf[k], pk[k]);
     js += item;
   }
   js += "}";

@@ -399,6 +399,7 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
   plan.segs.clear();
   plan.split.clear();
   plan.flops = 0;
+  plan.c_elems = 0;
   for (int64_t e = 0; e < ecount; e++) {
     int64_t r = e;
     bool skip = false;
@@ -429,6 +430,7 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
     }
     if (M == 0 || N == 0) continue;
     if (M > INT32_MAX || N > INT32_MAX) return fail(DPD_EINVAL, "GEMM block too large");
+    plan.c_elems += M * N;
     tk.M = (int32_t)M;
     tk.N = (int32_t)N;
     tk.c_off = a3.base;
@@ -764,7 +766,10 @@ int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const P
 // own operands (packed into temporary buffers, results into a scratch C) and
 // keeps the fastest for this plan and precision.  Runs once per plan.
 int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double f1, double f2) {
-  static const int cand[] = {GEMM_32x32, GEMM_64x64, GEMM_128x64, GEMM_64x128, GEMM_128x128, GEMM_64x192};
+  // candidates in order of preference: among shapes within 5% of the fastest
+  // the first listed wins, which keeps the choice stable from run to run
+  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x128, GEMM_64x128, GEMM_64x64, GEMM_32x32};
+  float tms[sizeof cand / sizeof cand[0]];
   const size_t es = pk_esize(mode);
   DevBuf cs;
   int rc = cs.alloc(sizeof(double) * std::max<int64_t>(c_span(p), 1));
@@ -774,7 +779,8 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
   DPD_HIP_TRY(hipEventCreate(&e1));
   int best = -1;
   float best_ms = 0.0f;
-  for (int cfg : cand) {
+  for (auto& t : tms) t = 1e30f;
+  for (const int& cfg : cand) {
     GemmVariant T;
     if ((rc = build_variant_cfg(p, T, mode, cfg))) break;
     DevBuf ta, tb, tp;
@@ -805,10 +811,18 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
     DPD_HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.0f;
     DPD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    tms[&cfg - cand] = ms;
     if (best < 0 || ms < best_ms) {
       best = cfg;
       best_ms = ms;
     }
+  }
+  if (!rc && best >= 0) {
+    for (size_t i = 0; i < sizeof cand / sizeof cand[0]; i++)
+      if (tms[i] <= 1.05f * best_ms) {
+        best = cand[i];
+        break;
+      }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1318,15 +1332,11 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   const bool packed = prec == PREC_BF16X3 || c->packing == 2 || (c->packing == 1 && (o1.static_id || o2.static_id));
   if (prec == PREC_BF16X3 && c->packing == 0) return fail(DPD_EINVAL, "BF16x3 needs packed panels (packing 0)");
   // packed panels read sorted constant operands straight from the unsorted
-  // tensors (packed once); sorted per-call operands are expanded first unless
-  // fuse_sorts is set
+  // tensors (packed once); sorted per-call operands are expanded by the gather
+  // kernel first (measured as fast as the fused sort for them, and simpler)
   const uint64_t id1 = o1.static_id, id2 = o2.static_id;
-  static const int fuse_env = [] {
-    const char* e = getenv("DPD_FUSE_SORTS");
-    return e ? atoi(e) : 0;
-  }();
-  const bool fuse1 = packed && P->exp1 && (id1 || fuse_env);
-  const bool fuse2 = packed && P->exp2 && (id2 || fuse_env);
+  const bool fuse1 = packed && P->exp1 && id1;
+  const bool fuse2 = packed && P->exp2 && id2;
   const bool gx1 = P->exp1 && !fuse1, gx2 = P->exp2 && !fuse2;
   size_t need = 0;
   if (packed) {
@@ -1387,8 +1397,20 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   const void* B = (use_f32 && p2f) ? (const void*)p2f : (const void*)p2;
   ProfRec* pr = nullptr;
   if (c->prof.on && tag && c->prof.used + 2 <= c->prof.pool.size()) {
+    // algorithmic bytes: both operands once as the kernel reads them (panels, or
+    // the tensors in their storage precision) and the result (read too when
+    // accumulated)
+    int64_t bytes = P->gemm.c_elems * 8 * (beta != 0.0 ? 2 : 1);
+    if (packed) {
+      const GemmVariant& V = P->gemm.pk[prec];
+      bytes += (V.pa_elems + V.pb_elems) * pk_esize(prec);
+    } else {
+      const int64_t l1 = (gx1 ? P->len1x : make_layout(c->sym, o1.d).len) * nb1;
+      const int64_t l2 = (gx2 ? P->len2x : make_layout(c->sym, o2.d).len) * nb2;
+      bytes += l1 * ((use_f32 && p1f) ? 4 : 8) + l2 * ((use_f32 && p2f) ? 4 : 8);
+    }
     c->prof.recs.push_back({tag, c->prof.pool[c->prof.used], c->prof.pool[c->prof.used + 1], P->gemm.flops,
-                            prec});
+                            prec, packed ? P->gemm.pk[prec].cfg : P->gemm.cfg, packed ? 1 : 0, bytes});
     c->prof.used += 2;
     pr = &c->prof.recs.back();
     // the packed path records the events around the GEMM kernel itself

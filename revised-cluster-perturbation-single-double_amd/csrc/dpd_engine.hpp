@@ -68,7 +68,8 @@ struct GemmPlan {
   int cfg_req = -1;  // requested configuration (-1: planner's choice)
   int max_split_mn = 0;
   int64_t partial_elems = 0;
-  int64_t flops = 0;  // 2*M*N*K summed (algorithmic, per launch)
+  int64_t flops = 0;    // 2*M*N*K summed (algorithmic, per launch)
+  int64_t c_elems = 0;  // M*N summed: result elements written per launch
   DevBuf dev;         // tasks | segs | split
   GemmTask* d_tasks = nullptr;
   GemmSeg* d_segs = nullptr;
@@ -123,6 +124,9 @@ struct ProfRec {
   hipEvent_t a, b;
   int64_t flops;
   int prec;
+  int cfg;        // GemmCfg tile shape of the launch
+  int packed;     // packed-panel kernel (1) or gathering kernel (0)
+  int64_t bytes;  // algorithmic bytes: operands as the kernel reads them once + result
 };
 struct Profiler {
   bool on = false;
