@@ -226,6 +226,37 @@ int dpd_problem_diag(dpd_problem *prob, double *diag);
  * writes nroot unit vectors (device, contiguous) and their OED energies (host) */
 int dpd_problem_guess(dpd_problem *prob, int64_t nroot, double *R, double *energies);
 
+/* ---- adjoint: transposed operator and gradients -----------------------
+ * No single reference routine: the reference has no gradient code for these
+ * methods.  The transposed operator is built from the same contraction terms as
+ * dpd_problem_sigma (CISdoo.SD.SF/calAR.f90:41-77, EE-CPSd.SF/calC2_VR2.f90)
+ * with operand roles exchanged; the gradient contracts the two vectors into the
+ * shape of each integral block (the transpose-contraction rule). */
+/* t = J^T L for nvec vectors (device, stride dim) */
+int dpd_problem_sigma_t(dpd_problem *prob, const double *L, double *t, int64_t nvec);
+/* named device tensors: inputs "eorb" (spin-free orbital energies, occupied
+ * irreps then virtual), "iabj", "ijka", "ijkl", "abij", "abci", "abcd" (layouts
+ * of dpd_synth_fill) and, read only, the intermediates "fac", "fki", "wkaci",
+ * "mp2t", "diag".  Length -1: unknown name, 0: not used by the method. */
+int64_t dpd_problem_tensor_len(dpd_problem *prob, const char *name);
+int dpd_problem_get(dpd_problem *prob, const char *name, double *dst_device);
+/* overwrite an input; the problem refuses sigma/Davidson until refreshed */
+int dpd_problem_set(dpd_problem *prob, const char *name, const double *src_device);
+/* recompute the diagonal, MP2 amplitudes and the CIS(Doo) intermediates from
+ * the current inputs (the steps of dpd_problem_create after the integrals) */
+int dpd_problem_refresh(dpd_problem *prob);
+/* scale * d(L . J(theta) R)/d(theta) accumulated (+=) into the gradient buffers
+ * (device, the layouts and lengths of the named inputs; a buffer the method does
+ * not use may be NULL).  theta are the stored elements of the inputs, so for a
+ * packed pair (p<q) the derivative is with respect to the stored element.
+ * With L, R the left and right eigenvectors of one root this is L^T R times the
+ * gradient of its eigenvalue (first-order perturbation theory of a
+ * non-symmetric matrix).  The MP2 amplitudes and intermediates are recomputed
+ * from the inputs, not stored. */
+int dpd_problem_grad(dpd_problem *prob, const double *L, const double *R, double scale, double *eorb_bar,
+                     double *iabj_bar, double *ijka_bar, double *ijkl_bar, double *abij_bar,
+                     double *abci_bar, double *abcd_bar);
+
 typedef struct {
   int64_t nroot;
   int64_t max_dim;   /* davidson_dim (default 20) */
@@ -233,6 +264,7 @@ typedef struct {
   double conv;       /* max-abs residual tolerance, 10^-davidson_conv */
   int lowest;        /* 0: root tracking by overlap (default), 1: lowest */
   int final_check;   /* 1: recompute |A R - w R|_inf for every root at the end */
+  int transpose;     /* 1: solve with the transposed operator J^T (left eigenvectors) */
   /* outputs */
   int64_t iterations;
   int64_t sigma_calls;

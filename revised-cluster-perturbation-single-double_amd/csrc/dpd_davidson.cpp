@@ -309,6 +309,7 @@ extern "C" int dpd_small_eig(int64_t n, const double* A_rowmajor, double* wr, do
 
 extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out) {
   if (!P || !o || !R || !w_out) return fail(DPD_EINVAL, "dpd_davidson: null argument");
+  if (P->stale) return fail(DPD_EINVAL, "problem inputs changed: call dpd_problem_refresh first");
   dpd_ctx* c = P->ctx;
   DPD_HIP_TRY(hipSetDevice(c->device));
   const int64_t n = P->dim;
@@ -352,7 +353,7 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
   int64_t nsig = 0, nskip = 0;
   auto sigma = [&](const double* x, double* y, int nv) -> int {
     nsig += nv;
-    return problem_sigma(P, x, y, nv);
+    return o->transpose ? problem_sigma_t(P, x, y, nv) : problem_sigma(P, x, y, nv);
   };
   // all dots X_i . Y_j (i < nx, j < ny) -> host[i + j*nx], one synchronisation
   auto block_dots = [&](const double* X, int nx, const double* Y, int ny, double* host) -> int {

@@ -671,6 +671,39 @@ __global__ void k_div(int64_t n, const double* __restrict__ a, const double* __r
     c[i] = a[i] / b[i];
 }
 
+// adjoint of the orbital-energy sums: for every element e of layout L,
+// w = coef * x[e] * y[e], and eorb_bar[p_d] += sign[d] * w for every dim d with
+// space[d] != 0 (1 virtual, 2 occupied) at its orbital p_d.  Global atomic adds
+// (vector memory), so the summation order is not fixed.
+__global__ __launch_bounds__(256) void k_eorb_adjoint(DevLayout L, const double* __restrict__ x,
+                                                      const double* __restrict__ y, EorbOff off, EorbAdj adj,
+                                                      double* eorb_bar) {
+  __shared__ DevLayout sl;
+  for (int i = threadIdx.x; i < (int)(sizeof(DevLayout) / 4); i += blockDim.x)
+    ((int32_t*)&sl)[i] = ((const int32_t*)&L)[i];
+  __syncthreads();
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < sl.len; o += (int64_t)gridDim.x * blockDim.x) {
+    const double w = adj.coef * x[o] * y[o];
+    if (w == 0.0) continue;
+    int h[4];
+    int64_t xi[4];
+    decode_elem(sl, o, h, xi);
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      if (adj.space[d] == 0) continue;
+      const int64_t pos = (adj.space[d] == 1 ? off.offv[h[d]] : off.offo[h[d]]) + xi[d];
+      atomicAdd(&eorb_bar[pos], adj.sign[d] * w);
+    }
+  }
+}
+
+hipError_t launch_eorb_adjoint(const DevLayout& L, const double* x, const double* y, const EorbOff& off,
+                               const EorbAdj& adj, double* eorb_bar, hipStream_t s) {
+  if (L.len <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_eorb_adjoint, dim3(grid_for(L.len)), dim3(256), 0, s, L, x, y, off, adj, eorb_bar);
+  return hipGetLastError();
+}
+
 hipError_t launch_orbital_energy(const DevLayout& L, const double* eorb, const EorbOff& off, int mode,
                                  double* out, hipStream_t s) {
   if (L.len == 0) return hipSuccess;

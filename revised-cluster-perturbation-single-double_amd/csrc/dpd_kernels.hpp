@@ -169,6 +169,14 @@ namespace dpd {
 struct EorbOff {
   int64_t offo[8], offv[8];
 };
+// adjoint of orbital-energy sums over a DPD layout (see k_eorb_adjoint)
+struct EorbAdj {
+  int space[4];    // 0 none, 1 virtual, 2 occupied
+  double sign[4];
+  double coef;
+};
+hipError_t launch_eorb_adjoint(const DevLayout& L, const double* x, const double* y, const EorbOff& off,
+                               const EorbAdj& adj, double* eorb_bar, hipStream_t s);
 hipError_t launch_orbital_energy(const DevLayout& L, const double* eorb, const EorbOff& off, int mode,
                                  double* out, hipStream_t s);
 hipError_t launch_div(int64_t n, const double* num, const double* den, double* out, hipStream_t s);

@@ -191,67 +191,22 @@ int problem_make_f32(dpd_problem* p) {
 
 using namespace dpd;
 
-extern "C" {
-
-int dpd_synth_fill(const dpd_synth_spec* spec, const char* which, int64_t iop, double* out, int64_t len);
-
-int dpd_problem_create(dpd_ctx* c, const dpd_synth_spec* spec_in, int method, int64_t symR, dpd_problem** out) {
-  if (!c || !spec_in || !out) return fail(DPD_EINVAL, "null argument");
-  if (method < 0 || method > 2) return fail(DPD_EINVAL, "method must be 0 (SF-CIS), 1 (CIS(Doo)) or 2 (SF-CPSd)");
-  DPD_HIP_TRY(hipSetDevice(c->device));
-  dpd_synth_spec spec = *spec_in;
-  int rc = dpd_synth_layout(&spec);
-  if (rc) return rc;
-  if (spec.nirrep != c->sym.nirrep) return fail(DPD_EINVAL, "spec nirrep differs from the context");
-  if (symR < 1 || symR > spec.nirrep) return fail(DPD_EINVAL, "symR out of range");
-  auto p = std::make_unique<dpd_problem>();
-  static std::atomic<uint64_t> next_uid{1};
-  p->uid = next_uid++;
-  p->ctx = c;
-  p->spec = spec;
-  p->method = method;
-  p->symR = (int)symR;
+// diagonal, MP2 amplitudes and the CIS(Doo) intermediates from the current
+// orbital energies and integrals (also after a caller changed them)
+static int problem_finalize(dpd_problem* p) {
+  dpd_ctx* c = p->ctx;
   const Sym& S = c->sym;
-  for (int h = 0; h < 8; h++) {
-    p->popo[h] = h < S.nirrep ? spec.popo[h] : 0;
-    p->popv[h] = h < S.nirrep ? spec.popv[h] : 0;
-    p->zorb[h] = h == 0;
-  }
   const int64_t *v = p->popv, *o = p->popo, *z = p->zorb;
-  p->l1 = S.irpdso((int)symR, v, o, 1);
-  p->l2 = method == 0 ? 0 : S.idsymsoc((int)symR, v, v, 0, o, o, 0);
-  p->dim = p->l1 + p->l2;
   hipStream_t st = c->stream;
-  // orbital energies
-  int64_t ne = 0;
-  for (int h = 0; h < S.nirrep; h++) ne += p->popo[h] + p->popv[h];
-  p->eorb_host.assign(ne, 0.0);
-  rc = dpd_synth_fill(&spec, "eorb", 0, p->eorb_host.data(), ne);
-  if (rc) return rc;
-  if ((rc = upload(p->eorb, p->eorb_host, st))) return rc;
-  auto gen = [&](const char* which, int iop, const Desc& d, DevBuf& dst, int64_t& len) -> int {
-    len = make_layout(S, d).len;
-    std::vector<double> h(len);
-    int r = dpd_synth_fill(&spec, which, iop, h.data(), len);
-    if (r) return r;
-    return upload(dst, h, st);
-  };
-  if ((rc = gen("iabj", 1, mkdesc(v, o, v, o, 1, 1, 1), p->iabj, p->len_iabj))) return rc;
+  int rc;
   // diagonal (OED1, OED2): lib_cc/Eorb_diff.f90
   if ((rc = p->diag.alloc(sizeof(double) * std::max<int64_t>(1, p->dim)))) return rc;
-  EorbOff eo = eorb_off(p.get());
-  DPD_HIP_TRY(launch_orbital_energy(make_layout(S, mkdesc(v, o, z, z, 1, 1, (int)symR)), (const double*)p->eorb.p,
+  EorbOff eo = eorb_off(p);
+  DPD_HIP_TRY(launch_orbital_energy(make_layout(S, mkdesc(v, o, z, z, 1, 1, p->symR)), (const double*)p->eorb.p,
                                     eo, 1, (double*)p->diag.p, st));
-  if (method >= 1) {
-    DPD_HIP_TRY(launch_orbital_energy(make_layout(S, mkdesc(v, v, o, o, 0, 0, (int)symR)),
+  if (p->method >= 1) {
+    DPD_HIP_TRY(launch_orbital_energy(make_layout(S, mkdesc(v, v, o, o, 0, 0, p->symR)),
                                       (const double*)p->eorb.p, eo, 2, (double*)p->diag.p + p->l1, st));
-    if ((rc = gen("ijka", 0, mkdesc(o, o, o, v, 0, 1, 1), p->ijka, p->len_ijka))) return rc;
-    if ((rc = gen("abij", 0, mkdesc(v, v, o, o, 0, 0, 1), p->abij, p->len_abij))) return rc;
-    if ((rc = gen("abci", 0, mkdesc(v, v, v, o, 0, 1, 1), p->abci, p->len_abci))) return rc;
-    if (method == 2) {
-      if ((rc = gen("ijkl", 0, mkdesc(o, o, o, o, 0, 0, 1), p->ijkl, p->len_ijkl))) return rc;
-      if ((rc = gen("abcd", 0, mkdesc(v, v, v, v, 0, 0, 1), p->abcd, p->len_abcd))) return rc;
-    }
     // MP2 amplitudes t = <ab||ij> / (e_i+e_j-e_a-e_b)   (denomT.f90:94-249)
     Desc dT = mkdesc(v, v, o, o, 0, 0, 1);
     if ((rc = p->mp2t.alloc(sizeof(double) * std::max<int64_t>(1, p->len_abij)))) return rc;
@@ -293,6 +248,273 @@ int dpd_problem_create(dpd_ctx* c, const dpd_synth_spec* spec_in, int method, in
     if (rc) return rc;
   }
   DPD_HIP_TRY(hipStreamSynchronize(st));
+  return DPD_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Transpose of the sigma operator and the adjoint (transpose-contraction)
+// gradient of L.sigma(R).  Every term c_out += f * (V x R) of problem_sigma has
+//   transpose:  t_R  += f * (V x l_out)      (the expansions and
+//                antisymmetrisers of the forward call become, in the transposed
+//                call, the expansion of l_out and the antisymmetrise+compress
+//                of t_R: the surface applies them from the labels)
+//   gradient:   Vbar += f * (l_out x R)     in V's labels and layout
+// The MP2 amplitudes and the CIS(Doo) intermediates are recomputed here from the
+// integrals and orbital energies, not stored from the forward pass.
+// ---------------------------------------------------------------------------
+static Operand lab_of(Operand x, const char* l, int rank) {
+  x.rank = rank;
+  std::memset(x.lab, 0, 4);
+  std::memcpy(x.lab, l, rank);
+  return x;
+}
+
+int dpd::problem_sigma_t(dpd_problem* p, const double* L, double* t, int64_t nvec) {
+  dpd_ctx* c = p->ctx;
+  const int64_t *v = p->popv, *o = p->popo, *z = p->zorb;
+  const int S = p->symR;
+  const int nb = (int)nvec;
+  const int pr = p->prec;
+  DPD_HIP_TRY(launch_fill(p->dim * nvec, 0.0, t, c->stream));
+  Desc dR1 = mkdesc(v, o, z, z, 1, 1, S);
+  Operand L1 = batched(op(dR1, "", 4, L), p->dim);
+  Operand T1 = batched(op(dR1, "", 4, t), p->dim);
+  int rc;
+  if (p->method == 0) {
+    DPD_HIP_TRY(launch_hadamard_acc(p->l1, (const double*)p->diag.p, L, t, nb, p->dim, p->dim, c->stream));
+    Operand W = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->iabj.p), p, 1);
+    return contract_call(c, 444, W, lab_of(L1, "ai+-", 4), lab_of(T1, "fn+-", 4), 1.0, 0, nb, pr, nullptr,
+                         "t_c1_r1iabj");
+  }
+  Desc dR2 = mkdesc(v, v, o, o, 0, 0, S);
+  Operand L2 = batched(op(dR2, "", 4, L + p->l1), p->dim);
+  Operand T2 = batched(op(dR2, "", 4, t + p->l1), p->dim);
+  Operand Fac = konst(op(mkdesc(v, v, z, z, 1, 1, 1), "ae=+", 4, (const double*)p->fac.p), p, 2);
+  Operand Fki = konst(op(mkdesc(o, o, z, z, 1, 1, 1), "mi=-", 4, (const double*)p->fki.p), p, 3);
+  Operand Wk = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->wkaci.p), p, 4);
+  Operand abci = konst(op(mkdesc(v, v, v, o, 0, 1, 1), "feam", 4, (const double*)p->abci.p), p, 5);
+  Operand ijka = konst(op(mkdesc(o, o, o, v, 0, 1, 1), "mnie", 4, (const double*)p->ijka.p), p, 6);
+  if ((rc = contract_call(c, 444, Fac, lab_of(L1, "ai+-", 4), lab_of(T1, "ei=-", 4), 1.0, 0, nb, pr, nullptr,
+                          "t_c1_r1Fae")))
+    return rc;
+  if ((rc = contract_call(c, 444, lab_of(L1, "ai+-", 4), Fki, lab_of(T1, "am=+", 4), -1.0, 0, nb, pr, nullptr,
+                          "t_c1_r1Fmi")))
+    return rc;
+  if ((rc = contract_call(c, 444, Wk, lab_of(L1, "ai+-", 4), lab_of(T1, "fn+-", 4), 1.0, 0, nb, pr, nullptr,
+                          "t_c1_r1iabj")))
+    return rc;
+  if ((rc = contract_call(c, 424, abci, lab_of(L1, "ai", 2), lab_of(T2, "feim", 4), 1.0, 2, nb, pr, nullptr,
+                          "t_c1_r2abci")))
+    return rc;
+  if ((rc = contract_call(c, 424, ijka, lab_of(L1, "ai", 2), lab_of(T2, "aemn", 4), -1.0, 1, nb, pr, nullptr,
+                          "t_c1_r2ijka")))
+    return rc;
+  DPD_HIP_TRY(launch_hadamard_acc(p->l2, (const double*)p->diag.p + p->l1, L + p->l1, t + p->l1, nb, p->dim,
+                                  p->dim, c->stream));
+  if ((rc = contract_call(c, 442, lab_of(abci, "abej", 4), lab_of(L2, "abij", 4), lab_of(T1, "ei", 2), 1.0, 0, nb,
+                          pr, nullptr, "t_c2_r1abci")))
+    return rc;
+  if ((rc = contract_call(c, 442, lab_of(L2, "abij", 4), lab_of(ijka, "ijmb", 4), lab_of(T1, "am", 2), -1.0, 0, nb,
+                          pr, nullptr, "t_c2_r1ijka")))
+    return rc;
+  if (p->method == 2) {
+    Operand W = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "embj", 4, (const double*)p->iabj.p), p, 7);
+    Operand ijkl = konst(op(mkdesc(o, o, o, o, 0, 0, 1), "mnij", 4, (const double*)p->ijkl.p), p, 8);
+    Operand abcd = konst(op(mkdesc(v, v, v, v, 0, 0, 1), "abef", 4, (const double*)p->abcd.p), p, 9);
+    if ((rc = contract_call(c, 444, lab_of(L2, "abij", 4), W, lab_of(T2, "aeim", 4), 1.0, 3, nb, pr, nullptr,
+                            "t_c2_r2Wmbej")))
+      return rc;
+    if ((rc = contract_call(c, 444, lab_of(L2, "abij", 4), ijkl, lab_of(T2, "abmn", 4), 1.0, 0, nb, pr, nullptr,
+                            "t_c2_r2Wmnij")))
+      return rc;
+    if ((rc = contract_call(c, 444, abcd, lab_of(L2, "abij", 4), lab_of(T2, "efij", 4), 1.0, 0, nb, pr, nullptr,
+                            "t_c2_r2abcd")))
+      return rc;
+  }
+  return DPD_OK;
+}
+
+// gradient buffers: eorb and the integral blocks, in the problem's layouts
+struct GradOut {
+  double* eorb = nullptr;
+  double *iabj = nullptr, *ijka = nullptr, *ijkl = nullptr, *abij = nullptr, *abci = nullptr, *abcd = nullptr;
+};
+
+// scale * d(L . sigma(R)) / d(theta), accumulated into g (single vectors L, R)
+static int problem_grad(dpd_problem* p, const double* L, const double* R, double scale, const GradOut& g) {
+  dpd_ctx* c = p->ctx;
+  const Sym& Sy = c->sym;
+  const int64_t *v = p->popv, *o = p->popo, *z = p->zorb;
+  const int S = p->symR;
+  const int pr = p->prec;
+  hipStream_t st = c->stream;
+  EorbOff eo = eorb_off(p);
+  Desc dR1 = mkdesc(v, o, z, z, 1, 1, S);
+  Operand L1 = op(dR1, "", 4, L), R1 = op(dR1, "", 4, R);
+  Desc dW = mkdesc(v, o, v, o, 1, 1, 1);
+  int rc;
+  auto need = [&](const double* ptr, const char* what) -> int {
+    return ptr ? DPD_OK : fail(DPD_EINVAL, "gradient buffer for %s is required", what);
+  };
+  if ((rc = need(g.eorb, "eorb")) || (rc = need(g.iabj, "iabj"))) return rc;
+  if (p->method == 0) {
+    // c1_r1iabj: iabj_bar(fn,ai) += R1(fn) L1(ai)
+    rc = contract_call(c, 444, lab_of(R1, "fn+-", 4), lab_of(L1, "ai+-", 4), op(dW, "fnai", 4, g.iabj), scale, 0, 1,
+                       pr);
+    if (rc) return rc;
+    // OED1 (e_a - e_i) r(a,i)
+    EorbAdj adj = {{1, 2, 0, 0}, {1.0, -1.0, 0.0, 0.0}, scale};
+    DPD_HIP_TRY(launch_eorb_adjoint(make_layout(Sy, dR1), L, R, eo, adj, g.eorb, st));
+    return DPD_OK;
+  }
+  if ((rc = need(g.ijka, "ijka")) || (rc = need(g.abij, "abij")) || (rc = need(g.abci, "abci"))) return rc;
+  if (p->method == 2 && ((rc = need(g.ijkl, "ijkl")) || (rc = need(g.abcd, "abcd")))) return rc;
+  Desc dR2 = mkdesc(v, v, o, o, 0, 0, S), dT = mkdesc(v, v, o, o, 0, 0, 1);
+  Operand L2 = op(dR2, "", 4, L + p->l1), R2 = op(dR2, "", 4, R + p->l1);
+  Desc dFac = mkdesc(v, v, z, z, 1, 1, 1), dFki = mkdesc(o, o, z, z, 1, 1, 1);
+  Desc dabci = mkdesc(v, v, v, o, 0, 1, 1), dijka = mkdesc(o, o, o, v, 0, 1, 1);
+  DevBuf fac_b, fki_b, wk_b, t2_b, t2bar_b, den_b, tmp_b;
+  if ((rc = fac_b.alloc(sizeof(double) * p->len_fac)) || (rc = fki_b.alloc(sizeof(double) * p->len_fki)) ||
+      (rc = wk_b.alloc(sizeof(double) * p->len_iabj)) || (rc = t2_b.alloc(sizeof(double) * p->len_abij)) ||
+      (rc = t2bar_b.alloc(sizeof(double) * p->len_abij)) || (rc = den_b.alloc(sizeof(double) * p->len_abij)) ||
+      (rc = tmp_b.alloc(sizeof(double) * p->len_abij)))
+    return rc;
+  double *facb = (double*)fac_b.p, *fkib = (double*)fki_b.p, *wkb = (double*)wk_b.p;
+  double *t2 = (double*)t2_b.p, *t2b = (double*)t2bar_b.p, *den = (double*)den_b.p, *tmp = (double*)tmp_b.p;
+  DPD_HIP_TRY(launch_fill(p->len_fac, 0.0, facb, st));
+  DPD_HIP_TRY(launch_fill(p->len_fki, 0.0, fkib, st));
+  DPD_HIP_TRY(launch_fill(p->len_iabj, 0.0, wkb, st));
+  DPD_HIP_TRY(launch_fill(p->len_abij, 0.0, t2b, st));
+  auto call = [&](int kind, const Operand& a, const Operand& b, const Operand& out, double f, int anti) {
+    return contract_call(c, kind, a, b, out, f, anti, 1, pr);
+  };
+  // ---- singles terms
+  if ((rc = call(444, lab_of(L1, "ai+-", 4), lab_of(R1, "ei=-", 4), op(dFac, "ae=+", 4, facb), scale, 0))) return rc;
+  if ((rc = call(444, lab_of(R1, "am=+", 4), lab_of(L1, "ai+-", 4), op(dFki, "mi=-", 4, fkib), -scale, 0))) return rc;
+  if ((rc = call(444, lab_of(R1, "fn+-", 4), lab_of(L1, "ai+-", 4), op(dW, "fnai", 4, wkb), scale, 0))) return rc;
+  if ((rc = call(424, lab_of(R2, "feim", 4), lab_of(L1, "ai", 2), op(dabci, "feam", 4, g.abci), scale, 0))) return rc;
+  if ((rc = call(424, lab_of(R2, "aemn", 4), lab_of(L1, "ai", 2), op(dijka, "mnie", 4, g.ijka), -scale, 0))) return rc;
+  // ---- doubles terms
+  {
+    EorbAdj adj = {{1, 1, 2, 2}, {1.0, 1.0, -1.0, -1.0}, scale};  // OED2: e_a + e_b - e_i - e_j
+    DPD_HIP_TRY(launch_eorb_adjoint(make_layout(Sy, dR2), L + p->l1, R + p->l1, eo, adj, g.eorb, st));
+  }
+  if ((rc = call(424, lab_of(L2, "abij", 4), lab_of(R1, "ei", 2), op(dabci, "abej", 4, g.abci), scale, 0))) return rc;
+  if ((rc = call(424, lab_of(L2, "abij", 4), lab_of(R1, "am", 2), op(dijka, "ijmb", 4, g.ijka), -scale, 0))) return rc;
+  if (p->method == 2) {
+    if ((rc = call(444, lab_of(R2, "aeim", 4), lab_of(L2, "abij", 4), op(dW, "embj", 4, g.iabj), scale, 0)))
+      return rc;
+    if ((rc = call(444, lab_of(R2, "abmn", 4), lab_of(L2, "abij", 4),
+                   op(mkdesc(o, o, o, o, 0, 0, 1), "mnij", 4, g.ijkl), scale, 0)))
+      return rc;
+    if ((rc = call(444, lab_of(L2, "abij", 4), lab_of(R2, "efij", 4),
+                   op(mkdesc(v, v, v, v, 0, 0, 1), "abef", 4, g.abcd), scale, 0)))
+      return rc;
+  }
+  // ---- intermediates, recomputed: D = e_i + e_j - e_a - e_b, t = abij / D
+  const DevLayout LT = make_layout(Sy, dT);
+  DPD_HIP_TRY(launch_orbital_energy(LT, (const double*)p->eorb.p, eo, 3, den, st));
+  DPD_HIP_TRY(launch_div(p->len_abij, (const double*)p->abij.p, den, t2, st));
+  const double* V = (const double*)p->abij.p;
+  // Fac = f_vv - sum t(af,mn) v(ef,mn): diagonal -> eorb (virtual), then t and v
+  if ((rc = call(424, op(dT, "efmn", 4, V), op(dFac, "ae", 2, facb), op(dT, "afmn", 4, t2b), -1.0, 1))) return rc;
+  if ((rc = call(424, op(dT, "afmn", 4, t2), op(dFac, "ae", 2, facb), op(dT, "efmn", 4, g.abij), -1.0, 1))) return rc;
+  // Fki = f_oo + sum v(ef,mn) t(ef,in)
+  if ((rc = call(424, op(dT, "efin", 4, t2), op(dFki, "mi", 2, fkib), op(dT, "efmn", 4, g.abij), 1.0, 2))) return rc;
+  if ((rc = call(424, op(dT, "efmn", 4, V), op(dFki, "mi", 2, fkib), op(dT, "efin", 4, t2b), 1.0, 2))) return rc;
+  // Wk = iabj + sum v(fe,nm) t(bf,jn)
+  DPD_HIP_TRY(launch_axpy(p->len_iabj, 1.0, wkb, g.iabj, st));
+  if ((rc = call(444, op(dW, "embj", 4, wkb), op(dT, "bfjn", 4, t2), op(dT, "fenm", 4, g.abij), 1.0, 3))) return rc;
+  if ((rc = call(444, op(dT, "fenm", 4, V), op(dW, "embj", 4, wkb), op(dT, "bfjn", 4, t2b), 1.0, 3))) return rc;
+  // t = v / D: v_bar += t_bar / D; e_bar from dD (t_bar t / D, + for a,b and - for i,j)
+  DPD_HIP_TRY(launch_div(p->len_abij, t2b, den, tmp, st));
+  DPD_HIP_TRY(launch_axpy(p->len_abij, 1.0, tmp, g.abij, st));
+  {
+    EorbAdj adj = {{1, 1, 2, 2}, {1.0, 1.0, -1.0, -1.0}, 1.0};
+    DPD_HIP_TRY(launch_eorb_adjoint(LT, tmp, t2, eo, adj, g.eorb, st));
+  }
+  // diagonals of Fac / Fki -> eorb (host: tiny)
+  std::vector<double> hf(p->len_fac), hk(p->len_fki), de(p->eorb_host.size(), 0.0);
+  DPD_HIP_TRY(hipMemcpyAsync(hf.data(), facb, sizeof(double) * hf.size(), hipMemcpyDeviceToHost, st));
+  DPD_HIP_TRY(hipMemcpyAsync(hk.data(), fkib, sizeof(double) * hk.size(), hipMemcpyDeviceToHost, st));
+  DPD_HIP_TRY(hipStreamSynchronize(st));
+  {
+    int64_t offv = 0, offo = 0, ev = 0;
+    for (int h = 0; h < Sy.nirrep; h++) ev += p->popo[h];
+    for (int h = 0; h < Sy.nirrep; h++) {
+      const int64_t nv = p->popv[h], no = p->popo[h];
+      const int64_t bv = Sy.isymoffso(h + 1, 1, v, v, 1) - 1, bo = Sy.isymoffso(h + 1, 1, o, o, 1) - 1;
+      for (int64_t a = 0; a < nv; a++) de[ev + offv + a] += hf[bv + a * nv + a];
+      for (int64_t i = 0; i < no; i++) de[offo + i] += hk[bo + i * no + i];
+      offv += nv;
+      offo += no;
+    }
+  }
+  DevBuf de_b;
+  if ((rc = upload(de_b, de, st))) return rc;
+  DPD_HIP_TRY(launch_axpy((int64_t)de.size(), 1.0, (const double*)de_b.p, g.eorb, st));
+  DPD_HIP_TRY(hipStreamSynchronize(st));
+  return DPD_OK;
+}
+
+extern "C" {
+
+int dpd_synth_fill(const dpd_synth_spec* spec, const char* which, int64_t iop, double* out, int64_t len);
+
+int dpd_problem_create(dpd_ctx* c, const dpd_synth_spec* spec_in, int method, int64_t symR, dpd_problem** out) {
+  if (!c || !spec_in || !out) return fail(DPD_EINVAL, "null argument");
+  if (method < 0 || method > 2) return fail(DPD_EINVAL, "method must be 0 (SF-CIS), 1 (CIS(Doo)) or 2 (SF-CPSd)");
+  DPD_HIP_TRY(hipSetDevice(c->device));
+  dpd_synth_spec spec = *spec_in;
+  int rc = dpd_synth_layout(&spec);
+  if (rc) return rc;
+  if (spec.nirrep != c->sym.nirrep) return fail(DPD_EINVAL, "spec nirrep differs from the context");
+  if (symR < 1 || symR > spec.nirrep) return fail(DPD_EINVAL, "symR out of range");
+  auto p = std::make_unique<dpd_problem>();
+  static std::atomic<uint64_t> next_uid{1};
+  p->uid = next_uid++;
+  p->ctx = c;
+  p->spec = spec;
+  p->method = method;
+  p->symR = (int)symR;
+  const Sym& S = c->sym;
+  for (int h = 0; h < 8; h++) {
+    p->popo[h] = h < S.nirrep ? spec.popo[h] : 0;
+    p->popv[h] = h < S.nirrep ? spec.popv[h] : 0;
+    p->zorb[h] = h == 0;
+  }
+  const int64_t *v = p->popv, *o = p->popo;
+  p->l1 = S.irpdso((int)symR, v, o, 1);
+  p->l2 = method == 0 ? 0 : S.idsymsoc((int)symR, v, v, 0, o, o, 0);
+  p->dim = p->l1 + p->l2;
+  hipStream_t st = c->stream;
+  // orbital energies
+  int64_t ne = 0;
+  for (int h = 0; h < S.nirrep; h++) ne += p->popo[h] + p->popv[h];
+  p->eorb_host.assign(ne, 0.0);
+  rc = dpd_synth_fill(&spec, "eorb", 0, p->eorb_host.data(), ne);
+  if (rc) return rc;
+  if ((rc = upload(p->eorb, p->eorb_host, st))) return rc;
+  auto gen = [&](const char* which, int iop, const Desc& d, DevBuf& dst, int64_t& len) -> int {
+    len = make_layout(S, d).len;
+    std::vector<double> h(len);
+    int r = dpd_synth_fill(&spec, which, iop, h.data(), len);
+    if (r) return r;
+    return upload(dst, h, st);
+  };
+  if ((rc = gen("iabj", 1, mkdesc(v, o, v, o, 1, 1, 1), p->iabj, p->len_iabj))) return rc;
+  if (method >= 1) {
+    if ((rc = gen("ijka", 0, mkdesc(o, o, o, v, 0, 1, 1), p->ijka, p->len_ijka))) return rc;
+    if ((rc = gen("abij", 0, mkdesc(v, v, o, o, 0, 0, 1), p->abij, p->len_abij))) return rc;
+    if ((rc = gen("abci", 0, mkdesc(v, v, v, o, 0, 1, 1), p->abci, p->len_abci))) return rc;
+    if (method == 2) {
+      if ((rc = gen("ijkl", 0, mkdesc(o, o, o, o, 0, 0, 1), p->ijkl, p->len_ijkl))) return rc;
+      if ((rc = gen("abcd", 0, mkdesc(v, v, v, v, 0, 0, 1), p->abcd, p->len_abcd))) return rc;
+    }
+  }
+  if ((rc = problem_finalize(p.get()))) return rc;
+  DPD_HIP_TRY(hipStreamSynchronize(st));
   *out = p.release();
   return DPD_OK;
 }
@@ -317,6 +539,7 @@ int dpd_problem_set_precision(dpd_problem* p, int prec) {
 
 int dpd_problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) {
   if (!p || !R || !sigma || nvec < 1) return fail(DPD_EINVAL, "dpd_problem_sigma: bad arguments");
+  if (p->stale) return fail(DPD_EINVAL, "problem inputs changed: call dpd_problem_refresh first");
   DPD_HIP_TRY(hipSetDevice(p->ctx->device));
   return problem_sigma(p, R, sigma, nvec);
 }
@@ -378,4 +601,111 @@ int dpd_problem_guess(dpd_problem* p, int64_t nroot, double* R, double* energies
   return DPD_OK;
 }
 
+
+int dpd_problem_sigma_t(dpd_problem* p, const double* L, double* t, int64_t nvec) {
+  if (!p || !L || !t || nvec < 1) return fail(DPD_EINVAL, "dpd_problem_sigma_t: bad arguments");
+  if (p->stale) return fail(DPD_EINVAL, "problem inputs changed: call dpd_problem_refresh first");
+  DPD_HIP_TRY(hipSetDevice(p->ctx->device));
+  return dpd::problem_sigma_t(p, L, t, nvec);
+}
+
+// named device tensors of a problem: the inputs (orbital energies and
+// integrals) and the intermediates finalize derives from them
+static int problem_tensor(dpd_problem* p, const char* name, DevBuf** buf, int64_t* len, bool* input) {
+  struct {
+    const char* n;
+    DevBuf* b;
+    int64_t l;
+    bool in;
+  } tab[] = {{"eorb", &p->eorb, (int64_t)p->eorb_host.size(), true}, {"iabj", &p->iabj, p->len_iabj, true},
+             {"ijka", &p->ijka, p->len_ijka, true},   {"ijkl", &p->ijkl, p->len_ijkl, true},
+             {"abij", &p->abij, p->len_abij, true},   {"abci", &p->abci, p->len_abci, true},
+             {"abcd", &p->abcd, p->len_abcd, true},   {"fac", &p->fac, p->len_fac, false},
+             {"fki", &p->fki, p->len_fki, false},     {"wkaci", &p->wkaci, p->len_iabj, false},
+             {"mp2t", &p->mp2t, p->len_abij, false},  {"diag", &p->diag, p->dim, false}};
+  for (auto& e : tab)
+    if (std::strcmp(e.n, name) == 0) {
+      *buf = e.b;
+      *len = e.b->p ? e.l : 0;
+      *input = e.in;
+      return DPD_OK;
+    }
+  return fail(DPD_EINVAL, "unknown problem tensor '%s'", name);
+}
+
+int64_t dpd_problem_tensor_len(dpd_problem* p, const char* name) {
+  DevBuf* b;
+  int64_t len;
+  bool in;
+  if (!p || !name || problem_tensor(p, name, &b, &len, &in)) return -1;
+  return len;
+}
+
+int dpd_problem_get(dpd_problem* p, const char* name, double* dst) {
+  if (!p || !name || !dst) return fail(DPD_EINVAL, "dpd_problem_get: null argument");
+  DevBuf* b;
+  int64_t len;
+  bool in;
+  int rc = problem_tensor(p, name, &b, &len, &in);
+  if (rc) return rc;
+  DPD_HIP_TRY(hipSetDevice(p->ctx->device));
+  if (len > 0)
+    DPD_HIP_TRY(hipMemcpyAsync(dst, b->p, sizeof(double) * len, hipMemcpyDeviceToDevice, p->ctx->stream));
+  return DPD_OK;
+}
+
+int dpd_problem_set(dpd_problem* p, const char* name, const double* src) {
+  if (!p || !name || !src) return fail(DPD_EINVAL, "dpd_problem_set: null argument");
+  DevBuf* b;
+  int64_t len;
+  bool in;
+  int rc = problem_tensor(p, name, &b, &len, &in);
+  if (rc) return rc;
+  if (!in) return fail(DPD_EINVAL, "'%s' is derived from the inputs; set the inputs and refresh", name);
+  if (len == 0) return fail(DPD_EINVAL, "'%s' is not used by this method", name);
+  DPD_HIP_TRY(hipSetDevice(p->ctx->device));
+  DPD_HIP_TRY(hipMemcpyAsync(b->p, src, sizeof(double) * len, hipMemcpyDeviceToDevice, p->ctx->stream));
+  p->stale = true;
+  return DPD_OK;
+}
+
+int dpd_problem_refresh(dpd_problem* p) {
+  if (!p) return fail(DPD_EINVAL, "null problem");
+  DPD_HIP_TRY(hipSetDevice(p->ctx->device));
+  hipStream_t st = p->ctx->stream;
+  DPD_HIP_TRY(hipMemcpyAsync(p->eorb_host.data(), p->eorb.p, sizeof(double) * p->eorb_host.size(),
+                             hipMemcpyDeviceToHost, st));
+  DPD_HIP_TRY(hipStreamSynchronize(st));
+  static std::atomic<uint64_t> next_gen{1ull << 40};
+  p->uid = next_gen++;  // constant tensors changed: packed panels are rebuilt under new ids
+  p->iabj32.reset();   // FP32 copies are remade on the next FP32 sigma
+  p->ijka32.reset();
+  p->ijkl32.reset();
+  p->abci32.reset();
+  p->abcd32.reset();
+  p->fac32.reset();
+  p->fki32.reset();
+  p->wkaci32.reset();
+  int rc = problem_finalize(p);
+  if (rc) return rc;
+  p->stale = false;
+  return DPD_OK;
+}
+
+int dpd_problem_grad(dpd_problem* p, const double* L, const double* R, double scale, double* eorb_bar,
+                     double* iabj_bar, double* ijka_bar, double* ijkl_bar, double* abij_bar, double* abci_bar,
+                     double* abcd_bar) {
+  if (!p || !L || !R) return fail(DPD_EINVAL, "dpd_problem_grad: null argument");
+  if (p->stale) return fail(DPD_EINVAL, "problem inputs changed: call dpd_problem_refresh first");
+  DPD_HIP_TRY(hipSetDevice(p->ctx->device));
+  GradOut g;
+  g.eorb = eorb_bar;
+  g.iabj = iabj_bar;
+  g.ijka = ijka_bar;
+  g.ijkl = ijkl_bar;
+  g.abij = abij_bar;
+  g.abci = abci_bar;
+  g.abcd = abcd_bar;
+  return problem_grad(p, L, R, scale, g);
+}
 }  // extern "C"

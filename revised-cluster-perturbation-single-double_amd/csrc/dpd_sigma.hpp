@@ -22,9 +22,11 @@ struct dpd_problem {
   int64_t len_fac = 0, len_fki = 0;
   int64_t sigma_flops = 0;  // algorithmic GEMM flops of one sigma (one vector)
   uint64_t uid = 0;         // process-unique id: constant tensors are packed once under it
+  bool stale = false;       // an input was set after the last finalize
 };
 
 namespace dpd {
 int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec);
 int problem_make_f32(dpd_problem* p);
+int problem_sigma_t(dpd_problem* p, const double* L, double* t, int64_t nvec);
 }

@@ -349,6 +349,59 @@ class Problem:
         check(lib().dpd_problem_sigma(self.h, R.data_ptr(), out.data_ptr(), nvec))
         return out
 
+    def sigma_t(self, L, out=None):
+        """t = J^T L (the transposed operator, same terms with operand roles exchanged)."""
+        torch = _torch()
+        nvec = L.numel() // self.dim
+        if out is None:
+            out = torch.empty_like(L)
+        check(lib().dpd_problem_sigma_t(self.h, L.data_ptr(), out.data_ptr(), nvec))
+        return out
+
+    INPUTS = ("eorb", "iabj", "ijka", "ijkl", "abij", "abci", "abcd")
+
+    def tensor_len(self, name):
+        n = int(lib().dpd_problem_tensor_len(self.h, name.encode()))
+        if n < 0:
+            raise KeyError(name)
+        return n
+
+    def get(self, name):
+        """Device copy of a named input or intermediate (float64, flat)."""
+        torch = _torch()
+        out = torch.empty(self.tensor_len(name), dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        if out.numel():
+            check(lib().dpd_problem_get(self.h, name.encode(), out.data_ptr()))
+        self.ctx.synchronize()
+        return out
+
+    def set(self, name, x):
+        """Overwrite an input with a device tensor of its length; call refresh() after."""
+        n = self.tensor_len(name)
+        if x.numel() != n or x.dtype != _torch().float64 or not x.is_cuda:
+            raise ValueError(f"{name}: expected a float64 device tensor of {n} elements")
+        x = x.contiguous()
+        check(lib().dpd_problem_set(self.h, name.encode(), x.data_ptr()))
+        self.ctx.synchronize()
+
+    def refresh(self):
+        check(lib().dpd_problem_refresh(self.h))
+
+    def grad(self, L, R, scale=1.0, out=None):
+        """scale * d(L . J R)/d(inputs), accumulated into out (dict name -> tensor,
+        zero-initialised when not given); returns the dict."""
+        torch = _torch()
+        if out is None:
+            out = {}
+            for k in self.INPUTS:
+                n = self.tensor_len(k)
+                if n:
+                    out[k] = torch.zeros(n, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        ptr = [out[k].data_ptr() if k in out else None for k in self.INPUTS]
+        check(lib().dpd_problem_grad(self.h, L.data_ptr(), R.data_ptr(), float(scale), *ptr))
+        self.ctx.synchronize()
+        return out
+
     def diag(self):
         torch = _torch()
         d = torch.empty(self.dim, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
@@ -362,13 +415,15 @@ class Problem:
         check(lib().dpd_problem_guess(self.h, nroot, R.data_ptr(), e.ctypes.data_as(_lib.c_double_p)))
         return R, e
 
-    def davidson(self, nroot, conv=1e-8, max_dim=20, max_cyc=50, lowest=False, final_check=True, R=None):
-        """Non-Hermitian Davidson (lib_cc/mod_davidson.f90:40-561)."""
+    def davidson(self, nroot, conv=1e-8, max_dim=20, max_cyc=50, lowest=False, final_check=True, R=None,
+                 transpose=False):
+        """Non-Hermitian Davidson (lib_cc/mod_davidson.f90:40-561); transpose=True
+        solves with J^T (left eigenvectors)."""
         if R is None:
             R, _ = self.guess(nroot)
         o = _lib.DavidsonOpts_()
         o.nroot, o.max_dim, o.max_cyc, o.conv = nroot, max_dim, max_cyc, conv
-        o.lowest, o.final_check = int(lowest), int(final_check)
+        o.lowest, o.final_check, o.transpose = int(lowest), int(final_check), int(transpose)
         w = np.zeros(nroot)
         rc = lib().dpd_davidson(self.h, ctypes.byref(o), R.data_ptr(), w.ctypes.data_as(_lib.c_double_p))
         if rc not in (0, _lib.DPD_ENOTCONV):

@@ -41,7 +41,8 @@ class SynthSpec_(ctypes.Structure):
 class DavidsonOpts_(ctypes.Structure):
     """dpd_davidson_opts"""
     _fields_ = [("nroot", i64), ("max_dim", i64), ("max_cyc", i64), ("conv", ctypes.c_double),
-                ("lowest", ctypes.c_int), ("final_check", ctypes.c_int), ("iterations", i64),
+                ("lowest", ctypes.c_int), ("final_check", ctypes.c_int), ("transpose", ctypes.c_int),
+                ("iterations", i64),
                 ("sigma_calls", i64), ("skipped", i64), ("residual", ctypes.c_double * 64), ("final_residual", ctypes.c_double * 64)]
 
 
@@ -119,6 +120,12 @@ _SIGS = {
     "dpd_problem_sigma": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i64]),
     "dpd_problem_diag": (ctypes.c_int, [c_void_p, c_void_p]),
     "dpd_problem_guess": (ctypes.c_int, [c_void_p, i64, c_void_p, c_double_p]),
+    "dpd_problem_sigma_t": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i64]),
+    "dpd_problem_tensor_len": (i64, [c_void_p, ctypes.c_char_p]),
+    "dpd_problem_get": (ctypes.c_int, [c_void_p, ctypes.c_char_p, c_void_p]),
+    "dpd_problem_set": (ctypes.c_int, [c_void_p, ctypes.c_char_p, c_void_p]),
+    "dpd_problem_refresh": (ctypes.c_int, [c_void_p]),
+    "dpd_problem_grad": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_double] + [c_void_p] * 7),
     "dpd_small_eig": (ctypes.c_int, [i64, c_double_p, c_double_p, c_double_p, c_double_p]),
     "dpd_davidson": (ctypes.c_int, [c_void_p, ctypes.POINTER(DavidsonOpts_), c_void_p, c_double_p]),
 }
