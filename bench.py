@@ -34,6 +34,9 @@ WORKLOADS = {
     "cfg4a": dict(nso=(30, 8, 14, 18, 26, 6, 10, 8), nalpha=12, nbeta=10, seed=2, nroot=16, group="D2h"),
     # mixed precision at C2v/180 (|R1|+|R2| = 5 981 115)
     "cfg3": dict(nso=(70, 18, 36, 56), nalpha=16, nbeta=14, seed=3, nroot=5, group="C2v"),
+    # gradient of the lowest SF-CPSd excitation energy w.r.t. the orbital
+    # energies and the integral blocks, checked by central differences
+    "cfg5": dict(nso=(40, 10, 20, 30), nalpha=10, nbeta=8, seed=5, nroot=1, group="C2v", kind="gradient"),
 }
 # MI355X dense MFMA peaks per ALGORITHMIC flop: FP64, FP32, and BF16x3 (three
 # BF16 MFMAs of the 2.5 PFLOP/s dense BF16 peak per algorithmic product)
@@ -62,6 +65,9 @@ def main():
                     help="counter-run mode: a few batched sigma calls in this precision (tile shape --tile), "
                          "then a plain 512 MiB device copy for calibrating FETCH_SIZE/WRITE_SIZE; no JSON line")
     ap.add_argument("--tile", type=int, default=-1, help="GEMM tile configuration for --probe (-1: autotuned)")
+    ap.add_argument("--fd-dirs", type=int, default=50, help="gradient workload: random integral directions")
+    ap.add_argument("--fd-h", type=float, default=1e-4, help="gradient workload: central-difference step")
+    ap.add_argument("--grad-conv", type=float, default=1e-9, help="gradient workload: FP64 residual tolerance")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
@@ -77,6 +83,9 @@ def main():
     nroot = cfg["nroot"]
     if args.probe:
         probe(cfg, args)
+        return
+    if cfg.get("kind") == "gradient":
+        gradient_bench(cfg, args)
         return
     lowest = args.tracking == "lowest"
     t0 = time.time()
@@ -214,6 +223,127 @@ def main():
                         "tflops": round(v["flops_total"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] > 0 else None}
                     for k, v in prof.items()},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def gradient_bench(cfg, args):
+    """Gradient of the lowest excitation energy (right and left Davidson solves,
+    then the adjoint contractions) in FP64 and FP32/BF16x3, timed; the FP64
+    gradient checked against central differences of the eigenvalue along seeded
+    random unit directions over the spin-allowed elements of the integral blocks
+    (and a few in the orbital energies), and the mixed-precision gradients
+    against the FP64 one."""
+    import numpy as np
+    import torch
+
+    import dpdhip
+    from dpdhip.gradient import excitation_energy
+
+    t0 = time.time()
+    ctx = dpdhip.Context(len(cfg["nso"]), 0)
+    prob = dpdhip.Problem(ctx, dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"]), "sf-cpsd", 1)
+    names = prob.INPUTS
+    base = {k: prob.get(k) for k in names}
+    log(f"problem ready: dim={prob.dim} setup {time.time() - t0:.1f}s")
+    solve = dict(nroot=cfg["nroot"], conv=args.grad_conv, max_dim=args.max_dim, max_cyc=args.max_cyc)
+
+    def run(prec, conv, x0=None, warm=None):
+        prob.set_precision(prec)
+        x = {k: (x0[k] if x0 else base[k]).clone().requires_grad_(True) for k in names}
+        info = {}
+        extra = {"R0": warm["R"], "L0": warm["L"]} if warm else {}
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        w = excitation_energy(prob, x, info=info, **dict(solve, conv=conv), **extra)
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        w.backward()
+        torch.cuda.synchronize()
+        c = time.perf_counter()
+        return float(w.detach()), {k: x[k].grad for k in names}, info, b - a, c - b
+
+    res = {}
+    for prec, conv in (("fp64", args.grad_conv), ("fp32", args.conv_mixed), ("bf16x3", args.conv_mixed)):
+        for _ in range(args.warmup):
+            run(prec, conv)
+        tf, tb = [], []
+        for _ in range(args.steps):
+            w, g, info, a, b = run(prec, conv)
+            tf.append(a)
+            tb.append(b)
+        res[prec] = dict(w=w, g=g, info=info, forward=float(np.median(tf)), backward=float(np.median(tb)))
+        log(f"{prec}: w={w:.12f} forward {np.median(tf):.3f}s backward {np.median(tb):.3f}s "
+            f"iterations right/left {info['right']['iterations']}/{info['left']['iterations']}")
+    g64 = res["fp64"]["g"]
+    gnorm = sum(float(v.norm() ** 2) for v in g64.values()) ** 0.5
+    gn_eri = sum(float(g64[k].norm() ** 2) for k in names if k != "eorb") ** 0.5
+    for prec in ("fp32", "bf16x3"):
+        err = sum(float((res[prec]["g"][k] - g64[k]).norm() ** 2) for k in names) ** 0.5
+        res[prec]["grad_rel_err_vs_fp64"] = err / gnorm
+        res[prec]["dw_vs_fp64"] = abs(res[prec]["w"] - res["fp64"]["w"])
+
+    # central differences along seeded random unit directions
+    gen = torch.Generator(device="cuda:0").manual_seed(cfg["seed"])
+    h = args.fd_h
+    warm = res["fp64"]["info"]
+    fd_rows = []
+    t_fd = time.time()
+    ndir_e = 5
+    for i in range(args.fd_dirs + ndir_e):
+        on_eorb = i >= args.fd_dirs
+        keys = ["eorb"] if on_eorb else [k for k in names if k != "eorb"]
+        # supported on the spin-allowed elements (nonzero in the inputs): a change
+        # of a spin-forbidden element couples other spin sectors, whose lowest
+        # state is another one
+        d = {k: torch.randn(base[k].numel(), dtype=torch.float64, device="cuda:0", generator=gen)
+             * (base[k] != 0) for k in keys}
+        nrm = sum(float(v.norm() ** 2) for v in d.values()) ** 0.5
+        d = {k: v / nrm for k, v in d.items()}
+        vals = []
+        for sgn in (1.0, -1.0):
+            xs = {k: (base[k] + sgn * h * d[k]) if k in d else base[k] for k in names}
+            vals.append(run("fp64", args.grad_conv, xs, warm)[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        an = sum(float(g64[k] @ d[k]) for k in keys)
+        scale = gnorm if on_eorb else gn_eri
+        fd_rows.append(dict(space="eorb" if on_eorb else "eri", fd=fd, analytic=an, abs_err=abs(fd - an),
+                            err_over_gnorm=abs(fd - an) / scale))
+        if (i + 1) % 10 == 0:
+            log(f"directions {i + 1}/{args.fd_dirs + ndir_e} done ({time.time() - t_fd:.1f}s)")
+    eri = [r for r in fd_rows if r["space"] == "eri"]
+    eo = [r for r in fd_rows if r["space"] == "eorb"]
+    line = {
+        "metric": "gradient of the lowest SF-CPSd excitation energy (right+left Davidson + adjoint contractions)",
+        "value": round(res["fp64"]["forward"] + res["fp64"]["backward"], 6),
+        "unit": "s/gradient",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * (res["fp64"]["forward"] + res["fp64"]["backward"]), 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "fp64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": f"cfg5: SF-CPSd {cfg['group']} {sum(cfg['nso'])} spatial orbitals {list(cfg['nso'])} "
+                               f"n_alpha={cfg['nalpha']} n_beta={cfg['nbeta']}, lowest root, dim {prob.dim}",
+                   "inputs": {k: int(base[k].numel()) for k in names}, "conv_fp64": args.grad_conv,
+                   "conv_mixed": args.conv_mixed},
+        "w_fp64": res["fp64"]["w"],
+        "precisions": {p: {"forward_s": round(r["forward"], 6), "backward_s": round(r["backward"], 6),
+                           "w": r["w"], "iterations_right": r["info"]["right"]["iterations"],
+                           "iterations_left": r["info"]["left"]["iterations"],
+                           **({"grad_rel_err_vs_fp64": r["grad_rel_err_vs_fp64"], "abs_dw_vs_fp64": r["dw_vs_fp64"]}
+                              if p != "fp64" else {})} for p, r in res.items()},
+        "grad_norm": {"all": gnorm, "eri": gn_eri, "eorb": float(g64["eorb"].norm())},
+        "finite_difference": {"h": h, "eri_directions": len(eri), "eorb_directions": len(eo),
+                              "max_abs_err_eri": max(r["abs_err"] for r in eri),
+                              "max_err_over_gnorm_eri": max(r["err_over_gnorm"] for r in eri),
+                              "median_abs_analytic_eri": float(np.median([abs(r["analytic"]) for r in eri])),
+                              "max_abs_err_eorb": max(r["abs_err"] for r in eo),
+                              "max_err_over_gnorm_eorb": max(r["err_over_gnorm"] for r in eo),
+                              "seconds": round(time.time() - t_fd, 1)},
     }
     print(json.dumps(line), flush=True)
 

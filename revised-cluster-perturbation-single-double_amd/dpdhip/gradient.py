@@ -19,6 +19,10 @@ theta are the stored elements of the inputs: the orbital energies ("eorb",
 occupied irreps then virtual) and the integral blocks "iabj", "ijka", "ijkl",
 "abij", "abci", "abcd" in their packed layouts, each block an independent
 input (the reference keeps them as separate arrays, CISdoo.SD.SF/allocate_mem.f90).
+Elements that vanish by spin conservation (an alpha and a beta orbital in
+positions the integral cannot connect) are zero in the inputs; the gradient
+has entries there too, but a change of such an element couples the spin-flip
+sector to others, so differences along it follow another state.
 """
 from __future__ import annotations
 
@@ -49,10 +53,12 @@ class ExcitationEnergy(torch.autograd.Function):
         root = int(solve.get("root", 0))
         kw = dict(conv=solve.get("conv", 1e-9), max_dim=solve.get("max_dim", 20),
                   max_cyc=solve.get("max_cyc", 100), lowest=True)
-        wr, R, ir = problem.davidson(nroot, R=solve.get("R0"), **kw)
+        # warm starts are copied: the solver overwrites its guesses in place
+        R0, L0 = solve.get("R0"), solve.get("L0")
+        wr, R, ir = problem.davidson(nroot, R=None if R0 is None else R0.clone(), **kw)
         if not ir["converged"]:
             raise RuntimeError(f"right eigenproblem did not converge: {ir['residual']}")
-        wl, L, il = problem.davidson(nroot, R=solve.get("L0"), transpose=True, **kw)
+        wl, L, il = problem.davidson(nroot, R=None if L0 is None else L0.clone(), transpose=True, **kw)
         if not il["converged"]:
             raise RuntimeError(f"left eigenproblem did not converge: {il['residual']}")
         k, dw = _pick(wr[root], wl)
@@ -61,13 +67,17 @@ class ExcitationEnergy(torch.autograd.Function):
         n = problem.dim
         Rv = R[root * n:(root + 1) * n].clone()
         Lv = L[k * n:(k + 1) * n].clone()
+        # two-sided Rayleigh quotient: its error is the product of the left and
+        # right residuals, so it is the eigenvalue to far below the tolerance
+        lr = torch.dot(Lv, Rv)
+        w = torch.dot(Lv, problem.sigma(Rv)) / lr
         ctx.problem = problem
         ctx.names = names
         ctx.save_for_backward(Lv, Rv)
-        ctx.info = {"right": ir, "left": il, "w_left": wl[k]}
+        ctx.info = {"right": ir, "left": il, "w_right": float(wr[root]), "w_left": float(wl[k]), "R": Rv, "L": Lv}
         if solve.get("info") is not None:
             solve["info"].update(ctx.info)
-        return torch.tensor(wr[root], dtype=torch.float64, device=Rv.device)
+        return w
 
     @staticmethod
     def backward(ctx, gw):
