@@ -160,6 +160,10 @@ int dpd_isymtrso(dpd_ctx *ctx, const double *in, double *out, const int64_t *pop
 int dpd_itranspso(dpd_ctx *ctx, const double *in, double *out, const int64_t *popp,
                   const int64_t *popq, int64_t ioppq, const int64_t *popr, const int64_t *pops,
                   int64_t ioprs, int64_t sym);
+/* libsrint/iassymso.f:1-142: antisymmetrise (iop 0,1,2,4,5) or symmetrise (iop 3)
+ * a pair of a 4-index tensor and compress where the output pair is packed. */
+int dpd_iassymso(dpd_ctx *ctx, const double *in, double *out, int64_t sym, const int64_t *popp,
+                 const int64_t *popq, const int64_t *popr, const int64_t *pops, int64_t iop);
 /* libsrint/sstgenso.f90:22-45 (sstgenso1), type4 e.g. "3142" */
 int dpd_sstgenso(dpd_ctx *ctx, const double *win, double *wout, const int64_t *pop1,
                  const int64_t *pop2, const int64_t *pop3, const int64_t *pop4, int64_t irrepx,

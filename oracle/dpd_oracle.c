@@ -15,6 +15,7 @@
  *   libsrint/icompso.f:1-93 (+sqsymso.f ...)                    -> orc_icompso
  *   libsrint/isymtrso.f:1-120 (+symtr1so.f, symtr3so.f)         -> orc_isymtrso
  *   libsrint/itranspso.f:1-53                                   -> orc_itranspso
+ *   libsrint/iassymso.f:1-142                                   -> orc_iassymso
  *   libsrint/sstgenso.f90:22-206                                -> orc_sstgenso
  *   libsrint/vv2v_dpd_444.f90:55-232 (dispatcher)               -> orc_vv2v_dpd_444
  *   libsrint/vv2v_dpd_444_d.f90:120-235 (fast path)             -> orc_vv2v_dpd_444_fast
@@ -437,6 +438,40 @@ int orc_itranspso(const double *in, double *out, const i64 *popp, const i64 *pop
   make_desc(&dout, popr, pops, popp, popq, ioprs, ioppq, sym);
   int perm[4] = {2, 3, 0, 1};
   run_sort(&din, in, &dout, out, perm);
+  return 0;
+}
+
+/* iassymso.f:1-142 (assym2so.f, assym3so.f, dasymrhfso.f, dsymrhfso.f):
+ * out = in -/+ in with one or both pairs swapped, compressed where the output
+ * pair is packed:
+ *   iop 0: (p<q,r,s) -> (p<q,r<s), swap rs     iop 1: (p,q,r,s) -> (p,q,r<s), swap rs
+ *   iop 2: (p,q,r,s) -> (p,q,r,s), swap both   iop 3: as 2 with +
+ *   iop 4: (p,q,r<s) -> (p<q,r<s), swap pq     iop 5: (p,q,r,s) -> (p<q,r,s), swap pq */
+int orc_iassymso(const double *in, double *out, i64 sym, const i64 *popp, const i64 *popq,
+                 const i64 *popr, const i64 *pops, i64 iop) {
+  static const int io[6][4] = {{0, 1, 0, 0}, {1, 1, 1, 0}, {1, 1, 1, 1},
+                               {1, 1, 1, 1}, {1, 0, 0, 0}, {1, 1, 0, 1}};
+  if (iop < 0 || iop > 5) return set_err(-1, "error of iop in iassymso");
+  int spq = (iop >= 2), srs = (iop <= 3);
+  if (spq && !pops_equal(popp, popq)) return set_err(-1, "iassymso: popp /= popq");
+  if (srs && !pops_equal(popr, pops)) return set_err(-1, "iassymso: popr /= pops");
+  orc_desc din, dout;
+  make_desc(&din, popp, popq, popr, pops, io[iop][0], io[iop][1], sym);
+  make_desc(&dout, popp, popq, popr, pops, io[iop][2], io[iop][3], sym);
+  orc_lay Li, Lo;
+  lay_init(&Li, &din);
+  lay_init(&Lo, &dout);
+  const double sgn = iop == 3 ? 1.0 : -1.0;
+  i64 n = orc_desc_len(&dout);
+#pragma omp parallel for schedule(static) num_threads(max_threads())
+  for (i64 o = 0; o < n; o++) {
+    i64 h[4], x[4], h2[4], x2[4];
+    decode_elem(&Lo, o, h, x);
+    for (int i = 0; i < 4; i++) { h2[i] = h[i]; x2[i] = x[i]; }
+    if (spq) { h2[0] = h[1]; x2[0] = x[1]; h2[1] = h[0]; x2[1] = x[0]; }
+    if (srs) { h2[2] = h[3]; x2[2] = x[3]; h2[3] = h[2]; x2[3] = x[2]; }
+    out[o] = fetch_packed(&Li, in, h, x) + sgn * fetch_packed(&Li, in, h2, x2);
+  }
   return 0;
 }
 

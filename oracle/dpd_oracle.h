@@ -65,6 +65,8 @@ int orc_isymtrso(const double *in, double *out, const i64 *popp, const i64 *popq
                  i64 icpq, const i64 *popr, const i64 *pops, i64 ioprs, i64 icrs, i64 sym);
 int orc_itranspso(const double *in, double *out, const i64 *popp, const i64 *popq, i64 ioppq,
                   const i64 *popr, const i64 *pops, i64 ioprs, i64 sym);
+int orc_iassymso(const double *in, double *out, i64 sym, const i64 *popp, const i64 *popq,
+                 const i64 *popr, const i64 *pops, i64 iop);
 int orc_sstgenso(const double *win, double *wout, const i64 *pop1, const i64 *pop2,
                  const i64 *pop3, const i64 *pop4, i64 irrepx, const char *type4);
 

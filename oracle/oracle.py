@@ -239,6 +239,16 @@ def itranspso(x, popp, popq, ioppq, popr, pops, ioprs, sym):
     return out[:len(x)]
 
 
+def iassymso(x, sym, popp, popq, popr, pops, iop):
+    """libsrint/iassymso.f:1-142"""
+    io = {0: (0, 1, 0, 0), 1: (1, 1, 1, 0), 2: (1, 1, 1, 1), 3: (1, 1, 1, 1), 4: (1, 0, 0, 0), 5: (1, 1, 0, 1)}[iop]
+    out_len = idsymsoc(sym, popp, popq, io[2], popr, pops, io[3])
+    out = np.zeros(max(out_len, 1))
+    ps = [_lp(p) for p in (popp, popq, popr, pops)]
+    _check(lib().orc_iassymso(_dp(x), _dp(out), i64(sym), ps[0][1], ps[1][1], ps[2][1], ps[3][1], i64(iop)))
+    return out[:out_len]
+
+
 def sstgenso(x, pop1, pop2, pop3, pop4, irrepx, type4):
     out = np.zeros(max(len(x), 1))
     ps = [_lp(p) for p in (pop1, pop2, pop3, pop4)]

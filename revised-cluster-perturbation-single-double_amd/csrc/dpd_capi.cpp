@@ -336,6 +336,46 @@ int dpd_itranspso(dpd_ctx* c, const double* in, double* out, const int64_t* popp
   return run_sort(c, dout, din, perm, in, out);
 }
 
+// libsrint/iassymso.f:1-142 (assym2so.f, assym3so.f, dasymrhfso.f, dsymrhfso.f):
+//   iop 0: out(p<q,r<s) = in(p<q,r,s) - in(p<q,s,r)
+//   iop 1: out(p,q,r<s) = in(p,q,r,s) - in(p,q,s,r)
+//   iop 2: out(p,q,r,s) = in(p,q,r,s) - in(q,p,s,r)
+//   iop 3: out(p,q,r,s) = in(p,q,r,s) + in(q,p,s,r)
+//   iop 4: out(p<q,r<s) = in(p,q,r<s) - in(q,p,r<s)
+//   iop 5: out(p<q,r,s) = in(p,q,r,s) - in(q,p,r,s)
+// one two-term gather; a pair that is mirrored must have equal populations
+int dpd_iassymso(dpd_ctx* c, const double* in, double* out, int64_t sym, const int64_t* popp,
+                 const int64_t* popq, const int64_t* popr, const int64_t* pops, int64_t iop) {
+  CHECK_CTX(c);
+  if (iop < 0 || iop > 5) return fail(DPD_EINVAL, "error of iop in iassymso");
+  const bool swap_pq = iop >= 2 && iop != 1, swap_rs = iop <= 3;
+  if (swap_pq && !same8(popp, popq)) return fail(DPD_EINVAL, "iassymso: popp /= popq");
+  if (swap_rs && !same8(popr, pops)) return fail(DPD_EINVAL, "iassymso: popr /= pops");
+  static const int io[6][4] = {// in iop pq, rs; out iop pq, rs
+                               {0, 1, 0, 0}, {1, 1, 1, 0}, {1, 1, 1, 1},
+                               {1, 1, 1, 1}, {1, 0, 0, 0}, {1, 1, 0, 1}};
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  if (sym < 1 || sym > c->sym.nirrep) return fail(DPD_EINVAL, "symmetry out of range");
+  MapArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.in = make_layout(c->sym, mk(popp, popq, popr, pops, io[iop][0], io[iop][1], (int)sym));
+  a.out = make_layout(c->sym, mk(popp, popq, popr, pops, io[iop][2], io[iop][3], (int)sym));
+  a.nterm = 2;
+  const int8_t id[4] = {0, 1, 2, 3};
+  const int8_t sw[4] = {(int8_t)(swap_pq ? 1 : 0), (int8_t)(swap_pq ? 0 : 1), (int8_t)(swap_rs ? 3 : 2),
+                        (int8_t)(swap_rs ? 2 : 3)};
+  for (int i = 0; i < 4; i++) {
+    a.term[0].perm[i] = id[i];
+    a.term[1].perm[i] = sw[i];
+  }
+  a.term[0].coef = 1.0;
+  a.term[1].coef = iop == 3 ? 1.0 : -1.0;
+  a.alpha = 1.0;
+  a.beta = 0.0;
+  a.nbatch = 1;
+  return gather(c, a, in, out);
+}
+
 int dpd_sstgenso(dpd_ctx* c, const double* win, double* wout, const int64_t* pop1, const int64_t* pop2,
                  const int64_t* pop3, const int64_t* pop4, int64_t irrepx, const char* type4) {
   CHECK_CTX(c);

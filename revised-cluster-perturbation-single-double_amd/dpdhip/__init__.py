@@ -218,6 +218,12 @@ def itranspso(ctx, x, out, popp, popq, ioppq, popr, pops, ioprs, sym):
                               ioprs, sym))
 
 
+def iassymso(ctx, x, out, sym, popp, popq, popr, pops, iop):
+    """libsrint/iassymso.f:1-142 (pair antisymmetriser / symmetriser)"""
+    ps = [_i64(p) for p in (popp, popq, popr, pops)]
+    check(lib().dpd_iassymso(ctx.h, x.data_ptr(), out.data_ptr(), sym, ps[0][1], ps[1][1], ps[2][1], ps[3][1], iop))
+
+
 def sstgenso(ctx, x, out, pop1, pop2, pop3, pop4, irrepx, type4):
     ps = [_i64(p) for p in (pop1, pop2, pop3, pop4)]
     check(lib().dpd_sstgenso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ps[2][1], ps[3][1], irrepx,

@@ -100,6 +100,7 @@ _SIGS = {
                                     i64, i64]),
     "dpd_itranspso": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_i64_p, c_i64_p, i64, c_i64_p, c_i64_p, i64,
                                      i64]),
+    "dpd_iassymso": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i64, c_i64_p, c_i64_p, c_i64_p, c_i64_p, i64]),
     "dpd_sstgenso": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_i64_p, c_i64_p, c_i64_p, c_i64_p, i64,
                                     ctypes.c_char_p]),
     "dpd_xgemm": (ctypes.c_int, [c_void_p, ctypes.c_char_p, ctypes.c_char_p, i64, i64, i64, ctypes.c_double,

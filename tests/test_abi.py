@@ -82,6 +82,54 @@ def test_host_context_metadata_matches_oracle():
         _lib.lib().dpd_ctx_destroy(h)
 
 
+def _fortran_symbols():
+    txt = open(os.path.join(ROOT, "include", "dpd_fortran.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b([a-z0-9_]+_)\s*\(", txt)))
+
+
+def test_library_exports_fortran_entry_points():
+    """the reference's external names (lower case, trailing underscore) that
+    include/dpd_fortran.h declares"""
+    L = _lib.lib()
+    syms = _fortran_symbols()
+    assert {"vv2v_dpd_444_", "vv2v_dpd_424_", "vv2v_dpd_442_", "iexpso_", "icompso_", "isymtrso_", "itranspso_",
+            "iassymso_", "sstgenso1_", "xgemmso_", "irpdso_", "idsymsoc_", "isymoffso_"} <= set(syms)
+    for s in syms:
+        assert hasattr(L, s), s
+
+
+def test_fortran_metadata_by_reference_host_only():
+    """irpdso_/idsymsoc_/isymoffso_ called the Fortran way (every argument by
+    reference, int64) on a host-only set-up agree with the oracle"""
+    L = _lib.lib()
+    i64 = ctypes.c_int64
+    ref = lambda v: ctypes.byref(i64(v))
+    arr = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64)).ctypes.data_as(_lib.c_i64_p)
+    for fn in ("irpdso_", "idsymsoc_", "isymoffso_"):
+        getattr(L, fn).restype = i64
+    dir_f = np.ascontiguousarray(O.xor_dirprd(4).reshape(8, 8).T.reshape(-1))  # column major
+    L.dpd_fortran_init_(ref(4), dir_f.ctypes.data_as(_lib.c_i64_p), ref(-1))
+    try:
+        s = O.Sym(4)
+        s.activate()
+        pa = np.array([5, 1, 3, 2, 0, 0, 0, 0])
+        pb = np.array([4, 2, 0, 3, 0, 0, 0, 0])
+        for sym in range(1, 5):
+            for iop in (0, 1, 2):
+                pq = pb if iop == 1 else pa
+                assert L.irpdso_(ref(sym), arr(pa), arr(pq), ref(iop)) == O.irpdso(sym, pa, pq, iop)
+                for iop2 in (0, 1, 2):
+                    rs = pb if iop2 == 1 else pa
+                    assert L.idsymsoc_(ref(sym), arr(pa), arr(pq), ref(iop), arr(pa), arr(rs), ref(iop2)) == \
+                        O.idsymsoc(sym, pa, pq, iop, pa, rs, iop2)
+                for irr in range(1, 5):
+                    assert L.isymoffso_(ref(irr), ref(sym), arr(pa), arr(pq), ref(iop)) == \
+                        O.isymoffso(irr, sym, pa, pq, iop)
+    finally:
+        L.dpd_fortran_finalize_()
+
+
 def test_error_paths_without_gpu():
     h = _host_ctx(4)
     t = _lib.Tensor_()

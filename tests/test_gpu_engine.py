@@ -305,6 +305,14 @@ def test_sort_kernels(torch, sym):
         out = torch.zeros(len(y) + 1, dtype=torch.float64, device="cuda:0")
         dpdhip.itranspso(c, dev(torch, y), out, V, Oc, 1, V, Oc, 1, irrepx)
         assert np.array_equal(out[: len(y)].cpu().numpy(), ref)
+        io = {0: (0, 1), 1: (1, 1), 2: (1, 1), 3: (1, 1), 4: (1, 0), 5: (1, 1)}
+        for iop in range(6):
+            di = O.Desc([V, V, Oc, Oc], io[iop], irrepx)
+            z = O.random_dpd(sym, di, rng)
+            ref = O.iassymso(z, irrepx, V, V, Oc, Oc, iop)
+            out = torch.zeros(len(ref) + 1, dtype=torch.float64, device="cuda:0")
+            dpdhip.iassymso(c, dev(torch, z), out, irrepx, V, V, Oc, Oc, iop)
+            assert np.array_equal(out[: len(ref)].cpu().numpy(), ref), iop
         for t in ("3142", "2413", "4321", "1243"):
             ref = O.sstgenso(y, V, Oc, V, Oc, irrepx, t)
             out = torch.zeros(len(y) + 1, dtype=torch.float64, device="cuda:0")

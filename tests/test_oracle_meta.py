@@ -193,3 +193,27 @@ def test_packed_iop0_has_antisymmetric_dense(sym):
     assert np.allclose(D, -D.transpose(0, 1, 3, 2))
     n = D.shape[0]
     assert np.all(D[np.arange(n), np.arange(n)] == 0)
+
+
+@pytest.mark.parametrize("iop", range(6))
+def test_iassymso_dense(sym, iop):
+    """libsrint/iassymso.f:1-142 against dense arrays: out = in -/+ in with the
+    rs pair (iop 0, 1), both pairs (2, 3) or the pq pair (4, 5) swapped."""
+    pa, pb = _pops(sym)
+    io = {0: (0, 1, 0, 0), 1: (1, 1, 1, 0), 2: (1, 1, 1, 1), 3: (1, 1, 1, 1), 4: (1, 0, 0, 0), 5: (1, 1, 0, 1)}[iop]
+    for irrepx in range(1, sym.nirrep + 1):
+        din = O.Desc([pa, pa, pb, pb], io[:2], irrepx)
+        dout = O.Desc([pa, pa, pb, pb], io[2:], irrepx)
+        x = _rand(sym, din, 3 + irrepx)
+        D = O.unpack(sym, din, x)
+        if iop in (0, 1):
+            E = D - D.transpose(0, 1, 3, 2)
+        elif iop == 2:
+            E = D - D.transpose(1, 0, 3, 2)
+        elif iop == 3:
+            E = D + D.transpose(1, 0, 3, 2)
+        else:
+            E = D - D.transpose(1, 0, 2, 3)
+        y = O.iassymso(x, irrepx, pa, pa, pb, pb, iop)
+        assert len(y) == dout.length()
+        assert np.allclose(O.unpack(sym, dout, y), E, rtol=0, atol=1e-14)
