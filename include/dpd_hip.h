@@ -285,6 +285,23 @@ int dpd_small_eig(int64_t n, const double *A_rowmajor, double *wr, double *wi, d
  * eigenvectors on exit; w (host, nroot) receives the eigenvalues. */
 int dpd_davidson(dpd_problem *prob, dpd_davidson_opts *opts, double *R, double *w);
 
+/* ---- batches of independent problems ------------------------------------
+ * No single reference routine: the reference solves one problem per run.
+ * sigma of n problems (same context, method and precision; sizes differ):
+ * every contraction of all of them runs in one grouped GEMM launch (tile shape
+ * timed for the group on first use).  R[i], sigma[i]: device, nvec[i] vectors. */
+int dpd_problem_sigma_group(dpd_problem *const *probs, const double *const *R, double *const *sigma,
+                            const int64_t *nvec, int64_t n);
+/* n Davidson solves (lib_cc/mod_davidson.f90:40-561 each) advanced in lockstep:
+ * the sigma requests of one step are applied together.  flags bit 0: grouped
+ * GEMM launches (else one problem after another); bit 1: the grouped sigma step
+ * is captured once per shape in a hipGraph and replayed.  opts[i], R[i], w[i]
+ * as for dpd_davidson.  stats (host, 4, may be NULL): batched sigma steps, graph
+ * replays, graphs captured, captures abandoned (run eagerly instead).  Returns
+ * DPD_ENOTCONV if any member did not converge. */
+int dpd_davidson_batch(dpd_problem *const *probs, int64_t n, dpd_davidson_opts *opts, double *const *R,
+                       double *const *w, int64_t flags, int64_t *stats);
+
 #ifdef __cplusplus
 }
 #endif

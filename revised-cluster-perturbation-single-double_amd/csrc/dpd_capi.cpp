@@ -470,6 +470,7 @@ int dpd_ctx_set_gemm_config(dpd_ctx* c, int cfg, int packing) {
   if (c->device >= 0) DPD_HIP_TRY(hipStreamSynchronize(c->stream));
   c->gemm_cfg = cfg;
   c->packing = packing;
+  c->group_cache.clear();  // holds pointers into contract_cache
   c->contract_cache.clear();
   c->engine_cache.clear();
   return DPD_OK;

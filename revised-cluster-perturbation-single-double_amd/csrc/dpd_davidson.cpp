@@ -24,7 +24,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "dpd_sigma.hpp"
@@ -307,7 +313,16 @@ extern "C" int dpd_small_eig(int64_t n, const double* A_rowmajor, double* wr, do
   return small_eig((int)n, A_rowmajor, wr, wi, V_colmajor);
 }
 
+// sigma of nv vectors for one Davidson; null: the problem's own operator
+typedef std::function<int(const double* x, double* y, int nv)> SigmaHook;
+
+static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out, const SigmaHook* hook);
+
 extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out) {
+  return davidson_run(P, o, R, w_out, nullptr);
+}
+
+static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out, const SigmaHook* hook) {
   if (!P || !o || !R || !w_out) return fail(DPD_EINVAL, "dpd_davidson: null argument");
   if (P->stale) return fail(DPD_EINVAL, "problem inputs changed: call dpd_problem_refresh first");
   dpd_ctx* c = P->ctx;
@@ -353,6 +368,7 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
   int64_t nsig = 0, nskip = 0;
   auto sigma = [&](const double* x, double* y, int nv) -> int {
     nsig += nv;
+    if (hook) return (*hook)(x, y, nv);
     return o->transpose ? problem_sigma_t(P, x, y, nv) : problem_sigma(P, x, y, nv);
   };
   // all dots X_i . Y_j (i < nx, j < ny) -> host[i + j*nx], one synchronisation
@@ -647,4 +663,233 @@ extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, dou
   o->skipped = nskip;
   if ((rc = dsync(c))) return rc;
   return converged ? DPD_OK : DPD_ENOTCONV;
+}
+
+// ---------------------------------------------------------------------------
+// Batched Davidson: n independent problems solved in lockstep.  Each problem's
+// Davidson runs in its own host thread; its sigma requests wait at a barrier
+// until every unfinished problem has one, and the last to arrive applies the
+// operator of all of them at once (problem_sigma_group: one grouped GEMM
+// launch per contraction), optionally replaying a hipGraph of that step.
+// The reference solves one problem per program run (lib_cc/mod_davidson.f90).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct SigmaReq {
+  int id = 0;
+  const double* x = nullptr;
+  double* y = nullptr;
+  int nv = 0;
+  bool done = false;
+  int rc = 0;
+};
+
+struct Batcher {
+  std::mutex m;
+  std::condition_variable cv;
+  int active = 0;
+  std::vector<SigmaReq*> pending;
+  std::function<int(std::vector<SigmaReq*>&)> run;
+  std::string err;  // message of a failed batch (set in the thread that ran it)
+
+  int submit(SigmaReq* r) {
+    std::unique_lock<std::mutex> lk(m);
+    r->done = false;
+    pending.push_back(r);
+    if ((int)pending.size() == active)
+      flush();
+    else
+      cv.wait(lk, [&] { return r->done; });
+    return r->rc;
+  }
+  // with the lock held: every unfinished problem waits in submit
+  void flush() {
+    std::vector<SigmaReq*> batch;
+    batch.swap(pending);
+    std::sort(batch.begin(), batch.end(), [](const SigmaReq* a, const SigmaReq* b) { return a->id < b->id; });
+    const int rc = run(batch);
+    if (rc) err = dpd_last_error();
+    for (SigmaReq* q : batch) {
+      q->rc = rc;
+      q->done = true;
+    }
+    cv.notify_all();
+  }
+  void leave() {
+    std::unique_lock<std::mutex> lk(m);
+    active--;
+    if (active > 0 && (int)pending.size() == active) flush();
+  }
+};
+
+struct GraphCache {
+  std::unordered_map<std::string, int> seen;
+  std::unordered_map<std::string, hipGraphExec_t> exec;
+  std::vector<hipGraph_t> graphs;
+  ~GraphCache() {
+    for (auto& e : exec) (void)hipGraphExecDestroy(e.second);
+    for (hipGraph_t g : graphs) (void)hipGraphDestroy(g);
+  }
+};
+
+}  // namespace
+
+extern "C" int dpd_problem_sigma_group(dpd_problem* const* probs, const double* const* R, double* const* sigma,
+                                       const int64_t* nvec, int64_t n) {
+  if (!probs || !R || !sigma || !nvec || n < 1) return fail(DPD_EINVAL, "dpd_problem_sigma_group: bad arguments");
+  for (int64_t i = 0; i < n; i++) {
+    if (!probs[i] || !R[i] || !sigma[i] || nvec[i] < 1) return fail(DPD_EINVAL, "dpd_problem_sigma_group: member %lld",
+                                                                    (long long)i);
+    if (probs[i]->stale) return fail(DPD_EINVAL, "problem inputs changed: call dpd_problem_refresh first");
+  }
+  DPD_HIP_TRY(hipSetDevice(probs[0]->ctx->device));
+  return problem_sigma_group(probs, R, sigma, nvec, (int)n);
+}
+
+extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davidson_opts* opts, double* const* R,
+                                  double* const* w, int64_t flags, int64_t* stats) {
+  int64_t stat[4] = {0, 0, 0, 0};  // batched steps, graph replays, captures, failed captures
+  if (!probs || !opts || !R || !w || n < 1 || n > 1024) return fail(DPD_EINVAL, "dpd_davidson_batch: bad arguments");
+  dpd_ctx* c = probs[0]->ctx;
+  for (int64_t i = 0; i < n; i++) {
+    if (!probs[i] || !R[i] || !w[i]) return fail(DPD_EINVAL, "dpd_davidson_batch: null member %lld", (long long)i);
+    if (probs[i]->ctx != c || probs[i]->method != probs[0]->method || probs[i]->prec != probs[0]->prec)
+      return fail(DPD_EINVAL, "dpd_davidson_batch: problems must share context, method and precision");
+    if (opts[i].transpose) return fail(DPD_EINVAL, "dpd_davidson_batch: transpose is not supported");
+  }
+  const bool grouped = flags & 1, graph = (flags & 2) && grouped;
+  DPD_HIP_TRY(hipSetDevice(c->device));
+  // a private stream (a graph cannot be captured on the legacy default stream),
+  // ordered after the caller's stream on entry and before it on exit
+  hipStream_t caller = c->stream, st = nullptr;
+  hipEvent_t ev = nullptr;
+  DPD_HIP_TRY(hipStreamCreate(&st));
+  DPD_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  DPD_HIP_TRY(hipEventRecord(ev, caller));
+  DPD_HIP_TRY(hipStreamWaitEvent(st, ev, 0));
+  c->stream = st;
+  // fixed staging vectors for graph replays (the solver's vectors move)
+  std::vector<DevBuf> X(graph ? n : 0), Y(graph ? n : 0);
+  int rc = DPD_OK;
+  for (int64_t i = 0; graph && i < n && !rc; i++) {
+    const size_t bytes = sizeof(double) * (size_t)probs[i]->dim * (size_t)std::max<int64_t>(1, opts[i].nroot);
+    if ((rc = X[i].alloc(bytes)) || (rc = Y[i].alloc(bytes))) break;
+  }
+  GraphCache gc;
+  Batcher B;
+  B.active = (int)n;
+  B.run = [&](std::vector<SigmaReq*>& reqs) -> int {
+    const int m = (int)reqs.size();
+    stat[0]++;
+    std::vector<dpd_problem*> ps(m);
+    std::vector<const double*> xs(m);
+    std::vector<double*> ys(m);
+    std::vector<int64_t> nv(m);
+    std::string key;
+    for (int i = 0; i < m; i++) {
+      ps[i] = probs[reqs[i]->id];
+      nv[i] = reqs[i]->nv;
+      xs[i] = graph ? (const double*)X[reqs[i]->id].p : reqs[i]->x;
+      ys[i] = graph ? (double*)Y[reqs[i]->id].p : reqs[i]->y;
+      key += std::to_string(reqs[i]->id) + ":" + std::to_string(nv[i]) + ",";
+    }
+    if (!grouped) {
+      for (int i = 0; i < m; i++) {
+        int r = problem_sigma(ps[i], xs[i], ys[i], nv[i]);
+        if (r) return r;
+      }
+      return DPD_OK;
+    }
+    if (!graph) return problem_sigma_group(ps.data(), xs.data(), ys.data(), nv.data(), m);
+    for (int i = 0; i < m; i++)
+      DPD_HIP_TRY(hipMemcpyAsync((void*)xs[i], reqs[i]->x, sizeof(double) * ps[i]->dim * nv[i],
+                                 hipMemcpyDeviceToDevice, st));
+    auto it = gc.exec.find(key);
+    int& seen = gc.seen[key];
+    if (it != gc.exec.end()) {
+      DPD_HIP_TRY(hipGraphLaunch(it->second, st));
+      stat[1]++;
+    } else if (seen < 2) {
+      // the first two times eagerly: plans, tiles (timed on the first call) and
+      // the panels and workspace of the chosen tiles are made here
+      seen++;
+      int r = problem_sigma_group(ps.data(), xs.data(), ys.data(), nv.data(), m);
+      if (r) return r;
+    } else {
+      bool ok = seen == 2;
+      if (ok) {
+        // capture; a step that still needs an allocation or a synchronisation
+        // fails the capture and stays eager
+        hipGraph_t g = nullptr;
+        DPD_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int r = problem_sigma_group(ps.data(), xs.data(), ys.data(), nv.data(), m);
+        hipError_t e = hipStreamEndCapture(st, &g);
+        ok = r == DPD_OK && e == hipSuccess && g;
+        if (ok) {
+          gc.graphs.push_back(g);
+          hipGraphExec_t ge = nullptr;
+          DPD_HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+          gc.exec.emplace(key, ge);
+          DPD_HIP_TRY(hipGraphLaunch(ge, st));
+          stat[1]++;
+          stat[2]++;
+        } else {
+          stat[3]++;
+          if (g) (void)hipGraphDestroy(g);
+          (void)hipGetLastError();
+          seen = 3;  // never again for this shape
+        }
+      }
+      if (!ok) {
+        int r = problem_sigma_group(ps.data(), xs.data(), ys.data(), nv.data(), m);
+        if (r) return r;
+      }
+    }
+    for (int i = 0; i < m; i++)
+      DPD_HIP_TRY(hipMemcpyAsync(reqs[i]->y, ys[i], sizeof(double) * ps[i]->dim * nv[i], hipMemcpyDeviceToDevice,
+                                 st));
+    return DPD_OK;
+  };
+  std::vector<int> rcs(n, DPD_OK);
+  std::vector<std::string> errs(n);
+  if (!rc) {
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int64_t i = 0; i < n; i++)
+      th.emplace_back([&, i] {
+        SigmaReq req;
+        req.id = (int)i;
+        SigmaHook hook = [&](const double* x, double* y, int nv) {
+          req.x = x;
+          req.y = y;
+          req.nv = nv;
+          const int r = B.submit(&req);
+          if (r) fail(r, "batched sigma: %s", B.err.c_str());
+          return r;
+        };
+        if (hipSetDevice(c->device) != hipSuccess)
+          rcs[i] = DPD_EHIP;
+        else
+          rcs[i] = davidson_run(probs[i], &opts[i], R[i], w[i], &hook);
+        if (rcs[i] && rcs[i] != DPD_ENOTCONV) errs[i] = dpd_last_error();
+        B.leave();
+      });
+    for (auto& t : th) t.join();
+  }
+  (void)hipStreamSynchronize(st);
+  c->stream = caller;
+  if (stats)
+    for (int k = 0; k < 4; k++) stats[k] = stat[k];
+  X.clear();
+  Y.clear();
+  (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(st);
+  if (rc) return rc;
+  // first hard error wins; otherwise not-converged if any member did not converge
+  int worst = DPD_OK;
+  for (int64_t i = 0; i < n; i++)
+    if (rcs[i] && rcs[i] != DPD_ENOTCONV) return fail(rcs[i], "member %lld: %s", (long long)i, errs[i].c_str());
+  for (int64_t i = 0; i < n; i++)
+    if (rcs[i] == DPD_ENOTCONV) worst = DPD_ENOTCONV;
+  return worst;
 }

@@ -703,11 +703,11 @@ static int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const Pa
 
 // packed-panel GEMM: static operands come from the variant's cache (packed on
 // first use), the others are packed into the workspace reserved by the caller
-int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double* C, double f1,
-                    double f2, hipEvent_t ev0, hipEvent_t ev1) {
-  GemmVariant& V = p.pk[mode];
+// panels of both operand sides of variant V: static operands from V's cache
+// (packed on first use), per-call operands packed into the workspace
+static int pack_sides(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int mode, const PackSrc& A, const PackSrc& B,
+                      const void* side_ptr[2]) {
   const size_t es = pk_esize(mode);
-  const void* side_ptr[2];
   const PackSrc* src[2] = {&A, &B};
   const int64_t elems[2] = {V.pa_elems, V.pb_elems};
   for (int sd = 0; sd < 2; sd++) {
@@ -736,6 +736,15 @@ int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const P
       side_ptr[sd] = w;
     }
   }
+  return DPD_OK;
+}
+
+int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double* C, double f1,
+                    double f2, hipEvent_t ev0, hipEvent_t ev1) {
+  GemmVariant& V = p.pk[mode];
+  const void* side_ptr[2];
+  int rc0 = pack_sides(c, p, V, mode, A, B, side_ptr);
+  if (rc0) return rc0;
   if (V.nwg == 0) {
     if (ev0) DPD_HIP_TRY(hipEventRecord(ev0, c->stream));
     if (ev1) DPD_HIP_TRY(hipEventRecord(ev1, c->stream));
@@ -1311,115 +1320,118 @@ static int build_contract_plan(const Sym& S, int kind, const Operand& o1, const 
   return build_gemm_plan(S, m1, m2, m3, nbatch, P.gemm);
 }
 
-int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
-                  double fact, int anti, int nbatch, int prec, int64_t* flops, const char* tag) {
+// one contraction between plan lookup and the GEMM: which operands are packed,
+// gathered or fused, where the (scratch) result goes
+struct CallPrep {
+  ContractPlan* P = nullptr;
+  bool packed = false, fuse1 = false, fuse2 = false, gx1 = false, gx2 = false;
+  int nb1 = 1, nb2 = 1;
+  const double* p1 = nullptr;
+  const double* p2 = nullptr;
+  const float* p1f = nullptr;
+  const float* p2f = nullptr;
+  double* p3 = nullptr;
+  double beta = 1.0;
+  PackSrc a, b;
+};
+
+static int plan_of(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, const Operand& o3, int nbatch,
+                   int anti, ContractPlan** out, std::string* key_out = nullptr) {
   std::string key = sig_of(kind, o1, o2, o3, nbatch, anti, 0);
   auto it = c->contract_cache.find(key);
-  ContractPlan* P;
   if (it == c->contract_cache.end()) {
     auto np = std::make_unique<ContractPlan>();
     np->gemm.cfg_req = c->gemm_cfg;
     int rc = build_contract_plan(c->sym, kind, o1, o2, o3, anti, nbatch, *np);
     if (rc) return rc;
-    P = np.get();
+    *out = np.get();
     c->contract_cache.emplace(key, std::move(np));
   } else {
-    P = it->second.get();
+    *out = it->second.get();
   }
-  if (flops) *flops += P->gemm.flops;
-  const int nb1 = o1.batched ? nbatch : 1, nb2 = o2.batched ? nbatch : 1;
+  if (key_out) *key_out = std::move(key);
+  return DPD_OK;
+}
+
+// decisions of a call and the workspace it needs
+static int call_setup(dpd_ctx* c, ContractPlan* P, const Operand& o1, const Operand& o2, int nbatch, int prec,
+                      CallPrep& r, size_t* need) {
+  r.P = P;
+  r.nb1 = o1.batched ? nbatch : 1;
+  r.nb2 = o2.batched ? nbatch : 1;
   // packed panels: always for BF16x3, otherwise when an operand is static
-  const bool packed = prec == PREC_BF16X3 || c->packing == 2 || (c->packing == 1 && (o1.static_id || o2.static_id));
+  r.packed = prec == PREC_BF16X3 || c->packing == 2 || (c->packing == 1 && (o1.static_id || o2.static_id));
   if (prec == PREC_BF16X3 && c->packing == 0) return fail(DPD_EINVAL, "BF16x3 needs packed panels (packing 0)");
   // packed panels read sorted constant operands straight from the unsorted
   // tensors (packed once); sorted per-call operands are expanded by the gather
   // kernel first (measured as fast as the fused sort for them, and simpler)
-  const uint64_t id1 = o1.static_id, id2 = o2.static_id;
-  const bool fuse1 = packed && P->exp1 && id1;
-  const bool fuse2 = packed && P->exp2 && id2;
-  const bool gx1 = P->exp1 && !fuse1, gx2 = P->exp2 && !fuse2;
-  size_t need = 0;
-  if (packed) {
+  r.fuse1 = r.packed && P->exp1 && o1.static_id;
+  r.fuse2 = r.packed && P->exp2 && o2.static_id;
+  r.gx1 = P->exp1 && !r.fuse1;
+  r.gx2 = P->exp2 && !r.fuse2;
+  size_t n = 0;
+  if (r.packed) {
     int rc0 = build_variant(P->gemm, prec);
     if (rc0) return rc0;
     const GemmVariant& V = P->gemm.pk[prec];
-    if (!id1) need += pk_esize(prec) * V.pa_elems + 256;
-    if (!id2) need += pk_esize(prec) * V.pb_elems + 256;
-    need += sizeof(double) * V.partial_elems + 256;
+    if (!o1.static_id) n += pk_esize(prec) * V.pa_elems + 256;
+    if (!o2.static_id) n += pk_esize(prec) * V.pb_elems + 256;
+    n += sizeof(double) * V.partial_elems + 256;
   }
-  if (gx1) need += sizeof(double) * P->len1x * nb1 + 256;
-  if (gx2) need += sizeof(double) * P->len2x * nb2 + 256;
-  if (P->epi_on) need += sizeof(double) * P->len3x * nbatch + 256;
-  if (!packed) need += sizeof(double) * P->gemm.partial_elems + 256;
-  ws_reset(c);
-  int rc = ws_reserve(c, need);
-  if (rc) return rc;
-  const double* p1 = o1.data;
-  const double* p2 = o2.data;
-  const float* p1f = o1.data_f32;
-  const float* p2f = o2.data_f32;
-  if (gx1) {
-    double* x = (double*)ws_take(c, sizeof(double) * P->len1x * nb1);
+  if (r.gx1) n += sizeof(double) * P->len1x * r.nb1 + 256;
+  if (r.gx2) n += sizeof(double) * P->len2x * r.nb2 + 256;
+  if (P->epi_on) n += sizeof(double) * P->len3x * nbatch + 256;
+  if (!r.packed) n += sizeof(double) * P->gemm.partial_elems + 256;
+  *need += n;
+  return DPD_OK;
+}
+
+// expansions into the workspace, the scratch result, the packing sources
+static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o2, const Operand& o3, int nbatch) {
+  ContractPlan* P = r.P;
+  r.p1 = o1.data;
+  r.p2 = o2.data;
+  r.p1f = o1.data_f32;
+  r.p2f = o2.data_f32;
+  if (r.gx1) {
+    double* x = (double*)ws_take(c, sizeof(double) * P->len1x * r.nb1);
     if (!x) return DPD_ENOMEM;
     MapArgs m = P->map1;
-    m.nbatch = nb1;
+    m.nbatch = r.nb1;
     m.in_bstride = o1.bstride;
     m.out_bstride = P->len1x;
-    rc = gather(c, m, o1.data, x);
+    int rc = gather(c, m, o1.data, x);
     if (rc) return rc;
-    p1 = x;
-    p1f = nullptr;
+    r.p1 = x;
+    r.p1f = nullptr;
   }
-  if (gx2) {
-    double* x = (double*)ws_take(c, sizeof(double) * P->len2x * nb2);
+  if (r.gx2) {
+    double* x = (double*)ws_take(c, sizeof(double) * P->len2x * r.nb2);
     if (!x) return DPD_ENOMEM;
     MapArgs m = P->map2;
-    m.nbatch = nb2;
+    m.nbatch = r.nb2;
     m.in_bstride = o2.bstride;
     m.out_bstride = P->len2x;
-    rc = gather(c, m, o2.data, x);
+    int rc = gather(c, m, o2.data, x);
     if (rc) return rc;
-    p2 = x;
-    p2f = nullptr;
+    r.p2 = x;
+    r.p2f = nullptr;
   }
   // scratch result: every element the epilogue reads lies in a block some GEMM
   // task writes (tasks without K segments write zeros), so it is written with
   // beta = 0 and needs no clearing
-  double* p3 = const_cast<double*>(o3.data);
-  double beta = 1.0;
+  r.p3 = const_cast<double*>(o3.data);
+  r.beta = 1.0;
   if (P->epi_on) {
-    p3 = (double*)ws_take(c, sizeof(double) * P->len3x * nbatch);
-    if (!p3) return DPD_ENOMEM;
-    beta = 0.0;
+    r.p3 = (double*)ws_take(c, sizeof(double) * P->len3x * nbatch);
+    if (!r.p3) return DPD_ENOMEM;
+    r.beta = 0.0;
   }
-  int use_f32 = (prec != PREC_F64);
-  const void* A = (use_f32 && p1f) ? (const void*)p1f : (const void*)p1;
-  const void* B = (use_f32 && p2f) ? (const void*)p2f : (const void*)p2;
-  ProfRec* pr = nullptr;
-  if (c->prof.on && tag && c->prof.used + 2 <= c->prof.pool.size()) {
-    // algorithmic bytes: both operands once as the kernel reads them (panels, or
-    // the tensors in their storage precision) and the result (read too when
-    // accumulated)
-    int64_t bytes = P->gemm.c_elems * 8 * (beta != 0.0 ? 2 : 1);
-    if (packed) {
-      const GemmVariant& V = P->gemm.pk[prec];
-      bytes += (V.pa_elems + V.pb_elems) * pk_esize(prec);
-    } else {
-      const int64_t l1 = (gx1 ? P->len1x : make_layout(c->sym, o1.d).len) * nb1;
-      const int64_t l2 = (gx2 ? P->len2x : make_layout(c->sym, o2.d).len) * nb2;
-      bytes += l1 * ((use_f32 && p1f) ? 4 : 8) + l2 * ((use_f32 && p2f) ? 4 : 8);
-    }
-    c->prof.recs.push_back({tag, c->prof.pool[c->prof.used], c->prof.pool[c->prof.used + 1], P->gemm.flops,
-                            prec, packed ? P->gemm.pk[prec].cfg : P->gemm.cfg, packed ? 1 : 0, bytes});
-    c->prof.used += 2;
-    pr = &c->prof.recs.back();
-    // the packed path records the events around the GEMM kernel itself
-    if (!packed) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
-  }
-  if (packed) {
-    PackSrc a, b;
-    a.data = fuse1 ? o1.data : p1;
-    a.id = id1;
+  if (r.packed) {
+    r.a = PackSrc();
+    r.b = PackSrc();
+    r.a.data = r.fuse1 ? o1.data : r.p1;
+    r.a.id = o1.static_id;
     // dims of a sorted operand that are GEMM rows: the ones it shares with the result
     auto rows_of = [&](const char* lab) {
       int m = 0;
@@ -1427,40 +1439,331 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
         if (std::memchr(P->lab3x, lab[d], 4)) m |= 1 << d;
       return m;
     };
-    if (fuse1) {
-      a.map = &P->map1;
-      a.rowmask = rows_of(P->lab1x);
-      a.xlen = P->len1x;
-      a.bstride = o1.bstride;
+    if (r.fuse1) {
+      r.a.map = &P->map1;
+      r.a.rowmask = rows_of(P->lab1x);
+      r.a.xlen = P->len1x;
+      r.a.bstride = o1.bstride;
     }
-    b.data = fuse2 ? o2.data : p2;
-    b.id = id2;
-    if (fuse2) {
-      b.map = &P->map2;
-      b.rowmask = rows_of(P->lab2x);
-      b.xlen = P->len2x;
-      b.bstride = o2.bstride;
+    r.b.data = r.fuse2 ? o2.data : r.p2;
+    r.b.id = o2.static_id;
+    if (r.fuse2) {
+      r.b.map = &P->map2;
+      r.b.rowmask = rows_of(P->lab2x);
+      r.b.xlen = P->len2x;
+      r.b.bstride = o2.bstride;
     }
-    rc = run_gemm_packed(c, P->gemm, prec, a, b, p3, fact, beta, pr ? pr->a : nullptr, pr ? pr->b : nullptr);
+  }
+  return DPD_OK;
+}
+
+// antisymmetrise + compress + accumulate the scratch result into v3
+static int call_post(dpd_ctx* c, const CallPrep& r, const Operand& o3, int nbatch) {
+  const ContractPlan* P = r.P;
+  if (!P->epi_on) return DPD_OK;
+  MapArgs m = P->epi;
+  m.nbatch = nbatch;
+  m.in_bstride = P->len3x;
+  m.out_bstride = o3.bstride;
+  return gather(c, m, r.p3, const_cast<double*>(o3.data));
+}
+
+// algorithmic bytes of one call: both operands once as the kernel reads them
+// (panels, or the tensors in their storage precision) and the result (read too
+// when accumulated)
+static int64_t call_bytes(dpd_ctx* c, const CallPrep& r, const Operand& o1, const Operand& o2, int prec) {
+  const ContractPlan* P = r.P;
+  const bool use_f32 = prec != PREC_F64;
+  int64_t bytes = P->gemm.c_elems * 8 * (r.beta != 0.0 ? 2 : 1);
+  if (r.packed) {
+    const GemmVariant& V = P->gemm.pk[prec];
+    bytes += (V.pa_elems + V.pb_elems) * pk_esize(prec);
+  } else {
+    const int64_t l1 = (r.gx1 ? P->len1x : make_layout(c->sym, o1.d).len) * r.nb1;
+    const int64_t l2 = (r.gx2 ? P->len2x : make_layout(c->sym, o2.d).len) * r.nb2;
+    bytes += l1 * ((use_f32 && r.p1f) ? 4 : 8) + l2 * ((use_f32 && r.p2f) ? 4 : 8);
+  }
+  return bytes;
+}
+
+static ProfRec* prof_open(dpd_ctx* c, const char* tag, int64_t flops, int prec, int cfg, int packed,
+                          int64_t bytes) {
+  if (!c->prof.on || !tag || c->prof.used + 2 > c->prof.pool.size()) return nullptr;
+  c->prof.recs.push_back({tag, c->prof.pool[c->prof.used], c->prof.pool[c->prof.used + 1], flops, prec, cfg, packed,
+                          bytes});
+  c->prof.used += 2;
+  return &c->prof.recs.back();
+}
+
+int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
+                  double fact, int anti, int nbatch, int prec, int64_t* flops, const char* tag) {
+  ContractPlan* P;
+  int rc = plan_of(c, kind, o1, o2, o3, nbatch, anti, &P);
+  if (rc) return rc;
+  if (flops) *flops += P->gemm.flops;
+  CallPrep r;
+  size_t need = 0;
+  if ((rc = call_setup(c, P, o1, o2, nbatch, prec, r, &need))) return rc;
+  ws_reset(c);
+  if ((rc = ws_reserve(c, need))) return rc;
+  if ((rc = call_pre(c, r, o1, o2, o3, nbatch))) return rc;
+  const bool use_f32 = (prec != PREC_F64);
+  const void* A = (use_f32 && r.p1f) ? (const void*)r.p1f : (const void*)r.p1;
+  const void* B = (use_f32 && r.p2f) ? (const void*)r.p2f : (const void*)r.p2;
+  ProfRec* pr = prof_open(c, tag, P->gemm.flops, prec, r.packed ? P->gemm.pk[prec].cfg : P->gemm.cfg,
+                          r.packed ? 1 : 0, c->prof.on && tag ? call_bytes(c, r, o1, o2, prec) : 0);
+  // the packed path records the events around the GEMM kernel itself
+  if (pr && !r.packed) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
+  if (r.packed) {
+    rc = run_gemm_packed(c, P->gemm, prec, r.a, r.b, r.p3, fact, r.beta, pr ? pr->a : nullptr,
+                         pr ? pr->b : nullptr);
     // first call of this plan in this precision: time every tile shape on these
     // operands (results go to a scratch C) and keep the fastest for later calls
     if (!rc && c->gemm_cfg == -1 && !P->gemm.pk[prec].tuned) {
-      rc = tune_variant(c, P->gemm, prec, a, b, fact, beta);
+      rc = tune_variant(c, P->gemm, prec, r.a, r.b, fact, r.beta);
       P->gemm.pk[prec].tuned = true;
     }
-  }
-  else
-    rc = run_gemm(c, P->gemm, A, (use_f32 && p1f) ? 1 : 0, B, (use_f32 && p2f) ? 1 : 0, p3, fact, beta,
+  } else {
+    rc = run_gemm(c, P->gemm, A, (use_f32 && r.p1f) ? 1 : 0, B, (use_f32 && r.p2f) ? 1 : 0, r.p3, fact, r.beta,
                   use_f32 ? PREC_F32 : PREC_F64);
+  }
   if (rc) return rc;
-  if (pr && !packed) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
-  if (P->epi_on) {
-    MapArgs m = P->epi;
-    m.nbatch = nbatch;
-    m.in_bstride = P->len3x;
-    m.out_bstride = o3.bstride;
-    rc = gather(c, m, p3, const_cast<double*>(o3.data));
+  if (pr && !r.packed) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
+  return call_post(c, r, o3, nbatch);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped contraction: the same contraction (kind, labels, anti) of n
+// independent problems whose tensors differ in size.  The pre-stages (gathers,
+// packing) run per problem; every GEMM task of all of them runs in ONE
+// packed-panel launch with one tile shape (task.group selects the problem's
+// panels and result); the epilogues run per problem.  The tile shape is timed
+// over the whole group on its first call (autotuner) unless one is requested.
+// ---------------------------------------------------------------------------
+static int group_variants(GroupPlan& G, int prec, int cfg) {
+  for (ContractPlan* P : G.members) {
+    GemmVariant& V = P->gemm.pk[prec];
+    if (V.built && V.cfg == cfg) continue;
+    for (auto& pc : V.cache) {
+      pc.src = nullptr;
+      pc.id = 0;
+      pc.buf.reset();
+    }
+    int rc = build_variant_cfg(P->gemm, V, prec, cfg);
     if (rc) return rc;
+    V.tuned = true;  // a member's tile is the group's
+  }
+  return DPD_OK;
+}
+
+// merged task list of variants (one per member, all built with one tile shape)
+static int group_tasks(const std::vector<const GemmVariant*>& Vs, DevBuf& dev, int* nwg, int* ntask) {
+  std::vector<GemmTask> all;
+  int wg = 0;
+  for (size_t i = 0; i < Vs.size(); i++)
+    for (GemmTask tk : Vs[i]->tasks) {
+      tk.group = (int32_t)i;
+      tk.ksplit = 1;
+      tk.part_off = 0;
+      tk.wg_begin = wg;
+      wg += tk.tiles_m * tk.tiles_n;
+      if (wg < 0) return fail(DPD_EINVAL, "grouped GEMM too large");
+      all.push_back(tk);
+    }
+  *nwg = wg;
+  *ntask = (int)all.size();
+  if (all.empty()) return DPD_OK;
+  int rc = dev.alloc(sizeof(GemmTask) * all.size());
+  if (rc) return rc;
+  DPD_HIP_TRY(hipMemcpy(dev.p, all.data(), sizeof(GemmTask) * all.size(), hipMemcpyHostToDevice));
+  return DPD_OK;
+}
+
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+  return st == hipStreamCaptureStatusActive;
+}
+
+static int tune_group(dpd_ctx* c, GroupPlan& G, int prec, const std::vector<CallPrep>& R, double fact) {
+  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x128, GEMM_64x128, GEMM_64x64, GEMM_32x32};
+  const int ncand = (int)(sizeof cand / sizeof cand[0]);
+  float tms[ncand];
+  const size_t es = pk_esize(prec);
+  const size_t n = G.members.size();
+  // scratch results (beta = 0) so the tuning leaves the outputs alone
+  std::vector<DevBuf> cs(n);
+  for (size_t i = 0; i < n; i++) {
+    int rc = cs[i].alloc(sizeof(double) * std::max<int64_t>(c_span(G.members[i]->gemm), 1));
+    if (rc) return rc;
+  }
+  hipEvent_t e0, e1;
+  DPD_HIP_TRY(hipEventCreate(&e0));
+  DPD_HIP_TRY(hipEventCreate(&e1));
+  int rc = DPD_OK, best = -1;
+  float best_ms = 0.0f;
+  for (int ci = 0; ci < ncand && !rc; ci++) {
+    tms[ci] = 1e30f;
+    const int cfg = cand[ci];
+    std::vector<GemmVariant> T(n);
+    std::vector<DevBuf> pa(n), pb(n);
+    std::vector<const GemmVariant*> Vs(n);
+    GroupTab gt;
+    std::memset(&gt, 0, sizeof gt);
+    for (size_t i = 0; i < n && !rc; i++) {
+      GemmPlan& p = G.members[i]->gemm;
+      if ((rc = build_variant_cfg(p, T[i], prec, cfg))) break;
+      if ((T[i].pa_elems && (rc = pa[i].alloc(es * T[i].pa_elems))) ||
+          (T[i].pb_elems && (rc = pb[i].alloc(es * T[i].pb_elems))))
+        break;
+      if ((T[i].pa_elems && (rc = pack_side(c, p, T[i], 0, R[i].a, pa[i].p))) ||
+          (T[i].pb_elems && (rc = pack_side(c, p, T[i], 1, R[i].b, pb[i].p))))
+        break;
+      Vs[i] = &T[i];
+      gt.e[i] = {pa[i].p, pb[i].p, (double*)cs[i].p, 0.0};
+    }
+    if (rc) break;
+    DevBuf dev;
+    int nwg = 0, ntask = 0;
+    if ((rc = group_tasks(Vs, dev, &nwg, &ntask))) break;
+    if (nwg == 0) continue;
+    GemmLaunch g;
+    std::memset(&g, 0, sizeof g);
+    g.tasks = (const GemmTask*)dev.p;
+    g.ntask = ntask;
+    g.nwg = nwg;
+    g.alpha = fact;
+    g.prec = prec;
+    g.cfg = cfg;
+    DPD_HIP_TRY(launch_gemm_pk_grouped(g, gt, c->stream));
+    DPD_HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < 3; r++) DPD_HIP_TRY(launch_gemm_pk_grouped(g, gt, c->stream));
+    DPD_HIP_TRY(hipEventRecord(e1, c->stream));
+    DPD_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    DPD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    tms[ci] = ms;
+    if (best < 0 || ms < best_ms) {
+      best = cfg;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (best >= 0) {
+    // among shapes within 5% of the fastest the first listed wins (stable choice)
+    for (int ci = 0; ci < ncand; ci++)
+      if (tms[ci] <= 1.05f * best_ms) {
+        best = cand[ci];
+        break;
+      }
+    G.cfg = best;
+  }
+  return DPD_OK;
+}
+
+int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, const Operand* o3, int n,
+                   double fact, int anti, const int* nbatch, int prec, const char* tag) {
+  if (n <= 0) return DPD_OK;
+  if (n > kMaxGroup) {
+    for (int i0 = 0; i0 < n; i0 += kMaxGroup) {
+      const int m = std::min(kMaxGroup, n - i0);
+      int rc = contract_group(c, kind, o1 + i0, o2 + i0, o3 + i0, m, fact, anti, nbatch + i0, prec, tag);
+      if (rc) return rc;
+    }
+    return DPD_OK;
+  }
+  std::vector<ContractPlan*> plans(n);
+  std::string gkey = std::to_string(prec) + "#";
+  bool all_packed = true;
+  for (int i = 0; i < n; i++) {
+    std::string k;
+    int rc = plan_of(c, kind, o1[i], o2[i], o3[i], nbatch[i], anti, &plans[i], &k);
+    if (rc) return rc;
+    gkey += k;
+    gkey += '|';
+    all_packed = all_packed && (prec == PREC_BF16X3 || c->packing == 2 ||
+                                (c->packing == 1 && (o1[i].static_id || o2[i].static_id)));
+  }
+  if (!all_packed) {
+    // the gathering kernels have no grouped form: one launch per problem
+    for (int i = 0; i < n; i++) {
+      int rc = contract_call(c, kind, o1[i], o2[i], o3[i], fact, anti, nbatch[i], prec, nullptr, tag);
+      if (rc) return rc;
+    }
+    return DPD_OK;
+  }
+  const bool cap = capturing(c->stream);
+  auto it = c->group_cache.find(gkey);
+  GroupPlan* G;
+  if (it == c->group_cache.end()) {
+    if (cap) return fail(DPD_EINVAL, "grouped contraction first seen during graph capture (internal)");
+    auto g = std::make_unique<GroupPlan>();
+    g->members = plans;
+    std::vector<GemmTask> all;
+    for (ContractPlan* P : plans) all.insert(all.end(), P->gemm.tasks.begin(), P->gemm.tasks.end());
+    g->cfg = c->gemm_cfg >= GEMM_64x64 ? c->gemm_cfg : choose_gemm_cfg(all, -1);
+    if (g->cfg == GEMM_V1) g->cfg = GEMM_64x64;
+    g->tuned = c->gemm_cfg != -1;
+    G = g.get();
+    c->group_cache.emplace(gkey, std::move(g));
+  } else {
+    G = it->second.get();
+  }
+  int rc = group_variants(*G, prec, G->cfg);
+  if (rc) return rc;
+  if (G->built_cfg != G->cfg) {
+    if (cap) return fail(DPD_EINVAL, "grouped task list built during graph capture (internal)");
+    std::vector<const GemmVariant*> Vs;
+    for (ContractPlan* P : plans) Vs.push_back(&P->gemm.pk[prec]);
+    if ((rc = group_tasks(Vs, G->dev, &G->nwg, &G->ntask))) return rc;
+    G->built_cfg = G->cfg;
+  }
+  std::vector<CallPrep> R(n);
+  size_t need = 0;
+  for (int i = 0; i < n; i++)
+    if ((rc = call_setup(c, plans[i], o1[i], o2[i], nbatch[i], prec, R[i], &need))) return rc;
+  ws_reset(c);
+  if (cap && c->ws.bytes < ((need + 255) & ~size_t(255)))
+    return fail(DPD_EINVAL, "workspace growth during graph capture (internal)");
+  if ((rc = ws_reserve(c, need))) return rc;
+  GroupTab gt;
+  std::memset(&gt, 0, sizeof gt);
+  int64_t flops = 0, bytes = 0;
+  for (int i = 0; i < n; i++) {
+    if ((rc = call_pre(c, R[i], o1[i], o2[i], o3[i], nbatch[i]))) return rc;
+    const void* side[2];
+    if ((rc = pack_sides(c, plans[i]->gemm, plans[i]->gemm.pk[prec], prec, R[i].a, R[i].b, side))) return rc;
+    gt.e[i] = {side[0], side[1], R[i].p3, R[i].beta};
+    flops += plans[i]->gemm.flops;
+    if (c->prof.on && tag) bytes += call_bytes(c, R[i], o1[i], o2[i], prec);
+  }
+  GemmLaunch g;
+  std::memset(&g, 0, sizeof g);
+  g.tasks = (const GemmTask*)G->dev.p;
+  g.ntask = G->ntask;
+  g.nwg = G->nwg;
+  g.alpha = fact;
+  g.prec = prec;
+  g.cfg = G->cfg;
+  ProfRec* pr = cap ? nullptr : prof_open(c, tag, flops, prec, G->cfg, 1, bytes);
+  if (pr) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
+  DPD_HIP_TRY(launch_gemm_pk_grouped(g, gt, c->stream));
+  if (pr) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
+  for (int i = 0; i < n; i++)
+    if ((rc = call_post(c, R[i], o3[i], nbatch[i]))) return rc;
+  if (!G->tuned && !cap) {
+    G->tuned = true;
+    const int before = G->cfg;
+    if ((rc = tune_group(c, *G, prec, R, fact))) return rc;
+    if (G->cfg != before) {
+      if ((rc = group_variants(*G, prec, G->cfg))) return rc;
+      std::vector<const GemmVariant*> Vs;
+      for (ContractPlan* P : plans) Vs.push_back(&P->gemm.pk[prec]);
+      if ((rc = group_tasks(Vs, G->dev, &G->nwg, &G->ntask))) return rc;
+      G->built_cfg = G->cfg;
+    }
   }
   return DPD_OK;
 }

@@ -114,6 +114,17 @@ struct ContractPlan {
   bool batch1 = false, batch2 = false;
   int64_t bstride1 = 0, bstride2 = 0, bstride3 = 0;
 };
+// one grouped contraction (the same contraction of several problems): the
+// members' plans, the tile shape they share and the merged task list
+struct GroupPlan {
+  int cfg = -1;
+  bool tuned = false;
+  int built_cfg = -1;
+  int nwg = 0;
+  int ntask = 0;
+  DevBuf dev;
+  std::vector<ContractPlan*> members;
+};
 
 }  // namespace dpd
 
@@ -153,6 +164,7 @@ struct dpd_ctx {
   // plan caches
   std::unordered_map<std::string, std::unique_ptr<dpd::ContractPlan>> contract_cache;
   std::unordered_map<std::string, std::unique_ptr<dpd::GemmPlan>> engine_cache;
+  std::unordered_map<std::string, std::unique_ptr<dpd::GroupPlan>> group_cache;
   std::vector<char> pinned_scratch;
 };
 
@@ -169,6 +181,10 @@ int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, cons
 int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
                   double fact, int anti, int nbatch, int prec, int64_t* flops = nullptr,
                   const char* tag = nullptr);
+// the same contraction of n problems (kind, labels and anti equal; sizes may
+// differ): one grouped GEMM launch for all of them where every member is packed
+int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, const Operand* o3, int n,
+                   double fact, int anti, const int* nbatch, int prec, const char* tag = nullptr);
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
              double f1, double f2, int prec);
 int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested);

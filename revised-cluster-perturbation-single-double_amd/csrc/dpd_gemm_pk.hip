@@ -276,9 +276,11 @@ struct PkTraits<PREC_BF16X3> {
   static constexpr int PLANES = 2;
 };
 
+// one workgroup's tile of task T (local = workgroup index inside the task)
 template <int MODE, int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm_pk(GemmLaunch g, const void* __restrict__ PA,
-                                                 const void* __restrict__ PB) {
+__device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask& T, int local,
+                                             const void* __restrict__ PA, const void* __restrict__ PB,
+                                             double* __restrict__ C, const double beta) {
   typedef PkTraits<MODE> Tr;
   typedef typename Tr::E E;
   typedef typename Tr::AT AT;
@@ -298,14 +300,6 @@ __global__ __launch_bounds__(256) void k_gemm_pk(GemmLaunch g, const void* __res
   __shared__ int64_t cm[BM], cn[BN];
 
   const int tid = threadIdx.x;
-  int wg;
-  {
-    const int x = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int q = g.nwg >> 3, r = g.nwg & 7;
-    wg = x * q + (x < r ? x : r) + slot;
-  }
-  const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
-  int local = wg - T.wg_begin;
   const int tn = local % T.tiles_n;
   local /= T.tiles_n;
   const int tm = local % T.tiles_m;
@@ -440,9 +434,74 @@ __global__ __launch_bounds__(256) void k_gemm_pk(GemmLaunch g, const void* __res
           g.partial[T.part_off + (int64_t)kp * T.M * T.N + m + (int64_t)n * T.M] = v;
         } else {
           const int64_t ad = T.c_off + cm[ml] + cn[nl];
-          g.C[ad] = (g.beta == 0.0) ? v : g.beta * g.C[ad] + v;
+          C[ad] = (beta == 0.0) ? v : beta * C[ad] + v;
         }
       }
+}
+
+// XCD-aware order: consecutive workgroup ids (neighbouring tiles that share
+// panels) go to one XCD's L2 (dispatch deals workgroups round-robin over 8 XCDs)
+__device__ __forceinline__ int xcd_wg(int nwg) {
+  const int x = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int q = nwg >> 3, r = nwg & 7;
+  return x * q + (x < r ? x : r) + slot;
+}
+
+template <int MODE, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_pk(GemmLaunch g, const void* __restrict__ PA,
+                                                 const void* __restrict__ PB) {
+  const int wg = xcd_wg(g.nwg);
+  const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
+  gemm_pk_tile<MODE, BM, BN>(g, T, wg - T.wg_begin, PA, PB, g.C, g.beta);
+}
+
+template <int MODE, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_pk_grp(GemmLaunch g, GroupTab gt) {
+  const int wg = xcd_wg(g.nwg);
+  const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
+  const GroupEnt& e = gt.e[T.group];
+  gemm_pk_tile<MODE, BM, BN>(g, T, wg - T.wg_begin, e.pa, e.pb, e.c, e.beta);
+}
+
+template <int MODE>
+static hipError_t launch_pk_grp_mode(const GemmLaunch& g, const GroupTab& gt, hipStream_t s) {
+  switch (g.cfg) {
+    case GEMM_64x64:
+      hipLaunchKernelGGL((k_gemm_pk_grp<MODE, 64, 64>), dim3(g.nwg), dim3(256), 0, s, g, gt);
+      break;
+    case GEMM_128x64:
+      hipLaunchKernelGGL((k_gemm_pk_grp<MODE, 128, 64>), dim3(g.nwg), dim3(256), 0, s, g, gt);
+      break;
+    case GEMM_64x128:
+      hipLaunchKernelGGL((k_gemm_pk_grp<MODE, 64, 128>), dim3(g.nwg), dim3(256), 0, s, g, gt);
+      break;
+    case GEMM_128x128:
+      hipLaunchKernelGGL((k_gemm_pk_grp<MODE, 128, 128>), dim3(g.nwg), dim3(256), 0, s, g, gt);
+      break;
+    case GEMM_32x32:
+      hipLaunchKernelGGL((k_gemm_pk_grp<MODE, 32, 32>), dim3(g.nwg), dim3(256), 0, s, g, gt);
+      break;
+    case GEMM_64x192:
+      hipLaunchKernelGGL((k_gemm_pk_grp<MODE, 64, 192>), dim3(g.nwg), dim3(256), 0, s, g, gt);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_pk_grouped(const GemmLaunch& g, const GroupTab& gt, hipStream_t s) {
+  if (g.nwg == 0) return hipSuccess;
+  switch (g.prec) {
+    case PREC_F64:
+      return launch_pk_grp_mode<PREC_F64>(g, gt, s);
+    case PREC_F32:
+      return launch_pk_grp_mode<PREC_F32>(g, gt, s);
+    case PREC_BF16X3:
+      return launch_pk_grp_mode<PREC_BF16X3>(g, gt, s);
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 template <int MODE>

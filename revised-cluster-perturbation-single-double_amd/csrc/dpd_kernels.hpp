@@ -51,7 +51,7 @@ struct GemmTask {
   int32_t ksplit;
   int32_t tiles_m, tiles_n;
   int32_t wg_begin;
-  int32_t pad_;
+  int32_t group;  // grouped launches: index into the launch's GroupTab
   int64_t part_off;  // split-K partial buffer offset (elements) if ksplit > 1
   // packed-panel kernel: K steps, first panel ids and panel offsets (elements)
   int32_t steps;
@@ -133,6 +133,20 @@ inline int pk_kstep(int mode) { return mode == PREC_F64 ? 16 : 32; }
 inline int pk_planes(int mode) { return mode == PREC_BF16X3 ? 2 : 1; }
 inline int pk_esize(int mode) { return mode == PREC_F64 ? 8 : mode == PREC_F32 ? 4 : 2; }
 hipError_t launch_gemm_pk(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s);
+// grouped launch: the tasks of several independent contractions (one per
+// problem of a batch) in one kernel; task.group selects the panels, result and
+// beta of its contraction.  No split-K (every task has ksplit = 1).
+constexpr int kMaxGroup = 32;
+struct GroupEnt {
+  const void* pa;
+  const void* pb;
+  double* c;
+  double beta;
+};
+struct GroupTab {
+  GroupEnt e[kMaxGroup];
+};
+hipError_t launch_gemm_pk_grouped(const GemmLaunch& g, const GroupTab& gt, hipStream_t s);
 
 // ---- vector kernels (FP64) ------------------------------------------------
 constexpr int kRedBlocks = 512;

@@ -75,7 +75,23 @@ static EorbOff eorb_off(const dpd_problem* p) {
   return e;
 }
 
-int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) {
+// one step of a sigma build: a contraction (kind 444/424/442), the clearing of
+// the output (kind 0) or a diagonal term y += d * x over nvec vectors (kind 1)
+struct SigmaTerm {
+  int kind = 0;
+  Operand a, b, c;
+  double fact = 1.0;
+  int anti = 0;
+  int nb = 1;
+  const char* tag = nullptr;
+  int64_t n = 0, stride = 0;
+  const double* d = nullptr;
+  const double* x = nullptr;
+  double* y = nullptr;
+};
+
+// the terms of sigma = J R for one problem, in order
+static int sigma_terms(dpd_problem* p, const double* R, double* sigma, int64_t nvec, std::vector<SigmaTerm>& T) {
   dpd_ctx* c = p->ctx;
   const int64_t *v = p->popv, *o = p->popo, *z = p->zorb;
   const int S = p->symR;
@@ -87,22 +103,52 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
     if (rc0) return rc0;
   }
   auto F = [&](const DevBuf& b) -> const float* { return f32 ? (const float*)b.p : nullptr; };
-  DPD_HIP_TRY(launch_fill(p->dim * nvec, 0.0, sigma, c->stream));
-  Desc dR1 = mkdesc(v, o, z, z, 1, 1, S);
-  Operand R1 = batched(op(dR1, "", 4, R), p->dim);
-  Operand C1 = batched(op(dR1, "", 4, sigma), p->dim);
-  int rc;
   auto lab = [](Operand x, const char* l, int rank) {
     x.rank = rank;
     std::memset(x.lab, 0, 4);
     std::memcpy(x.lab, l, rank);
     return x;
   };
+  auto con = [&](int kind, const Operand& a, const Operand& b, const Operand& cc, double fact, int anti,
+                 const char* tag) {
+    SigmaTerm t;
+    t.kind = kind;
+    t.a = a;
+    t.b = b;
+    t.c = cc;
+    t.fact = fact;
+    t.anti = anti;
+    t.nb = nb;
+    t.tag = tag;
+    T.push_back(t);
+  };
+  auto diag = [&](int64_t n, const double* d, const double* x, double* y) {
+    SigmaTerm t;
+    t.kind = 1;
+    t.n = n;
+    t.d = d;
+    t.x = x;
+    t.y = y;
+    t.nb = nb;
+    t.stride = p->dim;
+    T.push_back(t);
+  };
+  {
+    SigmaTerm t;
+    t.kind = 0;
+    t.n = p->dim * nvec;
+    t.y = sigma;
+    T.push_back(t);
+  }
+  Desc dR1 = mkdesc(v, o, z, z, 1, 1, S);
+  Operand R1 = batched(op(dR1, "", 4, R), p->dim);
+  Operand C1 = batched(op(dR1, "", 4, sigma), p->dim);
   if (p->method == 0) {
     // e(a,i) r(a,i)
-    DPD_HIP_TRY(launch_hadamard_acc(p->l1, (const double*)p->diag.p, R, sigma, nb, p->dim, p->dim, c->stream));
+    diag(p->l1, (const double*)p->diag.p, R, sigma);
     Operand W = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->iabj.p, F(p->iabj32)), p, 1);
-    return contract_call(c, 444, W, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec, nullptr, "c1_r1iabj");
+    con(444, W, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, "c1_r1iabj");
+    return DPD_OK;
   }
   Desc dR2 = mkdesc(v, v, o, o, 0, 0, S);
   Operand R2 = batched(op(dR2, "", 4, R + p->l1), p->dim);
@@ -114,46 +160,96 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
   Operand ijka = konst(op(mkdesc(o, o, o, v, 0, 1, 1), "mnie", 4, (const double*)p->ijka.p, F(p->ijka32)), p, 6);
   // ---- singles (CISdoo.SD.SF/calAR.f90:41-56)
   // c1_r1Fae: vv2v_222(Fac, r1, c1, 'ae','ei','ai')
-  rc = contract_call(c, 444, Fac, lab(R1, "ei=-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec, nullptr, "c1_r1Fae");
-  if (rc) return rc;
+  con(444, Fac, lab(R1, "ei=-", 4), lab(C1, "ai+-", 4), 1.0, 0, "c1_r1Fae");
   // c1_r1Fmi: vv2v_222(r1, Fki, c1, -1, 'am','mi','ai')
-  rc = contract_call(c, 444, lab(R1, "am=+", 4), Fki, lab(C1, "ai+-", 4), -1.0, 0, nb, p->prec, nullptr, "c1_r1Fmi");
-  if (rc) return rc;
+  con(444, lab(R1, "am=+", 4), Fki, lab(C1, "ai+-", 4), -1.0, 0, "c1_r1Fmi");
   // c1_r1iabj: vv2v_422(Wkaci, r1, c1, 'fnai','fn','ai')
-  rc = contract_call(c, 444, Wk, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, nb, p->prec, nullptr, "c1_r1iabj");
-  if (rc) return rc;
+  con(444, Wk, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, "c1_r1iabj");
   // c1_r2abci: vv2v_442(abci, r2, c1, 'feam','feim','ai')  (r2 iop 0: fact)
-  rc = contract_call(c, 442, abci, lab(R2, "feim", 4), lab(C1, "ai", 2), 1.0, 0, nb, p->prec, nullptr, "c1_r2abci");
-  if (rc) return rc;
+  con(442, abci, lab(R2, "feim", 4), lab(C1, "ai", 2), 1.0, 0, "c1_r2abci");
   // c1_r2ijka: vv2v_442(r2, ijka, c1, -1, 'aemn','mnie','ai')
-  rc = contract_call(c, 442, lab(R2, "aemn", 4), ijka, lab(C1, "ai", 2), -1.0, 0, nb, p->prec, nullptr, "c1_r2ijka");
-  if (rc) return rc;
+  con(442, lab(R2, "aemn", 4), ijka, lab(C1, "ai", 2), -1.0, 0, "c1_r2ijka");
   // ---- doubles (CISdoo.SD.SF/calAR.f90:63-77)
-  DPD_HIP_TRY(launch_hadamard_acc(p->l2, (const double*)p->diag.p + p->l1, R + p->l1, sigma + p->l1, nb, p->dim,
-                                  p->dim, c->stream));
+  diag(p->l2, (const double*)p->diag.p + p->l1, R + p->l1, sigma + p->l1);
   // c2_r1abci: vv2v_424(abci, r1, c2, 'abej','ei','abij', anti=2)
-  rc = contract_call(c, 424, lab(abci, "abej", 4), lab(R1, "ei", 2), lab(C2, "abij", 4), 1.0, 2, nb, p->prec, nullptr, "c2_r1abci");
-  if (rc) return rc;
+  con(424, lab(abci, "abej", 4), lab(R1, "ei", 2), lab(C2, "abij", 4), 1.0, 2, "c2_r1abci");
   // c2_r1ijka: vv2v_424(ijka, r1, c2, -1, 'ijmb','am','abij', anti=1)
-  rc = contract_call(c, 424, lab(ijka, "ijmb", 4), lab(R1, "am", 2), lab(C2, "abij", 4), -1.0, 1, nb, p->prec, nullptr, "c2_r1ijka");
-  if (rc) return rc;
+  con(424, lab(ijka, "ijmb", 4), lab(R1, "am", 2), lab(C2, "abij", 4), -1.0, 1, "c2_r1ijka");
   if (p->method == 2) {
     Operand W = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "embj", 4, (const double*)p->iabj.p, F(p->iabj32)), p, 7);
     Operand ijkl = konst(op(mkdesc(o, o, o, o, 0, 0, 1), "mnij", 4, (const double*)p->ijkl.p, F(p->ijkl32)), p, 8);
     Operand abcd = konst(op(mkdesc(v, v, v, v, 0, 0, 1), "abef", 4, (const double*)p->abcd.p, F(p->abcd32)), p, 9);
     // c2_r2Wmbej: vv2v_444(r2, W, c2, 'aeim','embj','abij', anti=3)
-    rc = contract_call(c, 444, lab(R2, "aeim", 4), W, lab(C2, "abij", 4), 1.0, 3, nb, p->prec, nullptr, "c2_r2Wmbej");
-    if (rc) return rc;
+    con(444, lab(R2, "aeim", 4), W, lab(C2, "abij", 4), 1.0, 3, "c2_r2Wmbej");
     // c2_r2Wmnij: vv2v_444(r2, ijkl, c2, 'abmn','mnij','abij')
-    rc = contract_call(c, 444, lab(R2, "abmn", 4), ijkl, lab(C2, "abij", 4), 1.0, 0, nb, p->prec, nullptr, "c2_r2Wmnij");
-    if (rc) return rc;
+    con(444, lab(R2, "abmn", 4), ijkl, lab(C2, "abij", 4), 1.0, 0, "c2_r2Wmnij");
     // c2_r2abcd: vv2v_444(abcd, r2, c2, 'abef','efij','abij')
-    rc = contract_call(c, 444, abcd, lab(R2, "efij", 4), lab(C2, "abij", 4), 1.0, 0, nb, p->prec, nullptr, "c2_r2abcd");
+    con(444, abcd, lab(R2, "efij", 4), lab(C2, "abij", 4), 1.0, 0, "c2_r2abcd");
+  }
+  return DPD_OK;
+}
+
+static int run_simple_term(dpd_ctx* c, const SigmaTerm& t) {
+  if (t.kind == 0) {
+    DPD_HIP_TRY(launch_fill(t.n, 0.0, t.y, c->stream));
+  } else {
+    DPD_HIP_TRY(launch_hadamard_acc(t.n, t.d, t.x, t.y, t.nb, t.stride, t.stride, c->stream));
+  }
+  return DPD_OK;
+}
+
+int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) {
+  std::vector<SigmaTerm> T;
+  int rc = sigma_terms(p, R, sigma, nvec, T);
+  if (rc) return rc;
+  dpd_ctx* c = p->ctx;
+  for (const SigmaTerm& t : T) {
+    if (t.kind == 0 || t.kind == 1)
+      rc = run_simple_term(c, t);
+    else
+      rc = contract_call(c, t.kind, t.a, t.b, t.c, t.fact, t.anti, t.nb, p->prec, nullptr, t.tag);
     if (rc) return rc;
   }
   return DPD_OK;
 }
 
+// sigma of n problems of one method and context, each contraction of all of
+// them in one grouped GEMM launch
+int problem_sigma_group(dpd_problem* const* ps, const double* const* R, double* const* S, const int64_t* nvec,
+                        int n) {
+  if (n <= 0) return DPD_OK;
+  dpd_ctx* c = ps[0]->ctx;
+  std::vector<std::vector<SigmaTerm>> T(n);
+  for (int i = 0; i < n; i++) {
+    if (ps[i]->ctx != c || ps[i]->method != ps[0]->method || ps[i]->prec != ps[0]->prec)
+      return fail(DPD_EINVAL, "grouped sigma: problems must share context, method and precision");
+    int rc = sigma_terms(ps[i], R[i], S[i], nvec[i], T[i]);
+    if (rc) return rc;
+    if (T[i].size() != T[0].size()) return fail(DPD_EINVAL, "grouped sigma: term lists differ (internal)");
+  }
+  std::vector<Operand> a(n), b(n), cc(n);
+  std::vector<int> nb(n);
+  for (size_t k = 0; k < T[0].size(); k++) {
+    const SigmaTerm& t0 = T[0][k];
+    if (t0.kind == 0 || t0.kind == 1) {
+      for (int i = 0; i < n; i++) {
+        int rc = run_simple_term(c, T[i][k]);
+        if (rc) return rc;
+      }
+      continue;
+    }
+    for (int i = 0; i < n; i++) {
+      a[i] = T[i][k].a;
+      b[i] = T[i][k].b;
+      cc[i] = T[i][k].c;
+      nb[i] = T[i][k].nb;
+    }
+    int rc = contract_group(c, t0.kind, a.data(), b.data(), cc.data(), n, t0.fact, t0.anti, nb.data(),
+                            ps[0]->prec, t0.tag);
+    if (rc) return rc;
+  }
+  return DPD_OK;
+}
 static int upload(DevBuf& b, const std::vector<double>& h, hipStream_t s) {
   int rc = b.alloc(std::max<size_t>(8, h.size() * sizeof(double)));
   if (rc) return rc;

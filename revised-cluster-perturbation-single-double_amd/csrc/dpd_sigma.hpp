@@ -29,4 +29,6 @@ namespace dpd {
 int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec);
 int problem_make_f32(dpd_problem* p);
 int problem_sigma_t(dpd_problem* p, const double* L, double* t, int64_t nvec);
+int problem_sigma_group(dpd_problem* const* ps, const double* const* R, double* const* S, const int64_t* nvec,
+                        int n);
 }

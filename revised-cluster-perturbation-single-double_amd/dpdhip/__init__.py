@@ -15,7 +15,7 @@ import numpy as np
 from . import _lib
 from ._lib import DpdError, check, lib
 
-__all__ = ["Context", "Opt", "Spec", "Problem", "DpdError", "xor_dirprd", "ZORB"]
+__all__ = ["Context", "Opt", "Spec", "Problem", "DpdError", "xor_dirprd", "ZORB", "sigma_group", "davidson_batch"]
 
 ZORB = np.array([1, 0, 0, 0, 0, 0, 0, 0], dtype=np.int64)
 
@@ -325,6 +325,50 @@ def small_eig(A):
 
 METHODS = {"sf-cis": 0, "cisdoo": 1, "sf-cpsd": 2}
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
+def sigma_group(problems, Rs, outs=None):
+    """sigma of several problems (same context, method, precision) with one
+    grouped GEMM launch per contraction"""
+    torch = _torch()
+    n = len(problems)
+    outs = outs if outs is not None else [torch.empty_like(R) for R in Rs]
+    P = (ctypes.c_void_p * n)(*[p.h.value for p in problems])
+    X = (ctypes.c_void_p * n)(*[R.data_ptr() for R in Rs])
+    Y = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+    nv = np.array([R.numel() // p.dim for p, R in zip(problems, Rs)], dtype=np.int64)
+    check(lib().dpd_problem_sigma_group(P, X, Y, nv.ctypes.data_as(_lib.c_i64_p), n))
+    return outs
+
+
+def davidson_batch(problems, nroot, conv=1e-8, max_dim=20, max_cyc=50, lowest=False, final_check=True, Rs=None,
+                   grouped=True, graph=True):
+    """Davidson solves of several problems in lockstep; their sigma steps are
+    applied together (grouped GEMM launches, replayed from a hipGraph)."""
+    n = len(problems)
+    if Rs is None:
+        Rs = [p.guess(nroot)[0] for p in problems]
+    opts = (_lib.DavidsonOpts_ * n)()
+    for o in opts:
+        o.nroot, o.max_dim, o.max_cyc, o.conv = nroot, max_dim, max_cyc, conv
+        o.lowest, o.final_check, o.transpose = int(lowest), int(final_check), 0
+    ws = [np.zeros(nroot) for _ in range(n)]
+    P = (ctypes.c_void_p * n)(*[p.h.value for p in problems])
+    R = (ctypes.c_void_p * n)(*[r.data_ptr() for r in Rs])
+    W = (ctypes.c_void_p * n)(*[w.ctypes.data for w in ws])
+    flags = (1 if grouped else 0) | (2 if graph else 0)
+    stats = np.zeros(4, dtype=np.int64)
+    rc = lib().dpd_davidson_batch(P, n, opts, R, W, flags, stats.ctypes.data_as(_lib.c_i64_p))
+    if rc not in (0, _lib.DPD_ENOTCONV):
+        check(rc)
+    infos = [{"converged": o.iterations <= max_cyc and all(r < conv for r in o.residual[:nroot]),
+              "iterations": o.iterations, "sigma_calls": o.sigma_calls, "skipped": o.skipped,
+              "residual": list(o.residual[:nroot]), "final_residual": list(o.final_residual[:nroot])}
+             for o in opts]
+    for i in infos:
+        i["batch"] = {"steps": int(stats[0]), "graph_replays": int(stats[1]), "graphs": int(stats[2]),
+                      "captures_abandoned": int(stats[3])}
+    return ws, Rs, infos
 
 
 class Problem:

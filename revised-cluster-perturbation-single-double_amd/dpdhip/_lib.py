@@ -127,6 +127,9 @@ _SIGS = {
     "dpd_problem_set": (ctypes.c_int, [c_void_p, ctypes.c_char_p, c_void_p]),
     "dpd_problem_refresh": (ctypes.c_int, [c_void_p]),
     "dpd_problem_grad": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_double] + [c_void_p] * 7),
+    "dpd_problem_sigma_group": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_i64_p, i64]),
+    "dpd_davidson_batch": (ctypes.c_int, [c_void_p, i64, ctypes.POINTER(DavidsonOpts_), c_void_p, c_void_p, i64,
+                                          c_i64_p]),
     "dpd_small_eig": (ctypes.c_int, [i64, c_double_p, c_double_p, c_double_p, c_double_p]),
     "dpd_davidson": (ctypes.c_int, [c_void_p, ctypes.POINTER(DavidsonOpts_), c_void_p, c_double_p]),
 }
