@@ -299,8 +299,8 @@ struct DavWork {
       idx;
 };
 
-static int dsync(dpd_ctx* c) {
-  DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+static int dsync(hipStream_t s) {
+  DPD_HIP_TRY(hipStreamSynchronize(s));
   return DPD_OK;
 }
 
@@ -316,13 +316,17 @@ extern "C" int dpd_small_eig(int64_t n, const double* A_rowmajor, double* wr, do
 // sigma of nv vectors for one Davidson; null: the problem's own operator
 typedef std::function<int(const double* x, double* y, int nv)> SigmaHook;
 
-static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out, const SigmaHook* hook);
+// stream: where the solver's own work goes (null: the context's stream, which
+// the problem's operator uses too)
+static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out, const SigmaHook* hook,
+                        hipStream_t stream);
 
 extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out) {
-  return davidson_run(P, o, R, w_out, nullptr);
+  return davidson_run(P, o, R, w_out, nullptr, nullptr);
 }
 
-static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out, const SigmaHook* hook) {
+static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out, const SigmaHook* hook,
+                        hipStream_t stream) {
   if (!P || !o || !R || !w_out) return fail(DPD_EINVAL, "dpd_davidson: null argument");
   if (P->stale) return fail(DPD_EINVAL, "problem inputs changed: call dpd_problem_refresh first");
   dpd_ctx* c = P->ctx;
@@ -336,7 +340,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   if (nroot >= dimAbar) return fail(DPD_EINVAL, "ERROR: too much roots required %d %d", nroot, dimAbar);
   if (dimAbar > 1024) return fail(DPD_EINVAL, "max_dim must be <= 1024");
   const int dimDel = dimAbar - nroot - 5;
-  hipStream_t st = c->stream;
+  hipStream_t st = stream ? stream : c->stream;
   DavWork W;
   int rc;
   const size_t vb = sizeof(double) * (size_t)n;
@@ -375,14 +379,14 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   auto block_dots = [&](const double* X, int nx, const double* Y, int ny, double* host) -> int {
     DPD_HIP_TRY(launch_block_dots(n, X, n, nx, Y, n, ny, part, dots, st));
     DPD_HIP_TRY(hipMemcpyAsync(host, dots, sizeof(double) * nx * ny, hipMemcpyDeviceToHost, st));
-    return dsync(c);
+    return dsync(st);
   };
   // e_v = A r_v - w_v r_v and max|e_v| for v < nroot (w already on the device)
   auto residuals = [&](const double* AR, double* res_host) -> int {
     DPD_HIP_TRY(launch_resid_amax(n, AR, R, wdev, E, nroot, (double*)W.amaxp.p, (int64_t*)W.amaxpi.p,
                                   (double*)W.amaxv.p, (int64_t*)W.amaxi.p, st));
     DPD_HIP_TRY(hipMemcpyAsync(res_host, W.amaxv.p, sizeof(double) * nroot, hipMemcpyDeviceToHost, st));
-    return dsync(c);
+    return dsync(st);
   };
 
   std::vector<double> Abar(nd2, 0.0);  // column-major Abar(i,j) = [i + j*dimAbar]
@@ -573,7 +577,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       std::vector<double> s2(nopt);
       if (nopt) {
         DPD_HIP_TRY(hipMemcpyAsync(s2.data(), dots, sizeof(double) * nopt, hipMemcpyDeviceToHost, st));
-        if ((rc = dsync(c))) return rc;
+        if ((rc = dsync(st))) return rc;
       }
       size_t k = 0;
       for (; k < nopt && std::sqrt(s2[k]) >= 1e-6; k++) {
@@ -589,7 +593,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
           if ((rc = schmidt(k, nNewBase, dots))) return rc;
           double t2;
           DPD_HIP_TRY(hipMemcpyAsync(&t2, dots, sizeof(double), hipMemcpyDeviceToHost, st));
-          if ((rc = dsync(c))) return rc;
+          if ((rc = dsync(st))) return rc;
           if (std::sqrt(t2) < 1e-6) {
             nskip++;
             continue;
@@ -637,7 +641,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
         off += (size_t)nv * nt;
       }
       DPD_HIP_TRY(hipMemcpyAsync(hd.data(), dots, sizeof(double) * off, hipMemcpyDeviceToHost, st));
-      if ((rc = dsync(c))) return rc;
+      if ((rc = dsync(st))) return rc;
       for (int c0 = 0, ci = 0; c0 < nNewBase; c0 += kMgsMulti, ci++) {
         const int nv = std::min(kMgsMulti, nNewBase - c0);
         for (int j = 0; j < dimIter; j++)
@@ -661,7 +665,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   }
   o->sigma_calls = nsig;
   o->skipped = nskip;
-  if ((rc = dsync(c))) return rc;
+  if ((rc = dsync(st))) return rc;
   return converged ? DPD_OK : DPD_ENOTCONV;
 }
 
@@ -775,12 +779,26 @@ extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davi
     const size_t bytes = sizeof(double) * (size_t)probs[i]->dim * (size_t)std::max<int64_t>(1, opts[i].nroot);
     if ((rc = X[i].alloc(bytes)) || (rc = Y[i].alloc(bytes))) break;
   }
+  // each solver's own work on a stream of its own, so that the vector algebra
+  // and host synchronisations of different problems overlap; the sigma step
+  // on the context stream waits for every member and they wait for it
+  std::vector<hipStream_t> ms(n, nullptr);
+  std::vector<hipEvent_t> mev(n, nullptr);
+  hipEvent_t done = nullptr;
+  for (int64_t i = 0; i < n && !rc; i++) {
+    if (hipStreamCreate(&ms[i]) != hipSuccess || hipEventCreateWithFlags(&mev[i], hipEventDisableTiming) != hipSuccess)
+      rc = fail(DPD_EHIP, "stream creation failed");
+    else if (hipStreamWaitEvent(ms[i], ev, 0) != hipSuccess)
+      rc = fail(DPD_EHIP, "stream wait failed");
+  }
+  if (!rc && hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess)
+    rc = fail(DPD_EHIP, "event creation failed");
   GraphCache gc;
   Batcher B;
   B.active = (int)n;
-  B.run = [&](std::vector<SigmaReq*>& reqs) -> int {
+  // one batched sigma step on the context stream
+  auto run_step = [&](std::vector<SigmaReq*>& reqs) -> int {
     const int m = (int)reqs.size();
-    stat[0]++;
     std::vector<dpd_problem*> ps(m);
     std::vector<const double*> xs(m);
     std::vector<double*> ys(m);
@@ -850,6 +868,18 @@ extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davi
                                  st));
     return DPD_OK;
   };
+  B.run = [&](std::vector<SigmaReq*>& reqs) -> int {
+    stat[0]++;
+    for (SigmaReq* q : reqs) {
+      DPD_HIP_TRY(hipEventRecord(mev[q->id], ms[q->id]));
+      DPD_HIP_TRY(hipStreamWaitEvent(st, mev[q->id], 0));
+    }
+    int brc = run_step(reqs);
+    if (brc) return brc;
+    DPD_HIP_TRY(hipEventRecord(done, st));
+    for (SigmaReq* q : reqs) DPD_HIP_TRY(hipStreamWaitEvent(ms[q->id], done, 0));
+    return DPD_OK;
+  };
   std::vector<int> rcs(n, DPD_OK);
   std::vector<std::string> errs(n);
   if (!rc) {
@@ -870,12 +900,20 @@ extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davi
         if (hipSetDevice(c->device) != hipSuccess)
           rcs[i] = DPD_EHIP;
         else
-          rcs[i] = davidson_run(probs[i], &opts[i], R[i], w[i], &hook);
+          rcs[i] = davidson_run(probs[i], &opts[i], R[i], w[i], &hook, ms[i]);
         if (rcs[i] && rcs[i] != DPD_ENOTCONV) errs[i] = dpd_last_error();
         B.leave();
       });
     for (auto& t : th) t.join();
   }
+  for (int64_t i = 0; i < n; i++) {
+    if (ms[i]) {
+      (void)hipStreamSynchronize(ms[i]);
+      (void)hipStreamDestroy(ms[i]);
+    }
+    if (mev[i]) (void)hipEventDestroy(mev[i]);
+  }
+  if (done) (void)hipEventDestroy(done);
   (void)hipStreamSynchronize(st);
   c->stream = caller;
   if (stats)

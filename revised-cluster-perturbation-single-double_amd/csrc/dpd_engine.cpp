@@ -1556,27 +1556,71 @@ static int group_variants(GroupPlan& G, int prec, int cfg) {
   return DPD_OK;
 }
 
-// merged task list of variants (one per member, all built with one tile shape)
-static int group_tasks(const std::vector<const GemmVariant*>& Vs, DevBuf& dev, int* nwg, int* ntask) {
+// merged task list of variants (one per member, all built with one tile shape);
+// split-K is decided again over the whole group (the same rule as for one
+// plan: split long-K tasks while the launch has fewer than 1024 tiles)
+static int group_tasks(const std::vector<const GemmVariant*>& Vs, int mode, GroupTasks& G) {
   std::vector<GemmTask> all;
-  int wg = 0;
   for (size_t i = 0; i < Vs.size(); i++)
     for (GemmTask tk : Vs[i]->tasks) {
       tk.group = (int32_t)i;
-      tk.ksplit = 1;
-      tk.part_off = 0;
-      tk.wg_begin = wg;
-      wg += tk.tiles_m * tk.tiles_n;
-      if (wg < 0) return fail(DPD_EINVAL, "grouped GEMM too large");
       all.push_back(tk);
     }
-  *nwg = wg;
-  *ntask = (int)all.size();
+  int64_t work = 0, tiles = 0;
+  for (const auto& tk : all) {
+    work += (int64_t)tk.tiles_m * tk.tiles_n * std::max(tk.steps, 1);
+    tiles += (int64_t)tk.tiles_m * tk.tiles_n;
+  }
+  const int64_t target = 1024;
+  const int64_t min_steps = mode == PREC_F64 ? 8 : 4;
+  const int64_t per_wg = std::max<int64_t>(min_steps, (work + target - 1) / target);
+  std::vector<int32_t> split;
+  int64_t partial = 0;
+  int wg = 0, max_mn = 0;
+  for (size_t i = 0; i < all.size(); i++) {
+    GemmTask& tk = all[i];
+    int ks = 1;
+    if (tiles < target && tk.steps > per_wg) ks = (int)std::min<int64_t>(64, (tk.steps + per_wg - 1) / per_wg);
+    tk.ksplit = ks;
+    tk.part_off = 0;
+    if (ks > 1) {
+      tk.part_off = partial;
+      partial += (int64_t)ks * tk.M * tk.N;
+      split.push_back((int32_t)i);
+      max_mn = std::max<int64_t>(max_mn, (int64_t)tk.M * tk.N);
+    }
+    tk.wg_begin = wg;
+    wg += tk.tiles_m * tk.tiles_n * ks;
+    if (wg < 0) return fail(DPD_EINVAL, "grouped GEMM too large");
+  }
+  G.nwg = wg;
+  G.ntask = (int)all.size();
+  G.nsplit = (int)split.size();
+  G.max_split_mn = max_mn;
+  G.partial_elems = partial;
   if (all.empty()) return DPD_OK;
-  int rc = dev.alloc(sizeof(GemmTask) * all.size());
+  const size_t bt = ((sizeof(GemmTask) * all.size() + 255) & ~size_t(255)), bs = sizeof(int32_t) * split.size();
+  int rc = G.dev.alloc(bt + bs + 256);
   if (rc) return rc;
-  DPD_HIP_TRY(hipMemcpy(dev.p, all.data(), sizeof(GemmTask) * all.size(), hipMemcpyHostToDevice));
+  G.d_tasks = (GemmTask*)G.dev.p;
+  G.d_split = (int32_t*)((char*)G.dev.p + bt);
+  DPD_HIP_TRY(hipMemcpy(G.d_tasks, all.data(), sizeof(GemmTask) * all.size(), hipMemcpyHostToDevice));
+  if (bs) DPD_HIP_TRY(hipMemcpy(G.d_split, split.data(), bs, hipMemcpyHostToDevice));
   return DPD_OK;
+}
+
+static void group_launch_args(const GroupTasks& T, int prec, int cfg, double fact, double* partial, GemmLaunch& g) {
+  std::memset(&g, 0, sizeof g);
+  g.tasks = T.d_tasks;
+  g.ntask = T.ntask;
+  g.nwg = T.nwg;
+  g.alpha = fact;
+  g.prec = prec;
+  g.cfg = cfg;
+  g.partial = partial;
+  g.split_tasks = T.d_split;
+  g.nsplit = T.nsplit;
+  g.max_split_mn = T.max_split_mn;
 }
 
 static bool capturing(hipStream_t s) {
@@ -1623,18 +1667,13 @@ static int tune_group(dpd_ctx* c, GroupPlan& G, int prec, const std::vector<Call
       gt.e[i] = {pa[i].p, pb[i].p, (double*)cs[i].p, 0.0};
     }
     if (rc) break;
-    DevBuf dev;
-    int nwg = 0, ntask = 0;
-    if ((rc = group_tasks(Vs, dev, &nwg, &ntask))) break;
-    if (nwg == 0) continue;
+    GroupTasks TT;
+    if ((rc = group_tasks(Vs, prec, TT))) break;
+    if (TT.nwg == 0) continue;
+    DevBuf part;
+    if (TT.partial_elems && (rc = part.alloc(sizeof(double) * TT.partial_elems))) break;
     GemmLaunch g;
-    std::memset(&g, 0, sizeof g);
-    g.tasks = (const GemmTask*)dev.p;
-    g.ntask = ntask;
-    g.nwg = nwg;
-    g.alpha = fact;
-    g.prec = prec;
-    g.cfg = cfg;
+    group_launch_args(TT, prec, cfg, fact, (double*)part.p, g);
     DPD_HIP_TRY(launch_gemm_pk_grouped(g, gt, c->stream));
     DPD_HIP_TRY(hipEventRecord(e0, c->stream));
     for (int r = 0; r < 3; r++) DPD_HIP_TRY(launch_gemm_pk_grouped(g, gt, c->stream));
@@ -1717,11 +1756,11 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
     if (cap) return fail(DPD_EINVAL, "grouped task list built during graph capture (internal)");
     std::vector<const GemmVariant*> Vs;
     for (ContractPlan* P : plans) Vs.push_back(&P->gemm.pk[prec]);
-    if ((rc = group_tasks(Vs, G->dev, &G->nwg, &G->ntask))) return rc;
+    if ((rc = group_tasks(Vs, prec, G->gt))) return rc;
     G->built_cfg = G->cfg;
   }
   std::vector<CallPrep> R(n);
-  size_t need = 0;
+  size_t need = sizeof(double) * G->gt.partial_elems + 256;
   for (int i = 0; i < n; i++)
     if ((rc = call_setup(c, plans[i], o1[i], o2[i], nbatch[i], prec, R[i], &need))) return rc;
   ws_reset(c);
@@ -1739,14 +1778,11 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
     flops += plans[i]->gemm.flops;
     if (c->prof.on && tag) bytes += call_bytes(c, R[i], o1[i], o2[i], prec);
   }
+  double* partial = nullptr;
+  if (G->gt.partial_elems && !(partial = (double*)ws_take(c, sizeof(double) * G->gt.partial_elems)))
+    return DPD_ENOMEM;
   GemmLaunch g;
-  std::memset(&g, 0, sizeof g);
-  g.tasks = (const GemmTask*)G->dev.p;
-  g.ntask = G->ntask;
-  g.nwg = G->nwg;
-  g.alpha = fact;
-  g.prec = prec;
-  g.cfg = G->cfg;
+  group_launch_args(G->gt, prec, G->cfg, fact, partial, g);
   ProfRec* pr = cap ? nullptr : prof_open(c, tag, flops, prec, G->cfg, 1, bytes);
   if (pr) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
   DPD_HIP_TRY(launch_gemm_pk_grouped(g, gt, c->stream));
@@ -1761,7 +1797,7 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
       if ((rc = group_variants(*G, prec, G->cfg))) return rc;
       std::vector<const GemmVariant*> Vs;
       for (ContractPlan* P : plans) Vs.push_back(&P->gemm.pk[prec]);
-      if ((rc = group_tasks(Vs, G->dev, &G->nwg, &G->ntask))) return rc;
+      if ((rc = group_tasks(Vs, prec, G->gt))) return rc;
       G->built_cfg = G->cfg;
     }
   }

@@ -116,13 +116,18 @@ struct ContractPlan {
 };
 // one grouped contraction (the same contraction of several problems): the
 // members' plans, the tile shape they share and the merged task list
+struct GroupTasks {
+  DevBuf dev;  // merged tasks, then the ids of the split-K tasks
+  GemmTask* d_tasks = nullptr;
+  int32_t* d_split = nullptr;
+  int nwg = 0, ntask = 0, nsplit = 0, max_split_mn = 0;
+  int64_t partial_elems = 0;
+};
 struct GroupPlan {
   int cfg = -1;
   bool tuned = false;
   int built_cfg = -1;
-  int nwg = 0;
-  int ntask = 0;
-  DevBuf dev;
+  GroupTasks gt;
   std::vector<ContractPlan*> members;
 };
 

@@ -492,16 +492,22 @@ static hipError_t launch_pk_grp_mode(const GemmLaunch& g, const GroupTab& gt, hi
 
 hipError_t launch_gemm_pk_grouped(const GemmLaunch& g, const GroupTab& gt, hipStream_t s) {
   if (g.nwg == 0) return hipSuccess;
+  hipError_t e;
   switch (g.prec) {
     case PREC_F64:
-      return launch_pk_grp_mode<PREC_F64>(g, gt, s);
+      e = launch_pk_grp_mode<PREC_F64>(g, gt, s);
+      break;
     case PREC_F32:
-      return launch_pk_grp_mode<PREC_F32>(g, gt, s);
+      e = launch_pk_grp_mode<PREC_F32>(g, gt, s);
+      break;
     case PREC_BF16X3:
-      return launch_pk_grp_mode<PREC_BF16X3>(g, gt, s);
+      e = launch_pk_grp_mode<PREC_BF16X3>(g, gt, s);
+      break;
     default:
       return hipErrorInvalidValue;
   }
+  if (e != hipSuccess) return e;
+  return launch_splitk_reduce_grouped(g, gt, s);
 }
 
 template <int MODE>

@@ -354,6 +354,32 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(GemmLaunch g) {
   }
 }
 
+// grouped launches: the result and beta of a task come from its group entry
+__global__ __launch_bounds__(256) void k_splitk_reduce_grp(GemmLaunch g, GroupTab gt) {
+  const GemmTask& T = g.tasks[g.split_tasks[blockIdx.y]];
+  const GroupEnt& ge = gt.e[T.group];
+  double* C = ge.c;
+  const double beta = ge.beta;
+  const int64_t mn = (int64_t)T.M * T.N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < mn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double sum = 0.0;
+    for (int kp = 0; kp < T.ksplit; kp++) sum += g.partial[T.part_off + kp * mn + e];
+    const int m = (int)(e % T.M), n = (int)(e / T.M);
+    const int64_t ad = T.c_off + digits3(m, T.md, T.sc_m) + digits3(n, T.nd, T.sc_n);
+    C[ad] = (beta == 0.0) ? sum : beta * C[ad] + sum;
+  }
+}
+
+hipError_t launch_splitk_reduce_grouped(const GemmLaunch& g, const GroupTab& gt, hipStream_t s) {
+  if (g.nsplit == 0) return hipSuccess;
+  int bx = (g.max_split_mn + 255) / 256;
+  if (bx > 1024) bx = 1024;
+  if (bx < 1) bx = 1;
+  hipLaunchKernelGGL(k_splitk_reduce_grp, dim3(bx, g.nsplit), dim3(256), 0, s, g, gt);
+  return hipGetLastError();
+}
+
 hipError_t launch_gemm(const GemmLaunch& g, hipStream_t s) {
   if (g.nwg == 0) return hipSuccess;
   if (g.cfg != GEMM_V1) {

@@ -135,7 +135,7 @@ inline int pk_esize(int mode) { return mode == PREC_F64 ? 8 : mode == PREC_F32 ?
 hipError_t launch_gemm_pk(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s);
 // grouped launch: the tasks of several independent contractions (one per
 // problem of a batch) in one kernel; task.group selects the panels, result and
-// beta of its contraction.  No split-K (every task has ksplit = 1).
+// beta of its contraction; split-K partials of all tasks share g.partial.
 constexpr int kMaxGroup = 32;
 struct GroupEnt {
   const void* pa;
@@ -147,6 +147,7 @@ struct GroupTab {
   GroupEnt e[kMaxGroup];
 };
 hipError_t launch_gemm_pk_grouped(const GemmLaunch& g, const GroupTab& gt, hipStream_t s);
+hipError_t launch_splitk_reduce_grouped(const GemmLaunch& g, const GroupTab& gt, hipStream_t s);
 
 // ---- vector kernels (FP64) ------------------------------------------------
 constexpr int kRedBlocks = 512;
