@@ -370,6 +370,43 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   auto bv = [&](int i) { return B + (size_t)i * n; };
   auto abv = [&](int i) { return AB + (size_t)i * n; };
   int64_t nsig = 0, nskip = 0;
+  // small host<->device transfers go through pinned buffers: device-to-host
+  // copies land in pin_out and are read after the synchronisation; uploads are
+  // staged in pin_in slots that stay untouched until the next synchronisation
+  struct Pinned {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~Pinned() {
+      if (p) (void)hipHostFree(p);
+    }
+  } pin_out, pin_in;
+  const size_t pin_bytes = sizeof(double) * (4 * nd2 + 4 * (size_t)dimAbar * nroot + 1024);
+  DPD_HIP_TRY(hipHostMalloc(&pin_out.p, pin_bytes, hipHostMallocDefault));
+  DPD_HIP_TRY(hipHostMalloc(&pin_in.p, pin_bytes, hipHostMallocDefault));
+  pin_out.bytes = pin_in.bytes = pin_bytes;
+  size_t in_top = 0;
+  auto fetch = [&](void* host, const void* dev, size_t bytes) -> int {
+    if (bytes > pin_out.bytes) return fail(DPD_EINVAL, "transfer too large (internal)");
+    if (bytes) DPD_HIP_TRY(hipMemcpyAsync(pin_out.p, dev, bytes, hipMemcpyDeviceToHost, st));
+    int r = dsync(st);
+    if (r) return r;
+    if (bytes) std::memcpy(host, pin_out.p, bytes);
+    in_top = 0;  // every staged upload has completed
+    return DPD_OK;
+  };
+  auto upload = [&](void* dev, const void* host, size_t bytes) -> int {
+    if (bytes > pin_in.bytes) return fail(DPD_EINVAL, "transfer too large (internal)");
+    if (in_top + bytes > pin_in.bytes) {
+      int r = dsync(st);
+      if (r) return r;
+      in_top = 0;
+    }
+    void* slot = (char*)pin_in.p + in_top;
+    std::memcpy(slot, host, bytes);
+    DPD_HIP_TRY(hipMemcpyAsync(dev, slot, bytes, hipMemcpyHostToDevice, st));
+    in_top += (bytes + 255) & ~size_t(255);
+    return DPD_OK;
+  };
   auto sigma = [&](const double* x, double* y, int nv) -> int {
     nsig += nv;
     if (hook) return (*hook)(x, y, nv);
@@ -378,15 +415,13 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   // all dots X_i . Y_j (i < nx, j < ny) -> host[i + j*nx], one synchronisation
   auto block_dots = [&](const double* X, int nx, const double* Y, int ny, double* host) -> int {
     DPD_HIP_TRY(launch_block_dots(n, X, n, nx, Y, n, ny, part, dots, st));
-    DPD_HIP_TRY(hipMemcpyAsync(host, dots, sizeof(double) * nx * ny, hipMemcpyDeviceToHost, st));
-    return dsync(st);
+    return fetch(host, dots, sizeof(double) * nx * ny);
   };
   // e_v = A r_v - w_v r_v and max|e_v| for v < nroot (w already on the device)
   auto residuals = [&](const double* AR, double* res_host) -> int {
     DPD_HIP_TRY(launch_resid_amax(n, AR, R, wdev, E, nroot, (double*)W.amaxp.p, (int64_t*)W.amaxpi.p,
                                   (double*)W.amaxv.p, (int64_t*)W.amaxi.p, st));
-    DPD_HIP_TRY(hipMemcpyAsync(res_host, W.amaxv.p, sizeof(double) * nroot, hipMemcpyDeviceToHost, st));
-    return dsync(st);
+    return fetch(res_host, W.amaxv.p, sizeof(double) * nroot);
   };
 
   std::vector<double> Abar(nd2, 0.0);  // column-major Abar(i,j) = [i + j*dimAbar]
@@ -467,7 +502,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
         nr = std::sqrt(nr);
         for (int i = 0; i < dimIter; i++) nw[i + (size_t)a * dimIter] /= nr;
       }
-      DPD_HIP_TRY(hipMemcpyAsync(W.coef.p, nw.data(), sizeof(double) * nw.size(), hipMemcpyHostToDevice, st));
+      if ((rc = upload(W.coef.p, nw.data(), sizeof(double) * nw.size()))) return rc;
       DPD_HIP_TRY(launch_lincomb(n, AB, n, dimIter, (const double*)W.coef.p, dimIter, ndimIter, (double*)W.NAB.p, st));
       DPD_HIP_TRY(launch_lincomb(n, B, n, dimIter, (const double*)W.coef.p, dimIter, ndimIter, (double*)W.NB.p, st));
       std::swap(W.B.p, W.NB.p);
@@ -486,15 +521,14 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       DPD_HIP_TRY(hipMemcpyAsync(R, B, vb * nroot, hipMemcpyDeviceToDevice, st));
       DPD_HIP_TRY(hipMemcpyAsync(AQ, AB, vb * nroot, hipMemcpyDeviceToDevice, st));
     } else {
-      DPD_HIP_TRY(hipMemcpyAsync(W.coef.p, vecNeed.data(), sizeof(double) * dimAbar * nroot, hipMemcpyHostToDevice,
-                                 st));
+      if ((rc = upload(W.coef.p, vecNeed.data(), sizeof(double) * dimAbar * nroot))) return rc;
       DPD_HIP_TRY(launch_lincomb(n, AB, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, AQ, st));
       DPD_HIP_TRY(launch_lincomb(n, B, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, R, st));
     }
     DPD_HIP_TRY(launch_pair_dots(n, R, n, R, n, nroot, part, dots, st));
     DPD_HIP_TRY(launch_renorm_pair(n, dots, R, AQ, nroot, st));
     // residuals (:396-406)
-    DPD_HIP_TRY(hipMemcpyAsync(wdev, eigenVal.data(), sizeof(double) * nroot, hipMemcpyHostToDevice, st));
+    if ((rc = upload(wdev, eigenVal.data(), sizeof(double) * nroot))) return rc;
     if ((rc = residuals(AQ, mv.data()))) return rc;
     int nconv = 0;
     cand.clear();
@@ -506,7 +540,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     // preconditioned directions of every unconverged root (:417-428)
     int nNewBase = 0;
     if (dimFree > 0 && !cand.empty()) {
-      DPD_HIP_TRY(hipMemcpyAsync(W.idx.p, cand.data(), sizeof(int32_t) * cand.size(), hipMemcpyHostToDevice, st));
+      if ((rc = upload(W.idx.p, cand.data(), sizeof(int32_t) * cand.size()))) return rc;
       DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p, (int)cand.size(), diag, E, PV, st));
     }
     // SchmidtS (:687-726) root by root: normalise, remove the components along
@@ -576,8 +610,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       }
       std::vector<double> s2(nopt);
       if (nopt) {
-        DPD_HIP_TRY(hipMemcpyAsync(s2.data(), dots, sizeof(double) * nopt, hipMemcpyDeviceToHost, st));
-        if ((rc = dsync(st))) return rc;
+        if ((rc = fetch(s2.data(), dots, sizeof(double) * nopt))) return rc;
       }
       size_t k = 0;
       for (; k < nopt && std::sqrt(s2[k]) >= 1e-6; k++) {
@@ -592,8 +625,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
           DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p + k, 1, diag, E, PV + k * n, st));
           if ((rc = schmidt(k, nNewBase, dots))) return rc;
           double t2;
-          DPD_HIP_TRY(hipMemcpyAsync(&t2, dots, sizeof(double), hipMemcpyDeviceToHost, st));
-          if ((rc = dsync(st))) return rc;
+          if ((rc = fetch(&t2, dots, sizeof(double)))) return rc;
           if (std::sqrt(t2) < 1e-6) {
             nskip++;
             continue;
@@ -640,8 +672,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
         off2.push_back(off);
         off += (size_t)nv * nt;
       }
-      DPD_HIP_TRY(hipMemcpyAsync(hd.data(), dots, sizeof(double) * off, hipMemcpyDeviceToHost, st));
-      if ((rc = dsync(st))) return rc;
+      if ((rc = fetch(hd.data(), dots, sizeof(double) * off))) return rc;
       for (int c0 = 0, ci = 0; c0 < nNewBase; c0 += kMgsMulti, ci++) {
         const int nv = std::min(kMgsMulti, nNewBase - c0);
         for (int j = 0; j < dimIter; j++)
@@ -659,7 +690,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   // final check (:527-542): e = A R - w R
   if (o->final_check) {
     if ((rc = sigma(R, (double*)W.NAB.p, nroot))) return rc;
-    DPD_HIP_TRY(hipMemcpyAsync(wdev, eigenVal.data(), sizeof(double) * nroot, hipMemcpyHostToDevice, st));
+    if ((rc = upload(wdev, eigenVal.data(), sizeof(double) * nroot))) return rc;
     if ((rc = residuals((const double*)W.NAB.p, mv.data()))) return rc;
     for (int i = 0; i < nroot && i < 64; i++) o->final_residual[i] = std::fabs(mv[i]);
   }

@@ -97,30 +97,43 @@ __global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const double
   int h[4];
   int64_t x[4];
   decode_pair(L.pop[2], L.pop[3], L.iop[1], L.dp, L.offR[xr], L.nirrep, xr, r, &h[2], &x[2], &h[3], &x[3]);
-  const int bt = blockIdx.y;
-  const double* inb = in + (int64_t)bt * a.in_bstride;
-  double* outb = out + (int64_t)bt * a.out_bstride;
   const int64_t col = L.blk[xr] + r * nl;
   const int64_t l1 = l0 + kGatherChunk < nl ? l0 + kGatherChunk : nl;
+  // the element's source addresses are decoded once and serve every tensor of
+  // the batch (the decode, not the memory traffic, bounds this kernel)
   for (int64_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
     decode_pair(L.pop[0], L.pop[1], L.iop[0], L.dp, L.offL[xl], L.nirrep, xl, l, &h[0], &x[0], &h[1], &x[1]);
-    double acc = 0.0;
-    for (int t = 0; t < a.nterm; t++) {
-      int hi[4];
-      int64_t xi[4];
+    int64_t ad[kMaxTerms];
+    double cf[kMaxTerms];
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        hi[a.term[t].perm[i]] = h[i];
-        xi[a.term[t].perm[i]] = x[i];
+    for (int t = 0; t < kMaxTerms; t++) {
+      ad[t] = -1;
+      cf[t] = 0.0;
+      if (t < a.nterm) {
+        int hi[4];
+        int64_t xi[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          hi[a.term[t].perm[i]] = h[i];
+          xi[a.term[t].perm[i]] = x[i];
+        }
+        double sg;
+        ad[t] = fetch_addr(a.in, hi[0], hi[1], hi[2], hi[3], xi[0], xi[1], xi[2], xi[3], &sg);
+        cf[t] = a.term[t].coef * sg;
       }
-      double sg;
-      const int64_t ad = fetch_addr(a.in, hi[0], hi[1], hi[2], hi[3], xi[0], xi[1], xi[2], xi[3], &sg);
-      if (ad >= 0) acc += a.term[t].coef * sg * inb[ad];
     }
     const int64_t o = col + l;
-    double v = a.alpha * acc;
-    if (a.beta != 0.0) v = a.beta * outb[o] + v;
-    outb[o] = v;
+    for (int bt = 0; bt < a.nbatch; bt++) {
+      const double* inb = in + (int64_t)bt * a.in_bstride;
+      double* outb = out + (int64_t)bt * a.out_bstride;
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < kMaxTerms; t++)
+        if (ad[t] >= 0) acc += cf[t] * inb[ad[t]];
+      double v = a.alpha * acc;
+      if (a.beta != 0.0) v = a.beta * outb[o] + v;
+      outb[o] = v;
+    }
   }
 }
 
@@ -139,7 +152,7 @@ hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hi
     hipLaunchKernelGGL(k_gather_map, dim3((unsigned)b2, (unsigned)a.nbatch), dim3(256), 0, s, a, in, out);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)blocks, (unsigned)a.nbatch), dim3(256), 0, s, a, in, out);
+  hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)blocks, 1), dim3(256), 0, s, a, in, out);
   return hipGetLastError();
 }
 
