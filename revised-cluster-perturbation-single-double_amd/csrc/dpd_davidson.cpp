@@ -296,7 +296,7 @@ int small_eig(int n, const double* Arow, double* wr, double* wi, double* V) {
 // ---------------------------------------------------------------------------
 struct DavWork {
   DevBuf B, AB, NB, NAB, AQ, E, P, coef, part, partA, partB, partC, partD, dots, amaxp, amaxpi, amaxv, amaxi, wdev,
-      idx;
+      idx, cgs;
 };
 
 static int dsync(hipStream_t s) {
@@ -312,6 +312,9 @@ extern "C" int dpd_small_eig(int64_t n, const double* A_rowmajor, double* wr, do
   if (n < 0 || n > 4096 || (n && (!A_rowmajor || !wr || !wi || !V_colmajor))) return fail(DPD_EINVAL, "bad args");
   return small_eig((int)n, A_rowmajor, wr, wi, V_colmajor);
 }
+
+// slots at the start of the transfer buffer for the norms of the new directions
+constexpr size_t kNormSlots = 64;
 
 // sigma of nv vectors for one Davidson; null: the problem's own operator
 typedef std::function<int(const double* x, double* y, int nv)> SigmaHook;
@@ -352,11 +355,11 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       (rc = W.part.alloc(sizeof(double) * kRedBlocks * std::max<size_t>(nd2, (size_t)kMgsMulti * dimAbar))) ||
       (rc = W.partA.alloc(sizeof(double) * kRedBlocks * kMgsMulti)) ||
       (rc = W.partB.alloc(sizeof(double) * kRedBlocks * kMgsMulti)) ||
-      (rc = W.partC.alloc(sizeof(double) * kRedBlocks)) || (rc = W.partD.alloc(sizeof(double) * kRedBlocks)) || (rc = W.dots.alloc(sizeof(double) * 2 * nd2)) ||
+      (rc = W.partC.alloc(sizeof(double) * kRedBlocks)) || (rc = W.partD.alloc(sizeof(double) * kRedBlocks)) || (rc = W.dots.alloc(sizeof(double) * (2 * nd2 + kNormSlots))) ||
       (rc = W.amaxp.alloc(sizeof(double) * kRedBlocks * nroot)) ||
       (rc = W.amaxpi.alloc(sizeof(int64_t) * kRedBlocks * nroot)) || (rc = W.amaxv.alloc(sizeof(double) * nroot)) ||
       (rc = W.amaxi.alloc(sizeof(int64_t) * nroot)) || (rc = W.wdev.alloc(sizeof(double) * nroot)) ||
-      (rc = W.idx.alloc(sizeof(int32_t) * nroot)))
+      (rc = W.idx.alloc(sizeof(int32_t) * nroot)) || (rc = W.cgs.alloc(256)))
     return rc;
   double* B = (double*)W.B.p;
   double* AB = (double*)W.AB.p;
@@ -427,7 +430,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   std::vector<double> Abar(nd2, 0.0);  // column-major Abar(i,j) = [i + j*dimAbar]
   auto A_ = [&](int i, int j) -> double& { return Abar[i + (size_t)j * dimAbar]; };
   std::vector<double> vecNeed((size_t)dimAbar * nroot, 0.0), eigenVal(nroot, 0.0), res(nroot, 0.0);
-  std::vector<double> hd(2 * nd2);
+  std::vector<double> hd(2 * nd2 + kNormSlots);
   // initDavidson (:563-610): b_i = guess_i, Ab_i = sigma(b_i), Abar(j,i) = g_j . Ab_i
   DPD_HIP_TRY(hipMemcpyAsync(B, R, vb * nroot, hipMemcpyDeviceToDevice, st));
   if ((rc = sigma(B, AB, nroot))) return rc;
@@ -539,6 +542,36 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     }
     // preconditioned directions of every unconverged root (:417-428)
     int nNewBase = 0;
+    size_t spec = 0;  // directions orthogonalised this iteration, assumed kept
+    // new Abar entries of nnew new vectors: rows against the old columns, then
+    // the new columns against every basis vector (each basis vector is read once
+    // per group of up to kMgsMulti new vectors); written to dots from `off` on
+    auto abar_dots = [&](int nnew, size_t& off, std::vector<size_t>& off1, std::vector<size_t>& off2) -> int {
+      const int nt = dimIter + nnew;
+      for (int c0 = 0; c0 < nnew; c0 += kMgsMulti) {
+        const int nv = std::min(kMgsMulti, nnew - c0);
+        DPD_HIP_TRY(launch_dots_multi(n, bv(dimIter + c0), nv, AB, dimIter, part, dots + off, st));
+        off1.push_back(off);
+        off += (size_t)nv * dimIter;
+      }
+      for (int c0 = 0; c0 < nnew; c0 += kMgsMulti) {
+        const int nv = std::min(kMgsMulti, nnew - c0);
+        DPD_HIP_TRY(launch_dots_multi(n, abv(dimIter + c0), nv, B, nt, part, dots + off, st));
+        off2.push_back(off);
+        off += (size_t)nv * nt;
+      }
+      return DPD_OK;
+    };
+    auto abar_fill = [&](int nnew, const std::vector<size_t>& off1, const std::vector<size_t>& off2) {
+      const int nt = dimIter + nnew;
+      for (int c0 = 0, ci = 0; c0 < nnew; c0 += kMgsMulti, ci++) {
+        const int nv = std::min(kMgsMulti, nnew - c0);
+        for (int j = 0; j < dimIter; j++)
+          for (int i = 0; i < nv; i++) A_(dimIter + c0 + i, j) = hd[off1[ci] + i + (size_t)j * nv];
+        for (int j = 0; j < nv; j++)
+          for (int i = 0; i < nt; i++) A_(i, dimIter + c0 + j) = hd[off2[ci] + j + (size_t)i * nv];
+      }
+    };
     if (dimFree > 0 && !cand.empty()) {
       if ((rc = upload(W.idx.p, cand.data(), sizeof(int32_t) * cand.size()))) return rc;
       DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p, (int)cand.size(), diag, E, PV, st));
@@ -584,9 +617,13 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
         // in exact arithmetic the same vector as the sequential projections)
         DPD_HIP_TRY(launch_pair_dots(n, P0, n, P0, n, nv, pin, nullptr, st));
         DPD_HIP_TRY(launch_mgs_multi(n, P0, nv, nullptr, nullptr, pin, 0, pout, st));
+        // the second projection runs only when some direction lost more than
+        // half its squared norm to the first (decided on the device)
+        int* need = (int*)W.cgs.p;
         for (int pass = 0; pass < 2; pass++) {
-          DPD_HIP_TRY(launch_dots_multi(n, P0, nv, B, dimIter, part, (double*)W.coef.p, st));
-          DPD_HIP_TRY(launch_lincomb_sub(n, B, dimIter, (const double*)W.coef.p, nv, P0, st));
+          DPD_HIP_TRY(launch_dots_multi(n, P0, nv, B, dimIter, part, (double*)W.coef.p, st, pass ? need : nullptr));
+          if (pass == 0 && dimIter > 0) DPD_HIP_TRY(launch_cgs_need((const double*)W.coef.p, nv, dimIter, need, st));
+          DPD_HIP_TRY(launch_lincomb_sub(n, B, dimIter, (const double*)W.coef.p, nv, P0, st, pass ? need : nullptr));
         }
         DPD_HIP_TRY(launch_pair_dots(n, P0, n, P0, n, nv, pin, nullptr, st));
         for (int j = 0; j < nv; j++) {
@@ -608,18 +645,30 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
           DPD_HIP_TRY(launch_scale_by_norm(n, pk, fin, bv(dimIter + (int)k), st));
         }
       }
-      std::vector<double> s2(nopt);
-      if (nopt) {
-        if ((rc = fetch(s2.data(), dots, sizeof(double) * nopt))) return rc;
-      }
+      spec = nopt;
+    }
+    // sigma of this iteration's new vectors, one batched evaluation (:442-444).
+    // Optimistically every direction is kept: sigma and the new rows and
+    // columns of Abar are launched before the norms are read, and norms and
+    // Abar entries come back in one transfer.  If a direction fell below 1e-6
+    // (rare) the ones from it on are dropped and redone one at a time.
+    bool abar_done = false;
+    if (spec > 0) {
+      if ((rc = sigma(bv(dimIter), abv(dimIter), (int)spec))) return rc;
+      std::vector<size_t> off1, off2;
+      size_t off = kNormSlots;
+      if ((rc = abar_dots((int)spec, off, off1, off2))) return rc;
+      if ((rc = fetch(hd.data(), dots, sizeof(double) * off))) return rc;
       size_t k = 0;
-      for (; k < nopt && std::sqrt(s2[k]) >= 1e-6; k++) {
-        dimFree--;
-        nNewBase++;
-      }
-      if (k < nopt) {
+      while (k < spec && std::sqrt(hd[k]) >= 1e-6) k++;
+      dimFree -= (int)k;
+      nNewBase = (int)k;
+      if (k == spec) {
+        abar_fill((int)spec, off1, off2);
+        abar_done = true;
+      } else {
         nskip++;
-        // root k is skipped: the roots after it were projected against it, redo them
+        // direction k is dropped: the ones after it were projected against it
         for (k = k + 1; k < cand.size(); k++) {
           if (dimFree == 0) continue;
           DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p + k, 1, diag, E, PV + k * n, st));
@@ -630,14 +679,13 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
             nskip++;
             continue;
           }
+          // its sigma now (the optimistic one was computed before the rollback)
+          if ((rc = sigma(bv(dimIter + nNewBase), abv(dimIter + nNewBase), 1))) return rc;
           dimFree--;
           nNewBase++;
         }
       }
     }
-    // sigma of this iteration's new vectors, one batched evaluation (:442-444)
-    if (nNewBase > 0)
-      if ((rc = sigma(bv(dimIter), abv(dimIter), nNewBase))) return rc;
     if (nconv == nroot) {
       converged = true;
       break;
@@ -655,31 +703,12 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     }
     // Abar update (:484-509): new rows against the old columns, then the new
     // columns against every basis vector
-    if (nNewBase > 0) {
-      // every basis vector is read once per group of up to kMgsMulti new vectors
-      const int nt = dimIter + nNewBase;
-      size_t off = 0;
+    if (nNewBase > 0 && !abar_done) {
       std::vector<size_t> off1, off2;
-      for (int c0 = 0; c0 < nNewBase; c0 += kMgsMulti) {
-        const int nv = std::min(kMgsMulti, nNewBase - c0);
-        DPD_HIP_TRY(launch_dots_multi(n, bv(dimIter + c0), nv, AB, dimIter, part, dots + off, st));
-        off1.push_back(off);
-        off += (size_t)nv * dimIter;
-      }
-      for (int c0 = 0; c0 < nNewBase; c0 += kMgsMulti) {
-        const int nv = std::min(kMgsMulti, nNewBase - c0);
-        DPD_HIP_TRY(launch_dots_multi(n, abv(dimIter + c0), nv, B, nt, part, dots + off, st));
-        off2.push_back(off);
-        off += (size_t)nv * nt;
-      }
+      size_t off = 0;
+      if ((rc = abar_dots(nNewBase, off, off1, off2))) return rc;
       if ((rc = fetch(hd.data(), dots, sizeof(double) * off))) return rc;
-      for (int c0 = 0, ci = 0; c0 < nNewBase; c0 += kMgsMulti, ci++) {
-        const int nv = std::min(kMgsMulti, nNewBase - c0);
-        for (int j = 0; j < dimIter; j++)
-          for (int i = 0; i < nv; i++) A_(dimIter + c0 + i, j) = hd[off1[ci] + i + (size_t)j * nv];
-        for (int j = 0; j < nv; j++)
-          for (int i = 0; i < nt; i++) A_(i, dimIter + c0 + j) = hd[off2[ci] + j + (size_t)i * nv];
-      }
+      abar_fill(nNewBase, off1, off2);
     }
     dimIter += nNewBase;
     iterations++;

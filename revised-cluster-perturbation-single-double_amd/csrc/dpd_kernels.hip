@@ -1002,8 +1002,10 @@ hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, c
 // vectors B (stride n); grid (kRedBlocks, ceil(nb/4)); each basis vector is read
 // once, partials reduced in a fixed order
 __global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __restrict__ P, int nv,
-                                                    const double* __restrict__ B, int nb, double* partial) {
+                                                    const double* __restrict__ B, int nb, double* partial,
+                                                    const int* __restrict__ need) {
   __shared__ double red[4];
+  if (need && *need == 0) return;
   const int k0 = blockIdx.y * 4;
   const int nk = min(4, nb - k0);
   double acc[kMgsMulti][4];
@@ -1035,17 +1037,19 @@ __global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __r
 }
 
 hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
-                             double* out, hipStream_t s) {
+                             double* out, hipStream_t s, const int* need) {
   if (nv <= 0 || nb <= 0) return hipSuccess;
   if (nv > kMgsMulti) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_dots_multi, dim3(kRedBlocks, (nb + 3) / 4), dim3(256), 0, s, n, P, nv, B, nb, partial);
+  hipLaunchKernelGGL(k_dots_multi, dim3(kRedBlocks, (nb + 3) / 4), dim3(256), 0, s, n, P, nv, B, nb, partial,
+                     need);
   hipLaunchKernelGGL(k_reduce_rows, dim3(nv * nb), dim3(256), 0, s, partial, kRedBlocks, out);
   return hipGetLastError();
 }
 
 // P_j -= sum_k C[j + k*nv] B_k for j < nv <= kMgsMulti (k ascending)
 __global__ void k_lincomb_sub(int64_t n, const double* __restrict__ B, int nb, const double* __restrict__ C, int nv,
-                              double* __restrict__ P) {
+                              double* __restrict__ P, const int* __restrict__ need) {
+  if (need && *need == 0) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double s[kMgsMulti];
 #pragma unroll
@@ -1062,10 +1066,33 @@ __global__ void k_lincomb_sub(int64_t n, const double* __restrict__ B, int nb, c
   }
 }
 
-hipError_t launch_lincomb_sub(int64_t n, const double* B, int nb, const double* C, int nv, double* P, hipStream_t s) {
+hipError_t launch_lincomb_sub(int64_t n, const double* B, int nb, const double* C, int nv, double* P, hipStream_t s,
+                              const int* need) {
   if (nv <= 0 || nb <= 0) return hipSuccess;
   if (nv > kMgsMulti) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_lincomb_sub, dim3(grid_for(n)), dim3(256), 0, s, n, B, nb, C, nv, P);
+  hipLaunchKernelGGL(k_lincomb_sub, dim3(grid_for(n)), dim3(256), 0, s, n, B, nb, C, nv, P, need);
+  return hipGetLastError();
+}
+
+// re-orthogonalisation test of classical Gram-Schmidt ("twice is enough",
+// Kahan-Parlett): vectors of unit norm whose projections C (nv x nb, [j + k*nv])
+// on an orthonormal basis all have sum_k c_jk^2 <= 1/2 keep at least 1/sqrt(2)
+// of their norm, and one projection is as accurate as two; need = 0 then
+__global__ void k_cgs_need(const double* __restrict__ C, int nv, int nb, int* need) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < nv; j += blockDim.x) {
+    double s = 0.0;
+    for (int k = 0; k < nb; k++) s += C[j + (int64_t)k * nv] * C[j + (int64_t)k * nv];
+    if (!(s <= 0.5)) atomicOr(&any, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *need = any;
+}
+
+hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream_t s) {
+  hipLaunchKernelGGL(k_cgs_need, dim3(1), dim3(64), 0, s, C, nv, nb, need);
   return hipGetLastError();
 }
 

@@ -225,9 +225,14 @@ hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, c
                             const double* part_in, int mode, double* part_out, hipStream_t s);
 // out[j + k*nv] = P_j . B_k (partial: kRedBlocks * nv * nb), nv <= kMgsMulti
 hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
-                             double* out, hipStream_t s);
+                             double* out, hipStream_t s,
+                             const int* need = nullptr);
 // P_j -= sum_k C[j + k*nv] B_k, nv <= kMgsMulti
-hipError_t launch_lincomb_sub(int64_t n, const double* B, int nb, const double* C, int nv, double* P, hipStream_t s);
+hipError_t launch_lincomb_sub(int64_t n, const double* B, int nb, const double* C, int nv, double* P, hipStream_t s,
+                              const int* need = nullptr);
+// need = 1 when any of nv unit vectors lost more than half its squared norm to
+// its projections C on an orthonormal basis (a second projection is needed)
+hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream_t s);
 // dst = p * (1/sqrt(sum(part))) ; returns nothing (norm read by the host separately)
 hipError_t launch_scale_by_norm(int64_t n, const double* p, const double* part, double* dst, hipStream_t s);
 // p = e / clamp(w - d) for the roots listed in idx (w on device)
