@@ -765,9 +765,11 @@ int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const P
   g.max_split_mn = V.max_split_mn;
   g.cfg = V.cfg;
   if (V.partial_elems > 0 && !(g.partial = (double*)ws_take(c, sizeof(double) * V.partial_elems))) return DPD_ENOMEM;
+  // the events bracket the GEMM kernel alone (the split-K reduce follows)
   if (ev0) DPD_HIP_TRY(hipEventRecord(ev0, c->stream));
-  DPD_HIP_TRY(launch_gemm_pk(g, side_ptr[0], side_ptr[1], c->stream));
+  DPD_HIP_TRY(launch_gemm_pk_only(g, side_ptr[0], side_ptr[1], c->stream));
   if (ev1) DPD_HIP_TRY(hipEventRecord(ev1, c->stream));
+  DPD_HIP_TRY(launch_splitk_reduce(g, c->stream));
   return DPD_OK;
 }
 

@@ -538,6 +538,12 @@ static hipError_t launch_pk_mode(const GemmLaunch& g, const void* PA, const void
 }
 
 hipError_t launch_gemm_pk(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s) {
+  hipError_t e = launch_gemm_pk_only(g, PA, PB, s);
+  if (e != hipSuccess || g.nwg == 0) return e;
+  return launch_splitk_reduce(g, s);
+}
+
+hipError_t launch_gemm_pk_only(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s) {
   if (g.nwg == 0) return hipSuccess;
   hipError_t e;
   switch (g.prec) {
@@ -553,8 +559,7 @@ hipError_t launch_gemm_pk(const GemmLaunch& g, const void* PA, const void* PB, h
     default:
       return hipErrorInvalidValue;
   }
-  if (e != hipSuccess) return e;
-  return launch_splitk_reduce(g, s);
+  return e;
 }
 
 }  // namespace dpd

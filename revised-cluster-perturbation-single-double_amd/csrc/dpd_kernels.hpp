@@ -133,6 +133,8 @@ inline int pk_kstep(int mode) { return mode == PREC_F64 ? 16 : 32; }
 inline int pk_planes(int mode) { return mode == PREC_BF16X3 ? 2 : 1; }
 inline int pk_esize(int mode) { return mode == PREC_F64 ? 8 : mode == PREC_F32 ? 4 : 2; }
 hipError_t launch_gemm_pk(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s);
+// the GEMM kernel alone (split-K tasks leave partials for launch_splitk_reduce)
+hipError_t launch_gemm_pk_only(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s);
 // grouped launch: the tasks of several independent contractions (one per
 // problem of a batch) in one kernel; task.group selects the panels, result and
 // beta of its contraction; split-K partials of all tasks share g.partial.
