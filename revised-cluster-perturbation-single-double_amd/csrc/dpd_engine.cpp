@@ -586,11 +586,15 @@ int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested) {
   return best;
 }
 
-static int build_variant_cfg(GemmPlan& p, GemmVariant& V, int mode, int cfg) {
+// split_k: split long-K tasks over several workgroups while the launch has
+// fewer than 1024 tiles (partials summed by a reduce kernel); off: one
+// workgroup per tile
+static int build_variant_cfg(GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k = true) {
   V.mode = mode;
   V.tasks = p.tasks;
   V.split.clear();
   V.cfg = cfg;
+  V.split_k = split_k;
   const int BM = gemm_bm(V.cfg), BN = gemm_bn(V.cfg), KS = pk_kstep(mode), PL = pk_planes(mode);
   for (auto& tk : V.tasks) {
     tk.steps = (int32_t)(((int64_t)tk.ktiles_total * 16 + KS - 1) / KS);
@@ -611,7 +615,8 @@ static int build_variant_cfg(GemmPlan& p, GemmVariant& V, int mode, int cfg) {
   for (size_t i = 0; i < V.tasks.size(); i++) {
     GemmTask& tk = V.tasks[i];
     int ks = 1;
-    if (tiles < target && tk.steps > per_wg) ks = (int)std::min<int64_t>(64, (tk.steps + per_wg - 1) / per_wg);
+    if (split_k && tiles < target && tk.steps > per_wg)
+      ks = (int)std::min<int64_t>(64, (tk.steps + per_wg - 1) / per_wg);
     tk.ksplit = ks;
     tk.part_off = 0;
     if (ks > 1) {
@@ -778,9 +783,11 @@ int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const P
 // keeps the fastest for this plan and precision.  Runs once per plan.
 int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double f1, double f2) {
   // candidates in order of preference: among shapes within 5% of the fastest
-  // the first listed wins, which keeps the choice stable from run to run
+  // the first listed wins, which keeps the choice stable from run to run; each
+  // tile shape with and without split-K (the panels are the same for both)
   static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x128, GEMM_64x128, GEMM_64x64, GEMM_32x32};
-  float tms[sizeof cand / sizeof cand[0]];
+  constexpr int NC = (int)(sizeof cand / sizeof cand[0]);
+  float tms[NC][2];
   const size_t es = pk_esize(mode);
   DevBuf cs;
   int rc = cs.alloc(sizeof(double) * std::max<int64_t>(c_span(p), 1));
@@ -788,66 +795,80 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
   hipEvent_t e0, e1;
   DPD_HIP_TRY(hipEventCreate(&e0));
   DPD_HIP_TRY(hipEventCreate(&e1));
-  int best = -1;
-  float best_ms = 0.0f;
-  for (auto& t : tms) t = 1e30f;
-  for (const int& cfg : cand) {
-    GemmVariant T;
-    if ((rc = build_variant_cfg(p, T, mode, cfg))) break;
-    DevBuf ta, tb, tp;
-    if ((T.pa_elems && (rc = ta.alloc(es * T.pa_elems))) || (T.pb_elems && (rc = tb.alloc(es * T.pb_elems))) ||
-        (T.partial_elems && (rc = tp.alloc(sizeof(double) * T.partial_elems))))
-      break;
-    if ((T.pa_elems && (rc = pack_side(c, p, T, 0, A, ta.p))) || (T.pb_elems && (rc = pack_side(c, p, T, 1, B, tb.p))))
-      break;
-    GemmLaunch g;
-    std::memset(&g, 0, sizeof g);
-    g.C = (double*)cs.p;
-    g.tasks = T.d_tasks;
-    g.segs = p.d_segs;
-    g.ntask = (int)T.tasks.size();
-    g.nwg = T.nwg;
-    g.alpha = f1;
-    g.beta = f2;
-    g.prec = mode;
-    g.split_tasks = T.d_split;
-    g.nsplit = (int)T.split.size();
-    g.max_split_mn = T.max_split_mn;
-    g.cfg = cfg;
-    g.partial = (double*)tp.p;
-    DPD_HIP_TRY(launch_gemm_pk(g, ta.p, tb.p, c->stream));
-    DPD_HIP_TRY(hipEventRecord(e0, c->stream));
-    for (int r = 0; r < 3; r++) DPD_HIP_TRY(launch_gemm_pk(g, ta.p, tb.p, c->stream));
-    DPD_HIP_TRY(hipEventRecord(e1, c->stream));
-    DPD_HIP_TRY(hipEventSynchronize(e1));
-    float ms = 0.0f;
-    DPD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    tms[&cfg - cand] = ms;
-    if (best < 0 || ms < best_ms) {
-      best = cfg;
-      best_ms = ms;
-    }
-  }
-  if (!rc && best >= 0) {
-    for (size_t i = 0; i < sizeof cand / sizeof cand[0]; i++)
-      if (tms[i] <= 1.05f * best_ms) {
-        best = cand[i];
-        break;
+  float best_ms = -1.0f;
+  for (int ci = 0; ci < NC && !rc; ci++) {
+    tms[ci][0] = tms[ci][1] = 1e30f;
+    const int cfg = cand[ci];
+    DevBuf ta, tb;
+    bool had_split = false;
+    for (int sk = 0; sk < 2 && !rc; sk++) {
+      GemmVariant T;
+      if ((rc = build_variant_cfg(p, T, mode, cfg, sk == 0))) break;
+      if (sk == 0) had_split = !T.split.empty();
+      if (sk == 1 && !had_split) {
+        tms[ci][1] = tms[ci][0];  // nothing was split: the same variant
+        continue;
       }
+      DevBuf tp;
+      if (sk == 0 && ((T.pa_elems && (rc = ta.alloc(es * T.pa_elems))) ||
+                      (T.pb_elems && (rc = tb.alloc(es * T.pb_elems)))))
+        break;
+      if (T.partial_elems && (rc = tp.alloc(sizeof(double) * T.partial_elems))) break;
+      if (sk == 0 && ((T.pa_elems && (rc = pack_side(c, p, T, 0, A, ta.p))) ||
+                      (T.pb_elems && (rc = pack_side(c, p, T, 1, B, tb.p)))))
+        break;
+      GemmLaunch g;
+      std::memset(&g, 0, sizeof g);
+      g.C = (double*)cs.p;
+      g.tasks = T.d_tasks;
+      g.segs = p.d_segs;
+      g.ntask = (int)T.tasks.size();
+      g.nwg = T.nwg;
+      g.alpha = f1;
+      g.beta = f2;
+      g.prec = mode;
+      g.split_tasks = T.d_split;
+      g.nsplit = (int)T.split.size();
+      g.max_split_mn = T.max_split_mn;
+      g.cfg = cfg;
+      g.partial = (double*)tp.p;
+      DPD_HIP_TRY(launch_gemm_pk(g, ta.p, tb.p, c->stream));
+      DPD_HIP_TRY(hipEventRecord(e0, c->stream));
+      for (int r = 0; r < 3; r++) DPD_HIP_TRY(launch_gemm_pk(g, ta.p, tb.p, c->stream));
+      DPD_HIP_TRY(hipEventRecord(e1, c->stream));
+      DPD_HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      DPD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      tms[ci][sk] = ms;
+      if (best_ms < 0.0f || ms < best_ms) best_ms = ms;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  if (best < 0) best = GEMM_64x64;
-  GemmVariant& V = p.pk[mode];
-  if (V.built && V.cfg == best) return DPD_OK;
-  // panels cached for the old tile shape are rebuilt on the next call
-  for (auto& pc : V.cache) {
-    pc.src = nullptr;
-    pc.id = 0;
-    pc.buf.reset();
+  int best = GEMM_64x64;
+  bool best_split = true;
+  if (best_ms >= 0.0f) {
+    bool found = false;
+    for (int ci = 0; ci < NC && !found; ci++)
+      for (int sk = 0; sk < 2 && !found; sk++)
+        if (tms[ci][sk] <= 1.05f * best_ms) {
+          best = cand[ci];
+          best_split = sk == 0;
+          found = true;
+        }
   }
-  return build_variant_cfg(p, V, mode, best);
+  GemmVariant& V = p.pk[mode];
+  if (V.built && V.cfg == best && V.split_k == best_split) return DPD_OK;
+  // panels cached for the old tile shape are rebuilt on the next call
+  if (!(V.built && V.cfg == best)) {
+    for (auto& pc : V.cache) {
+      pc.src = nullptr;
+      pc.id = 0;
+      pc.buf.reset();
+    }
+  }
+  return build_variant_cfg(p, V, mode, best, best_split);
 }
 
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,

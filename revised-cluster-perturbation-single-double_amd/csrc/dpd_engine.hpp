@@ -48,6 +48,7 @@ struct PackCache {
 struct GemmVariant {
   bool built = false;
   bool tuned = false;  // tile shape chosen by timing (autotuner)
+  bool split_k = true;  // long-K tasks split over workgroups (else one per tile)
   int mode = 0, cfg = 0;
   std::vector<GemmTask> tasks;
   std::vector<int32_t> split;
