@@ -509,10 +509,15 @@ __global__ __launch_bounds__(256) void k_amax_partial(int64_t n, const double* _
   }
 }
 
+// block b reduces the nb partials of vector b
 __global__ __launch_bounds__(256) void k_amax_final(int nb, const double* pv, const int64_t* pi,
                                                     double* ov, int64_t* oi) {
   __shared__ double sv[256];
   __shared__ int64_t si[256];
+  pv += (int64_t)blockIdx.x * nb;
+  pi += (int64_t)blockIdx.x * nb;
+  ov += blockIdx.x;
+  oi += blockIdx.x;
   double v = 0.0;
   int64_t idx = INT64_MAX;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) amax_merge(v, idx, pv[i], pi[i]);
@@ -916,9 +921,7 @@ __global__ __launch_bounds__(256) void k_resid_amax_partial(int64_t n, const dou
 hipError_t launch_resid_amax(int64_t n, const double* aq, const double* q, const double* w, double* e, int nv,
                              double* pv, int64_t* pi, double* ov, int64_t* oi, hipStream_t s) {
   hipLaunchKernelGGL(k_resid_amax_partial, dim3(kRedBlocks, nv), dim3(256), 0, s, n, aq, q, w, e, pv, pi);
-  for (int v = 0; v < nv; v++)
-    hipLaunchKernelGGL(k_amax_final, dim3(1), dim3(256), 0, s, kRedBlocks, pv + (int64_t)v * kRedBlocks,
-                       pi + (int64_t)v * kRedBlocks, ov + v, oi + v);
+  hipLaunchKernelGGL(k_amax_final, dim3(nv), dim3(256), 0, s, kRedBlocks, pv, pi, ov, oi);
   return hipGetLastError();
 }
 

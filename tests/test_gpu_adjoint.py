@@ -184,7 +184,9 @@ def test_set_requires_refresh(torch):
         gp.set("mp2t", gp.get("mp2t"))
     gp.refresh()
     y1 = gp.sigma(torch.ones(gp.dim, dtype=torch.float64, device="cuda:0"))
-    assert torch.equal(y0, y1)
+    # the intermediates are recomputed (possibly with another autotuned tile, so
+    # another summation order): equal to rounding
+    assert float((y0 - y1).abs().max()) <= 1e-13 * float(y0.abs().max())
 
 
 @pytest.mark.timeout(300)
