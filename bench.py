@@ -84,6 +84,7 @@ def main():
                     help="counter-run mode: a few batched sigma calls in this precision (tile shape --tile), "
                          "then a plain 512 MiB device copy for calibrating FETCH_SIZE/WRITE_SIZE; no JSON line")
     ap.add_argument("--tile", type=int, default=-1, help="GEMM tile configuration for --probe (-1: autotuned)")
+    ap.add_argument("--split", type=int, default=-1, help="split-K for --probe: -1 automatic, 0 off, 1 on")
     ap.add_argument("--fd-dirs", type=int, default=50, help="gradient workload: random integral directions")
     ap.add_argument("--fd-h", type=float, default=1e-4, help="gradient workload: central-difference step")
     ap.add_argument("--grad-conv", type=float, default=1e-9, help="gradient workload: FP64 residual tolerance")
@@ -190,7 +191,8 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        ent = json.load(open(tpath)).get(f"{args.workload}:{tag.rsplit(':t', 1)[0]}:cfg{st.get('cfg')}")
+        ent = json.load(open(tpath)).get(
+            f"{args.workload}:{tag.rsplit(':t', 1)[0]}:cfg{st.get('cfg')}:split{st.get('split', 1)}")
         traffic = ent["traffic_bytes"] if ent else None
 
     cpu = None
@@ -235,6 +237,7 @@ def main():
         "mixed_all": {k: {"s_per_solve": round(v["t"], 6), "iterations": v["info"]["iterations"],
                           "max_abs_dw": v["dw"]} for k, v in mixed.items()},
         "roofline": {"bound": bound, "kernel": f"{kname} ({tag})", "tile_cfg": st.get("cfg"),
+                     "split_k": bool(st.get("split", 0)),
                      "achieved": round(achieved, 3), "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
                      "traffic": traffic, "flops_per_launch": int(flops_per_launch),
                      "algorithmic_bytes_per_launch": int(bytes_per_launch),
@@ -493,6 +496,7 @@ def probe(cfg, args):
     import dpdhip
     ctx = dpdhip.Context(len(cfg["nso"]), 0)
     ctx.set_gemm_config(args.tile, 1)
+    ctx.set_split_k(args.split)
     prob = dpdhip.Problem(ctx, dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"]), "sf-cpsd", 1)
     prob.set_precision(args.probe)
     R = torch.randn(cfg["nroot"], prob.dim, device="cuda:0", dtype=torch.float64)

@@ -75,6 +75,11 @@ int dpd_ctx_synchronize(dpd_ctx *ctx);
  * panels always (BF16x3 always uses panels).  Clears the plan caches.  No
  * counterpart in the reference (its GEMMs are cuBLAS/BLAS calls). */
 int dpd_ctx_set_gemm_config(dpd_ctx *ctx, int cfg, int packing);
+/* No reference counterpart: split-K of the packed-panel GEMMs.  mode -1: timed
+ * together with the tile shape when the tile is autotuned, else on; 0: off (one
+ * workgroup per output tile); 1: long-K tasks split while a launch has fewer
+ * than 1024 tiles, partial sums reduced by a second kernel.  Clears the plans. */
+int dpd_ctx_set_split_k(dpd_ctx *ctx, int mode);
 const char *dpd_last_error(void);
 /* bytes of device scratch the context currently holds (grows on demand) */
 size_t dpd_ctx_workspace_bytes(const dpd_ctx *ctx);

@@ -476,6 +476,17 @@ int dpd_ctx_set_gemm_config(dpd_ctx* c, int cfg, int packing) {
   return DPD_OK;
 }
 
+int dpd_ctx_set_split_k(dpd_ctx* c, int mode) {
+  CHECK_CTX(c);
+  if (mode < -1 || mode > 1) return fail(DPD_EINVAL, "split-K mode must be -1, 0 or 1");
+  if (c->device >= 0) DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->split_k = mode;
+  c->group_cache.clear();
+  c->contract_cache.clear();
+  c->engine_cache.clear();
+  return DPD_OK;
+}
+
 // ---- profiler ------------------------------------------------------------------
 int dpd_profile_enable(dpd_ctx* c, int on, int max_records) {
   CHECK_CTX(c);
@@ -505,14 +516,14 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
   std::vector<std::string> tags;
   std::vector<double> ms;
   std::vector<int64_t> cnt, fl, fltot, by;
-  std::vector<int> pr, cf, pk;
+  std::vector<int> pr, cf, pk, sp;
   for (auto& r : c->prof.recs) {
     float t = 0.f;
     DPD_HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
     // one entry per tag, precision and tile shape (a tag's plans for different
     // vector counts may be tuned to different tiles)
     std::string key = r.tag + (r.prec == PREC_F64 ? ":fp64" : r.prec == PREC_F32 ? ":fp32" : ":bf16x3") + ":t" +
-                      std::to_string(r.cfg);
+                      std::to_string(r.cfg) + (r.split ? "s" : "");
     size_t k = std::find(tags.begin(), tags.end(), key) - tags.begin();
     if (k == tags.size()) {
       tags.push_back(key);
@@ -524,6 +535,7 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
       pr.push_back(r.prec);
       cf.push_back(r.cfg);
       pk.push_back(r.packed);
+      sp.push_back(r.split);
     }
     ms[k] += t;
     cnt[k] += 1;
@@ -539,9 +551,9 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
     char item[512];
     snprintf(item, sizeof item,
              "%s\"%s\": {\"count\": %lld, \"ms\": %.6f, \"flops_total\": %lld, \"bytes_total\": %lld, "
-             "\"prec\": %d, \"cfg\": %d, \"packed\": %d}",
+             "\"prec\": %d, \"cfg\": %d, \"packed\": %d, \"split\": %d}",
              k ? ", " : "", tags[k].c_str(), (long long)cnt[k], ms[k], (long long)fltot[k], (long long)by[k], pr[k],
-             cf[k], pk[k]);
+             cf[k], pk[k], sp[k]);
     js += item;
   }
   js += "}";

@@ -660,7 +660,7 @@ int build_variant(GemmPlan& p, int mode) {
   if (V.built) return DPD_OK;
   int cfg = choose_gemm_cfg(p.tasks, p.cfg_req >= GEMM_64x64 ? p.cfg_req : -1);
   if (cfg == GEMM_V1) cfg = GEMM_64x64;
-  return build_variant_cfg(p, V, mode, cfg);
+  return build_variant_cfg(p, V, mode, cfg, p.split_req != 0);
 }
 
 // one past the largest element of C a plan can write
@@ -802,10 +802,11 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
     DevBuf ta, tb;
     bool had_split = false;
     for (int sk = 0; sk < 2 && !rc; sk++) {
+      if ((sk == 0 && p.split_req == 0) || (sk == 1 && p.split_req == 1)) continue;  // requested split mode
       GemmVariant T;
       if ((rc = build_variant_cfg(p, T, mode, cfg, sk == 0))) break;
       if (sk == 0) had_split = !T.split.empty();
-      if (sk == 1 && !had_split) {
+      if (sk == 1 && !had_split && p.split_req != 0) {
         tms[ci][1] = tms[ci][0];  // nothing was split: the same variant
         continue;
       }
@@ -1365,6 +1366,7 @@ static int plan_of(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, c
   if (it == c->contract_cache.end()) {
     auto np = std::make_unique<ContractPlan>();
     np->gemm.cfg_req = c->gemm_cfg;
+    np->gemm.split_req = c->split_k;
     int rc = build_contract_plan(c->sym, kind, o1, o2, o3, anti, nbatch, *np);
     if (rc) return rc;
     *out = np.get();
@@ -1510,10 +1512,10 @@ static int64_t call_bytes(dpd_ctx* c, const CallPrep& r, const Operand& o1, cons
 }
 
 static ProfRec* prof_open(dpd_ctx* c, const char* tag, int64_t flops, int prec, int cfg, int packed,
-                          int64_t bytes) {
+                          int64_t bytes, int split) {
   if (!c->prof.on || !tag || c->prof.used + 2 > c->prof.pool.size()) return nullptr;
   c->prof.recs.push_back({tag, c->prof.pool[c->prof.used], c->prof.pool[c->prof.used + 1], flops, prec, cfg, packed,
-                          bytes});
+                          bytes, split});
   c->prof.used += 2;
   return &c->prof.recs.back();
 }
@@ -1534,7 +1536,8 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   const void* A = (use_f32 && r.p1f) ? (const void*)r.p1f : (const void*)r.p1;
   const void* B = (use_f32 && r.p2f) ? (const void*)r.p2f : (const void*)r.p2;
   ProfRec* pr = prof_open(c, tag, P->gemm.flops, prec, r.packed ? P->gemm.pk[prec].cfg : P->gemm.cfg,
-                          r.packed ? 1 : 0, c->prof.on && tag ? call_bytes(c, r, o1, o2, prec) : 0);
+                          r.packed ? 1 : 0, c->prof.on && tag ? call_bytes(c, r, o1, o2, prec) : 0,
+                          r.packed ? !P->gemm.pk[prec].split.empty() : !P->gemm.split.empty());
   // the packed path records the events around the GEMM kernel itself
   if (pr && !r.packed) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
   if (r.packed) {
@@ -1806,7 +1809,7 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
     return DPD_ENOMEM;
   GemmLaunch g;
   group_launch_args(G->gt, prec, G->cfg, fact, partial, g);
-  ProfRec* pr = cap ? nullptr : prof_open(c, tag, flops, prec, G->cfg, 1, bytes);
+  ProfRec* pr = cap ? nullptr : prof_open(c, tag, flops, prec, G->cfg, 1, bytes, G->gt.nsplit > 0);
   if (pr) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
   DPD_HIP_TRY(launch_gemm_pk_grouped(g, gt, c->stream));
   if (pr) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));

@@ -67,6 +67,7 @@ struct GemmPlan {
   int nwg = 0;
   int cfg = 0;       // GemmCfg tile configuration chosen by the planner
   int cfg_req = -1;  // requested configuration (-1: planner's choice)
+  int split_req = -1;  // split-K: -1 automatic (timed with the tile), 0 off, 1 on
   int max_split_mn = 0;
   int64_t partial_elems = 0;
   int64_t flops = 0;    // 2*M*N*K summed (algorithmic, per launch)
@@ -144,6 +145,7 @@ struct ProfRec {
   int cfg;        // GemmCfg tile shape of the launch
   int packed;     // packed-panel kernel (1) or gathering kernel (0)
   int64_t bytes;  // algorithmic bytes: operands as the kernel reads them once + result
+  int split;      // some task of the launch is split over K (partials + reduce)
 };
 struct Profiler {
   bool on = false;
@@ -162,6 +164,7 @@ struct dpd_ctx {
   int64_t max_threads_block = 256;
   int gemm_cfg = -1;  // GEMM tile configuration override (-1: automatic)
   int packing = 1;    // 0 gather kernels only, 1 packed panels when an operand is static, 2 always
+  int split_k = -1;   // packed-panel split-K: -1 automatic, 0 off, 1 on
   // scratch: grow-only device workspace (bump allocated per top-level call)
   dpd::DevBuf ws;
   size_t ws_top = 0;

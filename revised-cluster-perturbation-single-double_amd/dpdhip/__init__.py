@@ -81,6 +81,10 @@ class Context:
         operand is a problem constant (packed once), 2 packed panels always."""
         check(lib().dpd_ctx_set_gemm_config(self.h, int(cfg), int(packing)))
 
+    def set_split_k(self, mode=-1):
+        """split-K of the packed-panel GEMMs: -1 automatic, 0 off, 1 on"""
+        check(lib().dpd_ctx_set_split_k(self.h, mode))
+
     def profile(self, on=True, max_records=100000):
         """Enable/disable per-launch HIP-event timing of the tagged GEMM stages."""
         check(lib().dpd_profile_enable(self.h, int(on), int(max_records)))

@@ -56,6 +56,7 @@ _SIGS = {
     "dpd_ctx_set_stream": (ctypes.c_int, [c_void_p, c_void_p]),
     "dpd_ctx_synchronize": (ctypes.c_int, [c_void_p]),
     "dpd_ctx_set_gemm_config": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int]),
+    "dpd_ctx_set_split_k": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "dpd_ctx_workspace_bytes": (ctypes.c_size_t, [c_void_p]),
     "dpd_profile_enable": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int]),
     "dpd_profile_report": (ctypes.c_int, [c_void_p, ctypes.c_char_p, i64]),
