@@ -280,6 +280,11 @@ struct PkTraits<PREC_BF16X3> {
 // the 128x64 tiles run 4 instead of 2 waves per SIMD; the ring GEMM gains 16%
 // in BF16x3 and the FP64 solve 3%, the pp-ladder is unchanged)
 constexpr int pk_nbuf(int mode, int bm, int bn) { return (void)mode, (void)bm, (void)bn, 1; }
+#ifndef DPD_PK_STAGES
+#define DPD_PK_STAGES 3
+#endif
+// K steps held in registers ahead of the one being multiplied
+constexpr int pk_stages(int mode, int bm, int bn) { return (void)mode, (void)bm, (void)bn, DPD_PK_STAGES; }
 
 // one workgroup's tile of task T (local = workgroup index inside the task)
 template <int MODE, int BM, int BN>
@@ -333,33 +338,44 @@ __device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask
   (BF ? ((e) / ((rows) * KS)) * (rows) * (KS + Tr::PAD) + (((e) % ((rows) * KS)) / KS) * (KS + Tr::PAD) +    \
             ((e) % KS)                                                                                       \
       : ((e) / (rows)) * ((rows) + Tr::PAD) + ((e) % (rows)))
-  u4_t ra[CA], rb[CB];
+  // K steps are prefetched STAGES ahead into registers (ring of register
+  // slots, indexed at compile time by unrolling the loop STAGES times); the
+  // slot of step i is stored into the single LDS buffer before step i runs
+  constexpr int STAGES = pk_stages(MODE, BM, BN);
+  u4_t ra[STAGES][CA], rb[STAGES][CB];
   const int nst = s1 - s0;
+  auto load_step = [&](int slot_unused, int step, u4_t* xa, u4_t* xb) {
+    (void)slot_unused;
+    const u4_t* qa = pa + (int64_t)step * (A_PANEL / EPC);
+    const u4_t* qb = pb + (int64_t)step * (B_PANEL / EPC);
+#pragma unroll
+    for (int j = 0; j < CA; j++) xa[j] = qa[tid + j * kThreads];
+#pragma unroll
+    for (int j = 0; j < CB; j++) xb[j] = qb[tid + j * kThreads];
+  };
+  auto store_step = [&](const u4_t* xa, const u4_t* xb) {
+#pragma unroll
+    for (int j = 0; j < CA; j++) *(u4_t*)&As[0][PK_LDS_POS((tid + j * kThreads) * EPC, BM)] = xa[j];
+#pragma unroll
+    for (int j = 0; j < CB; j++) *(u4_t*)&Bs[0][PK_LDS_POS((tid + j * kThreads) * EPC, BN)] = xb[j];
+  };
   if (nst > 0) {
 #pragma unroll
-    for (int j = 0; j < CA; j++) ra[j] = pa[tid + j * kThreads];
-#pragma unroll
-    for (int j = 0; j < CB; j++) rb[j] = pb[tid + j * kThreads];
-#pragma unroll
-    for (int j = 0; j < CA; j++) *(u4_t*)&As[0][PK_LDS_POS((tid + j * kThreads) * EPC, BM)] = ra[j];
-#pragma unroll
-    for (int j = 0; j < CB; j++) *(u4_t*)&Bs[0][PK_LDS_POS((tid + j * kThreads) * EPC, BN)] = rb[j];
+    for (int u = 0; u < STAGES; u++)
+      if (u < nst) load_step(u, u, ra[u], rb[u]);
+    store_step(ra[0], rb[0]);
     __syncthreads();
-    for (int i = 0; i < nst; i++) {
-      const int buf = NBUF == 2 ? (i & 1) : 0;
-      const bool more = i + 1 < nst;
-      if (more) {
-        const u4_t* qa = pa + (int64_t)(i + 1) * (A_PANEL / EPC);
-        const u4_t* qb = pb + (int64_t)(i + 1) * (B_PANEL / EPC);
+    for (int i0 = 0; i0 < nst; i0 += STAGES) {
 #pragma unroll
-        for (int j = 0; j < CA; j++) ra[j] = qa[tid + j * kThreads];
-#pragma unroll
-        for (int j = 0; j < CB; j++) rb[j] = qb[tid + j * kThreads];
-      }
+      for (int u = 0; u < STAGES; u++) {
+        const int i = i0 + u;
+        if (i < nst) {
+          // slot u (step i) is in LDS already: refill it with step i + STAGES
+          if (i + STAGES < nst) load_step(u, i + STAGES, ra[u], rb[u]);
       if constexpr (!BF) {
         typedef typename Tr::CT CT;
-        const CT* as = (const CT*)As[buf];
-        const CT* bs = (const CT*)Bs[buf];
+        const CT* as = (const CT*)As[0];
+        const CT* bs = (const CT*)Bs[0];
 #pragma unroll
         for (int ks = 0; ks < KS / 4; ks++) {
           const int kr = ks * 4 + (lane >> 4);
@@ -384,14 +400,14 @@ __device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask
 #pragma unroll
         for (int x = 0; x < TM; x++) {
           const int row = wm + 16 * x + (lane & 15);
-          ah[x] = *(const bf8_t*)&As[buf][row * (KS + Tr::PAD) + kq];
-          al[x] = *(const bf8_t*)&As[buf][BM * (KS + Tr::PAD) + row * (KS + Tr::PAD) + kq];
+          ah[x] = *(const bf8_t*)&As[0][row * (KS + Tr::PAD) + kq];
+          al[x] = *(const bf8_t*)&As[0][BM * (KS + Tr::PAD) + row * (KS + Tr::PAD) + kq];
         }
 #pragma unroll
         for (int y = 0; y < TN; y++) {
           const int row = wn + 16 * y + (lane & 15);
-          bh[y] = *(const bf8_t*)&Bs[buf][row * (KS + Tr::PAD) + kq];
-          bl[y] = *(const bf8_t*)&Bs[buf][BN * (KS + Tr::PAD) + row * (KS + Tr::PAD) + kq];
+          bh[y] = *(const bf8_t*)&Bs[0][row * (KS + Tr::PAD) + kq];
+          bl[y] = *(const bf8_t*)&Bs[0][BN * (KS + Tr::PAD) + row * (KS + Tr::PAD) + kq];
         }
 #pragma unroll
         for (int x = 0; x < TM; x++)
@@ -402,15 +418,13 @@ __device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask
             acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
           }
       }
-      if (more) {
-        if constexpr (NBUF == 1) __syncthreads();  // every wave is done reading the buffer
-        const int nb = NBUF == 2 ? (buf ^ 1) : 0;
-#pragma unroll
-        for (int j = 0; j < CA; j++) *(u4_t*)&As[nb][PK_LDS_POS((tid + j * kThreads) * EPC, BM)] = ra[j];
-#pragma unroll
-        for (int j = 0; j < CB; j++) *(u4_t*)&Bs[nb][PK_LDS_POS((tid + j * kThreads) * EPC, BN)] = rb[j];
+          __syncthreads();  // every wave is done reading the buffer
+          if (i + 1 < nst) {
+            store_step(ra[(u + 1) % STAGES], rb[(u + 1) % STAGES]);
+            __syncthreads();
+          }
+        }
       }
-      __syncthreads();
     }
   }
 #undef PK_LDS_POS
