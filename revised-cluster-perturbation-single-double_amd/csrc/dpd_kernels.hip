@@ -63,9 +63,11 @@ __global__ __launch_bounds__(256) void k_gather_map(MapArgs a_arg, const double*
 // left pair (no search over blocks, no 64-bit division).
 constexpr int kGatherChunk = 1024;
 
-__device__ inline int64_t gather_chunks(const DevLayout& L, int xr) {
+// small outputs use chunks of one element per thread (more blocks in flight)
+constexpr int kGatherChunkSmall = 256;
+__host__ __device__ inline int64_t gather_chunks(const DevLayout& L, int xr, int chunk) {
   const int xl = L.dp[xr][L.sym];
-  return L.nR[xr] * ((L.nL[xl] + kGatherChunk - 1) / kGatherChunk);
+  return L.nR[xr] * ((L.nL[xl] + chunk - 1) / chunk);
 }
 
 // the layouts are indexed by per-lane irreps: staged in LDS so that those
@@ -78,27 +80,27 @@ __device__ inline void stage_map(const MapArgs& a, MapArgs& sa) {
 }
 
 __global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const double* __restrict__ in,
-                                                     double* __restrict__ out) {
+                                                     double* __restrict__ out, int chunk) {
   __shared__ MapArgs a;
   stage_map(a_arg, a);
   const DevLayout& L = a.out;
   int64_t b = blockIdx.x;
   int xr = 0;
   for (; xr < L.nirrep; xr++) {
-    const int64_t c = gather_chunks(L, xr);
+    const int64_t c = gather_chunks(L, xr, chunk);
     if (b < c) break;
     b -= c;
   }
   if (xr >= L.nirrep) return;
   const int xl = L.dp[xr][L.sym];
   const int64_t nl = L.nL[xl];
-  const int64_t per = (nl + kGatherChunk - 1) / kGatherChunk;
-  const int64_t r = b / per, l0 = (b % per) * kGatherChunk;
+  const int64_t per = (nl + chunk - 1) / chunk;
+  const int64_t r = b / per, l0 = (b % per) * chunk;
   int h[4];
   int64_t x[4];
   decode_pair(L.pop[2], L.pop[3], L.iop[1], L.dp, L.offR[xr], L.nirrep, xr, r, &h[2], &x[2], &h[3], &x[3]);
   const int64_t col = L.blk[xr] + r * nl;
-  const int64_t l1 = l0 + kGatherChunk < nl ? l0 + kGatherChunk : nl;
+  const int64_t l1 = l0 + chunk < nl ? l0 + chunk : nl;
   // the element's source addresses are decoded once and serve every tensor of
   // the batch (the decode, not the memory traffic, bounds this kernel)
   for (int64_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
@@ -139,11 +141,15 @@ __global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const double
 
 hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hipStream_t s) {
   if (a.out.len == 0 || a.nbatch == 0) return hipSuccess;
-  // one block per (column, chunk): the same count the kernel derives
+  // one block per (column, chunk): the same count the kernel derives; a
+  // launch of fewer than 1024 blocks takes one element per thread instead
   int64_t blocks = 0;
-  for (int xr = 0; xr < a.out.nirrep; xr++) {
-    const int xl = a.out.dp[xr][a.out.sym];
-    blocks += a.out.nR[xr] * ((a.out.nL[xl] + kGatherChunk - 1) / kGatherChunk);
+  int chunk = kGatherChunk;
+  for (int xr = 0; xr < a.out.nirrep; xr++) blocks += gather_chunks(a.out, xr, chunk);
+  if (blocks < 1024) {
+    chunk = kGatherChunkSmall;
+    blocks = 0;
+    for (int xr = 0; xr < a.out.nirrep; xr++) blocks += gather_chunks(a.out, xr, chunk);
   }
   if (blocks <= 0) return hipSuccess;
   if (blocks > 0x7fffffff) {
@@ -152,7 +158,7 @@ hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hi
     hipLaunchKernelGGL(k_gather_map, dim3((unsigned)b2, (unsigned)a.nbatch), dim3(256), 0, s, a, in, out);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)blocks, 1), dim3(256), 0, s, a, in, out);
+  hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)blocks, 1), dim3(256), 0, s, a, in, out, chunk);
   return hipGetLastError();
 }
 
