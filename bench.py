@@ -28,18 +28,18 @@ for _p in (os.path.join(ROOT, "revised-cluster-perturbation-single-double_amd"),
         sys.path.insert(0, _p)
 
 WORKLOADS = {
-    "cfg1": dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1, nroot=3, group="C2v"),
-    "cfg2": dict(nso=(30, 8, 14, 18, 26, 6, 10, 8), nalpha=12, nbeta=10, seed=2, nroot=5, group="D2h"),
+    "c2v-24": dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1, nroot=3, group="C2v"),
+    "d2h-120": dict(nso=(30, 8, 14, 18, 26, 6, 10, 8), nalpha=12, nbeta=10, seed=2, nroot=5, group="D2h"),
     # block Davidson: 16 roots of the D2h/120 Hamiltonian
-    "cfg4a": dict(nso=(30, 8, 14, 18, 26, 6, 10, 8), nalpha=12, nbeta=10, seed=2, nroot=16, group="D2h"),
+    "d2h-120-16roots": dict(nso=(30, 8, 14, 18, 26, 6, 10, 8), nalpha=12, nbeta=10, seed=2, nroot=16, group="D2h"),
     # mixed precision at C2v/180 (|R1|+|R2| = 5 981 115)
-    "cfg3": dict(nso=(70, 18, 36, 56), nalpha=16, nbeta=14, seed=3, nroot=5, group="C2v"),
+    "c2v-180": dict(nso=(70, 18, 36, 56), nalpha=16, nbeta=14, seed=3, nroot=5, group="C2v"),
     # gradient of the lowest SF-CPSd excitation energy w.r.t. the orbital
     # energies and the integral blocks, checked by central differences
-    "cfg5": dict(nso=(40, 10, 20, 30), nalpha=10, nbeta=8, seed=5, nroot=1, group="C2v", kind="gradient"),
+    "c2v-100-gradient": dict(nso=(40, 10, 20, 30), nalpha=10, nbeta=8, seed=5, nroot=1, group="C2v", kind="gradient"),
     # 32 independent C2v Hamiltonians (seeds 0..31): 30-80 spatial orbitals with a
     # random irrep split, n_alpha 4..10, n_beta = n_alpha - 2, 3 roots each
-    "cfg4b": dict(nham=32, norb=(30, 80), nalpha=(4, 10), nroot=3, group="C2v", kind="batch"),
+    "c2v-batch32": dict(nham=32, norb=(30, 80), nalpha=(4, 10), nroot=3, group="C2v", kind="batch"),
 }
 
 
@@ -72,7 +72,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="d2h-120", choices=sorted(WORKLOADS))
     ap.add_argument("--conv", type=float, default=1e-7, help="FP64 max-abs residual tolerance")
     ap.add_argument("--conv-mixed", type=float, default=1e-5, help="mixed-precision residual tolerance")
     ap.add_argument("--max-dim", type=int, default=40)
@@ -196,7 +196,7 @@ def main():
         traffic = ent["traffic_bytes"] if ent else None
 
     cpu = None
-    if not args.no_cpu and args.workload != "cfg3":
+    if not args.no_cpu and args.workload != "c2v-180":
         # (at C2v/180 one oracle sigma alone exceeds the bounded sample)
         cpu = cpu_baseline(cfg, args, infom, lowest)
 
@@ -341,7 +341,7 @@ def batch_bench(cfg, args):
         "vs_baseline": None,
         "dtype": "fp32-mfma/fp64",
         "data": "synthetic (seeded)",
-        "config": {"workload": f"cfg4b: {len(probs)} SF-CPSd C2v Hamiltonians, {cfg['norb'][0]}-{cfg['norb'][1]} "
+        "config": {"workload": f"c2v-batch32: {len(probs)} SF-CPSd C2v Hamiltonians, {cfg['norb'][0]}-{cfg['norb'][1]} "
                                f"spatial orbitals (random irrep splits, seeds 0..{len(probs) - 1}), n_alpha "
                                f"{cfg['nalpha'][0]}-{cfg['nalpha'][1]}, n_beta = n_alpha - 2, {nroot} roots each",
                    "dims": dims, "nso": [list(sp.nso) for sp in specs], "nalpha": [sp.nalpha for sp in specs],
@@ -465,7 +465,7 @@ def gradient_bench(cfg, args):
         "vs_baseline": None,
         "dtype": "fp64",
         "data": "synthetic (seeded)",
-        "config": {"workload": f"cfg5: SF-CPSd {cfg['group']} {sum(cfg['nso'])} spatial orbitals {list(cfg['nso'])} "
+        "config": {"workload": f"c2v-100-gradient: SF-CPSd {cfg['group']} {sum(cfg['nso'])} spatial orbitals {list(cfg['nso'])} "
                                f"n_alpha={cfg['nalpha']} n_beta={cfg['nbeta']}, lowest root, dim {prob.dim}",
                    "inputs": {k: int(base[k].numel()) for k in names}, "conv_fp64": args.grad_conv,
                    "conv_mixed": args.conv_mixed},

@@ -18,7 +18,7 @@ import oracle_sigma as OS
 pytestmark = pytest.mark.gpu
 
 TINY = dict(nso=(4, 1, 2, 3), nalpha=3, nbeta=1, seed=3)
-CFG1 = dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1)
+C2V24 = dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1)
 SMALL_D2H = dict(nso=(5, 2, 3, 3, 4, 1, 2, 2), nalpha=5, nbeta=3, seed=5)
 SMALL = dict(nso=(6, 2, 3, 5), nalpha=4, nbeta=2, seed=7)
 INPUTS = {"sf-cis": ("eorb", "iabj"), "cisdoo": ("eorb", "iabj", "ijka", "abij", "abci"),
@@ -57,7 +57,7 @@ def test_sigma_t_is_dense_transpose(torch, method):
     np.testing.assert_allclose(T, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("cfg,symR", [(CFG1, 1), (CFG1, 2), (SMALL_D2H, 1), (SMALL_D2H, 5)],
+@pytest.mark.parametrize("cfg,symR", [(C2V24, 1), (C2V24, 2), (SMALL_D2H, 1), (SMALL_D2H, 5)],
                          ids=["c2v-A1", "c2v-A2", "d2h-Ag", "d2h-B1u"])
 @pytest.mark.parametrize("method", ["sf-cis", "cisdoo", "sf-cpsd"])
 def test_adjoint_identity(torch, cfg, symR, method):

@@ -10,7 +10,7 @@ import oracle_sigma as OS
 
 pytestmark = pytest.mark.gpu
 
-CFG1 = dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1)
+C2V24 = dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1)
 TINY = dict(nso=(4, 1, 2, 3), nalpha=3, nbeta=1, seed=3)
 
 
@@ -42,7 +42,7 @@ def make(cfg, method, symR):
 @pytest.mark.parametrize("method,max_dim", [("sf-cis", 20), ("cisdoo", 20), ("sf-cpsd", 20), ("sf-cpsd", 12)],
                          ids=["sf-cis", "cisdoo", "sf-cpsd", "sf-cpsd-collapse"])
 def test_davidson_matches_oracle(torch, method, max_dim):
-    sp, op, gp = make(CFG1, method, 1)
+    sp, op, gp = make(C2V24, method, 1)
     nroot = 3
     guesses, _ = op.guess(nroot)
     ref = OS.davidson(op.sigma, op.diag(), guesses, nroot, conv=1e-7, max_dim=max_dim, max_cyc=60)
@@ -79,7 +79,7 @@ def test_davidson_not_converged_path(torch):
     """max_dim=10 with 3 roots collapses to 8 vectors every other iteration and
     does not converge in 60 cycles: both report it (mod_davidson.f90:168-178)
     after the same number of iterations with the same eigenvalues."""
-    sp, op, gp = make(CFG1, "sf-cpsd", 1)
+    sp, op, gp = make(C2V24, "sf-cpsd", 1)
     guesses, _ = op.guess(3)
     ref = OS.davidson(op.sigma, op.diag(), guesses, 3, conv=1e-7, max_dim=10, max_cyc=60)
     w, R, info = gp.davidson(3, conv=1e-7, max_dim=10, max_cyc=60)
@@ -93,7 +93,7 @@ def test_mixed_precision_davidson_energies(torch, prec):
     """mixed-precision contractions (FP64 vectors, orthogonalisation and
     subspace) converged to 1e-5, then FP64 Rayleigh quotients of the vectors:
     energies within 1e-6 Eh of the FP64 Davidson at 1e-7"""
-    sp, op, gp = make(CFG1, "sf-cpsd", 1)
+    sp, op, gp = make(C2V24, "sf-cpsd", 1)
     nroot = 3
     w64, _, info64 = gp.davidson(nroot, conv=1e-7, max_dim=20, max_cyc=60)
     gp.set_precision(prec)

@@ -20,7 +20,7 @@ def _golden(name):
     return json.load(open(path))
 
 
-@pytest.mark.parametrize("name", ["cfg1", "cfg2"])
+@pytest.mark.parametrize("name", ["c2v-24", "d2h-120"])
 def test_fp64_davidson_matches_cpu_oracle(name):
     import torch
 

@@ -9,7 +9,7 @@ import oracle_sigma as OS
 
 pytestmark = pytest.mark.gpu
 
-CFG1 = dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1)
+C2V24 = dict(nso=(11, 2, 4, 7), nalpha=6, nbeta=4, seed=1)
 SMALL_D2H = dict(nso=(5, 2, 3, 3, 4, 1, 2, 2), nalpha=5, nbeta=3, seed=5)
 
 
@@ -48,7 +48,7 @@ def oracle_problem(cfg, method, symR):
                                 symR)
 
 
-@pytest.mark.parametrize("cfg,symR", [(CFG1, 1), (CFG1, 2), (SMALL_D2H, 1), (SMALL_D2H, 5)],
+@pytest.mark.parametrize("cfg,symR", [(C2V24, 1), (C2V24, 2), (SMALL_D2H, 1), (SMALL_D2H, 5)],
                          ids=["c2v-A1", "c2v-A2", "d2h-Ag", "d2h-B1u"])
 @pytest.mark.parametrize("method", ["sf-cis", "cisdoo", "sf-cpsd"])
 def test_sigma_matches_oracle(torch, cfg, symR, method):
@@ -77,7 +77,7 @@ def test_sigma_matches_oracle(torch, cfg, symR, method):
 @pytest.mark.parametrize("prec,bound", [("fp32", 1e-5), ("bf16x3", 2e-4)])
 def test_sigma_mixed_close_to_fp64(torch, prec, bound):
     import dpdhip
-    sp = dpdhip.Spec(CFG1["nso"], CFG1["nalpha"], CFG1["nbeta"], CFG1["seed"])
+    sp = dpdhip.Spec(C2V24["nso"], C2V24["nalpha"], C2V24["nbeta"], C2V24["seed"])
     gp = dpdhip.Problem(ctx(4), sp, "sf-cpsd", 1)
     rng = np.random.default_rng(3)
     R = torch.from_numpy(rng.standard_normal(2 * gp.dim)).to("cuda:0")
@@ -93,7 +93,7 @@ def test_sigma_kernel_paths_agree(torch, packing):
     """the same FP64 sigma through the gathering kernel (packing 0) and with
     every operand packed (packing 2) as through the default path"""
     import dpdhip
-    sp = dpdhip.Spec(CFG1["nso"], CFG1["nalpha"], CFG1["nbeta"], CFG1["seed"])
+    sp = dpdhip.Spec(C2V24["nso"], C2V24["nalpha"], C2V24["nbeta"], C2V24["seed"])
     c = ctx(4)
     gp = dpdhip.Problem(c, sp, "sf-cpsd", 1)
     rng = np.random.default_rng(4)
