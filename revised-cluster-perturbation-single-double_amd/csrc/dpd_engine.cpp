@@ -745,7 +745,7 @@ static int pack_sides(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int mode, const P
 }
 
 int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double* C, double f1,
-                    double f2, hipEvent_t ev0, hipEvent_t ev1) {
+                    double f2, hipEvent_t ev0, hipEvent_t ev1, bool c_f32) {
   GemmVariant& V = p.pk[mode];
   const void* side_ptr[2];
   int rc0 = pack_sides(c, p, V, mode, A, B, side_ptr);
@@ -769,6 +769,7 @@ int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const P
   g.nsplit = (int)V.split.size();
   g.max_split_mn = V.max_split_mn;
   g.cfg = V.cfg;
+  g.c_f32 = c_f32 ? 1 : 0;
   if (V.partial_elems > 0 && !(g.partial = (double*)ws_take(c, sizeof(double) * V.partial_elems))) return DPD_ENOMEM;
   // the events bracket the GEMM kernel alone (the split-K reduce follows)
   if (ev0) DPD_HIP_TRY(hipEventRecord(ev0, c->stream));
@@ -1356,6 +1357,7 @@ struct CallPrep {
   const float* p2f = nullptr;
   double* p3 = nullptr;
   double beta = 1.0;
+  bool p3_f32 = false;  // the scratch result is FP32 (FP32-accumulating packed GEMM)
   PackSrc a, b;
 };
 
@@ -1405,7 +1407,10 @@ static int call_setup(dpd_ctx* c, ContractPlan* P, const Operand& o1, const Oper
   }
   if (r.gx1) n += sizeof(double) * P->len1x * r.nb1 + 256;
   if (r.gx2) n += sizeof(double) * P->len2x * r.nb2 + 256;
-  if (P->epi_on) n += sizeof(double) * P->len3x * nbatch + 256;
+  // the scratch result of an FP32-accumulating packed GEMM is kept in FP32:
+  // the accumulators are FP32, so nothing is lost and half the bytes move
+  r.p3_f32 = P->epi_on && r.packed && prec != PREC_F64;
+  if (P->epi_on) n += (r.p3_f32 ? sizeof(float) : sizeof(double)) * P->len3x * nbatch + 256;
   if (!r.packed) n += sizeof(double) * P->gemm.partial_elems + 256;
   *need += n;
   return DPD_OK;
@@ -1448,7 +1453,7 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
   r.p3 = const_cast<double*>(o3.data);
   r.beta = 1.0;
   if (P->epi_on) {
-    r.p3 = (double*)ws_take(c, sizeof(double) * P->len3x * nbatch);
+    r.p3 = (double*)ws_take(c, (r.p3_f32 ? sizeof(float) : sizeof(double)) * P->len3x * nbatch);
     if (!r.p3) return DPD_ENOMEM;
     r.beta = 0.0;
   }
@@ -1490,6 +1495,10 @@ static int call_post(dpd_ctx* c, const CallPrep& r, const Operand& o3, int nbatc
   m.nbatch = nbatch;
   m.in_bstride = P->len3x;
   m.out_bstride = o3.bstride;
+  if (r.p3_f32) {
+    DPD_HIP_TRY(launch_gather_map_f32in(m, (const float*)r.p3, const_cast<double*>(o3.data), c->stream));
+    return DPD_OK;
+  }
   return gather(c, m, r.p3, const_cast<double*>(o3.data));
 }
 
@@ -1542,7 +1551,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   if (pr && !r.packed) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
   if (r.packed) {
     rc = run_gemm_packed(c, P->gemm, prec, r.a, r.b, r.p3, fact, r.beta, pr ? pr->a : nullptr,
-                         pr ? pr->b : nullptr);
+                         pr ? pr->b : nullptr, r.p3_f32);
     // first call of this plan in this precision: time every tile shape on these
     // operands (results go to a scratch C) and keep the fastest for later calls
     if (!rc && c->gemm_cfg == -1 && !P->gemm.pk[prec].tuned) {
@@ -1690,7 +1699,7 @@ static int tune_group(dpd_ctx* c, GroupPlan& G, int prec, const std::vector<Call
           (T[i].pb_elems && (rc = pack_side(c, p, T[i], 1, R[i].b, pb[i].p))))
         break;
       Vs[i] = &T[i];
-      gt.e[i] = {pa[i].p, pb[i].p, (double*)cs[i].p, 0.0};
+      gt.e[i] = {pa[i].p, pb[i].p, (double*)cs[i].p, 0.0, 0, 0};
     }
     if (rc) break;
     GroupTasks TT;
@@ -1800,7 +1809,7 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
     if ((rc = call_pre(c, R[i], o1[i], o2[i], o3[i], nbatch[i]))) return rc;
     const void* side[2];
     if ((rc = pack_sides(c, plans[i]->gemm, plans[i]->gemm.pk[prec], prec, R[i].a, R[i].b, side))) return rc;
-    gt.e[i] = {side[0], side[1], R[i].p3, R[i].beta};
+    gt.e[i] = {side[0], side[1], R[i].p3, R[i].beta, R[i].p3_f32 ? 1 : 0, 0};
     flops += plans[i]->gemm.flops;
     if (c->prof.on && tag) bytes += call_bytes(c, R[i], o1[i], o2[i], prec);
   }

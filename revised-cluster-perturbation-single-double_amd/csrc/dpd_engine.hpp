@@ -213,7 +213,8 @@ struct PackSrc {
 };
 // packed-panel GEMM; uncached operands are packed into workspace the caller reserved
 int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double* C, double f1,
-                    double f2, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                    double f2, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                    bool c_f32 = false);
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out);
 // labels / checks shared by the C-ABI and the sigma builders
 int check_labels(const char* t, int n);

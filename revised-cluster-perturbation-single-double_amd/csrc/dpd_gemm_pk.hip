@@ -290,7 +290,7 @@ constexpr int pk_stages(int mode, int bm, int bn) { return (void)mode, (void)bm,
 template <int MODE, int BM, int BN>
 __device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask& T, int local,
                                              const void* __restrict__ PA, const void* __restrict__ PB,
-                                             double* __restrict__ C, const double beta) {
+                                             double* __restrict__ C, const double beta, const bool c32) {
   typedef PkTraits<MODE> Tr;
   typedef typename Tr::E E;
   typedef typename Tr::AT AT;
@@ -459,7 +459,10 @@ __device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask
           g.partial[T.part_off + (int64_t)kp * T.M * T.N + m + (int64_t)n * T.M] = v;
         } else {
           const int64_t ad = T.c_off + cm[ml] + cn[nl];
-          C[ad] = (beta == 0.0) ? v : beta * C[ad] + v;
+          if (c32)
+            ((float*)C)[ad] = (float)v;  // scratch result, beta = 0
+          else
+            C[ad] = (beta == 0.0) ? v : beta * C[ad] + v;
         }
       }
 }
@@ -477,7 +480,7 @@ __global__ __launch_bounds__(256) void k_gemm_pk(GemmLaunch g, const void* __res
                                                  const void* __restrict__ PB) {
   const int wg = xcd_wg(g.nwg);
   const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
-  gemm_pk_tile<MODE, BM, BN>(g, T, wg - T.wg_begin, PA, PB, g.C, g.beta);
+  gemm_pk_tile<MODE, BM, BN>(g, T, wg - T.wg_begin, PA, PB, g.C, g.beta, g.c_f32 != 0);
 }
 
 template <int MODE, int BM, int BN>
@@ -485,7 +488,7 @@ __global__ __launch_bounds__(256) void k_gemm_pk_grp(GemmLaunch g, GroupTab gt) 
   const int wg = xcd_wg(g.nwg);
   const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
   const GroupEnt& e = gt.e[T.group];
-  gemm_pk_tile<MODE, BM, BN>(g, T, wg - T.wg_begin, e.pa, e.pb, e.c, e.beta);
+  gemm_pk_tile<MODE, BM, BN>(g, T, wg - T.wg_begin, e.pa, e.pb, e.c, e.beta, e.c_f32 != 0);
 }
 
 template <int MODE>

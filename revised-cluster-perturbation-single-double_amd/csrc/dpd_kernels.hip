@@ -25,13 +25,14 @@ namespace dpd {
 // ===========================================================================
 __device__ inline void stage_map(const MapArgs& a, MapArgs& sa);
 
-__global__ __launch_bounds__(256) void k_gather_map(MapArgs a_arg, const double* __restrict__ in,
+template <typename TIN>
+__global__ __launch_bounds__(256) void k_gather_map(MapArgs a_arg, const TIN* __restrict__ in,
                                                     double* __restrict__ out) {
   __shared__ MapArgs a;
   stage_map(a_arg, a);
   const int64_t n = a.out.len;
   const int b = blockIdx.y;
-  const double* inb = in + (int64_t)b * a.in_bstride;
+  const TIN* inb = in + (int64_t)b * a.in_bstride;
   double* outb = out + (int64_t)b * a.out_bstride;
   for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n;
        o += (int64_t)gridDim.x * blockDim.x) {
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(256) void k_gather_map(MapArgs a_arg, const double*
       }
       double sg;
       int64_t ad = fetch_addr(a.in, hi[0], hi[1], hi[2], hi[3], xi[0], xi[1], xi[2], xi[3], &sg);
-      if (ad >= 0) acc += a.term[t].coef * sg * inb[ad];
+      if (ad >= 0) acc += a.term[t].coef * sg * (double)inb[ad];
     }
     double v = a.alpha * acc;
     if (a.beta != 0.0) v = a.beta * outb[o] + v;
@@ -79,7 +80,8 @@ __device__ inline void stage_map(const MapArgs& a, MapArgs& sa) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const double* __restrict__ in,
+template <typename TIN>
+__global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const TIN* __restrict__ in,
                                                      double* __restrict__ out, int chunk) {
   __shared__ MapArgs a;
   stage_map(a_arg, a);
@@ -126,12 +128,12 @@ __global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const double
     }
     const int64_t o = col + l;
     for (int bt = 0; bt < a.nbatch; bt++) {
-      const double* inb = in + (int64_t)bt * a.in_bstride;
+      const TIN* inb = in + (int64_t)bt * a.in_bstride;
       double* outb = out + (int64_t)bt * a.out_bstride;
       double acc = 0.0;
 #pragma unroll
       for (int t = 0; t < kMaxTerms; t++)
-        if (ad[t] >= 0) acc += cf[t] * inb[ad[t]];
+        if (ad[t] >= 0) acc += cf[t] * (double)inb[ad[t]];
       double v = a.alpha * acc;
       if (a.beta != 0.0) v = a.beta * outb[o] + v;
       outb[o] = v;
@@ -139,7 +141,8 @@ __global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const double
   }
 }
 
-hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hipStream_t s) {
+template <typename TIN>
+static hipError_t gather_map_t(const MapArgs& a, const TIN* in, double* out, hipStream_t s) {
   if (a.out.len == 0 || a.nbatch == 0) return hipSuccess;
   // one block per (column, chunk): the same count the kernel derives; a
   // launch of fewer than 1024 blocks takes one element per thread instead
@@ -155,11 +158,19 @@ hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hi
   if (blocks > 0x7fffffff) {
     int64_t b2 = (a.out.len + 255) / 256;
     if (b2 > 8192) b2 = 8192;
-    hipLaunchKernelGGL(k_gather_map, dim3((unsigned)b2, (unsigned)a.nbatch), dim3(256), 0, s, a, in, out);
+    hipLaunchKernelGGL(k_gather_map<TIN>, dim3((unsigned)b2, (unsigned)a.nbatch), dim3(256), 0, s, a, in, out);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)blocks, 1), dim3(256), 0, s, a, in, out, chunk);
+  hipLaunchKernelGGL(k_gather_cols<TIN>, dim3((unsigned)blocks, 1), dim3(256), 0, s, a, in, out, chunk);
   return hipGetLastError();
+}
+
+hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hipStream_t s) {
+  return gather_map_t<double>(a, in, out, s);
+}
+
+hipError_t launch_gather_map_f32in(const MapArgs& a, const float* in, double* out, hipStream_t s) {
+  return gather_map_t<float>(a, in, out, s);
 }
 
 // ===========================================================================
@@ -369,7 +380,10 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(GemmLaunch g) {
     for (int kp = 0; kp < T.ksplit; kp++) sum += g.partial[T.part_off + kp * mn + e];
     const int m = (int)(e % T.M), n = (int)(e / T.M);
     const int64_t ad = T.c_off + digits3(m, T.md, T.sc_m) + digits3(n, T.nd, T.sc_n);
-    g.C[ad] = (g.beta == 0.0) ? sum : g.beta * g.C[ad] + sum;
+    if (g.c_f32)
+      ((float*)g.C)[ad] = (float)sum;
+    else
+      g.C[ad] = (g.beta == 0.0) ? sum : g.beta * g.C[ad] + sum;
   }
 }
 
@@ -386,7 +400,10 @@ __global__ __launch_bounds__(256) void k_splitk_reduce_grp(GemmLaunch g, GroupTa
     for (int kp = 0; kp < T.ksplit; kp++) sum += g.partial[T.part_off + kp * mn + e];
     const int m = (int)(e % T.M), n = (int)(e / T.M);
     const int64_t ad = T.c_off + digits3(m, T.md, T.sc_m) + digits3(n, T.nd, T.sc_n);
-    C[ad] = (beta == 0.0) ? sum : beta * C[ad] + sum;
+    if (ge.c_f32)
+      ((float*)C)[ad] = (float)sum;
+    else
+      C[ad] = (beta == 0.0) ? sum : beta * C[ad] + sum;
   }
 }
 

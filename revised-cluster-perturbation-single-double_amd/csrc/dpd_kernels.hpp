@@ -27,6 +27,8 @@ struct MapArgs {
   int nbatch;
 };
 hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hipStream_t s);
+// the same from an FP32 source (scratch results of the FP32-accumulating GEMMs)
+hipError_t launch_gather_map_f32in(const MapArgs& a, const float* in, double* out, hipStream_t s);
 
 // ---- grouped DPD GEMM -------------------------------------------------------
 // Every task writes C(m,n) (m,n factorised into up to 3 mixed-radix digits with
@@ -102,6 +104,7 @@ struct GemmLaunch {
   int nsplit;
   int max_split_mn;
   int cfg;  // GemmCfg
+  int c_f32;  // C is FP32 (a scratch result written with beta = 0)
 };
 hipError_t launch_gemm(const GemmLaunch& g, hipStream_t s);
 hipError_t launch_gemm2(const GemmLaunch& g, hipStream_t s);
@@ -144,6 +147,8 @@ struct GroupEnt {
   const void* pb;
   double* c;
   double beta;
+  int c_f32;  // c is FP32 (scratch result, beta = 0)
+  int pad_;
 };
 struct GroupTab {
   GroupEnt e[kMaxGroup];
