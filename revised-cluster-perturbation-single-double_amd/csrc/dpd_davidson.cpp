@@ -295,7 +295,7 @@ int small_eig(int n, const double* Arow, double* wr, double* wi, double* V) {
 // Davidson
 // ---------------------------------------------------------------------------
 struct DavWork {
-  DevBuf B, AB, NB, NAB, AQ, E, P, coef, part, partA, partB, partC, partD, dots, amaxp, amaxpi, amaxv, amaxi, wdev,
+  DevBuf B, AB, NB, NAB, AQ, E, P, coef, part, partA, partB, partC, partD, dots, amaxp, amaxpi, amaxv, amaxi, wdev, tri,
       idx, cgs;
 };
 
@@ -359,7 +359,8 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       (rc = W.amaxp.alloc(sizeof(double) * kRedBlocks * nroot)) ||
       (rc = W.amaxpi.alloc(sizeof(int64_t) * kRedBlocks * nroot)) || (rc = W.amaxv.alloc(sizeof(double) * nroot)) ||
       (rc = W.amaxi.alloc(sizeof(int64_t) * nroot)) || (rc = W.wdev.alloc(sizeof(double) * nroot)) ||
-      (rc = W.idx.alloc(sizeof(int32_t) * nroot)) || (rc = W.cgs.alloc(256)))
+      (rc = W.idx.alloc(sizeof(int32_t) * nroot)) || (rc = W.cgs.alloc(256)) ||
+      (rc = W.tri.alloc(sizeof(double) * (2 * kMgsMulti * kMgsMulti + kMgsMulti))))
     return rc;
   double* B = (double*)W.B.p;
   double* AB = (double*)W.AB.p;
@@ -580,12 +581,13 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     // every basis vector (the new ones of this iteration included), normalise;
     // skipped when the remaining norm is below 1e-6.  Against the old basis the
     // directions are projected together with block operations (classical
-    // Gram-Schmidt applied twice); against this iteration's new vectors one
-    // modified Gram-Schmidt step per vector (one launch that finishes the
-    // previous reduction, updates p and starts the next).  The roots are
-    // orthogonalised back to back assuming none is skipped; the norms are
+    // Gram-Schmidt applied twice); among this iteration's new vectors by
+    // Cholesky QR (twice), whose pivots are the sequential remaining norms.
+    // The roots are orthogonalised assuming none is skipped; the norms are
     // checked once at the end and, in the rare case that one fell below 1e-6,
-    // the roots from there on are redone one at a time (sequential MGS).
+    // the roots from there on are redone one at a time (sequential MGS: one
+    // launch per step that finishes the previous reduction, updates p and
+    // starts the next).
     auto schmidt = [&](size_t k, int slot, double* nrm_dev) -> int {
       double* p = PV + k * n;
       double* pin = (double*)W.partA.p;
@@ -604,45 +606,47 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     };
     {
       const size_t nopt = std::min<size_t>(cand.size(), (size_t)std::max(dimFree, 0));
-      // phase 1: up to kMgsMulti directions at a time are normalised and
-      // projected on the old basis together (each basis vector read once);
-      // phase 2: each is projected on this iteration's earlier new vectors
+      // up to kMgsMulti directions at a time: normalised and projected together
+      // on the basis so far (each basis vector read once), then orthonormalised
+      // among themselves by Cholesky QR applied twice -- in exact arithmetic the
+      // vectors and remaining norms of the sequential Gram-Schmidt
+      double* T1 = (double*)W.tri.p;
+      double* T2 = T1 + kMgsMulti * kMgsMulti;
+      double* piv = T2 + kMgsMulti * kMgsMulti;
       for (size_t c0 = 0; c0 < nopt; c0 += kMgsMulti) {
         const int nv = (int)std::min<size_t>(kMgsMulti, nopt - c0);
         double* P0 = PV + c0 * n;
         double* pin = (double*)W.partA.p;
         double* pout = (double*)W.partB.p;
-        // normalise, then project on the old (orthonormal) basis twice with
-        // block operations (classical Gram-Schmidt with one re-orthogonalisation:
-        // in exact arithmetic the same vector as the sequential projections)
+        const int nb = dimIter + (int)c0;  // old basis and the new vectors of earlier groups
+        // normalise, then project on the (orthonormal) basis twice with block
+        // operations (classical Gram-Schmidt with one re-orthogonalisation)
         DPD_HIP_TRY(launch_pair_dots(n, P0, n, P0, n, nv, pin, nullptr, st));
         DPD_HIP_TRY(launch_mgs_multi(n, P0, nv, nullptr, nullptr, pin, 0, pout, st));
         // the second projection runs only when some direction lost more than
         // half its squared norm to the first (decided on the device)
         int* need = (int*)W.cgs.p;
-        for (int pass = 0; pass < 2; pass++) {
-          DPD_HIP_TRY(launch_dots_multi(n, P0, nv, B, dimIter, part, (double*)W.coef.p, st, pass ? need : nullptr));
-          if (pass == 0 && dimIter > 0) DPD_HIP_TRY(launch_cgs_need((const double*)W.coef.p, nv, dimIter, need, st));
-          DPD_HIP_TRY(launch_lincomb_sub(n, B, dimIter, (const double*)W.coef.p, nv, P0, st, pass ? need : nullptr));
-        }
-        DPD_HIP_TRY(launch_pair_dots(n, P0, n, P0, n, nv, pin, nullptr, st));
-        for (int j = 0; j < nv; j++) {
-          const size_t k = c0 + j;
-          double* pk = PV + k * n;
-          const double* fin = pin + (size_t)j * kRedBlocks;  // p_k . p_k after phase 1
-          if (k > 0) {
-            double* qa = (double*)W.partC.p;
-            double* qb = (double*)W.partD.p;
-            DPD_HIP_TRY(launch_pair_dots_partial(n, pk, bv(dimIter), qa, st));
-            for (size_t j2 = 0; j2 < k; j2++) {
-              const double* bnext = j2 + 1 < k ? bv(dimIter + (int)j2 + 1) : nullptr;
-              DPD_HIP_TRY(launch_mgs_step(n, pk, bv(dimIter + (int)j2), bnext, qa, 1, qb, st));
-              std::swap(qa, qb);
-            }
-            fin = qa;
+        if (nb > 0) {
+          for (int pass = 0; pass < 2; pass++) {
+            DPD_HIP_TRY(launch_dots_multi(n, P0, nv, B, nb, part, (double*)W.coef.p, st, pass ? need : nullptr));
+            if (pass == 0) DPD_HIP_TRY(launch_cgs_need((const double*)W.coef.p, nv, nb, need, st));
+            DPD_HIP_TRY(launch_lincomb_sub(n, B, nb, (const double*)W.coef.p, nv, P0, st, pass ? need : nullptr));
           }
-          DPD_HIP_TRY(launch_reduce_partials(fin, 1, dots + k, st));
-          DPD_HIP_TRY(launch_scale_by_norm(n, pk, fin, bv(dimIter + (int)k), st));
+        }
+        // Q = P L^-T with G = P^T P = L L^T; the pivots L_jj are the remaining
+        // norms (a direction below 1e-6 zeroes itself and the ones after it).
+        // A second pass on Q restores the orthogonality the first loses to
+        // the conditioning of G; its squared pivots times the first ones' are
+        // the norms the host checks (dots[c0 + j])
+        DPD_HIP_TRY(launch_tri_apply(n, P0, nv, nullptr, nullptr, part, st));
+        if (nv == 1) {
+          DPD_HIP_TRY(launch_chol_tri(part, nv, nullptr, T1, nullptr, dots + c0, st));
+          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T1, bv(dimIter + (int)c0), nullptr, st));
+        } else {
+          DPD_HIP_TRY(launch_chol_tri(part, nv, nullptr, T1, piv, nullptr, st));
+          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T1, P0, part, st));
+          DPD_HIP_TRY(launch_chol_tri(part, nv, piv, T2, nullptr, dots + c0, st));
+          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T2, bv(dimIter + (int)c0), nullptr, st));
         }
       }
       spec = nopt;

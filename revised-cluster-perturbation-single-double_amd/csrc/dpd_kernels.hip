@@ -1122,6 +1122,166 @@ hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream
   return hipGetLastError();
 }
 
+// Y_j = sum_{i<=j} T[i + j*nv] X_i for j < nv <= kMgsMulti (T == nullptr: Y = X;
+// Y == nullptr: nothing stored; Y may be X: each thread reads its elements of
+// every X_i before it writes); with partial given, also the partials of the
+// Gram matrix Y_j . Y_k, j <= k, in rows j + k*nv of kRedBlocks as
+// k_dots_multi writes them.  Grid: kRedBlocks blocks of 256.
+__global__ __launch_bounds__(256) void k_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y,
+                                                   double* partial) {
+  __shared__ double red[4];
+  double t[kMgsMulti][kMgsMulti];
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++)
+#pragma unroll
+    for (int i = 0; i < kMgsMulti; i++) t[i][j] = (i <= j && j < nv) ? (T ? T[i + j * nv] : (i == j ? 1.0 : 0.0)) : 0.0;
+  double g[kMgsMulti][kMgsMulti];
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++)
+#pragma unroll
+    for (int k = 0; k < kMgsMulti; k++) g[j][k] = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double x[kMgsMulti], y[kMgsMulti];
+#pragma unroll
+    for (int i = 0; i < kMgsMulti; i++) x[i] = i < nv ? X[(int64_t)i * n + e] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kMgsMulti; j++) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i <= j; i++) s += t[i][j] * x[i];
+      y[j] = s;
+    }
+    if (Y) {
+#pragma unroll
+      for (int j = 0; j < kMgsMulti; j++)
+        if (j < nv) Y[(int64_t)j * n + e] = y[j];
+    }
+    if (partial) {
+#pragma unroll
+      for (int k = 0; k < kMgsMulti; k++)
+#pragma unroll
+        for (int j = 0; j <= k; j++) g[j][k] += y[j] * y[k];
+    }
+  }
+  if (!partial) return;
+#pragma unroll
+  for (int k = 0; k < kMgsMulti; k++)
+#pragma unroll
+    for (int j = 0; j <= k; j++)
+      if (k < nv) {
+        const double v = block_sum_256(g[j][k], red);
+        if (threadIdx.x == 0) partial[((int64_t)j + (int64_t)k * nv) * kRedBlocks + blockIdx.x] = v;
+      }
+}
+
+hipError_t launch_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y, double* partial,
+                            hipStream_t s) {
+  if (nv <= 0 || nv > kMgsMulti) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_tri_apply, dim3(kRedBlocks), dim3(256), 0, s, n, X, nv, T, Y, partial);
+  return hipGetLastError();
+}
+
+// Cholesky QR step for nv <= kMgsMulti vectors from the partials of their Gram
+// matrix G (rows j + k*nv, j <= k): G = L L^T and T = L^-T (upper triangular,
+// [i + j*nv]), so that Q = P T has orthonormal columns and L_jj is the norm of
+// p_j left after its components along q_0 .. q_j-1 are removed -- the norm the
+// sequential Gram-Schmidt compares with 1e-6.  The factorisation stops at the
+// first j whose remaining norm, times piv_in[j] when given (the pivot of an
+// earlier pass), is below 1e-6 or not a number; columns j.. of T are zero.
+// piv_out[j] (if given) = that combined norm, nrm2_out[j] (if given) = its
+// square, both 0 from the stopping column on.  One wave.
+__global__ __launch_bounds__(64) void k_chol_tri(const double* partial, int nv, const double* piv_in, double* T,
+                                                 double* piv_out, double* nrm2_out) {
+  __shared__ double G[kMgsMulti * kMgsMulti];
+  const int lane = threadIdx.x;
+  for (int r = 0; r < nv * nv; r++) {
+    const int j = r % nv, k = r / nv;
+    if (j > k) continue;
+    const double* p = partial + (int64_t)r * kRedBlocks;
+    double s = 0.0;
+    for (int i = lane; i < kRedBlocks; i += 64) s += p[i];
+    s = wave_sum(s);
+    if (lane == 0) {
+      G[j + k * nv] = s;
+      G[k + j * nv] = s;
+    }
+  }
+  __syncthreads();
+  if (lane != 0) return;
+  double L[kMgsMulti][kMgsMulti], Ld[kMgsMulti], f[kMgsMulti];
+#pragma unroll
+  for (int i = 0; i < kMgsMulti; i++) {
+    Ld[i] = 0.0;
+    f[i] = (piv_in && i < nv) ? piv_in[i] : 1.0;
+#pragma unroll
+    for (int j = 0; j < kMgsMulti; j++) L[i][j] = 0.0;
+  }
+  int nvalid = nv;
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++) {
+    if (j < nvalid) {
+      double d = G[j + j * nv];
+#pragma unroll
+      for (int m = 0; m < j; m++) d -= L[j][m] * L[j][m];
+      if (!(d * f[j] * f[j] >= 1e-12)) {
+        nvalid = j;
+      } else {
+        const double l = sqrt(d);
+        L[j][j] = l;
+        Ld[j] = 1.0 / l;
+#pragma unroll
+        for (int i = j + 1; i < kMgsMulti; i++) {
+          if (i < nv) {
+            double s = G[i + j * nv];
+#pragma unroll
+            for (int m = 0; m < j; m++) s -= L[i][m] * L[j][m];
+            L[i][j] = s * Ld[j];
+          }
+        }
+      }
+    }
+  }
+  // L^-1 (lower triangular) of the leading nvalid columns
+  double Li[kMgsMulti][kMgsMulti];
+#pragma unroll
+  for (int i = 0; i < kMgsMulti; i++)
+#pragma unroll
+    for (int j = 0; j < kMgsMulti; j++) Li[i][j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++) {
+    if (j < nvalid) {
+      Li[j][j] = Ld[j];
+#pragma unroll
+      for (int i = j + 1; i < kMgsMulti; i++) {
+        if (i < nvalid) {
+          double s = 0.0;
+#pragma unroll
+          for (int m = j; m < i; m++) s += L[i][m] * Li[m][j];
+          Li[i][j] = -s * Ld[i];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++) {
+    if (j < nv) {
+#pragma unroll
+      for (int i = 0; i < kMgsMulti; i++)
+        if (i < nv) T[i + j * nv] = (j < nvalid && i <= j) ? Li[j][i] : 0.0;
+      const double pv = j < nvalid ? L[j][j] * f[j] : 0.0;
+      if (piv_out) piv_out[j] = pv;
+      if (nrm2_out) nrm2_out[j] = pv * pv;
+    }
+  }
+}
+
+hipError_t launch_chol_tri(const double* partial, int nv, const double* piv_in, double* T, double* piv_out,
+                           double* nrm2_out, hipStream_t s) {
+  if (nv <= 0 || nv > kMgsMulti) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_chol_tri, dim3(1), dim3(64), 0, s, partial, nv, piv_in, T, piv_out, nrm2_out);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_scale_by_norm(int64_t n, const double* __restrict__ p, const double* part,
                                                        double* __restrict__ dst) {
   __shared__ double red[4];

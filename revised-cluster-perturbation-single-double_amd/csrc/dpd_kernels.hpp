@@ -240,6 +240,15 @@ hipError_t launch_lincomb_sub(int64_t n, const double* B, int nb, const double* 
 // need = 1 when any of nv unit vectors lost more than half its squared norm to
 // its projections C on an orthonormal basis (a second projection is needed)
 hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream_t s);
+// Y_j = sum_{i<=j} T[i + j*nv] X_i (T null: identity; Y null: not stored; Y may
+// be X) and, with partial given, the Gram partials of Y (rows j + k*nv, j <= k)
+hipError_t launch_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y, double* partial,
+                            hipStream_t s);
+// Cholesky QR step from those Gram partials: T = L^-T, stopping at the first
+// remaining norm (times piv_in) below 1e-6; piv_out / nrm2_out: the norms and
+// their squares (0 from the stopping column on)
+hipError_t launch_chol_tri(const double* partial, int nv, const double* piv_in, double* T, double* piv_out,
+                           double* nrm2_out, hipStream_t s);
 // dst = p * (1/sqrt(sum(part))) ; returns nothing (norm read by the host separately)
 hipError_t launch_scale_by_norm(int64_t n, const double* p, const double* part, double* dst, hipStream_t s);
 // p = e / clamp(w - d) for the roots listed in idx (w on device)
