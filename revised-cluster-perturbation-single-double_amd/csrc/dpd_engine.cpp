@@ -1508,7 +1508,7 @@ static int call_post(dpd_ctx* c, const CallPrep& r, const Operand& o3, int nbatc
 static int64_t call_bytes(dpd_ctx* c, const CallPrep& r, const Operand& o1, const Operand& o2, int prec) {
   const ContractPlan* P = r.P;
   const bool use_f32 = prec != PREC_F64;
-  int64_t bytes = P->gemm.c_elems * 8 * (r.beta != 0.0 ? 2 : 1);
+  int64_t bytes = P->gemm.c_elems * (r.p3_f32 ? 4 : 8) * (r.beta != 0.0 ? 2 : 1);
   if (r.packed) {
     const GemmVariant& V = P->gemm.pk[prec];
     bytes += (V.pa_elems + V.pb_elems) * pk_esize(prec);

@@ -1027,47 +1027,57 @@ hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, c
 // C[j + k*nv] = P_j . B_k for nv <= kMgsMulti vectors P (stride n) and nb basis
 // vectors B (stride n); grid (kRedBlocks, ceil(nb/4)); each basis vector is read
 // once, partials reduced in a fixed order
+constexpr int kDotsKW = 4;  // basis vectors per block of k_dots_multi
 __global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __restrict__ P, int nv,
                                                     const double* __restrict__ B, int nb, double* partial,
                                                     const int* __restrict__ need) {
-  __shared__ double red[4];
+  __shared__ double red[4][kMgsMulti * kDotsKW];
   if (need && *need == 0) return;
-  const int k0 = blockIdx.y * 4;
-  const int nk = min(4, nb - k0);
-  double acc[kMgsMulti][4];
+  const int k0 = blockIdx.y * kDotsKW;
+  const int nk = min(kDotsKW, nb - k0);
+  double acc[kMgsMulti][kDotsKW];
 #pragma unroll
   for (int j = 0; j < kMgsMulti; j++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) acc[j][q] = 0.0;
+    for (int q = 0; q < kDotsKW; q++) acc[j][q] = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double b[4];
+    double b[kDotsKW];
 #pragma unroll
-    for (int q = 0; q < 4; q++) b[q] = q < nk ? B[(int64_t)(k0 + q) * n + i] : 0.0;
+    for (int q = 0; q < kDotsKW; q++) b[q] = q < nk ? B[(int64_t)(k0 + q) * n + i] : 0.0;
 #pragma unroll
     for (int j = 0; j < kMgsMulti; j++) {
       if (j < nv) {
         const double x = P[(int64_t)j * n + i];
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[j][q] += x * b[q];
+        for (int q = 0; q < kDotsKW; q++) acc[j][q] += x * b[q];
       }
     }
   }
+  // every wave reduces its sums, then one barrier; wave sums added in a fixed order
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int j = 0; j < kMgsMulti; j++)
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < kDotsKW; q++)
       if (j < nv && q < nk) {
-        const double v = block_sum_256(acc[j][q], red);
-        if (threadIdx.x == 0) partial[((int64_t)j + (int64_t)(k0 + q) * nv) * kRedBlocks + blockIdx.x] = v;
+        const double v = wave_sum(acc[j][q]);
+        if (lane == 0) red[wave][j * kDotsKW + q] = v;
       }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kMgsMulti * kDotsKW; t += blockDim.x) {
+    const int j = t / kDotsKW, q = t % kDotsKW;
+    if (j < nv && q < nk)
+      partial[((int64_t)j + (int64_t)(k0 + q) * nv) * kRedBlocks + blockIdx.x] =
+          (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+  }
 }
 
 hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
                              double* out, hipStream_t s, const int* need) {
   if (nv <= 0 || nb <= 0) return hipSuccess;
   if (nv > kMgsMulti) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_dots_multi, dim3(kRedBlocks, (nb + 3) / 4), dim3(256), 0, s, n, P, nv, B, nb, partial,
-                     need);
+  hipLaunchKernelGGL(k_dots_multi, dim3(kRedBlocks, (nb + kDotsKW - 1) / kDotsKW), dim3(256), 0, s, n, P, nv, B, nb,
+                     partial, need);
   hipLaunchKernelGGL(k_reduce_rows, dim3(nv * nb), dim3(256), 0, s, partial, kRedBlocks, out);
   return hipGetLastError();
 }
@@ -1129,7 +1139,7 @@ hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream
 // k_dots_multi writes them.  Grid: kRedBlocks blocks of 256.
 __global__ __launch_bounds__(256) void k_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y,
                                                    double* partial) {
-  __shared__ double red[4];
+  __shared__ double red[4][kMgsMulti * kMgsMulti];
   double t[kMgsMulti][kMgsMulti];
 #pragma unroll
   for (int j = 0; j < kMgsMulti; j++)
@@ -1164,14 +1174,21 @@ __global__ __launch_bounds__(256) void k_tri_apply(int64_t n, const double* X, i
     }
   }
   if (!partial) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < kMgsMulti; k++)
 #pragma unroll
     for (int j = 0; j <= k; j++)
       if (k < nv) {
-        const double v = block_sum_256(g[j][k], red);
-        if (threadIdx.x == 0) partial[((int64_t)j + (int64_t)k * nv) * kRedBlocks + blockIdx.x] = v;
+        const double v = wave_sum(g[j][k]);
+        if (lane == 0) red[wave][j + k * kMgsMulti] = v;
       }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kMgsMulti * kMgsMulti; t += blockDim.x) {
+    const int j = t % kMgsMulti, k = t / kMgsMulti;
+    if (j <= k && k < nv)
+      partial[((int64_t)j + (int64_t)k * nv) * kRedBlocks + blockIdx.x] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+  }
 }
 
 hipError_t launch_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y, double* partial,
@@ -1189,25 +1206,50 @@ hipError_t launch_tri_apply(int64_t n, const double* X, int nv, const double* T,
 // first j whose remaining norm, times piv_in[j] when given (the pivot of an
 // earlier pass), is below 1e-6 or not a number; columns j.. of T are zero.
 // piv_out[j] (if given) = that combined norm, nrm2_out[j] (if given) = its
-// square, both 0 from the stopping column on.  One wave.
-__global__ __launch_bounds__(64) void k_chol_tri(const double* partial, int nv, const double* piv_in, double* T,
-                                                 double* piv_out, double* nrm2_out) {
-  __shared__ double G[kMgsMulti * kMgsMulti];
-  const int lane = threadIdx.x;
-  for (int r = 0; r < nv * nv; r++) {
-    const int j = r % nv, k = r / nv;
-    if (j > k) continue;
-    const double* p = partial + (int64_t)r * kRedBlocks;
+// square, both 0 from the stopping column on.  One block of 256.
+__global__ __launch_bounds__(256) void k_chol_tri(const double* partial, int nv, const double* piv_in, double* T,
+                                                  double* piv_out, double* nrm2_out) {
+  // the nv(nv+1)/2 Gram rows are dealt round-robin to the 4 waves; every load
+  // of a wave is issued before the first sum (one memory latency, not one per row)
+  constexpr int kRowsPerWave = (kMgsMulti * (kMgsMulti + 1) / 2 + 3) / 4;
+  constexpr int kPer = kRedBlocks / 64;
+  __shared__ double Gs[kMgsMulti * kMgsMulti];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nr = nv * (nv + 1) / 2;
+  double v[kRowsPerWave][kPer];
+#pragma unroll
+  for (int q = 0; q < kRowsPerWave; q++) {
+    const int r = wave + 4 * q;
+    int k = 0;
+    while ((k + 1) * (k + 2) / 2 <= r) k++;
+    const int j = r - k * (k + 1) / 2;
+    // rows past nr read row 0 (always present) and are discarded
+    const double* p = partial + (r < nr ? (int64_t)j + (int64_t)k * nv : 0) * kRedBlocks;
+#pragma unroll
+    for (int i = 0; i < kPer; i++) v[q][i] = p[lane + 64 * i];
+  }
+#pragma unroll
+  for (int q = 0; q < kRowsPerWave; q++) {
+    const int r = wave + 4 * q;
     double s = 0.0;
-    for (int i = lane; i < kRedBlocks; i += 64) s += p[i];
+#pragma unroll
+    for (int i = 0; i < kPer; i++) s += v[q][i];
     s = wave_sum(s);
-    if (lane == 0) {
-      G[j + k * nv] = s;
-      G[k + j * nv] = s;
+    if (lane == 0 && r < nr) {
+      int k = 0;
+      while ((k + 1) * (k + 2) / 2 <= r) k++;
+      const int j = r - k * (k + 1) / 2;
+      Gs[j + k * nv] = s;
+      Gs[k + j * nv] = s;
     }
   }
   __syncthreads();
-  if (lane != 0) return;
+  if (threadIdx.x != 0) return;
+  double G[kMgsMulti][kMgsMulti];
+#pragma unroll
+  for (int i = 0; i < kMgsMulti; i++)
+#pragma unroll
+    for (int j = 0; j < kMgsMulti; j++) G[i][j] = (i < nv && j < nv) ? Gs[i + j * nv] : 0.0;
   double L[kMgsMulti][kMgsMulti], Ld[kMgsMulti], f[kMgsMulti];
 #pragma unroll
   for (int i = 0; i < kMgsMulti; i++) {
@@ -1220,7 +1262,7 @@ __global__ __launch_bounds__(64) void k_chol_tri(const double* partial, int nv, 
 #pragma unroll
   for (int j = 0; j < kMgsMulti; j++) {
     if (j < nvalid) {
-      double d = G[j + j * nv];
+      double d = G[j][j];
 #pragma unroll
       for (int m = 0; m < j; m++) d -= L[j][m] * L[j][m];
       if (!(d * f[j] * f[j] >= 1e-12)) {
@@ -1232,7 +1274,7 @@ __global__ __launch_bounds__(64) void k_chol_tri(const double* partial, int nv, 
 #pragma unroll
         for (int i = j + 1; i < kMgsMulti; i++) {
           if (i < nv) {
-            double s = G[i + j * nv];
+            double s = G[i][j];
 #pragma unroll
             for (int m = 0; m < j; m++) s -= L[i][m] * L[j][m];
             L[i][j] = s * Ld[j];
@@ -1278,7 +1320,7 @@ __global__ __launch_bounds__(64) void k_chol_tri(const double* partial, int nv, 
 hipError_t launch_chol_tri(const double* partial, int nv, const double* piv_in, double* T, double* piv_out,
                            double* nrm2_out, hipStream_t s) {
   if (nv <= 0 || nv > kMgsMulti) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_chol_tri, dim3(1), dim3(64), 0, s, partial, nv, piv_in, T, piv_out, nrm2_out);
+  hipLaunchKernelGGL(k_chol_tri, dim3(1), dim3(256), 0, s, partial, nv, piv_in, T, piv_out, nrm2_out);
   return hipGetLastError();
 }
 
