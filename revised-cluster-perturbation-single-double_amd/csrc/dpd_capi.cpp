@@ -471,6 +471,7 @@ int dpd_ctx_set_gemm_config(dpd_ctx* c, int cfg, int packing) {
   c->gemm_cfg = cfg;
   c->packing = packing;
   c->group_cache.clear();  // holds pointers into contract_cache
+  dpd::bump_mem_gen();
   c->contract_cache.clear();
   c->engine_cache.clear();
   return DPD_OK;
@@ -482,6 +483,7 @@ int dpd_ctx_set_split_k(dpd_ctx* c, int mode) {
   if (c->device >= 0) DPD_HIP_TRY(hipStreamSynchronize(c->stream));
   c->split_k = mode;
   c->group_cache.clear();
+  dpd::bump_mem_gen();
   c->contract_cache.clear();
   c->engine_cache.clear();
   return DPD_OK;

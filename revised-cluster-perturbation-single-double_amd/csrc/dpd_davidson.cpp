@@ -794,10 +794,15 @@ struct GraphCache {
   std::unordered_map<std::string, int> seen;
   std::unordered_map<std::string, hipGraphExec_t> exec;
   std::vector<hipGraph_t> graphs;
-  ~GraphCache() {
+  uint64_t gen = 0;  // device-memory generation the graphs were captured at
+  void drop() {
     for (auto& e : exec) (void)hipGraphExecDestroy(e.second);
     for (hipGraph_t g : graphs) (void)hipGraphDestroy(g);
+    exec.clear();
+    graphs.clear();
+    for (auto& kv : seen) kv.second = 0;
   }
+  ~GraphCache() { drop(); }
 };
 
 }  // namespace
@@ -883,6 +888,15 @@ extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davi
       return DPD_OK;
     }
     if (!graph) return problem_sigma_group(ps.data(), xs.data(), ys.data(), nv.data(), m);
+    // graphs hold addresses of the workspace, task tables and packed panels:
+    // once any of those was freed, moved or repacked they are dropped and the
+    // shapes learnt again (two eager runs, then a capture)
+    const uint64_t gen = g_mem_gen.load(std::memory_order_relaxed);
+    if (gen != gc.gen) {
+      if (!gc.exec.empty()) DPD_HIP_TRY(hipStreamSynchronize(st));
+      gc.drop();
+      gc.gen = gen;
+    }
     for (int i = 0; i < m; i++)
       DPD_HIP_TRY(hipMemcpyAsync((void*)xs[i], reqs[i]->x, sizeof(double) * ps[i]->dim * nv[i],
                                  hipMemcpyDeviceToDevice, st));
@@ -900,13 +914,16 @@ extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davi
     } else {
       bool ok = seen == 2;
       if (ok) {
-        // capture; a step that still needs an allocation or a synchronisation
-        // fails the capture and stays eager
+        // capture; a step that would still allocate, free, repack or
+        // synchronise stops before doing so (g_capturing), fails the capture
+        // and stays eager
         hipGraph_t g = nullptr;
         DPD_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        g_capturing = true;
         int r = problem_sigma_group(ps.data(), xs.data(), ys.data(), nv.data(), m);
+        g_capturing = false;
         hipError_t e = hipStreamEndCapture(st, &g);
-        ok = r == DPD_OK && e == hipSuccess && g;
+        ok = r == DPD_OK && e == hipSuccess && g && g_mem_gen.load(std::memory_order_relaxed) == gc.gen;
         if (ok) {
           gc.graphs.push_back(g);
           hipGraphExec_t ge = nullptr;

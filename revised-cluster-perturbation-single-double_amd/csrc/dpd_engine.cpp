@@ -50,14 +50,28 @@ int DevBuf::alloc(size_t b) {
   return DPD_OK;
 }
 
+std::atomic<uint64_t> g_mem_gen{0};
+thread_local bool g_capturing = false;
+
 int ws_reserve(dpd_ctx* c, size_t bytes) {
   bytes = (bytes + 255) & ~size_t(255);
   if (c->ws.bytes >= c->ws_top + bytes) return DPD_OK;
   if (c->ws_top != 0)
     return fail(DPD_ENOMEM, "workspace must be reserved before it is used (internal)");
+  if (g_capturing) return fail(DPD_EINVAL, "workspace growth during graph capture (internal)");
   DPD_HIP_TRY(hipStreamSynchronize(c->stream));
   size_t nb = std::max(bytes, c->ws.bytes * 2);
+  bump_mem_gen();
   return c->ws.alloc(nb);
+}
+
+void clear_pack_cache(GemmVariant& V) {
+  bool any = false;
+  for (auto& side : V.cache) {
+    any = any || !side.empty();
+    side.clear();
+  }
+  if (any) bump_mem_gen();
 }
 
 void* ws_take(dpd_ctx* c, size_t bytes) {
@@ -590,6 +604,8 @@ int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested) {
 // fewer than 1024 tiles (partials summed by a reduce kernel); off: one
 // workgroup per tile
 static int build_variant_cfg(GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k = true) {
+  if (g_capturing) return fail(DPD_EINVAL, "tile variant built during graph capture (internal)");
+  bump_mem_gen();  // the variant's task tables are rewritten and reallocated
   V.mode = mode;
   V.tasks = p.tasks;
   V.split.clear();
@@ -706,10 +722,9 @@ static int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const Pa
   return DPD_OK;
 }
 
-// packed-panel GEMM: static operands come from the variant's cache (packed on
-// first use), the others are packed into the workspace reserved by the caller
 // panels of both operand sides of variant V: static operands from V's cache
-// (packed on first use), per-call operands packed into the workspace
+// (packed on first use; one entry per operand, found by data pointer and id),
+// per-call operands packed into the workspace reserved by the caller
 static int pack_sides(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int mode, const PackSrc& A, const PackSrc& B,
                       const void* side_ptr[2]) {
   const size_t es = pk_esize(mode);
@@ -721,18 +736,35 @@ static int pack_sides(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int mode, const P
       continue;
     }
     if (src[sd]->id) {
-      PackCache& pc = V.cache[sd];
-      if (pc.src != src[sd]->data || pc.id != src[sd]->id || !pc.buf.p) {
-        if (!pc.buf.p) {
-          int rc = pc.buf.alloc(es * elems[sd]);
-          if (rc) return rc;
-        }
-        int rc = pack_side(c, p, V, sd, *src[sd], pc.buf.p);
-        if (rc) return rc;
-        pc.src = src[sd]->data;
-        pc.id = src[sd]->id;
+      PackCache* hit = nullptr;
+      PackCache* same_src = nullptr;
+      for (auto& e : V.cache[sd]) {
+        if (e->src == src[sd]->data && e->id == src[sd]->id) hit = e.get();
+        else if (e->src == src[sd]->data) same_src = e.get();
       }
-      side_ptr[sd] = pc.buf.p;
+      if (!hit) {
+        if (g_capturing) return fail(DPD_EINVAL, "operand panels packed during graph capture (internal)");
+        if (same_src) {
+          // the operand's contents changed (new id at the same address): repack in place
+          hit = same_src;
+          bump_mem_gen();
+        } else {
+          V.cache[sd].push_back(std::make_unique<PackCache>());
+          hit = V.cache[sd].back().get();
+          int rc = hit->buf.alloc(es * elems[sd]);
+          if (rc) {
+            V.cache[sd].pop_back();
+            return rc;
+          }
+        }
+        hit->src = nullptr;
+        hit->id = 0;
+        int rc = pack_side(c, p, V, sd, *src[sd], hit->buf.p);
+        if (rc) return rc;
+        hit->src = src[sd]->data;
+        hit->id = src[sd]->id;
+      }
+      side_ptr[sd] = hit->buf.p;
     } else {
       void* w = ws_take(c, es * elems[sd]);
       if (!w) return DPD_ENOMEM;
@@ -863,13 +895,7 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
   GemmVariant& V = p.pk[mode];
   if (V.built && V.cfg == best && V.split_k == best_split) return DPD_OK;
   // panels cached for the old tile shape are rebuilt on the next call
-  if (!(V.built && V.cfg == best)) {
-    for (auto& pc : V.cache) {
-      pc.src = nullptr;
-      pc.id = 0;
-      pc.buf.reset();
-    }
-  }
+  if (!(V.built && V.cfg == best)) clear_pack_cache(V);
   return build_variant_cfg(p, V, mode, best, best_split);
 }
 
@@ -1579,11 +1605,8 @@ static int group_variants(GroupPlan& G, int prec, int cfg) {
   for (ContractPlan* P : G.members) {
     GemmVariant& V = P->gemm.pk[prec];
     if (V.built && V.cfg == cfg) continue;
-    for (auto& pc : V.cache) {
-      pc.src = nullptr;
-      pc.id = 0;
-      pc.buf.reset();
-    }
+    if (g_capturing) return fail(DPD_EINVAL, "tile variant rebuilt during graph capture (internal)");
+    clear_pack_cache(V);
     int rc = build_variant_cfg(P->gemm, V, prec, cfg);
     if (rc) return rc;
     V.tuned = true;  // a member's tile is the group's
@@ -1595,6 +1618,8 @@ static int group_variants(GroupPlan& G, int prec, int cfg) {
 // split-K is decided again over the whole group (the same rule as for one
 // plan: split long-K tasks while the launch has fewer than 1024 tiles)
 static int group_tasks(const std::vector<const GemmVariant*>& Vs, int mode, GroupTasks& G) {
+  if (g_capturing) return fail(DPD_EINVAL, "grouped task list built during graph capture (internal)");
+  bump_mem_gen();  // the group's task tables are reallocated
   std::vector<GemmTask> all;
   for (size_t i = 0; i < Vs.size(); i++)
     for (GemmTask tk : Vs[i]->tasks) {
@@ -1780,6 +1805,19 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
     g->cfg = c->gemm_cfg >= GEMM_64x64 ? c->gemm_cfg : choose_gemm_cfg(all, -1);
     if (g->cfg == GEMM_V1) g->cfg = GEMM_64x64;
     g->tuned = c->gemm_cfg != -1;
+    // members already built with one common tile (e.g. a batch whose member
+    // set shrinks) keep it: no re-tuning, no rebuilt panels
+    int common = -1;
+    bool same = c->gemm_cfg == -1;
+    for (ContractPlan* P : plans) {
+      const GemmVariant& V = P->gemm.pk[prec];
+      if (!V.built || (common >= 0 && V.cfg != common)) same = false;
+      common = V.cfg;
+    }
+    if (same && common >= GEMM_64x64) {
+      g->cfg = common;
+      g->tuned = true;
+    }
     G = g.get();
     c->group_cache.emplace(gkey, std::move(g));
   } else {

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -17,7 +18,7 @@ namespace dpd {
 #define DPD_HIP_TRY(expr)                                                              \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess) return fail(DPD_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    if (e_ != hipSuccess) return fail(DPD_EHIP, "%s: %s (hip %d)", #expr, hipGetErrorString(e_), (int)e_); \
   } while (0)
 
 // device buffer owned by RAII
@@ -57,8 +58,21 @@ struct GemmVariant {
   DevBuf dev;
   GemmTask* d_tasks = nullptr;
   int32_t* d_split = nullptr;
-  PackCache cache[2];
+  // per side, one packed copy per static operand (problems of the same shape
+  // share the plan, each keeps its own panels)
+  std::vector<std::unique_ptr<PackCache>> cache[2];
 };
+// drops the packed copies of V (buffers freed)
+void clear_pack_cache(GemmVariant& V);
+
+// Generation of the device memory that captured graphs may reference: bumped
+// whenever the workspace, a plan's task tables or a packed-panel buffer is
+// freed, moved or repacked.  A graph captured at another generation is stale.
+extern std::atomic<uint64_t> g_mem_gen;
+inline void bump_mem_gen() { g_mem_gen.fetch_add(1, std::memory_order_relaxed); }
+// set by a thread while it captures a graph: every path that would allocate,
+// free or repack fails instead (before any HIP call)
+extern thread_local bool g_capturing;
 
 struct GemmPlan {
   std::vector<GemmTask> tasks;
