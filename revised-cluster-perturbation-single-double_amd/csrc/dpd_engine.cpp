@@ -710,6 +710,7 @@ static int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const Pa
     a.xlen = src.xlen;
     a.src_bstride = src.bstride;
   }
+  a.src_f32 = src.f32 && !src.map ? 1 : 0;
   a.dst = dst;
   a.tasks = V.d_tasks;
   a.segs = p.d_segs;
@@ -1384,6 +1385,7 @@ struct CallPrep {
   double* p3 = nullptr;
   double beta = 1.0;
   bool p3_f32 = false;  // the scratch result is FP32 (FP32-accumulating packed GEMM)
+  bool x1_f32 = false, x2_f32 = false;  // expanded operands held in FP32 (they only feed the pack)
   PackSrc a, b;
 };
 
@@ -1431,8 +1433,12 @@ static int call_setup(dpd_ctx* c, ContractPlan* P, const Operand& o1, const Oper
     if (!o2.static_id) n += pk_esize(prec) * V.pb_elems + 256;
     n += sizeof(double) * V.partial_elems + 256;
   }
-  if (r.gx1) n += sizeof(double) * P->len1x * r.nb1 + 256;
-  if (r.gx2) n += sizeof(double) * P->len2x * r.nb2 + 256;
+  // an expanded operand of an FP32 or BF16x3 packed GEMM is only read by the
+  // pack, which rounds it to FP32 or BF16 hi + lo anyway: it is gathered in FP32
+  r.x1_f32 = r.gx1 && r.packed && prec != PREC_F64;
+  r.x2_f32 = r.gx2 && r.packed && prec != PREC_F64;
+  if (r.gx1) n += (r.x1_f32 ? sizeof(float) : sizeof(double)) * P->len1x * r.nb1 + 256;
+  if (r.gx2) n += (r.x2_f32 ? sizeof(float) : sizeof(double)) * P->len2x * r.nb2 + 256;
   // the scratch result of an FP32-accumulating packed GEMM is kept in FP32:
   // the accumulators are FP32, so nothing is lost and half the bytes move
   r.p3_f32 = P->epi_on && r.packed && prec != PREC_F64;
@@ -1449,28 +1455,31 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
   r.p2 = o2.data;
   r.p1f = o1.data_f32;
   r.p2f = o2.data_f32;
-  if (r.gx1) {
-    double* x = (double*)ws_take(c, sizeof(double) * P->len1x * r.nb1);
+  // expansion of operand 1 or 2 into the workspace (FP32 when it only feeds a pack)
+  auto expand = [&](const MapArgs& map, const Operand& o, int nb, int64_t lenx, bool f32, const double** dst) -> int {
+    void* x = ws_take(c, (f32 ? sizeof(float) : sizeof(double)) * lenx * nb);
     if (!x) return DPD_ENOMEM;
-    MapArgs m = P->map1;
-    m.nbatch = r.nb1;
-    m.in_bstride = o1.bstride;
-    m.out_bstride = P->len1x;
-    int rc = gather(c, m, o1.data, x);
+    MapArgs m = map;
+    m.nbatch = nb;
+    m.in_bstride = o.bstride;
+    m.out_bstride = lenx;
+    if (f32) {
+      if (m.out.len > 0 && m.nbatch > 0) DPD_HIP_TRY(launch_gather_map_f32out(m, o.data, (float*)x, c->stream));
+    } else {
+      int rc = gather(c, m, o.data, (double*)x);
+      if (rc) return rc;
+    }
+    *dst = (const double*)x;
+    return DPD_OK;
+  };
+  if (r.gx1) {
+    int rc = expand(P->map1, o1, r.nb1, P->len1x, r.x1_f32, &r.p1);
     if (rc) return rc;
-    r.p1 = x;
     r.p1f = nullptr;
   }
   if (r.gx2) {
-    double* x = (double*)ws_take(c, sizeof(double) * P->len2x * r.nb2);
-    if (!x) return DPD_ENOMEM;
-    MapArgs m = P->map2;
-    m.nbatch = r.nb2;
-    m.in_bstride = o2.bstride;
-    m.out_bstride = P->len2x;
-    int rc = gather(c, m, o2.data, x);
+    int rc = expand(P->map2, o2, r.nb2, P->len2x, r.x2_f32, &r.p2);
     if (rc) return rc;
-    r.p2 = x;
     r.p2f = nullptr;
   }
   // scratch result: every element the epilogue reads lies in a block some GEMM
@@ -1487,6 +1496,7 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
     r.a = PackSrc();
     r.b = PackSrc();
     r.a.data = r.fuse1 ? o1.data : r.p1;
+    r.a.f32 = !r.fuse1 && r.x1_f32;
     r.a.id = o1.static_id;
     // dims of a sorted operand that are GEMM rows: the ones it shares with the result
     auto rows_of = [&](const char* lab) {
@@ -1502,6 +1512,7 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
       r.a.bstride = o1.bstride;
     }
     r.b.data = r.fuse2 ? o2.data : r.p2;
+    r.b.f32 = !r.fuse2 && r.x2_f32;
     r.b.id = o2.static_id;
     if (r.fuse2) {
       r.b.map = &P->map2;

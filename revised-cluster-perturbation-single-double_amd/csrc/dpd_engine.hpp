@@ -219,7 +219,8 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
 // into the GEMM operand (fused into the pack: `data` is then the unsorted tensor,
 // xlen the sorted length per batch member, bstride the unsorted batch stride)
 struct PackSrc {
-  const double* data = nullptr;
+  const double* data = nullptr;  // floats when f32
+  bool f32 = false;
   uint64_t id = 0;
   const MapArgs* map = nullptr;
   int rowmask = 0;  // dims of the sorted operand that are GEMM rows (M or N)

@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
     const int64_t ro = rowoff[g][r], ko = koff[k];
     if (ro >= 0 && ko >= 0) {
       if (!p.use_map) {
-        v = p.src[ro + ko];
+        v = p.src_f32 ? (double)((const float*)p.src)[ro + ko] : p.src[ro + ko];
       } else {
         int hi[4];
         int64_t xi[4];

@@ -25,15 +25,15 @@ namespace dpd {
 // ===========================================================================
 __device__ inline void stage_map(const MapArgs& a, MapArgs& sa);
 
-template <typename TIN>
+template <typename TIN, typename TOUT>
 __global__ __launch_bounds__(256) void k_gather_map(MapArgs a_arg, const TIN* __restrict__ in,
-                                                    double* __restrict__ out) {
+                                                    TOUT* __restrict__ out) {
   __shared__ MapArgs a;
   stage_map(a_arg, a);
   const int64_t n = a.out.len;
   const int b = blockIdx.y;
   const TIN* inb = in + (int64_t)b * a.in_bstride;
-  double* outb = out + (int64_t)b * a.out_bstride;
+  TOUT* outb = out + (int64_t)b * a.out_bstride;
   for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n;
        o += (int64_t)gridDim.x * blockDim.x) {
     int h[4];
@@ -53,8 +53,8 @@ __global__ __launch_bounds__(256) void k_gather_map(MapArgs a_arg, const TIN* __
       if (ad >= 0) acc += a.term[t].coef * sg * (double)inb[ad];
     }
     double v = a.alpha * acc;
-    if (a.beta != 0.0) v = a.beta * outb[o] + v;
-    outb[o] = v;
+    if (a.beta != 0.0) v = a.beta * (double)outb[o] + v;
+    outb[o] = (TOUT)v;
   }
 }
 
@@ -80,9 +80,9 @@ __device__ inline void stage_map(const MapArgs& a, MapArgs& sa) {
   __syncthreads();
 }
 
-template <typename TIN>
+template <typename TIN, typename TOUT>
 __global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const TIN* __restrict__ in,
-                                                     double* __restrict__ out, int chunk) {
+                                                     TOUT* __restrict__ out, int chunk) {
   __shared__ MapArgs a;
   stage_map(a_arg, a);
   const DevLayout& L = a.out;
@@ -129,20 +129,20 @@ __global__ __launch_bounds__(256) void k_gather_cols(MapArgs a_arg, const TIN* _
     const int64_t o = col + l;
     for (int bt = 0; bt < a.nbatch; bt++) {
       const TIN* inb = in + (int64_t)bt * a.in_bstride;
-      double* outb = out + (int64_t)bt * a.out_bstride;
+      TOUT* outb = out + (int64_t)bt * a.out_bstride;
       double acc = 0.0;
 #pragma unroll
       for (int t = 0; t < kMaxTerms; t++)
         if (ad[t] >= 0) acc += cf[t] * (double)inb[ad[t]];
       double v = a.alpha * acc;
-      if (a.beta != 0.0) v = a.beta * outb[o] + v;
-      outb[o] = v;
+      if (a.beta != 0.0) v = a.beta * (double)outb[o] + v;
+      outb[o] = (TOUT)v;
     }
   }
 }
 
-template <typename TIN>
-static hipError_t gather_map_t(const MapArgs& a, const TIN* in, double* out, hipStream_t s) {
+template <typename TIN, typename TOUT>
+static hipError_t gather_map_t(const MapArgs& a, const TIN* in, TOUT* out, hipStream_t s) {
   if (a.out.len == 0 || a.nbatch == 0) return hipSuccess;
   // one block per (column, chunk): the same count the kernel derives; a
   // launch of fewer than 1024 blocks takes one element per thread instead
@@ -158,19 +158,24 @@ static hipError_t gather_map_t(const MapArgs& a, const TIN* in, double* out, hip
   if (blocks > 0x7fffffff) {
     int64_t b2 = (a.out.len + 255) / 256;
     if (b2 > 8192) b2 = 8192;
-    hipLaunchKernelGGL(k_gather_map<TIN>, dim3((unsigned)b2, (unsigned)a.nbatch), dim3(256), 0, s, a, in, out);
+    hipLaunchKernelGGL((k_gather_map<TIN, TOUT>), dim3((unsigned)b2, (unsigned)a.nbatch), dim3(256), 0, s, a, in,
+                       out);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_gather_cols<TIN>, dim3((unsigned)blocks, 1), dim3(256), 0, s, a, in, out, chunk);
+  hipLaunchKernelGGL((k_gather_cols<TIN, TOUT>), dim3((unsigned)blocks, 1), dim3(256), 0, s, a, in, out, chunk);
   return hipGetLastError();
 }
 
 hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hipStream_t s) {
-  return gather_map_t<double>(a, in, out, s);
+  return gather_map_t<double, double>(a, in, out, s);
 }
 
 hipError_t launch_gather_map_f32in(const MapArgs& a, const float* in, double* out, hipStream_t s) {
-  return gather_map_t<float>(a, in, out, s);
+  return gather_map_t<float, double>(a, in, out, s);
+}
+
+hipError_t launch_gather_map_f32out(const MapArgs& a, const double* in, float* out, hipStream_t s) {
+  return gather_map_t<double, float>(a, in, out, s);
 }
 
 // ===========================================================================

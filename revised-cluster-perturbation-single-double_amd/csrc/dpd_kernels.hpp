@@ -29,6 +29,9 @@ struct MapArgs {
 hipError_t launch_gather_map(const MapArgs& a, const double* in, double* out, hipStream_t s);
 // the same from an FP32 source (scratch results of the FP32-accumulating GEMMs)
 hipError_t launch_gather_map_f32in(const MapArgs& a, const float* in, double* out, hipStream_t s);
+// the same sort writing FP32 (an expanded operand that only feeds an FP32 or
+// BF16x3 panel pack)
+hipError_t launch_gather_map_f32out(const MapArgs& a, const double* in, float* out, hipStream_t s);
 
 // ---- grouped DPD GEMM -------------------------------------------------------
 // Every task writes C(m,n) (m,n factorised into up to 3 mixed-radix digits with
@@ -129,6 +132,7 @@ struct PackArgs {
   int rfast;    // read the source with rows (1) or k's (0) varying fastest
   int64_t xlen, src_bstride;
   MapArgs map;
+  int src_f32;  // src holds floats (no fused sort)
 };
 hipError_t launch_pack(const PackArgs& p, hipStream_t s);
 // K step of a packed panel and its element count factor (BF16x3 keeps two planes)
