@@ -65,6 +65,21 @@ int ws_reserve(dpd_ctx* c, size_t bytes) {
   return c->ws.alloc(nb);
 }
 
+void drop_pack_entries(dpd_ctx* c, bool (*match)(uint64_t id, uint64_t key), uint64_t key) {
+  auto drop = [&](GemmPlan& g) {
+    for (auto& V : g.pk)
+      for (auto& side : V.cache) {
+        const size_t before = side.size();
+        side.erase(std::remove_if(side.begin(), side.end(),
+                                  [&](const std::unique_ptr<PackCache>& e) { return e->id && match(e->id, key); }),
+                   side.end());
+        if (side.size() != before) bump_mem_gen();
+      }
+  };
+  for (auto& kv : c->contract_cache) drop(kv.second->gemm);
+  for (auto& kv : c->engine_cache) drop(*kv.second);
+}
+
 void clear_pack_cache(GemmVariant& V) {
   bool any = false;
   for (auto& side : V.cache) {

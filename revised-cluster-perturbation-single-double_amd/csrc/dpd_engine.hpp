@@ -196,6 +196,9 @@ namespace dpd {
 int ws_reserve(dpd_ctx* c, size_t bytes);
 void* ws_take(dpd_ctx* c, size_t bytes);
 inline void ws_reset(dpd_ctx* c) { c->ws_top = 0; }
+// frees the packed panels of every cached plan whose static-operand id the
+// predicate accepts (a destroyed or refreshed problem's integrals)
+void drop_pack_entries(dpd_ctx* c, bool (*match)(uint64_t id, uint64_t key), uint64_t key);
 
 // engine level (libsrint vv2v_dpd_* semantics)
 int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, const Operand& o3,

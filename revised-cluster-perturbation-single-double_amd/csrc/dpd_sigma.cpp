@@ -48,6 +48,9 @@ static Operand op(const Desc& d, const char* lab, int rank, const double* data, 
   return o;
 }
 
+// static-operand ids are uid * 64 + tensor + 1: does id belong to problem uid?
+static bool id_of_problem(uint64_t id, uint64_t uid) { return (id - 1) / 64 == uid; }
+
 // a constant tensor of the problem: packed GEMM panels of it are built once
 static Operand konst(Operand o, const dpd_problem* p, int which) {
   o.static_id = p->uid * 64 + (uint64_t)which + 1;
@@ -618,6 +621,8 @@ int dpd_problem_create(dpd_ctx* c, const dpd_synth_spec* spec_in, int method, in
 int dpd_problem_destroy(dpd_problem* p) {
   if (!p) return DPD_OK;
   (void)hipStreamSynchronize(p->ctx->stream);
+  // its integral panels go with it (plans are shared by problems of one shape)
+  dpd::drop_pack_entries(p->ctx, id_of_problem, p->uid);
   delete p;
   return DPD_OK;
 }
@@ -773,6 +778,7 @@ int dpd_problem_refresh(dpd_problem* p) {
                              hipMemcpyDeviceToHost, st));
   DPD_HIP_TRY(hipStreamSynchronize(st));
   static std::atomic<uint64_t> next_gen{1ull << 40};
+  dpd::drop_pack_entries(p->ctx, id_of_problem, p->uid);
   p->uid = next_gen++;  // constant tensors changed: packed panels are rebuilt under new ids
   p->iabj32.reset();   // FP32 copies are remade on the next FP32 sigma
   p->ijka32.reset();
