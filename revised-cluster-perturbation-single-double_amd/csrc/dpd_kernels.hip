@@ -1032,57 +1032,77 @@ hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, c
 // C[j + k*nv] = P_j . B_k for nv <= kMgsMulti vectors P (stride n) and nb basis
 // vectors B (stride n); grid (kRedBlocks, ceil(nb/4)); each basis vector is read
 // once, partials reduced in a fixed order
-constexpr int kDotsKW = 4;  // basis vectors per block of k_dots_multi
-__global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __restrict__ P, int nv,
+// C[j + k*nv] = P_j . B_k for NV (<= kMgsMulti) vectors P (stride n) against KW
+// basis vectors B_k0.. per block (stride n); grid (kRedBlocks, ceil(nb/KW)).  Each
+// basis vector is read once; the P vectors are re-read by every block column,
+// so KW is as wide as the registers of NV accumulator rows allow.  Partials
+// are reduced in a fixed order.
+template <int NV, int KW>
+__global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __restrict__ P,
                                                     const double* __restrict__ B, int nb, double* partial,
                                                     const int* __restrict__ need) {
-  __shared__ double red[4][kMgsMulti * kDotsKW];
+  __shared__ double red[4][NV * KW];
   if (need && *need == 0) return;
-  const int k0 = blockIdx.y * kDotsKW;
-  const int nk = min(kDotsKW, nb - k0);
-  double acc[kMgsMulti][kDotsKW];
+  const int k0 = blockIdx.y * KW;
+  const int nk = min(KW, nb - k0);
+  double acc[NV][KW];
 #pragma unroll
-  for (int j = 0; j < kMgsMulti; j++)
+  for (int j = 0; j < NV; j++)
 #pragma unroll
-    for (int q = 0; q < kDotsKW; q++) acc[j][q] = 0.0;
+    for (int q = 0; q < KW; q++) acc[j][q] = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double b[kDotsKW];
+    double b[KW];
 #pragma unroll
-    for (int q = 0; q < kDotsKW; q++) b[q] = q < nk ? B[(int64_t)(k0 + q) * n + i] : 0.0;
+    for (int q = 0; q < KW; q++) b[q] = q < nk ? B[(int64_t)(k0 + q) * n + i] : 0.0;
 #pragma unroll
-    for (int j = 0; j < kMgsMulti; j++) {
-      if (j < nv) {
-        const double x = P[(int64_t)j * n + i];
+    for (int j = 0; j < NV; j++) {
+      const double x = P[(int64_t)j * n + i];
 #pragma unroll
-        for (int q = 0; q < kDotsKW; q++) acc[j][q] += x * b[q];
-      }
+      for (int q = 0; q < KW; q++) acc[j][q] += x * b[q];
     }
   }
   // every wave reduces its sums, then one barrier; wave sums added in a fixed order
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-  for (int j = 0; j < kMgsMulti; j++)
+  for (int j = 0; j < NV; j++)
 #pragma unroll
-    for (int q = 0; q < kDotsKW; q++)
-      if (j < nv && q < nk) {
+    for (int q = 0; q < KW; q++)
+      if (q < nk) {
         const double v = wave_sum(acc[j][q]);
-        if (lane == 0) red[wave][j * kDotsKW + q] = v;
+        if (lane == 0) red[wave][j * KW + q] = v;
       }
   __syncthreads();
-  for (int t = threadIdx.x; t < kMgsMulti * kDotsKW; t += blockDim.x) {
-    const int j = t / kDotsKW, q = t % kDotsKW;
-    if (j < nv && q < nk)
-      partial[((int64_t)j + (int64_t)(k0 + q) * nv) * kRedBlocks + blockIdx.x] =
+  for (int t = threadIdx.x; t < NV * KW; t += blockDim.x) {
+    const int j = t / KW, q = t % KW;
+    if (q < nk)
+      partial[((int64_t)j + (int64_t)(k0 + q) * NV) * kRedBlocks + blockIdx.x] =
           (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
   }
+}
+
+template <int NV>
+static void launch_dots_nv(int64_t n, const double* P, const double* B, int nb, double* partial, hipStream_t s,
+                           const int* need) {
+  constexpr int KW = NV <= 5 ? 8 : 4;
+  hipLaunchKernelGGL((k_dots_multi<NV, KW>), dim3(kRedBlocks, (nb + KW - 1) / KW), dim3(256), 0, s, n, P, B, nb,
+                     partial, need);
 }
 
 hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
                              double* out, hipStream_t s, const int* need) {
   if (nv <= 0 || nb <= 0) return hipSuccess;
   if (nv > kMgsMulti) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_dots_multi, dim3(kRedBlocks, (nb + kDotsKW - 1) / kDotsKW), dim3(256), 0, s, n, P, nv, B, nb,
-                     partial, need);
+  static_assert(kMgsMulti == 8, "one instantiation per vector count");
+  switch (nv) {
+    case 1: launch_dots_nv<1>(n, P, B, nb, partial, s, need); break;
+    case 2: launch_dots_nv<2>(n, P, B, nb, partial, s, need); break;
+    case 3: launch_dots_nv<3>(n, P, B, nb, partial, s, need); break;
+    case 4: launch_dots_nv<4>(n, P, B, nb, partial, s, need); break;
+    case 5: launch_dots_nv<5>(n, P, B, nb, partial, s, need); break;
+    case 6: launch_dots_nv<6>(n, P, B, nb, partial, s, need); break;
+    case 7: launch_dots_nv<7>(n, P, B, nb, partial, s, need); break;
+    default: launch_dots_nv<8>(n, P, B, nb, partial, s, need); break;
+  }
   hipLaunchKernelGGL(k_reduce_rows, dim3(nv * nb), dim3(256), 0, s, partial, kRedBlocks, out);
   return hipGetLastError();
 }
