@@ -615,16 +615,30 @@ __global__ void k_lincomb(int64_t n, const double* __restrict__ X, int64_t xs, i
                           const double* __restrict__ coef, int ldc, int nout, double* __restrict__ out) {
   const int j0 = blockIdx.y * kLinOut;
   const int nj = min(kLinOut, nout - j0);
+  // columns past nj repeat the last one (computed, never stored): no branches,
+  // so the loads of four basis vectors are in flight together
+  const double* cj[kLinOut];
+#pragma unroll
+  for (int j = 0; j < kLinOut; j++) cj[j] = coef + (int64_t)(j0 + min(j, nj - 1)) * ldc;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     double s[kLinOut];
 #pragma unroll
     for (int j = 0; j < kLinOut; j++) s[j] = 0.0;
-    for (int k = 0; k < nk; k++) {
+    int k = 0;
+    for (; k + 4 <= nk; k += 4) {
+      double x[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) x[u] = X[(int64_t)(k + u) * xs + i];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int j = 0; j < kLinOut; j++) s[j] += cj[j][k + u] * x[u];
+    }
+    for (; k < nk; k++) {
       const double x = X[(int64_t)k * xs + i];
 #pragma unroll
-      for (int j = 0; j < kLinOut; j++)
-        if (j < nj) s[j] += coef[k + (int64_t)(j0 + j) * ldc] * x;
+      for (int j = 0; j < kLinOut; j++) s[j] += cj[j][k] * x;
     }
 #pragma unroll
     for (int j = 0; j < kLinOut; j++)
@@ -1111,15 +1125,29 @@ hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B
 __global__ void k_lincomb_sub(int64_t n, const double* __restrict__ B, int nb, const double* __restrict__ C, int nv,
                               double* __restrict__ P, const int* __restrict__ need) {
   if (need && *need == 0) return;
+  // rows past nv repeat the last one (computed, never stored): no branches,
+  // so the loads of four basis vectors are in flight together
+  int jc[kMgsMulti];
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++) jc[j] = min(j, nv - 1);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double s[kMgsMulti];
 #pragma unroll
     for (int j = 0; j < kMgsMulti; j++) s[j] = 0.0;
-    for (int k = 0; k < nb; k++) {
+    int k = 0;
+    for (; k + 4 <= nb; k += 4) {
+      double b[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) b[u] = B[(int64_t)(k + u) * n + i];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int j = 0; j < kMgsMulti; j++) s[j] += C[jc[j] + (int64_t)(k + u) * nv] * b[u];
+    }
+    for (; k < nb; k++) {
       const double b = B[(int64_t)k * n + i];
 #pragma unroll
-      for (int j = 0; j < kMgsMulti; j++)
-        if (j < nv) s[j] += C[j + (int64_t)k * nv] * b;
+      for (int j = 0; j < kMgsMulti; j++) s[j] += C[jc[j] + (int64_t)k * nv] * b;
     }
 #pragma unroll
     for (int j = 0; j < kMgsMulti; j++)
