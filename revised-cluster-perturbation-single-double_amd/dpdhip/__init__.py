@@ -488,9 +488,11 @@ class Problem:
         return w, R, info
 
     def close(self):
-        if getattr(self, "h", None):
+        # a problem lives in its context: once the context was closed explicitly
+        # the library must not be asked to destroy it (its buffers stay until exit)
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             lib().dpd_problem_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
