@@ -406,12 +406,12 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
   if ((ta != 'N' && ta != 'T') || (tb != 'N' && tb != 'T')) return fail(DPD_EINVAL, "trans must be 'N' or 'T'");
   if (m < 0 || n < 0 || k < 0 || m > INT32_MAX || n > INT32_MAX || k > INT32_MAX)
     return fail(DPD_EINVAL, "bad GEMM sizes");
+  if (m == 0 || n == 0) return DPD_OK;  // empty C (BLAS quick return, before the leading-dimension rules)
   if (ldc < std::max<int64_t>(1, m) || lda < std::max<int64_t>(1, ta == 'N' ? m : k) ||
       ldb < std::max<int64_t>(1, tb == 'N' ? k : n))
     return fail(DPD_EINVAL, "bad leading dimension");
   if (prec != PREC_F64 && prec != PREC_F32 && prec != PREC_BF16X3)
     return fail(DPD_EINVAL, "precision %d is not available", prec);
-  if (m == 0 || n == 0) return DPD_OK;
   GemmPlan p;
   GemmTask tk;
   std::memset(&tk, 0, sizeof tk);

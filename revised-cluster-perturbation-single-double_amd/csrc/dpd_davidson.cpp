@@ -40,7 +40,7 @@ namespace dpd {
 // ---------------------------------------------------------------------------
 // small dense nonsymmetric eigenproblem (FP64, row-major n x n)
 // Householder reduction A = Q H Q^T to upper Hessenberg form (Q accumulated),
-// Francis double-shift QR for the eigenvalues, and for the requested columns
+// the Francis double-shift QR algorithm for the eigenvalues, and for the requested columns
 // inverse iteration on H (O(n^2) per solve) mapped back by Q; vectors have
 // unit 2-norm (as dgeev's).
 // ---------------------------------------------------------------------------
@@ -84,128 +84,154 @@ static void hessenberg(std::vector<double>& a, int n, std::vector<double>& q) {
   }
 }
 
-static inline double sgn(double a, double b) { return b >= 0.0 ? std::fabs(a) : -std::fabs(a); }
+// Householder vector of a 2- or 3-vector x (Golub & Van Loan, Matrix
+// Computations, Algorithm 5.1.1): v[0] = 1 and beta such that
+// (I - beta v v^T) x is a multiple of e_1; beta = 0 when x already is one.
+static void house_small(const double* x, int len, double* v, double& beta) {
+  double sigma = 0.0;
+  for (int i = 1; i < len; i++) sigma += x[i] * x[i];
+  v[0] = 1.0;
+  for (int i = 1; i < len; i++) v[i] = x[i];
+  if (sigma == 0.0) {
+    beta = 0.0;
+    return;
+  }
+  const double mu = std::sqrt(x[0] * x[0] + sigma);
+  const double v0 = x[0] <= 0.0 ? x[0] - mu : -sigma / (x[0] + mu);
+  beta = 2.0 * v0 * v0 / (sigma + v0 * v0);
+  for (int i = 1; i < len; i++) v[i] /= v0;
+}
 
-// Francis double-shift QR on an upper Hessenberg matrix (eigenvalues only)
-static int hqr(std::vector<double>& hm, int n, double* wr, double* wi) {
-  auto h = [&](int i, int j) -> double& { return hm[i * n + j]; };
-  double anorm = 0.0;
+// rows r0..r0+len-1 of H, columns c0..c1:  H <- (I - beta v v^T) H
+static void reflect_rows(std::vector<double>& H, int n, int r0, int len, int c0, int c1, const double* v,
+                         double beta) {
+  if (beta == 0.0) return;
+  for (int j = c0; j <= c1; j++) {
+    double s = 0.0;
+    for (int i = 0; i < len; i++) s += v[i] * H[(size_t)(r0 + i) * n + j];
+    s *= beta;
+    for (int i = 0; i < len; i++) H[(size_t)(r0 + i) * n + j] -= s * v[i];
+  }
+}
+
+// columns c0..c0+len-1 of H, rows r0..r1:  H <- H (I - beta v v^T)
+static void reflect_cols(std::vector<double>& H, int n, int c0, int len, int r0, int r1, const double* v,
+                         double beta) {
+  if (beta == 0.0) return;
+  for (int i = r0; i <= r1; i++) {
+    double* hi = &H[(size_t)i * n + c0];
+    double s = 0.0;
+    for (int j = 0; j < len; j++) s += hi[j] * v[j];
+    s *= beta;
+    for (int j = 0; j < len; j++) hi[j] -= s * v[j];
+  }
+}
+
+// One implicit double-shift (Francis) QR step on the unreduced window
+// lo..hi (hi - lo >= 2) of the upper Hessenberg matrix H, with the shift pair
+// given as the trace `tr` and determinant `det` of its 2x2 generator (Golub &
+// Van Loan, Algorithm 7.5.1).  Only the window is transformed: the eigenvalues
+// are all that is wanted, and the window's are independent of the rest of H
+// once the subdiagonal entries around it are zero.
+static void francis_step(std::vector<double>& H, int n, int lo, int hi, double tr, double det) {
+  auto h = [&](int i, int j) -> double& { return H[(size_t)i * n + j]; };
+  // first column of (H - s1 I)(H - s2 I), which has three nonzeros
+  double x[3];
+  x[0] = h(lo, lo) * h(lo, lo) + h(lo, lo + 1) * h(lo + 1, lo) - tr * h(lo, lo) + det;
+  x[1] = h(lo + 1, lo) * (h(lo, lo) + h(lo + 1, lo + 1) - tr);
+  x[2] = h(lo + 1, lo) * h(lo + 2, lo + 1);
+  double v[3], beta;
+  for (int k = lo; k <= hi - 2; k++) {
+    house_small(x, 3, v, beta);
+    reflect_rows(H, n, k, 3, std::max(lo, k - 1), hi, v, beta);
+    reflect_cols(H, n, k, 3, lo, std::min(k + 3, hi), v, beta);
+    if (k > lo) h(k + 1, k - 1) = h(k + 2, k - 1) = 0.0;  // the bulge has moved on
+    x[0] = h(k + 1, k);
+    x[1] = h(k + 2, k);
+    x[2] = k < hi - 2 ? h(k + 3, k) : 0.0;
+  }
+  house_small(x, 2, v, beta);
+  reflect_rows(H, n, hi - 1, 2, hi - 2, hi, v, beta);
+  reflect_cols(H, n, hi - 1, 2, lo, hi, v, beta);
+  h(hi, hi - 2) = 0.0;
+}
+
+// eigenvalues of the 2x2 block [[a, b], [c, d]]: with p = (a - d)/2 they are
+// d + mu for the roots mu of mu^2 - 2 p mu - b c = 0; the root of larger size
+// is formed without cancellation and the other from the product -b c.
+static void eig2x2(double a, double b, double c, double d, double* re, double* im) {
+  const double p = 0.5 * (a - d);
+  const double disc = p * p + b * c;
+  if (disc >= 0.0) {
+    const double t = p + std::copysign(std::sqrt(disc), p);
+    re[0] = d + t;
+    re[1] = t != 0.0 ? d - b * c / t : d;
+    im[0] = im[1] = 0.0;
+  } else {
+    re[0] = re[1] = d + p;
+    im[0] = std::sqrt(-disc);
+    im[1] = -im[0];
+  }
+}
+
+// Eigenvalues (wr, wi) of an upper Hessenberg matrix by the Francis QR
+// algorithm (Golub & Van Loan, Algorithm 7.5.2): the active window is the
+// trailing unreduced block; a subdiagonal entry below eps times its two
+// diagonal neighbours is set to zero, a 1x1 or 2x2 trailing block is read off
+// and removed.  The standard shifts are the eigenvalues of the window's
+// trailing 2x2 block; after every 10 steps without a deflation the ad hoc
+// shift pair of EISPACK's hqr (public domain) is used once: mu +- i nu with
+// mu = h(hi,hi) + 0.75 s and nu^2 = 0.4375 s^2, s = |h(hi,hi-1)| + |h(hi-1,hi-2)|.
+static int francis_eigenvalues(std::vector<double>& H, int n, double* wr, double* wi) {
+  auto h = [&](int i, int j) -> double& { return H[(size_t)i * n + j]; };
+  const double eps = 2.220446049250313e-16;
+  double hnorm = 0.0;
   for (int i = 0; i < n; i++)
-    for (int j = std::max(i - 1, 0); j < n; j++) anorm += std::fabs(h(i, j));
-  int nn = n - 1;
-  double t = 0.0, p = 0, q = 0, r = 0, s, w, x, y, z;
-  while (nn >= 0) {
-    int its = 0, l;
-    do {
-      for (l = nn; l >= 1; l--) {
-        s = std::fabs(h(l - 1, l - 1)) + std::fabs(h(l, l));
-        if (s == 0.0) s = anorm;
-        if (std::fabs(h(l, l - 1)) + s == s) {
-          h(l, l - 1) = 0.0;
-          break;
-        }
+    for (int j = std::max(i - 1, 0); j < n; j++) hnorm = std::max(hnorm, std::fabs(h(i, j)));
+  if (hnorm == 0.0) hnorm = 1.0;
+  int hi = n - 1, steps = 0;
+  const int max_steps = 30 * std::max(n, 1);
+  int since_deflation = 0;
+  while (hi >= 0) {
+    int lo = hi;
+    for (; lo > 0; lo--) {
+      double s = std::fabs(h(lo - 1, lo - 1)) + std::fabs(h(lo, lo));
+      if (s == 0.0) s = hnorm;
+      if (std::fabs(h(lo, lo - 1)) <= eps * s) {
+        h(lo, lo - 1) = 0.0;
+        break;
       }
-      x = h(nn, nn);
-      if (l == nn) {
-        wr[nn] = x + t;
-        wi[nn--] = 0.0;
-      } else {
-        y = h(nn - 1, nn - 1);
-        w = h(nn, nn - 1) * h(nn - 1, nn);
-        if (l == nn - 1) {
-          p = 0.5 * (y - x);
-          q = p * p + w;
-          z = std::sqrt(std::fabs(q));
-          x += t;
-          if (q >= 0.0) {
-            z = p + sgn(z, p);
-            wr[nn - 1] = wr[nn] = x + z;
-            if (z != 0.0) wr[nn] = x - w / z;
-            wi[nn - 1] = wi[nn] = 0.0;
-          } else {
-            wr[nn - 1] = wr[nn] = x + p;
-            wi[nn - 1] = -(wi[nn] = z);
-          }
-          nn -= 2;
-        } else {
-          if (its == 60) return -1;
-          if (its == 10 || its == 20 || its == 40) {
-            t += x;
-            for (int i = 0; i <= nn; i++) h(i, i) -= x;
-            s = std::fabs(h(nn, nn - 1)) + std::fabs(h(nn - 1, nn - 2));
-            y = x = 0.75 * s;
-            w = -0.4375 * s * s;
-          }
-          ++its;
-          int m;
-          for (m = nn - 2; m >= l; m--) {
-            z = h(m, m);
-            r = x - z;
-            s = y - z;
-            p = (r * s - w) / h(m + 1, m) + h(m, m + 1);
-            q = h(m + 1, m + 1) - z - r - s;
-            r = h(m + 2, m + 1);
-            s = std::fabs(p) + std::fabs(q) + std::fabs(r);
-            p /= s;
-            q /= s;
-            r /= s;
-            if (m == l) break;
-            double u = std::fabs(h(m, m - 1)) * (std::fabs(q) + std::fabs(r));
-            double v = std::fabs(p) * (std::fabs(h(m - 1, m - 1)) + std::fabs(z) + std::fabs(h(m + 1, m + 1)));
-            if (u + v == v) break;
-          }
-          for (int i = m + 2; i <= nn; i++) {
-            h(i, i - 2) = 0.0;
-            if (i != m + 2) h(i, i - 3) = 0.0;
-          }
-          for (int k = m; k <= nn - 1; k++) {
-            if (k != m) {
-              p = h(k, k - 1);
-              q = h(k + 1, k - 1);
-              r = 0.0;
-              if (k != nn - 1) r = h(k + 2, k - 1);
-              if ((x = std::fabs(p) + std::fabs(q) + std::fabs(r)) != 0.0) {
-                p /= x;
-                q /= x;
-                r /= x;
-              }
-            }
-            if ((s = sgn(std::sqrt(p * p + q * q + r * r), p)) != 0.0) {
-              if (k == m) {
-                if (l != m) h(k, k - 1) = -h(k, k - 1);
-              } else {
-                h(k, k - 1) = -s * x;
-              }
-              p += s;
-              x = p / s;
-              y = q / s;
-              z = r / s;
-              q /= p;
-              r /= p;
-              for (int j = k; j <= nn; j++) {
-                p = h(k, j) + q * h(k + 1, j);
-                if (k != nn - 1) {
-                  p += r * h(k + 2, j);
-                  h(k + 2, j) -= p * z;
-                }
-                h(k + 1, j) -= p * y;
-                h(k, j) -= p * x;
-              }
-              int mmin = nn < k + 3 ? nn : k + 3;
-              for (int i = l; i <= mmin; i++) {
-                p = x * h(i, k) + y * h(i, k + 1);
-                if (k != nn - 1) {
-                  p += z * h(i, k + 2);
-                  h(i, k + 2) -= p * r;
-                }
-                h(i, k + 1) -= p * q;
-                h(i, k) -= p;
-              }
-            }
-          }
-        }
-      }
-    } while (l < nn - 1);
+    }
+    if (lo == hi) {
+      wr[hi] = h(hi, hi);
+      wi[hi] = 0.0;
+      hi -= 1;
+      since_deflation = 0;
+      continue;
+    }
+    if (lo == hi - 1) {
+      double re[2], im[2];
+      eig2x2(h(hi - 1, hi - 1), h(hi - 1, hi), h(hi, hi - 1), h(hi, hi), re, im);
+      wr[hi - 1] = re[0];
+      wi[hi - 1] = im[0];
+      wr[hi] = re[1];
+      wi[hi] = im[1];
+      hi -= 2;
+      since_deflation = 0;
+      continue;
+    }
+    if (++steps > max_steps) return -1;
+    double tr, det;
+    if (++since_deflation % 10 == 0) {
+      const double s = std::fabs(h(hi, hi - 1)) + std::fabs(h(hi - 1, hi - 2));
+      const double mu = h(hi, hi) + 0.75 * s;
+      tr = 2.0 * mu;
+      det = mu * mu + 0.4375 * s * s;
+    } else {
+      tr = h(hi - 1, hi - 1) + h(hi, hi);
+      det = h(hi - 1, hi - 1) * h(hi, hi) - h(hi - 1, hi) * h(hi, hi - 1);
+    }
+    francis_step(H, n, lo, hi, tr, det);
   }
   return 0;
 }
@@ -252,7 +278,7 @@ struct SmallEig {
     H.assign(Arow, Arow + (size_t)n * n);
     hessenberg(H, n, Q);
     T = H;
-    if (hqr(T, n, wr_, wi_) != 0) return fail(DPD_EINVAL, "subspace eigensolver did not converge");
+    if (francis_eigenvalues(T, n, wr_, wi_) != 0) return fail(DPD_EINVAL, "subspace eigensolver did not converge");
     anorm = 0.0;
     for (int i = 0; i < n * n; i++) anorm = std::max(anorm, std::fabs(Arow[i]));
     if (anorm == 0.0) anorm = 1.0;

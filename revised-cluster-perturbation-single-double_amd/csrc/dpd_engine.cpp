@@ -860,13 +860,15 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
         continue;
       }
       DevBuf tp;
-      if (sk == 0 && ((T.pa_elems && (rc = ta.alloc(es * T.pa_elems))) ||
-                      (T.pb_elems && (rc = tb.alloc(es * T.pb_elems)))))
+      // the panels do not depend on the split mode: packed by whichever pass runs first
+      if ((T.pa_elems && !ta.p && ((rc = ta.alloc(es * T.pa_elems)) || (rc = pack_side(c, p, T, 0, A, ta.p)))) ||
+          (T.pb_elems && !tb.p && ((rc = tb.alloc(es * T.pb_elems)) || (rc = pack_side(c, p, T, 1, B, tb.p)))))
         break;
+      if ((T.pa_elems && !ta.p) || (T.pb_elems && !tb.p)) {
+        rc = fail(DPD_EINVAL, "autotuner: operand panel missing (internal)");
+        break;
+      }
       if (T.partial_elems && (rc = tp.alloc(sizeof(double) * T.partial_elems))) break;
-      if (sk == 0 && ((T.pa_elems && (rc = pack_side(c, p, T, 0, A, ta.p))) ||
-                      (T.pb_elems && (rc = pack_side(c, p, T, 1, B, tb.p)))))
-        break;
       GemmLaunch g;
       std::memset(&g, 0, sizeof g);
       g.C = (double*)cs.p;
@@ -1643,7 +1645,7 @@ static int group_variants(GroupPlan& G, int prec, int cfg) {
 // merged task list of variants (one per member, all built with one tile shape);
 // split-K is decided again over the whole group (the same rule as for one
 // plan: split long-K tasks while the launch has fewer than 1024 tiles)
-static int group_tasks(const std::vector<const GemmVariant*>& Vs, int mode, GroupTasks& G) {
+static int group_tasks(const std::vector<const GemmVariant*>& Vs, int mode, bool allow_split, GroupTasks& G) {
   if (g_capturing) return fail(DPD_EINVAL, "grouped task list built during graph capture (internal)");
   bump_mem_gen();  // the group's task tables are reallocated
   std::vector<GemmTask> all;
@@ -1666,7 +1668,8 @@ static int group_tasks(const std::vector<const GemmVariant*>& Vs, int mode, Grou
   for (size_t i = 0; i < all.size(); i++) {
     GemmTask& tk = all[i];
     int ks = 1;
-    if (tiles < target && tk.steps > per_wg) ks = (int)std::min<int64_t>(64, (tk.steps + per_wg - 1) / per_wg);
+    if (allow_split && tiles < target && tk.steps > per_wg)
+      ks = (int)std::min<int64_t>(64, (tk.steps + per_wg - 1) / per_wg);
     tk.ksplit = ks;
     tk.part_off = 0;
     if (ks > 1) {
@@ -1754,7 +1757,7 @@ static int tune_group(dpd_ctx* c, GroupPlan& G, int prec, const std::vector<Call
     }
     if (rc) break;
     GroupTasks TT;
-    if ((rc = group_tasks(Vs, prec, TT))) break;
+    if ((rc = group_tasks(Vs, prec, c->split_k != 0, TT))) break;
     if (TT.nwg == 0) continue;
     DevBuf part;
     if (TT.partial_elems && (rc = part.alloc(sizeof(double) * TT.partial_elems))) break;
@@ -1855,7 +1858,7 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
     if (cap) return fail(DPD_EINVAL, "grouped task list built during graph capture (internal)");
     std::vector<const GemmVariant*> Vs;
     for (ContractPlan* P : plans) Vs.push_back(&P->gemm.pk[prec]);
-    if ((rc = group_tasks(Vs, prec, G->gt))) return rc;
+    if ((rc = group_tasks(Vs, prec, c->split_k != 0, G->gt))) return rc;
     G->built_cfg = G->cfg;
   }
   std::vector<CallPrep> R(n);
@@ -1896,7 +1899,7 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
       if ((rc = group_variants(*G, prec, G->cfg))) return rc;
       std::vector<const GemmVariant*> Vs;
       for (ContractPlan* P : plans) Vs.push_back(&P->gemm.pk[prec]);
-      if ((rc = group_tasks(Vs, prec, G->gt))) return rc;
+      if ((rc = group_tasks(Vs, prec, c->split_k != 0, G->gt))) return rc;
       G->built_cfg = G->cfg;
     }
   }

@@ -17,6 +17,7 @@
 // error prints the message and stops the program, as the reference's
 // `write(6,*) ...; stop` does, but with exit status 1.  FP64 data only (the
 // reference's prec = 8).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -208,12 +209,16 @@ void xgemmso_(const char* transa, const char* transb, const int64_t* m, const in
   need_ctx("xgemmso");
   const bool ta = *transa == 'T' || *transa == 't' || *transa == 'C' || *transa == 'c';
   const bool tb = *transb == 'T' || *transb == 't' || *transb == 'C' || *transb == 'c';
-  const int64_t na = *lda * (ta ? *m : *k), nb = *ldb * (tb ? *k : *n), nc = *ldc * *n;
+  // the reference wraps every leading dimension as MAX(1,LD) (xgemmso.f:11-22):
+  // its callers pass block sizes that are 0 for empty irrep blocks
+  const int64_t la = std::max<int64_t>(1, *lda), lb = std::max<int64_t>(1, *ldb), lc = std::max<int64_t>(1, *ldc);
+  if (*m <= 0 || *n <= 0) return;  // empty C: nothing is read or written (DGEMM's quick return)
+  const int64_t na = la * (ta ? *m : *k), nb = lb * (tb ? *k : *n), nc = lc * *n;
   const double* dA = view(0, A, na, true, "xgemmso");
   const double* dB = view(1, B, nb, true, "xgemmso");
   double* dC = view(2, C, nc, true, "xgemmso");
   const char sa[2] = {ta ? 'T' : 'N', 0}, sb[2] = {tb ? 'T' : 'N', 0};
-  if (dpd_xgemm(g_ctx, sa, sb, *m, *n, *k, *alpha, dA, *lda, dB, *ldb, *beta, dC, *ldc, 0)) stop_with("xgemmso");
+  if (dpd_xgemm(g_ctx, sa, sb, *m, *n, *k, *alpha, dA, la, dB, lb, *beta, dC, lc, 0)) stop_with("xgemmso");
   finish(C, dC, nc, "xgemmso");
 }
 

@@ -331,12 +331,32 @@ METHODS = {"sf-cis": 0, "cisdoo": 1, "sf-cpsd": 2}
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
+def _check_vectors(x, what, dim, device, nvec=None):
+    """x must be a contiguous float64 tensor on cuda:<device> holding a whole
+    number (nvec, when given) of dim-long vectors; returns the vector count."""
+    torch = _torch()
+    if not isinstance(x, torch.Tensor) or x.dtype != torch.float64 or not x.is_cuda \
+            or x.device.index != device or not x.is_contiguous():
+        raise ValueError(f"{what}: expected a contiguous float64 tensor on cuda:{device}")
+    n = x.numel()
+    if dim <= 0 or n == 0 or n % dim:
+        raise ValueError(f"{what}: {n} elements is not a whole number of vectors of length {dim}")
+    if nvec is not None and n != nvec * dim:
+        raise ValueError(f"{what}: expected {nvec} vector(s) of length {dim}, got {n} elements")
+    return n // dim
+
+
 def sigma_group(problems, Rs, outs=None):
     """sigma of several problems (same context, method, precision) with one
     grouped GEMM launch per contraction"""
     torch = _torch()
     n = len(problems)
+    if len(Rs) != n or (outs is not None and len(outs) != n):
+        raise ValueError("one R (and one out) per problem")
+    nv = [p._vectors(R, f"Rs[{i}]") for i, (p, R) in enumerate(zip(problems, Rs))]
     outs = outs if outs is not None else [torch.empty_like(R) for R in Rs]
+    for i, (p, o) in enumerate(zip(problems, outs)):
+        p._vectors(o, f"outs[{i}]", nv[i])
     P = (ctypes.c_void_p * n)(*[p.h.value for p in problems])
     X = (ctypes.c_void_p * n)(*[R.data_ptr() for R in Rs])
     Y = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
@@ -352,6 +372,10 @@ def davidson_batch(problems, nroot, conv=1e-8, max_dim=20, max_cyc=50, lowest=Fa
     n = len(problems)
     if Rs is None:
         Rs = [p.guess(nroot)[0] for p in problems]
+    if len(Rs) != n:
+        raise ValueError("one R per problem")
+    for i, (p, R) in enumerate(zip(problems, Rs)):
+        p._vectors(R, f"Rs[{i}]", nroot)
     opts = (_lib.DavidsonOpts_ * n)()
     for o in opts:
         o.nroot, o.max_dim, o.max_cyc, o.conv = nroot, max_dim, max_cyc, conv
@@ -390,25 +414,34 @@ class Problem:
         self.h = h
         self.dim = int(lib().dpd_problem_dim(h))
         self.precision = "fp64"
+        self.version = 0  # moves on with every set() and refresh()
 
     def set_precision(self, prec):
         check(lib().dpd_problem_set_precision(self.h, PRECISIONS[prec]))
         self.precision = prec
 
+    def _vectors(self, x, what, nvec=None):
+        """Number of vectors in x, which the library reads or writes as nvec * dim
+        contiguous doubles on this problem's device; anything else is refused
+        here, before a device pointer is handed over."""
+        return _check_vectors(x, what, self.dim, self.ctx.device, nvec)
+
     def sigma(self, R, out=None):
         torch = _torch()
-        nvec = R.numel() // self.dim
+        nvec = self._vectors(R, "R")
         if out is None:
             out = torch.empty_like(R)
+        self._vectors(out, "out", nvec)
         check(lib().dpd_problem_sigma(self.h, R.data_ptr(), out.data_ptr(), nvec))
         return out
 
     def sigma_t(self, L, out=None):
         """t = J^T L (the transposed operator, same terms with operand roles exchanged)."""
         torch = _torch()
-        nvec = L.numel() // self.dim
+        nvec = self._vectors(L, "L")
         if out is None:
             out = torch.empty_like(L)
+        self._vectors(out, "out", nvec)
         check(lib().dpd_problem_sigma_t(self.h, L.data_ptr(), out.data_ptr(), nvec))
         return out
 
@@ -432,14 +465,16 @@ class Problem:
     def set(self, name, x):
         """Overwrite an input with a device tensor of its length; call refresh() after."""
         n = self.tensor_len(name)
-        if x.numel() != n or x.dtype != _torch().float64 or not x.is_cuda:
-            raise ValueError(f"{name}: expected a float64 device tensor of {n} elements")
+        if x.numel() != n or x.dtype != _torch().float64 or not x.is_cuda or x.device.index != self.ctx.device:
+            raise ValueError(f"{name}: expected a float64 tensor of {n} elements on cuda:{self.ctx.device}")
         x = x.contiguous()
         check(lib().dpd_problem_set(self.h, name.encode(), x.data_ptr()))
         self.ctx.synchronize()
+        self.version += 1
 
     def refresh(self):
         check(lib().dpd_problem_refresh(self.h))
+        self.version += 1
 
     def grad(self, L, R, scale=1.0, out=None):
         """scale * d(L . J R)/d(inputs), accumulated into out (dict name -> tensor,
@@ -451,6 +486,11 @@ class Problem:
                 n = self.tensor_len(k)
                 if n:
                     out[k] = torch.zeros(n, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        self._vectors(L, "L", 1)
+        self._vectors(R, "R", 1)
+        for k in self.INPUTS:
+            if k in out:
+                _check_vectors(out[k], f"out[{k!r}]", self.tensor_len(k), self.ctx.device, 1)
         ptr = [out[k].data_ptr() if k in out else None for k in self.INPUTS]
         check(lib().dpd_problem_grad(self.h, L.data_ptr(), R.data_ptr(), float(scale), *ptr))
         self.ctx.synchronize()
@@ -475,6 +515,7 @@ class Problem:
         solves with J^T (left eigenvectors)."""
         if R is None:
             R, _ = self.guess(nroot)
+        self._vectors(R, "R", nroot)
         o = _lib.DavidsonOpts_()
         o.nroot, o.max_dim, o.max_cyc, o.conv = nroot, max_dim, max_cyc, conv
         o.lowest, o.final_check, o.transpose = int(lowest), int(final_check), int(transpose)

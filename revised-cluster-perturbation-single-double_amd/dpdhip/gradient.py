@@ -73,7 +73,11 @@ class ExcitationEnergy(torch.autograd.Function):
         w = torch.dot(Lv, problem.sigma(Rv)) / lr
         ctx.problem = problem
         ctx.names = names
-        ctx.save_for_backward(Lv, Rv)
+        # another forward on the same problem may overwrite its inputs before
+        # this backward runs: the inputs are kept and the problem's version
+        # recorded, so that backward evaluates the gradient at this theta
+        ctx.version = problem.version
+        ctx.save_for_backward(Lv, Rv, *tensors)
         ctx.info = {"right": ir, "left": il, "w_right": float(wr[root]), "w_left": float(wl[k]), "R": Rv, "L": Lv}
         if solve.get("info") is not None:
             solve["info"].update(ctx.info)
@@ -81,11 +85,16 @@ class ExcitationEnergy(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gw):
-        Lv, Rv = ctx.saved_tensors
+        Lv, Rv, *tensors = ctx.saved_tensors
         lr = float(torch.dot(Lv, Rv))
         if abs(lr) < 1e-12:
             raise RuntimeError("left and right eigenvectors are orthogonal: the eigenvalue is not simple")
-        g = ctx.problem.grad(Lv, Rv, scale=1.0 / lr)
+        problem = ctx.problem
+        if problem.version != ctx.version:
+            for n, t in zip(ctx.names, tensors):
+                problem.set(n, t.detach())
+            problem.refresh()
+        g = problem.grad(Lv, Rv, scale=1.0 / lr)
         out = [gw * g[n] if n in g else None for n in ctx.names]
         return (None, None, None, *out)
 

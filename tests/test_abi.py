@@ -257,7 +257,23 @@ def test_subspace_eigensolver_matches_lapack():
                     v = V[:, k]
                     assert abs(np.linalg.norm(v) - 1.0) < 1e-12
                     assert np.linalg.norm(A @ v - wr[k] * v) < 1e-10
-    # a general random matrix with complex pairs: eigenvalues still agree
-    A = rng.standard_normal((12, 12))
+    # general random matrices with complex pairs: eigenvalues still agree
+    for n in (3, 4, 7, 12, 25, 40):
+        A = rng.standard_normal((n, n))
+        wr, wi, _ = small_eig(A)
+        assert np.allclose(np.sort_complex(wr + 1j * wi), np.sort_complex(np.linalg.eigvals(A)), atol=1e-10)
+    # the cyclic shift (eigenvalues: the n-th roots of unity) stalls the standard
+    # double shift on its first steps; the exceptional shifts must get it moving
+    for n in (3, 4, 6, 9):
+        A = np.roll(np.eye(n), 1, axis=0)
+        wr, wi, _ = small_eig(A)
+        ref = np.exp(2j * np.pi * np.arange(n) / n)
+        assert np.allclose(np.sort_complex(wr + 1j * wi), np.sort_complex(ref), atol=1e-10)
+    # symmetric (real spectrum, eigenvectors orthogonal) and a repeated eigenvalue
+    B = rng.standard_normal((10, 10))
+    A = B + B.T
+    wr, wi, V = small_eig(A)
+    assert np.all(wi == 0.0) and np.allclose(np.sort(wr), np.linalg.eigvalsh(A), atol=1e-11)
+    A = np.diag([1.0, 1.0, 2.0, 3.0]) + np.triu(0.1 * np.ones((4, 4)), 2)
     wr, wi, _ = small_eig(A)
-    assert np.allclose(np.sort_complex(wr + 1j * wi), np.sort_complex(np.linalg.eigvals(A)), atol=1e-10)
+    assert np.allclose(np.sort(wr), [1.0, 1.0, 2.0, 3.0], atol=1e-12) and np.all(wi == 0.0)

@@ -208,3 +208,33 @@ def test_fortran_xgemmso(torch, fsym, where):
                    a.i(B.shape[0]), a.d(2.0), hc[1], a.i(m), ctypes.c_size_t(1), ctypes.c_size_t(1))
         got = _back(hc[0]).reshape(n, m).T
         np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_fortran_xgemmso_empty_blocks(torch, fsym, where):
+    """Callers loop over every irrep with leading dimensions equal to block sizes
+    that can be 0 (e.g. VpqmnVrsmn_to_Vpqrs.f:93 passes LDA=INUMPQ(IRREPPQ));
+    the reference wraps each one as MAX(1,LD) (xgemmso.f:11-22).  m = 0 or
+    n = 0 leaves C alone; k = 0 gives C = beta*C."""
+    from dpdhip import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(3)
+    one = ctypes.c_size_t(1)
+    # m = 0, ldc = lda = 0: nothing is read or written
+    C = rng.standard_normal(6)
+    a = Args()
+    hc = _ptr(torch, C.copy(), where)
+    hx = _ptr(torch, rng.standard_normal(4), where)
+    L.xgemmso_(a.s("N"), a.s("N"), a.i(0), a.i(3), a.i(2), a.d(1.0), hx[1], a.i(0), hx[1], a.i(2), a.d(0.5),
+               hc[1], a.i(0), one, one)
+    np.testing.assert_array_equal(_back(hc[0]), C)
+    # n = 0, ldb = 0
+    a = Args()
+    L.xgemmso_(a.s("N"), a.s("T"), a.i(3), a.i(0), a.i(2), a.d(1.0), hx[1], a.i(3), hx[1], a.i(0), a.d(0.5),
+               hc[1], a.i(3), one, one)
+    np.testing.assert_array_equal(_back(hc[0]), C)
+    # k = 0 with lda = 3, ldb = 0: C = beta * C (3 x 2)
+    a = Args()
+    L.xgemmso_(a.s("N"), a.s("N"), a.i(3), a.i(2), a.i(0), a.d(1.0), hx[1], a.i(3), hx[1], a.i(0), a.d(0.5),
+               hc[1], a.i(3), one, one)
+    np.testing.assert_allclose(_back(hc[0]), 0.5 * C, rtol=0, atol=1e-15)
