@@ -57,15 +57,21 @@ class OracleProblem:
     def opt(self, name, arr=None):
         return O.Opt(self.d[name], arr if arr is not None else self.ints[name])
 
+    def _mp2_amplitudes(self):
+        """MP2t(ab,ij) = -<ab||ij>/e(ab,ij) (denomT.f90:94-249, called at
+        CISdoo.SD.SF/run.f90:32), in the layout of abij"""
+        v, o = self.popv, self.popo
+        dT = self.d["abij"]
+        t = np.zeros(max(dT.length(), 1))
+        O.lib().orc_denomT2(O._dp(t), O._dp(self.ints["abij"]), O._dp(self.eorb), O._lp(v)[1], O._lp(o)[1],
+                            O.i64(1), O.i64(0))
+        return t[: dT.length()]
+
     def _intermediates(self):
         """CISdoo.SD.SF/run.f90:31-49"""
         v, o = self.popv, self.popo
         lib = O.lib()
-        dT = self.d["abij"]
-        t = np.zeros(max(dT.length(), 1))
-        lib.orc_denomT2(O._dp(t), O._dp(self.ints["abij"]), O._dp(self.eorb), O._lp(v)[1], O._lp(o)[1], O.i64(1),
-                        O.i64(0))
-        self.mp2t = O.Opt(dT, t[: dT.length()])
+        self.mp2t = O.Opt(self.d["abij"], self._mp2_amplitudes())
         abij = self.opt("abij")
         # Fac = f_vv + Fae_r2abij(MP2t, abij, 1.0)
         fac = np.zeros(max(self.d["fab"].length(), 1))
