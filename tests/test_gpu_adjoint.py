@@ -189,6 +189,33 @@ def test_set_requires_refresh(torch):
     assert float((y0 - y1).abs().max()) <= 1e-13 * float(y0.abs().max())
 
 
+def test_interleaved_forwards_one_backward(torch):
+    """Two forwards on the same problem with different inputs, then backward
+    of the first: the gradient is evaluated at the first forward's inputs (the
+    problem is re-set and refreshed), equal to a lone forward + backward."""
+    import dpdhip
+    from dpdhip.gradient import excitation_energy
+    from test_gpu_sigma import ctx
+    cfg = SMALL
+    gp = dpdhip.Problem(ctx(len(cfg["nso"])), dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"]),
+                        "sf-cpsd", 1)
+    print(f"[adjoint] interleaved forwards dim={gp.dim}", flush=True)
+    names = INPUTS["sf-cpsd"]
+    base = {k: gp.get(k) for k in names}
+    solve = dict(nroot=2, conv=1e-10, max_dim=24, max_cyc=200)
+    x = {k: v.clone().requires_grad_(True) for k, v in base.items()}
+    excitation_energy(gp, x, **solve).backward()
+    ref = {k: x[k].grad.clone() for k in names}
+    x1 = {k: v.clone().requires_grad_(True) for k, v in base.items()}
+    w1 = excitation_energy(gp, x1, **solve)
+    other = {k: (v * 1.05).requires_grad_(True) for k, v in base.items()}
+    excitation_energy(gp, other, **solve)
+    w1.backward()
+    for k in names:
+        err = float((x1[k].grad - ref[k]).abs().max())
+        assert err <= 1e-8 * max(1.0, float(ref[k].abs().max())), (k, err)
+
+
 @pytest.mark.timeout(300)
 def test_gradient_c2v_100_orbitals_50_directions(torch):
     """The gradient workload at full size (C2v, 100 spatial orbitals [40,10,20,30],
