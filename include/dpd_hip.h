@@ -183,6 +183,22 @@ int dpd_xgemm(dpd_ctx *ctx, const char *transa, const char *transb, int64_t m, i
               int64_t k, double alpha, const double *A, int64_t lda, const double *B,
               int64_t ldb, double beta, double *C, int64_t ldc, int prec);
 
+/* No counterpart in the reference.  dpd_xgemm for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the same arguments and result, but no hipMalloc,
+ * hipFree or stream synchronisation.  The task tables and the packed operand
+ * panels (BF16x3, or packing 2) live in one static 64 MiB device array per GPU,
+ * held until the process ends; a call whose panels do not fit returns
+ * DPD_EINVAL (FP64/FP32 on the gather kernel need only the tables and take any
+ * size).  Long-K tiles are not split over workgroups, so results may differ from
+ * dpd_xgemm's in the last bits.  Every call on one GPU uses that GPU's array:
+ * calls on one stream are ordered, but calls that overlap on one GPU (other
+ * streams, other threads or contexts) race. */
+int dpd_xgemm_noalloc(dpd_ctx *ctx, const char *transa, const char *transb, int64_t m,
+                      int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                      const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
+                      int prec);
+
 /* ---- Davidson vector primitives (device vectors, FP64) ----------------- */
 /* result written to a host double (synchronises the stream) */
 int dpd_dot(dpd_ctx *ctx, int64_t n, const double *x, const double *y, double *result);

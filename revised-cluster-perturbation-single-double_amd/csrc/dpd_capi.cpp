@@ -4,6 +4,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "dpd_engine.hpp"
@@ -21,6 +22,48 @@ static int set_device(dpd_ctx* c) {
   if (c->device < 0) return fail(DPD_ENODEV, "host-only context (device -1) cannot run device work");
   DPD_HIP_TRY(hipSetDevice(c->device));
   return DPD_OK;
+}
+
+// ---- scratch of the allocation-free GEMM (dpd_xgemm_noalloc) -----------------
+// One file-scope device array per GPU, held until the process ends: the task
+// and segment tables of the call at its start, the packed operand panels after
+// them.  In host code `scratch` names a host-side stand-in, not device memory:
+// its device address is looked up once per device (hipGetSymbolAddress, with
+// that device current, when the first context on it is created) and only that
+// address is handed to kernels.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchTableBytes = 4096;  // GemmTask at 0, GemmSeg at 1024
+static_assert(sizeof(GemmTask) <= 1024 && sizeof(GemmSeg) <= 1024, "scratch table slots");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchMaxDevices = 64;
+static void* g_scratch_dev[kScratchMaxDevices];
+static std::mutex g_scratch_mu;
+
+// device address of this device's scratch (the device is current)
+static int scratch_setup(int device) {
+  if (device < 0 || device >= kScratchMaxDevices) return fail(DPD_ENODEV, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return DPD_OK;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(DPD_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+  g_scratch_dev[device] = ptr;
+  return DPD_OK;
+}
+
+// the tables of a one-task GEMM, stored into scratch by a kernel (arguments
+// by value: no host buffer has to outlive the call, and a graph capture
+// records a kernel node, not a copy from host memory)
+struct XgemmTables {
+  GemmTask task;
+  GemmSeg seg;
+};
+__global__ void k_put_xgemm_tables(unsigned char* dst, XgemmTables t) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *(GemmTask*)dst = t.task;
+    *(GemmSeg*)(dst + 1024) = t.seg;
+  }
 }
 
 extern "C" {
@@ -57,7 +100,8 @@ int dpd_ctx_create(dpd_ctx** out, int64_t nirrep, const int64_t* dirprd64, int d
     delete c;
     return DPD_EHIP;
   }
-  int rc = c->red.alloc(sizeof(double) * (kRedBlocks * 72 + 64) + sizeof(int64_t) * (kRedBlocks + 8));
+  int rc = scratch_setup(device);
+  if (!rc) rc = c->red.alloc(sizeof(double) * (kRedBlocks * 72 + 64) + sizeof(int64_t) * (kRedBlocks + 8));
   if (rc) {
     delete c;
     return rc;
@@ -396,23 +440,27 @@ int dpd_sstgenso(dpd_ctx* c, const double* win, double* wout, const int64_t* pop
 }
 
 // ---- dense GEMM (libsrint/xgemmso.f:1-96 router: XGEMMD/XGEMMDG/XGEMMSG) ----------
-int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int64_t n, int64_t k, double alpha,
-              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
-              int prec) {
-  CHECK_CTX(c);
-  { int rc_ = set_device(c); if (rc_) return rc_; }
+}  // extern "C"
+
+// checks the arguments of a column-major GEMM call and describes it as one
+// task with one K segment; *empty is set when C is empty (nothing to do)
+static int xgemm_describe(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int64_t n, int64_t k,
+                          int64_t lda, int64_t ldb, int64_t ldc, int prec, GemmPlan& p, bool* empty) {
+  *empty = false;
   if (!transa || !transb) return fail(DPD_EINVAL, "null trans flag");
   char ta = transa[0], tb = transb[0];
   if ((ta != 'N' && ta != 'T') || (tb != 'N' && tb != 'T')) return fail(DPD_EINVAL, "trans must be 'N' or 'T'");
   if (m < 0 || n < 0 || k < 0 || m > INT32_MAX || n > INT32_MAX || k > INT32_MAX)
     return fail(DPD_EINVAL, "bad GEMM sizes");
-  if (m == 0 || n == 0) return DPD_OK;  // empty C (BLAS quick return, before the leading-dimension rules)
+  if (m == 0 || n == 0) {  // empty C (BLAS quick return, before the leading-dimension rules)
+    *empty = true;
+    return DPD_OK;
+  }
   if (ldc < std::max<int64_t>(1, m) || lda < std::max<int64_t>(1, ta == 'N' ? m : k) ||
       ldb < std::max<int64_t>(1, tb == 'N' ? k : n))
     return fail(DPD_EINVAL, "bad leading dimension");
   if (prec != PREC_F64 && prec != PREC_F32 && prec != PREC_BF16X3)
     return fail(DPD_EINVAL, "precision %d is not available", prec);
-  GemmPlan p;
   GemmTask tk;
   std::memset(&tk, 0, sizeof tk);
   tk.M = (int32_t)m;
@@ -442,8 +490,21 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
   p.tasks.push_back(tk);
   p.segs.push_back(sg);
   p.nwg = tk.tiles_m * tk.tiles_n;
-  int rc = p.upload();
-  if (rc) return rc;
+  return DPD_OK;
+}
+
+extern "C" {
+
+int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int64_t n, int64_t k, double alpha,
+              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+              int prec) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  GemmPlan p;
+  bool empty = false;
+  int rc = xgemm_describe(c, transa, transb, m, n, k, lda, ldb, ldc, prec, p, &empty);
+  if (rc || empty) return rc;
+  if ((rc = p.upload())) return rc;
   if (prec == PREC_BF16X3 || c->packing == 2) {
     if ((rc = build_variant(p, prec))) return rc;
     const GemmVariant& V = p.pk[prec];
@@ -460,6 +521,70 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
   if (rc) return rc;
   // the plan's device arrays are freed on return: wait for the launch
   DPD_HIP_TRY(hipStreamSynchronize(c->stream));
+  return DPD_OK;
+}
+
+// dpd_xgemm without device allocation or synchronisation: the tables and the
+// packed panels live in the device's static scratch (kScratchBytes); long-K
+// tiles are not split over workgroups (no partial sums to hold)
+int dpd_xgemm_noalloc(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int64_t n, int64_t k,
+                      double alpha, const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                      double* C, int64_t ldc, int prec) {
+  CHECK_CTX(c);
+  { int rc_ = set_device(c); if (rc_) return rc_; }
+  void* sp = g_scratch_dev[c->device];
+  if (!sp) return fail(DPD_EINVAL, "dpd_xgemm_noalloc: no scratch address for device %d (internal)", c->device);
+  GemmPlan p;
+  bool empty = false;
+  int rc = xgemm_describe(c, transa, transb, m, n, k, lda, ldb, ldc, prec, p, &empty);
+  if (rc || empty) return rc;
+  unsigned char* base = (unsigned char*)sp;
+  const bool packed = prec == PREC_BF16X3 || c->packing == 2;
+  GemmVariant V;
+  size_t need = kScratchTableBytes;
+  if (packed) {
+    // the tile dpd_xgemm's packed path takes (build_variant's choice)
+    int cfg = choose_gemm_cfg(p.tasks, -1);
+    if (cfg == GEMM_V1) cfg = GEMM_64x64;
+    if ((rc = layout_variant(p, V, prec, cfg, false))) return rc;
+    need += pk_esize(prec) * (size_t)(V.pa_elems + V.pb_elems) + 512;
+  }
+  if (need > kScratchBytes)
+    return fail(DPD_EINVAL,
+                "dpd_xgemm_noalloc: m=%lld n=%lld k=%lld needs %zu bytes of scratch, more than its %zu; "
+                "use dpd_xgemm",
+                (long long)m, (long long)n, (long long)k, need, kScratchBytes);
+  XgemmTables t;
+  std::memset(&t, 0, sizeof t);
+  t.task = packed ? V.tasks[0] : p.tasks[0];
+  t.seg = p.segs[0];
+  hipLaunchKernelGGL(k_put_xgemm_tables, dim3(1), dim3(64), 0, c->stream, base, t);
+  DPD_HIP_TRY(hipGetLastError());
+  p.d_tasks = (GemmTask*)base;
+  p.d_segs = (GemmSeg*)(base + 1024);
+  if (!packed) return run_gemm(c, p, A, 0, B, 0, C, alpha, beta, prec);
+  V.d_tasks = (GemmTask*)base;
+  V.d_split = nullptr;
+  unsigned char* pa = base + kScratchTableBytes;
+  unsigned char* pb = pa + ((pk_esize(prec) * (size_t)V.pa_elems + 255) & ~size_t(255));
+  PackSrc a, b;
+  a.data = A;
+  b.data = B;
+  if (V.pa_elems && (rc = pack_side(c, p, V, 0, a, pa))) return rc;
+  if (V.pb_elems && (rc = pack_side(c, p, V, 1, b, pb))) return rc;
+  if (V.nwg == 0) return DPD_OK;
+  GemmLaunch g;
+  std::memset(&g, 0, sizeof g);
+  g.C = C;
+  g.tasks = V.d_tasks;
+  g.segs = p.d_segs;
+  g.ntask = 1;
+  g.nwg = V.nwg;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.prec = prec;
+  g.cfg = V.cfg;
+  DPD_HIP_TRY(launch_gemm_pk_only(g, pa, pb, c->stream));
   return DPD_OK;
 }
 

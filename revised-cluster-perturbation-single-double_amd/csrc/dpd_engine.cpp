@@ -618,9 +618,7 @@ int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested) {
 // split_k: split long-K tasks over several workgroups while the launch has
 // fewer than 1024 tiles (partials summed by a reduce kernel); off: one
 // workgroup per tile
-static int build_variant_cfg(GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k = true) {
-  if (g_capturing) return fail(DPD_EINVAL, "tile variant built during graph capture (internal)");
-  bump_mem_gen();  // the variant's task tables are rewritten and reallocated
+int layout_variant(const GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k) {
   V.mode = mode;
   V.tasks = p.tasks;
   V.split.clear();
@@ -675,8 +673,16 @@ static int build_variant_cfg(GemmPlan& p, GemmVariant& V, int mode, int cfg, boo
   V.pb_elems = pb;
   V.partial_elems = partial;
   V.max_split_mn = max_mn;
+  return DPD_OK;
+}
+
+static int build_variant_cfg(GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k = true) {
+  if (g_capturing) return fail(DPD_EINVAL, "tile variant built during graph capture (internal)");
+  bump_mem_gen();  // the variant's task tables are rewritten and reallocated
+  int rc = layout_variant(p, V, mode, cfg, split_k);
+  if (rc) return rc;
   size_t bt = V.tasks.size() * sizeof(GemmTask), bp = V.split.size() * sizeof(int32_t);
-  int rc = V.dev.alloc(((bt + 255) & ~size_t(255)) + bp + 256);
+  rc = V.dev.alloc(((bt + 255) & ~size_t(255)) + bp + 256);
   if (rc) return rc;
   V.d_tasks = (GemmTask*)V.dev.p;
   V.d_split = (int32_t*)((char*)V.dev.p + ((bt + 255) & ~size_t(255)));
@@ -706,7 +712,7 @@ static int64_t c_span(const GemmPlan& p) {
 }
 
 // pack one operand side of a variant into dst
-static int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const PackSrc& src, void* dst) {
+int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const PackSrc& src, void* dst) {
   PackArgs a;
   std::memset(&a, 0, sizeof a);
   a.src = src.data;

@@ -233,6 +233,12 @@ struct PackSrc {
 int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double* C, double f1,
                     double f2, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
                     bool c_f32 = false);
+// host part of a packed-panel tiling (task tables, panel offsets, split-K);
+// touches no device memory: V.d_tasks / V.d_split stay for the caller to set
+int layout_variant(const GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k);
+// packs one operand side (0: A, 1: B) of variant V into dst (V.d_tasks and
+// p.d_segs must point at device copies of the tables)
+int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const PackSrc& src, void* dst);
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out);
 // labels / checks shared by the C-ABI and the sigma builders
 int check_labels(const char* t, int n);

@@ -106,6 +106,9 @@ _SIGS = {
                                     ctypes.c_char_p]),
     "dpd_xgemm": (ctypes.c_int, [c_void_p, ctypes.c_char_p, ctypes.c_char_p, i64, i64, i64, ctypes.c_double,
                                  c_void_p, i64, c_void_p, i64, ctypes.c_double, c_void_p, i64, ctypes.c_int]),
+    "dpd_xgemm_noalloc": (ctypes.c_int, [c_void_p, ctypes.c_char_p, ctypes.c_char_p, i64, i64, i64,
+                                         ctypes.c_double, c_void_p, i64, c_void_p, i64, ctypes.c_double, c_void_p,
+                                         i64, ctypes.c_int]),
     "dpd_dot": (ctypes.c_int, [c_void_p, i64, c_void_p, c_void_p, c_double_p]),
     "dpd_nrm2": (ctypes.c_int, [c_void_p, i64, c_void_p, c_double_p]),
     "dpd_amax": (ctypes.c_int, [c_void_p, i64, c_void_p, c_i64_p, c_double_p]),
