@@ -85,6 +85,12 @@ def main():
                          "then a plain 512 MiB device copy for calibrating FETCH_SIZE/WRITE_SIZE; no JSON line")
     ap.add_argument("--tile", type=int, default=-1, help="GEMM tile configuration for --probe (-1: autotuned)")
     ap.add_argument("--split", type=int, default=-1, help="split-K for --probe: -1 automatic, 0 off, 1 on")
+    ap.add_argument("--profile-solve", default=None, choices=["fp64", "fp32", "bf16x3"],
+                    help="trace mode: a warm-up solve in this precision, then one solve between two marker "
+                         "kernels (k_precond on one element) for rocprofv3 --kernel-trace; no JSON line")
+    ap.add_argument("--summarize-trace", default=None, metavar="CSV",
+                    help="summarise the kernel_trace.csv of a --profile-solve run (the kernels between the "
+                         "markers, grouped by class) as markdown and exit")
     ap.add_argument("--fd-dirs", type=int, default=50, help="gradient workload: random integral directions")
     ap.add_argument("--fd-h", type=float, default=1e-4, help="gradient workload: central-difference step")
     ap.add_argument("--grad-conv", type=float, default=1e-9, help="gradient workload: FP64 residual tolerance")
@@ -93,6 +99,10 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
               "benchmark", file=sys.stderr)
         sys.exit(2)
+
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
+        return
 
     import numpy as np
     import torch
@@ -103,6 +113,9 @@ def main():
     nroot = cfg["nroot"]
     if args.probe:
         probe(cfg, args)
+        return
+    if args.profile_solve:
+        profile_solve(cfg, args)
         return
     if cfg.get("kind") == "gradient":
         gradient_bench(cfg, args)
@@ -515,6 +528,98 @@ def probe(cfg, args):
         b.copy_(a)
     torch.cuda.synchronize()
     log("probe done")
+
+
+def profile_solve(cfg, args):
+    """The program traced for the per-iteration split: one warm-up solve (GEMM
+    tiles tuned, integral panels packed), then exactly one solve of the same
+    precision between two marker kernels; summarize_trace() reads the trace."""
+    import torch
+
+    import dpdhip
+    ctx = dpdhip.Context(len(cfg["nso"]), 0)
+    prob = dpdhip.Problem(ctx, dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"]), "sf-cpsd", 1)
+    nroot = cfg["nroot"]
+    guess, _ = prob.guess(nroot)
+    conv = args.conv if args.profile_solve == "fp64" else args.conv_mixed
+    prob.set_precision(args.profile_solve)
+    kw = dict(conv=conv, max_dim=args.max_dim, max_cyc=args.max_cyc, lowest=args.tracking == "lowest",
+              final_check=False)
+    prob.davidson(nroot, R=guess.clone(), **kw)
+    one = torch.ones(1, dtype=torch.float64, device="cuda:0")
+    out = torch.empty_like(one)
+    R = guess.clone()
+    torch.cuda.synchronize()
+    dpdhip.precondition(ctx, 0.0, one, one, out)  # marker: start
+    a = time.perf_counter()
+    w, R, info = prob.davidson(nroot, R=R, **kw)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - a
+    dpdhip.precondition(ctx, 0.0, one, one, out)  # marker: end
+    torch.cuda.synchronize()
+    log(f"profiled solve: {args.profile_solve} conv {conv:g} {info['iterations']} iterations "
+        f"{info['sigma_calls']} sigma calls {t:.4f}s")
+
+
+# kernel classes of the per-iteration split (substring of the kernel name)
+KERNEL_CLASSES = [
+    ("GEMM (MFMA)", ("k_gemm_pk", "k_gemm2", "k_gemm<", "k_splitk_reduce")),
+    ("operand packing", ("k_pack",)),
+    ("gathers (sorts, expansions, epilogues)", ("k_gather",)),
+    ("Davidson vector algebra", ("k_multidot", "k_reduce_rows", "k_lincomb", "k_dots_multi", "k_cgs_need",
+                                 "k_tri_apply", "k_chol_tri", "k_mgs", "k_pair_dots", "k_renorm_pair",
+                                 "k_resid", "k_amax", "k_precond_batched", "k_scale_by_norm", "k_block_dots",
+                                 "k_davidson")),
+    ("elementwise sigma terms, copies", ("k_hadamard", "k_div", "k_orbital_energy", "k_convert", "k_copy",
+                                         "k_axpy", "k_scal", "k_fill")),
+]
+
+
+def summarize_trace(path):
+    """Markdown summary of the kernels between the two k_precond markers of a
+    --profile-solve trace: time per class and per kernel, and the idle time of
+    the GPU inside the window (window length minus kernel time)."""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    name_key = "Kernel_Name"
+    recs = sorted(((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r[name_key]) for r in rows),
+                  key=lambda x: x[0])
+
+    def is_marker(n):
+        return n.split("(")[0].strip().endswith("k_precond")
+    marks = [i for i, r in enumerate(recs) if is_marker(r[2])]
+    if len(marks) < 2:
+        raise SystemExit("no marker pair in the trace")
+    i0, i1 = marks[-2], marks[-1]
+    win = recs[i0 + 1:i1]
+    t0, t1 = recs[i0][1], recs[i1][0]
+    span = (t1 - t0) * 1e-3  # us
+    per_kernel = {}
+    for a, b, n in win:
+        base = n.split("(")[0].strip()
+        e = per_kernel.setdefault(base, [0, 0.0])
+        e[0] += 1
+        e[1] += (b - a) * 1e-3
+    cls = {c: 0.0 for c, _ in KERNEL_CLASSES}
+    cls["other"] = 0.0
+    for base, (cnt, us) in per_kernel.items():
+        for c, keys in KERNEL_CLASSES:
+            if any(k in base for k in keys):
+                cls[c] += us
+                break
+        else:
+            cls["other"] += us
+    busy = sum(cls.values())
+    print(f"window: {span / 1e3:.3f} ms between the markers, {len(win)} kernels, busy {busy / 1e3:.3f} ms, "
+          f"idle {(span - busy) / 1e3:.3f} ms\n")
+    print("| class | ms | share |\n|---|---:|---:|")
+    for c, us in sorted(cls.items(), key=lambda kv: -kv[1]):
+        if us > 0:
+            print(f"| {c} | {us / 1e3:.3f} | {100 * us / span:.1f}% |")
+    print(f"| GPU idle inside the window | {(span - busy) / 1e3:.3f} | {100 * (span - busy) / span:.1f}% |\n")
+    print("| kernel | launches | ms total | us avg |\n|---|---:|---:|---:|")
+    for base, (cnt, us) in sorted(per_kernel.items(), key=lambda kv: -kv[1][1])[:30]:
+        print(f"| `{base}` | {cnt} | {us / 1e3:.3f} | {us / cnt:.1f} |")
 
 
 def _cpu_model():

@@ -49,6 +49,13 @@ void vv2v_dpd_442_(const double *v1, const int64_t *p1_1, const int64_t *p1_2, c
                    const int64_t *sym2, const char *type2, double *v3, const int64_t *p3_1, const int64_t *p3_2,
                    const int64_t *iop3_12, const int64_t *sym3, const char *type3, const double *fact1,
                    const double *fact2, size_t len1, size_t len2, size_t len3);
+/* libsrint/vv2v_dpd_323.f90:22-25 (type1*3, type2*2, type3*3) */
+void vv2v_dpd_323_(const double *v1, const int64_t *p1_1, const int64_t *p1_2, const int64_t *iop1_12,
+                   const int64_t *p1_3, const int64_t *sym1, const char *type1, const double *v2,
+                   const int64_t *p2_1, const int64_t *p2_2, const int64_t *iop2_12, const int64_t *sym2,
+                   const char *type2, double *v3, const int64_t *p3_1, const int64_t *p3_2,
+                   const int64_t *iop3_12, const int64_t *p3_3, const int64_t *sym3, const char *type3,
+                   const double *fact1, const double *fact2, size_t len1, size_t len2, size_t len3);
 /* libsrint/xgemmso.f:1-25 (XGEMMSO) */
 void xgemmso_(const char *transa, const char *transb, const int64_t *m, const int64_t *n, const int64_t *k,
               const double *alpha, const double *a, const int64_t *lda, const double *b, const int64_t *ldb,

@@ -124,6 +124,15 @@ int dpd_vv2v_dpd_442(dpd_ctx *ctx, const dpd_tensor *v1, const char *type1,
                      const dpd_tensor *v2, const char *type2, dpd_tensor *v3,
                      const char *type3, double fact1, double fact2);
 
+/* libsrint/vv2v_dpd_323.f90:22-145: 3-index x 2-index -> 3-index
+ * (e.g. 'pqm' x 'mr' -> 'pqr').  Rank-3 tensors use pop[0..2], iop[0] (the pair
+ * of their first two indices) and sym; v2 uses pop[0..1] and must have iop 1.
+ * The third index of v1 is the contracted one.  A pair kept intact may be packed
+ * (iop 0/2); when it is reordered or split, v1 and v3 need iop 1. */
+int dpd_vv2v_dpd_323(dpd_ctx *ctx, const dpd_tensor *v1, const char *type1,
+                     const dpd_tensor *v2, const char *type2, dpd_tensor *v3,
+                     const char *type3, double fact1, double fact2);
+
 /* ---- contract surface: v3 += fact * P{v1 (x) v2} (lib_cc/mod_contract.f90) */
 /* Scratch (the reference's iCore/maxCore) comes from the context's workspace. */
 /* mod_contract.f90:10-162, anti = 0..3 antisymmetrises none/pq/rs/both of v3 */
@@ -296,6 +305,11 @@ typedef struct {
   int64_t skipped;   /* new directions dropped by the norm < 1e-6 test (:437) */
   double residual[64];
   double final_residual[64];
+  /* 1: the sigma step of every iteration is replayed from a hipGraph captured
+   * per vector count (on a private stream ordered with the context's; staged
+   * through fixed device vectors; eager while the context profiler is on) */
+  int graph;
+  int64_t graph_replays; /* out: sigma steps replayed from a graph */
 } dpd_davidson_opts;
 /* Subspace eigensolve (the dgeev call of lib_cc/mod_davidson.f90:186), host FP64:
  * eigenvalues (wr, wi) and unit-norm right eigenvectors (columns of V, column
@@ -322,6 +336,73 @@ int dpd_problem_sigma_group(dpd_problem *const *probs, const double *const *R, d
  * DPD_ENOTCONV if any member did not converge. */
 int dpd_davidson_batch(dpd_problem *const *probs, int64_t n, dpd_davidson_opts *opts, double *const *R,
                        double *const *w, int64_t flags, int64_t *stats);
+
+/* Out-of-core <ab||ef> (the reference's c2_r2abcd.f90:29-104 / p_c2_r2abcd_h
+ * path for integrals beyond memory): on = 1 moves abcd to pinned host memory
+ * and the pp-ladder streams it irrep block by irrep block through two device
+ * staging buffers, the copy of one block overlapping the GEMM of the previous;
+ * on = 0 brings it back.  While out of core, dpd_problem_sigma_t,
+ * dpd_problem_grad, grouped sigma and get/set of "abcd" return DPD_EINVAL. */
+int dpd_problem_set_abcd_out_of_core(dpd_problem *prob, int on);
+
+/* ---- perturbation drivers (CPSd.SF, EE-CPSd.SF) ---------------------------
+ * Both need a problem of method 2 (every integral block).  DIIS settings as in
+ * lib_cc/mod_common.f90:15-18 (diis_dim 10, diis_cyc 50, diis_max_err 1e-7 when
+ * left 0); the DIIS history (libsrint/dosodiis2_d.f) stays in device memory. */
+typedef struct {
+  int64_t diis_dim;      /* DIIS subspace size */
+  int64_t diis_cyc;      /* iteration cap of each DIIS solve */
+  double diis_max_err;   /* |largest element of the extrapolated residual| to stop */
+  double energy[3];      /* out (ground state): E^(2), E^(3), E^(4) */
+  int64_t iterations[2]; /* out (ground state): DIIS iterations of the T1 equations, orders 2, 3 */
+} dpd_cpsd_opts;
+/* Length of one order of the ground-state amplitudes: |T1| + |T2(a<b,i<j)| of
+ * the totally symmetric irrep. */
+int64_t dpd_problem_cpsd_len(const dpd_problem *prob);
+/* CPSd.SF/run.f90 (run1/run2/run3.f90): ground-state CPS(D) amplitudes
+ * T^(1..3) and correlation energies E^(2..4).  T (device, may be NULL): receives
+ * 3 x dpd_problem_cpsd_len() doubles, (T1 | T2) of orders 1, 2, 3 -- the f_T
+ * file's records in order. */
+int dpd_problem_cpsd(dpd_problem *prob, dpd_cpsd_opts *opts, double *T);
+/* EE-CPSd.SF/run.f90 + run_order.f90: EE-CPS(D) excitation energies of nroot
+ * CIS roots of the problem's symmetry, order by order.  T: the ground-state
+ * amplitudes of dpd_problem_cpsd (device); w0 (host, nroot): CIS energies;
+ * L (device, nroot x |R1|): CIS vectors (f_CIS).  w (host, 3*nroot): w^(1),
+ * w^(2), w^(3) per root (the total is w0 + w^(2) + w^(3)); iterations (host,
+ * nroot, may be NULL): DIIS iterations of the order-2 R1 solve. */
+int dpd_problem_eecpsd(dpd_problem *prob, dpd_cpsd_opts *opts, const double *T, int64_t nroot,
+                       const double *w0, const double *L, double *w, int64_t *iterations);
+
+/* ---- on-disk formats of the reference (host side; no GPU needed) ------------
+ * Tensor data may be host or device pointers. */
+/* pyscf2cc/mod_interface.f90: the "interface" node file (stream file: metaPos,
+ * nNode, data, 36-byte node records; names lower-cased, at most 20 characters).
+ * type: 'i' int64, 'd' double, 's' float, 'c' characters; count elements. */
+int dpd_node_init(const char *path);
+int dpd_node_write(const char *path, const char *name, char type, const void *data, int64_t count);
+int dpd_node_read(const char *path, const char *name, char type, void *data, int64_t count);
+/* bytes held by the node, -1 when absent */
+int64_t dpd_node_size(const char *path, const char *name);
+/* libsrint/clmoioso0.f + putallso / getallso_d.f: a symmetry-packed list as the
+ * data file <path> (records of 4096 doubles, irrep blocks record-aligned) and
+ * its control file <path>.c (Fortran sequential records, INTEGER*8). */
+int dpd_moio_write(const dpd_ctx *ctx, const char *path, int64_t irrepx, const int64_t *popp,
+                   const int64_t *popq, int64_t ioppq, const int64_t *popr, const int64_t *pops,
+                   int64_t ioprs, const double *data);
+/* irrepx, pops (4 x 8), iop (2) and the packed length of a list (each may be NULL) */
+int dpd_moio_info(const char *path, int64_t *irrepx, int64_t *pops, int64_t *iop, int64_t *length);
+int dpd_moio_read(const char *path, double *dst, int64_t length);
+/* lib_cc/IO.f90:87-154 (IO_stream): element idx (1-based, 8-byte units); mode 1
+ * val and vec, 2 val only, 3 vec only; replace creates the file anew. */
+int dpd_stream_write(const char *path, int64_t idx, double val, const double *vec, int64_t n, int mode,
+                     int replace);
+int dpd_stream_read(const char *path, int64_t idx, double *val, double *vec, int64_t n, int mode);
+/* the Davidson eigenvector file (lib_cc/mod_davidson.f90:388-390 writes, :588-589
+ * reads the guesses): root r (0-based) of state block istate as [w, R(dim)] at
+ * element (istate + r - 1) * (1 + dim) + 1. */
+int dpd_eigvec_write(const char *path, int64_t istate, int64_t nroot, int64_t dim, const double *w,
+                     const double *R, int replace);
+int dpd_eigvec_read(const char *path, int64_t istate, int64_t nroot, int64_t dim, double *w, double *R);
 
 #ifdef __cplusplus
 }

@@ -278,6 +278,33 @@ def vv2v_dpd_442(v1, d1, t1, v2, d2, t2, v3, d3, t3, f1, f2):
     _eng(lib().orc_vv2v_dpd_442, v1, d1, t1, v2, d2, t2, v3, d3, t3, f1, f2)
 
 
+def vv2v_dpd_323(v1, d1, t1, v2, d2, t2, v3, d3, t3, f1, f2):
+    """libsrint/vv2v_dpd_323.f90:22-145: v3(pqr) = f1 * v1(pqm) v2(mr) + f2 * v3.
+    d1/d3 describe the rank-3 tensors (pops [P, Q, M, ZORB], iop (pair, 1)),
+    d2 the 2-index v2 (pops [M, R, ZORB, ZORB], iop (1, 1)).  The wrapper's
+    checks (:55-88), then the 444 call with the single indices paired with
+    1-sized dummies named by the first two free letters (:90-145)."""
+    cc = t1[2]
+    if cc in t3:
+        raise OracleError("vv2v_dpd_323: the third index of v1 must be contracted")
+    intact = t1[:2] == t3[:2]
+    if d2.iop[0] != 1:
+        raise OracleError("vv2v_dpd_323: v2 must have iop 1")
+    if not intact and (d1.iop[0] != 1 or d3.iop[0] != 1):
+        raise OracleError("vv2v_dpd_323: reordered pair needs iop 1")
+    free = [ch for ch in "abcdefghijklmnopqrstuvwxyz" if ch not in t1 + t2 + t3][:2]
+    c1, c2 = free
+    t1x, t3x = t1 + c1, t3 + c2
+    if t2[0] == cc:
+        t2x = t2[0] + c1 + t2[1] + c2
+    elif t2[1] == cc:
+        t2x = t2[0] + c2 + t2[1] + c1
+    else:
+        raise OracleError("vv2v_dpd_323: contracted index not in v2")
+    d2x = Desc([d2.pops[0], ZORB, d2.pops[1], ZORB], (1, 1), d2.sym)
+    vv2v_dpd_444(v1, d1, t1x, v2, d2x, t2x, v3, d3, t3x, f1, f2)
+
+
 def contract_general(v1, d1, t1, v2, d2, t2, v3, d3, t3, f1, f2):
     c1, c2, c3 = d1.c(), d2.c(), d3.c()
     _check(lib().orc_contract_general(_dp(v1), ctypes.byref(c1), t1.encode(), ctypes.c_int(len(t1)), _dp(v2),

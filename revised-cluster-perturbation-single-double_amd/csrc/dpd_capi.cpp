@@ -239,6 +239,64 @@ int dpd_vv2v_dpd_442(dpd_ctx* c, const dpd_tensor* v1, const char* t1, const dpd
   return engine_entry(c, 442, v1, t1, v2, t2, v3, t3, f1, f2);
 }
 
+// libsrint/vv2v_dpd_323.f90:22-145: 3-index x 2-index -> 3-index through the 444
+// engine.  The third index of v1 is the contracted one; each rank-3 tensor's
+// single index is paired with a dummy index of population 1 in irrep 1, and v2
+// with two of them, so that v1(pq,m c1) x v2(m c1,r c2) -> v3(pq,r c2) is the
+// engine's standard contraction.  An intact pair (v1 and v3 open with the same
+// two labels) may be packed; a pair that is reordered or split needs iop 1.
+int dpd_vv2v_dpd_323(dpd_ctx* c, const dpd_tensor* v1, const char* type1, const dpd_tensor* v2, const char* type2,
+                     dpd_tensor* v3, const char* type3, double f1, double f2) {
+  CHECK_CTX(c);
+  if (!v1 || !v2 || !v3) return fail(DPD_EINVAL, "null tensor");
+  if (!type1 || !type2 || !type3) return fail(DPD_EINVAL, "null label string");
+  const char cc = type1[2];
+  for (int i = 0; i < 3; i++)
+    if (type3[i] == cc)
+      return fail(DPD_EINVAL, "vv2v_dpd_323: the third index of v1 (%c) must be the contracted one", cc);
+  const bool intact = type1[0] == type3[0] && type1[1] == type3[1];
+  if (v2->iop[0] != 1) return fail(DPD_EINVAL, "vv2v_dpd_323: the 2-index tensor v2 must be uncompressed (iop 1)");
+  if (!intact && (v1->iop[0] != 1 || v3->iop[0] != 1))
+    return fail(DPD_EINVAL, "vv2v_dpd_323: pair %.2s -> %.2s is reordered or split: v1 and v3 need iop 1", type1,
+                type3);
+  // two letters used by none of the label strings name the dummy indices
+  char d1 = 0, d2 = 0;
+  for (char l = 'a'; l <= 'z' && !d2; l++) {
+    bool used = false;
+    for (int i = 0; i < 3; i++) used = used || type1[i] == l || type3[i] == l;
+    used = used || type2[0] == l || type2[1] == l;
+    if (used) continue;
+    if (!d1) d1 = l;
+    else d2 = l;
+  }
+  if (!d2) return fail(DPD_EINVAL, "vv2v_dpd_323: no free letters for the dummy indices");
+  char l1[4] = {type1[0], type1[1], type1[2], d1};
+  char l3[4] = {type3[0], type3[1], type3[2], d2};
+  char l2[4];
+  if (type2[0] == cc) {
+    l2[0] = type2[0], l2[1] = d1, l2[2] = type2[1], l2[3] = d2;
+  } else if (type2[1] == cc) {
+    l2[0] = type2[0], l2[1] = d2, l2[2] = type2[1], l2[3] = d1;
+  } else {
+    return fail(DPD_EINVAL, "vv2v_dpd_323: contracted index '%c' not found in v2 (%.2s)", cc, type2);
+  }
+  int64_t pdum[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+  dpd_tensor a = *v1, b, d = *v3;
+  std::memcpy(a.pop[3], pdum, sizeof pdum);
+  a.iop[1] = 1;
+  std::memcpy(d.pop[3], pdum, sizeof pdum);
+  d.iop[1] = 1;
+  std::memset(&b, 0, sizeof b);
+  std::memcpy(b.pop[0], v2->pop[0], sizeof pdum);
+  std::memcpy(b.pop[1], pdum, sizeof pdum);
+  std::memcpy(b.pop[2], v2->pop[1], sizeof pdum);
+  std::memcpy(b.pop[3], pdum, sizeof pdum);
+  b.iop[0] = b.iop[1] = 1;
+  b.sym = v2->sym;
+  b.data = v2->data;
+  return engine_entry(c, 444, &a, l1, &b, l2, &d, l3, f1, f2);
+}
+
 // ---- contract surface ---------------------------------------------------------
 static int contract_entry(dpd_ctx* c, int kind, const dpd_tensor* v1, const char* t1, int r1,
                           const dpd_tensor* v2, const char* t2, int r2, dpd_tensor* v3, const char* t3,

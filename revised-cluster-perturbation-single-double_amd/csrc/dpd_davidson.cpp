@@ -350,7 +350,11 @@ typedef std::function<int(const double* x, double* y, int nv)> SigmaHook;
 static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out, const SigmaHook* hook,
                         hipStream_t stream);
 
+static int davidson_graph(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out);
+
 extern "C" int dpd_davidson(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out) {
+  if (o) o->graph_replays = 0;
+  if (o && o->graph && P) return davidson_graph(P, o, R, w_out);
   return davidson_run(P, o, R, w_out, nullptr, nullptr);
 }
 
@@ -832,6 +836,101 @@ struct GraphCache {
 };
 
 }  // namespace
+
+// dpd_davidson with the sigma step replayed from hipGraphs: the solver runs on
+// a private stream (capture needs one that is not the legacy default stream),
+// ordered after the context's stream on entry and before it on exit; the new
+// vectors of a step are copied into fixed device vectors, the sigma of that
+// vector count runs eagerly twice (plans, tuned tiles, panels and workspace
+// settle) and is then captured once and replayed; graphs are dropped whenever
+// device memory they reference moved (g_mem_gen).  With the context profiler on
+// the steps stay eager (its events are recorded per launch).
+static int davidson_graph(dpd_problem* P, dpd_davidson_opts* o, double* R, double* w_out) {
+  if (!R || !w_out) return fail(DPD_EINVAL, "dpd_davidson: null argument");
+  dpd_ctx* c = P->ctx;
+  DPD_HIP_TRY(hipSetDevice(c->device));
+  const int64_t maxv = std::max<int64_t>(1, o->nroot);
+  const size_t bytes = sizeof(double) * (size_t)P->dim * (size_t)maxv;
+  DevBuf X, Y;
+  int rc;
+  if ((rc = X.alloc(bytes)) || (rc = Y.alloc(bytes))) return rc;
+  hipStream_t caller = c->stream, st = nullptr;
+  hipEvent_t ev = nullptr;
+  DPD_HIP_TRY(hipStreamCreate(&st));
+  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(st);
+    return fail(DPD_EHIP, "event creation failed");
+  }
+  (void)hipEventRecord(ev, caller);
+  (void)hipStreamWaitEvent(st, ev, 0);
+  c->stream = st;
+  int64_t replays = 0;
+  {
+    GraphCache gc;
+    SigmaHook hook = [&](const double* x, double* y, int nv) -> int {
+      if (nv > maxv || c->prof.on) return o->transpose ? problem_sigma_t(P, x, y, nv) : problem_sigma(P, x, y, nv);
+      auto sig = [&]() { return o->transpose ? problem_sigma_t(P, (const double*)X.p, (double*)Y.p, nv)
+                                             : problem_sigma(P, (const double*)X.p, (double*)Y.p, nv); };
+      const uint64_t gen = g_mem_gen.load(std::memory_order_relaxed);
+      if (gen != gc.gen) {
+        if (!gc.exec.empty()) DPD_HIP_TRY(hipStreamSynchronize(st));
+        gc.drop();
+        gc.gen = gen;
+      }
+      DPD_HIP_TRY(hipMemcpyAsync(X.p, x, sizeof(double) * (size_t)P->dim * nv, hipMemcpyDeviceToDevice, st));
+      const std::string key = std::to_string(nv);
+      auto it = gc.exec.find(key);
+      int& seen = gc.seen[key];
+      if (it != gc.exec.end()) {
+        DPD_HIP_TRY(hipGraphLaunch(it->second, st));
+        replays++;
+      } else if (seen < 2) {
+        seen++;
+        int r = sig();
+        if (r) return r;
+      } else {
+        bool ok = seen == 2;
+        if (ok) {
+          hipGraph_t g = nullptr;
+          DPD_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          g_capturing = true;
+          int r = sig();
+          g_capturing = false;
+          hipError_t e = hipStreamEndCapture(st, &g);
+          ok = r == DPD_OK && e == hipSuccess && g && g_mem_gen.load(std::memory_order_relaxed) == gc.gen;
+          if (ok) {
+            gc.graphs.push_back(g);
+            hipGraphExec_t ge = nullptr;
+            DPD_HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            gc.exec.emplace(key, ge);
+            DPD_HIP_TRY(hipGraphLaunch(ge, st));
+            replays++;
+          } else {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            seen = 3;  // stays eager for this vector count
+          }
+        }
+        if (!ok) {
+          int r = sig();
+          if (r) return r;
+        }
+      }
+      DPD_HIP_TRY(hipMemcpyAsync(y, Y.p, sizeof(double) * (size_t)P->dim * nv, hipMemcpyDeviceToDevice, st));
+      return DPD_OK;
+    };
+    rc = davidson_run(P, o, R, w_out, &hook, st);
+    (void)hipStreamSynchronize(st);
+  }
+  c->stream = caller;
+  (void)hipEventRecord(ev, st);
+  (void)hipStreamWaitEvent(caller, ev, 0);
+  (void)hipStreamSynchronize(st);
+  (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(st);
+  o->graph_replays = replays;
+  return rc;
+}
 
 extern "C" int dpd_problem_sigma_group(dpd_problem* const* probs, const double* const* R, double* const* sigma,
                                        const int64_t* nvec, int64_t n) {

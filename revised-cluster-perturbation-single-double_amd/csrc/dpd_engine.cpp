@@ -523,7 +523,8 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
   // pipelined kernel handles every task shape, the first-generation kernel is
   // kept selectable (dpd_ctx_set_gemm_config) for comparison
   plan.cfg = choose_gemm_cfg(plan.tasks, plan.cfg_req);
-  if (plan.cfg == GEMM_64x192) plan.cfg = GEMM_64x128;  // the gathering kernel has no 192-wide tile
+  // the gathering kernel has no 192-wide tile
+  if (plan.cfg == GEMM_64x192 || plan.cfg == GEMM_128x192D) plan.cfg = GEMM_64x128;
   const int BM = gemm_bm(plan.cfg), BN = gemm_bn(plan.cfg);
   for (auto& tk : plan.tasks) {
     tk.tiles_m = (tk.M + BM - 1) / BM;
@@ -619,6 +620,7 @@ int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested) {
 // fewer than 1024 tiles (partials summed by a reduce kernel); off: one
 // workgroup per tile
 int layout_variant(const GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k) {
+  if (cfg == GEMM_128x192D && mode != PREC_BF16X3) cfg = GEMM_64x192;  // direct-A tile: BF16x3 only
   V.mode = mode;
   V.tasks = p.tasks;
   V.split.clear();
@@ -840,7 +842,8 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
   // candidates in order of preference: among shapes within 5% of the fastest
   // the first listed wins, which keeps the choice stable from run to run; each
   // tile shape with and without split-K (the panels are the same for both)
-  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x128, GEMM_64x128, GEMM_64x64, GEMM_32x32};
+  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x192D, GEMM_128x128, GEMM_64x128, GEMM_64x64,
+                             GEMM_32x32};
   constexpr int NC = (int)(sizeof cand / sizeof cand[0]);
   float tms[NC][2];
   const size_t es = pk_esize(mode);
@@ -854,6 +857,7 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
   for (int ci = 0; ci < NC && !rc; ci++) {
     tms[ci][0] = tms[ci][1] = 1e30f;
     const int cfg = cand[ci];
+    if (cfg == GEMM_128x192D && mode != PREC_BF16X3) continue;  // the same as GEMM_64x192 there
     DevBuf ta, tb;
     bool had_split = false;
     for (int sk = 0; sk < 2 && !rc; sk++) {
@@ -1725,7 +1729,8 @@ static bool capturing(hipStream_t s) {
 }
 
 static int tune_group(dpd_ctx* c, GroupPlan& G, int prec, const std::vector<CallPrep>& R, double fact) {
-  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x128, GEMM_64x128, GEMM_64x64, GEMM_32x32};
+  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x192D, GEMM_128x128, GEMM_64x128, GEMM_64x64,
+                             GEMM_32x32};
   const int ncand = (int)(sizeof cand / sizeof cand[0]);
   float tms[ncand];
   const size_t es = pk_esize(prec);
@@ -1744,6 +1749,7 @@ static int tune_group(dpd_ctx* c, GroupPlan& G, int prec, const std::vector<Call
   for (int ci = 0; ci < ncand && !rc; ci++) {
     tms[ci] = 1e30f;
     const int cfg = cand[ci];
+    if (cfg == GEMM_128x192D && prec != PREC_BF16X3) continue;  // the same as GEMM_64x192 there
     std::vector<GemmVariant> T(n);
     std::vector<DevBuf> pa(n), pb(n);
     std::vector<const GemmVariant*> Vs(n);

@@ -104,6 +104,21 @@ dpd_tensor tensor2(const int64_t* p1, const int64_t* p2, const int64_t* iop12, c
   return t;
 }
 
+// a rank-3 operand (vv2v_dpd_323): pair (p,q) and a single index r, stored as
+// the 4-index tensor with a 1-sized dummy fourth index
+dpd_tensor tensor3(const int64_t* p1, const int64_t* p2, const int64_t* iop12, const int64_t* p3, const int64_t* sym) {
+  dpd_tensor t;
+  std::memset(&t, 0, sizeof t);
+  std::memcpy(t.pop[0], p1, 64);
+  std::memcpy(t.pop[1], p2, 64);
+  std::memcpy(t.pop[2], p3, 64);
+  t.pop[3][0] = 1;
+  t.iop[0] = *iop12;
+  t.iop[1] = 1;
+  t.sym = *sym;
+  return t;
+}
+
 int64_t len_of(const dpd_tensor& t) { return dpd_tensor_len(g_ctx, &t); }
 
 // one engine call: stage, run, copy back
@@ -200,6 +215,20 @@ void vv2v_dpd_442_(const double* v1, const int64_t* p1_1, const int64_t* p1_2, c
   dpd_tensor c = tensor2(p3_1, p3_2, iop3_12, sym3);
   engine("vv2v_dpd_442", a, v1, b, v2, c, v3,
          [&] { return dpd_vv2v_dpd_442(g_ctx, &a, type1, &b, type2, &c, type3, *fact1, *fact2); });
+}
+
+// libsrint/vv2v_dpd_323.f90:22-25 (CHARACTER*3 / *2 / *3 label strings)
+void vv2v_dpd_323_(const double* v1, const int64_t* p1_1, const int64_t* p1_2, const int64_t* iop1_12,
+                   const int64_t* p1_3, const int64_t* sym1, const char* type1, const double* v2,
+                   const int64_t* p2_1, const int64_t* p2_2, const int64_t* iop2_12, const int64_t* sym2,
+                   const char* type2, double* v3, const int64_t* p3_1, const int64_t* p3_2, const int64_t* iop3_12,
+                   const int64_t* p3_3, const int64_t* sym3, const char* type3, const double* fact1,
+                   const double* fact2, size_t, size_t, size_t) {
+  dpd_tensor a = tensor3(p1_1, p1_2, iop1_12, p1_3, sym1);
+  dpd_tensor b = tensor2(p2_1, p2_2, iop2_12, sym2);
+  dpd_tensor c = tensor3(p3_1, p3_2, iop3_12, p3_3, sym3);
+  engine("vv2v_dpd_323", a, v1, b, v2, c, v3,
+         [&] { return dpd_vv2v_dpd_323(g_ctx, &a, type1, &b, type2, &c, type3, *fact1, *fact2); });
 }
 
 // libsrint/xgemmso.f:1-25 (XGEMMSO, the prec/igpu router), column major

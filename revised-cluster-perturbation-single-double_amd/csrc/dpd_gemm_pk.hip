@@ -467,6 +467,149 @@ __device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask
       }
 }
 
+// ---------------------------------------------------------------------------
+// BF16x3 tile with the A operand read straight from HBM into registers
+// (GEMM_128x192D).  The wide-N shape of the contractions that stream a large
+// constant operand against a few vectors (the pp-ladder: A = <ab||ef> panels,
+// N = ij pairs x vectors) reads every A element once per N tile, so A is the
+// traffic.  Each of the 4 waves owns 32 rows of the 128-row tile and loads its
+// A fragments from the packed panel (16 rows x 32 k per MFMA tile, hi and lo
+// planes: one contiguous KiB each, the layout the pack kernel writes), so A
+// never passes through LDS.  Only B (192 x 32 per K step, hi and lo) is
+// staged, double buffered with one barrier per K step; the next step's A
+// fragments and B chunks are loaded while this step's 72 MFMAs per wave run.
+// Products are accumulated in the order of gemm_pk_tile (lo*hi, hi*lo, hi*hi).
+// ---------------------------------------------------------------------------
+template <int BN>
+__device__ __forceinline__ void gemm_pkd_tile(const GemmLaunch& g, const GemmTask& T, int local,
+                                              const void* __restrict__ PA, const void* __restrict__ PB,
+                                              double* __restrict__ C, const double beta, const bool c32) {
+  constexpr int BM = 128, KS = 32, LDK = KS + 8;  // LDS row = 40 bf16 = 80 bytes
+  constexpr int NT = BN / 16;                     // 16-wide column tiles per wave
+  constexpr int A_PANEL = BM * KS * 2;            // bf16 elements per packed A panel (hi + lo)
+  constexpr int B_PANEL = BN * KS * 2;
+  constexpr int CB = B_PANEL / 8 / kThreads;      // 16-byte chunks of B per thread per K step
+  static_assert(CB * 8 * kThreads == B_PANEL, "B panel / threads");
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][2 * BN * LDK];
+  __shared__ int64_t cm[BM], cn[BN];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tn = local % T.tiles_n;
+  local /= T.tiles_n;
+  const int tm = local % T.tiles_m;
+  const int kp = local / T.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int s0 = (int)((int64_t)kp * T.steps / T.ksplit);
+  const int s1 = (int)((int64_t)(kp + 1) * T.steps / T.ksplit);
+  const int nst = s1 - s0;
+  const uint16_t* pa = (const uint16_t*)PA + T.pa_off + ((int64_t)tm * T.steps + s0) * A_PANEL;
+  const u4_t* pb = (const u4_t*)((const uint16_t*)PB + T.pb_off + ((int64_t)tn * T.steps + s0) * B_PANEL);
+  // this lane's A fragment offset inside a panel: row (lane & 15) of the
+  // wave's 16-row tile t, k chunk 8 * (lane >> 4)
+  const int aoff = (32 * wave + (lane & 15)) * KS + 8 * (lane >> 4);
+  const int kq = 8 * (lane >> 4);
+
+  f4_t acc[2][NT];
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int y = 0; y < NT; y++) acc[t][y] = f4_t{0, 0, 0, 0};
+
+  bf8_t ah[2], al[2], nh[2], nl[2];
+  u4_t rb[CB];
+  auto load_a = [&](int step, bf8_t* h, bf8_t* l) {
+    const uint16_t* q = pa + (int64_t)step * A_PANEL + aoff;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      h[t] = *(const bf8_t*)(q + 16 * t * KS);
+      l[t] = *(const bf8_t*)(q + 16 * t * KS + BM * KS);
+    }
+  };
+  auto load_b = [&](int step) {
+    const u4_t* q = pb + (int64_t)step * (B_PANEL / 8);
+#pragma unroll
+    for (int j = 0; j < CB; j++) rb[j] = q[tid + j * kThreads];
+  };
+  auto store_b = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < CB; j++) {
+      const int e = (tid + j * kThreads) * 8;  // element of the panel: [plane][n][k]
+      const int plane = e / (BN * KS), rem = e % (BN * KS);
+      *(u4_t*)&Bs[buf][plane * BN * LDK + (rem / KS) * LDK + (rem % KS)] = rb[j];
+    }
+  };
+  if (nst > 0) {
+    load_b(0);
+    load_a(0, ah, al);
+    store_b(0);
+    __syncthreads();
+    for (int i = 0; i < nst; i++) {
+      const int buf = i & 1;
+      const bool more = i + 1 < nst;
+      if (more) {
+        load_b(i + 1);
+        load_a(i + 1, nh, nl);
+      }
+      const uint16_t* bs = Bs[buf];
+#pragma unroll
+      for (int y = 0; y < NT; y++) {
+        const bf8_t bh = *(const bf8_t*)&bs[(16 * y + (lane & 15)) * LDK + kq];
+        const bf8_t bl = *(const bf8_t*)&bs[BN * LDK + (16 * y + (lane & 15)) * LDK + kq];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          acc[t][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], bh, acc[t][y], 0, 0, 0);
+          acc[t][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bl, acc[t][y], 0, 0, 0);
+          acc[t][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bh, acc[t][y], 0, 0, 0);
+        }
+      }
+      if (more) {
+        store_b(buf ^ 1);  // the other buffer: every wave left it at the last barrier
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          ah[t] = nh[t];
+          al[t] = nl[t];
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  const bool split = T.ksplit > 1;
+  if (!split) {
+    for (int i = tid; i < BM + BN; i += kThreads) {
+      if (i < BM) {
+        const int m = m0 + i;
+        cm[i] = m < T.M ? digit_off(m, T.md, T.sc_m) : -1;
+      } else {
+        const int n = n0 + i - BM;
+        cn[i - BM] = n < T.N ? digit_off(n, T.nd, T.sc_n) : -1;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int y = 0; y < NT; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int ml = 32 * wave + 16 * t + 4 * (lane >> 4) + r;  // FP32 accumulator row
+        const int nl = 16 * y + (lane & 15);
+        const int m = m0 + ml, n = n0 + nl;
+        if (m >= T.M || n >= T.N) continue;
+        const double v = g.alpha * (double)acc[t][y][r];
+        if (split) {
+          g.partial[T.part_off + (int64_t)kp * T.M * T.N + m + (int64_t)n * T.M] = v;
+        } else {
+          const int64_t ad = T.c_off + cm[ml] + cn[nl];
+          if (c32)
+            ((float*)C)[ad] = (float)v;
+          else
+            C[ad] = (beta == 0.0) ? v : beta * C[ad] + v;
+        }
+      }
+}
+
 // XCD-aware order: consecutive workgroup ids (neighbouring tiles that share
 // panels) go to one XCD's L2 (dispatch deals workgroups round-robin over 8 XCDs)
 __device__ __forceinline__ int xcd_wg(int nwg) {
@@ -491,8 +634,29 @@ __global__ __launch_bounds__(256) void k_gemm_pk_grp(GemmLaunch g, GroupTab gt) 
   gemm_pk_tile<MODE, BM, BN>(g, T, wg - T.wg_begin, e.pa, e.pb, e.c, e.beta, e.c_f32 != 0);
 }
 
+template <int BN>
+__global__ __launch_bounds__(256, 2) void k_gemm_pkd(GemmLaunch g, const void* __restrict__ PA,
+                                                     const void* __restrict__ PB) {
+  const int wg = xcd_wg(g.nwg);
+  const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
+  gemm_pkd_tile<BN>(g, T, wg - T.wg_begin, PA, PB, g.C, g.beta, g.c_f32 != 0);
+}
+
+template <int BN>
+__global__ __launch_bounds__(256, 2) void k_gemm_pkd_grp(GemmLaunch g, GroupTab gt) {
+  const int wg = xcd_wg(g.nwg);
+  const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
+  const GroupEnt& e = gt.e[T.group];
+  gemm_pkd_tile<BN>(g, T, wg - T.wg_begin, e.pa, e.pb, e.c, e.beta, e.c_f32 != 0);
+}
+
 template <int MODE>
 static hipError_t launch_pk_grp_mode(const GemmLaunch& g, const GroupTab& gt, hipStream_t s) {
+  if (g.cfg == GEMM_128x192D) {
+    if (MODE != PREC_BF16X3) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_gemm_pkd_grp<192>), dim3(g.nwg), dim3(256), 0, s, g, gt);
+    return hipGetLastError();
+  }
   switch (g.cfg) {
     case GEMM_64x64:
       hipLaunchKernelGGL((k_gemm_pk_grp<MODE, 64, 64>), dim3(g.nwg), dim3(256), 0, s, g, gt);
@@ -540,6 +704,11 @@ hipError_t launch_gemm_pk_grouped(const GemmLaunch& g, const GroupTab& gt, hipSt
 
 template <int MODE>
 static hipError_t launch_pk_mode(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s) {
+  if (g.cfg == GEMM_128x192D) {
+    if (MODE != PREC_BF16X3) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_gemm_pkd<192>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
+    return hipGetLastError();
+  }
   switch (g.cfg) {
     case GEMM_64x64:
       hipLaunchKernelGGL((k_gemm_pk<MODE, 64, 64>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);

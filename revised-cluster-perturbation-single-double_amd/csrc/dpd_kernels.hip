@@ -739,8 +739,9 @@ __global__ __launch_bounds__(256) void k_orbital_energy(DevLayout L, const doubl
     int64_t x[4];
     decode_elem(L, o, h, x);
     double v;
-    if (mode == 1) {
+    if (mode == 1 || mode == 4) {
       v = e[off.offv[h[0]] + x[0]] - e[off.offo[h[1]] + x[1]];
+      if (mode == 4) v = -v;
     } else {
       const double ea = e[off.offv[h[0]] + x[0]], eb = e[off.offv[h[1]] + x[1]];
       const double ei = e[off.offo[h[2]] + x[2]], ej = e[off.offo[h[3]] + x[3]];
@@ -794,6 +795,19 @@ hipError_t launch_orbital_energy(const DevLayout& L, const double* eorb, const E
                                  double* out, hipStream_t s) {
   if (L.len == 0) return hipSuccess;
   hipLaunchKernelGGL(k_orbital_energy, dim3(grid_for(L.len)), dim3(256), 0, s, L, eorb, off, mode, out);
+  return hipGetLastError();
+}
+
+__global__ void k_div_shift(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double shift,
+                            double* __restrict__ c) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    c[i] = a[i] / (b[i] - shift);
+}
+
+hipError_t launch_div_shift(int64_t n, const double* a, const double* b, double shift, double* c, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_div_shift, dim3(grid_for(n)), dim3(256), 0, s, n, a, b, shift, c);
   return hipGetLastError();
 }
 

@@ -77,16 +77,19 @@ enum GemmCfg {
   GEMM_64x128 = 3,
   GEMM_128x128 = 4,
   GEMM_32x32 = 5,
-  GEMM_64x192 = 6
+  GEMM_64x192 = 6,
+  // BF16x3 packed-panel kernel with A fragments loaded straight into registers
+  // (other precisions take GEMM_64x192 in its place)
+  GEMM_128x192D = 7
 };
-constexpr int kGemmCfgMax = GEMM_64x192;
+constexpr int kGemmCfgMax = GEMM_128x192D;
 inline int gemm_bm(int cfg) {
-  return (cfg == GEMM_128x64 || cfg == GEMM_128x128) ? 128 : cfg == GEMM_32x32 ? 32 : 64;
+  return (cfg == GEMM_128x64 || cfg == GEMM_128x128 || cfg == GEMM_128x192D) ? 128 : cfg == GEMM_32x32 ? 32 : 64;
 }
 inline int gemm_bn(int cfg) {
   return (cfg == GEMM_64x128 || cfg == GEMM_128x128) ? 128
          : cfg == GEMM_32x32                         ? 32
-         : cfg == GEMM_64x192                        ? 192
+         : (cfg == GEMM_64x192 || cfg == GEMM_128x192D) ? 192
                                                      : 64;
 }
 
@@ -192,6 +195,7 @@ namespace dpd {
 // mode 1: e(a,i) = e_a - e_i        (layout (v,o,-,-))
 // mode 2: e(ab,ij) = e_a+e_b-e_i-e_j (layout (v,v,o,o))
 // mode 3: D(ab,ij) = e_i+e_j-e_a-e_b
+// mode 4: D(a,i) = e_i - e_a         (layout (v,o,-,-))
 struct EorbOff {
   int64_t offo[8], offv[8];
 };
@@ -206,6 +210,9 @@ hipError_t launch_eorb_adjoint(const DevLayout& L, const double* x, const double
 hipError_t launch_orbital_energy(const DevLayout& L, const double* eorb, const EorbOff& off, int mode,
                                  double* out, hipStream_t s);
 hipError_t launch_div(int64_t n, const double* num, const double* den, double* out, hipStream_t s);
+// out = num / (den - shift)
+hipError_t launch_div_shift(int64_t n, const double* num, const double* den, double shift, double* out,
+                            hipStream_t s);
 }  // namespace dpd
 
 namespace dpd {

@@ -181,13 +181,26 @@ static int sigma_terms(dpd_problem* p, const double* R, double* sigma, int64_t n
   if (p->method == 2) {
     Operand W = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "embj", 4, (const double*)p->iabj.p, F(p->iabj32)), p, 7);
     Operand ijkl = konst(op(mkdesc(o, o, o, o, 0, 0, 1), "mnij", 4, (const double*)p->ijkl.p, F(p->ijkl32)), p, 8);
-    Operand abcd = konst(op(mkdesc(v, v, v, v, 0, 0, 1), "abef", 4, (const double*)p->abcd.p, F(p->abcd32)), p, 9);
     // c2_r2Wmbej: vv2v_444(r2, W, c2, 'aeim','embj','abij', anti=3)
     con(444, lab(R2, "aeim", 4), W, lab(C2, "abij", 4), 1.0, 3, "c2_r2Wmbej");
     // c2_r2Wmnij: vv2v_444(r2, ijkl, c2, 'abmn','mnij','abij')
     con(444, lab(R2, "abmn", 4), ijkl, lab(C2, "abij", 4), 1.0, 0, "c2_r2Wmnij");
-    // c2_r2abcd: vv2v_444(abcd, r2, c2, 'abef','efij','abij')
-    con(444, abcd, lab(R2, "efij", 4), lab(C2, "abij", 4), 1.0, 0, "c2_r2abcd");
+    if (p->abcd_host) {
+      // c2_r2abcd with <ab||ef> streamed from host memory (c2_r2abcd.f90:29-104)
+      SigmaTerm t;
+      t.kind = 2;
+      t.x = R + p->l1;
+      t.y = sigma + p->l1;
+      t.nb = nb;
+      t.stride = p->dim;
+      t.fact = 1.0;
+      T.push_back(t);
+    } else {
+      Operand abcd =
+          konst(op(mkdesc(v, v, v, v, 0, 0, 1), "abef", 4, (const double*)p->abcd.p, F(p->abcd32)), p, 9);
+      // c2_r2abcd: vv2v_444(abcd, r2, c2, 'abef','efij','abij')
+      con(444, abcd, lab(R2, "efij", 4), lab(C2, "abij", 4), 1.0, 0, "c2_r2abcd");
+    }
   }
   return DPD_OK;
 }
@@ -209,6 +222,8 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
   for (const SigmaTerm& t : T) {
     if (t.kind == 0 || t.kind == 1)
       rc = run_simple_term(c, t);
+    else if (t.kind == 2)
+      rc = problem_pp_ladder_streamed(p, t.x, t.y, t.nb, t.stride, p->symR, t.fact);
     else
       rc = contract_call(c, t.kind, t.a, t.b, t.c, t.fact, t.anti, t.nb, p->prec, nullptr, t.tag);
     if (rc) return rc;
@@ -226,6 +241,7 @@ int problem_sigma_group(dpd_problem* const* ps, const double* const* R, double* 
   for (int i = 0; i < n; i++) {
     if (ps[i]->ctx != c || ps[i]->method != ps[0]->method || ps[i]->prec != ps[0]->prec)
       return fail(DPD_EINVAL, "grouped sigma: problems must share context, method and precision");
+    if (ps[i]->abcd_host) return fail(DPD_EINVAL, "grouped sigma: abcd out of core is not supported");
     int rc = sigma_terms(ps[i], R[i], S[i], nvec[i], T[i]);
     if (rc) return rc;
     if (T[i].size() != T[0].size()) return fail(DPD_EINVAL, "grouped sigma: term lists differ (internal)");
@@ -281,7 +297,7 @@ int problem_make_f32(dpd_problem* p) {
   }
   if (p->method == 2) {
     if ((rc = make_f32(p->ijkl32, p->ijkl, p->len_ijkl, s))) return rc;
-    if ((rc = make_f32(p->abcd32, p->abcd, p->len_abcd, s))) return rc;
+    if (!p->abcd_host && (rc = make_f32(p->abcd32, p->abcd, p->len_abcd, s))) return rc;
   }
   DPD_HIP_TRY(hipStreamSynchronize(s));
   return DPD_OK;
@@ -370,6 +386,7 @@ static Operand lab_of(Operand x, const char* l, int rank) {
 }
 
 int dpd::problem_sigma_t(dpd_problem* p, const double* L, double* t, int64_t nvec) {
+  if (p->abcd_host) return fail(DPD_EINVAL, "transposed operator: abcd out of core is not supported");
   dpd_ctx* c = p->ctx;
   const int64_t *v = p->popv, *o = p->popo, *z = p->zorb;
   const int S = p->symR;
@@ -713,6 +730,8 @@ int dpd_problem_sigma_t(dpd_problem* p, const double* L, double* t, int64_t nvec
 // named device tensors of a problem: the inputs (orbital energies and
 // integrals) and the intermediates finalize derives from them
 static int problem_tensor(dpd_problem* p, const char* name, DevBuf** buf, int64_t* len, bool* input) {
+  if (p->abcd_host && std::strcmp(name, "abcd") == 0)
+    return fail(DPD_EINVAL, "abcd is out of core (dpd_problem_set_abcd_out_of_core)");
   struct {
     const char* n;
     DevBuf* b;
@@ -799,6 +818,7 @@ int dpd_problem_grad(dpd_problem* p, const double* L, const double* R, double sc
                      double* abcd_bar) {
   if (!p || !L || !R) return fail(DPD_EINVAL, "dpd_problem_grad: null argument");
   if (p->stale) return fail(DPD_EINVAL, "problem inputs changed: call dpd_problem_refresh first");
+  if (p->abcd_host) return fail(DPD_EINVAL, "gradient: abcd out of core is not supported");
   DPD_HIP_TRY(hipSetDevice(p->ctx->device));
   GradOut g;
   g.eorb = eorb_bar;

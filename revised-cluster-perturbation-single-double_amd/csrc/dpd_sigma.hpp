@@ -1,6 +1,7 @@
 // dpd_sigma.hpp -- device-resident problem (integrals, intermediates, diagonal)
 // and the batched sigma builders.
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "dpd_engine.hpp"
@@ -23,6 +24,15 @@ struct dpd_problem {
   int64_t sigma_flops = 0;  // algorithmic GEMM flops of one sigma (one vector)
   uint64_t uid = 0;         // process-unique id: constant tensors are packed once under it
   bool stale = false;       // an input was set after the last finalize
+  // <ab||ef> out of core: a pinned host copy streamed irrep block by irrep block
+  // through two device staging buffers (abcd itself is then freed)
+  double* abcd_host = nullptr;
+  dpd::DevBuf abcd_stage[2];
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
+  std::vector<std::unique_ptr<dpd::GemmPlan>> ooc_plans;  // per (nvec, stride, symmetry, block)
+  std::vector<int64_t> ooc_keys;
+  ~dpd_problem();
 };
 
 namespace dpd {
@@ -31,4 +41,9 @@ int problem_make_f32(dpd_problem* p);
 int problem_sigma_t(dpd_problem* p, const double* L, double* t, int64_t nvec);
 int problem_sigma_group(dpd_problem* const* ps, const double* const* R, double* const* S, const int64_t* nvec,
                         int n);
+// c2_r2abcd (pp-ladder) with <ab||ef> streamed from the pinned host copy:
+// C2(ab,ij) += fact * sum_{e<f} <ab||ef> R2(ef,ij) for nvec vectors of symmetry
+// sym (R2 / C2 of vector v at R2 + v*stride / C2 + v*stride)
+int problem_pp_ladder_streamed(dpd_problem* p, const double* R2, double* C2, int64_t nvec, int64_t stride, int sym,
+                               double fact);
 }

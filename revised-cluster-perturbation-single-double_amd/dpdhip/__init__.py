@@ -154,6 +154,14 @@ def vv2v_dpd_444(v1: Opt, t1, v2: Opt, t2, v3: Opt, t3, fact1, fact2):
                                  _lab(t3), fact1, fact2))
 
 
+def vv2v_dpd_323(v1: Opt, t1, v2: Opt, t2, v3: Opt, t3, fact1, fact2):
+    """libsrint/vv2v_dpd_323.f90: v3(pqr) = fact1*v1(pqm)v2(mr) + fact2*v3; the
+    rank-3 Opts use popp, popq, popr and ioppq, v2 popp, popq"""
+    a, b, c = v1._struct(), v2._struct(), v3._struct()
+    check(lib().dpd_vv2v_dpd_323(v1.ctx.h, ctypes.byref(a), _lab(t1), ctypes.byref(b), _lab(t2), ctypes.byref(c),
+                                 _lab(t3), fact1, fact2))
+
+
 def vv2v_dpd_424(v1: Opt, t1, v2: Opt, t2, v3: Opt, t3, fact1, fact2):
     a, b, c = v1._struct(), v2._struct(), v3._struct()
     check(lib().dpd_vv2v_dpd_424(v1.ctx.h, ctypes.byref(a), _lab(t1), ctypes.byref(b), _lab(t2), ctypes.byref(c),
@@ -510,23 +518,66 @@ class Problem:
         return R, e
 
     def davidson(self, nroot, conv=1e-8, max_dim=20, max_cyc=50, lowest=False, final_check=True, R=None,
-                 transpose=False):
+                 transpose=False, graph=False):
         """Non-Hermitian Davidson (lib_cc/mod_davidson.f90:40-561); transpose=True
-        solves with J^T (left eigenvectors)."""
+        solves with J^T (left eigenvectors); graph=True replays each iteration's
+        sigma step from a hipGraph."""
         if R is None:
             R, _ = self.guess(nroot)
         self._vectors(R, "R", nroot)
         o = _lib.DavidsonOpts_()
         o.nroot, o.max_dim, o.max_cyc, o.conv = nroot, max_dim, max_cyc, conv
         o.lowest, o.final_check, o.transpose = int(lowest), int(final_check), int(transpose)
+        o.graph = int(graph)
         w = np.zeros(nroot)
         rc = lib().dpd_davidson(self.h, ctypes.byref(o), R.data_ptr(), w.ctypes.data_as(_lib.c_double_p))
         if rc not in (0, _lib.DPD_ENOTCONV):
             check(rc)
         info = {"converged": rc == 0, "iterations": o.iterations, "sigma_calls": o.sigma_calls,
-                "skipped": o.skipped,
+                "skipped": o.skipped, "graph_replays": o.graph_replays,
                 "residual": list(o.residual[:nroot]), "final_residual": list(o.final_residual[:nroot])}
         return w, R, info
+
+    def set_abcd_out_of_core(self, on=True):
+        """keep <ab||ef> in pinned host memory and stream it block by block into
+        the pp-ladder (the reference's out-of-core c2_r2abcd, c2_r2abcd.f90:29-104);
+        on=False brings it back to the device"""
+        check(lib().dpd_problem_set_abcd_out_of_core(self.h, int(bool(on))))
+
+    def cpsd(self, diis_dim=10, diis_cyc=50, diis_max_err=1e-7):
+        """Ground-state CPS(D) (CPSd.SF/run.f90): returns (T, info) with T the
+        amplitudes of orders 1-3 as a (3, dpd_problem_cpsd_len) device tensor,
+        each row (T1 | T2(a<b,i<j)) of the totally symmetric irrep, and info the
+        correlation energies E^(2), E^(3), E^(4) and the DIIS iteration counts."""
+        torch = _torch()
+        n = int(lib().dpd_problem_cpsd_len(self.h))
+        T = torch.empty(3 * n, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        o = _lib.CpsdOpts_()
+        o.diis_dim, o.diis_cyc, o.diis_max_err = diis_dim, diis_cyc, diis_max_err
+        check(lib().dpd_problem_cpsd(self.h, ctypes.byref(o), T.data_ptr()))
+        return T.view(3, n), {"energy": list(o.energy), "iterations": list(o.iterations)}
+
+    def eecpsd(self, T, w0, L, diis_dim=10, diis_cyc=50, diis_max_err=1e-7):
+        """EE-CPS(D) excitation energies (EE-CPSd.SF/run_order.f90) of CIS roots
+        of this problem's symmetry: w0 (nroot CIS energies), L (nroot CIS vectors
+        of length |R1|, device), T from cpsd().  Returns (w, iterations) with w of
+        shape (nroot, 3): w^(1), w^(2), w^(3); the total is w0 + w^(2) + w^(3)."""
+        torch = _torch()
+        w0 = np.ascontiguousarray(np.atleast_1d(np.asarray(w0, dtype=np.float64)))
+        nroot = len(w0)
+        lay = self.spec.layout()
+        l1 = self.ctx.irpdso(self.symR, lay["popv"], lay["popo"], 1)
+        _check_vectors(L, "L", l1, self.ctx.device, nroot)
+        n = int(lib().dpd_problem_cpsd_len(self.h))
+        _check_vectors(T, "T", 3 * n, self.ctx.device, 1)
+        o = _lib.CpsdOpts_()
+        o.diis_dim, o.diis_cyc, o.diis_max_err = diis_dim, diis_cyc, diis_max_err
+        w = np.zeros(3 * nroot)
+        its = np.zeros(nroot, dtype=np.int64)
+        check(lib().dpd_problem_eecpsd(self.h, ctypes.byref(o), T.data_ptr(), nroot,
+                                       w0.ctypes.data_as(_lib.c_double_p), L.data_ptr(),
+                                       w.ctypes.data_as(_lib.c_double_p), its.ctypes.data_as(_lib.c_i64_p)))
+        return w.reshape(nroot, 3), its
 
     def close(self):
         # a problem lives in its context: once the context was closed explicitly

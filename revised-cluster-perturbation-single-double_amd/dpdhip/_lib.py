@@ -43,7 +43,14 @@ class DavidsonOpts_(ctypes.Structure):
     _fields_ = [("nroot", i64), ("max_dim", i64), ("max_cyc", i64), ("conv", ctypes.c_double),
                 ("lowest", ctypes.c_int), ("final_check", ctypes.c_int), ("transpose", ctypes.c_int),
                 ("iterations", i64),
-                ("sigma_calls", i64), ("skipped", i64), ("residual", ctypes.c_double * 64), ("final_residual", ctypes.c_double * 64)]
+                ("sigma_calls", i64), ("skipped", i64), ("residual", ctypes.c_double * 64), ("final_residual", ctypes.c_double * 64),
+                ("graph", ctypes.c_int), ("graph_replays", i64)]
+
+
+class CpsdOpts_(ctypes.Structure):
+    """dpd_cpsd_opts"""
+    _fields_ = [("diis_dim", i64), ("diis_cyc", i64), ("diis_max_err", ctypes.c_double),
+                ("energy", ctypes.c_double * 3), ("iterations", i64 * 2)]
 
 
 _lib = None
@@ -67,6 +74,9 @@ _SIGS = {
     "dpd_getidx": (ctypes.c_int, [c_void_p, i64, c_i64_p, ctypes.POINTER(Tensor_)]),
     "dpd_putidx": (i64, [c_void_p, c_i64_p, ctypes.POINTER(Tensor_)]),
     "dpd_vv2v_dpd_444": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.POINTER(Tensor_),
+                                        ctypes.c_char_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.c_double,
+                                        ctypes.c_double]),
+    "dpd_vv2v_dpd_323": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.POINTER(Tensor_),
                                         ctypes.c_char_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.c_double,
                                         ctypes.c_double]),
     "dpd_vv2v_dpd_424": (ctypes.c_int, [c_void_p, ctypes.POINTER(Tensor_), ctypes.c_char_p, ctypes.POINTER(Tensor_),
@@ -132,6 +142,24 @@ _SIGS = {
     "dpd_problem_refresh": (ctypes.c_int, [c_void_p]),
     "dpd_problem_grad": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_double] + [c_void_p] * 7),
     "dpd_problem_sigma_group": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_i64_p, i64]),
+    "dpd_node_init": (ctypes.c_int, [ctypes.c_char_p]),
+    "dpd_node_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char, c_void_p, i64]),
+    "dpd_node_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char, c_void_p, i64]),
+    "dpd_node_size": (i64, [ctypes.c_char_p, ctypes.c_char_p]),
+    "dpd_moio_write": (ctypes.c_int, [c_void_p, ctypes.c_char_p, i64, c_i64_p, c_i64_p, i64, c_i64_p, c_i64_p, i64,
+                                      c_void_p]),
+    "dpd_moio_info": (ctypes.c_int, [ctypes.c_char_p, c_i64_p, c_i64_p, c_i64_p, c_i64_p]),
+    "dpd_moio_read": (ctypes.c_int, [ctypes.c_char_p, c_void_p, i64]),
+    "dpd_stream_write": (ctypes.c_int, [ctypes.c_char_p, i64, ctypes.c_double, c_void_p, i64, ctypes.c_int,
+                                        ctypes.c_int]),
+    "dpd_stream_read": (ctypes.c_int, [ctypes.c_char_p, i64, c_double_p, c_void_p, i64, ctypes.c_int]),
+    "dpd_eigvec_write": (ctypes.c_int, [ctypes.c_char_p, i64, i64, i64, c_double_p, c_void_p, ctypes.c_int]),
+    "dpd_eigvec_read": (ctypes.c_int, [ctypes.c_char_p, i64, i64, i64, c_double_p, c_void_p]),
+    "dpd_problem_set_abcd_out_of_core": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "dpd_problem_cpsd_len": (i64, [c_void_p]),
+    "dpd_problem_cpsd": (ctypes.c_int, [c_void_p, ctypes.POINTER(CpsdOpts_), c_void_p]),
+    "dpd_problem_eecpsd": (ctypes.c_int, [c_void_p, ctypes.POINTER(CpsdOpts_), c_void_p, i64, c_double_p, c_void_p,
+                                          c_double_p, c_i64_p]),
     "dpd_davidson_batch": (ctypes.c_int, [c_void_p, i64, ctypes.POINTER(DavidsonOpts_), c_void_p, c_void_p, i64,
                                           c_i64_p]),
     "dpd_small_eig": (ctypes.c_int, [i64, c_double_p, c_double_p, c_double_p, c_double_p]),

@@ -153,3 +153,27 @@ def test_block_davidson_16_roots(torch):
     assert info["iterations"] == ref["iterations"]
     assert np.max(np.abs(w - ref["w"])) <= 1e-8
     assert max(info["final_residual"]) < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_davidson_graph_replay_equals_eager(torch, prec):
+    """graph=True replays each iteration's sigma step from a hipGraph: the same
+    vectors go through the same kernels, so eigenvalues, iteration count and
+    eigenvectors equal the eager solve's, and most steps are replays"""
+    sp, op, gp = make(C2V24, "sf-cpsd", 1)
+    gp.set_precision(prec)
+    try:
+        kw = dict(conv=1e-7 if prec == "fp64" else 1e-5, max_dim=20, max_cyc=100, lowest=True, final_check=False)
+        g, _ = gp.guess(3)
+        w0, R0, i0 = gp.davidson(3, R=g.clone(), **kw)
+        print(f"[graph] {prec} eager {i0['iterations']} iterations", flush=True)
+        w1, R1, i1 = gp.davidson(3, R=g.clone(), graph=True, **kw)
+        print(f"[graph] {prec} graph {i1['iterations']} iterations, {i1['graph_replays']} replays", flush=True)
+    finally:
+        gp.set_precision("fp64")
+    assert i0["converged"] and i1["converged"]
+    assert i1["iterations"] == i0["iterations"]
+    assert np.array_equal(w1, w0)
+    assert torch.equal(R1, R0)
+    assert i1["graph_replays"] >= i1["iterations"] // 2
+    gp.close()

@@ -45,9 +45,9 @@ def ctx_for(n):
 # kernel; gathering kernel with 64x64 / 128x64 / 64x128 / 128x128 tiles; the
 # packed-panel kernel with automatic and with each fixed tile shape
 GEMM_CFGS = [(-1, 1), (-2, 1), (0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (-1, 2), (1, 2), (2, 2), (3, 2),
-             (4, 2), (5, 2), (6, 2)]
+             (4, 2), (5, 2), (6, 2), (7, 2)]
 GEMM_IDS = ["auto", "model", "v1", "g64x64", "g128x64", "g64x128", "g128x128", "g32x32", "pk_auto", "pk64x64",
-            "pk128x64", "pk64x128", "pk128x128", "pk32x32", "pk64x192"]
+            "pk128x64", "pk64x128", "pk128x128", "pk32x32", "pk64x192", "pk128x192d"]
 
 
 @pytest.fixture(params=GEMM_CFGS, ids=GEMM_IDS)
@@ -231,10 +231,11 @@ def _surface_case(torch, sym, kind, ops, t, fact, anti, seed=1):
     x1, x2, x3 = O.random_dpd(sym, d1, rng), O.random_dpd(sym, d2, rng), O.random_dpd(sym, d3, rng)
     v3 = O.Opt(d3, x3.copy())
     t1, t2, t3 = t
-    fo = {"444": O.vv2v_444, "424": O.vv2v_424, "442": O.vv2v_442, "222": O.vv2v_222, "422": O.vv2v_422}[kind]
+    fo = {"444": O.vv2v_444, "424": O.vv2v_424, "442": O.vv2v_442, "222": O.vv2v_222, "422": O.vv2v_422,
+          "224": O.vv2v_224}[kind]
     fg = {"444": dpdhip.vv2v_444, "424": dpdhip.vv2v_424, "442": dpdhip.vv2v_442, "222": dpdhip.vv2v_222,
-          "422": dpdhip.vv2v_422}[kind]
-    extra = (anti,) if kind in ("444", "424") else ()
+          "422": dpdhip.vv2v_422, "224": dpdhip.vv2v_224}[kind]
+    extra = (anti,) if kind in ("444", "424", "224") else ()
     fo(O.Opt(d1, x1), O.Opt(d2, x2), v3, fact, t1, t2, t3, *extra)
     g3 = gopt(torch, sym, d3, x3)
     fg(gopt(torch, sym, d1, x1), gopt(torch, sym, d2, x2), g3, fact, t1, t2, t3, *extra)
@@ -274,6 +275,55 @@ def test_contract_surface_sigma_calls(torch, sym, large, gemm_cfg):
     _surface_case(torch, sym, "442", (t2, t2, fab), ("afmn", "efmn", "ae"), -1.0, 0)
     _surface_case(torch, sym, "442", (t2, t2, fij), ("efmn", "efin", "mi"), 1.0, 0)
     _surface_case(torch, sym, "444", (t2, t2, iabj), ("fenm", "bfjn", "embj"), 1.0, 0)
+
+
+def test_contract_surface_224(torch, sym):
+    """mod_contract.f90:203-219 (vv2v_224): two 2-index tensors into a 4-index
+    one, r1(a,i) r1(b,j) -> c2(ab,ij) with every antisymmetriser, and
+    f(a,b) t(i,j) -> x(ab,ij) into a packed and an unpacked output"""
+    V, Oc = spaces(sym, False)
+    Z = O.ZORB
+    for S1, S2 in ((1, 1), (2, 3)):
+        S3 = O.xor_dirprd(sym.nirrep)[S1 - 1][S2 - 1]
+        r1a = O.Desc([V, Oc, Z, Z], (1, 1), S1)
+        r1b = O.Desc([V, Oc, Z, Z], (1, 1), S2)
+        for anti, iop in ((0, 1), (1, 1), (2, 1), (3, 0), (3, 1)):
+            c2 = O.Desc([V, V, Oc, Oc], (iop, iop), S3)
+            _surface_case(torch, sym, "224", (r1a, r1b, c2), ("ai", "bj", "abij"), 0.5, anti)
+        fab = O.Desc([V, V, Z, Z], (1, 1), S1)
+        tij = O.Desc([Oc, Oc, Z, Z], (1, 1), S2)
+        x = O.Desc([V, V, Oc, Oc], (1, 1), S3)
+        _surface_case(torch, sym, "224", (fab, tij, x), ("ab", "ij", "abij"), -1.25, 0)
+
+
+def test_engine_323(torch, sym):
+    """vv2v_dpd_323 (libsrint/vv2v_dpd_323.f90) on the seven cases of its harness
+    (vv2v_dpd_323_check.f90:98-280) against the oracle, and its error codes"""
+    import dpdhip
+    from test_oracle_engine import _cases323, _desc3
+    V, Oc = spaces(sym, False)
+    for case, (t1, p1, t2, p2, t3, p3, iop, f2) in enumerate(_cases323(V, Oc)):
+        for s1, s2 in ((1, 1), (2, 3)):
+            s3 = O.xor_dirprd(sym.nirrep)[s1 - 1][s2 - 1]
+            rng = np.random.default_rng(case * 10 + s1)
+            d1 = _desc3(p1, iop if t1 == "pqm" else 1, s1)
+            d2 = O.desc2(p2[0], p2[1], 1, s2)
+            d3 = _desc3(p3, iop if t3 == "pqr" else 1, s3)
+            x1, x2, x3 = (O.random_dpd(sym, d, rng) for d in (d1, d2, d3))
+            ref = x3.copy()
+            O.vv2v_dpd_323(x1, d1, t1, x2, d2, t2, ref, d3, t3, 1.5, f2)
+            g1, g2, g3 = gopt(torch, sym, d1, x1), gopt(torch, sym, d2, x2), gopt(torch, sym, d3, x3)
+            dpdhip.vv2v_dpd_323(g1, t1, g2, t2, g3, t3, 1.5, f2)
+            np.testing.assert_allclose(g3.numpy(), ref, rtol=RTOL, atol=ATOL, err_msg=f"case {case}")
+    d1 = _desc3((V, Oc, Oc), 1, 1)
+    d2 = O.desc2(Oc, V, 1, 1)
+    d3 = _desc3((V, Oc, V), 1, 1)
+    g1, g2, g3 = (gopt(torch, sym, d, np.zeros(d.length())) for d in (d1, d2, d3))
+    with pytest.raises(dpdhip.DpdError):  # the contracted index kept in v3
+        dpdhip.vv2v_dpd_323(g1, "pqm", g2, "mr", g3, "pqm", 1.0, 0.0)
+    gp = gopt(torch, sym, O.desc2(Oc, V, 0, 1), np.zeros(O.desc2(Oc, V, 0, 1).length()))
+    with pytest.raises(dpdhip.DpdError):  # v2 packed
+        dpdhip.vv2v_dpd_323(g1, "pqm", gp, "mr", g3, "pqr", 1.0, 0.0)
 
 
 def test_sort_kernels(torch, sym):

@@ -138,6 +138,20 @@ def test_fortran_engines(torch, fsym, where):
                     *_t2(a, V, Oc, 1, 3, h3[1]), a.s("ai"), a.d(-1.0), a.d(0.0),
                     ctypes.c_size_t(4), ctypes.c_size_t(4), ctypes.c_size_t(2))
     np.testing.assert_allclose(_back(h3[0]), ref, rtol=RTOL, atol=ATOL)
+    # 323: v(q,p,m) w(r,m) -> x(p,q,r) (the harness's scramble case)
+    d1 = O.Desc([Oc, V, V, O.ZORB], (1, 1), 2)
+    d2 = O.desc2(Oc, V, 1, 3)
+    d3 = O.Desc([V, Oc, Oc, O.ZORB], (1, 1), O.xor_dirprd(fsym.nirrep)[1][2])
+    x1, x2, x3 = (O.random_dpd(fsym, di, rng) for di in (d1, d2, d3))
+    ref = x3.copy()
+    O.vv2v_dpd_323(x1, d1, "qpm", x2, d2, "rm", ref, d3, "pqr", 0.5, 0.25)
+    a = Args()
+    h1, h2, h3 = (_ptr(torch, xi.copy(), where) for xi in (x1, x2, x3))
+    L.vv2v_dpd_323_(h1[1], a.pop(Oc), a.pop(V), a.i(1), a.pop(V), a.i(2), a.s("qpm"),
+                    *_t2(a, Oc, V, 1, 3, h2[1]), a.s("rm"),
+                    h3[1], a.pop(V), a.pop(Oc), a.i(1), a.pop(Oc), a.i(d3.sym), a.s("pqr"), a.d(0.5), a.d(0.25),
+                    ctypes.c_size_t(3), ctypes.c_size_t(2), ctypes.c_size_t(3))
+    np.testing.assert_allclose(_back(h3[0]), ref, rtol=RTOL, atol=ATOL)
 
 
 @pytest.mark.parametrize("where", ["host", "device"])

@@ -143,3 +143,43 @@ def test_sigma_split_k_off_with_autotuned_tiles(torch, prec):
     for g, r in zip(got_grouped, ref_grouped):
         np.testing.assert_allclose(g, r, rtol=0, atol=tol * np.abs(r).max())
     np.testing.assert_allclose(got_grouped[0], ref_single, rtol=0, atol=tol * scale)
+
+
+@pytest.mark.parametrize("split", [0, -1])
+def test_bf16x3_direct_a_tile_equals_staged_tile(torch, split):
+    """The BF16x3 tile that loads A straight into registers (GEMM config 7,
+    128x192) accumulates every output element over the same K steps in the same
+    MFMA order as the LDS-staged 128x64 tile (config 2): with split-K off the
+    sigma vectors are bit-identical, single and grouped; with split-K automatic
+    they agree to FP32 accumulation accuracy."""
+    import dpdhip
+    sp1 = dpdhip.Spec(SMALL_D2H["nso"], SMALL_D2H["nalpha"], SMALL_D2H["nbeta"], SMALL_D2H["seed"])
+    sp2 = dpdhip.Spec((4, 3, 3, 2, 3, 2, 2, 1), 4, 2, 9)
+    out = {}
+    for cfg in (2, 7):
+        print(f"[direct-a] split={split} cfg={cfg}", flush=True)
+        c = dpdhip.Context(8, 0)
+        c.set_gemm_config(cfg, 1)
+        c.set_split_k(split)
+        p1 = dpdhip.Problem(c, sp1, "sf-cpsd", 1)
+        p2 = dpdhip.Problem(c, sp2, "sf-cpsd", 1)
+        for p in (p1, p2):
+            p.set_precision("bf16x3")
+        rng = np.random.default_rng(11)
+        R1 = torch.from_numpy(rng.standard_normal(5 * p1.dim)).cuda()
+        R2 = torch.from_numpy(rng.standard_normal(2 * p2.dim)).cuda()
+        single = p1.sigma(R1).cpu().numpy()
+        grouped = [x.cpu().numpy() for x in dpdhip.sigma_group([p1, p2], [R1, R2])]
+        out[cfg] = (single, grouped)
+        p1.close()
+        p2.close()
+        c.close()
+    (s2, g2), (s7, g7) = out[2], out[7]
+    if split == 0:
+        assert np.array_equal(s7, s2)
+        for a, b in zip(g7, g2):
+            assert np.array_equal(a, b)
+    else:
+        np.testing.assert_allclose(s7, s2, rtol=0, atol=1e-5 * np.abs(s2).max())
+        for a, b in zip(g7, g2):
+            np.testing.assert_allclose(a, b, rtol=0, atol=1e-5 * np.abs(b).max())

@@ -146,7 +146,23 @@ def _run_surface(sym, kind, ops, t, fact, anti, half, seed=1):
         O.vv2v_222(v1, v2, v3, fact, t1, t2, t3)
     elif kind == "422":
         O.vv2v_422(v1, v2, v3, fact, t1, t2, t3)
+    elif kind == "224":
+        O.vv2v_224(v1, v2, v3, fact, t1, t2, t3, anti)
     np.testing.assert_allclose(v3.arr, exp, rtol=1e-12, atol=1e-12)
+
+
+def test_contract_surface_224(sym):
+    """mod_contract.f90:203-219: outer products of two 2-index tensors into a
+    4-index one with P(pq)/P(rs) antisymmetrisers, packed and unpacked outputs"""
+    V, Oc = _spaces(sym)
+    Z = O.ZORB
+    for S1, S2 in ((1, 1), (2, 3)):
+        S3 = O.xor_dirprd(sym.nirrep)[S1 - 1][S2 - 1]
+        r1a = O.Desc([V, Oc, Z, Z], (1, 1), S1)
+        r1b = O.Desc([V, Oc, Z, Z], (1, 1), S2)
+        for anti, iop in ((0, 1), (1, 1), (2, 1), (3, 0), (3, 1)):
+            c2 = O.Desc([V, V, Oc, Oc], (iop, iop), S3)
+            _run_surface(sym, "224", (r1a, r1b, c2), ("ai", "bj", "abij"), 0.5, anti, 1.0)
 
 
 def test_sigma_contractions(sym):
@@ -203,3 +219,62 @@ def test_424_442_error_paths(sym):
         O.vv2v_424(O.Opt(abci), O.Opt(r1), O.Opt(r2), 1.0, "abej", "ei", "abij", 2)
     with pytest.raises(O.OracleError):  # anti out of range
         O.vv2v_444(O.Opt(r2), O.Opt(r2), O.Opt(r2), 1.0, "abef", "efij", "abij", 4)
+
+
+# libsrint/vv2v_dpd_323_check.f90:98-280, the seven cases of the harness:
+# (v1 labels, v1 pops, v2 labels, v2 pops, v3 labels, v3 pops, pair iop, fact2)
+def _cases323(V, Oc):
+    P, Q, M, R = V, Oc, Oc, V
+    return [
+        ("pqm", (P, Q, M), "mr", (M, R), "pqr", (P, Q, R), 1, 0.0),   # standard
+        ("pqm", (P, Q, M), "rm", (R, M), "pqr", (P, Q, R), 1, 0.0),   # swapped 2-index
+        ("qpm", (Q, P, M), "rm", (R, M), "pqr", (P, Q, R), 1, 0.0),   # extreme scramble
+        ("pqm", (P, Q, M), "mr", (M, R), "pqr", (P, Q, R), 1, 0.5),   # accumulation
+        ("pqm", (P, P, M), "mr", (M, R), "pqr", (P, P, R), 0, 0.0),   # intact pair, iop 0
+        ("pqm", (P, P, M), "mr", (M, R), "pqr", (P, P, R), 2, 0.0),   # intact pair, iop 2
+        ("pqm", (P, Q, M), "mr", (M, R), "rpq", (R, P, Q), 1, 0.0),   # output scramble
+    ]
+
+
+def _desc3(pops, iop, sym):
+    return O.Desc([pops[0], pops[1], pops[2], O.ZORB], (iop, 1), sym)
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_323_harness_cases(sym, case):
+    """vv2v_dpd_323 against dense einsum 'pqm,mr->pqr' for every harness case and
+    two symmetry combinations"""
+    V, Oc = _spaces(sym)
+    t1, p1, t2, p2, t3, p3, iop, f2 = _cases323(V, Oc)[case]
+    for s1, s2 in ((1, 1), (2, 3)):
+        s3 = O.xor_dirprd(sym.nirrep)[s1 - 1][s2 - 1]
+        rng = np.random.default_rng(case * 10 + s1)
+        d1 = _desc3(p1, iop if t1 == "pqm" else 1, s1)
+        d2 = O.desc2(p2[0], p2[1], 1, s2)
+        d3 = _desc3(p3, iop if t3 == "pqr" else 1, s3)
+        x1, x2, x3 = (O.random_dpd(sym, d, rng, antisym=True) for d in (d1, d2, d3))
+        D1 = O.unpack(sym, d1, x1)[:, :, :, 0]
+        D2 = O.unpack(sym, d2, x2)[:, :, 0, 0]
+        D3 = O.unpack(sym, d3, x3)[:, :, :, 0]
+        E = 1.5 * np.einsum(f"{t1},{t2}->{t3}", D1, D2) + f2 * D3
+        exp = O.pack(sym, d3, E[:, :, :, None])
+        out = x3.copy()
+        O.vv2v_dpd_323(x1, d1, t1, x2, d2, t2, out, d3, t3, 1.5, f2)
+        np.testing.assert_allclose(out, exp, rtol=1e-12, atol=1e-12)
+
+
+def test_323_error_paths(sym):
+    V, Oc = _spaces(sym)
+    d1 = _desc3((V, Oc, Oc), 1, 1)
+    d2 = O.desc2(Oc, V, 1, 1)
+    d3 = _desc3((V, Oc, V), 1, 1)
+    z = [np.zeros(d.length() + 1) for d in (d1, d2, d3)]
+    with pytest.raises(O.OracleError):  # contracted index kept in v3
+        O.vv2v_dpd_323(z[0], d1, "pqm", z[1], d2, "mr", z[2], d3, "pqm", 1.0, 0.0)
+    with pytest.raises(O.OracleError):  # v2 packed
+        O.vv2v_dpd_323(z[0], d1, "pqm", z[1], O.desc2(Oc, V, 0, 1), "mr", z[2], d3, "pqr", 1.0, 0.0)
+    dp = _desc3((V, V, Oc), 0, 1)
+    dq = _desc3((V, V, V), 0, 1)
+    with pytest.raises(O.OracleError):  # reordered pair with iop 0
+        O.vv2v_dpd_323(np.zeros(dp.length() + 1), dp, "pqm", z[1], O.desc2(Oc, V, 1, 1), "mr",
+                       np.zeros(dq.length() + 1), dq, "qpr", 1.0, 0.0)
