@@ -191,7 +191,9 @@ def main():
     # dominant GEMM kernel of the measured mixed run; it is MFMA-bound when its
     # algorithmic intensity (flops per algorithmic byte) exceeds the ridge point
     # peak_flops / 8 TB/s, else HBM-bound
-    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    gemm_prof = {k: v for k, v in prof.items() if v.get("kind", "gemm") == "gemm"}
+    gather_prof = {k: v for k, v in prof.items() if v.get("kind") == "gather"}
+    dom = max(gemm_prof.items(), key=lambda kv: kv[1]["ms"])
     tag, st = dom
     avg_ms = st["ms"] / st["count"]
     flops_per_launch = st["flops_total"] / st["count"]
@@ -216,10 +218,13 @@ def main():
         # (at C2v/180 one oracle sigma alone exceeds the bounded sample)
         cpu = cpu_baseline(cfg, args, info64["sigma_calls"], lowest)
 
-    tiles = {1: (64, 64), 2: (128, 64), 3: (64, 128), 4: (128, 128), 5: (32, 32), 6: (64, 192)}
+    tiles = {1: (64, 64), 2: (128, 64), 3: (64, 128), 4: (128, 128), 5: (32, 32), 6: (64, 192), 7: (128, 192)}
     bm, bn = tiles.get(st.get("cfg", 1), (64, 64))
     mode = {0: 0, 1: 1, 2: 2}[st["prec"]]
-    kname = f"k_gemm_pk<{mode}, {bm}, {bn}>" if st.get("packed", 1) else f"k_gemm2<{bm}, {bn}>"
+    if st.get("packed", 1):
+        kname = "k_gemm_pkd<192>" if st.get("cfg") == 7 else f"k_gemm_pk<{mode}, {bm}, {bn}>"
+    else:
+        kname = f"k_gemm2<{bm}, {bn}>"
     line = {
         "metric": f"Davidson time to {nroot} roots at <=1e-6 Eh: mixed-prec MFMA vs FP64 GPU"
         if nroot != 5 else "Davidson time to 5 roots at <=1e-6 Eh: mixed-prec MFMA vs FP64 GPU",
@@ -265,7 +270,16 @@ def main():
                      "avg_launch_ms": round(avg_ms, 5), "launches": st["count"]},
         "kernels": {k: {"launches": v["count"], "ms_total": round(v["ms"], 3),
                         "tflops": round(v["flops_total"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] > 0 else None}
-                    for k, v in prof.items()},
+                    for k, v in gemm_prof.items()},
+        # expansions (x1/x2) and epilogues (epi) of each tagged contraction:
+        # algorithmic bytes (input once, output once, twice when accumulated)
+        # over the event-timed duration
+        "gathers": {k: {"launches": v["count"], "ms_total": round(v["ms"], 3),
+                        "gbs": round(v["bytes_total"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+                    for k, v in gather_prof.items()},
+        "gathers_total": {"ms_per_solve": round(sum(v["ms"] for v in gather_prof.values()) / max(args.steps, 1), 3),
+                          "gbs": round(sum(v["bytes_total"] for v in gather_prof.values()) /
+                                       max(sum(v["ms"] for v in gather_prof.values()) * 1e-3, 1e-12) / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -6,8 +6,10 @@
  * objects that define these names.  Arrays may be device pointers (the
  * reference's igpu = 1 contract) or host memory (staged through the device);
  * each call returns with its result complete; an error prints the message and
- * stops the program (the reference's `write(6,*); stop`, exit status 1).  FP64
- * data (prec = 8) only. */
+ * stops the program (the reference's `write(6,*); stop`, exit status 1).  Data
+ * are FP64 (prec = 8) unless dpd_fortran_set_prec_ selects FP32 (prec = 4): the
+ * engines, sorts and xgemmso then take REAL*4 arrays through the same
+ * DOUBLE PRECISION dummy arguments, as the reference's routers do. */
 #ifndef DPD_FORTRAN_H
 #define DPD_FORTRAN_H
 #include <stddef.h>
@@ -23,6 +25,9 @@ extern "C" {
  * program before the first engine call. */
 void dpd_fortran_init_(const int64_t *nirrep, const int64_t *dirprd, const int64_t *device);
 void dpd_fortran_finalize_(void);
+/* No reference counterpart as a call: sets what the reference keeps in the
+ * mod_iop variable `prec` (libsrint/readinp.f90:120-127), 8 or 4. */
+void dpd_fortran_set_prec_(const int64_t *prec);
 
 /* libsrint/vv2v_dpd_444.f90:55-57 */
 void vv2v_dpd_444_(const double *v1, const int64_t *p1_1, const int64_t *p1_2, const int64_t *iop1_12,

@@ -52,7 +52,8 @@ typedef struct {
   int64_t pop[4][8]; /* popp, popq, popr, pops */
   int64_t iop[2];    /* ioppq, ioprs */
   int64_t sym;       /* 1-based irrep of the tensor */
-  double *data;      /* device pointer, FP64, length dpd_tensor_len() */
+  double *data;      /* device pointer, FP64 (FP32 under dpd_ctx_set_data_precision(ctx, 4)),
+                        length dpd_tensor_len() */
 } dpd_tensor;
 
 /* ---- context ------------------------------------------------------------ */
@@ -80,6 +81,20 @@ int dpd_ctx_set_gemm_config(dpd_ctx *ctx, int cfg, int packing);
  * workgroup per output tile); 1: long-K tasks split while a launch has fewer
  * than 1024 tiles, partial sums reduced by a second kernel.  Clears the plans. */
 int dpd_ctx_set_split_k(dpd_ctx *ctx, int mode);
+/* libsrint/readinp.f90:15,120-127 (the mod_iop `prec` switch that routes
+ * vv2v_dpd_444.f90:206-229, sstgenso.f90:35-42 and xgemmso.f:11-22 to the _s/_sg
+ * code).  prec 8 (default): tensor data are FP64.  prec 4: the `data` pointers
+ * given to the engines (dpd_vv2v_dpd_444/424/442/323), the sort kernels
+ * (dpd_iexpso ... dpd_sstgenso) and dpd_xgemm point at FP32 arrays (the
+ * reference's real*4 storage, vv2v_dpd_444_sg.f90:20-21); their GEMMs read FP32
+ * and run FP32 MFMA, and the result update v3 = fact2*v3 + fact1*C is formed
+ * in FP64 and rounded to FP32 once (vv2v_dpd_444_sg.f90:431,445).  Sorts move
+ * FP32 values exactly (iassymso's one sum or difference is rounded once).  The
+ * contract surface (dpd_vv2v_444 ...) stays FP64, as lib_cc always runs with
+ * prec = 8 (lib_cc/readmodule.f90:64), and returns DPD_EINVAL under prec 4;
+ * dpd_problem_* own their storage and are not affected. */
+int dpd_ctx_set_data_precision(dpd_ctx *ctx, int prec);
+int dpd_ctx_data_precision(const dpd_ctx *ctx);
 const char *dpd_last_error(void);
 /* bytes of device scratch the context currently holds (grows on demand) */
 size_t dpd_ctx_workspace_bytes(const dpd_ctx *ctx);

@@ -11,6 +11,10 @@ problem of oracle_sigma.py:
     calV_C.f90, calC_JR.f90, calC_VTR.f90, calC1_VR2.f90, calC2_VR1.f90,
     calC2_VR2.f90, solver.f90, orthogonalR2L.f90: EE-CPS(D) excitation
     energies order by order, w = w^(0) + w^(2) + w^(3) per CIS root
+  * grCPSd.SF/run.f90, run1-3.f90, solver.f90, calA0R1.f90, calA0R2.f90,
+    calA1R1.f90 (calA2R1), calA1R2.f90, calC1.f90, calC2.f90, calL1_VTT.f90,
+    calL2_VTT.f90, calEcorr.f90 (+ lib_cc/effectT2.f90): grCPS(D) ground state,
+    the CIS(Doo) Jacobian as zeroth order, T^(1..3) and E^(2..4)
   * the named contractions they call (lib_cc/*.f90, one function each).
 Only tests/ import this module.
 """
@@ -182,6 +186,32 @@ def Wmnij_r1ijka(w, r1, ijka, fact):    # Wmnij_r1ijka.f90
     O.vv2v_424(ijka, r1, w, fact, "mnie", "ej", "mnij", 2)
 
 
+def Wkbij_r1iabj(w, r1, iabj, fact):    # Wkbij_r1iabj.f90
+    O.vv2v_424(iabj, r1, w, -fact, "ckbj", "ci", "ijkb", 1)
+
+
+def Wabdj_r1abcd(w, r1, abcd, fact):    # Wabdj_r1abcd.f90 (both in core)
+    O.vv2v_424(abcd, r1, w, fact, "abdc", "cj", "abdj", 0)
+
+
+def Wkbij_r1Uklij(w, r1, u, fact):      # Wkbij_r1Uklij.f90
+    O.vv2v_424(u, r1, w, -fact, "klij", "bl", "ijkb", 0)
+
+
+def Fme_r1abij(f, r1, abij, fact):      # Fme_r1abij.f90
+    O.vv2v_422(abij, r1, f, fact, "efmn", "fn", "em")
+
+
+def c1_r2Fme(c1, r2, f, fact):          # c1_r2Fme.f90
+    O.vv2v_422(r2, f, c1, fact, "aeim", "em", "ai")
+
+
+def effectT2(t2, t1a, t1b, fact, kind):  # effectT2.f90 (p_effectT2 types 1 and 2)
+    """t2(ab,ij) += fact * [t1a(a,i) t1b(b,j)] antisymmetrised in ij (type 1,
+    t1a = t1b) or in ab and ij (type 2), kept in t2's packing"""
+    O.vv2v_224(t1a, t1b, t2, fact, "ai", "bj", "abij", 2 if kind == 1 else 3)
+
+
 # ---------------------------------------------------------------------------
 # CPS(D) ground state (CPSd.SF)
 # ---------------------------------------------------------------------------
@@ -286,6 +316,145 @@ class Perturb:
         Wkl = O.Opt(O.Desc([o, o, o, o], (0, 0), 1))
         Wmnij_r2abij(Wkl, T2, abij, 0.5)
         c2_r2Wmnij(C2, T2, Wkl, 2.0 * fact)
+
+    # -----------------------------------------------------------------------
+    # grCPS(D) ground state (grCPSd.SF): zeroth-order Jacobian = CIS(Doo)
+    # -----------------------------------------------------------------------
+    def _gr_intermediates(self):
+        """grCPSd.SF/run.f90:29-47: MP2t, Fli, Fad and Wkaci (iabj layout)"""
+        v, o = self.v, self.o
+        abij = self._opt("abij")
+        X = {"MP2t": O.Opt(self.dT2, self.denomT2(abij, 1))}
+        X["Fli"] = O.Opt(O.Desc([o, o, Z, Z], (1, 1), 1))
+        Fmi_r2abij(X["Fli"], X["MP2t"], abij, -1.0)
+        X["Fad"] = O.Opt(O.Desc([v, v, Z, Z], (1, 1), 1))
+        Fae_r2abij(X["Fad"], X["MP2t"], abij, -1.0)
+        X["Wkaci"] = O.Opt(O.Desc([v, o, v, o], (1, 1), 1), self._opt("iabj").arr.copy())
+        Wmbej_r2abij(X["Wkaci"], X["MP2t"], abij, 2.0)
+        return X
+
+    def _gr_a0(self, X, T1, T2, R1, R2):
+        """calA0R1.f90 + calA0R2.f90: R += A0' T (A0 without its orbital-energy diagonal)"""
+        c1_r1iabj(R1, T1, X["Wkaci"], 1.0)
+        c1_r1Fmi(R1, T1, X["Fli"], -1.0)
+        c1_r1Fae(R1, T1, X["Fad"], -1.0)
+        c1_r2ijka(R1, T2, self._opt("ijka"), 1.0)
+        c1_r2abci(R1, T2, self._opt("abci"), 1.0)
+        c2_r1ijka(R2, T1, self._opt("ijka"), 1.0)
+        c2_r1abci(R2, T1, self._opt("abci"), 1.0)
+
+    def _gr_ecorr(self, order, T2, T1s):
+        """calEcorr.f90: E^(k+1) = <abij| T2^(k) + [T1 T1]^(k)>"""
+        t = O.Opt(self.dT2, T2.arr.copy())
+        if order == 2:
+            effectT2(t, T1s[1], T1s[1], 1.0, 1)
+        elif order == 3:
+            effectT2(t, T1s[2], T1s[1], 1.0, 2)
+        return float(np.dot(t.arr, self._opt("abij").arr))
+
+    def _gr_solver(self, X, C1, C2, order, T1s):
+        """solver.f90: T = (C + A0' T) / D by Jacobi steps with DIIS on (T1 | T2)"""
+        l1 = self.dT1.length()
+        diis = Diis(self.diis_dim)
+        R1 = O.Opt(self.dT1, C1.arr.copy())
+        R2 = O.Opt(self.dT2, C2.arr.copy())
+        it = 0
+        E = 0.0
+        while True:
+            it += 1
+            x = np.concatenate([self.denomT1(R1, 1), self.denomT2(R2, 1)])
+            x, err = diis.step(it, x)
+            T1 = O.Opt(self.dT1, x[:l1].copy())
+            T2 = O.Opt(self.dT2, x[l1:].copy())
+            E = self._gr_ecorr(order, T2, T1s)
+            if it > 1 and abs(err) < self.diis_max_err:
+                break
+            if it > self.diis_cyc:
+                break
+            R1 = O.Opt(self.dT1, C1.arr.copy())
+            R2 = O.Opt(self.dT2, C2.arr.copy())
+            self._gr_a0(X, T1, T2, R1, R2)
+        return T1, T2, E, it
+
+    def _gr_l1_vtt(self, L1, T1, T2):
+        """calL1_VTT.f90 with T^(1)"""
+        v, o = self.v, self.o
+        abij, ijka, abci = self._opt("abij"), self._opt("ijka"), self._opt("abci")
+        Gli = O.Opt(O.Desc([o, o, Z, Z], (1, 1), 1))
+        Fmi_r1ijka(Gli, T1, ijka, -1.0)
+        Fmi_r2abij(Gli, T2, abij, -1.0)
+        Gad = O.Opt(O.Desc([v, v, Z, Z], (1, 1), 1))
+        Fae_r1abci(Gad, T1, abci, -1.0)
+        Fae_r2abij(Gad, T2, abij, -1.0)
+        Gld = O.Opt(O.Desc([v, o, Z, Z], (1, 1), 1))
+        Fme_r1abij(Gld, T1, abij, 1.0)
+        c1_r1Fmi(L1, T1, Gli, -1.0)
+        c1_r1Fae(L1, T1, Gad, -1.0)
+        c1_r2Fme(L1, T2, Gld, 1.0)
+
+    def _gr_l2_vtt(self, L2, T1, T2):
+        """calL2_VTT.f90 with T^(1)"""
+        v, o = self.v, self.o
+        abij, ijka, abci, iabj = self._opt("abij"), self._opt("ijka"), self._opt("abci"), self._opt("iabj")
+        d_ooov = O.Desc([o, o, o, v], (0, 1), 1)
+        tmp = O.Opt(d_ooov)
+        Wkbij_r1iabj(tmp, T1, iabj, 1.0)
+        c2_r1ijka(L2, T1, tmp, -1.0)
+        tmp = O.Opt(O.Desc([v, v, v, o], (0, 1), 1))
+        Wabdj_r1abcd(tmp, T1, self._opt("abcd"), 1.0)
+        c2_r1abci(L2, T1, tmp, 0.5)
+        tmp = O.Opt(d_ooov)
+        Wkbij_r1Uklij(tmp, T1, self._opt("ijkl"), 1.0)
+        c2_r1ijka(L2, T1, tmp, 0.5)
+        tmp = O.Opt(O.Desc([o, o, Z, Z], (1, 1), 1))
+        Fmi_r1ijka(tmp, T1, ijka, 1.0)
+        Fmi_r2abij(tmp, T2, abij, 1.0)
+        c2_r2Fmi(L2, T2, tmp, 1.0)
+        tmp = O.Opt(O.Desc([v, v, Z, Z], (1, 1), 1))
+        Fae_r1abci(tmp, T1, abci, 1.0)
+        Fae_r2abij(tmp, T2, abij, 1.0)
+        c2_r2Fae(L2, T2, tmp, 1.0)
+        tmp = O.Opt(O.Desc([v, o, v, o], (1, 1), 1))
+        Wmbej_r1ijka(tmp, T1, ijka, 1.0)
+        Wmbej_r1abci(tmp, T1, abci, 1.0)
+        Wmbej_r2abij(tmp, T2, abij, 1.0)
+        c2_r2Wmbej(L2, T2, tmp, 1.0)
+        tmp = O.Opt(d_ooov)
+        Wkbij_r2abci(tmp, T2, abci, 1.0)
+        c2_r1ijka(L2, T1, tmp, 1.0)
+        tmp = O.Opt(O.Desc([o, o, o, o], (0, 0), 1))
+        Wmnij_r1ijka(tmp, T1, ijka, 0.5)
+        Wmnij_r2abij(tmp, T2, abij, 0.5)
+        c2_r2Wmnij(L2, T2, tmp, 2.0)
+
+    def gr_ground_state(self):
+        """grCPSd.SF/run.f90 (run1.f90, run2.f90, run3.f90, calC1.f90, calC2.f90,
+        calA1R1.f90 (calA2R1), calA1R2.f90, calL1_VTT.f90, calL2_VTT.f90):
+        orders 1-3 of A0 T^(n) + C^(n) = 0 with the CIS(Doo) Jacobian as A0.
+        Returns T1/T2 per order, E2..E4 and the DIIS iteration counts."""
+        X = self._gr_intermediates()
+        T1, T2, E, iters = {}, {}, {}, {}
+        for n in (1, 2, 3):
+            C1 = O.Opt(self.dT1)
+            C2 = O.Opt(self.dT2)
+            if n == 1:
+                C2.arr = self._opt("abij").arr.copy()  # run1.f90:18-19
+            else:
+                if n == 3:
+                    # calA2R1: -[c1_r1iabj(Wkaci - iabj) - Fli - Fad] on T1^(1)
+                    W = O.Opt(X["Wkaci"].desc, X["Wkaci"].arr - self._opt("iabj").arr)
+                    c1_r1iabj(C1, T1[1], W, -1.0)
+                    c1_r1Fmi(C1, T1[1], X["Fli"], 1.0)
+                    c1_r1Fae(C1, T1[1], X["Fad"], 1.0)
+                    self._gr_l1_vtt(C1, T1[1], T2[1])
+                # calA1R2: ring + hh + pp ladder on T2^(n-1)
+                c2_r2Wmbej(C2, T2[n - 1], self._opt("iabj"), 1.0)
+                c2_r2Wmnij(C2, T2[n - 1], self._opt("ijkl"), 1.0)
+                c2_r2abcd(C2, T2[n - 1], self._opt("abcd"), 1.0)
+                if n == 3:
+                    self._gr_l2_vtt(C2, T1[1], T2[1])
+            T1[n], T2[n], E[n + 1], iters[n] = self._gr_solver(X, C1, C2, n, T1)
+        return dict(T1=T1, T2=T2, E=E, iterations=iters)
 
     # -----------------------------------------------------------------------
     # EE-CPS(D) (EE-CPSd.SF)

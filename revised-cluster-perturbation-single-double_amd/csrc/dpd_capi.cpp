@@ -223,7 +223,7 @@ static int engine_entry(dpd_ctx* c, int kind, const dpd_tensor* v1, const char* 
   if (!rc) rc = make_operand(c, v2, t2, r2, o2);
   if (!rc) rc = make_operand(c, v3, t3, r3, o3);
   if (rc) return rc;
-  return engine_call(c, kind, o1, o2, o3, f1, f2, 1);
+  return engine_call(c, kind, o1, o2, o3, f1, f2, 1, c->data_prec == 4);
 }
 
 int dpd_vv2v_dpd_444(dpd_ctx* c, const dpd_tensor* v1, const char* t1, const dpd_tensor* v2, const char* t2,
@@ -302,6 +302,9 @@ static int contract_entry(dpd_ctx* c, int kind, const dpd_tensor* v1, const char
                           const dpd_tensor* v2, const char* t2, int r2, dpd_tensor* v3, const char* t3,
                           int r3, double fact, int64_t anti) {
   CHECK_CTX(c);
+  if (c->data_prec != 8)
+    return fail(DPD_EINVAL, "the contract surface takes FP64 tensors only (lib_cc sets prec = 8, "
+                            "readmodule.f90:64); data precision is %d", c->data_prec);
   { int rc_ = set_device(c); if (rc_) return rc_; }
   int maxanti = (kind == 444) ? 3 : (kind == 424 ? 2 : 0);
   if (anti < 0 || anti > maxanti)
@@ -376,6 +379,7 @@ static int run_sort(dpd_ctx* c, const Desc& dout, const Desc& din, const int8_t*
   a.alpha = 1.0;
   a.beta = 0.0;
   a.nbatch = 1;
+  if (c->data_prec == 4) return gather_f32(c, a, (const float*)in, (float*)out);
   return gather(c, a, in, out);
 }
 
@@ -475,6 +479,9 @@ int dpd_iassymso(dpd_ctx* c, const double* in, double* out, int64_t sym, const i
   a.alpha = 1.0;
   a.beta = 0.0;
   a.nbatch = 1;
+  // FP32 data: the one sum or difference is formed in FP64 and rounded once,
+  // which is the FP32 result itself
+  if (c->data_prec == 4) return gather_f32(c, a, (const float*)in, (float*)out);
   return gather(c, a, in, out);
 }
 
@@ -560,10 +567,17 @@ int dpd_xgemm(dpd_ctx* c, const char* transa, const char* transb, int64_t m, int
   { int rc_ = set_device(c); if (rc_) return rc_; }
   GemmPlan p;
   bool empty = false;
+  // FP32 arrays (XGEMMSG, xgemmso.f:21-22): FP32 MFMA, FP64 alpha and beta
+  const bool f32data = c->data_prec == 4;
+  if (f32data) prec = PREC_F32;
   int rc = xgemm_describe(c, transa, transb, m, n, k, lda, ldb, ldc, prec, p, &empty);
   if (rc || empty) return rc;
   if ((rc = p.upload())) return rc;
-  if (prec == PREC_BF16X3 || c->packing == 2) {
+  if (f32data) {
+    ws_reset(c);
+    if ((rc = ws_reserve(c, sizeof(double) * p.partial_elems + 256))) return rc;
+    rc = run_gemm(c, p, A, 1, B, 1, C, alpha, beta, PREC_F32, true);
+  } else if (prec == PREC_BF16X3 || c->packing == 2) {
     if ((rc = build_variant(p, prec))) return rc;
     const GemmVariant& V = p.pk[prec];
     ws_reset(c);
@@ -592,6 +606,7 @@ int dpd_xgemm_noalloc(dpd_ctx* c, const char* transa, const char* transb, int64_
   { int rc_ = set_device(c); if (rc_) return rc_; }
   void* sp = g_scratch_dev[c->device];
   if (!sp) return fail(DPD_EINVAL, "dpd_xgemm_noalloc: no scratch address for device %d (internal)", c->device);
+  if (c->data_prec != 8) return fail(DPD_EINVAL, "dpd_xgemm_noalloc: FP64 arrays only (data precision 8)");
   GemmPlan p;
   bool empty = false;
   int rc = xgemm_describe(c, transa, transb, m, n, k, lda, ldb, ldc, prec, p, &empty);
@@ -672,6 +687,18 @@ int dpd_ctx_set_split_k(dpd_ctx* c, int mode) {
   return DPD_OK;
 }
 
+// libsrint/readinp.f90:15,120-127 (the mod_iop `prec` switch): 8 FP64 tensors,
+// 4 FP32 tensors for the engines, sorts and xgemm (vv2v_dpd_444.f90:206-229
+// routes to the _s/_sg engines, sstgenso.f90:35-42, xgemmso.f:14-22)
+int dpd_ctx_set_data_precision(dpd_ctx* c, int prec) {
+  CHECK_CTX(c);
+  if (prec != 8 && prec != 4) return fail(DPD_EINVAL, "prec must be 8 or 4 (got %d)", prec);
+  c->data_prec = prec;
+  return DPD_OK;
+}
+
+int dpd_ctx_data_precision(const dpd_ctx* c) { return c ? c->data_prec : 0; }
+
 // ---- profiler ------------------------------------------------------------------
 int dpd_profile_enable(dpd_ctx* c, int on, int max_records) {
   CHECK_CTX(c);
@@ -701,14 +728,15 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
   std::vector<std::string> tags;
   std::vector<double> ms;
   std::vector<int64_t> cnt, fl, fltot, by;
-  std::vector<int> pr, cf, pk, sp;
+  std::vector<int> pr, cf, pk, sp, kd;
   for (auto& r : c->prof.recs) {
     float t = 0.f;
     DPD_HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
     // one entry per tag, precision and tile shape (a tag's plans for different
     // vector counts may be tuned to different tiles)
-    std::string key = r.tag + (r.prec == PREC_F64 ? ":fp64" : r.prec == PREC_F32 ? ":fp32" : ":bf16x3") + ":t" +
-                      std::to_string(r.cfg) + (r.split ? "s" : "");
+    std::string key = r.kind == 1 ? r.tag
+                                  : r.tag + (r.prec == PREC_F64 ? ":fp64" : r.prec == PREC_F32 ? ":fp32" : ":bf16x3") +
+                                        ":t" + std::to_string(r.cfg) + (r.split ? "s" : "");
     size_t k = std::find(tags.begin(), tags.end(), key) - tags.begin();
     if (k == tags.size()) {
       tags.push_back(key);
@@ -721,6 +749,7 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
       cf.push_back(r.cfg);
       pk.push_back(r.packed);
       sp.push_back(r.split);
+      kd.push_back(r.kind);
     }
     ms[k] += t;
     cnt[k] += 1;
@@ -736,9 +765,9 @@ int dpd_profile_report(dpd_ctx* c, char* buf, int64_t len) {
     char item[512];
     snprintf(item, sizeof item,
              "%s\"%s\": {\"count\": %lld, \"ms\": %.6f, \"flops_total\": %lld, \"bytes_total\": %lld, "
-             "\"prec\": %d, \"cfg\": %d, \"packed\": %d, \"split\": %d}",
+             "\"prec\": %d, \"cfg\": %d, \"packed\": %d, \"split\": %d, \"kind\": \"%s\"}",
              k ? ", " : "", tags[k].c_str(), (long long)cnt[k], ms[k], (long long)fltot[k], (long long)by[k], pr[k],
-             cf[k], pk[k], sp[k]);
+             cf[k], pk[k], sp[k], kd[k] ? "gather" : "gemm");
     js += item;
   }
   js += "}";

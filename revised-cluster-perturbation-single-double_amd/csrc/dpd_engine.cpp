@@ -928,7 +928,7 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
 }
 
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
-             double f1, double f2, int prec) {
+             double f1, double f2, int prec, bool c_f32) {
   if (p.nwg == 0) return DPD_OK;
   GemmLaunch g;
   std::memset(&g, 0, sizeof g);
@@ -944,6 +944,7 @@ int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, i
   g.prec = prec;
   g.a_f32 = a_f32;
   g.b_f32 = b_f32;
+  g.c_f32 = c_f32 ? 1 : 0;
   g.split_tasks = p.d_split;
   g.nsplit = (int)p.split.size();
   g.max_split_mn = p.max_split_mn;
@@ -955,6 +956,11 @@ int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, i
 
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out) {
   DPD_HIP_TRY(launch_gather_map(a, in, out, c->stream));
+  return DPD_OK;
+}
+
+int gather_f32(dpd_ctx* c, MapArgs a, const float* in, float* out) {
+  DPD_HIP_TRY(launch_gather_map_f32(a, in, out, c->stream));
   return DPD_OK;
 }
 
@@ -1086,7 +1092,7 @@ static std::string sig_of(int kind, const Operand& o1, const Operand& o2, const 
 }
 
 int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, const Operand& o3,
-                double f1, double f2, int nbatch) {
+                double f1, double f2, int nbatch, bool f32data) {
   Operand m1, m2, m3;
   int rc = check_engine(c->sym, kind, o1, o2, o3, m1, m2, m3);
   if (rc) return rc;
@@ -1104,6 +1110,13 @@ int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, cons
     p = it->second.get();
   }
   ws_reset(c);
+  if (f32data) {
+    // FP32 storage (libsrint/vv2v_dpd_444_sg.f90): the gathering kernel reads the
+    // FP32 tensors where they lie and runs FP32 MFMA
+    rc = ws_reserve(c, sizeof(double) * (p->partial_elems + 32));
+    if (rc) return rc;
+    return run_gemm(c, *p, o1.data, 1, o2.data, 1, const_cast<double*>(o3.data), f1, f2, PREC_F32, true);
+  }
   if (c->packing == 2) {
     if ((rc = build_variant(*p, PREC_F64))) return rc;
     const GemmVariant& V = p->pk[PREC_F64];
@@ -1475,37 +1488,68 @@ static int call_setup(dpd_ctx* c, ContractPlan* P, const Operand& o1, const Oper
   return DPD_OK;
 }
 
+static ProfRec* prof_open(dpd_ctx* c, const char* tag, int64_t flops, int prec, int cfg, int packed,
+                          int64_t bytes, int split, int kind = 0) {
+  if (!c->prof.on || !tag || c->prof.used + 2 > c->prof.pool.size()) return nullptr;
+  c->prof.recs.push_back({tag, c->prof.pool[c->prof.used], c->prof.pool[c->prof.used + 1], flops, prec, cfg, packed,
+                          bytes, split, kind});
+  c->prof.used += 2;
+  return &c->prof.recs.back();
+}
+
+// a gather of a tagged contraction, bracketed by events when profiling: its
+// algorithmic bytes are the input once, the output once (twice when it is
+// accumulated into)
+static ProfRec* prof_gather(dpd_ctx* c, const char* tag, const char* what, const MapArgs& m, int esz_in,
+                            int esz_out) {
+  if (!c->prof.on || !tag) return nullptr;
+  thread_local std::string name;
+  name = std::string(tag) + ":" + what;
+  const int64_t nb = m.nbatch;
+  const int64_t bytes = nb * (m.in.len * esz_in + m.out.len * esz_out * (m.beta != 0.0 ? 2 : 1));
+  ProfRec* pr = prof_open(c, name.c_str(), 0, PREC_F64, 0, 0, bytes, 0, 1);
+  if (pr) (void)hipEventRecord(pr->a, c->stream);
+  return pr;
+}
+static void prof_close(dpd_ctx* c, ProfRec* pr) {
+  if (pr) (void)hipEventRecord(pr->b, c->stream);
+}
+
 // expansions into the workspace, the scratch result, the packing sources
-static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o2, const Operand& o3, int nbatch) {
+static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o2, const Operand& o3, int nbatch,
+                    const char* tag) {
   ContractPlan* P = r.P;
   r.p1 = o1.data;
   r.p2 = o2.data;
   r.p1f = o1.data_f32;
   r.p2f = o2.data_f32;
   // expansion of operand 1 or 2 into the workspace (FP32 when it only feeds a pack)
-  auto expand = [&](const MapArgs& map, const Operand& o, int nb, int64_t lenx, bool f32, const double** dst) -> int {
+  auto expand = [&](const MapArgs& map, const Operand& o, int nb, int64_t lenx, bool f32, const double** dst,
+                    const char* what) -> int {
     void* x = ws_take(c, (f32 ? sizeof(float) : sizeof(double)) * lenx * nb);
     if (!x) return DPD_ENOMEM;
     MapArgs m = map;
     m.nbatch = nb;
     m.in_bstride = o.bstride;
     m.out_bstride = lenx;
+    ProfRec* pr = prof_gather(c, tag, what, m, 8, f32 ? 4 : 8);
     if (f32) {
       if (m.out.len > 0 && m.nbatch > 0) DPD_HIP_TRY(launch_gather_map_f32out(m, o.data, (float*)x, c->stream));
     } else {
       int rc = gather(c, m, o.data, (double*)x);
       if (rc) return rc;
     }
+    prof_close(c, pr);
     *dst = (const double*)x;
     return DPD_OK;
   };
   if (r.gx1) {
-    int rc = expand(P->map1, o1, r.nb1, P->len1x, r.x1_f32, &r.p1);
+    int rc = expand(P->map1, o1, r.nb1, P->len1x, r.x1_f32, &r.p1, "x1");
     if (rc) return rc;
     r.p1f = nullptr;
   }
   if (r.gx2) {
-    int rc = expand(P->map2, o2, r.nb2, P->len2x, r.x2_f32, &r.p2);
+    int rc = expand(P->map2, o2, r.nb2, P->len2x, r.x2_f32, &r.p2, "x2");
     if (rc) return rc;
     r.p2f = nullptr;
   }
@@ -1552,18 +1596,22 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
 }
 
 // antisymmetrise + compress + accumulate the scratch result into v3
-static int call_post(dpd_ctx* c, const CallPrep& r, const Operand& o3, int nbatch) {
+static int call_post(dpd_ctx* c, const CallPrep& r, const Operand& o3, int nbatch, const char* tag) {
   const ContractPlan* P = r.P;
   if (!P->epi_on) return DPD_OK;
   MapArgs m = P->epi;
   m.nbatch = nbatch;
   m.in_bstride = P->len3x;
   m.out_bstride = o3.bstride;
+  ProfRec* pr = prof_gather(c, tag, "epi", m, r.p3_f32 ? 4 : 8, 8);
   if (r.p3_f32) {
     DPD_HIP_TRY(launch_gather_map_f32in(m, (const float*)r.p3, const_cast<double*>(o3.data), c->stream));
-    return DPD_OK;
+  } else {
+    int rc = gather(c, m, r.p3, const_cast<double*>(o3.data));
+    if (rc) return rc;
   }
-  return gather(c, m, r.p3, const_cast<double*>(o3.data));
+  prof_close(c, pr);
+  return DPD_OK;
 }
 
 // algorithmic bytes of one call: both operands once as the kernel reads them
@@ -1584,14 +1632,6 @@ static int64_t call_bytes(dpd_ctx* c, const CallPrep& r, const Operand& o1, cons
   return bytes;
 }
 
-static ProfRec* prof_open(dpd_ctx* c, const char* tag, int64_t flops, int prec, int cfg, int packed,
-                          int64_t bytes, int split) {
-  if (!c->prof.on || !tag || c->prof.used + 2 > c->prof.pool.size()) return nullptr;
-  c->prof.recs.push_back({tag, c->prof.pool[c->prof.used], c->prof.pool[c->prof.used + 1], flops, prec, cfg, packed,
-                          bytes, split});
-  c->prof.used += 2;
-  return &c->prof.recs.back();
-}
 
 int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
                   double fact, int anti, int nbatch, int prec, int64_t* flops, const char* tag) {
@@ -1604,7 +1644,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   if ((rc = call_setup(c, P, o1, o2, nbatch, prec, r, &need))) return rc;
   ws_reset(c);
   if ((rc = ws_reserve(c, need))) return rc;
-  if ((rc = call_pre(c, r, o1, o2, o3, nbatch))) return rc;
+  if ((rc = call_pre(c, r, o1, o2, o3, nbatch, tag))) return rc;
   const bool use_f32 = (prec != PREC_F64);
   const void* A = (use_f32 && r.p1f) ? (const void*)r.p1f : (const void*)r.p1;
   const void* B = (use_f32 && r.p2f) ? (const void*)r.p2f : (const void*)r.p2;
@@ -1628,7 +1668,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   }
   if (rc) return rc;
   if (pr && !r.packed) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
-  return call_post(c, r, o3, nbatch);
+  return call_post(c, r, o3, nbatch, tag);
 }
 
 // ---------------------------------------------------------------------------
@@ -1885,7 +1925,7 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
   std::memset(&gt, 0, sizeof gt);
   int64_t flops = 0, bytes = 0;
   for (int i = 0; i < n; i++) {
-    if ((rc = call_pre(c, R[i], o1[i], o2[i], o3[i], nbatch[i]))) return rc;
+    if ((rc = call_pre(c, R[i], o1[i], o2[i], o3[i], nbatch[i], cap ? nullptr : tag))) return rc;
     const void* side[2];
     if ((rc = pack_sides(c, plans[i]->gemm, plans[i]->gemm.pk[prec], prec, R[i].a, R[i].b, side))) return rc;
     gt.e[i] = {side[0], side[1], R[i].p3, R[i].beta, R[i].p3_f32 ? 1 : 0, 0};
@@ -1902,7 +1942,7 @@ int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, c
   DPD_HIP_TRY(launch_gemm_pk_grouped(g, gt, c->stream));
   if (pr) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
   for (int i = 0; i < n; i++)
-    if ((rc = call_post(c, R[i], o3[i], nbatch[i]))) return rc;
+    if ((rc = call_post(c, R[i], o3[i], nbatch[i], cap ? nullptr : tag))) return rc;
   if (!G->tuned && !cap) {
     G->tuned = true;
     const int before = G->cfg;

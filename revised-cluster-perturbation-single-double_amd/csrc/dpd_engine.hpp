@@ -160,6 +160,7 @@ struct ProfRec {
   int packed;     // packed-panel kernel (1) or gathering kernel (0)
   int64_t bytes;  // algorithmic bytes: operands as the kernel reads them once + result
   int split;      // some task of the launch is split over K (partials + reduce)
+  int kind;       // 0 GEMM launch, 1 gather (expansion, sort or epilogue)
 };
 struct Profiler {
   bool on = false;
@@ -179,6 +180,7 @@ struct dpd_ctx {
   int gemm_cfg = -1;  // GEMM tile configuration override (-1: automatic)
   int packing = 1;    // 0 gather kernels only, 1 packed panels when an operand is static, 2 always
   int split_k = -1;   // packed-panel split-K: -1 automatic, 0 off, 1 on
+  int data_prec = 8;  // element size of the tensors given to the engines, sorts and xgemm (8 or 4)
   // scratch: grow-only device workspace (bump allocated per top-level call)
   dpd::DevBuf ws;
   size_t ws_top = 0;
@@ -201,8 +203,11 @@ inline void ws_reset(dpd_ctx* c) { c->ws_top = 0; }
 void drop_pack_entries(dpd_ctx* c, bool (*match)(uint64_t id, uint64_t key), uint64_t key);
 
 // engine level (libsrint vv2v_dpd_* semantics)
+// f32data: the three tensors are FP32 arrays (the reference's prec = 4 storage);
+// the GEMM then reads FP32, accumulates in FP32 MFMA and writes
+// v3 = fl32(f2 * v3 + f1 * C) formed in FP64
 int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, const Operand& o3,
-                double f1, double f2, int nbatch);
+                double f1, double f2, int nbatch, bool f32data = false);
 // contract surface (mod_contract semantics) with optional batching of one operand
 int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
                   double fact, int anti, int nbatch, int prec, int64_t* flops = nullptr,
@@ -212,7 +217,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
 int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, const Operand* o3, int n,
                    double fact, int anti, const int* nbatch, int prec, const char* tag = nullptr);
 int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, int b_f32, double* C,
-             double f1, double f2, int prec);
+             double f1, double f2, int prec, bool c_f32 = false);
 int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested);
 int build_variant(GemmPlan& p, int mode);
 struct PackSrc;
@@ -240,6 +245,8 @@ int layout_variant(const GemmPlan& p, GemmVariant& V, int mode, int cfg, bool sp
 // p.d_segs must point at device copies of the tables)
 int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const PackSrc& src, void* dst);
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out);
+// the same on FP32 arrays (sums formed in FP64, rounded once)
+int gather_f32(dpd_ctx* c, MapArgs a, const float* in, float* out);
 // labels / checks shared by the C-ABI and the sigma builders
 int check_labels(const char* t, int n);
 int map444(const char* type1, const char* type2, const char* type3, char* t1, char* t2, char* t3);

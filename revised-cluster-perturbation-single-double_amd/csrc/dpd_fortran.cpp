@@ -15,8 +15,9 @@
 // are staged through grow-only device buffers and the output is copied back.
 // Every call returns after its result is complete, as the reference's do.  An
 // error prints the message and stops the program, as the reference's
-// `write(6,*) ...; stop` does, but with exit status 1.  FP64 data only (the
-// reference's prec = 8).
+// `write(6,*) ...; stop` does, but with exit status 1.  Data are FP64 (the
+// reference's prec = 8) or, after dpd_fortran_set_prec_(4), FP32 arrays passed
+// through the same dummy arguments (the reference's prec = 4).
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -55,12 +56,15 @@ bool on_device(const void* p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
 }
 
-// device view of an operand of n doubles in slot k; host data is copied in
+// bytes of one tensor element under the current prec (8 or 4)
+size_t elem_bytes() { return g_ctx && dpd_ctx_data_precision(g_ctx) == 4 ? sizeof(float) : sizeof(double); }
+
+// device view of an operand of n elements in slot k; host data is copied in
 // when `load` is set
 double* view(int k, const double* p, int64_t n, bool load, const char* where) {
   if (on_device(p) || n <= 0) return const_cast<double*>(p);
   DevBuf& b = g_stage[k];
-  const size_t bytes = sizeof(double) * (size_t)n;
+  const size_t bytes = elem_bytes() * (size_t)n;
   if (b.bytes < bytes) {
     b.reset();
     if (b.alloc(bytes)) stop_with(where);
@@ -72,7 +76,7 @@ double* view(int k, const double* p, int64_t n, bool load, const char* where) {
 
 void finish(double* host, const double* dev, int64_t n, const char* where) {
   if (host != dev && n > 0 &&
-      hipMemcpyAsync(host, dev, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, g_ctx->stream) != hipSuccess)
+      hipMemcpyAsync(host, dev, elem_bytes() * (size_t)n, hipMemcpyDeviceToHost, g_ctx->stream) != hipSuccess)
     stop_with(where);
   if (hipStreamSynchronize(g_ctx->stream) != hipSuccess) stop_with(where);
 }
@@ -163,6 +167,11 @@ void dpd_fortran_init_(const int64_t* nirrep, const int64_t* dirprd, const int64
   if (g_ctx) dpd_ctx_destroy(g_ctx);
   g_ctx = nullptr;
   if (dpd_ctx_create(&g_ctx, *nirrep, rm, (int)*device)) stop_with("dpd_fortran_init");
+}
+
+void dpd_fortran_set_prec_(const int64_t* prec) {
+  need_ctx("dpd_fortran_set_prec");
+  if (dpd_ctx_set_data_precision(g_ctx, (int)*prec)) stop_with("dpd_fortran_set_prec");
 }
 
 void dpd_fortran_finalize_() {

@@ -266,7 +266,12 @@ __global__ __launch_bounds__(256) void k_gemm2(GemmLaunch g) {
           g.partial[T.part_off + (int64_t)kp * T.M * T.N + m + (int64_t)n * T.M] = v;
         } else {
           const int64_t ad = T.c_off + cm[ml] + cn[nl];
-          g.C[ad] = (g.beta == 0.0) ? v : g.beta * g.C[ad] + v;
+          if (g.c_f32) {
+            float* Cf = (float*)g.C;
+            Cf[ad] = (float)((g.beta == 0.0) ? v : g.beta * (double)Cf[ad] + v);
+          } else {
+            g.C[ad] = (g.beta == 0.0) ? v : g.beta * g.C[ad] + v;
+          }
         }
       }
 }

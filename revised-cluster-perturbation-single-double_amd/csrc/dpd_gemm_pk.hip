@@ -459,10 +459,12 @@ __device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask
           g.partial[T.part_off + (int64_t)kp * T.M * T.N + m + (int64_t)n * T.M] = v;
         } else {
           const int64_t ad = T.c_off + cm[ml] + cn[nl];
-          if (c32)
-            ((float*)C)[ad] = (float)v;  // scratch result, beta = 0
-          else
+          if (c32) {  // FP32 result: the update is formed in FP64 and rounded once
+            float* Cf = (float*)C;
+            Cf[ad] = (float)((beta == 0.0) ? v : beta * (double)Cf[ad] + v);
+          } else {
             C[ad] = (beta == 0.0) ? v : beta * C[ad] + v;
+          }
         }
       }
 }
@@ -602,10 +604,12 @@ __device__ __forceinline__ void gemm_pkd_tile(const GemmLaunch& g, const GemmTas
           g.partial[T.part_off + (int64_t)kp * T.M * T.N + m + (int64_t)n * T.M] = v;
         } else {
           const int64_t ad = T.c_off + cm[ml] + cn[nl];
-          if (c32)
-            ((float*)C)[ad] = (float)v;
-          else
+          if (c32) {  // FP32 result: the update is formed in FP64 and rounded once
+            float* Cf = (float*)C;
+            Cf[ad] = (float)((beta == 0.0) ? v : beta * (double)Cf[ad] + v);
+          } else {
             C[ad] = (beta == 0.0) ? v : beta * C[ad] + v;
+          }
         }
       }
 }

@@ -178,6 +178,10 @@ hipError_t launch_gather_map_f32out(const MapArgs& a, const double* in, float* o
   return gather_map_t<double, float>(a, in, out, s);
 }
 
+hipError_t launch_gather_map_f32(const MapArgs& a, const float* in, float* out, hipStream_t s) {
+  return gather_map_t<float, float>(a, in, out, s);
+}
+
 // ===========================================================================
 // grouped GEMM
 // ===========================================================================
@@ -371,7 +375,12 @@ __global__ __launch_bounds__(256) void k_gemm(GemmLaunch g) {
           g.partial[T.part_off + (int64_t)kp * T.M * T.N + m + (int64_t)n * T.M] = v;
         } else {
           const int64_t ad = T.c_off + digits3(m, T.md, T.sc_m) + digits3(n, T.nd, T.sc_n);
-          g.C[ad] = (g.beta == 0.0) ? v : g.beta * g.C[ad] + v;
+          if (g.c_f32) {
+            float* Cf = (float*)g.C;
+            Cf[ad] = (float)((g.beta == 0.0) ? v : g.beta * (double)Cf[ad] + v);
+          } else {
+            g.C[ad] = (g.beta == 0.0) ? v : g.beta * g.C[ad] + v;
+          }
         }
       }
 }
@@ -385,10 +394,12 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(GemmLaunch g) {
     for (int kp = 0; kp < T.ksplit; kp++) sum += g.partial[T.part_off + kp * mn + e];
     const int m = (int)(e % T.M), n = (int)(e / T.M);
     const int64_t ad = T.c_off + digits3(m, T.md, T.sc_m) + digits3(n, T.nd, T.sc_n);
-    if (g.c_f32)
-      ((float*)g.C)[ad] = (float)sum;
-    else
+    if (g.c_f32) {
+      float* Cf = (float*)g.C;
+      Cf[ad] = (float)((g.beta == 0.0) ? sum : g.beta * (double)Cf[ad] + sum);
+    } else {
       g.C[ad] = (g.beta == 0.0) ? sum : g.beta * g.C[ad] + sum;
+    }
   }
 }
 
@@ -405,10 +416,12 @@ __global__ __launch_bounds__(256) void k_splitk_reduce_grp(GemmLaunch g, GroupTa
     for (int kp = 0; kp < T.ksplit; kp++) sum += g.partial[T.part_off + kp * mn + e];
     const int m = (int)(e % T.M), n = (int)(e / T.M);
     const int64_t ad = T.c_off + digits3(m, T.md, T.sc_m) + digits3(n, T.nd, T.sc_n);
-    if (ge.c_f32)
-      ((float*)C)[ad] = (float)sum;
-    else
+    if (ge.c_f32) {
+      float* Cf = (float*)C;
+      Cf[ad] = (float)((beta == 0.0) ? sum : beta * (double)Cf[ad] + sum);
+    } else {
       C[ad] = (beta == 0.0) ? sum : beta * C[ad] + sum;
+    }
   }
 }
 

@@ -32,6 +32,8 @@ hipError_t launch_gather_map_f32in(const MapArgs& a, const float* in, double* ou
 // the same sort writing FP32 (an expanded operand that only feeds an FP32 or
 // BF16x3 panel pack)
 hipError_t launch_gather_map_f32out(const MapArgs& a, const double* in, float* out, hipStream_t s);
+// FP32 storage on both sides (tensors of a context whose data precision is 4)
+hipError_t launch_gather_map_f32(const MapArgs& a, const float* in, float* out, hipStream_t s);
 
 // ---- grouped DPD GEMM -------------------------------------------------------
 // Every task writes C(m,n) (m,n factorised into up to 3 mixed-radix digits with
@@ -110,7 +112,7 @@ struct GemmLaunch {
   int nsplit;
   int max_split_mn;
   int cfg;  // GemmCfg
-  int c_f32;  // C is FP32 (a scratch result written with beta = 0)
+  int c_f32;  // C is FP32: C = fl32(beta * C + alpha * AB), formed in FP64
 };
 hipError_t launch_gemm(const GemmLaunch& g, hipStream_t s);
 hipError_t launch_gemm2(const GemmLaunch& g, hipStream_t s);
@@ -154,7 +156,7 @@ struct GroupEnt {
   const void* pb;
   double* c;
   double beta;
-  int c_f32;  // c is FP32 (scratch result, beta = 0)
+  int c_f32;  // c is FP32 (update formed in FP64, rounded once)
   int pad_;
 };
 struct GroupTab {

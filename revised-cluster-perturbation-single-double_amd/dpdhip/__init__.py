@@ -52,6 +52,7 @@ class Context:
         d, dp = _i64(self.dirprd.reshape(-1))
         check(lib().dpd_ctx_create(ctypes.byref(h), nirrep, dp, device))
         self.h = h
+        self.data_prec = 8
         self.use_current_stream()
 
     def use_current_stream(self):
@@ -81,6 +82,17 @@ class Context:
         operand is a problem constant (packed once), 2 packed panels always."""
         check(lib().dpd_ctx_set_gemm_config(self.h, int(cfg), int(packing)))
 
+    def set_data_precision(self, prec):
+        """the reference's mod_iop prec (libsrint/readinp.f90:120-127): 8 FP64
+        tensors, 4 FP32 tensors for the engines, sorts and xgemm (FP32 MFMA, the
+        result update formed in FP64 and rounded once); the contract surface
+        stays FP64"""
+        check(lib().dpd_ctx_set_data_precision(self.h, int(prec)))
+        self.data_prec = int(prec)
+
+    def dtype(self):
+        return _torch().float32 if self.data_prec == 4 else _torch().float64
+
     def set_split_k(self, mode=-1):
         """split-K of the packed-panel GEMMs: -1 automatic, 0 off, 1 on"""
         check(lib().dpd_ctx_set_split_k(self.h, mode))
@@ -91,7 +103,7 @@ class Context:
 
     def profile_report(self):
         import json
-        buf = ctypes.create_string_buffer(1 << 16)
+        buf = ctypes.create_string_buffer(1 << 18)
         check(lib().dpd_profile_report(self.h, buf, len(buf)))
         return json.loads(buf.value.decode())
 
@@ -113,7 +125,8 @@ class Context:
 
 class Opt:
     """Mirror of `type opt` (lib_cc/mod_operator.f90:8-25): populations of the
-    four indices, pair packing (ioppq, ioprs), symmetry and an FP64 device array."""
+    four indices, pair packing (ioppq, ioprs), symmetry and a device array (FP64,
+    or FP32 when the context's data precision is 4)."""
 
     def __init__(self, ctx: Context, popp, popq, popr=ZORB, pops=ZORB, ioppq=1, ioprs=1, sym=1, arr=None):
         torch = _torch()
@@ -123,7 +136,7 @@ class Opt:
         t = self._struct()
         self.len = int(lib().dpd_tensor_len(ctx.h, ctypes.byref(t)))
         if arr is None:
-            arr = torch.zeros(max(self.len, 1), dtype=torch.float64, device=f"cuda:{ctx.device}")
+            arr = torch.zeros(max(self.len, 1), dtype=ctx.dtype(), device=f"cuda:{ctx.device}")
         self.arr = arr
 
     def _struct(self, arr=None):
@@ -133,7 +146,7 @@ class Opt:
                 t.pop[k][h] = int(self.pops[k][h])
         t.iop[0], t.iop[1], t.sym = self.ioppq, self.ioprs, self.sym
         a = arr if arr is not None else getattr(self, "arr", None)
-        t.data = a.data_ptr() if a is not None else None
+        t.data = _data(self.ctx, a, self.len) if a is not None else None
         return t
 
     def norm(self):
@@ -141,6 +154,22 @@ class Opt:
 
     def numpy(self):
         return self.arr[: self.len].cpu().numpy()
+
+
+def _data(ctx, x, n=None):
+    """device pointer of a tensor the library reads or writes as elements of the
+    context's data precision (FP64, or FP32 under prec 4); anything else is
+    refused before the pointer is handed over"""
+    torch = _torch()
+    if not isinstance(x, torch.Tensor) or x.device.type != "cuda" or x.device.index != ctx.device:
+        raise ValueError(f"tensor data must be a tensor on cuda:{ctx.device}")
+    if x.dtype != ctx.dtype():
+        raise ValueError(f"tensor data must be {ctx.dtype()} (data precision {ctx.data_prec}), got {x.dtype}")
+    if not x.is_contiguous():
+        raise ValueError("tensor data must be contiguous")
+    if n is not None and x.numel() < n:
+        raise ValueError(f"tensor data holds {x.numel()} elements, the layout needs {n}")
+    return x.data_ptr()
 
 
 def _lab(s):
@@ -208,44 +237,44 @@ def vv2v_442(v1, v2, v3, fact, t1, t2, t3):
 # ---- sort kernels (device tensors in and out) ---------------------------------
 def iexpso(ctx, x, out, popp, popq, ioppq, iexppq, popr, pops, ioprs, iexprs, sym):
     ps = [_i64(p) for p in (popp, popq, popr, pops)]
-    check(lib().dpd_iexpso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ioppq, iexppq, ps[2][1],
+    check(lib().dpd_iexpso(ctx.h, _data(ctx, x), _data(ctx, out), ps[0][1], ps[1][1], ioppq, iexppq, ps[2][1],
                            ps[3][1], ioprs, iexprs, sym))
 
 
 def icompso(ctx, x, out, popp, popq, ioppq, icpq, popr, pops, ioprs, icrs, sym):
     ps = [_i64(p) for p in (popp, popq, popr, pops)]
-    check(lib().dpd_icompso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ioppq, icpq, ps[2][1],
+    check(lib().dpd_icompso(ctx.h, _data(ctx, x), _data(ctx, out), ps[0][1], ps[1][1], ioppq, icpq, ps[2][1],
                             ps[3][1], ioprs, icrs, sym))
 
 
 def isymtrso(ctx, x, out, popp, popq, ioppq, icpq, popr, pops, ioprs, icrs, sym):
     ps = [_i64(p) for p in (popp, popq, popr, pops)]
-    check(lib().dpd_isymtrso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ioppq, icpq, ps[2][1],
+    check(lib().dpd_isymtrso(ctx.h, _data(ctx, x), _data(ctx, out), ps[0][1], ps[1][1], ioppq, icpq, ps[2][1],
                              ps[3][1], ioprs, icrs, sym))
 
 
 def itranspso(ctx, x, out, popp, popq, ioppq, popr, pops, ioprs, sym):
     ps = [_i64(p) for p in (popp, popq, popr, pops)]
-    check(lib().dpd_itranspso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ioppq, ps[2][1], ps[3][1],
+    check(lib().dpd_itranspso(ctx.h, _data(ctx, x), _data(ctx, out), ps[0][1], ps[1][1], ioppq, ps[2][1], ps[3][1],
                               ioprs, sym))
 
 
 def iassymso(ctx, x, out, sym, popp, popq, popr, pops, iop):
     """libsrint/iassymso.f:1-142 (pair antisymmetriser / symmetriser)"""
     ps = [_i64(p) for p in (popp, popq, popr, pops)]
-    check(lib().dpd_iassymso(ctx.h, x.data_ptr(), out.data_ptr(), sym, ps[0][1], ps[1][1], ps[2][1], ps[3][1], iop))
+    check(lib().dpd_iassymso(ctx.h, _data(ctx, x), _data(ctx, out), sym, ps[0][1], ps[1][1], ps[2][1], ps[3][1], iop))
 
 
 def sstgenso(ctx, x, out, pop1, pop2, pop3, pop4, irrepx, type4):
     ps = [_i64(p) for p in (pop1, pop2, pop3, pop4)]
-    check(lib().dpd_sstgenso(ctx.h, x.data_ptr(), out.data_ptr(), ps[0][1], ps[1][1], ps[2][1], ps[3][1], irrepx,
+    check(lib().dpd_sstgenso(ctx.h, _data(ctx, x), _data(ctx, out), ps[0][1], ps[1][1], ps[2][1], ps[3][1], irrepx,
                              type4.encode()))
 
 
 # ---- dense GEMM (libsrint/xgemmso.f router) ----------------------------------
 def xgemm(ctx, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec="fp64"):
-    check(lib().dpd_xgemm(ctx.h, transa.encode(), transb.encode(), m, n, k, alpha, A.data_ptr(), lda,
-                          B.data_ptr(), ldb, beta, C.data_ptr(), ldc, PRECISIONS[prec]))
+    check(lib().dpd_xgemm(ctx.h, transa.encode(), transb.encode(), m, n, k, alpha, _data(ctx, A), lda,
+                          _data(ctx, B), ldb, beta, _data(ctx, C), ldc, PRECISIONS[prec]))
 
 
 # ---- vector primitives ------------------------------------------------------
@@ -457,8 +486,8 @@ class Problem:
 
     def tensor_len(self, name):
         n = int(lib().dpd_problem_tensor_len(self.h, name.encode()))
-        if n < 0:
-            raise KeyError(name)
+        if n < 0:  # unknown name, or a tensor that is not on the device (abcd out of core)
+            raise _lib.DpdError(_lib.DPD_EINVAL, f"{name}: {lib().dpd_last_error().decode()}")
         return n
 
     def get(self, name):

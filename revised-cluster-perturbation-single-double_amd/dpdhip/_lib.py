@@ -17,6 +17,7 @@ c_i64_p = ctypes.POINTER(ctypes.c_int64)
 c_void_p = ctypes.c_void_p
 
 DPD_OK = 0
+DPD_EINVAL = -1
 DPD_ENOTCONV = -8
 
 
@@ -64,6 +65,8 @@ _SIGS = {
     "dpd_ctx_synchronize": (ctypes.c_int, [c_void_p]),
     "dpd_ctx_set_gemm_config": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int]),
     "dpd_ctx_set_split_k": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "dpd_ctx_set_data_precision": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "dpd_ctx_data_precision": (ctypes.c_int, [c_void_p]),
     "dpd_ctx_workspace_bytes": (ctypes.c_size_t, [c_void_p]),
     "dpd_profile_enable": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int]),
     "dpd_profile_report": (ctypes.c_int, [c_void_p, ctypes.c_char_p, i64]),

@@ -42,7 +42,8 @@ def test_sigma_out_of_core_equals_in_core(torch, cfg, symR):
     p.set_precision("fp64")
     p.set_abcd_out_of_core(False)
     assert torch.equal(p.get("abcd"), abcd)
-    assert torch.equal(p.sigma(R), ref)
+    # panels repacked and the pp-ladder's tile timed again: the same sums, maybe in another order
+    assert float((p.sigma(R) - ref).abs().max()) <= 1e-12 * scale
     p.close()
     ctx.close()
 
