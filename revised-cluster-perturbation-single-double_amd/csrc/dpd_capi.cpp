@@ -672,6 +672,8 @@ int dpd_ctx_set_gemm_config(dpd_ctx* c, int cfg, int packing) {
   dpd::bump_mem_gen();
   c->contract_cache.clear();
   c->engine_cache.clear();
+  c->map_cache.clear();
+  c->map_bytes = 0;
   return DPD_OK;
 }
 
@@ -684,6 +686,8 @@ int dpd_ctx_set_split_k(dpd_ctx* c, int mode) {
   dpd::bump_mem_gen();
   c->contract_cache.clear();
   c->engine_cache.clear();
+  c->map_cache.clear();
+  c->map_bytes = 0;
   return DPD_OK;
 }
 

@@ -47,7 +47,7 @@ namespace dpd {
 static void hessenberg(std::vector<double>& a, int n, std::vector<double>& q) {
   q.assign((size_t)n * n, 0.0);
   for (int i = 0; i < n; i++) q[(size_t)i * n + i] = 1.0;
-  std::vector<double> v(n);
+  std::vector<double> v(n), w(n);
   for (int k = 0; k < n - 2; k++) {
     double alpha = 0.0;
     for (int i = k + 1; i < n; i++) alpha += a[i * n + k] * a[i * n + k];
@@ -60,24 +60,34 @@ static void hessenberg(std::vector<double>& a, int n, std::vector<double>& q) {
     double vv = 0.0;
     for (int i = k + 1; i < n; i++) vv += v[i] * v[i];
     if (vv == 0.0) continue;
-    // A = H A H with H = I - 2 v v^T / (v^T v); Q = Q H
-    for (int j = 0; j < n; j++) {
-      double s = 0.0;
-      for (int i = k + 1; i < n; i++) s += v[i] * a[i * n + j];
-      s = 2.0 * s / vv;
-      for (int i = k + 1; i < n; i++) a[i * n + j] -= s * v[i];
+    const double tau = 2.0 / vv;
+    // A = H A H with H = I - tau v v^T; Q = Q H.  From the left: w^T = v^T A
+    // accumulated row by row (contiguous), then A -= tau v w^T
+    std::fill(w.begin(), w.end(), 0.0);
+    for (int i = k + 1; i < n; i++) {
+      const double vi = v[i];
+      const double* ai = &a[(size_t)i * n];
+      for (int j = 0; j < n; j++) w[j] += vi * ai[j];
     }
+    for (int i = k + 1; i < n; i++) {
+      const double f = tau * v[i];
+      double* ai = &a[(size_t)i * n];
+      for (int j = 0; j < n; j++) ai[j] -= f * w[j];
+    }
+    // from the right, row by row: A(i,:) -= tau (A(i,:) v) v^T, Q likewise
     for (int i = 0; i < n; i++) {
+      double* ai = &a[(size_t)i * n];
+      double* qi = &q[(size_t)i * n];
       double s = 0.0, t = 0.0;
       for (int j = k + 1; j < n; j++) {
-        s += a[i * n + j] * v[j];
-        t += q[(size_t)i * n + j] * v[j];
+        s += ai[j] * v[j];
+        t += qi[j] * v[j];
       }
-      s = 2.0 * s / vv;
-      t = 2.0 * t / vv;
+      s *= tau;
+      t *= tau;
       for (int j = k + 1; j < n; j++) {
-        a[i * n + j] -= s * v[j];
-        q[(size_t)i * n + j] -= t * v[j];
+        ai[j] -= s * v[j];
+        qi[j] -= t * v[j];
       }
     }
     for (int i = k + 2; i < n; i++) a[i * n + k] = 0.0;
@@ -320,10 +330,12 @@ int small_eig(int n, const double* Arow, double* wr, double* wi, double* V) {
 // ---------------------------------------------------------------------------
 // Davidson
 // ---------------------------------------------------------------------------
-struct DavWork {
-  DevBuf B, AB, NB, NAB, AQ, E, P, coef, part, partA, partB, partC, partD, dots, amaxp, amaxpi, amaxv, amaxi, wdev, tri,
-      idx, cgs;
-};
+// buf holds at least `bytes` (reallocated only to grow)
+static int grow(DevBuf& buf, size_t bytes) {
+  if (buf.p && buf.bytes >= bytes) return DPD_OK;
+  buf.reset();
+  return buf.alloc(bytes);
+}
 
 static int dsync(hipStream_t s) {
   DPD_HIP_TRY(hipStreamSynchronize(s));
@@ -374,23 +386,28 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   if (dimAbar > 1024) return fail(DPD_EINVAL, "max_dim must be <= 1024");
   const int dimDel = dimAbar - nroot - 5;
   hipStream_t st = stream ? stream : c->stream;
-  DavWork W;
+  // the workspace stays with the problem between solves (allocation and the
+  // pinned staging cost about 3 ms per solve otherwise); one solve at a time
+  if (!P->dav) P->dav.reset(new DavWork);
+  DavWork& W = *P->dav;
   int rc;
   const size_t vb = sizeof(double) * (size_t)n;
   const size_t nd2 = (size_t)dimAbar * dimAbar;
-  if ((rc = W.B.alloc(vb * dimAbar)) || (rc = W.AB.alloc(vb * dimAbar)) || (rc = W.NB.alloc(vb * dimAbar)) ||
-      (rc = W.NAB.alloc(vb * dimAbar)) || (rc = W.AQ.alloc(vb * nroot)) || (rc = W.E.alloc(vb * nroot)) ||
-      (rc = W.P.alloc(vb * nroot)) ||
-      (rc = W.coef.alloc(sizeof(double) * std::max<size_t>(nd2, (size_t)kMgsMulti * dimAbar))) ||
-      (rc = W.part.alloc(sizeof(double) * kRedBlocks * std::max<size_t>(nd2, (size_t)kMgsMulti * dimAbar))) ||
-      (rc = W.partA.alloc(sizeof(double) * kRedBlocks * kMgsMulti)) ||
-      (rc = W.partB.alloc(sizeof(double) * kRedBlocks * kMgsMulti)) ||
-      (rc = W.partC.alloc(sizeof(double) * kRedBlocks)) || (rc = W.partD.alloc(sizeof(double) * kRedBlocks)) || (rc = W.dots.alloc(sizeof(double) * (2 * nd2 + kNormSlots))) ||
-      (rc = W.amaxp.alloc(sizeof(double) * kRedBlocks * nroot)) ||
-      (rc = W.amaxpi.alloc(sizeof(int64_t) * kRedBlocks * nroot)) || (rc = W.amaxv.alloc(sizeof(double) * nroot)) ||
-      (rc = W.amaxi.alloc(sizeof(int64_t) * nroot)) || (rc = W.wdev.alloc(sizeof(double) * nroot)) ||
-      (rc = W.idx.alloc(sizeof(int32_t) * nroot)) || (rc = W.cgs.alloc(256)) ||
-      (rc = W.tri.alloc(sizeof(double) * (2 * kMgsMulti * kMgsMulti + kMgsMulti))))
+  // B/NB and AB/NAB swap pointers at a collapse: each pair is grown to one size
+  const size_t subb = std::max({vb * dimAbar, W.B.bytes, W.NB.bytes, W.AB.bytes, W.NAB.bytes});
+  if ((rc = grow(W.B, subb)) || (rc = grow(W.AB, subb)) || (rc = grow(W.NB, subb)) || (rc = grow(W.NAB, subb)) ||
+      (rc = grow(W.AQ, vb * nroot)) || (rc = grow(W.E, vb * nroot)) || (rc = grow(W.P, vb * nroot)) ||
+      (rc = grow(W.coef, sizeof(double) * std::max<size_t>(nd2, (size_t)kMgsMulti * dimAbar))) ||
+      (rc = grow(W.part, sizeof(double) * kRedBlocks * std::max<size_t>(nd2, (size_t)kMgsMulti * dimAbar))) ||
+      (rc = grow(W.partA, sizeof(double) * kRedBlocks * kMgsMulti)) ||
+      (rc = grow(W.partB, sizeof(double) * kRedBlocks * kMgsMulti)) ||
+      (rc = grow(W.partC, sizeof(double) * kRedBlocks)) || (rc = grow(W.partD, sizeof(double) * kRedBlocks)) ||
+      (rc = grow(W.dots, sizeof(double) * (2 * nd2 + kNormSlots))) ||
+      (rc = grow(W.amaxp, sizeof(double) * kRedBlocks * nroot)) ||
+      (rc = grow(W.amaxpi, sizeof(int64_t) * kRedBlocks * nroot)) || (rc = grow(W.amaxv, sizeof(double) * nroot)) ||
+      (rc = grow(W.amaxi, sizeof(int64_t) * nroot)) || (rc = grow(W.wdev, sizeof(double) * nroot)) ||
+      (rc = grow(W.idx, sizeof(int32_t) * nroot)) || (rc = grow(W.cgs, 256)) ||
+      (rc = grow(W.tri, sizeof(double) * (2 * kMgsMulti * kMgsMulti + kMgsMulti))))
     return rc;
   double* B = (double*)W.B.p;
   double* AB = (double*)W.AB.p;
@@ -407,17 +424,21 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   // small host<->device transfers go through pinned buffers: device-to-host
   // copies land in pin_out and are read after the synchronisation; uploads are
   // staged in pin_in slots that stay untouched until the next synchronisation
-  struct Pinned {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~Pinned() {
-      if (p) (void)hipHostFree(p);
-    }
-  } pin_out, pin_in;
   const size_t pin_bytes = sizeof(double) * (4 * nd2 + 4 * (size_t)dimAbar * nroot + 1024);
-  DPD_HIP_TRY(hipHostMalloc(&pin_out.p, pin_bytes, hipHostMallocDefault));
-  DPD_HIP_TRY(hipHostMalloc(&pin_in.p, pin_bytes, hipHostMallocDefault));
-  pin_out.bytes = pin_in.bytes = pin_bytes;
+  if (W.pin_bytes < pin_bytes) {
+    // no transfer of an earlier solve is in flight: every solve ends synchronised
+    if (W.pin_out) (void)hipHostFree(W.pin_out);
+    if (W.pin_in) (void)hipHostFree(W.pin_in);
+    W.pin_out = W.pin_in = nullptr;
+    W.pin_bytes = 0;
+    DPD_HIP_TRY(hipHostMalloc(&W.pin_out, pin_bytes, hipHostMallocDefault));
+    DPD_HIP_TRY(hipHostMalloc(&W.pin_in, pin_bytes, hipHostMallocDefault));
+    W.pin_bytes = pin_bytes;
+  }
+  struct {
+    void* p;
+    size_t bytes;
+  } pin_out{W.pin_out, W.pin_bytes}, pin_in{W.pin_in, W.pin_bytes};
   size_t in_top = 0;
   auto fetch = [&](void* host, const void* dev, size_t bytes) -> int {
     if (bytes > pin_out.bytes) return fail(DPD_EINVAL, "transfer too large (internal)");

@@ -1515,6 +1515,36 @@ static void prof_close(dpd_ctx* c, ProfRec* pr) {
   if (pr) (void)hipEventRecord(pr->b, c->stream);
 }
 
+// Source codes of a plan's gather (slot 0/1: expansions of operand 1/2, 2: the
+// epilogue), built on the first call outside graph capture and shared through
+// the context's cache by every plan with the same layouts and terms.  Inputs
+// of 2^31 elements or more, and codes beyond kMapCacheBytes in total, keep the
+// decoding kernel.
+constexpr size_t kMapCacheBytes = size_t(2) << 30;
+static const int32_t* plan_code(dpd_ctx* c, ContractPlan* P, int slot, const MapArgs& m) {
+  if (P->code[slot] || P->code_tried[slot] || g_capturing) return P->code[slot];
+  P->code_tried[slot] = true;
+  if (m.nterm < 1 || m.nterm > kMaxTerms || m.in.len >= INT32_MAX || m.out.len <= 0) return nullptr;
+  std::string key((const char*)&m.out, sizeof(DevLayout));
+  key.append((const char*)&m.in, sizeof(DevLayout));
+  for (int t = 0; t < m.nterm; t++) key.append((const char*)m.term[t].perm, 4);
+  key.push_back((char)m.nterm);
+  auto it = c->map_cache.find(key);
+  if (it != c->map_cache.end()) return P->code[slot] = (const int32_t*)it->second->p;
+  const size_t bytes = sizeof(int32_t) * (size_t)m.nterm * (size_t)m.out.len;
+  if (c->map_bytes + bytes > kMapCacheBytes) return nullptr;
+  auto buf = std::make_unique<DevBuf>();
+  if (buf->alloc(bytes)) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (launch_map_build(m, (int32_t*)buf->p, c->stream) != hipSuccess) return nullptr;
+  c->map_bytes += bytes;
+  P->code[slot] = (const int32_t*)buf->p;
+  c->map_cache.emplace(key, std::move(buf));
+  return P->code[slot];
+}
+
 // expansions into the workspace, the scratch result, the packing sources
 static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o2, const Operand& o3, int nbatch,
                     const char* tag) {
@@ -1525,15 +1555,20 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
   r.p2f = o2.data_f32;
   // expansion of operand 1 or 2 into the workspace (FP32 when it only feeds a pack)
   auto expand = [&](const MapArgs& map, const Operand& o, int nb, int64_t lenx, bool f32, const double** dst,
-                    const char* what) -> int {
+                    const char* what, int slot) -> int {
     void* x = ws_take(c, (f32 ? sizeof(float) : sizeof(double)) * lenx * nb);
     if (!x) return DPD_ENOMEM;
     MapArgs m = map;
     m.nbatch = nb;
     m.in_bstride = o.bstride;
     m.out_bstride = lenx;
+    const int32_t* code = plan_code(c, P, slot, m);
     ProfRec* pr = prof_gather(c, tag, what, m, 8, f32 ? 4 : 8);
-    if (f32) {
+    if (code && f32) {
+      DPD_HIP_TRY(launch_gather_code_f32out(m, code, o.data, (float*)x, c->stream));
+    } else if (code) {
+      DPD_HIP_TRY(launch_gather_code(m, code, o.data, (double*)x, c->stream));
+    } else if (f32) {
       if (m.out.len > 0 && m.nbatch > 0) DPD_HIP_TRY(launch_gather_map_f32out(m, o.data, (float*)x, c->stream));
     } else {
       int rc = gather(c, m, o.data, (double*)x);
@@ -1544,12 +1579,12 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
     return DPD_OK;
   };
   if (r.gx1) {
-    int rc = expand(P->map1, o1, r.nb1, P->len1x, r.x1_f32, &r.p1, "x1");
+    int rc = expand(P->map1, o1, r.nb1, P->len1x, r.x1_f32, &r.p1, "x1", 0);
     if (rc) return rc;
     r.p1f = nullptr;
   }
   if (r.gx2) {
-    int rc = expand(P->map2, o2, r.nb2, P->len2x, r.x2_f32, &r.p2, "x2");
+    int rc = expand(P->map2, o2, r.nb2, P->len2x, r.x2_f32, &r.p2, "x2", 1);
     if (rc) return rc;
     r.p2f = nullptr;
   }
@@ -1603,8 +1638,13 @@ static int call_post(dpd_ctx* c, const CallPrep& r, const Operand& o3, int nbatc
   m.nbatch = nbatch;
   m.in_bstride = P->len3x;
   m.out_bstride = o3.bstride;
+  const int32_t* code = plan_code(c, const_cast<ContractPlan*>(P), 2, m);
   ProfRec* pr = prof_gather(c, tag, "epi", m, r.p3_f32 ? 4 : 8, 8);
-  if (r.p3_f32) {
+  if (code && r.p3_f32) {
+    DPD_HIP_TRY(launch_gather_code_f32in(m, code, (const float*)r.p3, const_cast<double*>(o3.data), c->stream));
+  } else if (code) {
+    DPD_HIP_TRY(launch_gather_code(m, code, r.p3, const_cast<double*>(o3.data), c->stream));
+  } else if (r.p3_f32) {
     DPD_HIP_TRY(launch_gather_map_f32in(m, (const float*)r.p3, const_cast<double*>(o3.data), c->stream));
   } else {
     int rc = gather(c, m, r.p3, const_cast<double*>(o3.data));

@@ -121,6 +121,11 @@ struct ContractPlan {
   Desc d3x;         // descriptor of the scratch result (anti != 0)
   MapArgs map1, map2;  // expansion gathers
   MapArgs epi;         // antisymmetrise + compress + accumulate into v3
+  // precomputed source codes of map1, map2, epi (shared by plans with the same
+  // layouts through the context's map cache); null until first used, or when
+  // a gather keeps decoding its indices (input too long for the code)
+  const int32_t* code[3] = {nullptr, nullptr, nullptr};
+  bool code_tried[3] = {false, false, false};
   bool epi_on = false;  // the GEMM writes a scratch result that epi maps into v3
   // labels of the GEMM operands (differ from the caller's when regrouped)
   char lab1x[4], lab2x[4], lab3x[4];
@@ -190,6 +195,9 @@ struct dpd_ctx {
   std::unordered_map<std::string, std::unique_ptr<dpd::ContractPlan>> contract_cache;
   std::unordered_map<std::string, std::unique_ptr<dpd::GemmPlan>> engine_cache;
   std::unordered_map<std::string, std::unique_ptr<dpd::GroupPlan>> group_cache;
+  // source codes of the plans' gathers, keyed by layouts and terms
+  std::unordered_map<std::string, std::unique_ptr<dpd::DevBuf>> map_cache;
+  size_t map_bytes = 0;
   std::vector<char> pinned_scratch;
 };
 

@@ -182,6 +182,190 @@ hipError_t launch_gather_map_f32(const MapArgs& a, const float* in, float* out, 
   return gather_map_t<float, float>(a, in, out, s);
 }
 
+// ---------------------------------------------------------------------------
+// precomputed gathers: the source of every output element and term decoded
+// once per layout (k_map_build) into a code, +(address + 1) or -(address + 1)
+// for the sign of the mirrored element, 0 where the term contributes nothing;
+// k_gather_code then needs no index arithmetic and reads each code once for
+// all tensors of the batch.  The terms are summed in the same order and with
+// the same products as k_gather_cols, so the results are the same bits.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_map_build(MapArgs a_arg, int32_t* __restrict__ code) {
+  __shared__ MapArgs a;
+  stage_map(a_arg, a);
+  const int64_t n = a.out.len;
+  const int nt = a.nterm;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (int64_t)gridDim.x * blockDim.x) {
+    int h[4];
+    int64_t x[4];
+    decode_elem(a.out, o, h, x);
+    for (int t = 0; t < nt; t++) {
+      int hi[4];
+      int64_t xi[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        hi[a.term[t].perm[i]] = h[i];
+        xi[a.term[t].perm[i]] = x[i];
+      }
+      double sg;
+      const int64_t ad = fetch_addr(a.in, hi[0], hi[1], hi[2], hi[3], xi[0], xi[1], xi[2], xi[3], &sg);
+      int32_t cd = 0;
+      if (ad >= 0 && sg != 0.0) cd = sg > 0.0 ? (int32_t)(ad + 1) : -(int32_t)(ad + 1);
+      code[o * nt + t] = cd;
+    }
+  }
+}
+
+hipError_t launch_map_build(const MapArgs& a, int32_t* code, hipStream_t s) {
+  if (a.out.len == 0 || a.nterm < 1) return hipSuccess;
+  int64_t b = (a.out.len + 255) / 256;
+  if (b > 16384) b = 16384;
+  hipLaunchKernelGGL(k_map_build, dim3((unsigned)b), dim3(256), 0, s, a, code);
+  return hipGetLastError();
+}
+
+struct CodeArgs {
+  double coef[kMaxTerms];
+  double alpha, beta;
+  int64_t n, in_bstride, out_bstride;
+  int nbatch;
+};
+
+template <typename TIN, typename TOUT, int NT>
+__global__ __launch_bounds__(256) void k_gather_code(CodeArgs g, const int32_t* __restrict__ code,
+                                                     const TIN* __restrict__ in, TOUT* __restrict__ out) {
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.n; o += (int64_t)gridDim.x * blockDim.x) {
+    int32_t cd[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) cd[t] = code[o * NT + t];
+    int64_t ad[NT];
+    double cf[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      ad[t] = cd[t] > 0 ? (int64_t)cd[t] - 1 : (int64_t)(-cd[t]) - 1;
+      cf[t] = cd[t] > 0 ? g.coef[t] : -g.coef[t];
+    }
+#pragma unroll 2
+    for (int b = 0; b < g.nbatch; b++) {
+      const TIN* inb = in + (int64_t)b * g.in_bstride;
+      TOUT* outb = out + (int64_t)b * g.out_bstride;
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+        if (cd[t] != 0) acc += cf[t] * (double)inb[ad[t]];
+      double v = g.alpha * acc;
+      if (g.beta != 0.0) v = g.beta * (double)outb[o] + v;
+      outb[o] = (TOUT)v;
+    }
+  }
+}
+
+template <typename TIN, typename TOUT>
+static hipError_t gather_code_t(const MapArgs& a, const int32_t* code, const TIN* in, TOUT* out, hipStream_t s) {
+  if (a.out.len == 0 || a.nbatch == 0) return hipSuccess;
+  CodeArgs g;
+  for (int t = 0; t < kMaxTerms; t++) g.coef[t] = t < a.nterm ? a.term[t].coef : 0.0;
+  g.alpha = a.alpha;
+  g.beta = a.beta;
+  g.n = a.out.len;
+  g.in_bstride = a.in_bstride;
+  g.out_bstride = a.out_bstride;
+  g.nbatch = a.nbatch;
+  int64_t b = (a.out.len + 255) / 256;
+  if (b > 16384) b = 16384;
+  const dim3 grid((unsigned)b), blk(256);
+  switch (a.nterm) {
+    case 1: hipLaunchKernelGGL((k_gather_code<TIN, TOUT, 1>), grid, blk, 0, s, g, code, in, out); break;
+    case 2: hipLaunchKernelGGL((k_gather_code<TIN, TOUT, 2>), grid, blk, 0, s, g, code, in, out); break;
+    case 3: hipLaunchKernelGGL((k_gather_code<TIN, TOUT, 3>), grid, blk, 0, s, g, code, in, out); break;
+    case 4: hipLaunchKernelGGL((k_gather_code<TIN, TOUT, 4>), grid, blk, 0, s, g, code, in, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_code(const MapArgs& a, const int32_t* code, const double* in, double* out, hipStream_t s) {
+  return gather_code_t<double, double>(a, code, in, out, s);
+}
+hipError_t launch_gather_code_f32in(const MapArgs& a, const int32_t* code, const float* in, double* out,
+                                    hipStream_t s) {
+  return gather_code_t<float, double>(a, code, in, out, s);
+}
+hipError_t launch_gather_code_f32out(const MapArgs& a, const int32_t* code, const double* in, float* out,
+                                     hipStream_t s) {
+  return gather_code_t<double, float>(a, code, in, out, s);
+}
+
+// ===========================================================================
+// singles-from-singles terms of the sigma builders in one launch:
+//   c(a,i) += sum_fn W(fn,ai) r(f,n) + sum_e Fac(a,e) r(e,i) - sum_m r(a,m) Fki(m,i)
+// One block per output element (a,i) of every vector; W is the dense l1 x l1
+// block of the (v,o,v,o) operator whose pairs have R1's symmetry (rows and
+// columns both in R1's element order), Fac/Fki the (v,v)/(o,o) Fock-like
+// blocks (absent for SF-CIS).  FP64; fixed-order block reductions.
+// ===========================================================================
+template <int NV>
+__global__ __launch_bounds__(256) void k_singles(SinglesArgs g, const double* __restrict__ R, double* __restrict__ S) {
+  const int64_t col = blockIdx.x;
+  const int v0 = blockIdx.y * NV;
+  const int nv = min(NV, g.nvec - v0);
+  int h = 0;
+  while (h + 1 < g.nblk && col >= g.start[h + 1]) h++;
+  const int64_t loc = col - g.start[h];
+  const int na = g.na[h];
+  const int a = (int)(loc % na), i = (int)(loc / na);
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) acc[v] = 0.0;
+  const double* Rv = R + (int64_t)v0 * g.stride;
+  if (g.W) {
+    const double* wc = g.W + col * g.l1;
+    for (int64_t k = threadIdx.x; k < g.l1; k += blockDim.x) {
+      const double w = wc[k];
+#pragma unroll
+      for (int v = 0; v < NV; v++)
+        if (v < nv) acc[v] += w * Rv[(int64_t)v * g.stride + k];
+    }
+  }
+  if (g.Fac) {
+    const double* f = g.Fac + g.fac_off[h] + a;  // Fac(a,e) at + na*e
+    const double* ri = Rv + g.start[h] + (int64_t)na * i;  // r(e,i) at + e
+    for (int e = threadIdx.x; e < na; e += blockDim.x) {
+      const double fe = f[(int64_t)na * e];
+#pragma unroll
+      for (int v = 0; v < NV; v++)
+        if (v < nv) acc[v] += fe * ri[(int64_t)v * g.stride + e];
+    }
+    const int ni = g.ni[h];
+    const double* k2 = g.Fki + g.fki_off[h] + (int64_t)ni * i;  // Fki(m,i) at + m
+    const double* ra = Rv + g.start[h] + a;                     // r(a,m) at + na*m
+    for (int m = threadIdx.x; m < ni; m += blockDim.x) {
+      const double km = k2[m];
+#pragma unroll
+      for (int v = 0; v < NV; v++)
+        if (v < nv) acc[v] -= ra[(int64_t)v * g.stride + (int64_t)na * m] * km;
+    }
+  }
+  __shared__ double red[NV][256];
+#pragma unroll
+  for (int v = 0; v < NV; v++) red[v][threadIdx.x] = acc[v];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int v = 0; v < NV; v++) red[v][threadIdx.x] += red[v][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < nv) S[(int64_t)(v0 + threadIdx.x) * g.stride + col] += red[threadIdx.x][0];
+}
+
+hipError_t launch_singles(const SinglesArgs& g, const double* R, double* S, hipStream_t s) {
+  if (g.l1 <= 0 || g.nvec <= 0) return hipSuccess;
+  if (g.l1 > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_singles<8>), dim3((unsigned)g.l1, (g.nvec + 7) / 8), dim3(256), 0, s, g, R, S);
+  return hipGetLastError();
+}
+
 // ===========================================================================
 // grouped GEMM
 // ===========================================================================

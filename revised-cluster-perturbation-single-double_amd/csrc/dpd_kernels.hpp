@@ -34,6 +34,27 @@ hipError_t launch_gather_map_f32in(const MapArgs& a, const float* in, double* ou
 hipError_t launch_gather_map_f32out(const MapArgs& a, const double* in, float* out, hipStream_t s);
 // FP32 storage on both sides (tensors of a context whose data precision is 4)
 hipError_t launch_gather_map_f32(const MapArgs& a, const float* in, float* out, hipStream_t s);
+// the same gathers through a precomputed code per output element and term
+// (nterm int32 each): launch_map_build writes it, launch_gather_code* apply it
+hipError_t launch_map_build(const MapArgs& a, int32_t* code, hipStream_t s);
+hipError_t launch_gather_code(const MapArgs& a, const int32_t* code, const double* in, double* out, hipStream_t s);
+hipError_t launch_gather_code_f32in(const MapArgs& a, const int32_t* code, const float* in, double* out,
+                                    hipStream_t s);
+hipError_t launch_gather_code_f32out(const MapArgs& a, const int32_t* code, const double* in, float* out,
+                                     hipStream_t s);
+
+// ---- singles-from-singles sigma terms (W r, Fac r, r Fki) -----------------
+struct SinglesArgs {
+  int64_t l1, stride;  // R1 elements per vector; distance between vectors
+  int nvec, nblk;      // vectors; irrep sub-blocks of R1 (by the irrep of i)
+  int64_t start[9];    // first element of each sub-block (start[nblk] = l1)
+  int32_t na[8], ni[8];
+  int64_t fac_off[8], fki_off[8];  // Fac block of a's irrep, Fki block of i's irrep
+  const double* W;     // dense l1 x l1 block (column (a,i), row (f,n)), or null
+  const double* Fac;   // null: no Fock terms
+  const double* Fki;
+};
+hipError_t launch_singles(const SinglesArgs& g, const double* R, double* S, hipStream_t s);
 
 // ---- grouped DPD GEMM -------------------------------------------------------
 // Every task writes C(m,n) (m,n factorised into up to 3 mixed-radix digits with

@@ -91,7 +91,39 @@ struct SigmaTerm {
   const double* d = nullptr;
   const double* x = nullptr;
   double* y = nullptr;
+  SinglesArgs sg;  // kind 3
 };
+
+// kind 3: the singles-from-singles terms W r (+ Fac r - r Fki) in one launch;
+// W is the (v,o,v,o) operator of symmetry 1 whose pairs have R1's symmetry S
+static SinglesArgs singles_args(const dpd_problem* p, const double* W, bool fock) {
+  const Sym& Sy = p->ctx->sym;
+  const int X = p->symR - 1;
+  const int64_t *v = p->popv, *o = p->popo, *z = p->zorb;
+  const DevLayout LR = make_layout(Sy, mkdesc(v, o, z, z, 1, 1, p->symR));
+  const DevLayout LW = make_layout(Sy, mkdesc(v, o, v, o, 1, 1, 1));
+  const DevLayout LF = make_layout(Sy, mkdesc(v, v, z, z, 1, 1, 1));
+  const DevLayout LK = make_layout(Sy, mkdesc(o, o, z, z, 1, 1, 1));
+  SinglesArgs g;
+  std::memset(&g, 0, sizeof g);
+  g.l1 = LR.len;
+  g.stride = p->dim;
+  for (int hi = 0; hi < Sy.nirrep; hi++) {
+    const int ha = Sy.dp[hi][X];
+    if (v[ha] == 0 || o[hi] == 0) continue;
+    g.start[g.nblk] = LR.blk[0] + LR.offL[X][hi];
+    g.na[g.nblk] = (int32_t)v[ha];
+    g.ni[g.nblk] = (int32_t)o[hi];
+    g.fac_off[g.nblk] = LF.offL[0][ha];
+    g.fki_off[g.nblk] = LK.offL[0][hi];
+    g.nblk++;
+  }
+  g.start[g.nblk] = g.l1;
+  g.W = W + LW.blk[X];
+  g.Fac = fock ? (const double*)p->fac.p : nullptr;
+  g.Fki = fock ? (const double*)p->fki.p : nullptr;
+  return g;
+}
 
 // the terms of sigma = J R for one problem, in order
 static int sigma_terms(dpd_problem* p, const double* R, double* sigma, int64_t nvec, std::vector<SigmaTerm>& T) {
@@ -125,6 +157,15 @@ static int sigma_terms(dpd_problem* p, const double* R, double* sigma, int64_t n
     t.tag = tag;
     T.push_back(t);
   };
+  auto singles = [&](const double* W, bool fock) {
+    SigmaTerm t;
+    t.kind = 3;
+    t.sg = singles_args(p, W, fock);
+    t.sg.nvec = nb;
+    t.x = R;
+    t.y = sigma;
+    T.push_back(t);
+  };
   auto diag = [&](int64_t n, const double* d, const double* x, double* y) {
     SigmaTerm t;
     t.kind = 1;
@@ -149,25 +190,20 @@ static int sigma_terms(dpd_problem* p, const double* R, double* sigma, int64_t n
   if (p->method == 0) {
     // e(a,i) r(a,i)
     diag(p->l1, (const double*)p->diag.p, R, sigma);
-    Operand W = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->iabj.p, F(p->iabj32)), p, 1);
-    con(444, W, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, "c1_r1iabj");
+    // c1_r1iabj: vv2v_422(iabj, r1, c1, 'fnai','fn','ai') as one FP64 launch
+    singles((const double*)p->iabj.p, false);
     return DPD_OK;
   }
   Desc dR2 = mkdesc(v, v, o, o, 0, 0, S);
   Operand R2 = batched(op(dR2, "", 4, R + p->l1), p->dim);
   Operand C2 = batched(op(dR2, "", 4, sigma + p->l1), p->dim);
-  Operand Fac = konst(op(mkdesc(v, v, z, z, 1, 1, 1), "ae=+", 4, (const double*)p->fac.p, F(p->fac32)), p, 2);
-  Operand Fki = konst(op(mkdesc(o, o, z, z, 1, 1, 1), "mi=-", 4, (const double*)p->fki.p, F(p->fki32)), p, 3);
-  Operand Wk = konst(op(mkdesc(v, o, v, o, 1, 1, 1), "fnai", 4, (const double*)p->wkaci.p, F(p->wkaci32)), p, 4);
   Operand abci = konst(op(mkdesc(v, v, v, o, 0, 1, 1), "feam", 4, (const double*)p->abci.p, F(p->abci32)), p, 5);
   Operand ijka = konst(op(mkdesc(o, o, o, v, 0, 1, 1), "mnie", 4, (const double*)p->ijka.p, F(p->ijka32)), p, 6);
   // ---- singles (CISdoo.SD.SF/calAR.f90:41-56)
-  // c1_r1Fae: vv2v_222(Fac, r1, c1, 'ae','ei','ai')
-  con(444, Fac, lab(R1, "ei=-", 4), lab(C1, "ai+-", 4), 1.0, 0, "c1_r1Fae");
-  // c1_r1Fmi: vv2v_222(r1, Fki, c1, -1, 'am','mi','ai')
-  con(444, lab(R1, "am=+", 4), Fki, lab(C1, "ai+-", 4), -1.0, 0, "c1_r1Fmi");
-  // c1_r1iabj: vv2v_422(Wkaci, r1, c1, 'fnai','fn','ai')
-  con(444, Wk, lab(R1, "fn+-", 4), lab(C1, "ai+-", 4), 1.0, 0, "c1_r1iabj");
+  // c1_r1Fae: vv2v_222(Fac, r1, c1, 'ae','ei','ai'), c1_r1Fmi: vv2v_222(r1, Fki, c1,
+  // -1, 'am','mi','ai') and c1_r1iabj: vv2v_422(Wkaci, r1, c1, 'fnai','fn','ai'):
+  // three small products on R1 (a few hundred elements per vector), one FP64 launch
+  singles((const double*)p->wkaci.p, true);
   // c1_r2abci: vv2v_442(abci, r2, c1, 'feam','feim','ai')  (r2 iop 0: fact)
   con(442, abci, lab(R2, "feim", 4), lab(C1, "ai", 2), 1.0, 0, "c1_r2abci");
   // c1_r2ijka: vv2v_442(r2, ijka, c1, -1, 'aemn','mnie','ai')
@@ -208,6 +244,8 @@ static int sigma_terms(dpd_problem* p, const double* R, double* sigma, int64_t n
 static int run_simple_term(dpd_ctx* c, const SigmaTerm& t) {
   if (t.kind == 0) {
     DPD_HIP_TRY(launch_fill(t.n, 0.0, t.y, c->stream));
+  } else if (t.kind == 3) {
+    DPD_HIP_TRY(launch_singles(t.sg, t.x, t.y, c->stream));
   } else {
     DPD_HIP_TRY(launch_hadamard_acc(t.n, t.d, t.x, t.y, t.nb, t.stride, t.stride, c->stream));
   }
@@ -220,7 +258,7 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
   if (rc) return rc;
   dpd_ctx* c = p->ctx;
   for (const SigmaTerm& t : T) {
-    if (t.kind == 0 || t.kind == 1)
+    if (t.kind == 0 || t.kind == 1 || t.kind == 3)
       rc = run_simple_term(c, t);
     else if (t.kind == 2)
       rc = problem_pp_ladder_streamed(p, t.x, t.y, t.nb, t.stride, p->symR, t.fact);
@@ -250,7 +288,7 @@ int problem_sigma_group(dpd_problem* const* ps, const double* const* R, double* 
   std::vector<int> nb(n);
   for (size_t k = 0; k < T[0].size(); k++) {
     const SigmaTerm& t0 = T[0][k];
-    if (t0.kind == 0 || t0.kind == 1) {
+    if (t0.kind == 0 || t0.kind == 1 || t0.kind == 3) {
       for (int i = 0; i < n; i++) {
         int rc = run_simple_term(c, T[i][k]);
         if (rc) return rc;

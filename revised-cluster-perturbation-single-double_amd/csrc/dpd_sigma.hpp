@@ -6,6 +6,23 @@
 
 #include "dpd_engine.hpp"
 
+namespace dpd {
+// Davidson workspace of a problem, kept between solves (grow-only): the
+// subspace (b, Ab and their collapsed copies), the per-root vectors, the
+// reduction scratch and the pinned host staging of the small transfers
+struct DavWork {
+  DevBuf B, AB, NB, NAB, AQ, E, P, coef, part, partA, partB, partC, partD, dots, amaxp, amaxpi, amaxv, amaxi, wdev, tri,
+      idx, cgs;
+  void* pin_out = nullptr;
+  void* pin_in = nullptr;
+  size_t pin_bytes = 0;
+  ~DavWork() {
+    if (pin_out) (void)hipHostFree(pin_out);
+    if (pin_in) (void)hipHostFree(pin_in);
+  }
+};
+}  // namespace dpd
+
 struct dpd_problem {
   dpd_ctx* ctx = nullptr;
   dpd_synth_spec spec;
@@ -32,6 +49,7 @@ struct dpd_problem {
   hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
   std::vector<std::unique_ptr<dpd::GemmPlan>> ooc_plans;  // per (nvec, stride, symmetry, block)
   std::vector<int64_t> ooc_keys;
+  std::unique_ptr<dpd::DavWork> dav;  // Davidson workspace (grow-only, freed with the problem)
   ~dpd_problem();
 };
 
