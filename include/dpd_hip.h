@@ -379,6 +379,15 @@ int64_t dpd_problem_cpsd_len(const dpd_problem *prob);
  * 3 x dpd_problem_cpsd_len() doubles, (T1 | T2) of orders 1, 2, 3 -- the f_T
  * file's records in order. */
 int dpd_problem_cpsd(dpd_problem *prob, dpd_cpsd_opts *opts, double *T);
+/* grCPSd.SF/run.f90 (run1-3.f90, solver.f90): ground-state grCPS(D), the
+ * CIS(Doo) Jacobian as zeroth order -- T^(n) from A0 T^(n) + C^(n) = 0 by DIIS
+ * on (T1 | T2) at n = 1, 2, 3, with the [[V,T^(1)],T^(1)] terms of
+ * calL1_VTT.f90/calL2_VTT.f90 at n = 3; E^(n+1) = <abij| T2^(n) + [T1 T1]^(n)>
+ * (calEcorr.f90, effectT2.f90).  T as for dpd_problem_cpsd (the iCPSd_T_
+ * records); opts->energy: E^(2..4), opts->iterations: DIIS iterations of orders
+ * 2, 3; iterations (host, 3, may be NULL): those of orders 1, 2, 3.  Needs
+ * <ab||ef> on the device. */
+int dpd_problem_grcpsd(dpd_problem *prob, dpd_cpsd_opts *opts, double *T, int64_t *iterations);
 /* EE-CPSd.SF/run.f90 + run_order.f90: EE-CPS(D) excitation energies of nroot
  * CIS roots of the problem's symmetry, order by order.  T: the ground-state
  * amplitudes of dpd_problem_cpsd (device); w0 (host, nroot): CIS energies;

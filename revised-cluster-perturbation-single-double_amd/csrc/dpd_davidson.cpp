@@ -577,14 +577,16 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       DPD_HIP_TRY(hipMemcpyAsync(AQ, AB, vb * nroot, hipMemcpyDeviceToDevice, st));
     } else {
       if ((rc = upload(W.coef.p, vecNeed.data(), sizeof(double) * dimAbar * nroot))) return rc;
-      DPD_HIP_TRY(launch_lincomb(n, AB, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, AQ, st));
-      DPD_HIP_TRY(launch_lincomb(n, B, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, R, st));
+      DPD_HIP_TRY(launch_lincomb2(n, AB, B, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, AQ, R, st));
     }
+    // |q|^2, then in one pass: q and Aq renormalised (:378-386), the residuals
+    // e = Aq - w q and their largest elements (:396-406), and the preconditioned
+    // directions of every root (:417-428), kept for the unconverged ones
     DPD_HIP_TRY(launch_pair_dots(n, R, n, R, n, nroot, part, dots, st));
-    DPD_HIP_TRY(launch_renorm_pair(n, dots, R, AQ, nroot, st));
-    // residuals (:396-406)
     if ((rc = upload(wdev, eigenVal.data(), sizeof(double) * nroot))) return rc;
-    if ((rc = residuals(AQ, mv.data()))) return rc;
+    DPD_HIP_TRY(launch_ritz_resid(n, dots, wdev, diag, R, AQ, E, PV, nroot, (double*)W.amaxp.p,
+                                  (int64_t*)W.amaxpi.p, (double*)W.amaxv.p, (int64_t*)W.amaxi.p, st));
+    if ((rc = fetch(mv.data(), W.amaxv.p, sizeof(double) * nroot))) return rc;
     int nconv = 0;
     cand.clear();
     for (int iroot = 0; iroot < nroot; iroot++) {
@@ -625,8 +627,12 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       }
     };
     if (dimFree > 0 && !cand.empty()) {
+      // the candidates' directions moved to the front of PV (root cand[k] to
+      // slot k; cand is increasing, so no source is overwritten before it is read)
       if ((rc = upload(W.idx.p, cand.data(), sizeof(int32_t) * cand.size()))) return rc;
-      DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p, (int)cand.size(), diag, E, PV, st));
+      for (size_t k = 0; k < cand.size(); k++)
+        if (cand[k] != (int32_t)k)
+          DPD_HIP_TRY(hipMemcpyAsync(PV + k * n, PV + (size_t)cand[k] * n, vb, hipMemcpyDeviceToDevice, st));
     }
     // SchmidtS (:687-726) root by root: normalise, remove the components along
     // every basis vector (the new ones of this iteration included), normalise;

@@ -801,11 +801,12 @@ static int pack_sides(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int mode, const P
 }
 
 int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double* C, double f1,
-                    double f2, hipEvent_t ev0, hipEvent_t ev1, bool c_f32) {
+                    double f2, hipEvent_t ev0, hipEvent_t ev1, bool c_f32, hipEvent_t ev_pack) {
   GemmVariant& V = p.pk[mode];
   const void* side_ptr[2];
   int rc0 = pack_sides(c, p, V, mode, A, B, side_ptr);
   if (rc0) return rc0;
+  if (ev_pack) DPD_HIP_TRY(hipEventRecord(ev_pack, c->stream));
   if (V.nwg == 0) {
     if (ev0) DPD_HIP_TRY(hipEventRecord(ev0, c->stream));
     if (ev1) DPD_HIP_TRY(hipEventRecord(ev1, c->stream));
@@ -1694,8 +1695,16 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   // the packed path records the events around the GEMM kernel itself
   if (pr && !r.packed) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
   if (r.packed) {
+    // the packing of the per-call operands, timed as a gather entry "<tag>:pack"
+    ProfRec* pp = nullptr;
+    if (pr) {
+      thread_local std::string pname;
+      pname = std::string(tag) + ":pack";
+      pp = prof_open(c, pname.c_str(), 0, prec, 0, 1, 0, 0, 1);
+      if (pp) DPD_HIP_TRY(hipEventRecord(pp->a, c->stream));
+    }
     rc = run_gemm_packed(c, P->gemm, prec, r.a, r.b, r.p3, fact, r.beta, pr ? pr->a : nullptr,
-                         pr ? pr->b : nullptr, r.p3_f32);
+                         pr ? pr->b : nullptr, r.p3_f32, pp ? pp->b : nullptr);
     // first call of this plan in this precision: time every tile shape on these
     // operands (results go to a scratch C) and keep the fastest for later calls
     if (!rc && c->gemm_cfg == -1 && !P->gemm.pk[prec].tuned) {

@@ -243,9 +243,10 @@ struct PackSrc {
   int64_t xlen = 0, bstride = 0;
 };
 // packed-panel GEMM; uncached operands are packed into workspace the caller reserved
+// ev0/ev1 bracket the GEMM kernel; ev_pack is recorded when the packing is done
 int run_gemm_packed(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const PackSrc& B, double* C, double f1,
                     double f2, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
-                    bool c_f32 = false);
+                    bool c_f32 = false, hipEvent_t ev_pack = nullptr);
 // host part of a packed-panel tiling (task tables, panel offsets, split-K);
 // touches no device memory: V.d_tasks / V.d_split stay for the caller to set
 int layout_variant(const GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k);

@@ -808,8 +808,14 @@ __global__ void k_copy(int64_t n, const double* __restrict__ x, double* __restri
 // out_j = sum_k coef[k + j*ldc] X_k  (in the reference's order: k ascending); every thread
 // forms up to kLinOut outputs so each X element is read once per grid row
 constexpr int kLinOut = 8;
+// blockIdx.z = 1 forms the same combinations of a second set (X2 -> out2)
 __global__ void k_lincomb(int64_t n, const double* __restrict__ X, int64_t xs, int nk,
-                          const double* __restrict__ coef, int ldc, int nout, double* __restrict__ out) {
+                          const double* __restrict__ coef, int ldc, int nout, double* __restrict__ out,
+                          const double* __restrict__ X2, double* __restrict__ out2) {
+  if (blockIdx.z == 1) {
+    X = X2;
+    out = out2;
+  }
   const int j0 = blockIdx.y * kLinOut;
   const int nj = min(kLinOut, nout - j0);
   // columns past nj repeat the last one (computed, never stored): no branches,
@@ -903,7 +909,14 @@ hipError_t launch_lincomb(int64_t n, const double* X, int64_t xs, int nk, const 
                           int ldc, int nout, double* out, hipStream_t s) {
   if (nout == 0) return hipSuccess;
   hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n), (nout + kLinOut - 1) / kLinOut), dim3(256), 0, s, n, X, xs,
-                     nk, coef, ldc, nout, out);
+                     nk, coef, ldc, nout, out, (const double*)nullptr, (double*)nullptr);
+  return hipGetLastError();
+}
+hipError_t launch_lincomb2(int64_t n, const double* X, const double* X2, int64_t xs, int nk, const double* coef,
+                           int ldc, int nout, double* out, double* out2, hipStream_t s) {
+  if (nout == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n), (nout + kLinOut - 1) / kLinOut, 2), dim3(256), 0, s, n, X, xs,
+                     nk, coef, ldc, nout, out, X2, out2);
   return hipGetLastError();
 }
 hipError_t launch_precondition(int64_t n, double w, const double* d, const double* e, double* p,
@@ -1169,6 +1182,70 @@ __global__ __launch_bounds__(256) void k_resid_amax_partial(int64_t n, const dou
     pv[(int64_t)v * gridDim.x + blockIdx.x] = sv[0];
     pi[(int64_t)v * gridDim.x + blockIdx.x] = si[0];
   }
+}
+
+// Ritz vector, residual and preconditioned direction of every root in one pass
+// (lib_cc/mod_davidson.f90:378-386, :396-406, :417-428): q = R, Aq = AQ scaled
+// by 1/|q| when ||q| - 1| > 1e-10 (q written back), e = Aq - w q into E,
+// p = e / (w - d) (|w - d| clamped to 1e-4) into P, and the largest |e| per root.
+// The same operations as k_renorm_pair + k_resid_amax_partial + k_precond_batched.
+__global__ __launch_bounds__(256) void k_ritz_resid(int64_t n, const double* nrm2, const double* w,
+                                                    const double* __restrict__ d, double* __restrict__ q,
+                                                    const double* __restrict__ aq, double* __restrict__ e,
+                                                    double* __restrict__ p, double* pv, int64_t* pi) {
+  __shared__ double sv[256];
+  __shared__ int64_t si[256];
+  const int v = blockIdx.y;
+  const double wv = w[v];
+  const double t = sqrt(nrm2[v]);
+  const bool scale = fabs(t - 1.0) > 1e-10;
+  const double f = 1.0 / t;
+  double* qv = q + (int64_t)v * n;
+  const double* aqv = aq + (int64_t)v * n;
+  double* ev = e + (int64_t)v * n;
+  double* pvv = p + (int64_t)v * n;
+  double best = 0.0;
+  int64_t bi = INT64_MAX;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double x = qv[i], y = aqv[i];
+    if (scale) {
+      x *= f;
+      y *= f;
+      qv[i] = x;
+    }
+    const double r = y + (-wv) * x;
+    ev[i] = r;
+    double tt = wv - d[i];
+    if (fabs(tt) < 1.0e-4) tt = (tt < 0.0) ? -1.0e-4 : 1.0e-4;
+    pvv[i] = r / tt;
+    amax_merge(best, bi, r, i);
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      double v1 = sv[threadIdx.x];
+      int64_t i1 = si[threadIdx.x];
+      amax_merge(v1, i1, sv[threadIdx.x + off], si[threadIdx.x + off]);
+      sv[threadIdx.x] = v1;
+      si[threadIdx.x] = i1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[(int64_t)v * gridDim.x + blockIdx.x] = sv[0];
+    pi[(int64_t)v * gridDim.x + blockIdx.x] = si[0];
+  }
+}
+
+hipError_t launch_ritz_resid(int64_t n, const double* nrm2, const double* w, const double* d, double* q,
+                             const double* aq, double* e, double* p, int nv, double* pv, int64_t* pi, double* ov,
+                             int64_t* oi, hipStream_t s) {
+  if (nv == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ritz_resid, dim3(kRedBlocks, nv), dim3(256), 0, s, n, nrm2, w, d, q, aq, e, p, pv, pi);
+  hipLaunchKernelGGL(k_amax_final, dim3(nv), dim3(256), 0, s, kRedBlocks, pv, pi, ov, oi);
+  return hipGetLastError();
 }
 
 hipError_t launch_resid_amax(int64_t n, const double* aq, const double* q, const double* w, double* e, int nv,

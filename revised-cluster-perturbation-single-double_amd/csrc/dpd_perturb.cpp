@@ -15,6 +15,8 @@
 //       [J^(0) - w0] R^(n) = sum_{k=2}^{n} w^(k) R^(n-k) - C^(n),  R^(n) orthogonal
 //       to the CIS vector L, C^(n) = sum_k J^(k) R^(n-k), w^(n) = <L|C1^(n)>,
 //       orders 1-3 with the terms CPS(D-3) keeps.
+//   * grCPS(D) ground state (grCPSd.SF/run.f90, solver.f90): the CIS(Doo)
+//     Jacobian as zeroth order, T^(1..3) by DIIS on (T1 | T2) at every order.
 // Every contraction is one call of the contract surface on the problem's
 // integral blocks; the DIIS history (libsrint/dosodiis2_d.f, which keeps it on
 // disk) stays in HBM, only its small error matrix is solved on the host.
@@ -196,6 +198,33 @@ struct Work {
   int Wmnij_r1ijka(double* W, const double* R1, int s, double f) {
     return k(424, I("ijka", "mnie"), opnd(dR1(s), "ej", 2, R1), opnd(dWoooo(s), "mnij", 4, W), f, 2,
              "Wmnij_r1ijka");
+  }
+  // ---- grCPSd.SF terms (lib_cc/*.f90) ------------------------------------------
+  Desc dWvvvo(int s) const { return mk(v, v, v, o, 0, 1, s); }
+  int Wkbij_r1iabj(double* W, const double* R1, int s, double f) {
+    return k(424, I("iabj", "ckbj"), opnd(dR1(s), "ci", 2, R1), opnd(dWovoo(s), "ijkb", 4, W), -f, 1,
+             "Wkbij_r1iabj");
+  }
+  int Wabdj_r1abcd(double* W, const double* R1, int s, double f) {  // both in core
+    return k(424, I("abcd", "abdc"), opnd(dR1(s), "cj", 2, R1), opnd(dWvvvo(s), "abdj", 4, W), f, 0,
+             "Wabdj_r1abcd");
+  }
+  int Wkbij_r1Uklij(double* W, const double* R1, int s, double f) {
+    return k(424, I("ijkl", "klij"), opnd(dR1(s), "bl", 2, R1), opnd(dWovoo(s), "ijkb", 4, W), -f, 0,
+             "Wkbij_r1Uklij");
+  }
+  int Fme_r1abij(double* F, const double* R1, int s, double f) {
+    return k(444, I("abij", "efmn"), opnd(dR1(s), "fn+-", 4, R1), opnd(dR1(s), "em+-", 4, F), f, 0, "Fme_r1abij");
+  }
+  int c1_r2Fme(double* C1, const double* R2, int s, const double* F, int sf, double f) {
+    return k(444, opnd(dR2(s), "aeim", 4, R2), opnd(dR1(sf), "em+-", 4, F), opnd(dR1(s), "ai+-", 4, C1), f, 0,
+             "c1_r2Fme");
+  }
+  // effectT2.f90 (p_effectT2 types 1 and 2): T2 += f [t1a(a,i) t1b(b,j)]
+  // antisymmetrised in ij (type 1, t1a = t1b) or in ab and ij (type 2)
+  int effectT2(double* T2, const double* T1a, const double* T1b, double f, int type) {
+    return k(444, opnd(dR1(1), "ai+-", 4, T1a), opnd(dR1(1), "bj+-", 4, T1b), opnd(dR2(1), "abij", 4, T2), f,
+             type == 1 ? 2 : 3, "effectT2");
   }
   // ---- vector helpers --------------------------------------------------------
   EorbOff eoff() const {
@@ -434,6 +463,180 @@ int cpsd_run(Work& w, dpd_cpsd_opts* o, double* T) {
   return DPD_OK;
 }
 
+// grCPSd.SF/run.f90 (run1-3.f90, solver.f90, calA0R1.f90, calA0R2.f90,
+// calA1R1.f90 (calA2R1), calA1R2.f90, calC1.f90, calC2.f90, calL1_VTT.f90,
+// calL2_VTT.f90, calEcorr.f90): the ground state with the CIS(Doo) Jacobian as
+// zeroth order.  For n = 1, 2, 3:  T = (C + A0' T) / D  (A0' = A0 without its
+// orbital-energy diagonal D) by Jacobi steps with DIIS on (T1 | T2);
+// C^(1) = (0, <ab||ij>), C^(2) = (0, [V,T2^(1)]), C^(3) = (-D1 T1^(1) + L1, [V,T2^(2)] + L2)
+// with D1 the MP2 dressing of A0's singles block and L1/L2 the [[V,T^(1)],T^(1)]
+// terms; E^(n+1) = <abij| T2^(n) + [T1 T1]^(n)>.
+struct GrInt {
+  DevBuf buf;
+  double *mp2, *fli, *fad, *wk, *wdiff, *gli, *gad, *gld, *tmp, *t2e;
+};
+
+int gr_setup(Work& w, GrInt& X) {
+  const int64_t l2 = w.len(w.dR2(1)), lfoo = w.len(w.dFoo(1)), lfvv = w.len(w.dFvv(1)), lw = w.len(w.dWovvo(1));
+  const int64_t l1 = w.len(w.dR1(1));
+  const int64_t ltmp = std::max({w.len(w.dWovoo(1)), w.len(w.dWvvvo(1)), lfoo, lfvv, lw, w.len(w.dWoooo(1))});
+  const int64_t tot = l2 + lfoo + lfvv + 2 * lw + lfoo + lfvv + l1 + ltmp + l2 + 16;
+  int rc = X.buf.alloc(sizeof(double) * (size_t)tot);
+  if (rc) return rc;
+  double* q = (double*)X.buf.p;
+  auto take = [&](int64_t n) {
+    double* r = q;
+    q += n + 1;
+    return r;
+  };
+  X.mp2 = take(l2);
+  X.fli = take(lfoo);
+  X.fad = take(lfvv);
+  X.wk = take(lw);
+  X.wdiff = take(lw);
+  X.gli = take(lfoo);
+  X.gad = take(lfvv);
+  X.gld = take(l1);
+  X.tmp = take(ltmp);
+  X.t2e = take(l2);
+  return DPD_OK;
+}
+
+// C1 += -D1 T1 + L1 (calA2R1 in calA1R1.f90, calL1_VTT.f90), T = T^(1)
+int gr_c1_order3(Work& w, GrInt& X, double* C1, const double* T1, const double* T2) {
+  const int64_t lw = w.len(w.dWovvo(1)), lfoo = w.len(w.dFoo(1)), lfvv = w.len(w.dFvv(1)), l1 = w.len(w.dR1(1));
+  const double* iabj = (const double*)w.p->iabj.p;
+  int rc;
+  // calA2R1: Wkaci - iabj (the MP2 part), c1_r1iabj x -1, c1_r1Fmi / c1_r1Fae x +1
+  DPD_HIP_TRY(launch_copy(lw, X.wk, X.wdiff, w.st));
+  DPD_HIP_TRY(launch_axpy(lw, -1.0, iabj, X.wdiff, w.st));
+  if ((rc = w.c1_r1iabj(C1, T1, 1, opnd(w.dWovvo(1), "fnai", 4, X.wdiff), -1.0)) ||
+      (rc = w.c1_r1Fmi(C1, T1, 1, X.fli, 1, 1.0)) || (rc = w.c1_r1Fae(C1, T1, 1, X.fad, 1, 1.0)))
+    return rc;
+  // calL1_VTT: Gli, Gad, Gld from T^(1), then their singles terms
+  DPD_HIP_TRY(launch_fill(lfoo, 0.0, X.gli, w.st));
+  DPD_HIP_TRY(launch_fill(lfvv, 0.0, X.gad, w.st));
+  DPD_HIP_TRY(launch_fill(l1, 0.0, X.gld, w.st));
+  if ((rc = w.Fmi_r1ijka(X.gli, T1, 1, -1.0)) || (rc = w.Fmi_r2abij(X.gli, T2, 1, -1.0)) ||
+      (rc = w.Fae_r1abci(X.gad, T1, 1, -1.0)) || (rc = w.Fae_r2abij(X.gad, T2, 1, -1.0)) ||
+      (rc = w.Fme_r1abij(X.gld, T1, 1, 1.0)) || (rc = w.c1_r1Fmi(C1, T1, 1, X.gli, 1, -1.0)) ||
+      (rc = w.c1_r1Fae(C1, T1, 1, X.gad, 1, -1.0)) || (rc = w.c1_r2Fme(C1, T2, 1, X.gld, 1, 1.0)))
+    return rc;
+  return DPD_OK;
+}
+
+// C2 += L2 (calL2_VTT.f90), T = T^(1)
+int gr_l2_vtt(Work& w, GrInt& X, double* C2, const double* T1, const double* T2) {
+  int rc;
+  double* t = X.tmp;
+  auto clear = [&](const Desc& d) { return launch_fill(w.len(d), 0.0, t, w.st); };
+  DPD_HIP_TRY(clear(w.dWovoo(1)));
+  if ((rc = w.Wkbij_r1iabj(t, T1, 1, 1.0)) ||
+      (rc = w.c2_r1ijka(C2, T1, 1, opnd(w.dWovoo(1), "ijmb", 4, t), -1.0)))
+    return rc;
+  DPD_HIP_TRY(clear(w.dWvvvo(1)));
+  if ((rc = w.Wabdj_r1abcd(t, T1, 1, 1.0)) ||
+      (rc = w.k(424, opnd(w.dWvvvo(1), "abej", 4, t), opnd(w.dR1(1), "ei", 2, T1), opnd(w.dR2(1), "abij", 4, C2),
+                0.5, 2, "c2_r1abci")))
+    return rc;
+  DPD_HIP_TRY(clear(w.dWovoo(1)));
+  if ((rc = w.Wkbij_r1Uklij(t, T1, 1, 1.0)) ||
+      (rc = w.c2_r1ijka(C2, T1, 1, opnd(w.dWovoo(1), "ijmb", 4, t), 0.5)))
+    return rc;
+  DPD_HIP_TRY(clear(w.dFoo(1)));
+  if ((rc = w.Fmi_r1ijka(t, T1, 1, 1.0)) || (rc = w.Fmi_r2abij(t, T2, 1, 1.0)) ||
+      (rc = w.c2_r2Fmi(C2, 1, T2, 1, t, 1, 1.0)))
+    return rc;
+  DPD_HIP_TRY(clear(w.dFvv(1)));
+  if ((rc = w.Fae_r1abci(t, T1, 1, 1.0)) || (rc = w.Fae_r2abij(t, T2, 1, 1.0)) ||
+      (rc = w.c2_r2Fae(C2, 1, T2, 1, t, 1, 1.0)))
+    return rc;
+  DPD_HIP_TRY(clear(w.dWovvo(1)));
+  if ((rc = w.Wmbej_r1ijka(t, T1, 1, 1.0)) || (rc = w.Wmbej_r1abci(t, T1, 1, 1.0)) ||
+      (rc = w.Wmbej_r2abij(t, T2, 1, 1.0)) ||
+      (rc = w.c2_r2Wmbej(C2, 1, T2, 1, opnd(w.dWovvo(1), "embj", 4, t), 1.0)))
+    return rc;
+  DPD_HIP_TRY(clear(w.dWovoo(1)));
+  if ((rc = w.Wkbij_r2abci(t, T2, 1, 1.0)) ||
+      (rc = w.c2_r1ijka(C2, T1, 1, opnd(w.dWovoo(1), "ijmb", 4, t), 1.0)))
+    return rc;
+  DPD_HIP_TRY(clear(w.dWoooo(1)));
+  if ((rc = w.Wmnij_r1ijka(t, T1, 1, 0.5)) || (rc = w.Wmnij_r2abij(t, T2, 1, 0.5)) ||
+      (rc = w.c2_r2Wmnij(C2, 1, T2, 1, opnd(w.dWoooo(1), "mnij", 4, t), 2.0)))
+    return rc;
+  return DPD_OK;
+}
+
+// T (device): T^(1..3) as (T1 | T2) per order; it3 (host, 3): DIIS iterations
+int grcpsd_run(Work& w, dpd_cpsd_opts* o, double* T, int64_t* it3) {
+  if (w.p->abcd_host) return fail(DPD_EINVAL, "grCPS(D) needs <ab||ef> on the device (Wabdj_r1abcd)");
+  const int64_t l1 = w.len(w.dR1(1)), l2 = w.len(w.dR2(1)), lt = l1 + l2, lw = w.len(w.dWovvo(1));
+  DevBuf den1, den2, Cb, Rb;
+  GrInt X;
+  int rc;
+  if ((rc = den1.alloc(sizeof(double) * std::max<int64_t>(l1, 1))) ||
+      (rc = den2.alloc(sizeof(double) * std::max<int64_t>(l2, 1))) ||
+      (rc = Cb.alloc(sizeof(double) * (size_t)(lt + 1))) || (rc = Rb.alloc(sizeof(double) * (size_t)(lt + 1))) ||
+      (rc = gr_setup(w, X)))
+    return rc;
+  double* d1 = (double*)den1.p;
+  double* d2 = (double*)den2.p;
+  if ((rc = w.denominators(w.dR1(1), 4, d1)) || (rc = w.denominators(w.dR2(1), 3, d2))) return rc;
+  double *C1 = (double*)Cb.p, *C2 = C1 + l1, *R1 = (double*)Rb.p, *R2 = R1 + l1;
+  auto t1 = [&](int n) { return T + (size_t)(n - 1) * lt; };
+  auto t2 = [&](int n) { return T + (size_t)(n - 1) * lt + l1; };
+  const double* abij = (const double*)w.p->abij.p;
+  // run.f90:29-47: MP2t, Fli = -Fmi(MP2t), Fad = -Fae(MP2t), Wkaci = iabj + 2 Wmbej(MP2t)
+  DPD_HIP_TRY(launch_div(l2, abij, d2, X.mp2, w.st));
+  DPD_HIP_TRY(launch_fill(w.len(w.dFoo(1)), 0.0, X.fli, w.st));
+  DPD_HIP_TRY(launch_fill(w.len(w.dFvv(1)), 0.0, X.fad, w.st));
+  DPD_HIP_TRY(launch_copy(lw, (const double*)w.p->iabj.p, X.wk, w.st));
+  if ((rc = w.Fmi_r2abij(X.fli, X.mp2, 1, -1.0)) || (rc = w.Fae_r2abij(X.fad, X.mp2, 1, -1.0)) ||
+      (rc = w.Wmbej_r2abij(X.wk, X.mp2, 1, 2.0)))
+    return rc;
+  const Operand Wk = opnd(w.dWovvo(1), "fnai", 4, X.wk);
+  for (int n = 1; n <= 3; n++) {
+    DPD_HIP_TRY(launch_fill(lt, 0.0, C1, w.st));
+    if (n == 1) {
+      DPD_HIP_TRY(launch_copy(l2, abij, C2, w.st));  // run1.f90:18-19
+    } else {
+      if (n == 3 && (rc = gr_c1_order3(w, X, C1, t1(1), t2(1)))) return rc;
+      if ((rc = w.c2_vr2(C2, t2(n - 1), 1, 1.0))) return rc;  // calA1R2: ring + hh + pp ladder
+      if (n == 3 && (rc = gr_l2_vtt(w, X, C2, t1(1), t2(1)))) return rc;
+    }
+    // solver.f90
+    Diis d;
+    if ((rc = d.init(lt, (int)o->diis_dim))) return rc;
+    DPD_HIP_TRY(launch_copy(lt, C1, R1, w.st));
+    int it = 0;
+    double E = 0.0;
+    while (true) {
+      it++;
+      DPD_HIP_TRY(launch_div(l1, R1, d1, t1(n), w.st));  // denomT1
+      DPD_HIP_TRY(launch_div(l2, R2, d2, t2(n), w.st));  // denomT2
+      double err = 0.0;
+      if ((rc = d.step(w, it, t1(n), &err))) return rc;
+      // calEcorr.f90: E = <abij| T2 + [T1 T1]>
+      DPD_HIP_TRY(launch_copy(l2, t2(n), X.t2e, w.st));
+      if (n == 2 && (rc = w.effectT2(X.t2e, t1(1), t1(1), 1.0, 1))) return rc;
+      if (n == 3 && (rc = w.effectT2(X.t2e, t1(2), t1(1), 1.0, 2))) return rc;
+      if ((rc = w.dots(l2, X.t2e, abij, 0, 1, &E))) return rc;
+      if (it > 1 && std::fabs(err) < o->diis_max_err) break;
+      if (it > o->diis_cyc) break;
+      // R = C + A0' T (calA0R1.f90, calA0R2.f90)
+      DPD_HIP_TRY(launch_copy(lt, C1, R1, w.st));
+      if ((rc = w.c1_r1iabj(R1, t1(n), 1, Wk, 1.0)) || (rc = w.c1_r1Fmi(R1, t1(n), 1, X.fli, 1, -1.0)) ||
+          (rc = w.c1_r1Fae(R1, t1(n), 1, X.fad, 1, -1.0)) || (rc = w.c1_vr2(R1, t2(n), 1, 1.0)) ||
+          (rc = w.c2_vr1(R2, t1(n), 1, 1.0)))
+        return rc;
+    }
+    o->energy[n - 1] = E;
+    if (n >= 2) o->iterations[n - 2] = it;
+    if (it3) it3[n - 1] = it;
+  }
+  return DPD_OK;
+}
+
 // EE-CPSd.SF intermediates (calJ.f90), symmetry 1
 struct Jint {
   DevBuf buf;
@@ -629,6 +832,24 @@ extern "C" int dpd_problem_cpsd(dpd_problem* P, dpd_cpsd_opts* o, double* T) {
     T = (double*)own.p;
   }
   rc = cpsd_run(w, o, T);
+  if (rc) return rc;
+  DPD_HIP_TRY(hipStreamSynchronize(w.st));
+  return DPD_OK;
+}
+
+extern "C" int dpd_problem_grcpsd(dpd_problem* P, dpd_cpsd_opts* o, double* T, int64_t* iterations) {
+  if (!o) return fail(DPD_EINVAL, "null options");
+  Work w;
+  int rc = setup(P, w);
+  if (rc) return rc;
+  defaults(o);
+  const int64_t lt = w.len(w.dR1(1)) + w.len(w.dR2(1));
+  DevBuf own;
+  if (!T) {
+    if ((rc = own.alloc(sizeof(double) * (size_t)(3 * lt)))) return rc;
+    T = (double*)own.p;
+  }
+  rc = grcpsd_run(w, o, T, iterations);
   if (rc) return rc;
   DPD_HIP_TRY(hipStreamSynchronize(w.st));
   return DPD_OK;

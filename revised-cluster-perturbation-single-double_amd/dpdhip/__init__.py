@@ -586,6 +586,19 @@ class Problem:
         check(lib().dpd_problem_cpsd(self.h, ctypes.byref(o), T.data_ptr()))
         return T.view(3, n), {"energy": list(o.energy), "iterations": list(o.iterations)}
 
+    def grcpsd(self, diis_dim=10, diis_cyc=50, diis_max_err=1e-7):
+        """Ground-state grCPS(D) (grCPSd.SF/run.f90): the CIS(Doo) Jacobian as
+        zeroth order.  Returns (T, info) as cpsd() does; info["iterations"] holds
+        the DIIS iterations of orders 1, 2, 3."""
+        torch = _torch()
+        n = int(lib().dpd_problem_cpsd_len(self.h))
+        T = torch.empty(3 * n, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        o = _lib.CpsdOpts_()
+        o.diis_dim, o.diis_cyc, o.diis_max_err = diis_dim, diis_cyc, diis_max_err
+        it = np.zeros(3, dtype=np.int64)
+        check(lib().dpd_problem_grcpsd(self.h, ctypes.byref(o), T.data_ptr(), it.ctypes.data_as(_lib.c_i64_p)))
+        return T.view(3, n), {"energy": list(o.energy), "iterations": [int(x) for x in it]}
+
     def eecpsd(self, T, w0, L, diis_dim=10, diis_cyc=50, diis_max_err=1e-7):
         """EE-CPS(D) excitation energies (EE-CPSd.SF/run_order.f90) of CIS roots
         of this problem's symmetry: w0 (nroot CIS energies), L (nroot CIS vectors

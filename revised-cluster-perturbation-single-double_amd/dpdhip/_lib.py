@@ -161,6 +161,7 @@ _SIGS = {
     "dpd_problem_set_abcd_out_of_core": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "dpd_problem_cpsd_len": (i64, [c_void_p]),
     "dpd_problem_cpsd": (ctypes.c_int, [c_void_p, ctypes.POINTER(CpsdOpts_), c_void_p]),
+    "dpd_problem_grcpsd": (ctypes.c_int, [c_void_p, ctypes.POINTER(CpsdOpts_), c_void_p, c_i64_p]),
     "dpd_problem_eecpsd": (ctypes.c_int, [c_void_p, ctypes.POINTER(CpsdOpts_), c_void_p, i64, c_double_p, c_void_p,
                                           c_double_p, c_i64_p]),
     "dpd_davidson_batch": (ctypes.c_int, [c_void_p, i64, ctypes.POINTER(DavidsonOpts_), c_void_p, c_void_p, i64,

@@ -1,4 +1,4 @@
-"""GPU perturbation drivers (dpd_problem_cpsd, dpd_problem_eecpsd) against the
+"""GPU perturbation drivers (dpd_problem_cpsd, dpd_problem_grcpsd, dpd_problem_eecpsd) against the
 oracle (oracle/oracle_perturb.py, pinned by a dense restatement) on the same
 seeded Hamiltonians: ground-state CPS(D) amplitudes of orders 1-3 and the
 energies E^(2), E^(3), E^(4); EE-CPS(D) w^(1..3) of the lowest spin-flip CIS
@@ -54,6 +54,34 @@ def test_cpsd_ground_state_matches_oracle(torch, name):
     for k, e in zip((2, 3, 4), info["energy"]):
         assert e == pytest.approx(gs["E"][k], abs=1e-11), k
     print(f"[perturb] {name}: E2..E4 = {info['energy']}, DIIS iterations {info['iterations']}", flush=True)
+    prob.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_grcpsd_ground_state_matches_oracle(torch, name):
+    """grCPS(D) (grCPSd.SF/run.f90): the CIS(Doo) Jacobian as zeroth order; the
+    oracle's gr_ground_state is pinned by tests/test_oracle_grcpsd_dense.py"""
+    import dpdhip
+    cfg = SYSTEMS[name]
+    print(f"[perturb] grCPS(D) ground state {name}", flush=True)
+    ctx = dpdhip.Context(len(cfg["nso"]), 0)
+    prob = dpdhip.Problem(ctx, dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"]), "sf-cpsd", 1)
+    T, info = prob.grcpsd(**DIIS)
+    op = _oracle(cfg, "sf-cpsd", 1)
+    pt = OP.Perturb(op, **DIIS)
+    gs = pt.gr_ground_state()
+    Th = T.cpu().numpy()
+    for n in (1, 2, 3):
+        ref = np.concatenate([gs["T1"][n].arr, gs["T2"][n].arr])
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.abs(Th[n - 1] - ref).max() <= 1e-10 * scale, (n, np.abs(Th[n - 1] - ref).max())
+        assert abs(info["iterations"][n - 1] - gs["iterations"][n]) <= 1, (n, info["iterations"], gs["iterations"])
+    for k, e in zip((2, 3, 4), info["energy"]):
+        assert e == pytest.approx(gs["E"][k], abs=1e-11), k
+    # T1^(1) is coupled in by the CIS(Doo) dressing (zero in CPS(D))
+    assert np.abs(Th[0][: pt.dT1.length()]).max() > 0.0
+    print(f"[perturb] {name}: grCPS(D) E2..E4 = {info['energy']}, DIIS iterations {info['iterations']}", flush=True)
     prob.close()
     ctx.close()
 
