@@ -681,6 +681,10 @@ int layout_variant(const GemmPlan& p, GemmVariant& V, int mode, int cfg, bool sp
 static int build_variant_cfg(GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k = true) {
   if (g_capturing) return fail(DPD_EINVAL, "tile variant built during graph capture (internal)");
   bump_mem_gen();  // the variant's task tables are rewritten and reallocated
+  for (int sd = 0; sd < 2; sd++) {  // panel codes follow the tiling
+    V.pcode[sd].reset();
+    V.pcode_state[sd] = 0;
+  }
   int rc = layout_variant(p, V, mode, cfg, split_k);
   if (rc) return rc;
   size_t bt = V.tasks.size() * sizeof(GemmTask), bp = V.split.size() * sizeof(int32_t);
@@ -715,6 +719,54 @@ static int64_t c_span(const GemmPlan& p) {
 
 // pack one operand side of a variant into dst
 int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const PackSrc& src, void* dst) {
+  const int tile = side == 0 ? gemm_bm(V.cfg) : gemm_bn(V.cfg);
+  const int64_t npanel = side == 0 ? V.npan_a : V.npan_b;
+  const int64_t pan = (int64_t)tile * pk_kstep(V.mode);
+  // per-call operand of a cached plan: build the source code of every panel
+  // element once (the pack kernel's own address logic in code mode), then pack
+  // with one flat pass; not while capturing, and only for addresses below 2^31
+  if (src.coded && V.pcode_state[side] == 1 && V.pcode_comp[side] != (src.gcode != nullptr) && !g_capturing) {
+    V.pcode[side].reset();  // built for the other form of the operand
+    V.pcode_state[side] = 0;
+    bump_mem_gen();
+  }
+  if (src.coded && !src.map && !src.id && !g_capturing && V.pcode_state[side] == 0 &&
+      src.src_elems < (int64_t)INT32_MAX - 1 && (!src.gcode || src.gxlen > 0)) {
+    V.pcode_state[side] = -1;
+    if (!V.pcode[side].alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(npanel * pan, 1))) {
+      PackArgs a;
+      std::memset(&a, 0, sizeof a);
+      a.src = src.data;
+      if (!V.tasks.empty()) {
+        const GemmSeg& s0 = p.segs[V.tasks[0].seg_begin];
+        a.rfast = side == 0 ? (s0.sa_m[0] == 1) : (s0.sb_n[0] == 1);
+      }
+      a.dst = nullptr;
+      a.tasks = V.d_tasks;
+      a.segs = p.d_segs;
+      a.ntask = (int)V.tasks.size();
+      a.npanel = (int)npanel;
+      a.side = side;
+      a.mode = V.mode;
+      a.tile = tile;
+      a.code_out = (int32_t*)V.pcode[side].p;
+      a.gcode = src.gcode;
+      a.gxlen = src.gxlen;
+      a.gbstride = src.gbstride;
+      if (launch_pack(a, c->stream) == hipSuccess) {
+        V.pcode_state[side] = 1;
+        V.pcode_comp[side] = src.gcode != nullptr;
+      }
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  if (src.coded && !src.map && !src.id && V.pcode_state[side] == 1 && V.pcode_comp[side] == (src.gcode != nullptr)) {
+    DPD_HIP_TRY(launch_pack_code(V.mode, src.f32 ? 1 : 0, npanel, pan, (const int32_t*)V.pcode[side].p, src.data,
+                                 dst, c->stream));
+    return DPD_OK;
+  }
+  if (src.gcode) return fail(DPD_EINVAL, "composed pack without its code (internal)");
   PackArgs a;
   std::memset(&a, 0, sizeof a);
   a.src = src.data;
@@ -741,7 +793,7 @@ int pack_side(dpd_ctx* c, GemmPlan& p, GemmVariant& V, int side, const PackSrc& 
   a.npanel = side == 0 ? V.npan_a : V.npan_b;
   a.side = side;
   a.mode = V.mode;
-  a.tile = side == 0 ? gemm_bm(V.cfg) : gemm_bn(V.cfg);
+  a.tile = tile;
   DPD_HIP_TRY(launch_pack(a, c->stream));
   return DPD_OK;
 }
@@ -1579,12 +1631,20 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
     *dst = (const double*)x;
     return DPD_OK;
   };
-  if (r.gx1) {
+  // an expansion that only feeds a packed side is composed into the pack: the
+  // panels are written straight from the operand through the gather's code
+  auto composable = [&](int slot, const MapArgs& map) -> const int32_t* {
+    if (!r.packed || map.nterm != 1 || map.term[0].coef != 1.0 || map.alpha != 1.0 || map.beta != 0.0)
+      return nullptr;
+    return plan_code(c, P, slot, map);
+  };
+  const int32_t* comp[2] = {r.gx1 ? composable(0, P->map1) : nullptr, r.gx2 ? composable(1, P->map2) : nullptr};
+  if (r.gx1 && !comp[0]) {
     int rc = expand(P->map1, o1, r.nb1, P->len1x, r.x1_f32, &r.p1, "x1", 0);
     if (rc) return rc;
     r.p1f = nullptr;
   }
-  if (r.gx2) {
+  if (r.gx2 && !comp[1]) {
     int rc = expand(P->map2, o2, r.nb2, P->len2x, r.x2_f32, &r.p2, "x2", 1);
     if (rc) return rc;
     r.p2f = nullptr;
@@ -1627,6 +1687,24 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
       r.b.xlen = P->len2x;
       r.b.bstride = o2.bstride;
     }
+    // per-call operands pack through source codes kept with the plan
+    auto coded = [&](PackSrc& ps, const Operand& o, int nb, const int32_t* gc, int64_t lenx) {
+      if (o.static_id) return;
+      ps.coded = true;
+      const int64_t len = make_layout(c->sym, o.d).len;
+      ps.src_elems = nb > 1 ? (int64_t)(nb - 1) * o.bstride + len : len;
+      if (gc) {
+        ps.data = o.data;
+        ps.f32 = false;
+        ps.gcode = gc;
+        ps.gxlen = lenx;
+        ps.gbstride = o.bstride;
+      } else if (ps.data != o.data) {
+        ps.src_elems = lenx * nb;  // an expanded copy in the workspace
+      }
+    };
+    coded(r.a, o1, r.nb1, comp[0], P->len1x);
+    coded(r.b, o2, r.nb2, comp[1], P->len2x);
   }
   return DPD_OK;
 }

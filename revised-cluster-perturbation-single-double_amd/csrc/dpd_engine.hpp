@@ -61,6 +61,11 @@ struct GemmVariant {
   // per side, one packed copy per static operand (problems of the same shape
   // share the plan, each keeps its own panels)
   std::vector<std::unique_ptr<PackCache>> cache[2];
+  // per side, the source code of every panel element of the per-call operand
+  // (built on its first pack; composed with the operand's gather when comp)
+  DevBuf pcode[2];
+  int pcode_state[2] = {0, 0};  // 0 not built, 1 built, -1 not available
+  bool pcode_comp[2] = {false, false};
 };
 // drops the packed copies of V (buffers freed)
 void clear_pack_cache(GemmVariant& V);
@@ -241,6 +246,13 @@ struct PackSrc {
   const MapArgs* map = nullptr;
   int rowmask = 0;  // dims of the sorted operand that are GEMM rows (M or N)
   int64_t xlen = 0, bstride = 0;
+  // per-call operands of cached plans: pack through a source code kept with the
+  // variant (coded); gcode: the operand is the gather of `data` through the
+  // plan's gather code (one term), gxlen elements per batch member, members of
+  // `data` gbstride apart; src_elems bounds every source address
+  bool coded = false;
+  const int32_t* gcode = nullptr;
+  int64_t gxlen = 0, gbstride = 0, src_elems = 0;
 };
 // packed-panel GEMM; uncached operands are packed into workspace the caller reserved
 // ev0/ev1 bracket the GEMM kernel; ev_pack is recorded when the packing is done

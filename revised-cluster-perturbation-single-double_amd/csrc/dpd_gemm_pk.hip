@@ -175,6 +175,26 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
     const int g = k >> 4;
     double v = 0.0;
     const int64_t ro = rowoff[g][r], ko = koff[k];
+    if (p.code_out) {
+      // code mode (no fused sort): the source of this element
+      int32_t cd = 0;
+      if (ro >= 0 && ko >= 0) {
+        const int64_t ad = ro + ko;
+        if (p.gcode) {
+          const int64_t b = ad / p.gxlen, o = ad - b * p.gxlen;
+          const int32_t gc = p.gcode[o];
+          if (gc != 0) {
+            const int64_t a1 = b * p.gbstride + (gc > 0 ? (int64_t)gc : -(int64_t)gc);
+            cd = gc > 0 ? (int32_t)a1 : -(int32_t)a1;
+          }
+        } else {
+          cd = (int32_t)(ad + 1);
+        }
+      }
+      const int e = MODE == PREC_BF16X3 ? r * KS + k : k * p.tile + r;
+      p.code_out[(int64_t)pid * pan_elems + e] = cd;
+      continue;
+    }
     if (ro >= 0 && ko >= 0) {
       if (!p.use_map) {
         v = p.src_f32 ? (double)((const float*)p.src)[ro + ko] : p.src[ro + ko];
@@ -194,6 +214,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
     }
     tv[k][r] = v;
   }
+  if (p.code_out) return;  // uniform: every thread of the block took the code branch
   __syncthreads();
   for (int e = threadIdx.x; e < pan_elems; e += kThreads) {
     int r, k;
@@ -217,6 +238,54 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
       ((uint16_t*)p.dst)[base + pan_elems + e] = l;
     }
   }
+}
+
+// panels from precomputed source codes: one thread per element of the hi (or
+// only) plane, panels back to back (panel pid at pid * pan_elems * planes)
+template <int MODE, typename TIN>
+__global__ __launch_bounds__(256) void k_pack_code(int64_t total, int64_t pan_elems, const int32_t* __restrict__ code,
+                                                   const TIN* __restrict__ src, void* __restrict__ dst) {
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t cd = code[f];
+    double v = 0.0;
+    if (cd > 0) v = (double)src[cd - 1];
+    else if (cd < 0) v = -(double)src[-(int64_t)cd - 1];
+    if (MODE == PREC_F64) {
+      ((double*)dst)[f] = v;
+    } else if (MODE == PREC_F32) {
+      ((float*)dst)[f] = (float)v;
+    } else {
+      const int64_t pid = f / pan_elems, e = f - pid * pan_elems;
+      const int64_t base = pid * 2 * pan_elems + e;
+      const float fv = (float)v;
+      const uint16_t h = bf16_bits(fv);
+      const uint16_t l = bf16_bits((float)(v - (double)bf16_float(h)));
+      ((uint16_t*)dst)[base] = h;
+      ((uint16_t*)dst)[base + pan_elems] = l;
+    }
+  }
+}
+
+hipError_t launch_pack_code(int mode, int src_f32, int64_t npanel, int64_t pan_elems, const int32_t* code,
+                            const void* src, void* dst, hipStream_t s) {
+  const int64_t total = npanel * pan_elems;
+  if (total <= 0) return hipSuccess;
+  int64_t b = (total + 255) / 256;
+  if (b > 16384) b = 16384;
+  const dim3 grid((unsigned)b), blk(256);
+#define PKC(M)                                                                                              \
+  if (src_f32)                                                                                              \
+    hipLaunchKernelGGL((k_pack_code<M, float>), grid, blk, 0, s, total, pan_elems, code, (const float*)src, dst); \
+  else                                                                                                      \
+    hipLaunchKernelGGL((k_pack_code<M, double>), grid, blk, 0, s, total, pan_elems, code, (const double*)src, dst);
+  switch (mode) {
+    case PREC_F64: PKC(PREC_F64) break;
+    case PREC_F32: PKC(PREC_F32) break;
+    case PREC_BF16X3: PKC(PREC_BF16X3) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef PKC
+  return hipGetLastError();
 }
 
 template <int MODE>
@@ -482,6 +551,9 @@ __device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask
 // fragments and B chunks are loaded while this step's 72 MFMAs per wave run.
 // Products are accumulated in the order of gemm_pk_tile (lo*hi, hi*lo, hi*hi).
 // ---------------------------------------------------------------------------
+#ifndef DPD_PKD_PF
+#define DPD_PKD_PF 2
+#endif
 template <int BN>
 __device__ __forceinline__ void gemm_pkd_tile(const GemmLaunch& g, const GemmTask& T, int local,
                                               const void* __restrict__ PA, const void* __restrict__ PB,
@@ -517,7 +589,10 @@ __device__ __forceinline__ void gemm_pkd_tile(const GemmLaunch& g, const GemmTas
 #pragma unroll
     for (int y = 0; y < NT; y++) acc[t][y] = f4_t{0, 0, 0, 0};
 
-  bf8_t ah[2], al[2], nh[2], nl[2];
+  // A fragments of PF K steps in registers (a ring of slots indexed at compile
+  // time: the loop below is unrolled PF times)
+  constexpr int PF = DPD_PKD_PF;
+  bf8_t rh[PF][2], rl[PF][2];
   u4_t rb[CB];
   auto load_a = [&](int step, bf8_t* h, bf8_t* l) {
     const uint16_t* q = pa + (int64_t)step * A_PANEL + aoff;
@@ -542,37 +617,37 @@ __device__ __forceinline__ void gemm_pkd_tile(const GemmLaunch& g, const GemmTas
   };
   if (nst > 0) {
     load_b(0);
-    load_a(0, ah, al);
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+      if (u < nst) load_a(u, rh[u], rl[u]);
     store_b(0);
     __syncthreads();
-    for (int i = 0; i < nst; i++) {
-      const int buf = i & 1;
-      const bool more = i + 1 < nst;
-      if (more) {
-        load_b(i + 1);
-        load_a(i + 1, nh, nl);
-      }
-      const uint16_t* bs = Bs[buf];
+    for (int i0 = 0; i0 < nst; i0 += PF) {
 #pragma unroll
-      for (int y = 0; y < NT; y++) {
-        const bf8_t bh = *(const bf8_t*)&bs[(16 * y + (lane & 15)) * LDK + kq];
-        const bf8_t bl = *(const bf8_t*)&bs[BN * LDK + (16 * y + (lane & 15)) * LDK + kq];
+      for (int u = 0; u < PF; u++) {
+        const int i = i0 + u;
+        if (i < nst) {
+          const int buf = i & 1;
+          const bool more = i + 1 < nst;
+          if (more) load_b(i + 1);
+          const uint16_t* bs = Bs[buf];
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-          acc[t][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], bh, acc[t][y], 0, 0, 0);
-          acc[t][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bl, acc[t][y], 0, 0, 0);
-          acc[t][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bh, acc[t][y], 0, 0, 0);
+          for (int y = 0; y < NT; y++) {
+            const bf8_t bh = *(const bf8_t*)&bs[(16 * y + (lane & 15)) * LDK + kq];
+            const bf8_t bl = *(const bf8_t*)&bs[BN * LDK + (16 * y + (lane & 15)) * LDK + kq];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+              acc[t][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rl[u][t], bh, acc[t][y], 0, 0, 0);
+              acc[t][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rh[u][t], bl, acc[t][y], 0, 0, 0);
+              acc[t][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rh[u][t], bh, acc[t][y], 0, 0, 0);
+            }
+          }
+          // slot u is free again: refill it with step i + PF
+          if (i + PF < nst) load_a(i + PF, rh[u], rl[u]);
+          if (more) store_b(buf ^ 1);  // the other buffer: every wave left it at the last barrier
+          __syncthreads();
         }
       }
-      if (more) {
-        store_b(buf ^ 1);  // the other buffer: every wave left it at the last barrier
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-          ah[t] = nh[t];
-          al[t] = nl[t];
-        }
-      }
-      __syncthreads();
     }
   }
 
@@ -639,7 +714,10 @@ __global__ __launch_bounds__(256) void k_gemm_pk_grp(GemmLaunch g, GroupTab gt) 
 }
 
 template <int BN>
-__global__ __launch_bounds__(256, 2) void k_gemm_pkd(GemmLaunch g, const void* __restrict__ PA,
+#ifndef DPD_PKD_OCC
+#define DPD_PKD_OCC 2
+#endif
+__global__ __launch_bounds__(256, DPD_PKD_OCC) void k_gemm_pkd(GemmLaunch g, const void* __restrict__ PA,
                                                      const void* __restrict__ PB) {
   const int wg = xcd_wg(g.nwg);
   const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
@@ -647,7 +725,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_pkd(GemmLaunch g, const void* _
 }
 
 template <int BN>
-__global__ __launch_bounds__(256, 2) void k_gemm_pkd_grp(GemmLaunch g, GroupTab gt) {
+__global__ __launch_bounds__(256, DPD_PKD_OCC) void k_gemm_pkd_grp(GemmLaunch g, GroupTab gt) {
   const int wg = xcd_wg(g.nwg);
   const GemmTask& T = g.tasks[find_task(g.tasks, g.ntask, wg, 0)];
   const GroupEnt& e = gt.e[T.group];
@@ -658,7 +736,7 @@ template <int MODE>
 static hipError_t launch_pk_grp_mode(const GemmLaunch& g, const GroupTab& gt, hipStream_t s) {
   if (g.cfg == GEMM_128x192D) {
     if (MODE != PREC_BF16X3) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_gemm_pkd_grp<192>), dim3(g.nwg), dim3(256), 0, s, g, gt);
+    hipLaunchKernelGGL((k_gemm_pkd_grp<kPkdBN>), dim3(g.nwg), dim3(256), 0, s, g, gt);
     return hipGetLastError();
   }
   switch (g.cfg) {
@@ -710,7 +788,7 @@ template <int MODE>
 static hipError_t launch_pk_mode(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s) {
   if (g.cfg == GEMM_128x192D) {
     if (MODE != PREC_BF16X3) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_gemm_pkd<192>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
+    hipLaunchKernelGGL((k_gemm_pkd<kPkdBN>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
     return hipGetLastError();
   }
   switch (g.cfg) {

@@ -106,13 +106,19 @@ enum GemmCfg {
   GEMM_128x192D = 7
 };
 constexpr int kGemmCfgMax = GEMM_128x192D;
+// width of the direct-A tile (build-time choice, 64 or 192)
+#ifndef DPD_PKD_BN
+#define DPD_PKD_BN 192
+#endif
+constexpr int kPkdBN = DPD_PKD_BN;
 inline int gemm_bm(int cfg) {
   return (cfg == GEMM_128x64 || cfg == GEMM_128x128 || cfg == GEMM_128x192D) ? 128 : cfg == GEMM_32x32 ? 32 : 64;
 }
 inline int gemm_bn(int cfg) {
   return (cfg == GEMM_64x128 || cfg == GEMM_128x128) ? 128
          : cfg == GEMM_32x32                         ? 32
-         : (cfg == GEMM_64x192 || cfg == GEMM_128x192D) ? 192
+         : cfg == GEMM_64x192                        ? 192
+         : cfg == GEMM_128x192D                      ? kPkdBN
                                                      : 64;
 }
 
@@ -159,8 +165,19 @@ struct PackArgs {
   int64_t xlen, src_bstride;
   MapArgs map;
   int src_f32;  // src holds floats (no fused sort)
+  // code mode: write, per panel element (panel pid, element e at pid*pan + e),
+  // +-(source address + 1) or 0 instead of packing values; with gcode the
+  // operand is the gather of src through gcode (one term): X element o of batch
+  // member b is gcode[o] applied to member b of src (stride gbstride)
+  int32_t* code_out;
+  const int32_t* gcode;
+  int64_t gxlen, gbstride;
 };
 hipError_t launch_pack(const PackArgs& p, hipStream_t s);
+// panels from a code written by launch_pack's code mode: npanel panels of
+// pan_elems elements from src (FP64, or FP32 when src_f32) into dst (mode)
+hipError_t launch_pack_code(int mode, int src_f32, int64_t npanel, int64_t pan_elems, const int32_t* code,
+                            const void* src, void* dst, hipStream_t s);
 // K step of a packed panel and its element count factor (BF16x3 keeps two planes)
 inline int pk_kstep(int mode) { return mode == PREC_F64 ? 16 : 32; }
 inline int pk_planes(int mode) { return mode == PREC_BF16X3 ? 2 : 1; }
