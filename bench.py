@@ -91,6 +91,23 @@ def main():
     ap.add_argument("--summarize-trace", default=None, metavar="CSV",
                     help="summarise the kernel_trace.csv of a --profile-solve run (the kernels between the "
                          "markers, grouped by class) as markdown and exit")
+    ap.add_argument("--probe-gathers", default=None, choices=["fp64", "fp32", "bf16x3"],
+                    help="counter-run mode for the gathers and sorts: one profiled sigma call (5 vectors) in this "
+                         "precision, then the DPD sorts at this workload's sizes, each measured call between two "
+                         "marker kernels, then a plain 512 MiB device copy; writes the launches' algorithmic "
+                         "bytes to --probe-out; no JSON line")
+    ap.add_argument("--probe-out", default=None,
+                    help="launch list (JSON) written by --probe and --probe-gathers (default for the latter: "
+                         "gather_probe.json)")
+    ap.add_argument("--summarize-gemms", nargs=3, default=None, metavar=("TRACE_CSV", "PMC_CSV", "PROBE_JSON"),
+                    help="join the kernel trace and an SQ counter run of a --probe program with its launch list: "
+                         "per GEMM launch of the profiled sigma call, algorithmic and counted MFMA throughput and "
+                         "MFMA busy share, as markdown, and exit")
+    ap.add_argument("--summarize-gathers", nargs=4, default=None,
+                    metavar=("TRACE_CSV", "FETCH_CSV", "WRITE_CSV", "PROBE_JSON"),
+                    help="join the kernel trace and the FETCH_SIZE / WRITE_SIZE counter runs of a --probe-gathers "
+                         "program with its launch list: achieved and calibrated-traffic GB/s per gather class "
+                         "and per sort, as markdown, and exit")
     ap.add_argument("--fd-dirs", type=int, default=50, help="gradient workload: random integral directions")
     ap.add_argument("--fd-h", type=float, default=1e-4, help="gradient workload: central-difference step")
     ap.add_argument("--grad-conv", type=float, default=1e-9, help="gradient workload: FP64 residual tolerance")
@@ -103,6 +120,12 @@ def main():
     if args.summarize_trace:
         summarize_trace(args.summarize_trace)
         return
+    if args.summarize_gathers:
+        summarize_gathers(*args.summarize_gathers)
+        return
+    if args.summarize_gemms:
+        summarize_gemms(*args.summarize_gemms)
+        return
 
     import numpy as np
     import torch
@@ -113,6 +136,9 @@ def main():
     nroot = cfg["nroot"]
     if args.probe:
         probe(cfg, args)
+        return
+    if args.probe_gathers:
+        probe_gathers(cfg, args)
         return
     if args.profile_solve:
         profile_solve(cfg, args)
@@ -533,15 +559,283 @@ def probe(cfg, args):
     prob = dpdhip.Problem(ctx, dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"]), "sf-cpsd", 1)
     prob.set_precision(args.probe)
     R = torch.randn(cfg["nroot"], prob.dim, device="cuda:0", dtype=torch.float64)
-    for _ in range(3):
+    for _ in range(2):
         prob.sigma(R)
+    # the third call between two marker kernels, with the context profiler on
+    one = torch.ones(1, dtype=torch.float64, device="cuda:0")
+    out1 = torch.empty_like(one)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    dpdhip.precondition(ctx, 0.0, one, one, out1)
+    prob.sigma(R)
+    dpdhip.precondition(ctx, 0.0, one, one, out1)
+    rep = ctx.profile_report()
+    ctx.profile(False)
     torch.cuda.synchronize()
     a = torch.empty(1 << 26, dtype=torch.float64, device="cuda:0").fill_(1.0)
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
+    if args.probe_out:
+        records = [[k, v["kind"], v["count"], v["bytes_total"], v["flops_total"], v["prec"]] for k, v in rep.items()]
+        with open(args.probe_out, "w") as f:
+            json.dump({"precision": args.probe, "workload": args.workload, "sigma_records": records,
+                       "copy_bytes": 8 << 26}, f, indent=1)
     log("probe done")
+
+
+def probe_gathers(cfg, args):
+    """The program profiled for the gather and sort bandwidths: three warm
+    sigma calls (codes built, tiles tuned, integral panels packed), one sigma
+    call with the context profiler on (its per-launch algorithmic bytes), then
+    each DPD sort at the workload's spin-orbital populations, called once to
+    warm up and once more between two marker kernels (k_precond on one
+    element), then a plain copy of 2^26 doubles that calibrates FETCH_SIZE and
+    WRITE_SIZE.  The launch list goes to --probe-out as JSON."""
+    import torch
+
+    import dpdhip
+    ctx = dpdhip.Context(len(cfg["nso"]), 0)
+    spec = dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"])
+    prob = dpdhip.Problem(ctx, spec, "sf-cpsd", 1)
+    prob.set_precision(args.probe_gathers)
+    R = torch.randn(cfg["nroot"], prob.dim, device="cuda:0", dtype=torch.float64)
+    for _ in range(3):
+        prob.sigma(R)
+    one = torch.ones(1, dtype=torch.float64, device="cuda:0")
+    out1 = torch.empty_like(one)
+
+    def marker():
+        dpdhip.precondition(ctx, 0.0, one, one, out1)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    marker()
+    prob.sigma(R)
+    marker()
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    # one sigma call: the report's keys are in first-launch order
+    records = [[k, v["kind"], v["count"], v["bytes_total"]] for k, v in rep.items()]
+    lay = spec.layout()
+    V, Oc = lay["popv"], lay["popo"]
+    sym = 1
+    sorts = []
+
+    def sort_case(name, n_in, n_out, call):
+        x = torch.randn(n_in, dtype=torch.float64, device="cuda:0")
+        y = torch.empty(n_out + 1, dtype=torch.float64, device="cuda:0")
+        call(x, y)
+        torch.cuda.synchronize()
+        marker()
+        call(x, y)
+        marker()
+        torch.cuda.synchronize()
+        sorts.append([name, 8 * (n_in + n_out)])
+        del x, y
+    vovo = ctx.idsymsoc(sym, V, Oc, 1, V, Oc, 1)
+    for t in ("1243", "2143", "3142", "2413", "4321"):
+        sort_case(f"sstgenso '{t}' (V,O,V,O) -> permuted, {vovo} elements", vovo, vovo,
+                  lambda x, y, t=t: dpdhip.sstgenso(ctx, x, y, V, Oc, V, Oc, sym, t))
+    vvvo = ctx.idsymsoc(sym, V, V, 1, V, Oc, 1)
+    for t in ("1243", "2134", "3412", "4321"):
+        sort_case(f"sstgenso '{t}' (V,V,V,O) -> permuted, {vvvo} elements", vvvo, vvvo,
+                  lambda x, y, t=t: dpdhip.sstgenso(ctx, x, y, V, V, V, Oc, sym, t))
+    vvoo_p = ctx.idsymsoc(sym, V, V, 0, Oc, Oc, 0)
+    vvoo_e = ctx.idsymsoc(sym, V, V, 1, Oc, Oc, 0)
+    sort_case(f"iexpso ab packed -> full (V,V,O,O), {vvoo_p} -> {vvoo_e} elements", vvoo_p, vvoo_e,
+              lambda x, y: dpdhip.iexpso(ctx, x, y, V, V, 0, 1, Oc, Oc, 0, 0, sym))
+    sort_case(f"icompso ab full -> packed (V,V,O,O), {vvoo_e} -> {vvoo_p} elements", vvoo_e, vvoo_p,
+              lambda x, y: dpdhip.icompso(ctx, x, y, V, V, 1, 0, Oc, Oc, 0, 1, sym))
+    sort_case(f"itranspso (V,O|V,O) pair transpose, {vovo} elements", vovo, vovo,
+              lambda x, y: dpdhip.itranspso(ctx, x, y, V, Oc, 1, V, Oc, 1, sym))
+    a = torch.empty(1 << 26, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    with open(args.probe_out or "gather_probe.json", "w") as f:
+        json.dump({"precision": args.probe_gathers, "workload": args.workload, "sigma_records": records,
+                   "sorts": sorts, "copy_bytes": 8 << 26}, f, indent=1)
+    log(f"probe done: {len(records)} sigma records, {len(sorts)} sorts -> {args.probe_out}")
+
+
+def _trace_rows(path):
+    """(start, end, name) of every dispatch of a rocprofv3 kernel_trace.csv, in start order"""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    return sorted(((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in rows),
+                  key=lambda x: x[0])
+
+
+def _pmc_rows(path, counter):
+    """(dispatch id, name, value) per dispatch of a rocprofv3 counter_collection.csv (rows of one
+    dispatch and counter summed), in dispatch order"""
+    import csv
+    acc = {}
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        d = int(r["Dispatch_Id"])
+        e = acc.setdefault(d, [r["Kernel_Name"], 0.0])
+        e[1] += float(r["Counter_Value"])
+    return [(d, acc[d][0], acc[d][1]) for d in sorted(acc)]
+
+
+def _is_marker(name):
+    return name.split("(")[0].strip().endswith("k_precond")
+
+
+def _sections(names):
+    """index ranges strictly between marker pairs (0,1), (2,3), ... of a dispatch-name list"""
+    marks = [i for i, n in enumerate(names) if _is_marker(n)]
+    return [(marks[k] + 1, marks[k + 1]) for k in range(0, len(marks) - 1, 2)]
+
+
+def summarize_gathers(trace_csv, fetch_csv, write_csv, probe_json):
+    """Markdown: per gather class of the profiled sigma call and per sort, the
+    duration (kernel trace), the algorithmic bytes (the probe's launch list),
+    the HBM traffic (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled
+    by what it reads for the probe's plain copy) and both as GB/s."""
+    meta = json.load(open(probe_json))
+    tr = _trace_rows(trace_csv)
+    fe = _pmc_rows(fetch_csv, "FETCH_SIZE")
+    wr = _pmc_rows(write_csv, "WRITE_SIZE")
+    kib = meta["copy_bytes"] / 1024.0
+    # calibration: the last dispatch of each counter run is the plain copy
+    f_scale = kib / fe[-1][2]
+    w_scale = kib / wr[-1][2]
+    st, sf, sw = _sections([r[2] for r in tr]), _sections([r[1] for r in fe]), _sections([r[1] for r in wr])
+    nsort = len(meta["sorts"])
+    if not (len(st) == len(sf) == len(sw) == 1 + nsort):
+        raise SystemExit(f"marker sections differ: trace {len(st)}, fetch {len(sf)}, write {len(sw)}, "
+                         f"expected {1 + nsort}")
+
+    def dispatches(k):
+        (a, b), (fa, fb), (wa, wb) = st[k], sf[k], sw[k]
+        if not (b - a == fb - fa == wb - wa):
+            raise SystemExit(f"section {k}: dispatch counts differ ({b - a}, {fb - fa}, {wb - wa})")
+        return [(tr[a + i][2].split("(")[0].strip(), (tr[a + i][1] - tr[a + i][0]) * 1e-3,
+                 fe[fa + i][2] * f_scale * 1024.0, wr[wa + i][2] * w_scale * 1024.0) for i in range(b - a)]
+    print(f"Precision {meta['precision']}, workload {meta['workload']}.  Calibration copy of "
+          f"{meta['copy_bytes'] / 2**20:.0f} MiB: FETCH_SIZE reads {fe[-1][2]:.0f} KiB (scale {f_scale:.3f}), "
+          f"WRITE_SIZE {wr[-1][2]:.0f} KiB (scale {w_scale:.3f}).\n")
+    # the sigma call: gather records <-> k_gather dispatches, pack records <-> k_pack dispatches, in order
+    sig = dispatches(0)
+    recs = []
+    for key, kind, cnt, by in meta["sigma_records"]:
+        if kind != "gather":
+            continue
+        for _ in range(cnt):
+            recs.append((key, by / cnt))
+    g_disp = [d for d in sig if "k_gather" in d[0]]
+    p_disp = [d for d in sig if "k_pack" in d[0]]
+    g_rec = [r for r in recs if not r[0].endswith(":pack")]
+    p_rec = [r for r in recs if r[0].endswith(":pack")]
+    if len(g_disp) != len(g_rec) or len(p_disp) != len(p_rec):
+        raise SystemExit(f"sigma call: {len(g_disp)} gather / {len(p_disp)} pack dispatches against "
+                         f"{len(g_rec)} / {len(p_rec)} profiler records")
+    rows = [(r[0], r[1], d) for r, d in zip(g_rec + p_rec, g_disp + p_disp)]
+
+    def cls(key):
+        suf = key.rsplit(":", 1)[-1]
+        return {"x1": "expansions", "x2": "expansions", "epi": "antisymmetrising epilogues",
+                "pack": "operand packing"}.get(suf, suf)
+    agg = {}
+    for key, by, d in rows:
+        e = agg.setdefault(cls(key), [0, 0.0, 0.0, 0.0])
+        e[0] += 1
+        e[1] += d[1]
+        e[2] += by
+        e[3] += d[2] + d[3]
+
+    def line(name, n, us, alg, traffic):
+        gbs = alg / (us * 1e-6) / 1e9 if us > 0 else 0.0
+        tgbs = traffic / (us * 1e-6) / 1e9 if us > 0 else 0.0
+        return (f"| {name} | {n} | {us:.1f} | {alg / 1e6:.1f} | {gbs:.0f} | {gbs / PEAK_HBM_GBS:.2f} | "
+                f"{traffic / 1e6:.1f} | {tgbs:.0f} | {tgbs / PEAK_HBM_GBS:.2f} |")
+    head = ("| {0} | launches | us | algorithmic MB | GB/s | of 8 TB/s | traffic MB | traffic GB/s | "
+            "of 8 TB/s |\n|---|---:|---:|---:|---:|---:|---:|---:|---:|")
+    print("One sigma call, 5 vectors, per gather class:\n")
+    print(head.format("class"))
+    for k, (n, us, alg, traffic) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(line(k, n, us, alg, traffic))
+    print("\nThe same launches one by one (largest first):\n")
+    print(head.format("launch (kernel)"))
+    for key, by, d in sorted(rows, key=lambda r: -r[2][1]):
+        print(line(f"{key} (`{d[0].replace('void ', '')}`)", 1, d[1], by, d[2] + d[3]))
+    print("\nDPD sorts at the workload's spin-orbital populations (the second of two identical calls):\n")
+    print(head.format("sort (kernel)"))
+    for k, (name, by) in enumerate(meta["sorts"]):
+        ds = dispatches(1 + k)
+        if len(ds) != 1:
+            raise SystemExit(f"sort {name}: {len(ds)} dispatches")
+        d = ds[0]
+        print(line(f"{name} (`{d[0].replace('void ', '')}`)", 1, d[1], by, d[2] + d[3]))
+
+
+def summarize_gemms(trace_csv, pmc_csv, probe_json, clock_ghz=2.4, simds=1024):
+    """Markdown: per GEMM launch of a --probe program's profiled sigma call,
+    the duration (kernel trace), the algorithmic TFLOP/s against the precision's
+    dense MFMA peak, and the SQ counters of the counter run with two derived
+    figures: the MFMA busy share per SIMD (SQ_VALU_MFMA_BUSY_CYCLES over
+    simds x duration x clock) and the counted MFMA TFLOP/s
+    (SQ_INSTS_VALU_MFMA_MOPS_* x 512 flops over the duration)."""
+    import csv
+    meta = json.load(open(probe_json))
+    tr = _trace_rows(trace_csv)
+    per = {}
+    for r in csv.DictReader(open(pmc_csv)):
+        d = int(r["Dispatch_Id"])
+        e = per.setdefault(d, {"name": r["Kernel_Name"], "c": {}})
+        e["c"][r["Counter_Name"]] = e["c"].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    pm = [(d, per[d]["name"], per[d]["c"]) for d in sorted(per)]
+    st, sp = _sections([r[2] for r in tr]), _sections([r[1] for r in pm])
+    if not st or not sp:
+        raise SystemExit("no marker pair in the trace or the counter run")
+    (a, b), (pa, pb) = st[0], sp[0]
+    if b - a != pb - pa:
+        raise SystemExit(f"dispatch counts differ between the trace ({b - a}) and the counter run ({pb - pa})")
+
+    def is_gemm(n):
+        return "k_gemm" in n and "reduce" not in n
+    disp = [(tr[a + i], pm[pa + i]) for i in range(b - a) if is_gemm(tr[a + i][2])]
+    recs = []
+    for key, kind, cnt, _by, fl, prec in meta["sigma_records"]:
+        if kind == "gemm":
+            recs += [(key, fl / cnt, prec)] * cnt
+    if len(disp) != len(recs):
+        raise SystemExit(f"{len(disp)} GEMM dispatches against {len(recs)} profiler records")
+    names = sorted({k for _, p in disp for k in p[2]})
+    mops = [k for k in names if k.startswith("SQ_INSTS_VALU_MFMA_MOPS")]
+    print(f"Precision {meta['precision']}, workload {meta['workload']}; one sigma call (5 vectors).  Peaks per "
+          f"algorithmic flop: FP64 78.6, FP32 157.3, BF16x3 833 TFLOP/s (a third of 2.5 PFLOP/s).  MFMA busy = "
+          f"SQ_VALU_MFMA_BUSY_CYCLES / ({simds} SIMDs x duration x {clock_ghz} GHz).\n")
+    hdr = ["launch", "kernel", "us", "GFLOP", "TFLOP/s", "of peak"]
+    if "SQ_VALU_MFMA_BUSY_CYCLES" in names:
+        hdr.append("MFMA busy")
+    if mops:
+        hdr.append("counted TFLOP/s")
+    hdr += names
+    print("| " + " | ".join(hdr) + " |\n|" + "---|" * 2 + "---:|" * (len(hdr) - 2))
+    tot_us = tot_fl = 0.0
+    for (t, p), (key, fl, prec) in zip(disp, recs):
+        us = (t[1] - t[0]) * 1e-3
+        tot_us += us
+        tot_fl += fl
+        tf = fl / (us * 1e-6) / 1e12
+        row = [key, "`" + t[2].split("(")[0].replace("void ", "").strip() + "`", f"{us:.1f}", f"{fl / 1e9:.2f}",
+               f"{tf:.1f}", f"{tf / PEAK_TFLOPS[prec]:.2f}"]
+        c = p[2]
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in names:
+            row.append(f"{c.get('SQ_VALU_MFMA_BUSY_CYCLES', 0.0) / (simds * us * 1e-6 * clock_ghz * 1e9):.2f}")
+        if mops:
+            row.append(f"{sum(c.get(k, 0.0) for k in mops) * 512 / (us * 1e-6) / 1e12:.1f}")
+        row += [f"{c.get(k, 0.0):.0f}" for k in names]
+        print("| " + " | ".join(row) + " |")
+    if tot_us > 0:
+        print(f"\nAll GEMM launches of the call: {tot_us:.1f} us, {tot_fl / 1e9:.2f} GFLOP, "
+              f"{tot_fl / (tot_us * 1e-6) / 1e12:.1f} TFLOP/s.")
 
 
 def profile_solve(cfg, args):
@@ -583,9 +877,9 @@ KERNEL_CLASSES = [
     ("Davidson vector algebra", ("k_multidot", "k_reduce_rows", "k_lincomb", "k_dots_multi", "k_cgs_need",
                                  "k_tri_apply", "k_chol_tri", "k_mgs", "k_pair_dots", "k_renorm_pair",
                                  "k_resid", "k_amax", "k_precond_batched", "k_scale_by_norm", "k_block_dots",
-                                 "k_davidson")),
+                                 "k_davidson", "k_ritz_resid")),
     ("elementwise sigma terms, copies", ("k_hadamard", "k_div", "k_orbital_energy", "k_convert", "k_copy",
-                                         "k_axpy", "k_scal", "k_fill")),
+                                         "k_axpy", "k_scal", "k_fill", "k_singles", "copyBuffer")),
 ]
 
 

@@ -70,7 +70,8 @@ int dpd_ctx_synchronize(dpd_ctx *ctx);
  * the packed-panel kernel picked by timing every shape on a plan's first call),
  * -2 automatic by a padding cost model without timing, 0 the first-generation
  * 64x64 kernel, 1..6 64x64, 128x64, 64x128, 128x128, 32x32, 64x192 tiles (64x192
- * packed-panel kernel only).
+ * packed-panel kernel only), 7 the BF16x3 128x64 tile with A read straight into
+ * registers (other precisions: 128x64).
  * packing: 0 operands gathered inside the GEMM, 1 (default) packed panels when
  * an operand is a problem's constant tensor (packed once and kept), 2 packed
  * panels always (BF16x3 always uses panels).  Clears the plan caches.  No

@@ -524,7 +524,8 @@ int build_gemm_plan(const Sym& S, const Operand& o1, const Operand& o2, const Op
   // kept selectable (dpd_ctx_set_gemm_config) for comparison
   plan.cfg = choose_gemm_cfg(plan.tasks, plan.cfg_req);
   // the gathering kernel has no 192-wide tile
-  if (plan.cfg == GEMM_64x192 || plan.cfg == GEMM_128x192D) plan.cfg = GEMM_64x128;
+  if (plan.cfg == GEMM_64x192) plan.cfg = GEMM_64x128;
+  if (plan.cfg == GEMM_128x64D) plan.cfg = GEMM_128x64;
   const int BM = gemm_bm(plan.cfg), BN = gemm_bn(plan.cfg);
   for (auto& tk : plan.tasks) {
     tk.tiles_m = (tk.M + BM - 1) / BM;
@@ -620,7 +621,7 @@ int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int requested) {
 // fewer than 1024 tiles (partials summed by a reduce kernel); off: one
 // workgroup per tile
 int layout_variant(const GemmPlan& p, GemmVariant& V, int mode, int cfg, bool split_k) {
-  if (cfg == GEMM_128x192D && mode != PREC_BF16X3) cfg = GEMM_64x192;  // direct-A tile: BF16x3 only
+  if (cfg == GEMM_128x64D && mode != PREC_BF16X3) cfg = GEMM_128x64;  // direct-A tile: BF16x3 only
   V.mode = mode;
   V.tasks = p.tasks;
   V.split.clear();
@@ -895,7 +896,7 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
   // candidates in order of preference: among shapes within 5% of the fastest
   // the first listed wins, which keeps the choice stable from run to run; each
   // tile shape with and without split-K (the panels are the same for both)
-  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x192D, GEMM_128x128, GEMM_64x128, GEMM_64x64,
+  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x64D, GEMM_128x128, GEMM_64x128, GEMM_64x64,
                              GEMM_32x32};
   constexpr int NC = (int)(sizeof cand / sizeof cand[0]);
   float tms[NC][2];
@@ -910,7 +911,7 @@ int tune_variant(dpd_ctx* c, GemmPlan& p, int mode, const PackSrc& A, const Pack
   for (int ci = 0; ci < NC && !rc; ci++) {
     tms[ci][0] = tms[ci][1] = 1e30f;
     const int cfg = cand[ci];
-    if (cfg == GEMM_128x192D && mode != PREC_BF16X3) continue;  // the same as GEMM_64x192 there
+    if (cfg == GEMM_128x64D && mode != PREC_BF16X3) continue;  // the same as GEMM_128x64 there
     DevBuf ta, tb;
     bool had_split = false;
     for (int sk = 0; sk < 2 && !rc; sk++) {
@@ -1751,6 +1752,21 @@ static int64_t call_bytes(dpd_ctx* c, const CallPrep& r, const Operand& o1, cons
   return bytes;
 }
 
+// algorithmic bytes of the packing of a call's per-call operands: each source
+// element read once, each panel element written once (cached sides cost nothing)
+static int64_t pack_bytes(dpd_ctx* c, const CallPrep& r, const Operand& o1, const Operand& o2, int prec) {
+  const GemmVariant& V = r.P->gemm.pk[prec];
+  int64_t bytes = 0;
+  auto side = [&](const PackSrc& s, const Operand& o, bool gx, int nb, int64_t lenx, int64_t panel) {
+    if (s.id) return;
+    const bool from_operand = !gx || s.map || s.gcode;
+    const int64_t src = (from_operand ? make_layout(c->sym, o.d).len : lenx) * nb;
+    bytes += src * (s.f32 ? 4 : 8) + panel * (int64_t)pk_esize(prec);
+  };
+  side(r.a, o1, r.gx1, r.nb1, r.P->len1x, V.pa_elems);
+  side(r.b, o2, r.gx2, r.nb2, r.P->len2x, V.pb_elems);
+  return bytes;
+}
 
 int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
                   double fact, int anti, int nbatch, int prec, int64_t* flops, const char* tag) {
@@ -1778,7 +1794,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
     if (pr) {
       thread_local std::string pname;
       pname = std::string(tag) + ":pack";
-      pp = prof_open(c, pname.c_str(), 0, prec, 0, 1, 0, 0, 1);
+      pp = prof_open(c, pname.c_str(), 0, prec, 0, 1, pack_bytes(c, r, o1, o2, prec), 0, 1);
       if (pp) DPD_HIP_TRY(hipEventRecord(pp->a, c->stream));
     }
     rc = run_gemm_packed(c, P->gemm, prec, r.a, r.b, r.p3, fact, r.beta, pr ? pr->a : nullptr,
@@ -1896,7 +1912,7 @@ static bool capturing(hipStream_t s) {
 }
 
 static int tune_group(dpd_ctx* c, GroupPlan& G, int prec, const std::vector<CallPrep>& R, double fact) {
-  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x192D, GEMM_128x128, GEMM_64x128, GEMM_64x64,
+  static const int cand[] = {GEMM_128x64, GEMM_64x192, GEMM_128x64D, GEMM_128x128, GEMM_64x128, GEMM_64x64,
                              GEMM_32x32};
   const int ncand = (int)(sizeof cand / sizeof cand[0]);
   float tms[ncand];
@@ -1916,7 +1932,7 @@ static int tune_group(dpd_ctx* c, GroupPlan& G, int prec, const std::vector<Call
   for (int ci = 0; ci < ncand && !rc; ci++) {
     tms[ci] = 1e30f;
     const int cfg = cand[ci];
-    if (cfg == GEMM_128x192D && prec != PREC_BF16X3) continue;  // the same as GEMM_64x192 there
+    if (cfg == GEMM_128x64D && prec != PREC_BF16X3) continue;  // the same as GEMM_128x64 there
     std::vector<GemmVariant> T(n);
     std::vector<DevBuf> pa(n), pb(n);
     std::vector<const GemmVariant*> Vs(n);

@@ -540,15 +540,15 @@ __device__ __forceinline__ void gemm_pk_tile(const GemmLaunch& g, const GemmTask
 
 // ---------------------------------------------------------------------------
 // BF16x3 tile with the A operand read straight from HBM into registers
-// (GEMM_128x192D).  The wide-N shape of the contractions that stream a large
-// constant operand against a few vectors (the pp-ladder: A = <ab||ef> panels,
-// N = ij pairs x vectors) reads every A element once per N tile, so A is the
-// traffic.  Each of the 4 waves owns 32 rows of the 128-row tile and loads its
-// A fragments from the packed panel (16 rows x 32 k per MFMA tile, hi and lo
-// planes: one contiguous KiB each, the layout the pack kernel writes), so A
-// never passes through LDS.  Only B (192 x 32 per K step, hi and lo) is
-// staged, double buffered with one barrier per K step; the next step's A
-// fragments and B chunks are loaded while this step's 72 MFMAs per wave run.
+// (GEMM_128x64D; BN = DPD_PKD_BN).  Each of the 4 waves owns 32 rows of the
+// 128-row tile and loads its A fragments from the packed panel (16 rows x 32 k
+// per MFMA tile, hi and lo planes: one contiguous KiB each, the layout the pack
+// kernel writes), so A never passes through LDS and is never read by two
+// waves.  Only B (BN x 32 per K step, hi and lo) is staged, double buffered
+// with one barrier per K step; the A fragments of DPD_PKD_PF steps are held in
+// registers ahead of the one multiplied.  Measured at D2h/120: the ring GEMM
+// 103 instead of 115 us per launch, the pp-ladder 148 instead of 138 (the
+// autotuner keeps the staged 128x64 tile there).
 // Products are accumulated in the order of gemm_pk_tile (lo*hi, hi*lo, hi*hi).
 // ---------------------------------------------------------------------------
 #ifndef DPD_PKD_PF
@@ -734,7 +734,7 @@ __global__ __launch_bounds__(256, DPD_PKD_OCC) void k_gemm_pkd_grp(GemmLaunch g,
 
 template <int MODE>
 static hipError_t launch_pk_grp_mode(const GemmLaunch& g, const GroupTab& gt, hipStream_t s) {
-  if (g.cfg == GEMM_128x192D) {
+  if (g.cfg == GEMM_128x64D) {
     if (MODE != PREC_BF16X3) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_gemm_pkd_grp<kPkdBN>), dim3(g.nwg), dim3(256), 0, s, g, gt);
     return hipGetLastError();
@@ -786,7 +786,7 @@ hipError_t launch_gemm_pk_grouped(const GemmLaunch& g, const GroupTab& gt, hipSt
 
 template <int MODE>
 static hipError_t launch_pk_mode(const GemmLaunch& g, const void* PA, const void* PB, hipStream_t s) {
-  if (g.cfg == GEMM_128x192D) {
+  if (g.cfg == GEMM_128x64D) {
     if (MODE != PREC_BF16X3) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_gemm_pkd<kPkdBN>), dim3(g.nwg), dim3(256), 0, s, g, PA, PB);
     return hipGetLastError();

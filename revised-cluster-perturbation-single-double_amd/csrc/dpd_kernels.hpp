@@ -102,23 +102,24 @@ enum GemmCfg {
   GEMM_32x32 = 5,
   GEMM_64x192 = 6,
   // BF16x3 packed-panel kernel with A fragments loaded straight into registers
-  // (other precisions take GEMM_64x192 in its place)
-  GEMM_128x192D = 7
+  // (other precisions take GEMM_128x64 in its place)
+  GEMM_128x64D = 7
 };
-constexpr int kGemmCfgMax = GEMM_128x192D;
-// width of the direct-A tile (build-time choice, 64 or 192)
+constexpr int kGemmCfgMax = GEMM_128x64D;
+// width of the direct-A tile (build-time choice: 64, or 192 for a tile that
+// covers a whole narrow N; 64 is faster on the D2h/120 ring and equal elsewhere)
 #ifndef DPD_PKD_BN
-#define DPD_PKD_BN 192
+#define DPD_PKD_BN 64
 #endif
 constexpr int kPkdBN = DPD_PKD_BN;
 inline int gemm_bm(int cfg) {
-  return (cfg == GEMM_128x64 || cfg == GEMM_128x128 || cfg == GEMM_128x192D) ? 128 : cfg == GEMM_32x32 ? 32 : 64;
+  return (cfg == GEMM_128x64 || cfg == GEMM_128x128 || cfg == GEMM_128x64D) ? 128 : cfg == GEMM_32x32 ? 32 : 64;
 }
 inline int gemm_bn(int cfg) {
   return (cfg == GEMM_64x128 || cfg == GEMM_128x128) ? 128
          : cfg == GEMM_32x32                         ? 32
          : cfg == GEMM_64x192                        ? 192
-         : cfg == GEMM_128x192D                      ? kPkdBN
+         : cfg == GEMM_128x64D                      ? kPkdBN
                                                      : 64;
 }
 

@@ -74,7 +74,8 @@ int ooc_plan(dpd_problem* p, int64_t nvec, int64_t stride, int sym, int r, GemmP
   tk.ktiles_total = sg.ktiles;
   tk.ksplit = 1;
   plan->cfg = choose_gemm_cfg({tk}, p->ctx->gemm_cfg);
-  if (plan->cfg == GEMM_64x192 || plan->cfg == GEMM_128x192D) plan->cfg = GEMM_64x128;
+  if (plan->cfg == GEMM_64x192) plan->cfg = GEMM_64x128;
+  if (plan->cfg == GEMM_128x64D) plan->cfg = GEMM_128x64;
   tk.tiles_m = (int32_t)((nr + gemm_bm(plan->cfg) - 1) / gemm_bm(plan->cfg));
   tk.tiles_n = (int32_t)((tk.N + gemm_bn(plan->cfg) - 1) / gemm_bn(plan->cfg));
   plan->tasks.push_back(tk);

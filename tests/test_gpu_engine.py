@@ -47,7 +47,7 @@ def ctx_for(n):
 GEMM_CFGS = [(-1, 1), (-2, 1), (0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (-1, 2), (1, 2), (2, 2), (3, 2),
              (4, 2), (5, 2), (6, 2), (7, 2)]
 GEMM_IDS = ["auto", "model", "v1", "g64x64", "g128x64", "g64x128", "g128x128", "g32x32", "pk_auto", "pk64x64",
-            "pk128x64", "pk64x128", "pk128x128", "pk32x32", "pk64x192", "pk128x192d"]
+            "pk128x64", "pk64x128", "pk128x128", "pk32x32", "pk64x192", "pk128x64d"]
 
 
 @pytest.fixture(params=GEMM_CFGS, ids=GEMM_IDS)

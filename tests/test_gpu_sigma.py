@@ -148,7 +148,7 @@ def test_sigma_split_k_off_with_autotuned_tiles(torch, prec):
 @pytest.mark.parametrize("split", [0, -1])
 def test_bf16x3_direct_a_tile_equals_staged_tile(torch, split):
     """The BF16x3 tile that loads A straight into registers (GEMM config 7,
-    128x192) accumulates every output element over the same K steps in the same
+    128x64) accumulates every output element over the same K steps in the same
     MFMA order as the LDS-staged 128x64 tile (config 2): with split-K off the
     sigma vectors are bit-identical, single and grouped; with split-K automatic
     they agree to FP32 accumulation accuracy."""
