@@ -83,8 +83,10 @@ def main():
     ap.add_argument("--probe", default=None, choices=["fp64", "fp32", "bf16x3"],
                     help="counter-run mode: a few batched sigma calls in this precision (tile shape --tile), "
                          "then a plain 512 MiB device copy for calibrating FETCH_SIZE/WRITE_SIZE; no JSON line")
-    ap.add_argument("--tile", type=int, default=-1, help="GEMM tile configuration for --probe (-1: autotuned)")
-    ap.add_argument("--split", type=int, default=-1, help="split-K for --probe: -1 automatic, 0 off, 1 on")
+    ap.add_argument("--tile", type=int, default=-1,
+                    help="GEMM tile configuration for --probe and --probe-gathers (-1: autotuned)")
+    ap.add_argument("--split", type=int, default=-1,
+                    help="split-K for --probe and --probe-gathers: -1 automatic, 0 off, 1 on")
     ap.add_argument("--profile-solve", default=None, choices=["fp64", "fp32", "bf16x3"],
                     help="trace mode: a warm-up solve in this precision, then one solve between two marker "
                          "kernels (k_precond on one element) for rocprofv3 --kernel-trace; no JSON line")
@@ -597,6 +599,9 @@ def probe_gathers(cfg, args):
 
     import dpdhip
     ctx = dpdhip.Context(len(cfg["nso"]), 0)
+    # a fixed tile shape (--tile) keeps the launches of the trace and counter runs the same
+    ctx.set_gemm_config(args.tile, 1)
+    ctx.set_split_k(args.split)
     spec = dpdhip.Spec(cfg["nso"], cfg["nalpha"], cfg["nbeta"], cfg["seed"])
     prob = dpdhip.Problem(ctx, spec, "sf-cpsd", 1)
     prob.set_precision(args.probe_gathers)
@@ -669,17 +674,21 @@ def _trace_rows(path):
 
 
 def _pmc_rows(path, counter):
-    """(dispatch id, name, value) per dispatch of a rocprofv3 counter_collection.csv (rows of one
-    dispatch and counter summed), in dispatch order"""
+    """(dispatch id, name, value, start, end) per dispatch of a rocprofv3 counter_collection.csv
+    (rows of one dispatch and counter summed), in dispatch order"""
     import csv
     acc = {}
     for r in csv.DictReader(open(path)):
         if r["Counter_Name"] != counter:
             continue
         d = int(r["Dispatch_Id"])
-        e = acc.setdefault(d, [r["Kernel_Name"], 0.0])
+        e = acc.setdefault(d, [r["Kernel_Name"], 0.0, int(r["Start_Timestamp"]), int(r["End_Timestamp"])])
         e[1] += float(r["Counter_Value"])
-    return [(d, acc[d][0], acc[d][1]) for d in sorted(acc)]
+    return [(d, e[0], e[1], e[2], e[3]) for d, e in sorted(acc.items())]
+
+
+def _base(name):
+    return name.split("(")[0].replace("void ", "").strip()
 
 
 def _is_marker(name):
@@ -711,12 +720,22 @@ def summarize_gathers(trace_csv, fetch_csv, write_csv, probe_json):
         raise SystemExit(f"marker sections differ: trace {len(st)}, fetch {len(sf)}, write {len(sw)}, "
                          f"expected {1 + nsort}")
 
+    notes = []
+
     def dispatches(k):
+        """(kernel, us, fetched bytes, written bytes) of section k; the durations come from the
+        kernel trace, or from the FETCH_SIZE run when the trace launched other kernels there"""
         (a, b), (fa, fb), (wa, wb) = st[k], sf[k], sw[k]
-        if not (b - a == fb - fa == wb - wa):
-            raise SystemExit(f"section {k}: dispatch counts differ ({b - a}, {fb - fa}, {wb - wa})")
-        return [(tr[a + i][2].split("(")[0].strip(), (tr[a + i][1] - tr[a + i][0]) * 1e-3,
-                 fe[fa + i][2] * f_scale * 1024.0, wr[wa + i][2] * w_scale * 1024.0) for i in range(b - a)]
+        nf = [_base(fe[fa + i][1]) for i in range(fb - fa)]
+        if nf != [_base(wr[wa + i][1]) for i in range(wb - wa)]:
+            raise SystemExit(f"section {k}: the two counter runs launched different kernels")
+        if nf == [_base(tr[a + i][2]) for i in range(b - a)]:
+            dur = [(tr[a + i][1] - tr[a + i][0]) * 1e-3 for i in range(b - a)]
+        else:
+            notes.append(k)
+            dur = [(fe[fa + i][4] - fe[fa + i][3]) * 1e-3 for i in range(fb - fa)]
+        return [(nf[i], dur[i], fe[fa + i][2] * f_scale * 1024.0, wr[wa + i][2] * w_scale * 1024.0)
+                for i in range(fb - fa)]
     print(f"Precision {meta['precision']}, workload {meta['workload']}.  Calibration copy of "
           f"{meta['copy_bytes'] / 2**20:.0f} MiB: FETCH_SIZE reads {fe[-1][2]:.0f} KiB (scale {f_scale:.3f}), "
           f"WRITE_SIZE {wr[-1][2]:.0f} KiB (scale {w_scale:.3f}).\n")
@@ -772,6 +791,9 @@ def summarize_gathers(trace_csv, fetch_csv, write_csv, probe_json):
             raise SystemExit(f"sort {name}: {len(ds)} dispatches")
         d = ds[0]
         print(line(f"{name} (`{d[0].replace('void ', '')}`)", 1, d[1], by, d[2] + d[3]))
+    if notes:
+        print(f"\nSections {notes} launched other kernels in the kernel-trace run (the autotuner times tile "
+              "shapes); their durations are the FETCH_SIZE run's own timestamps.")
 
 
 def summarize_gemms(trace_csv, pmc_csv, probe_json, clock_ghz=2.4, simds=1024):
@@ -787,51 +809,54 @@ def summarize_gemms(trace_csv, pmc_csv, probe_json, clock_ghz=2.4, simds=1024):
     per = {}
     for r in csv.DictReader(open(pmc_csv)):
         d = int(r["Dispatch_Id"])
-        e = per.setdefault(d, {"name": r["Kernel_Name"], "c": {}})
+        e = per.setdefault(d, {"name": r["Kernel_Name"], "c": {}, "t": (int(r["Start_Timestamp"]),
+                                                                          int(r["End_Timestamp"]))})
         e["c"][r["Counter_Name"]] = e["c"].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
-    pm = [(d, per[d]["name"], per[d]["c"]) for d in sorted(per)]
+    pm = [(d, per[d]["name"], per[d]["c"], per[d]["t"]) for d in sorted(per)]
     st, sp = _sections([r[2] for r in tr]), _sections([r[1] for r in pm])
     if not st or not sp:
         raise SystemExit("no marker pair in the trace or the counter run")
     (a, b), (pa, pb) = st[0], sp[0]
-    if b - a != pb - pa:
-        raise SystemExit(f"dispatch counts differ between the trace ({b - a}) and the counter run ({pb - pa})")
+    names = [_base(pm[pa + i][1]) for i in range(pb - pa)]
+    # durations from the kernel trace when it launched the same kernels, else the counter run's own
+    same = names == [_base(tr[a + i][2]) for i in range(b - a)]
+    dur = [((tr[a + i][1] - tr[a + i][0]) if same else (pm[pa + i][3][1] - pm[pa + i][3][0])) * 1e-3
+           for i in range(pb - pa)]
 
     def is_gemm(n):
         return "k_gemm" in n and "reduce" not in n
-    disp = [(tr[a + i], pm[pa + i]) for i in range(b - a) if is_gemm(tr[a + i][2])]
+    disp = [(names[i], dur[i], pm[pa + i][2]) for i in range(pb - pa) if is_gemm(names[i])]
     recs = []
     for key, kind, cnt, _by, fl, prec in meta["sigma_records"]:
         if kind == "gemm":
             recs += [(key, fl / cnt, prec)] * cnt
     if len(disp) != len(recs):
         raise SystemExit(f"{len(disp)} GEMM dispatches against {len(recs)} profiler records")
-    names = sorted({k for _, p in disp for k in p[2]})
-    mops = [k for k in names if k.startswith("SQ_INSTS_VALU_MFMA_MOPS")]
+    cnames = sorted({k for _, _, c in disp for k in c})
+    mops = [k for k in cnames if k.startswith("SQ_INSTS_VALU_MFMA_MOPS")]
     print(f"Precision {meta['precision']}, workload {meta['workload']}; one sigma call (5 vectors).  Peaks per "
           f"algorithmic flop: FP64 78.6, FP32 157.3, BF16x3 833 TFLOP/s (a third of 2.5 PFLOP/s).  MFMA busy = "
-          f"SQ_VALU_MFMA_BUSY_CYCLES / ({simds} SIMDs x duration x {clock_ghz} GHz).\n")
+          f"SQ_VALU_MFMA_BUSY_CYCLES / ({simds} SIMDs x duration x {clock_ghz} GHz).  Durations from the "
+          + ("kernel trace." if same else "counter run's timestamps (the kernel-trace run tuned other tiles).")
+          + "\n")
     hdr = ["launch", "kernel", "us", "GFLOP", "TFLOP/s", "of peak"]
-    if "SQ_VALU_MFMA_BUSY_CYCLES" in names:
+    if "SQ_VALU_MFMA_BUSY_CYCLES" in cnames:
         hdr.append("MFMA busy")
     if mops:
         hdr.append("counted TFLOP/s")
-    hdr += names
+    hdr += cnames
     print("| " + " | ".join(hdr) + " |\n|" + "---|" * 2 + "---:|" * (len(hdr) - 2))
     tot_us = tot_fl = 0.0
-    for (t, p), (key, fl, prec) in zip(disp, recs):
-        us = (t[1] - t[0]) * 1e-3
+    for (kname, us, c), (key, fl, prec) in zip(disp, recs):
         tot_us += us
         tot_fl += fl
         tf = fl / (us * 1e-6) / 1e12
-        row = [key, "`" + t[2].split("(")[0].replace("void ", "").strip() + "`", f"{us:.1f}", f"{fl / 1e9:.2f}",
-               f"{tf:.1f}", f"{tf / PEAK_TFLOPS[prec]:.2f}"]
-        c = p[2]
-        if "SQ_VALU_MFMA_BUSY_CYCLES" in names:
+        row = [key, "`" + kname + "`", f"{us:.1f}", f"{fl / 1e9:.2f}", f"{tf:.1f}", f"{tf / PEAK_TFLOPS[prec]:.2f}"]
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in cnames:
             row.append(f"{c.get('SQ_VALU_MFMA_BUSY_CYCLES', 0.0) / (simds * us * 1e-6 * clock_ghz * 1e9):.2f}")
         if mops:
             row.append(f"{sum(c.get(k, 0.0) for k in mops) * 512 / (us * 1e-6) / 1e12:.1f}")
-        row += [f"{c.get(k, 0.0):.0f}" for k in names]
+        row += [f"{c.get(k, 0.0):.0f}" for k in cnames]
         print("| " + " | ".join(row) + " |")
     if tot_us > 0:
         print(f"\nAll GEMM launches of the call: {tot_us:.1f} us, {tot_fl / 1e9:.2f} GFLOP, "

@@ -386,10 +386,25 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   if (dimAbar > 1024) return fail(DPD_EINVAL, "max_dim must be <= 1024");
   const int dimDel = dimAbar - nroot - 5;
   hipStream_t st = stream ? stream : c->stream;
-  // the workspace stays with the problem between solves (allocation and the
-  // pinned staging cost about 3 ms per solve otherwise); one solve at a time
-  if (!P->dav) P->dav.reset(new DavWork);
-  DavWork& W = *P->dav;
+  // the workspace of a single solve stays with the problem between solves
+  // (allocation and the pinned staging cost about 3 ms per solve otherwise);
+  // a member of a batched solve (own stream, run from a worker thread) takes
+  // a workspace of its own for the call
+  std::unique_ptr<DavWork> own;
+  if (stream) {
+    own.reset(new DavWork);
+  } else if (!P->dav) {
+    P->dav.reset(new DavWork);
+  }
+  DavWork& W = stream ? *own : *P->dav;
+  // a call's own workspace is freed only after its last transfer has completed
+  struct SyncOnExit {
+    hipStream_t s;
+    bool on;
+    ~SyncOnExit() {
+      if (on) (void)hipStreamSynchronize(s);
+    }
+  } sync_on_exit{st, (bool)own};
   int rc;
   const size_t vb = sizeof(double) * (size_t)n;
   const size_t nd2 = (size_t)dimAbar * dimAbar;

@@ -15,6 +15,7 @@
 //     so the kernel addresses A, B, C with mixed-radix strides and never packs.
 // Plans hold no data pointers and are cached per signature.
 #include "dpd_engine.hpp"
+#include "dpd_sortplan.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -1008,12 +1009,51 @@ int run_gemm(dpd_ctx* c, GemmPlan& p, const void* A, int a_f32, const void* B, i
   return DPD_OK;
 }
 
+// the tiled permutation plan of a plain copy between full-pair layouts, built
+// and uploaded on first use (outside graph capture); null when the generic
+// gather kernels apply
+constexpr size_t kSortCacheEntries = 512;
+static const SortPlan* sort_plan(dpd_ctx* c, const MapArgs& a) {
+  if (!sort_tiles_apply(a)) return nullptr;
+  std::string key((const char*)&a.out, sizeof(DevLayout));
+  key.append((const char*)&a.in, sizeof(DevLayout));
+  key.append((const char*)a.term[0].perm, 4);
+  auto it = c->sort_cache.find(key);
+  if (it != c->sort_cache.end()) return it->second->ok ? it->second.get() : nullptr;
+  if (g_capturing) return nullptr;
+  if (c->sort_cache.size() >= kSortCacheEntries) {
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+    bump_mem_gen();  // captured graphs may hold the task tables
+    c->sort_cache.clear();
+  }
+  auto P = std::make_unique<SortPlan>();
+  if (plan_sort(a, P->tasks, &P->nitems, &P->direct) && !P->tasks.empty() && P->nitems > 0 &&
+      P->dev.alloc(sizeof(SortTask) * P->tasks.size()) == DPD_OK &&
+      hipMemcpyAsync(P->dev.p, P->tasks.data(), sizeof(SortTask) * P->tasks.size(), hipMemcpyHostToDevice,
+                     c->stream) == hipSuccess)
+    P->ok = true;
+  (void)hipGetLastError();
+  const SortPlan* r = P->ok ? P.get() : nullptr;
+  c->sort_cache.emplace(std::move(key), std::move(P));
+  return r;
+}
+
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out) {
+  if (const SortPlan* P = sort_plan(c, a)) {
+    DPD_HIP_TRY(launch_sort_tiles((const SortTask*)P->dev.p, (int)P->tasks.size(), P->nitems, P->direct, in, out,
+                                  c->stream));
+    return DPD_OK;
+  }
   DPD_HIP_TRY(launch_gather_map(a, in, out, c->stream));
   return DPD_OK;
 }
 
 int gather_f32(dpd_ctx* c, MapArgs a, const float* in, float* out) {
+  if (const SortPlan* P = sort_plan(c, a)) {
+    DPD_HIP_TRY(launch_sort_tiles_f32((const SortTask*)P->dev.p, (int)P->tasks.size(), P->nitems, P->direct, in,
+                                      out, c->stream));
+    return DPD_OK;
+  }
   DPD_HIP_TRY(launch_gather_map_f32(a, in, out, c->stream));
   return DPD_OK;
 }

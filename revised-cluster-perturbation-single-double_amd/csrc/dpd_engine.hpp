@@ -149,6 +149,15 @@ struct GroupTasks {
   int nwg = 0, ntask = 0, nsplit = 0, max_split_mn = 0;
   int64_t partial_elems = 0;
 };
+// task table of a tiled DPD permutation (plan_sort), kept per layouts and permutation
+struct SortPlan {
+  std::vector<SortTask> tasks;
+  DevBuf dev;
+  int64_t nitems = 0;
+  bool direct = false;  // no transposing digits: tiles are copied straight
+  bool ok = false;      // false: the sort keeps the generic gather kernels
+};
+
 struct GroupPlan {
   int cfg = -1;
   bool tuned = false;
@@ -203,6 +212,8 @@ struct dpd_ctx {
   // source codes of the plans' gathers, keyed by layouts and terms
   std::unordered_map<std::string, std::unique_ptr<dpd::DevBuf>> map_cache;
   size_t map_bytes = 0;
+  // tiled permutation plans of the sorts, keyed by layouts and permutation
+  std::unordered_map<std::string, std::unique_ptr<dpd::SortPlan>> sort_cache;
   std::vector<char> pinned_scratch;
 };
 

@@ -297,6 +297,190 @@ hipError_t launch_gather_code_f32out(const MapArgs& a, const int32_t* code, cons
 }
 
 // ===========================================================================
+// tiled DPD permutation: one workgroup per work item of a SortTask (the task
+// found by bisection over the tasks' first items).  A transposing tile is read
+// with the input-contiguous digits fastest (coalesced reads), staged in LDS
+// with one padding element per row, and written with the output-contiguous
+// digits fastest (coalesced writes); tasks without B are copied straight.
+// Element indices are split into digits by multiplication with a reciprocal
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", 1994: q = (umulhi(n, m) + n) >> l for n < 2^31), and every
+// thread keeps four loads in flight.
+// ===========================================================================
+struct FastDiv {
+  uint32_t d, m, l;
+  __device__ void init(uint32_t dd) {
+    d = dd;
+    l = 0;
+    while ((1u << l) < d) l++;
+    m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+  }
+  __device__ uint32_t div(uint32_t n) const { return (__umulhi(n, m) + n) >> l; }
+};
+
+struct SortItem {
+  int64_t ib0, ob0;  // bases of the item (C digits' tile origin and digit 2)
+  int a_lo, a_n, b_lo, b_n, c0_n, c1_n;
+};
+
+// the task of work item `item` staged in LDS, and the item's tile
+__device__ inline void sort_item(const SortTask* __restrict__ tasks, int ntask, SortTask& st, SortItem& it) {
+  __shared__ int s_k;
+  const int64_t item = blockIdx.x;
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = ntask - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tasks[mid].first <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    s_k = lo;
+  }
+  __syncthreads();
+  static_assert(sizeof(SortTask) % 4 == 0, "SortTask size");
+  for (int i = threadIdx.x; i < (int)(sizeof(SortTask) / 4); i += blockDim.x)
+    ((int32_t*)&st)[i] = ((const int32_t*)&tasks[s_k])[i];
+  __syncthreads();
+  int local = (int)(item - st.first);
+  const int tb = local % st.tB;
+  local /= st.tB;
+  const int ta = local % st.tA;
+  local /= st.tA;
+  const int tc0 = local % st.tC0;
+  local /= st.tC0;
+  const int tc1 = local % st.tC1;
+  const int c2 = local / st.tC1;
+  it.a_lo = ta * st.TA;
+  it.a_n = min(st.TA, st.nA - it.a_lo);
+  it.b_lo = tb * st.TB;
+  it.b_n = st.nB ? min(st.TB, st.nB - it.b_lo) : 1;
+  const int c0_lo = tc0 * st.TC0, c1_lo = tc1 * st.TC1;
+  it.c0_n = min(st.TC0, st.nC[0] - c0_lo);
+  it.c1_n = min(st.TC1, st.nC[1] - c1_lo);
+  it.ib0 = st.in_base + (int64_t)c0_lo * st.inC[0] + (int64_t)c1_lo * st.inC[1] + (int64_t)c2 * st.inC[2];
+  it.ob0 = st.out_base + (int64_t)c0_lo * st.outC[0] + (int64_t)c1_lo * st.outC[1] + (int64_t)c2 * st.outC[2];
+}
+
+constexpr int kSortU = 4;  // loads in flight per thread
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sort_copy(const SortTask* __restrict__ tasks, int ntask,
+                                                   const T* __restrict__ in, T* __restrict__ out) {
+  __shared__ SortTask st;
+  SortItem it;
+  sort_item(tasks, ntask, st, it);
+  FastDiv fa, f0;
+  fa.init(it.a_n);
+  f0.init(it.c0_n);
+  const int n = it.a_n * it.c0_n * it.c1_n;
+  for (int e0 = threadIdx.x; e0 < n; e0 += kSortU * blockDim.x) {
+    T v[kSortU];
+    int64_t oa[kSortU];
+#pragma unroll
+    for (int u = 0; u < kSortU; u++) {
+      const int e = e0 + u * blockDim.x;
+      oa[u] = -1;
+      if (e < n) {
+        const int r = (int)fa.div(e), ia = e - r * it.a_n;
+        const int i1 = (int)f0.div(r), i0 = r - i1 * it.c0_n;
+        const int64_t a = it.a_lo + ia;
+        v[u] = in[it.ib0 + (int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1] + a];
+        oa[u] = it.ob0 + (int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1] + a;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSortU; u++)
+      if (oa[u] >= 0) out[oa[u]] = v[u];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sort_tiles(const SortTask* __restrict__ tasks, int ntask,
+                                                    const T* __restrict__ in, T* __restrict__ out) {
+  __shared__ T lds[kSortLds];
+  __shared__ SortTask st;
+  SortItem it;
+  sort_item(tasks, ntask, st, it);
+  FastDiv fa, fb, f0, fa0, fb0;
+  fa.init(it.a_n);
+  fb.init(it.b_n);
+  f0.init(it.c0_n);
+  fa0.init(st.nA0);
+  fb0.init(st.nB0);
+  const int pitch = it.b_n + 1;
+  const int n = it.a_n * it.b_n * it.c0_n * it.c1_n;
+  // read: B fastest, then A, C0, C1
+  for (int e0 = threadIdx.x; e0 < n; e0 += kSortU * blockDim.x) {
+    T v[kSortU];
+    int li[kSortU];
+#pragma unroll
+    for (int u = 0; u < kSortU; u++) {
+      const int e = e0 + u * blockDim.x;
+      li[u] = -1;
+      if (e < n) {
+        int r = (int)fb.div(e);
+        const int ib = e - r * it.b_n;
+        const int r2 = (int)fa.div(r), ia = r - r2 * it.a_n;
+        const int i1 = (int)f0.div(r2), i0 = r2 - i1 * it.c0_n;
+        const int A = it.a_lo + ia;
+        const int a1 = (int)fa0.div(A), a0 = A - a1 * st.nA0;
+        const int64_t ad = it.ib0 + (int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1] + (int64_t)a0 * st.inA[0] +
+                           (int64_t)a1 * st.inA[1] + it.b_lo + ib;
+        v[u] = in[ad];
+        li[u] = r * pitch + ib;  // r = (i1 * c0_n + i0) * a_n + ia
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSortU; u++)
+      if (li[u] >= 0) lds[li[u]] = v[u];
+  }
+  __syncthreads();
+  // write: A fastest, then B, C0, C1
+  for (int e0 = threadIdx.x; e0 < n; e0 += kSortU * blockDim.x) {
+    T v[kSortU];
+    int64_t oa[kSortU];
+#pragma unroll
+    for (int u = 0; u < kSortU; u++) {
+      const int e = e0 + u * blockDim.x;
+      oa[u] = -1;
+      if (e < n) {
+        const int r = (int)fa.div(e), ia = e - r * it.a_n;
+        const int r2 = (int)fb.div(r), ib = r - r2 * it.b_n;
+        const int i1 = (int)f0.div(r2), i0 = r2 - i1 * it.c0_n;
+        const int B = it.b_lo + ib;
+        const int b1 = (int)fb0.div(B), b0 = B - b1 * st.nB0;
+        v[u] = lds[(r2 * it.a_n + ia) * pitch + ib];
+        oa[u] = it.ob0 + (int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1] + (int64_t)b0 * st.outB[0] +
+                (int64_t)b1 * st.outB[1] + it.a_lo + ia;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSortU; u++)
+      if (oa[u] >= 0) out[oa[u]] = v[u];
+  }
+}
+
+template <typename T>
+static hipError_t sort_tiles_t(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const T* in, T* out,
+                               hipStream_t s) {
+  if (ntask <= 0 || nitems <= 0) return hipSuccess;
+  if (nitems > 0x7fffffff) return hipErrorInvalidValue;
+  if (direct)
+    hipLaunchKernelGGL((k_sort_copy<T>), dim3((unsigned)nitems), dim3(256), 0, s, tasks, ntask, in, out);
+  else
+    hipLaunchKernelGGL((k_sort_tiles<T>), dim3((unsigned)nitems), dim3(256), 0, s, tasks, ntask, in, out);
+  return hipGetLastError();
+}
+hipError_t launch_sort_tiles(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const double* in,
+                             double* out, hipStream_t s) {
+  return sort_tiles_t<double>(tasks, ntask, nitems, direct, in, out, s);
+}
+hipError_t launch_sort_tiles_f32(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const float* in,
+                                 float* out, hipStream_t s) {
+  return sort_tiles_t<float>(tasks, ntask, nitems, direct, in, out, s);
+}
+
+// ===========================================================================
 // singles-from-singles terms of the sigma builders in one launch:
 //   c(a,i) += sum_fn W(fn,ai) r(f,n) + sum_e Fac(a,e) r(e,i) - sum_m r(a,m) Fki(m,i)
 // One block per output element (a,i) of every vector; W is the dense l1 x l1

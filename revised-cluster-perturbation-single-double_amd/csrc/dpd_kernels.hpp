@@ -43,6 +43,35 @@ hipError_t launch_gather_code_f32in(const MapArgs& a, const int32_t* code, const
 hipError_t launch_gather_code_f32out(const MapArgs& a, const int32_t* code, const double* in, float* out,
                                      hipStream_t s);
 
+// ---- tiled DPD permutation (sorts between full-pair layouts) ---------------
+// Within one irrep quadruple both tensors are dense 4-index arrays with their
+// own strides.  The output digits split into A (output-contiguous: output
+// dim 0, with dim 1 when A is short), B (input-contiguous: the output dims of
+// input dims 0 and 1) and C (the rest, up to three digits).  A work item is a
+// TC1 x TC0 x TA x TB tile (C digits 1 and 0, A, B), read with B fastest into
+// LDS and written with A fastest; without B (the first input and output dims
+// coincide) every tile is copied straight through (k_sort_copy).
+struct SortTask {
+  int64_t first;              // first work item of the task
+  int64_t in_base, out_base;  // element (0,0,0,0) of the irrep block in each tensor
+  int64_t inA[2];             // input strides of the A digits (output: contiguous)
+  int64_t outB[2];            // output strides of the B digits (input: contiguous)
+  int64_t inC[3], outC[3];    // strides of the C digits
+  int32_t nA0, nA, nB0, nB;   // first digit and flattened extent of A and B (nB = 0: no B)
+  int32_t nC[3];
+  int32_t TA, TB, TC0, TC1;   // tile extents along A, B and C digits 0 and 1
+  int32_t tA, tB, tC0, tC1;   // tiles along A, B and C digits 0 and 1
+  int32_t pad_;
+};
+constexpr int kSortLds = 2304;     // LDS elements of one tile: TC1 * TC0 * TA * (TB + 1) at most
+constexpr int kSortTileB = 2048;   // TA * TB at most
+constexpr int kSortCopy = 4096;    // elements of a copied tile at most
+// direct: every task without B (k_sort_copy), else k_sort_tiles
+hipError_t launch_sort_tiles(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const double* in,
+                             double* out, hipStream_t s);
+hipError_t launch_sort_tiles_f32(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const float* in,
+                                 float* out, hipStream_t s);
+
 // ---- singles-from-singles sigma terms (W r, Fac r, r Fki) -----------------
 struct SinglesArgs {
   int64_t l1, stride;  // R1 elements per vector; distance between vectors

@@ -77,7 +77,8 @@ class Context:
     def set_gemm_config(self, cfg=-1, packing=1):
         """GEMM kernels of later plans.  cfg: -1 automatic (tile shape timed on a
         plan's first call), -2 automatic by cost model, 0 first-generation kernel,
-        1..6 tiles 64x64, 128x64, 64x128, 128x128, 32x32, 64x192 (packed kernel).
+        1..7 tiles 64x64, 128x64, 64x128, 128x128, 32x32, 64x192 (packed kernel), 7 the
+        BF16x3 128x64 tile that reads A straight into registers (128x64 in the other precisions).
         packing: 0 operands gathered inside the GEMM, 1 packed panels when an
         operand is a problem constant (packed once), 2 packed panels always."""
         check(lib().dpd_ctx_set_gemm_config(self.h, int(cfg), int(packing)))

@@ -47,9 +47,10 @@ def test_summarize_gathers_joins_trace_counters_and_launch_list(tmp_path):
     for name, vals, div in (("fetch.csv", fetched, 2048.0), ("write.csv", written, 1024.0)):
         with open(tmp_path / name, "w", newline="") as f:
             w = csv.writer(f)
-            w.writerow(["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"])
-            for i, ((kname, _), v) in enumerate(zip(seq, vals)):
-                w.writerow([i + 1, kname, "FETCH_SIZE" if name == "fetch.csv" else "WRITE_SIZE", v / div])
+            w.writerow(["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value", "Start_Timestamp",
+                        "End_Timestamp"])
+            for i, ((kname, dur), v) in enumerate(zip(seq, vals)):
+                w.writerow([i + 1, kname, "FETCH_SIZE" if name == "fetch.csv" else "WRITE_SIZE", v / div, 0, dur])
     meta = {"precision": "bf16x3", "workload": "d2h-120", "copy_bytes": 2**29,
             "sigma_records": [["t:x1", "gather", 1, 3e6], ["t:bf16x3:t2", "gemm", 1, 9e6], ["t:pack", "gather", 1, 2e6]],
             "sorts": [["sstgenso '3142'", 16e6]]}
@@ -90,10 +91,11 @@ def test_summarize_gemms_derives_busy_share_and_counted_flops(tmp_path):
     busy = 0.5 * 1024 * 10e-6 * 2.4e9
     with open(tmp_path / "pmc.csv", "w", newline="") as f:
         w = csv.writer(f)
-        w.writerow(["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"])
-        for i, (kname, _) in enumerate(seq):
-            w.writerow([i + 1, kname, "SQ_VALU_MFMA_BUSY_CYCLES", busy if "gemm" in kname else 0])
-            w.writerow([i + 1, kname, "SQ_INSTS_VALU_MFMA_MOPS_F32", 2e9 / 512 if "gemm" in kname else 0])
+        w.writerow(["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value", "Start_Timestamp",
+                    "End_Timestamp"])
+        for i, (kname, dur) in enumerate(seq):
+            w.writerow([i + 1, kname, "SQ_VALU_MFMA_BUSY_CYCLES", busy if "gemm" in kname else 0, 0, dur])
+            w.writerow([i + 1, kname, "SQ_INSTS_VALU_MFMA_MOPS_F32", 2e9 / 512 if "gemm" in kname else 0, 0, dur])
     meta = {"precision": "fp32", "workload": "d2h-120",
             "sigma_records": [["t:x1", "gather", 1, 1e6, 0, 0], ["t:fp32:t2", "gemm", 1, 5e6, 1e9, 1]]}
     pj = tmp_path / "probe.json"
