@@ -39,14 +39,17 @@ namespace dpd {
 
 // ---------------------------------------------------------------------------
 // small dense nonsymmetric eigenproblem (FP64, row-major n x n)
-// Householder reduction A = Q H Q^T to upper Hessenberg form (Q accumulated),
-// the Francis double-shift QR algorithm for the eigenvalues, and for the requested columns
-// inverse iteration on H (O(n^2) per solve) mapped back by Q; vectors have
-// unit 2-norm (as dgeev's).
+// Householder reduction A = Q H Q^T to upper Hessenberg form (Golub & Van Loan,
+// Algorithm 7.4.2; the reflectors are kept, Q is never formed), the Francis
+// double-shift QR algorithm for the eigenvalues, and for the requested columns
+// inverse iteration on H (one LU of the shifted H per vector, O(n^2)) mapped
+// back by the reflectors; vectors have unit 2-norm (as dgeev's).
 // ---------------------------------------------------------------------------
-static void hessenberg(std::vector<double>& a, int n, std::vector<double>& q) {
-  q.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; i++) q[(size_t)i * n + i] = 1.0;
+// reflector k (k < n - 2): I - tau[k] v v^T with v[k+1..n) in column k of `vr`
+// (row-major n x n, entry (i, k)), zero elsewhere; tau[k] = 0: no reflection
+static void hessenberg(std::vector<double>& a, int n, std::vector<double>& vr, std::vector<double>& tau) {
+  vr.assign((size_t)n * n, 0.0);
+  tau.assign((size_t)std::max(n, 1), 0.0);
   std::vector<double> v(n), w(n);
   for (int k = 0; k < n - 2; k++) {
     double alpha = 0.0;
@@ -54,41 +57,35 @@ static void hessenberg(std::vector<double>& a, int n, std::vector<double>& q) {
     alpha = std::sqrt(alpha);
     if (alpha == 0.0) continue;
     if (a[(k + 1) * n + k] > 0) alpha = -alpha;
-    std::fill(v.begin(), v.end(), 0.0);
     v[k + 1] = a[(k + 1) * n + k] - alpha;
     for (int i = k + 2; i < n; i++) v[i] = a[i * n + k];
     double vv = 0.0;
     for (int i = k + 1; i < n; i++) vv += v[i] * v[i];
     if (vv == 0.0) continue;
-    const double tau = 2.0 / vv;
-    // A = H A H with H = I - tau v v^T; Q = Q H.  From the left: w^T = v^T A
-    // accumulated row by row (contiguous), then A -= tau v w^T
-    std::fill(w.begin(), w.end(), 0.0);
+    const double tk = 2.0 / vv;
+    tau[k] = tk;
+    for (int i = k + 1; i < n; i++) vr[(size_t)i * n + k] = v[i];
+    // A = H A H with H = I - tau v v^T.  From the left on columns k..n-1 (the
+    // columns before k are zero in rows k+1..n-1): w^T = v^T A accumulated row
+    // by row (contiguous), then A -= tau v w^T
+    for (int j = k; j < n; j++) w[j] = 0.0;
     for (int i = k + 1; i < n; i++) {
       const double vi = v[i];
       const double* ai = &a[(size_t)i * n];
-      for (int j = 0; j < n; j++) w[j] += vi * ai[j];
+      for (int j = k; j < n; j++) w[j] += vi * ai[j];
     }
     for (int i = k + 1; i < n; i++) {
-      const double f = tau * v[i];
+      const double f = tk * v[i];
       double* ai = &a[(size_t)i * n];
-      for (int j = 0; j < n; j++) ai[j] -= f * w[j];
+      for (int j = k; j < n; j++) ai[j] -= f * w[j];
     }
-    // from the right, row by row: A(i,:) -= tau (A(i,:) v) v^T, Q likewise
+    // from the right, row by row: A(i,:) -= tau (A(i,:) v) v^T
     for (int i = 0; i < n; i++) {
       double* ai = &a[(size_t)i * n];
-      double* qi = &q[(size_t)i * n];
-      double s = 0.0, t = 0.0;
-      for (int j = k + 1; j < n; j++) {
-        s += ai[j] * v[j];
-        t += qi[j] * v[j];
-      }
-      s *= tau;
-      t *= tau;
-      for (int j = k + 1; j < n; j++) {
-        ai[j] -= s * v[j];
-        qi[j] -= t * v[j];
-      }
+      double s = 0.0;
+      for (int j = k + 1; j < n; j++) s += ai[j] * v[j];
+      s *= tk;
+      for (int j = k + 1; j < n; j++) ai[j] -= s * v[j];
     }
     for (int i = k + 2; i < n; i++) a[i * n + k] = 0.0;
   }
@@ -246,27 +243,40 @@ static int francis_eigenvalues(std::vector<double>& H, int n, double* wr, double
   return 0;
 }
 
-// solve (H - mu I) y = b for upper Hessenberg H: LU with partial pivoting
-// between neighbouring rows (O(n^2)); tiny pivots are replaced by +-tiny
-static void hess_solve(const std::vector<double>& H, int n, double mu, double* b, double tiny,
-                       std::vector<double>& M) {
+// LU factors of (H - mu I) for upper Hessenberg H with partial pivoting between
+// neighbouring rows (O(n^2)): U in M (row-major), the multiplier of step k in
+// f[k] and whether rows k, k+1 were swapped in sw[k]; tiny pivots are replaced
+// by +-tiny
+static void hess_factor(const std::vector<double>& H, int n, double mu, double tiny, std::vector<double>& M,
+                        std::vector<double>& f, std::vector<char>& sw) {
   M = H;
+  f.assign((size_t)std::max(n, 1), 0.0);
+  sw.assign((size_t)std::max(n, 1), 0);
   for (int i = 0; i < n; i++) M[(size_t)i * n + i] -= mu;
   for (int k = 0; k < n - 1; k++) {
     double* rk = &M[(size_t)k * n];
     double* r1 = &M[(size_t)(k + 1) * n];
     if (std::fabs(r1[k]) > std::fabs(rk[k])) {
       for (int j = k; j < n; j++) std::swap(rk[j], r1[j]);
-      std::swap(b[k], b[k + 1]);
+      sw[k] = 1;
     }
     if (std::fabs(rk[k]) < tiny) rk[k] = rk[k] < 0 ? -tiny : tiny;
-    const double f = r1[k] / rk[k];
+    const double fk = r1[k] / rk[k];
+    f[k] = fk;
     r1[k] = 0.0;
-    for (int j = k + 1; j < n; j++) r1[j] -= f * rk[j];
-    b[k + 1] -= f * b[k];
+    for (int j = k + 1; j < n; j++) r1[j] -= fk * rk[j];
   }
   double* rl = &M[(size_t)(n - 1) * n];
-  if (std::fabs(rl[n - 1]) < tiny) rl[n - 1] = rl[n - 1] < 0 ? -tiny : tiny;
+  if (n > 0 && std::fabs(rl[n - 1]) < tiny) rl[n - 1] = rl[n - 1] < 0 ? -tiny : tiny;
+}
+
+// b <- (H - mu I)^-1 b from hess_factor's factors
+static void hess_apply(const std::vector<double>& M, int n, const std::vector<double>& f,
+                       const std::vector<char>& sw, double* b) {
+  for (int k = 0; k < n - 1; k++) {
+    if (sw[k]) std::swap(b[k], b[k + 1]);
+    b[k + 1] -= f[k] * b[k];
+  }
   for (int i = n - 1; i >= 0; i--) {
     double s = b[i];
     const double* ri = &M[(size_t)i * n];
@@ -278,7 +288,8 @@ static void hess_solve(const std::vector<double>& H, int n, double mu, double* b
 // two-phase solver: eigenvalues first, then eigenvectors of chosen real eigenvalues
 struct SmallEig {
   int n = 0;
-  std::vector<double> H, Q, T, y, M;
+  std::vector<double> H, VR, tau, T, y, M, f;
+  std::vector<char> sw;
   double anorm = 1.0;
   const double* wr = nullptr;
   int values(int n_, const double* Arow, double* wr_, double* wi_) {
@@ -286,7 +297,7 @@ struct SmallEig {
     wr = wr_;
     if (n == 0) return DPD_OK;
     H.assign(Arow, Arow + (size_t)n * n);
-    hessenberg(H, n, Q);
+    hessenberg(H, n, VR, tau);
     T = H;
     if (francis_eigenvalues(T, n, wr_, wi_) != 0) return fail(DPD_EINVAL, "subspace eigensolver did not converge");
     anorm = 0.0;
@@ -300,20 +311,24 @@ struct SmallEig {
     y.resize(n);
     for (int i = 0; i < n; i++) y[i] = 1.0 + 0.01 * ((i * 7 + 3) % 11);
     const double mu = wr[k] + eps * anorm * 4.0;
+    hess_factor(H, n, mu, eps * anorm, M, f, sw);
     for (int it = 0; it < 3; it++) {
-      hess_solve(H, n, mu, y.data(), eps * anorm, M);
+      hess_apply(M, n, f, sw, y.data());
       double nrm = 0.0;
       for (int i = 0; i < n; i++) nrm += y[i] * y[i];
       nrm = std::sqrt(nrm);
       for (int i = 0; i < n; i++) y[i] /= nrm;
     }
-    // x = Q y (Q orthogonal: unit norm kept)
-    for (int i = 0; i < n; i++) {
+    // x = Q y = H_0 H_1 ... H_{n-3} y (orthogonal: unit norm kept)
+    for (int kk = n - 3; kk >= 0; kk--) {
+      const double tk = tau[kk];
+      if (tk == 0.0) continue;
       double s = 0.0;
-      const double* qi = &Q[(size_t)i * n];
-      for (int j = 0; j < n; j++) s += qi[j] * y[j];
-      x[i] = s;
+      for (int i = kk + 1; i < n; i++) s += VR[(size_t)i * n + kk] * y[i];
+      s *= tk;
+      for (int i = kk + 1; i < n; i++) y[i] -= s * VR[(size_t)i * n + kk];
     }
+    for (int i = 0; i < n; i++) x[i] = y[i];
   }
 };
 

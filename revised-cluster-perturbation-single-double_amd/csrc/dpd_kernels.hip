@@ -1515,9 +1515,6 @@ hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, c
   return hipGetLastError();
 }
 
-// C[j + k*nv] = P_j . B_k for nv <= kMgsMulti vectors P (stride n) and nb basis
-// vectors B (stride n); grid (kRedBlocks, ceil(nb/4)); each basis vector is read
-// once, partials reduced in a fixed order
 // C[j + k*nv] = P_j . B_k for NV (<= kMgsMulti) vectors P (stride n) against KW
 // basis vectors B_k0.. per block (stride n); grid (kRedBlocks, ceil(nb/KW)).  Each
 // basis vector is read once; the P vectors are re-read by every block column,
