@@ -381,8 +381,11 @@ def batch_bench(cfg, args):
     log(f"batched fp32 grouped, no graph: {t_b32g:.3f}s")
     t_b32s, (_, w_b32s, inf_b32s) = timed(batch, "fp32", args.conv_mixed, False, False, True)
     log(f"batched fp32 one launch per problem: {t_b32s:.3f}s")
-    ctx.profile(True, 200000)
     t_b32, (_, w_b32, inf_b32) = timed(batch, "fp32", args.conv_mixed, True, True, True)
+    # the grouped GEMMs' own times from one more (untimed) run with the context
+    # profiler on: the graph steps then run eagerly, one event pair per launch
+    ctx.profile(True, 200000)
+    batch("fp32", args.conv_mixed, True, False, True)
     prof = ctx.profile_report()
     ctx.profile(False)
     log(f"batched fp32 grouped + graph + fp64 refine: {t_b32:.3f}s steps {inf_b32[0]['batch']}")

@@ -1029,6 +1029,9 @@ extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davi
   for (int64_t i = 0; graph && i < n && !rc; i++) {
     const size_t bytes = sizeof(double) * (size_t)probs[i]->dim * (size_t)std::max<int64_t>(1, opts[i].nroot);
     if ((rc = X[i].alloc(bytes)) || (rc = Y[i].alloc(bytes))) break;
+    // a member of a graph's shape that has no request in a step is computed
+    // on whatever its staging vectors hold: start them finite
+    if (hipMemsetAsync(X[i].p, 0, bytes, st) != hipSuccess) rc = fail(DPD_EHIP, "staging clear failed");
   }
   // each solver's own work on a stream of its own, so that the vector algebra
   // and host synchronisations of different problems overlap; the sigma step
@@ -1047,29 +1050,72 @@ extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davi
   GraphCache gc;
   Batcher B;
   B.active = (int)n;
+  // graph shape: the members a captured step covers, each with all its roots
+  // (a member's request never exceeds them), so that members converging or
+  // asking for fewer vectors replay the same graph; a new shape (the members
+  // still asking) is taken when a request falls outside it or the requested
+  // work drops below a quarter of the shape's (computing converged members
+  // costs more than the two eager steps a new shape takes beyond that)
+#ifndef DPD_BATCH_SHAPE_FRAC
+#define DPD_BATCH_SHAPE_FRAC 0.25
+#endif
+  std::vector<char> in_shape(n, 1);
+  double shape_work = 0.0;
+  for (int64_t i = 0; i < n; i++) shape_work += (double)probs[i]->dim * std::max<int64_t>(1, opts[i].nroot);
+  int shape_id = 0;
   // one batched sigma step on the context stream
   auto run_step = [&](std::vector<SigmaReq*>& reqs) -> int {
-    const int m = (int)reqs.size();
-    std::vector<dpd_problem*> ps(m);
-    std::vector<const double*> xs(m);
-    std::vector<double*> ys(m);
-    std::vector<int64_t> nv(m);
-    std::string key;
-    for (int i = 0; i < m; i++) {
-      ps[i] = probs[reqs[i]->id];
-      nv[i] = reqs[i]->nv;
-      xs[i] = graph ? (const double*)X[reqs[i]->id].p : reqs[i]->x;
-      ys[i] = graph ? (double*)Y[reqs[i]->id].p : reqs[i]->y;
-      key += std::to_string(reqs[i]->id) + ":" + std::to_string(nv[i]) + ",";
-    }
-    if (!grouped) {
+    bool staged = graph;  // every request fits its member's staging vectors
+    for (SigmaReq* q : reqs) staged = staged && q->nv <= std::max<int64_t>(1, opts[q->id].nroot);
+    if (!staged) {
+      const int m = (int)reqs.size();
+      std::vector<dpd_problem*> ps(m);
+      std::vector<const double*> xs(m);
+      std::vector<double*> ys(m);
+      std::vector<int64_t> nv(m);
       for (int i = 0; i < m; i++) {
-        int r = problem_sigma(ps[i], xs[i], ys[i], nv[i]);
-        if (r) return r;
+        ps[i] = probs[reqs[i]->id];
+        nv[i] = reqs[i]->nv;
+        xs[i] = reqs[i]->x;
+        ys[i] = reqs[i]->y;
       }
-      return DPD_OK;
+      if (!grouped) {
+        for (int i = 0; i < m; i++) {
+          int r = problem_sigma(ps[i], xs[i], ys[i], nv[i]);
+          if (r) return r;
+        }
+        return DPD_OK;
+      }
+      return problem_sigma_group(ps.data(), xs.data(), ys.data(), nv.data(), m);
     }
-    if (!graph) return problem_sigma_group(ps.data(), xs.data(), ys.data(), nv.data(), m);
+    double work = 0.0;
+    bool fits = true;
+    for (SigmaReq* q : reqs) {
+      work += (double)probs[q->id]->dim * q->nv;
+      fits = fits && in_shape[q->id];
+    }
+    if (!fits || work < DPD_BATCH_SHAPE_FRAC * shape_work) {
+      std::fill(in_shape.begin(), in_shape.end(), 0);
+      shape_work = 0.0;
+      for (SigmaReq* q : reqs) {
+        in_shape[q->id] = 1;
+        shape_work += (double)probs[q->id]->dim * std::max<int64_t>(1, opts[q->id].nroot);
+      }
+      shape_id++;
+    }
+    std::vector<dpd_problem*> ps;
+    std::vector<const double*> xs;
+    std::vector<double*> ys;
+    std::vector<int64_t> nv;
+    for (int64_t i = 0; i < n; i++)
+      if (in_shape[i]) {
+        ps.push_back(probs[i]);
+        nv.push_back(std::max<int64_t>(1, opts[i].nroot));
+        xs.push_back((const double*)X[i].p);
+        ys.push_back((double*)Y[i].p);
+      }
+    const int m = (int)ps.size();
+    const std::string key = std::to_string(shape_id);
     // graphs hold addresses of the workspace, task tables and packed panels:
     // once any of those was freed, moved or repacked they are dropped and the
     // shapes learnt again (two eager runs, then a capture)
@@ -1079,8 +1125,8 @@ extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davi
       gc.drop();
       gc.gen = gen;
     }
-    for (int i = 0; i < m; i++)
-      DPD_HIP_TRY(hipMemcpyAsync((void*)xs[i], reqs[i]->x, sizeof(double) * ps[i]->dim * nv[i],
+    for (SigmaReq* q : reqs)
+      DPD_HIP_TRY(hipMemcpyAsync(X[q->id].p, q->x, sizeof(double) * probs[q->id]->dim * q->nv,
                                  hipMemcpyDeviceToDevice, st));
     auto it = gc.exec.find(key);
     int& seen = gc.seen[key];
@@ -1126,9 +1172,9 @@ extern "C" int dpd_davidson_batch(dpd_problem* const* probs, int64_t n, dpd_davi
         if (r) return r;
       }
     }
-    for (int i = 0; i < m; i++)
-      DPD_HIP_TRY(hipMemcpyAsync(reqs[i]->y, ys[i], sizeof(double) * ps[i]->dim * nv[i], hipMemcpyDeviceToDevice,
-                                 st));
+    for (SigmaReq* q : reqs)
+      DPD_HIP_TRY(hipMemcpyAsync(q->y, Y[q->id].p, sizeof(double) * probs[q->id]->dim * q->nv,
+                                 hipMemcpyDeviceToDevice, st));
     return DPD_OK;
   };
   B.run = [&](std::vector<SigmaReq*>& reqs) -> int {

@@ -123,9 +123,10 @@ def test_sigma_group_same_shape_members(torch, ctx, prec):
 
 
 def test_davidson_batch_graph_shrinking_member_set(torch, ctx):
-    """8 members that converge at different iterations: the grouped sigma runs
-    over shrinking member sets, each captured after two eager runs; every
-    member reproduces its one-at-a-time solve"""
+    """8 members that converge at different iterations: the captured grouped
+    sigma covers every root of its members, so it is replayed while members
+    converge (a smaller member set is captured once the requests fall below
+    a quarter of the covered work); every member reproduces its one-at-a-time solve"""
     import dpdhip
     probs = [dpdhip.Problem(ctx, sp, "sf-cpsd", 1) for sp in batch_specs(8, seed0=500, lo=10, hi=24)]
     print(f"[batch] shrinking set dims={[p.dim for p in probs]}", flush=True)
@@ -134,6 +135,9 @@ def test_davidson_batch_graph_shrinking_member_set(torch, ctx):
                                           graph=True)
     print(f"[batch] {infos[0]['batch']} iterations={[i['iterations'] for i in infos]}", flush=True)
     assert len({i["iterations"] for i in infos}) > 1  # the member set does shrink
+    # a graph covers all roots of its members, so converging members keep it alive
+    b = infos[0]["batch"]
+    assert b["captures_abandoned"] == 0 and b["graph_replays"] >= 0.6 * b["steps"], b
     for (w0, R0, i0), w, i in zip(ref, ws, infos):
         assert i0["converged"] and i["converged"]
         assert i["iterations"] == i0["iterations"]
