@@ -1520,6 +1520,7 @@ struct CallPrep {
   double beta = 1.0;
   bool p3_f32 = false;  // the scratch result is FP32 (FP32-accumulating packed GEMM)
   bool x1_f32 = false, x2_f32 = false;  // expanded operands held in FP32 (they only feed the pack)
+  void* p3_ext = nullptr;  // scratch result outside the workspace (a held-back epilogue's slot)
   PackSrc a, b;
 };
 
@@ -1696,7 +1697,8 @@ static int call_pre(dpd_ctx* c, CallPrep& r, const Operand& o1, const Operand& o
   r.p3 = const_cast<double*>(o3.data);
   r.beta = 1.0;
   if (P->epi_on) {
-    r.p3 = (double*)ws_take(c, (r.p3_f32 ? sizeof(float) : sizeof(double)) * P->len3x * nbatch);
+    r.p3 = r.p3_ext ? (double*)r.p3_ext
+                    : (double*)ws_take(c, (r.p3_f32 ? sizeof(float) : sizeof(double)) * P->len3x * nbatch);
     if (!r.p3) return DPD_ENOMEM;
     r.beta = 0.0;
   }
@@ -1808,8 +1810,82 @@ static int64_t pack_bytes(dpd_ctx* c, const CallPrep& r, const Operand& o1, cons
   return bytes;
 }
 
+// the held-back epilogue of a call (its scratch result in the context slot
+// d->slot): only an accumulating epilogue that runs through a source code;
+// false when the call is to finish its own epilogue
+static bool epi_defer_setup(dpd_ctx* c, ContractPlan* P, CallPrep& r, const Operand& o3, int nbatch,
+                            EpiDefer* d) {
+  d->deferred = false;
+  if (!P->epi_on || d->slot < 0 || d->slot >= kEpiMax) return false;
+  MapArgs m = P->epi;
+  m.nbatch = nbatch;
+  m.in_bstride = P->len3x;
+  m.out_bstride = o3.bstride;
+  if (m.beta != 1.0 || m.nterm < 1 || m.nterm > kMaxTerms || m.out.len <= 0) return false;
+  const int32_t* code = plan_code(c, P, 2, m);
+  if (!code) return false;
+  const size_t bytes = (r.p3_f32 ? sizeof(float) : sizeof(double)) * (size_t)P->len3x * nbatch;
+  DevBuf& b = c->epi_slot[d->slot];
+  if (!b.p || b.bytes < bytes) {
+    if (g_capturing) return false;  // grown on an eager call only
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return false;
+    bump_mem_gen();  // captured graphs may hold the old slot
+    b.reset();
+    if (b.alloc(bytes) != DPD_OK) {
+      (void)hipGetLastError();
+      return false;
+    }
+  }
+  r.p3_ext = b.p;
+  d->f32 = r.p3_f32;
+  d->src.code = code;
+  d->src.in = b.p;
+  d->src.in_bstride = m.in_bstride;
+  d->src.nt = m.nterm;
+  for (int t = 0; t < kMaxTerms; t++) d->src.coef[t] = t < m.nterm ? m.term[t].coef : 0.0;
+  d->src.alpha = m.alpha;
+  d->n = m.out.len;
+  d->out_bstride = m.out_bstride;
+  d->nbatch = nbatch;
+  d->out = const_cast<double*>(o3.data);
+  return true;
+}
+
+int epi_flush(dpd_ctx* c, EpiDefer* d, int n, const char* tag) {
+  int i = 0;
+  while (i < n) {
+    if (!d[i].deferred) {
+      i++;
+      continue;
+    }
+    // a run of held-back epilogues into the same output, tensors and precision
+    EpiFuse g;
+    std::memset(&g, 0, sizeof g);
+    g.n = d[i].n;
+    g.out_bstride = d[i].out_bstride;
+    g.nbatch = d[i].nbatch;
+    int j = i;
+    while (j < n && d[j].deferred && g.ns < kEpiMax && d[j].out == d[i].out && d[j].n == d[i].n &&
+           d[j].out_bstride == d[i].out_bstride && d[j].nbatch == d[i].nbatch && d[j].f32 == d[i].f32)
+      g.s[g.ns++] = d[j++].src;
+    ProfRec* pr = nullptr;
+    if (c->prof.on && tag) {
+      // algorithmic bytes: the output read and written once, every scratch result read once
+      int64_t bytes = 2 * 8 * g.n * g.nbatch;
+      for (int s = 0; s < g.ns; s++) bytes += (d[i].f32 ? 4 : 8) * g.s[s].in_bstride * g.nbatch;
+      pr = prof_open(c, tag, 0, PREC_F64, 0, 0, bytes, 0, 1);
+      if (pr) DPD_HIP_TRY(hipEventRecord(pr->a, c->stream));
+    }
+    DPD_HIP_TRY(launch_epi_fused(g, d[i].f32, d[i].out, c->stream));
+    prof_close(c, pr);
+    for (int k = i; k < j; k++) d[k].deferred = false;
+    i = j;
+  }
+  return DPD_OK;
+}
+
 int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
-                  double fact, int anti, int nbatch, int prec, int64_t* flops, const char* tag) {
+                  double fact, int anti, int nbatch, int prec, int64_t* flops, const char* tag, EpiDefer* defer) {
   ContractPlan* P;
   int rc = plan_of(c, kind, o1, o2, o3, nbatch, anti, &P);
   if (rc) return rc;
@@ -1817,6 +1893,7 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   CallPrep r;
   size_t need = 0;
   if ((rc = call_setup(c, P, o1, o2, nbatch, prec, r, &need))) return rc;
+  const bool held = defer && epi_defer_setup(c, P, r, o3, nbatch, defer);
   ws_reset(c);
   if ((rc = ws_reserve(c, need))) return rc;
   if ((rc = call_pre(c, r, o1, o2, o3, nbatch, tag))) return rc;
@@ -1851,6 +1928,10 @@ int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Op
   }
   if (rc) return rc;
   if (pr && !r.packed) DPD_HIP_TRY(hipEventRecord(pr->b, c->stream));
+  if (held) {
+    defer->deferred = true;
+    return DPD_OK;
+  }
   return call_post(c, r, o3, nbatch, tag);
 }
 

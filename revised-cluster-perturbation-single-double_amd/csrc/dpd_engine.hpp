@@ -212,6 +212,8 @@ struct dpd_ctx {
   // source codes of the plans' gathers, keyed by layouts and terms
   std::unordered_map<std::string, std::unique_ptr<dpd::DevBuf>> map_cache;
   size_t map_bytes = 0;
+  // scratch results of held-back epilogues (grow-only, one per slot)
+  dpd::DevBuf epi_slot[dpd::kEpiMax];
   // tiled permutation plans of the sorts, keyed by layouts and permutation
   std::unordered_map<std::string, std::unique_ptr<dpd::SortPlan>> sort_cache;
   std::vector<char> pinned_scratch;
@@ -232,10 +234,26 @@ void drop_pack_entries(dpd_ctx* c, bool (*match)(uint64_t id, uint64_t key), uin
 // v3 = fl32(f2 * v3 + f1 * C) formed in FP64
 int engine_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, const Operand& o3,
                 double f1, double f2, int nbatch, bool f32data = false);
-// contract surface (mod_contract semantics) with optional batching of one operand
+// an accumulating epilogue held back so that several contractions into one
+// output are finished by one launch (launch_epi_fused): the source code of the
+// gather, the scratch result (kept in a context slot until the launch) and the
+// output; deferred is false when the call finished its own epilogue
+struct EpiDefer {
+  int slot = 0;  // context scratch slot for the result (0 .. kEpiMax-1)
+  bool deferred = false;
+  bool f32 = false;
+  EpiSrc src{};
+  int64_t n = 0, out_bstride = 0;
+  int nbatch = 0;
+  double* out = nullptr;
+};
+// contract surface (mod_contract semantics) with optional batching of one operand;
+// defer: hold back an accumulating epilogue that has a source code (see EpiDefer)
 int contract_call(dpd_ctx* c, int kind, const Operand& o1, const Operand& o2, Operand o3,
                   double fact, int anti, int nbatch, int prec, int64_t* flops = nullptr,
-                  const char* tag = nullptr);
+                  const char* tag = nullptr, EpiDefer* defer = nullptr);
+// finishes held-back epilogues of one output in order (one launch)
+int epi_flush(dpd_ctx* c, EpiDefer* d, int n, const char* tag);
 // the same contraction of n problems (kind, labels and anti equal; sizes may
 // differ): one grouped GEMM launch for all of them where every member is packed
 int contract_group(dpd_ctx* c, int kind, const Operand* o1, const Operand* o2, const Operand* o3, int n,

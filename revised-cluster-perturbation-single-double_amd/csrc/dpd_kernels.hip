@@ -297,6 +297,56 @@ hipError_t launch_gather_code_f32out(const MapArgs& a, const int32_t* code, cons
 }
 
 // ===========================================================================
+// fused accumulating epilogues: the antisymmetrising epilogues of several
+// contractions into one output, which is then read and written once
+// ===========================================================================
+template <typename TIN>
+__global__ __launch_bounds__(256) void k_epi_fused(EpiFuse g, double* __restrict__ out) {
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.n; o += (int64_t)gridDim.x * blockDim.x) {
+    int32_t cd[kEpiMax][kMaxTerms];
+#pragma unroll
+    for (int s = 0; s < kEpiMax; s++)
+#pragma unroll
+      for (int t = 0; t < kMaxTerms; t++)
+        cd[s][t] = (s < g.ns && t < g.s[s].nt) ? g.s[s].code[o * g.s[s].nt + t] : 0;
+#pragma unroll 2
+    for (int b = 0; b < g.nbatch; b++) {
+      double* outb = out + (int64_t)b * g.out_bstride;
+      double v = outb[o];
+#pragma unroll
+      for (int s = 0; s < kEpiMax; s++) {
+        if (s >= g.ns) break;
+        const TIN* inb = (const TIN*)g.s[s].in + (int64_t)b * g.s[s].in_bstride;
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < kMaxTerms; t++) {
+          const int32_t c = cd[s][t];
+          if (c != 0) {
+            const int64_t ad = c > 0 ? (int64_t)c - 1 : (int64_t)(-c) - 1;
+            const double cf = c > 0 ? g.s[s].coef[t] : -g.s[s].coef[t];
+            acc += cf * (double)inb[ad];
+          }
+        }
+        v = 1.0 * v + g.s[s].alpha * acc;
+      }
+      outb[o] = v;
+    }
+  }
+}
+
+hipError_t launch_epi_fused(const EpiFuse& g, bool in_f32, double* out, hipStream_t s) {
+  if (g.n <= 0 || g.nbatch <= 0 || g.ns <= 0) return hipSuccess;
+  if (g.ns > kEpiMax) return hipErrorInvalidValue;
+  int64_t b = (g.n + 255) / 256;
+  if (b > 16384) b = 16384;
+  if (in_f32)
+    hipLaunchKernelGGL((k_epi_fused<float>), dim3((unsigned)b), dim3(256), 0, s, g, out);
+  else
+    hipLaunchKernelGGL((k_epi_fused<double>), dim3((unsigned)b), dim3(256), 0, s, g, out);
+  return hipGetLastError();
+}
+
+// ===========================================================================
 // tiled DPD permutation: one workgroup per work item of a SortTask (the task
 // found by bisection over the tasks' first items).  A transposing tile is read
 // with the input-contiguous digits fastest (coalesced reads), staged in LDS

@@ -43,6 +43,27 @@ hipError_t launch_gather_code_f32in(const MapArgs& a, const int32_t* code, const
 hipError_t launch_gather_code_f32out(const MapArgs& a, const int32_t* code, const double* in, float* out,
                                      hipStream_t s);
 
+// ---- fused accumulating epilogues ------------------------------------------
+// out[o] += alpha_s * sum_t coef_{s,t} * in_s(code_s[o*nt_s + t]) for the sources
+// s in order (the same products and order as one k_gather_code launch per
+// source with beta = 1), every tensor of the batch, out read and written once
+struct EpiSrc {
+  const int32_t* code;
+  const void* in;  // float or double, as the launch says
+  int64_t in_bstride;
+  int nt;
+  double coef[kMaxTerms];
+  double alpha;
+};
+constexpr int kEpiMax = 4;
+struct EpiFuse {
+  EpiSrc s[kEpiMax];
+  int ns;
+  int64_t n, out_bstride;
+  int nbatch;
+};
+hipError_t launch_epi_fused(const EpiFuse& g, bool in_f32, double* out, hipStream_t s);
+
 // ---- tiled DPD permutation (sorts between full-pair layouts) ---------------
 // Within one irrep quadruple both tensors are dense 4-index arrays with their
 // own strides.  The output digits split into A (output-contiguous: output

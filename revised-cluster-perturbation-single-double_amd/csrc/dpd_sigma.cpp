@@ -257,16 +257,34 @@ int problem_sigma(dpd_problem* p, const double* R, double* sigma, int64_t nvec) 
   int rc = sigma_terms(p, R, sigma, nvec, T);
   if (rc) return rc;
   dpd_ctx* c = p->ctx;
+  // the antisymmetrising epilogues of consecutive contractions into sigma2
+  // (c2_r1abci, c2_r1ijka, c2_r2Wmbej) are held back and finished by one
+  // launch, in the same order, before the next term that touches sigma
+  EpiDefer held[kEpiMax];
+  int nheld = 0;
   for (const SigmaTerm& t : T) {
-    if (t.kind == 0 || t.kind == 1 || t.kind == 3)
+    const bool deferrable = (t.kind == 424 || t.kind == 444) && t.anti != 0 && nheld < kEpiMax &&
+                            (nheld == 0 || held[0].out == t.c.data);
+    if (!deferrable && nheld) {
+      if ((rc = epi_flush(c, held, nheld, "c2:epi"))) return rc;
+      nheld = 0;
+    }
+    if (t.kind == 0 || t.kind == 1 || t.kind == 3) {
       rc = run_simple_term(c, t);
-    else if (t.kind == 2)
+    } else if (t.kind == 2) {
       rc = problem_pp_ladder_streamed(p, t.x, t.y, t.nb, t.stride, p->symR, t.fact);
-    else
+    } else if (deferrable) {
+      held[nheld] = EpiDefer();
+      held[nheld].slot = nheld;
+      rc = contract_call(c, t.kind, t.a, t.b, t.c, t.fact, t.anti, t.nb, p->prec, nullptr, t.tag, &held[nheld]);
+      if (!rc && held[nheld].deferred) nheld++;
+    } else {
       rc = contract_call(c, t.kind, t.a, t.b, t.c, t.fact, t.anti, t.nb, p->prec, nullptr, t.tag);
+    }
     if (rc) return rc;
   }
-  return DPD_OK;
+  if (nheld) rc = epi_flush(c, held, nheld, "c2:epi");
+  return rc;
 }
 
 // sigma of n problems of one method and context, each contraction of all of
