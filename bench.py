@@ -750,7 +750,7 @@ def summarize_gathers(trace_csv, fetch_csv, write_csv, probe_json):
             continue
         for _ in range(cnt):
             recs.append((key, by / cnt))
-    g_disp = [d for d in sig if "k_gather" in d[0]]
+    g_disp = [d for d in sig if "k_gather" in d[0] or "k_epi" in d[0]]
     p_disp = [d for d in sig if "k_pack" in d[0]]
     g_rec = [r for r in recs if not r[0].endswith(":pack")]
     p_rec = [r for r in recs if r[0].endswith(":pack")]
@@ -901,7 +901,7 @@ def profile_solve(cfg, args):
 KERNEL_CLASSES = [
     ("GEMM (MFMA)", ("k_gemm_pk", "k_gemm2", "k_gemm<", "k_splitk_reduce")),
     ("operand packing", ("k_pack",)),
-    ("gathers (sorts, expansions, epilogues)", ("k_gather",)),
+    ("gathers (sorts, expansions, epilogues)", ("k_gather", "k_epi_fused", "k_sort")),
     ("Davidson vector algebra", ("k_multidot", "k_reduce_rows", "k_lincomb", "k_dots_multi", "k_cgs_need",
                                  "k_tri_apply", "k_chol_tri", "k_mgs", "k_pair_dots", "k_renorm_pair",
                                  "k_resid", "k_amax", "k_precond_batched", "k_scale_by_norm", "k_block_dots",

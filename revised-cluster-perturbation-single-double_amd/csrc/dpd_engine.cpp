@@ -1027,20 +1027,29 @@ static const SortPlan* sort_plan(dpd_ctx* c, const MapArgs& a) {
     c->sort_cache.clear();
   }
   auto P = std::make_unique<SortPlan>();
-  if (plan_sort(a, P->tasks, &P->nitems, &P->direct) && !P->tasks.empty() && P->nitems > 0 &&
-      P->dev.alloc(sizeof(SortTask) * P->tasks.size()) == DPD_OK &&
-      hipMemcpyAsync(P->dev.p, P->tasks.data(), sizeof(SortTask) * P->tasks.size(), hipMemcpyHostToDevice,
-                     c->stream) == hipSuccess)
-    P->ok = true;
+  if (plan_sort(a, P->tasks, &P->nitems, &P->direct) && !P->tasks.empty() && P->nitems > 0) {
+    // the task table, then the task of every work item (one int32 each)
+    const size_t nt = P->tasks.size(), tb = sizeof(SortTask) * nt;
+    P->host.resize(tb + sizeof(int32_t) * (size_t)P->nitems);
+    std::memcpy(P->host.data(), P->tasks.data(), tb);
+    sort_item_map(P->tasks, P->nitems, (int32_t*)(P->host.data() + tb));
+    if (P->dev.alloc(P->host.size()) == DPD_OK &&
+        hipMemcpyAsync(P->dev.p, P->host.data(), P->host.size(), hipMemcpyHostToDevice, c->stream) == hipSuccess)
+      P->ok = true;
+  }
   (void)hipGetLastError();
   const SortPlan* r = P->ok ? P.get() : nullptr;
   c->sort_cache.emplace(std::move(key), std::move(P));
   return r;
 }
 
+static const int32_t* sort_item_task(const SortPlan* P) {
+  return (const int32_t*)((const char*)P->dev.p + sizeof(SortTask) * P->tasks.size());
+}
+
 int gather(dpd_ctx* c, MapArgs a, const double* in, double* out) {
   if (const SortPlan* P = sort_plan(c, a)) {
-    DPD_HIP_TRY(launch_sort_tiles((const SortTask*)P->dev.p, (int)P->tasks.size(), P->nitems, P->direct, in, out,
+    DPD_HIP_TRY(launch_sort_tiles((const SortTask*)P->dev.p, sort_item_task(P), P->nitems, P->direct, in, out,
                                   c->stream));
     return DPD_OK;
   }
@@ -1050,7 +1059,7 @@ int gather(dpd_ctx* c, MapArgs a, const double* in, double* out) {
 
 int gather_f32(dpd_ctx* c, MapArgs a, const float* in, float* out) {
   if (const SortPlan* P = sort_plan(c, a)) {
-    DPD_HIP_TRY(launch_sort_tiles_f32((const SortTask*)P->dev.p, (int)P->tasks.size(), P->nitems, P->direct, in,
+    DPD_HIP_TRY(launch_sort_tiles_f32((const SortTask*)P->dev.p, sort_item_task(P), P->nitems, P->direct, in,
                                       out, c->stream));
     return DPD_OK;
   }

@@ -152,6 +152,7 @@ struct GroupTasks {
 // task table of a tiled DPD permutation (plan_sort), kept per layouts and permutation
 struct SortPlan {
   std::vector<SortTask> tasks;
+  std::vector<char> host;  // the tables as uploaded: tasks, then the task of every work item
   DevBuf dev;
   int64_t nitems = 0;
   bool direct = false;  // no transposing digits: tiles are copied straight

@@ -350,7 +350,7 @@ hipError_t launch_epi_fused(const EpiFuse& g, bool in_f32, double* out, hipStrea
 // tiled DPD permutation: one workgroup per work item of a SortTask (the task
 // found by bisection over the tasks' first items).  A transposing tile is read
 // with the input-contiguous digits fastest (coalesced reads), staged in LDS
-// with one padding element per row, and written with the output-contiguous
+// with rows padded to an odd length, and written with the output-contiguous
 // digits fastest (coalesced writes); tasks without B are copied straight.
 // Element indices are split into digits by multiplication with a reciprocal
 // (Granlund & Montgomery, "Division by invariant integers using
@@ -373,23 +373,15 @@ struct SortItem {
   int a_lo, a_n, b_lo, b_n, c0_n, c1_n;
 };
 
-// the task of work item `item` staged in LDS, and the item's tile
-__device__ inline void sort_item(const SortTask* __restrict__ tasks, int ntask, SortTask& st, SortItem& it) {
-  __shared__ int s_k;
+// the task of work item `item` (from the plan's item -> task table) staged in
+// LDS, and the item's tile
+__device__ inline void sort_item(const SortTask* __restrict__ tasks, const int32_t* __restrict__ item_task,
+                                 SortTask& st, SortItem& it) {
   const int64_t item = blockIdx.x;
-  if (threadIdx.x == 0) {
-    int lo = 0, hi = ntask - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (tasks[mid].first <= item) lo = mid;
-      else hi = mid - 1;
-    }
-    s_k = lo;
-  }
-  __syncthreads();
+  const int k = item_task[item];
   static_assert(sizeof(SortTask) % 4 == 0, "SortTask size");
   for (int i = threadIdx.x; i < (int)(sizeof(SortTask) / 4); i += blockDim.x)
-    ((int32_t*)&st)[i] = ((const int32_t*)&tasks[s_k])[i];
+    ((int32_t*)&st)[i] = ((const int32_t*)&tasks[k])[i];
   __syncthreads();
   int local = (int)(item - st.first);
   const int tb = local % st.tB;
@@ -414,11 +406,12 @@ __device__ inline void sort_item(const SortTask* __restrict__ tasks, int ntask, 
 constexpr int kSortU = 4;  // loads in flight per thread
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_sort_copy(const SortTask* __restrict__ tasks, int ntask,
-                                                   const T* __restrict__ in, T* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_sort_copy(const SortTask* __restrict__ tasks,
+                                                   const int32_t* __restrict__ item_task, const T* __restrict__ in,
+                                                   T* __restrict__ out) {
   __shared__ SortTask st;
   SortItem it;
-  sort_item(tasks, ntask, st, it);
+  sort_item(tasks, item_task, st, it);
   FastDiv fa, f0;
   fa.init(it.a_n);
   f0.init(it.c0_n);
@@ -445,19 +438,22 @@ __global__ __launch_bounds__(256) void k_sort_copy(const SortTask* __restrict__ 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_sort_tiles(const SortTask* __restrict__ tasks, int ntask,
-                                                    const T* __restrict__ in, T* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_sort_tiles(const SortTask* __restrict__ tasks,
+                                                    const int32_t* __restrict__ item_task, const T* __restrict__ in,
+                                                    T* __restrict__ out) {
   __shared__ T lds[kSortLds];
   __shared__ SortTask st;
   SortItem it;
-  sort_item(tasks, ntask, st, it);
+  sort_item(tasks, item_task, st, it);
   FastDiv fa, fb, f0, fa0, fb0;
   fa.init(it.a_n);
   fb.init(it.b_n);
   f0.init(it.c0_n);
   fa0.init(st.nA0);
   fb0.init(st.nB0);
-  const int pitch = it.b_n + 1;
+  // an odd row pitch (in elements): the A-fastest reads of the write phase then
+  // fall on distinct LDS banks
+  const int pitch = it.b_n | 1;
   const int n = it.a_n * it.b_n * it.c0_n * it.c1_n;
   // read: B fastest, then A, C0, C1
   for (int e0 = threadIdx.x; e0 < n; e0 += kSortU * blockDim.x) {
@@ -511,23 +507,23 @@ __global__ __launch_bounds__(256) void k_sort_tiles(const SortTask* __restrict__
 }
 
 template <typename T>
-static hipError_t sort_tiles_t(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const T* in, T* out,
-                               hipStream_t s) {
-  if (ntask <= 0 || nitems <= 0) return hipSuccess;
+static hipError_t sort_tiles_t(const SortTask* tasks, const int32_t* item_task, int64_t nitems, bool direct,
+                               const T* in, T* out, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
   if (nitems > 0x7fffffff) return hipErrorInvalidValue;
   if (direct)
-    hipLaunchKernelGGL((k_sort_copy<T>), dim3((unsigned)nitems), dim3(256), 0, s, tasks, ntask, in, out);
+    hipLaunchKernelGGL((k_sort_copy<T>), dim3((unsigned)nitems), dim3(256), 0, s, tasks, item_task, in, out);
   else
-    hipLaunchKernelGGL((k_sort_tiles<T>), dim3((unsigned)nitems), dim3(256), 0, s, tasks, ntask, in, out);
+    hipLaunchKernelGGL((k_sort_tiles<T>), dim3((unsigned)nitems), dim3(256), 0, s, tasks, item_task, in, out);
   return hipGetLastError();
 }
-hipError_t launch_sort_tiles(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const double* in,
-                             double* out, hipStream_t s) {
-  return sort_tiles_t<double>(tasks, ntask, nitems, direct, in, out, s);
+hipError_t launch_sort_tiles(const SortTask* tasks, const int32_t* item_task, int64_t nitems, bool direct,
+                             const double* in, double* out, hipStream_t s) {
+  return sort_tiles_t<double>(tasks, item_task, nitems, direct, in, out, s);
 }
-hipError_t launch_sort_tiles_f32(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const float* in,
-                                 float* out, hipStream_t s) {
-  return sort_tiles_t<float>(tasks, ntask, nitems, direct, in, out, s);
+hipError_t launch_sort_tiles_f32(const SortTask* tasks, const int32_t* item_task, int64_t nitems, bool direct,
+                                 const float* in, float* out, hipStream_t s) {
+  return sort_tiles_t<float>(tasks, item_task, nitems, direct, in, out, s);
 }
 
 // ===========================================================================

@@ -87,11 +87,12 @@ struct SortTask {
 constexpr int kSortLds = 2304;     // LDS elements of one tile: TC1 * TC0 * TA * (TB + 1) at most
 constexpr int kSortTileB = 2048;   // TA * TB at most
 constexpr int kSortCopy = 4096;    // elements of a copied tile at most
-// direct: every task without B (k_sort_copy), else k_sort_tiles
-hipError_t launch_sort_tiles(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const double* in,
-                             double* out, hipStream_t s);
-hipError_t launch_sort_tiles_f32(const SortTask* tasks, int ntask, int64_t nitems, bool direct, const float* in,
-                                 float* out, hipStream_t s);
+// item_task[i]: the task of work item i; direct: every task without B
+// (k_sort_copy), else k_sort_tiles
+hipError_t launch_sort_tiles(const SortTask* tasks, const int32_t* item_task, int64_t nitems, bool direct,
+                             const double* in, double* out, hipStream_t s);
+hipError_t launch_sort_tiles_f32(const SortTask* tasks, const int32_t* item_task, int64_t nitems, bool direct,
+                                 const float* in, float* out, hipStream_t s);
 
 // ---- singles-from-singles sigma terms (W r, Fac r, r Fki) -----------------
 struct SinglesArgs {

@@ -169,4 +169,13 @@ inline bool plan_sort(const MapArgs& a, std::vector<SortTask>& tasks, int64_t* n
   return true;
 }
 
+// the task of every work item (k_sort_copy / k_sort_tiles read it)
+inline void sort_item_map(const std::vector<SortTask>& tasks, int64_t nitems, int32_t* map) {
+  const size_t nt = tasks.size();
+  for (size_t k = 0; k < nt; k++) {
+    const int64_t end = k + 1 < nt ? tasks[k + 1].first : nitems;
+    for (int64_t i = tasks[k].first; i < end; i++) map[i] = (int32_t)k;
+  }
+}
+
 }  // namespace dpd

@@ -31,10 +31,12 @@ static int check(const Sym& S, const Desc& din, const int8_t perm[4], long* nite
   struct FD { uint32_t d, m, l; void init(uint32_t dd){ d=dd; l=0; while ((1u<<l) < d) l++; m=(uint32_t)((((uint64_t)1<<32)*(((uint64_t)1<<l)-d))/d+1);} uint32_t div(uint32_t n) const { return (uint32_t)((((uint64_t)n*m)>>32) + n) >> l; } };
   for (const SortTask& t : T) if ((t.nB == 0) != direct) { printf("mixed modes\n"); return 1; }
   std::vector<int64_t> lds(kSortLds);
+  std::vector<int32_t> item_task(nitems);
+  sort_item_map(T, nitems, item_task.data());
   for (int64_t item = 0; item < nitems; item++) {
-    int lo = 0, hi = (int)T.size() - 1;
-    while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (T[mid].first <= item) lo = mid; else hi = mid - 1; }
-    const SortTask& st = T[lo];
+    const int k = item_task[item];
+    if (k < 0 || k >= (int)T.size() || T[k].first > item) { printf("item map\n"); return 1; }
+    const SortTask& st = T[k];
     int local = (int)(item - st.first);
     const int tb = local % st.tB; local /= st.tB;
     const int ta = local % st.tA; local /= st.tA;
@@ -67,7 +69,7 @@ static int check(const Sym& S, const Desc& din, const int8_t perm[4], long* nite
       continue;
     }
     FD fa, fb, f0, fa0, fb0; fa.init(a_n); fb.init(b_n); f0.init(c0_n); fa0.init(st.nA0); fb0.init(st.nB0);
-    const int pitch = b_n + 1, n = a_n * b_n * c0_n * c1_n;
+    const int pitch = b_n | 1, n = a_n * b_n * c0_n * c1_n;
     std::fill(lds.begin(), lds.end(), -7);
     for (int e = 0; e < n; e++) {
       int r = (int)fb.div(e); const int ib = e - r * b_n;
