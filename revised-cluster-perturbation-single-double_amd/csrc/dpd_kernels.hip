@@ -405,35 +405,47 @@ __device__ inline void sort_item(const SortTask* __restrict__ tasks, const int32
 
 constexpr int kSortU = 4;  // loads in flight per thread
 
+// Offsets of a tile's rows: every digit combination outside the contiguous
+// group is turned into its input and output offset once per work item (LDS
+// tables of int32, relative to the item's bases; tensors stay below 2^31
+// elements), so that an element costs one reciprocal split and table reads.
 template <typename T>
 __global__ __launch_bounds__(256) void k_sort_copy(const SortTask* __restrict__ tasks,
                                                    const int32_t* __restrict__ item_task, const T* __restrict__ in,
                                                    T* __restrict__ out) {
   __shared__ SortTask st;
+  __shared__ int32_t rin[kSortRows], rout[kSortRows];
   SortItem it;
   sort_item(tasks, item_task, st, it);
-  FastDiv fa, f0;
+  // rows: C digits 0 and 1 of the tile
+  const int nr = it.c0_n * it.c1_n;
+  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+    const int i0 = r % it.c0_n, i1 = r / it.c0_n;
+    rin[r] = (int32_t)((int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1]);
+    rout[r] = (int32_t)((int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1]);
+  }
+  __syncthreads();
+  FastDiv fa;
   fa.init(it.a_n);
-  f0.init(it.c0_n);
-  const int n = it.a_n * it.c0_n * it.c1_n;
+  const T* inb = in + it.ib0 + it.a_lo;
+  T* outb = out + it.ob0 + it.a_lo;
+  const int n = it.a_n * nr;
   for (int e0 = threadIdx.x; e0 < n; e0 += kSortU * blockDim.x) {
     T v[kSortU];
-    int64_t oa[kSortU];
+    int oa[kSortU];
 #pragma unroll
     for (int u = 0; u < kSortU; u++) {
       const int e = e0 + u * blockDim.x;
       oa[u] = -1;
       if (e < n) {
         const int r = (int)fa.div(e), ia = e - r * it.a_n;
-        const int i1 = (int)f0.div(r), i0 = r - i1 * it.c0_n;
-        const int64_t a = it.a_lo + ia;
-        v[u] = in[it.ib0 + (int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1] + a];
-        oa[u] = it.ob0 + (int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1] + a;
+        v[u] = inb[rin[r] + ia];
+        oa[u] = rout[r] + ia;
       }
     }
 #pragma unroll
     for (int u = 0; u < kSortU; u++)
-      if (oa[u] >= 0) out[oa[u]] = v[u];
+      if (oa[u] >= 0) outb[oa[u]] = v[u];
   }
 }
 
@@ -442,20 +454,43 @@ __global__ __launch_bounds__(256) void k_sort_tiles(const SortTask* __restrict__
                                                     const int32_t* __restrict__ item_task, const T* __restrict__ in,
                                                     T* __restrict__ out) {
   __shared__ T lds[kSortLds];
+  __shared__ int32_t tab[kSortLds];  // input rows, then output columns
   __shared__ SortTask st;
   SortItem it;
   sort_item(tasks, item_task, st, it);
-  FastDiv fa, fb, f0, fa0, fb0;
-  fa.init(it.a_n);
-  fb.init(it.b_n);
-  f0.init(it.c0_n);
-  fa0.init(st.nA0);
-  fb0.init(st.nB0);
   // an odd row pitch (in elements): the A-fastest reads of the write phase then
   // fall on distinct LDS banks
   const int pitch = it.b_n | 1;
-  const int n = it.a_n * it.b_n * it.c0_n * it.c1_n;
-  // read: B fastest, then A, C0, C1
+  const int nc = it.c0_n * it.c1_n;
+  // input rows (A digit, C digits: B contiguous within) and output columns
+  // (B digit, C digits: A contiguous within)
+  const int nrow = it.a_n * nc, ncol = it.b_n * nc;
+  int32_t* rin = tab;
+  int32_t* cout = tab + nrow;
+  for (int r = threadIdx.x; r < nrow; r += blockDim.x) {
+    const int ia = r % it.a_n, q = r / it.a_n;
+    const int i0 = q % it.c0_n, i1 = q / it.c0_n;
+    const int A = it.a_lo + ia;
+    const int a0 = A % st.nA0, a1 = A / st.nA0;
+    rin[r] = (int32_t)((int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1] + (int64_t)a0 * st.inA[0] +
+                       (int64_t)a1 * st.inA[1]);
+  }
+  for (int q = threadIdx.x; q < ncol; q += blockDim.x) {
+    const int ib = q % it.b_n, r2 = q / it.b_n;
+    const int i0 = r2 % it.c0_n, i1 = r2 / it.c0_n;
+    const int B = it.b_lo + ib;
+    const int b0 = B % st.nB0, b1 = B / st.nB0;
+    cout[q] = (int32_t)((int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1] + (int64_t)b0 * st.outB[0] +
+                        (int64_t)b1 * st.outB[1]);
+  }
+  __syncthreads();
+  FastDiv fa, fb;
+  fa.init(it.a_n);
+  fb.init(it.b_n);
+  const T* inb = in + it.ib0 + it.b_lo;
+  T* outb = out + it.ob0 + it.a_lo;
+  const int n = it.a_n * it.b_n * nc;
+  // read: B fastest; row r = (C1 * c0_n + C0) * a_n + A
   for (int e0 = threadIdx.x; e0 < n; e0 += kSortU * blockDim.x) {
     T v[kSortU];
     int li[kSortU];
@@ -464,16 +499,9 @@ __global__ __launch_bounds__(256) void k_sort_tiles(const SortTask* __restrict__
       const int e = e0 + u * blockDim.x;
       li[u] = -1;
       if (e < n) {
-        int r = (int)fb.div(e);
-        const int ib = e - r * it.b_n;
-        const int r2 = (int)fa.div(r), ia = r - r2 * it.a_n;
-        const int i1 = (int)f0.div(r2), i0 = r2 - i1 * it.c0_n;
-        const int A = it.a_lo + ia;
-        const int a1 = (int)fa0.div(A), a0 = A - a1 * st.nA0;
-        const int64_t ad = it.ib0 + (int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1] + (int64_t)a0 * st.inA[0] +
-                           (int64_t)a1 * st.inA[1] + it.b_lo + ib;
-        v[u] = in[ad];
-        li[u] = r * pitch + ib;  // r = (i1 * c0_n + i0) * a_n + ia
+        const int r = (int)fb.div(e), ib = e - r * it.b_n;
+        v[u] = inb[rin[r] + ib];
+        li[u] = r * pitch + ib;
       }
     }
 #pragma unroll
@@ -481,28 +509,24 @@ __global__ __launch_bounds__(256) void k_sort_tiles(const SortTask* __restrict__
       if (li[u] >= 0) lds[li[u]] = v[u];
   }
   __syncthreads();
-  // write: A fastest, then B, C0, C1
+  // write: A fastest; column q = (C1 * c0_n + C0) * b_n + B
   for (int e0 = threadIdx.x; e0 < n; e0 += kSortU * blockDim.x) {
     T v[kSortU];
-    int64_t oa[kSortU];
+    int oa[kSortU];
 #pragma unroll
     for (int u = 0; u < kSortU; u++) {
       const int e = e0 + u * blockDim.x;
       oa[u] = -1;
       if (e < n) {
-        const int r = (int)fa.div(e), ia = e - r * it.a_n;
-        const int r2 = (int)fb.div(r), ib = r - r2 * it.b_n;
-        const int i1 = (int)f0.div(r2), i0 = r2 - i1 * it.c0_n;
-        const int B = it.b_lo + ib;
-        const int b1 = (int)fb0.div(B), b0 = B - b1 * st.nB0;
+        const int q = (int)fa.div(e), ia = e - q * it.a_n;
+        const int r2 = (int)fb.div(q), ib = q - r2 * it.b_n;
         v[u] = lds[(r2 * it.a_n + ia) * pitch + ib];
-        oa[u] = it.ob0 + (int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1] + (int64_t)b0 * st.outB[0] +
-                (int64_t)b1 * st.outB[1] + it.a_lo + ia;
+        oa[u] = cout[q] + ia;
       }
     }
 #pragma unroll
     for (int u = 0; u < kSortU; u++)
-      if (oa[u] >= 0) out[oa[u]] = v[u];
+      if (oa[u] >= 0) outb[oa[u]] = v[u];
   }
 }
 

@@ -87,6 +87,7 @@ struct SortTask {
 constexpr int kSortLds = 2304;     // LDS elements of one tile: TC1 * TC0 * TA * (TB + 1) at most
 constexpr int kSortTileB = 2048;   // TA * TB at most
 constexpr int kSortCopy = 4096;    // elements of a copied tile at most
+constexpr int kSortRows = 1024;    // C digit combinations of a copied tile at most
 // item_task[i]: the task of work item i; direct: every task without B
 // (k_sort_copy), else k_sort_tiles
 hipError_t launch_sort_tiles(const SortTask* tasks, const int32_t* item_task, int64_t nitems, bool direct,

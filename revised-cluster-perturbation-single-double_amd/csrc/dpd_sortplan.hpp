@@ -9,6 +9,7 @@
 // pairs).  plan_sort turns each quadruple into one SortTask.
 #pragma once
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "dpd_kernels.hpp"
@@ -21,6 +22,7 @@ inline bool sort_tiles_apply(const MapArgs& a) {
   if (a.nterm != 1 || a.term[0].coef != 1.0 || a.alpha != 1.0 || a.beta != 0.0 || a.nbatch != 1) return false;
   if (a.out.iop[0] != 1 || a.out.iop[1] != 1 || a.in.iop[0] != 1 || a.in.iop[1] != 1) return false;
   if (a.out.sym != a.in.sym || a.out.nirrep != a.in.nirrep || a.out.len != a.in.len) return false;
+  if (a.out.len >= INT32_MAX) return false;  // the kernels keep offsets in int32
   int seen = 0;
   for (int i = 0; i < 4; i++) {
     const int p = a.term[0].perm[i];
@@ -140,22 +142,25 @@ inline bool plan_sort(const MapArgs& a, std::vector<SortTask>& tasks, int64_t* n
           *cnt = (int32_t)c;
           *T = (int32_t)((ext + c - 1) / c);
         };
-        int64_t row;  // tile elements per C index (LDS rows padded by one for a transposing tile)
-        int64_t budget;
+        // (a transposing tile's LDS rows are padded by one at most; its row and
+        // column offset tables, TC1 TC0 (TA + TB) entries, fit the same bound)
+        int64_t row;  // tile elements per C index
+        int64_t budget, rows = kSortLds;
         if (nb == 0) {
           split(ea, kSortCopy, &t.TA, &t.tA);
           t.TB = 1;
           t.tB = 1;
           row = t.TA;
           budget = kSortCopy;
+          rows = kSortRows;
         } else {
           split(ea, 64, &t.TA, &t.tA);
           split(eb, kSortTileB / t.TA, &t.TB, &t.tB);
           row = (int64_t)t.TA * (t.TB + 1);
           budget = kSortLds;
         }
-        split(t.nC[0], budget / row, &t.TC0, &t.tC0);
-        split(t.nC[1], budget / (row * t.TC0), &t.TC1, &t.tC1);
+        split(t.nC[0], std::min(budget / row, rows), &t.TC0, &t.tC0);
+        split(t.nC[1], std::min(budget / (row * t.TC0), rows / t.TC0), &t.TC1, &t.tC1);
         const int64_t items = (int64_t)t.tA * t.tB * t.tC0 * t.tC1 * t.nC[2];
         t.first = total;
         total += items;

@@ -11,6 +11,7 @@
 #include <random>
 #include <vector>
 #include <algorithm>
+#include <climits>
 #include "dpd_sortplan.hpp"
 using namespace dpd;
 namespace dpd { void set_error(const std::string&) {} int fail(int c, const char*, ...) { return c; } }
@@ -56,40 +57,55 @@ static int check(const Sym& S, const Desc& din, const int8_t perm[4], long* nite
       src[o] = i; return 0;
     };
     if (direct) {
-      FD fa, f0; fa.init(a_n); f0.init(c0_n);
-      const int n = a_n * c0_n * c1_n;
-      if (n > kSortCopy) { printf("copy tile too big\n"); return 1; }
+      const int nr = c0_n * c1_n, n = a_n * nr;
+      if (n > kSortCopy || nr > kSortRows) { printf("copy tile too big\n"); return 1; }
+      std::vector<int64_t> rin(nr), rout(nr);
+      for (int r = 0; r < nr; r++) {
+        const int i0 = r % c0_n, i1 = r / c0_n;
+        rin[r] = (int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1];
+        rout[r] = (int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1];
+        if (rin[r] > INT32_MAX || rout[r] > INT32_MAX) { printf("offset exceeds int32\n"); return 1; }
+      }
+      FD fa; fa.init(a_n);
       for (int e = 0; e < n; e++) {
         const int r = (int)fa.div(e), ia = e - r * a_n;
-        const int i1 = (int)f0.div(r), i0 = r - i1 * c0_n;
-        if (ia != e % a_n || i0 != r % c0_n || i1 >= c1_n) { printf("digit error\n"); return 1; }
-        const int64_t aa = a_lo + ia;
-        if (put(ob0 + (int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1] + aa, ib0 + (int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1] + aa)) return 1;
+        if (ia != e % a_n || r >= nr) { printf("digit error\n"); return 1; }
+        if (put(ob0 + a_lo + rout[r] + ia, ib0 + a_lo + rin[r] + ia)) return 1;
       }
       continue;
     }
-    FD fa, fb, f0, fa0, fb0; fa.init(a_n); fb.init(b_n); f0.init(c0_n); fa0.init(st.nA0); fb0.init(st.nB0);
-    const int pitch = b_n | 1, n = a_n * b_n * c0_n * c1_n;
+    const int pitch = b_n | 1, nc = c0_n * c1_n, n = a_n * b_n * nc;
+    const int nrow = a_n * nc, ncol = b_n * nc;
+    if (nc * a_n * pitch > kSortLds || nrow + ncol > kSortLds) { printf("LDS overflow\n"); return 1; }
+    std::vector<int64_t> rin(nrow), cout(ncol);
+    for (int r = 0; r < nrow; r++) {
+      const int ia = r % a_n, q = r / a_n, i0 = q % c0_n, i1 = q / c0_n, A = a_lo + ia;
+      const int a0 = A % st.nA0, a1 = A / st.nA0;
+      rin[r] = (int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1] + (int64_t)a0 * st.inA[0] + (int64_t)a1 * st.inA[1];
+      if (rin[r] > INT32_MAX) { printf("offset exceeds int32\n"); return 1; }
+    }
+    for (int q = 0; q < ncol; q++) {
+      const int ib = q % b_n, r2 = q / b_n, i0 = r2 % c0_n, i1 = r2 / c0_n, B = b_lo + ib;
+      const int b0 = B % st.nB0, b1 = B / st.nB0;
+      cout[q] = (int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1] + (int64_t)b0 * st.outB[0] + (int64_t)b1 * st.outB[1];
+      if (cout[q] > INT32_MAX) { printf("offset exceeds int32\n"); return 1; }
+    }
+    FD fa, fb; fa.init(a_n); fb.init(b_n);
     std::fill(lds.begin(), lds.end(), -7);
     for (int e = 0; e < n; e++) {
-      int r = (int)fb.div(e); const int ib = e - r * b_n;
-      const int r2 = (int)fa.div(r), ia = r - r2 * a_n;
-      const int i1 = (int)f0.div(r2), i0 = r2 - i1 * c0_n;
-      if (i1 >= c1_n) { printf("i1 range\n"); return 1; }
-      const int A = a_lo + ia; const int a1 = (int)fa0.div(A), a0 = A - a1 * st.nA0;
-      const int64_t ad = ib0 + (int64_t)i0 * st.inC[0] + (int64_t)i1 * st.inC[1] + (int64_t)a0 * st.inA[0] + (int64_t)a1 * st.inA[1] + b_lo + ib;
+      const int r = (int)fb.div(e), ib = e - r * b_n;
+      if (r >= nrow) { printf("row range\n"); return 1; }
       const int li = r * pitch + ib;
       if (li < 0 || li >= kSortLds) { printf("LDS index %d\n", li); return 1; }
-      lds[li] = ad;
+      lds[li] = ib0 + b_lo + rin[r] + ib;
     }
     for (int e = 0; e < n; e++) {
-      const int r = (int)fa.div(e), ia = e - r * a_n;
-      const int r2 = (int)fb.div(r), ib = r - r2 * b_n;
-      const int i1 = (int)f0.div(r2), i0 = r2 - i1 * c0_n;
-      const int B = b_lo + ib; const int b1 = (int)fb0.div(B), b0 = B - b1 * st.nB0;
+      const int q = (int)fa.div(e), ia = e - q * a_n;
+      const int r2 = (int)fb.div(q), ib = q - r2 * b_n;
+      if (q >= ncol) { printf("column range\n"); return 1; }
       const int li = (r2 * a_n + ia) * pitch + ib;
       if (li < 0 || li >= kSortLds || lds[li] == -7) { printf("LDS read %d\n", li); return 1; }
-      if (put(ob0 + (int64_t)i0 * st.outC[0] + (int64_t)i1 * st.outC[1] + (int64_t)b0 * st.outB[0] + (int64_t)b1 * st.outB[1] + a_lo + ia, lds[li])) return 1;
+      if (put(ob0 + a_lo + cout[q] + ia, lds[li])) return 1;
     }
   }
   // compare with the generic map
