@@ -605,6 +605,12 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     if (dim_exceed) {
       DPD_HIP_TRY(hipMemcpyAsync(R, B, vb * nroot, hipMemcpyDeviceToDevice, st));
       DPD_HIP_TRY(hipMemcpyAsync(AQ, AB, vb * nroot, hipMemcpyDeviceToDevice, st));
+    } else if ((int64_t)dimIter * nroot <= kLinCoef) {
+      // the coefficients travel in the kernel arguments
+      LinCoef cf;
+      for (int j = 0; j < nroot; j++)
+        for (int k = 0; k < dimIter; k++) cf.v[k + j * dimIter] = vecNeed[k + (size_t)j * dimAbar];
+      DPD_HIP_TRY(launch_lincomb2_v(n, AB, B, n, dimIter, cf, nroot, AQ, R, st));
     } else {
       if ((rc = upload(W.coef.p, vecNeed.data(), sizeof(double) * dimAbar * nroot))) return rc;
       DPD_HIP_TRY(launch_lincomb2(n, AB, B, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, AQ, R, st));
@@ -613,9 +619,11 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     // e = Aq - w q and their largest elements (:396-406), and the preconditioned
     // directions of every root (:417-428), kept for the unconverged ones
     DPD_HIP_TRY(launch_pair_dots(n, R, n, R, n, nroot, part, dots, st));
-    if ((rc = upload(wdev, eigenVal.data(), sizeof(double) * nroot))) return rc;
-    DPD_HIP_TRY(launch_ritz_resid(n, dots, wdev, diag, R, AQ, E, PV, nroot, (double*)W.amaxp.p,
+    RootVals wv;
+    for (int v = 0; v < nroot; v++) wv.w[v] = eigenVal[v];
+    DPD_HIP_TRY(launch_ritz_resid(n, dots, wv, diag, R, AQ, E, PV, nroot, (double*)W.amaxp.p,
                                   (int64_t*)W.amaxpi.p, (double*)W.amaxv.p, (int64_t*)W.amaxi.p, st));
+    bool roots_on_device = false;  // eigenvalues and candidates uploaded (the rare redo path needs them)
     if ((rc = fetch(mv.data(), W.amaxv.p, sizeof(double) * nroot))) return rc;
     int nconv = 0;
     cand.clear();
@@ -659,10 +667,14 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     if (dimFree > 0 && !cand.empty()) {
       // the candidates' directions moved to the front of PV (root cand[k] to
       // slot k; cand is increasing, so no source is overwritten before it is read)
-      if ((rc = upload(W.idx.p, cand.data(), sizeof(int32_t) * cand.size()))) return rc;
-      for (size_t k = 0; k < cand.size(); k++)
-        if (cand[k] != (int32_t)k)
-          DPD_HIP_TRY(hipMemcpyAsync(PV + k * n, PV + (size_t)cand[k] * n, vb, hipMemcpyDeviceToDevice, st));
+      RowList rows;
+      rows.n = (int)cand.size();
+      bool moved = false;
+      for (size_t k = 0; k < cand.size(); k++) {
+        rows.src[k] = cand[k];
+        moved = moved || cand[k] != (int32_t)k;
+      }
+      if (moved) DPD_HIP_TRY(launch_compact_rows(n, PV, rows, st));
     }
     // SchmidtS (:687-726) root by root: normalise, remove the components along
     // every basis vector (the new ones of this iteration included), normalise;
@@ -762,6 +774,12 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
         // direction k is dropped: the ones after it were projected against it
         for (k = k + 1; k < cand.size(); k++) {
           if (dimFree == 0) continue;
+          if (!roots_on_device) {
+            if ((rc = upload(wdev, eigenVal.data(), sizeof(double) * nroot)) ||
+                (rc = upload(W.idx.p, cand.data(), sizeof(int32_t) * cand.size())))
+              return rc;
+            roots_on_device = true;
+          }
           DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p + k, 1, diag, E, PV + k * n, st));
           if ((rc = schmidt(k, nNewBase, dots))) return rc;
           double t2;

@@ -1102,6 +1102,51 @@ __global__ void k_lincomb(int64_t n, const double* __restrict__ X, int64_t xs, i
       if (j < nj) out[(int64_t)(j0 + j) * n + i] = s[j];
   }
 }
+// k_lincomb with the coefficients in the kernel arguments (v[k + j*nk])
+__global__ void k_lincomb_v(int64_t n, const double* __restrict__ X, int64_t xs, int nk, LinCoef cf, int nout,
+                            double* __restrict__ out, const double* __restrict__ X2, double* __restrict__ out2) {
+  if (blockIdx.z == 1) {
+    X = X2;
+    out = out2;
+  }
+  const int j0 = blockIdx.y * kLinOut;
+  const int nj = min(kLinOut, nout - j0);
+  int jc[kLinOut];
+#pragma unroll
+  for (int j = 0; j < kLinOut; j++) jc[j] = (j0 + min(j, nj - 1)) * nk;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s[kLinOut];
+#pragma unroll
+    for (int j = 0; j < kLinOut; j++) s[j] = 0.0;
+    int k = 0;
+    for (; k + 4 <= nk; k += 4) {
+      double x[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) x[u] = X[(int64_t)(k + u) * xs + i];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int j = 0; j < kLinOut; j++) s[j] += cf.v[jc[j] + k + u] * x[u];
+    }
+    for (; k < nk; k++) {
+      const double x = X[(int64_t)k * xs + i];
+#pragma unroll
+      for (int j = 0; j < kLinOut; j++) s[j] += cf.v[jc[j] + k] * x;
+    }
+#pragma unroll
+    for (int j = 0; j < kLinOut; j++)
+      if (j < nj) out[(int64_t)(j0 + j) * n + i] = s[j];
+  }
+}
+
+__global__ void k_compact_rows(int64_t n, double* __restrict__ P, RowList rows) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    for (int k = 0; k < rows.n; k++)
+      if (rows.src[k] != k) P[(int64_t)k * n + i] = P[(int64_t)rows.src[k] * n + i];
+}
+
 // lib_cc/mod_davidson.f90:417-428
 __global__ void k_precond(int64_t n, double w, const double* __restrict__ d,
                           const double* __restrict__ e, double* __restrict__ p) {
@@ -1171,6 +1216,22 @@ hipError_t launch_lincomb2(int64_t n, const double* X, const double* X2, int64_t
   if (nout == 0) return hipSuccess;
   hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n), (nout + kLinOut - 1) / kLinOut, 2), dim3(256), 0, s, n, X, xs,
                      nk, coef, ldc, nout, out, X2, out2);
+  return hipGetLastError();
+}
+hipError_t launch_lincomb2_v(int64_t n, const double* X, const double* X2, int64_t xs, int nk, const LinCoef& cf,
+                             int nout, double* out, double* out2, hipStream_t s) {
+  if (nout == 0) return hipSuccess;
+  if ((int64_t)nk * nout > kLinCoef) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_lincomb_v, dim3(grid_for(n), (nout + kLinOut - 1) / kLinOut, 2), dim3(256), 0, s, n, X, xs,
+                     nk, cf, nout, out, X2, out2);
+  return hipGetLastError();
+}
+hipError_t launch_compact_rows(int64_t n, double* P, const RowList& rows, hipStream_t s) {
+  if (rows.n <= 0) return hipSuccess;
+  if (rows.n > kRootVals) return hipErrorInvalidValue;
+  for (int k = 0; k < rows.n; k++)
+    if (rows.src[k] < k || (k > 0 && rows.src[k] <= rows.src[k - 1])) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(n)), dim3(256), 0, s, n, P, rows);
   return hipGetLastError();
 }
 hipError_t launch_precondition(int64_t n, double w, const double* d, const double* e, double* p,
@@ -1443,14 +1504,14 @@ __global__ __launch_bounds__(256) void k_resid_amax_partial(int64_t n, const dou
 // by 1/|q| when ||q| - 1| > 1e-10 (q written back), e = Aq - w q into E,
 // p = e / (w - d) (|w - d| clamped to 1e-4) into P, and the largest |e| per root.
 // The same operations as k_renorm_pair + k_resid_amax_partial + k_precond_batched.
-__global__ __launch_bounds__(256) void k_ritz_resid(int64_t n, const double* nrm2, const double* w,
+__global__ __launch_bounds__(256) void k_ritz_resid(int64_t n, const double* nrm2, RootVals w,
                                                     const double* __restrict__ d, double* __restrict__ q,
                                                     const double* __restrict__ aq, double* __restrict__ e,
                                                     double* __restrict__ p, double* pv, int64_t* pi) {
   __shared__ double sv[256];
   __shared__ int64_t si[256];
   const int v = blockIdx.y;
-  const double wv = w[v];
+  const double wv = w.w[v];
   const double t = sqrt(nrm2[v]);
   const bool scale = fabs(t - 1.0) > 1e-10;
   const double f = 1.0 / t;
@@ -1493,10 +1554,11 @@ __global__ __launch_bounds__(256) void k_ritz_resid(int64_t n, const double* nrm
   }
 }
 
-hipError_t launch_ritz_resid(int64_t n, const double* nrm2, const double* w, const double* d, double* q,
+hipError_t launch_ritz_resid(int64_t n, const double* nrm2, const RootVals& w, const double* d, double* q,
                              const double* aq, double* e, double* p, int nv, double* pv, int64_t* pi, double* ov,
                              int64_t* oi, hipStream_t s) {
   if (nv == 0) return hipSuccess;
+  if (nv > kRootVals) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_ritz_resid, dim3(kRedBlocks, nv), dim3(256), 0, s, n, nrm2, w, d, q, aq, e, p, pv, pi);
   hipLaunchKernelGGL(k_amax_final, dim3(nv), dim3(256), 0, s, kRedBlocks, pv, pi, ov, oi);
   return hipGetLastError();

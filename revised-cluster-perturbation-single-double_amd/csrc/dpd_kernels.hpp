@@ -319,14 +319,33 @@ hipError_t launch_block_dots(int64_t n, const double* X, int64_t xs, int nx, con
                              double* partial, double* out, hipStream_t s);
 // x_v, y_v scaled by 1/sqrt(nrm2[v]) unless that norm is 1 within 1e-10
 hipError_t launch_renorm_pair(int64_t n, const double* nrm2, double* x, double* y, int nv, hipStream_t s);
+// small per-iteration coefficient sets passed to the kernels by value (kernel
+// arguments: no host-to-device copy ahead of the launch)
+constexpr int kRootVals = 64;
+struct RootVals {
+  double w[kRootVals];
+};
+constexpr int kLinCoef = 256;
+struct LinCoef {
+  double v[kLinCoef];  // v[k + j*nk]
+};
+struct RowList {
+  int32_t src[kRootVals];
+  int n;
+};
 // renormalised Ritz vectors, residuals (E), preconditioned directions of every
-// root (P) and the largest |residual| per root (ov/oi) in one pass
-hipError_t launch_ritz_resid(int64_t n, const double* nrm2, const double* w, const double* d, double* q,
+// root (P) and the largest |residual| per root (ov/oi) in one pass (nv <= kRootVals)
+hipError_t launch_ritz_resid(int64_t n, const double* nrm2, const RootVals& w, const double* d, double* q,
                              const double* aq, double* e, double* p, int nv, double* pv, int64_t* pi, double* ov,
                              int64_t* oi, hipStream_t s);
 // launch_lincomb of two sets (X -> out and X2 -> out2) with the same coefficients
 hipError_t launch_lincomb2(int64_t n, const double* X, const double* X2, int64_t xs, int nk, const double* coef,
                            int ldc, int nout, double* out, double* out2, hipStream_t s);
+// the same with the coefficients by value (nk * nout <= kLinCoef)
+hipError_t launch_lincomb2_v(int64_t n, const double* X, const double* X2, int64_t xs, int nk, const LinCoef& cf,
+                             int nout, double* out, double* out2, hipStream_t s);
+// rows P_k = P_src[k] (k ascending; src[k] >= k, increasing): in place
+hipError_t launch_compact_rows(int64_t n, double* P, const RowList& rows, hipStream_t s);
 // partial[0..kRedBlocks) of dot(x, y); out[r] = sum of row r of kRedBlocks partials
 hipError_t launch_pair_dots_partial(int64_t n, const double* x, const double* y, double* partial, hipStream_t s);
 hipError_t launch_reduce_partials(const double* partial, int nrows, double* out, hipStream_t s);
