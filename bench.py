@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--probe-out", default=None,
                     help="launch list (JSON) written by --probe and --probe-gathers (default for the latter: "
                          "gather_probe.json)")
+    ap.add_argument("--summarize-kernels", nargs=3, default=None, metavar=("TRACE_CSV", "KERNEL", "GRID"),
+                    help="markdown of a kernel_trace.csv: every kernel's dispatches and time, then the dispatches "
+                         "of KERNEL with grid size GRID (one plan's launches) with their mean and median, and exit")
     ap.add_argument("--summarize-gemms", nargs=3, default=None, metavar=("TRACE_CSV", "PMC_CSV", "PROBE_JSON"),
                     help="join the kernel trace and an SQ counter run of a --probe program with its launch list: "
                          "per GEMM launch of the profiled sigma call, algorithmic and counted MFMA throughput and "
@@ -127,6 +130,9 @@ def main():
         return
     if args.summarize_gemms:
         summarize_gemms(*args.summarize_gemms)
+        return
+    if args.summarize_kernels:
+        summarize_kernels(*args.summarize_kernels)
         return
 
     import numpy as np
@@ -797,6 +803,37 @@ def summarize_gathers(trace_csv, fetch_csv, write_csv, probe_json):
     if notes:
         print(f"\nSections {notes} launched other kernels in the kernel-trace run (the autotuner times tile "
               "shapes); their durations are the FETCH_SIZE run's own timestamps.")
+
+
+def summarize_kernels(trace_csv, kernel, grid, top=25):
+    """Markdown of a whole kernel trace: the `top` kernels by total time, then
+    the dispatches of one kernel name with one grid size (the launches of one
+    plan) with their count, mean and median duration."""
+    import csv
+    import statistics
+    rows = list(csv.DictReader(open(trace_csv)))
+    per = {}
+    sel = []
+    total = 0.0
+    for r in rows:
+        name = _base(r["Kernel_Name"])
+        us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+        total += us
+        e = per.setdefault(name, [0, 0.0])
+        e[0] += 1
+        e[1] += us
+        g = int(r["Grid_Size"]) if "Grid_Size" in r else int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        if name == kernel and g == int(grid):
+            sel.append(us)
+    print(f"Total kernel time: {total / 1e3:.1f} ms over {len(rows)} dispatches.\n")
+    print("| kernel | calls | total ms | avg us | % |\n|---|---:|---:|---:|---:|")
+    for name, (cnt, us) in sorted(per.items(), key=lambda kv: -kv[1][1])[:top]:
+        print(f"| `{name}` | {cnt} | {us / 1e3:.2f} | {us / cnt:.2f} | {100 * us / total:.1f} |")
+    if sel:
+        print(f"\n`{kernel}` with grid {grid}: {len(sel)} dispatches, mean {statistics.mean(sel):.1f} us, "
+              f"median {statistics.median(sel):.1f} us.")
+    else:
+        print(f"\nno dispatch of `{kernel}` with grid {grid}")
 
 
 def summarize_gemms(trace_csv, pmc_csv, probe_json, clock_ghz=2.4, simds=1024):
