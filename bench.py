@@ -942,7 +942,8 @@ KERNEL_CLASSES = [
     ("Davidson vector algebra", ("k_multidot", "k_reduce_rows", "k_lincomb", "k_dots_multi", "k_cgs_need",
                                  "k_tri_apply", "k_chol_tri", "k_mgs", "k_pair_dots", "k_renorm_pair",
                                  "k_resid", "k_amax", "k_precond_batched", "k_scale_by_norm", "k_block_dots",
-                                 "k_davidson", "k_ritz_resid", "k_compact_rows")),
+                                 "k_davidson", "k_ritz_resid", "k_ritz_final", "k_cgs_update",
+                                 "k_compact_rows")),
     ("elementwise sigma terms, copies", ("k_hadamard", "k_div", "k_orbital_energy", "k_convert", "k_copy",
                                          "k_axpy", "k_scal", "k_fill", "k_singles", "copyBuffer")),
 ]

@@ -437,7 +437,10 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       (rc = grow(W.amaxpi, sizeof(int64_t) * kRedBlocks * nroot)) || (rc = grow(W.amaxv, sizeof(double) * nroot)) ||
       (rc = grow(W.amaxi, sizeof(int64_t) * nroot)) || (rc = grow(W.wdev, sizeof(double) * nroot)) ||
       (rc = grow(W.idx, sizeof(int32_t) * nroot)) || (rc = grow(W.cgs, 256)) ||
-      (rc = grow(W.tri, sizeof(double) * (2 * kMgsMulti * kMgsMulti + kMgsMulti))))
+      (rc = grow(W.tri, sizeof(double) * (2 * kMgsMulti * kMgsMulti + kMgsMulti))) ||
+      (rc = grow(W.gram, sizeof(double) * kRedBlocks * kMgsMulti * kMgsMulti)) ||
+      (rc = grow(W.qpart, sizeof(double) * kRedBlocks * nroot)) ||
+      (rc = grow(W.ppart, sizeof(double) * kRedBlocks * nroot)) || (rc = grow(W.rstat, sizeof(double) * 3 * nroot)))
     return rc;
   double* B = (double*)W.B.p;
   double* AB = (double*)W.AB.p;
@@ -447,6 +450,8 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   double* dots = (double*)W.dots.p;
   double* part = (double*)W.part.p;
   double* wdev = (double*)W.wdev.p;
+  double* rstat = (double*)W.rstat.p;  // per root: max|e|, |q|^2, |p|^2 of the last Ritz pass
+  double* gram = (double*)W.gram.p;
   const double* diag = (const double*)P->diag.p;
   auto bv = [&](int i) { return B + (size_t)i * n; };
   auto abv = [&](int i) { return AB + (size_t)i * n; };
@@ -525,7 +530,8 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
   int dimIter = nroot, dimFree = dimAbar - dimIter;
   bool dim_exceed = false, converged = false;
   int ndimIter = 0, ndimFree = 0;
-  std::vector<double> wr(dimAbar), wi(dimAbar), V(nd2), sub(nd2), mv(nroot);
+  std::vector<double> wr(dimAbar), wi(dimAbar), V(nd2), sub(nd2), mv(2 * nroot);
+  bool r_unscaled = false;  // R of the last Ritz pass still awaits its renormalisation
   std::vector<int> lowestE(dimAbar);
   std::vector<int32_t> cand;
   SmallEig eig;
@@ -615,21 +621,30 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       if ((rc = upload(W.coef.p, vecNeed.data(), sizeof(double) * dimAbar * nroot))) return rc;
       DPD_HIP_TRY(launch_lincomb2(n, AB, B, n, dimIter, (const double*)W.coef.p, dimAbar, nroot, AQ, R, st));
     }
-    // |q|^2, then in one pass: q and Aq renormalised (:378-386), the residuals
-    // e = Aq - w q and their largest elements (:396-406), and the preconditioned
-    // directions of every root (:417-428), kept for the unconverged ones
-    DPD_HIP_TRY(launch_pair_dots(n, R, n, R, n, nroot, part, dots, st));
+    // in one pass: the residuals e = Aq - w q and their largest elements
+    // (:396-406), the preconditioned directions of every root (:417-428), kept
+    // for the unconverged ones, and |q|^2 and |p|^2.  The renormalisation of q
+    // and Aq (:378-386, by 1/|q| when ||q| - 1| > 1e-10) is applied afterwards:
+    // e and max|e| scale with it (the host scales max|e|), the directions are
+    // normalised anyway, and q (the eigenvector output R) is rescaled on exit
     RootVals wv;
     for (int v = 0; v < nroot; v++) wv.w[v] = eigenVal[v];
-    DPD_HIP_TRY(launch_ritz_resid(n, dots, wv, diag, R, AQ, E, PV, nroot, (double*)W.amaxp.p,
-                                  (int64_t*)W.amaxpi.p, (double*)W.amaxv.p, (int64_t*)W.amaxi.p, st));
+    DPD_HIP_TRY(launch_ritz_resid_part(n, wv, diag, R, AQ, E, PV, nroot, (double*)W.amaxp.p, (int64_t*)W.amaxpi.p,
+                                       (double*)W.qpart.p, (double*)W.ppart.p, rstat, (int64_t*)W.amaxi.p, st));
     bool roots_on_device = false;  // eigenvalues and candidates uploaded (the rare redo path needs them)
-    if ((rc = fetch(mv.data(), W.amaxv.p, sizeof(double) * nroot))) return rc;
+    if ((rc = fetch(mv.data(), rstat, sizeof(double) * 2 * nroot))) return rc;
     int nconv = 0;
     cand.clear();
+    r_unscaled = false;
     for (int iroot = 0; iroot < nroot; iroot++) {
-      res[iroot] = std::fabs(mv[iroot]);
-      if (std::fabs(mv[iroot]) < maxError) nconv++;
+      const double t = std::sqrt(mv[nroot + iroot]);
+      double m = std::fabs(mv[iroot]);
+      if (std::fabs(t - 1.0) > 1e-10) {
+        m /= t;
+        r_unscaled = true;
+      }
+      res[iroot] = m;
+      if (m < maxError) nconv++;
       else cand.push_back(iroot);
     }
     // preconditioned directions of every unconverged root (:417-428)
@@ -715,36 +730,41 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       for (size_t c0 = 0; c0 < nopt; c0 += kMgsMulti) {
         const int nv = (int)std::min<size_t>(kMgsMulti, nopt - c0);
         double* P0 = PV + c0 * n;
-        double* pin = (double*)W.partA.p;
-        double* pout = (double*)W.partB.p;
         const int nb = dimIter + (int)c0;  // old basis and the new vectors of earlier groups
-        // normalise, then project on the (orthonormal) basis twice with block
-        // operations (classical Gram-Schmidt with one re-orthogonalisation)
-        DPD_HIP_TRY(launch_pair_dots(n, P0, n, P0, n, nv, pin, nullptr, st));
-        DPD_HIP_TRY(launch_mgs_multi(n, P0, nv, nullptr, nullptr, pin, 0, pout, st));
-        // the second projection runs only when some direction lost more than
-        // half its squared norm to the first (decided on the device)
+        // normalised and projected on the (orthonormal) basis twice with block
+        // operations (classical Gram-Schmidt with one re-orthogonalisation).
+        // The first projections are taken of the unnormalised directions (whose
+        // squared norms the Ritz pass left in rstat) and the normalisation is
+        // applied with the first update; the second projection runs only when
+        // some direction lost more than half its squared norm to the first
+        // (decided on the device).  Each update leaves the Gram partials of
+        // its result for the Cholesky QR below.
         int* need = (int*)W.cgs.p;
+        const double* pn2 = rstat + 2 * nroot;
+        MgsSrc src;
+        for (int j = 0; j < kMgsMulti; j++) src.src[j] = j < nv ? cand[c0 + j] : 0;
         if (nb > 0) {
-          for (int pass = 0; pass < 2; pass++) {
-            DPD_HIP_TRY(launch_dots_multi(n, P0, nv, B, nb, part, (double*)W.coef.p, st, pass ? need : nullptr));
-            if (pass == 0) DPD_HIP_TRY(launch_cgs_need((const double*)W.coef.p, nv, nb, need, st));
-            DPD_HIP_TRY(launch_lincomb_sub(n, B, nb, (const double*)W.coef.p, nv, P0, st, pass ? need : nullptr));
-          }
+          double* C = (double*)W.coef.p;
+          DPD_HIP_TRY(launch_dots_multi(n, P0, nv, B, nb, part, C, st));
+          DPD_HIP_TRY(launch_cgs_need(C, nv, nb, need, st, pn2, src));
+          DPD_HIP_TRY(launch_cgs_update(n, B, nb, C, nv, P0, nullptr, pn2, src, gram, st));
+          DPD_HIP_TRY(launch_dots_multi(n, P0, nv, B, nb, part, C, st, need));
+          DPD_HIP_TRY(launch_cgs_update(n, B, nb, C, nv, P0, need, nullptr, src, gram, st));
+        } else {
+          DPD_HIP_TRY(launch_cgs_update(n, B, 0, nullptr, nv, P0, nullptr, pn2, src, gram, st));
         }
         // Q = P L^-T with G = P^T P = L L^T; the pivots L_jj are the remaining
         // norms (a direction below 1e-6 zeroes itself and the ones after it).
         // A second pass on Q restores the orthogonality the first loses to
         // the conditioning of G; its squared pivots times the first ones' are
         // the norms the host checks (dots[c0 + j])
-        DPD_HIP_TRY(launch_tri_apply(n, P0, nv, nullptr, nullptr, part, st));
         if (nv == 1) {
-          DPD_HIP_TRY(launch_chol_tri(part, nv, nullptr, T1, nullptr, dots + c0, st));
+          DPD_HIP_TRY(launch_chol_tri(gram, nv, nullptr, T1, nullptr, dots + c0, st));
           DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T1, bv(dimIter + (int)c0), nullptr, st));
         } else {
-          DPD_HIP_TRY(launch_chol_tri(part, nv, nullptr, T1, piv, nullptr, st));
-          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T1, P0, part, st));
-          DPD_HIP_TRY(launch_chol_tri(part, nv, piv, T2, nullptr, dots + c0, st));
+          DPD_HIP_TRY(launch_chol_tri(gram, nv, nullptr, T1, piv, nullptr, st));
+          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T1, P0, gram, st));
+          DPD_HIP_TRY(launch_chol_tri(gram, nv, piv, T2, nullptr, dots + c0, st));
           DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T2, bv(dimIter + (int)c0), nullptr, st));
         }
       }
@@ -822,6 +842,9 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     dimIter += nNewBase;
     iterations++;
   }
+  // the last Ritz pass's q is the eigenvector output: renormalised now where its
+  // norm was off 1 by more than 1e-10 (the same test on the device)
+  if (r_unscaled) DPD_HIP_TRY(launch_renorm_pair(n, rstat + nroot, R, AQ, nroot, st));
   for (int i = 0; i < nroot; i++) w_out[i] = eigenVal[i];
   o->iterations = iterations;
   for (int i = 0; i < nroot && i < 64; i++) o->residual[i] = res[i];

@@ -988,15 +988,10 @@ __global__ __launch_bounds__(256) void k_amax_partial(int64_t n, const double* _
   }
 }
 
-// block b reduces the nb partials of vector b
-__global__ __launch_bounds__(256) void k_amax_final(int nb, const double* pv, const int64_t* pi,
-                                                    double* ov, int64_t* oi) {
+// one block: the largest |pv[i]| (first index on ties) of nb partials, in a fixed order
+__device__ inline void k_amax_final_body(int nb, const double* pv, const int64_t* pi, double* ov, int64_t* oi) {
   __shared__ double sv[256];
   __shared__ int64_t si[256];
-  pv += (int64_t)blockIdx.x * nb;
-  pi += (int64_t)blockIdx.x * nb;
-  ov += blockIdx.x;
-  oi += blockIdx.x;
   double v = 0.0;
   int64_t idx = INT64_MAX;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) amax_merge(v, idx, pv[i], pi[i]);
@@ -1017,6 +1012,13 @@ __global__ __launch_bounds__(256) void k_amax_final(int nb, const double* pv, co
     *ov = sv[0];
     *oi = si[0];
   }
+}
+
+// block b reduces the nb partials of vector b
+__global__ __launch_bounds__(256) void k_amax_final(int nb, const double* pv, const int64_t* pi,
+                                                    double* ov, int64_t* oi) {
+  k_amax_final_body(nb, pv + (int64_t)blockIdx.x * nb, pi + (int64_t)blockIdx.x * nb, ov + blockIdx.x,
+                    oi + blockIdx.x);
 }
 
 hipError_t launch_amax(int64_t n, const double* x, double* pv, int64_t* pi, double* ov,
@@ -1424,14 +1426,6 @@ hipError_t launch_block_dots(int64_t n, const double* X, int64_t xs, int nx, con
   return hipGetLastError();
 }
 
-hipError_t launch_pair_dots(int64_t n, const double* x, int64_t xs, const double* y, int64_t ys, int nv,
-                            double* partial, double* out, hipStream_t s) {
-  if (nv == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pair_dots, dim3(kRedBlocks, nv), dim3(256), 0, s, n, x, xs, y, ys, partial);
-  if (out) hipLaunchKernelGGL(k_reduce_rows, dim3(nv), dim3(256), 0, s, partial, kRedBlocks, out);
-  return hipGetLastError();
-}
-
 // x_v *= 1/t and y_v *= 1/t with t = sqrt(nrm2[v]) when |t - 1| > 1e-10
 // (lib_cc/mod_davidson.f90:378-386: the Ritz vector is renormalised only then)
 __global__ void k_renorm_pair(int64_t n, const double* nrm2, double* __restrict__ x, double* __restrict__ y) {
@@ -1499,41 +1493,44 @@ __global__ __launch_bounds__(256) void k_resid_amax_partial(int64_t n, const dou
   }
 }
 
-// Ritz vector, residual and preconditioned direction of every root in one pass
-// (lib_cc/mod_davidson.f90:378-386, :396-406, :417-428): q = R, Aq = AQ scaled
-// by 1/|q| when ||q| - 1| > 1e-10 (q written back), e = Aq - w q into E,
-// p = e / (w - d) (|w - d| clamped to 1e-4) into P, and the largest |e| per root.
-// The same operations as k_renorm_pair + k_resid_amax_partial + k_precond_batched.
-__global__ __launch_bounds__(256) void k_ritz_resid(int64_t n, const double* nrm2, RootVals w,
-                                                    const double* __restrict__ d, double* __restrict__ q,
-                                                    const double* __restrict__ aq, double* __restrict__ e,
-                                                    double* __restrict__ p, double* pv, int64_t* pi) {
+// Residual and preconditioned direction of every root in one pass
+// (lib_cc/mod_davidson.f90:396-406, :417-428): e = Aq - w q into E,
+// p = e / (w - d) (|w - d| clamped to 1e-4) into P, the largest |e| per root,
+// and the partial sums of |q|^2 and |p|^2 (kRedBlocks per root).  q and Aq
+// are taken as they are: their renormalisation (:378-386) is applied by the
+// caller afterwards (e, p and max|e| are linear in it).
+__global__ __launch_bounds__(256) void k_ritz_resid(int64_t n, RootVals w, const double* __restrict__ d,
+                                                    const double* __restrict__ q, const double* __restrict__ aq,
+                                                    double* __restrict__ e, double* __restrict__ p, double* pv,
+                                                    int64_t* pi, double* qpart, double* ppart) {
   __shared__ double sv[256];
   __shared__ int64_t si[256];
+  __shared__ double red[4];
   const int v = blockIdx.y;
   const double wv = w.w[v];
-  const double t = sqrt(nrm2[v]);
-  const bool scale = fabs(t - 1.0) > 1e-10;
-  const double f = 1.0 / t;
-  double* qv = q + (int64_t)v * n;
+  const double* qv = q + (int64_t)v * n;
   const double* aqv = aq + (int64_t)v * n;
   double* ev = e + (int64_t)v * n;
   double* pvv = p + (int64_t)v * n;
-  double best = 0.0;
+  double best = 0.0, sq = 0.0, sp = 0.0;
   int64_t bi = INT64_MAX;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double x = qv[i], y = aqv[i];
-    if (scale) {
-      x *= f;
-      y *= f;
-      qv[i] = x;
-    }
+    const double x = qv[i], y = aqv[i];
     const double r = y + (-wv) * x;
     ev[i] = r;
     double tt = wv - d[i];
     if (fabs(tt) < 1.0e-4) tt = (tt < 0.0) ? -1.0e-4 : 1.0e-4;
-    pvv[i] = r / tt;
+    const double pp = r / tt;
+    pvv[i] = pp;
+    sq += x * x;
+    sp += pp * pp;
     amax_merge(best, bi, r, i);
+  }
+  sq = block_sum_256(sq, red);
+  sp = block_sum_256(sp, red);
+  if (threadIdx.x == 0) {
+    qpart[(int64_t)v * gridDim.x + blockIdx.x] = sq;
+    ppart[(int64_t)v * gridDim.x + blockIdx.x] = sp;
   }
   sv[threadIdx.x] = best;
   si[threadIdx.x] = bi;
@@ -1554,13 +1551,33 @@ __global__ __launch_bounds__(256) void k_ritz_resid(int64_t n, const double* nrm
   }
 }
 
-hipError_t launch_ritz_resid(int64_t n, const double* nrm2, const RootVals& w, const double* d, double* q,
-                             const double* aq, double* e, double* p, int nv, double* pv, int64_t* pi, double* ov,
-                             int64_t* oi, hipStream_t s) {
+// per root v (one block each): the largest |e| with its first index, and the
+// sums of the |q|^2 and |p|^2 partials, all in a fixed order, into
+// stat[v], stat[nv + v], stat[2 nv + v] (and its index into oi[v])
+__global__ __launch_bounds__(256) void k_ritz_final(const double* pv, const int64_t* pi, const double* qpart,
+                                                    const double* ppart, double* stat, int64_t* oi) {
+  __shared__ double red[4];
+  const int v = blockIdx.x, nv = gridDim.x;
+  const double sq = block_sum_256(qpart[(int64_t)v * kRedBlocks + threadIdx.x] +
+                                      qpart[(int64_t)v * kRedBlocks + threadIdx.x + 256], red);
+  const double sp = block_sum_256(ppart[(int64_t)v * kRedBlocks + threadIdx.x] +
+                                      ppart[(int64_t)v * kRedBlocks + threadIdx.x + 256], red);
+  if (threadIdx.x == 0) {
+    stat[nv + v] = sq;
+    stat[2 * nv + v] = sp;
+  }
+  k_amax_final_body(kRedBlocks, pv + (int64_t)v * kRedBlocks, pi + (int64_t)v * kRedBlocks, stat + v, oi + v);
+}
+
+hipError_t launch_ritz_resid_part(int64_t n, const RootVals& w, const double* d, const double* q, const double* aq,
+                                  double* e, double* p, int nv, double* pv, int64_t* pi, double* qpart,
+                                  double* ppart, double* stat, int64_t* oi, hipStream_t s) {
   if (nv == 0) return hipSuccess;
   if (nv > kRootVals) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_ritz_resid, dim3(kRedBlocks, nv), dim3(256), 0, s, n, nrm2, w, d, q, aq, e, p, pv, pi);
-  hipLaunchKernelGGL(k_amax_final, dim3(nv), dim3(256), 0, s, kRedBlocks, pv, pi, ov, oi);
+  static_assert(kRedBlocks == 512, "k_ritz_final sums two partials per thread");
+  hipLaunchKernelGGL(k_ritz_resid, dim3(kRedBlocks, nv), dim3(256), 0, s, n, w, d, q, aq, e, p, pv, pi, qpart,
+                     ppart);
+  hipLaunchKernelGGL(k_ritz_final, dim3(nv), dim3(256), 0, s, pv, pi, qpart, ppart, stat, oi);
   return hipGetLastError();
 }
 
@@ -1593,57 +1610,6 @@ __global__ __launch_bounds__(256) void k_mgs_step(int64_t n, double* __restrict_
 hipError_t launch_mgs_step(int64_t n, double* p, const double* bprev, const double* bnext, const double* part_in,
                            int mode, double* part_out, hipStream_t s) {
   hipLaunchKernelGGL(k_mgs_step, dim3(kRedBlocks), dim3(256), 0, s, n, p, bprev, bnext, part_in, mode, part_out);
-  return hipGetLastError();
-}
-
-// the same step for nv <= kMgsMulti vectors p_j = P + j*n against one basis
-// vector: every block reduces all nv partial rows, updates its chunk of every
-// p_j and writes the partials of p_j . b_next (or p_j . p_j)
-__global__ __launch_bounds__(256) void k_mgs_multi(int64_t n, double* __restrict__ P, int nv,
-                                                   const double* __restrict__ bprev,
-                                                   const double* __restrict__ bnext, const double* part_in,
-                                                   int mode, double* part_out) {
-  __shared__ double red[4];
-  double f[kMgsMulti];
-#pragma unroll
-  for (int j = 0; j < kMgsMulti; j++) {
-    f[j] = 0.0;
-    if (j < nv) {
-      const double c = reduce_parts(part_in + (int64_t)j * kRedBlocks, red);
-      f[j] = (mode == 0) ? 1.0 / sqrt(c) : c;
-    }
-  }
-  double s[kMgsMulti];
-#pragma unroll
-  for (int j = 0; j < kMgsMulti; j++) s[j] = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double bp = mode == 0 ? 0.0 : bprev[i];
-    const double bn = bnext ? bnext[i] : 0.0;
-#pragma unroll
-    for (int j = 0; j < kMgsMulti; j++) {
-      if (j < nv) {
-        double x = P[(int64_t)j * n + i];
-        if (mode == 0) x = x * f[j];
-        else x = x + (-f[j]) * bp;
-        P[(int64_t)j * n + i] = x;
-        s[j] += x * (bnext ? bn : x);
-      }
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < kMgsMulti; j++) {
-    if (j < nv) {
-      const double v = block_sum_256(s[j], red);
-      if (threadIdx.x == 0) part_out[(int64_t)j * kRedBlocks + blockIdx.x] = v;
-    }
-  }
-}
-
-hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, const double* bnext,
-                            const double* part_in, int mode, double* part_out, hipStream_t s) {
-  if (nv <= 0 || nv > kMgsMulti) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_mgs_multi, dim3(kRedBlocks), dim3(256), 0, s, n, P, nv, bprev, bnext, part_in, mode,
-                     part_out);
   return hipGetLastError();
 }
 
@@ -1722,15 +1688,31 @@ hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B
   return hipGetLastError();
 }
 
-// P_j -= sum_k C[j + k*nv] B_k for j < nv <= kMgsMulti (k ascending)
-__global__ void k_lincomb_sub(int64_t n, const double* __restrict__ B, int nb, const double* __restrict__ C, int nv,
-                              double* __restrict__ P, const int* __restrict__ need) {
+// one projection of classical Gram-Schmidt for nv <= kMgsMulti directions:
+// P_j <- f_j (P_j - sum_k C[j + k*nv] B_k) (k ascending) with f_j =
+// 1/sqrt(pn2[src[j]]) when pn2 is given (C then holds the projections of the
+// unnormalised P_j: the result is the projection of P_j / |P_j|), else 1; and
+// the partials of the Gram matrix of the new P_j (rows j + k*nv, j <= k, as
+// k_tri_apply writes them).  need given and *need == 0: nothing is done.
+// Grid: kRedBlocks blocks of 256.
+__global__ __launch_bounds__(256) void k_cgs_update(int64_t n, const double* __restrict__ B, int nb,
+                                                    const double* __restrict__ C, int nv, double* __restrict__ P,
+                                                    const int* __restrict__ need, const double* pn2, MgsSrc src,
+                                                    double* __restrict__ gram) {
+  __shared__ double red[4][kMgsMulti * kMgsMulti];
   if (need && *need == 0) return;
-  // rows past nv repeat the last one (computed, never stored): no branches,
-  // so the loads of four basis vectors are in flight together
+  double f[kMgsMulti];
   int jc[kMgsMulti];
 #pragma unroll
-  for (int j = 0; j < kMgsMulti; j++) jc[j] = min(j, nv - 1);
+  for (int j = 0; j < kMgsMulti; j++) {
+    f[j] = (pn2 && j < nv) ? 1.0 / sqrt(pn2[src.src[j]]) : 1.0;
+    jc[j] = min(j, nv - 1);  // rows past nv repeat the last one (computed, never stored)
+  }
+  double g[kMgsMulti][kMgsMulti];
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++)
+#pragma unroll
+    for (int k = 0; k < kMgsMulti; k++) g[j][k] = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double s[kMgsMulti];
 #pragma unroll
@@ -1750,39 +1732,78 @@ __global__ void k_lincomb_sub(int64_t n, const double* __restrict__ B, int nb, c
 #pragma unroll
       for (int j = 0; j < kMgsMulti; j++) s[j] += C[jc[j] + (int64_t)k * nv] * b;
     }
+    double x[kMgsMulti];
 #pragma unroll
-    for (int j = 0; j < kMgsMulti; j++)
-      if (j < nv) P[(int64_t)j * n + i] -= s[j];
+    for (int j = 0; j < kMgsMulti; j++) {
+      x[j] = 0.0;
+      if (j < nv) {
+        x[j] = (P[(int64_t)j * n + i] - s[j]) * f[j];
+        P[(int64_t)j * n + i] = x[j];
+      }
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < kMgsMulti; k2++)
+#pragma unroll
+      for (int j = 0; j <= k2; j++) g[j][k2] += x[j] * x[k2];
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < kMgsMulti; k++)
+#pragma unroll
+    for (int j = 0; j <= k; j++)
+      if (k < nv) {
+        const double v = wave_sum(g[j][k]);
+        if (lane == 0) red[wave][j + k * kMgsMulti] = v;
+      }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kMgsMulti * kMgsMulti; t += blockDim.x) {
+    const int j = t % kMgsMulti, k = t / kMgsMulti;
+    if (j <= k && k < nv)
+      gram[((int64_t)j + (int64_t)k * nv) * kRedBlocks + blockIdx.x] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
   }
 }
 
-hipError_t launch_lincomb_sub(int64_t n, const double* B, int nb, const double* C, int nv, double* P, hipStream_t s,
-                              const int* need) {
-  if (nv <= 0 || nb <= 0) return hipSuccess;
-  if (nv > kMgsMulti) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_lincomb_sub, dim3(grid_for(n)), dim3(256), 0, s, n, B, nb, C, nv, P, need);
+hipError_t launch_cgs_update(int64_t n, const double* B, int nb, const double* C, int nv, double* P,
+                             const int* need, const double* pn2, const MgsSrc& src, double* gram, hipStream_t s) {
+  if (nv <= 0 || nv > kMgsMulti || nb < 0 || (nb > 0 && !C) || !gram) return hipErrorInvalidValue;
+  for (int j = 0; pn2 && j < nv; j++)
+    if (src.src[j] < 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_cgs_update, dim3(kRedBlocks), dim3(256), 0, s, n, B, nb, C, nv, P, need, pn2, src, gram);
   return hipGetLastError();
 }
 
 // re-orthogonalisation test of classical Gram-Schmidt ("twice is enough",
 // Kahan-Parlett): vectors of unit norm whose projections C (nv x nb, [j + k*nv])
 // on an orthonormal basis all have sum_k c_jk^2 <= 1/2 keep at least 1/sqrt(2)
-// of their norm, and one projection is as accurate as two; need = 0 then
-__global__ void k_cgs_need(const double* __restrict__ C, int nv, int nb, int* need) {
+// of their norm, and one projection is as accurate as two; need = 0 then.
+// pn2 given: C are the projections of unnormalised vectors P_j with
+// |P_j|^2 = pn2[src[j]], and the test is sum_k c_jk^2 <= |P_j|^2 / 2.
+__global__ void k_cgs_need(const double* __restrict__ C, int nv, int nb, int* need, const double* pn2,
+                           MgsSrc src) {
   __shared__ int any;
   if (threadIdx.x == 0) any = 0;
   __syncthreads();
-  for (int j = threadIdx.x; j < nv; j += blockDim.x) {
+  if (threadIdx.x < nv) {
+    const int j = threadIdx.x;
+    int r = 0;
+#pragma unroll
+    for (int q = 0; q < kMgsMulti; q++)
+      if (q == j) r = src.src[q];
+    const double lim = pn2 ? 0.5 * pn2[r] : 0.5;
     double s = 0.0;
     for (int k = 0; k < nb; k++) s += C[j + (int64_t)k * nv] * C[j + (int64_t)k * nv];
-    if (!(s <= 0.5)) atomicOr(&any, 1);
+    if (!(s <= lim)) atomicOr(&any, 1);
   }
   __syncthreads();
   if (threadIdx.x == 0) *need = any;
 }
 
-hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream_t s) {
-  hipLaunchKernelGGL(k_cgs_need, dim3(1), dim3(64), 0, s, C, nv, nb, need);
+hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream_t s, const double* pn2,
+                           const MgsSrc& src) {
+  if (nv <= 0 || nv > kMgsMulti) return hipErrorInvalidValue;
+  for (int j = 0; pn2 && j < nv; j++)
+    if (src.src[j] < 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_cgs_need, dim3(1), dim3(64), 0, s, C, nv, nb, need, pn2, src);
   return hipGetLastError();
 }
 

@@ -310,10 +310,6 @@ hipError_t launch_div_shift(int64_t n, const double* num, const double* den, dou
 
 namespace dpd {
 // ---- batched Davidson kernels --------------------------------------------
-// out[v] = dot(x + v*xs, y + v*ys) for v < nv   (partial: kRedBlocks*nv; out may be
-// null: partials only)
-hipError_t launch_pair_dots(int64_t n, const double* x, int64_t xs, const double* y, int64_t ys, int nv,
-                            double* partial, double* out, hipStream_t s);
 // out[xi + yj*nx] = X_xi . Y_yj  (partial: kRedBlocks*nx*ny)
 hipError_t launch_block_dots(int64_t n, const double* X, int64_t xs, int nx, const double* Y, int64_t ys, int ny,
                              double* partial, double* out, hipStream_t s);
@@ -333,11 +329,15 @@ struct RowList {
   int32_t src[kRootVals];
   int n;
 };
-// renormalised Ritz vectors, residuals (E), preconditioned directions of every
-// root (P) and the largest |residual| per root (ov/oi) in one pass (nv <= kRootVals)
-hipError_t launch_ritz_resid(int64_t n, const double* nrm2, const RootVals& w, const double* d, double* q,
-                             const double* aq, double* e, double* p, int nv, double* pv, int64_t* pi, double* ov,
-                             int64_t* oi, hipStream_t s);
+// residuals e_v = Aq_v - w_v q_v (E) and preconditioned directions (P) of every
+// root in one pass (nv <= kRootVals), and in one more launch stat[v] = max|e_v|
+// (first index oi[v]), stat[nv + v] = |q_v|^2 and stat[2 nv + v] = |p_v|^2, each
+// summed in a fixed order (qpart, ppart: kRedBlocks * nv partials).  q and Aq
+// are used as they are: e, p and max|e| scale with 1/|q| when the caller
+// renormalises q.
+hipError_t launch_ritz_resid_part(int64_t n, const RootVals& w, const double* d, const double* q, const double* aq,
+                                  double* e, double* p, int nv, double* pv, int64_t* pi, double* qpart,
+                                  double* ppart, double* stat, int64_t* oi, hipStream_t s);
 // launch_lincomb of two sets (X -> out and X2 -> out2) with the same coefficients
 hipError_t launch_lincomb2(int64_t n, const double* X, const double* X2, int64_t xs, int nk, const double* coef,
                            int ldc, int nout, double* out, double* out2, hipStream_t s);
@@ -357,20 +357,28 @@ hipError_t launch_resid_amax(int64_t n, const double* aq, const double* q, const
 //   then part_out[block] = dot(p, b_next) (b_next == nullptr: dot(p, p))
 hipError_t launch_mgs_step(int64_t n, double* p, const double* b_prev, const double* b_next, const double* part_in,
                            int mode, double* part_out, hipStream_t s);
-// the same step for nv <= kMgsMulti vectors P + j*n at once (partials: kRedBlocks per vector)
+// directions handled together by the block kernels below
 constexpr int kMgsMulti = 8;
-hipError_t launch_mgs_multi(int64_t n, double* P, int nv, const double* bprev, const double* bnext,
-                            const double* part_in, int mode, double* part_out, hipStream_t s);
 // out[j + k*nv] = P_j . B_k (partial: kRedBlocks * nv * nb), nv <= kMgsMulti
 hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
                              double* out, hipStream_t s,
                              const int* need = nullptr);
-// P_j -= sum_k C[j + k*nv] B_k, nv <= kMgsMulti
-hipError_t launch_lincomb_sub(int64_t n, const double* B, int nb, const double* C, int nv, double* P, hipStream_t s,
-                              const int* need = nullptr);
+// the root of each of up to kMgsMulti directions (an index into per-root arrays)
+struct MgsSrc {
+  int32_t src[kMgsMulti];
+};
+// one classical Gram-Schmidt projection: P_j <- f_j (P_j - sum_k C[j + k*nv] B_k)
+// with f_j = 1/sqrt(pn2[src[j]]) (pn2 given; C the projections of the
+// unnormalised P_j) or 1, and the Gram partials of the result (rows j + k*nv,
+// j <= k, kRedBlocks each, as launch_tri_apply writes them); need given and
+// *need == 0: nothing done.  nb may be 0 (normalisation only).
+hipError_t launch_cgs_update(int64_t n, const double* B, int nb, const double* C, int nv, double* P,
+                             const int* need, const double* pn2, const MgsSrc& src, double* gram, hipStream_t s);
 // need = 1 when any of nv unit vectors lost more than half its squared norm to
-// its projections C on an orthonormal basis (a second projection is needed)
-hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream_t s);
+// its projections C on an orthonormal basis (a second projection is needed);
+// pn2 given: the vectors P_j are unnormalised, |P_j|^2 = pn2[src[j]]
+hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream_t s,
+                           const double* pn2 = nullptr, const MgsSrc& src = MgsSrc{});
 // Y_j = sum_{i<=j} T[i + j*nv] X_i (T null: identity; Y null: not stored; Y may
 // be X) and, with partial given, the Gram partials of Y (rows j + k*nv, j <= k)
 hipError_t launch_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y, double* partial,

@@ -12,7 +12,7 @@ namespace dpd {
 // reduction scratch and the pinned host staging of the small transfers
 struct DavWork {
   DevBuf B, AB, NB, NAB, AQ, E, P, coef, part, partA, partB, partC, partD, dots, amaxp, amaxpi, amaxv, amaxi, wdev, tri,
-      idx, cgs;
+      idx, cgs, gram, qpart, ppart, rstat;
   void* pin_out = nullptr;
   void* pin_in = nullptr;
   size_t pin_bytes = 0;
