@@ -655,6 +655,15 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     // per group of up to kMgsMulti new vectors); written to dots from `off` on
     auto abar_dots = [&](int nnew, size_t& off, std::vector<size_t>& off1, std::vector<size_t>& off2) -> int {
       const int nt = dimIter + nnew;
+      if (nnew <= kMgsMulti) {
+        // one launch for both: the second block lands right after the first
+        DPD_HIP_TRY(launch_dots_pair(n, nnew, bv(dimIter), AB, dimIter, abv(dimIter), B, nt, part, dots + off, st));
+        off1.push_back(off);
+        off += (size_t)nnew * dimIter;
+        off2.push_back(off);
+        off += (size_t)nnew * nt;
+        return DPD_OK;
+      }
       for (int c0 = 0; c0 < nnew; c0 += kMgsMulti) {
         const int nv = std::min(kMgsMulti, nnew - c0);
         DPD_HIP_TRY(launch_dots_multi(n, bv(dimIter + c0), nv, AB, dimIter, part, dots + off, st));

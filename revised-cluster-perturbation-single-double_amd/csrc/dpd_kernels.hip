@@ -1617,14 +1617,26 @@ hipError_t launch_mgs_step(int64_t n, double* p, const double* bprev, const doub
 // basis vectors B_k0.. per block (stride n); grid (kRedBlocks, ceil(nb/KW)).  Each
 // basis vector is read once; the P vectors are re-read by every block column,
 // so KW is as wide as the registers of NV accumulator rows allow.  Partials
-// are reduced in a fixed order.
+// are reduced in a fixed order.  Block columns from cols1 on compute a second,
+// independent set of the same kind (P2, B2, nb2 into partial2): two sets in
+// one launch.
 template <int NV, int KW>
 __global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __restrict__ P,
                                                     const double* __restrict__ B, int nb, double* partial,
-                                                    const int* __restrict__ need) {
+                                                    const int* __restrict__ need, int cols1,
+                                                    const double* __restrict__ P2, const double* __restrict__ B2,
+                                                    int nb2, double* partial2) {
   __shared__ double red[4][NV * KW];
   if (need && *need == 0) return;
-  const int k0 = blockIdx.y * KW;
+  int by = blockIdx.y;
+  if (by >= cols1) {
+    P = P2;
+    B = B2;
+    nb = nb2;
+    partial = partial2;
+    by -= cols1;
+  }
+  const int k0 = by * KW;
   const int nk = min(KW, nb - k0);
   double acc[NV][KW];
 #pragma unroll
@@ -1663,28 +1675,43 @@ __global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __r
 
 template <int NV>
 static void launch_dots_nv(int64_t n, const double* P, const double* B, int nb, double* partial, hipStream_t s,
-                           const int* need) {
+                           const int* need, const double* P2, const double* B2, int nb2) {
   constexpr int KW = NV <= 5 ? 8 : 4;
-  hipLaunchKernelGGL((k_dots_multi<NV, KW>), dim3(kRedBlocks, (nb + KW - 1) / KW), dim3(256), 0, s, n, P, B, nb,
-                     partial, need);
+  const int cols1 = (nb + KW - 1) / KW, cols2 = (nb2 + KW - 1) / KW;
+  hipLaunchKernelGGL((k_dots_multi<NV, KW>), dim3(kRedBlocks, cols1 + cols2), dim3(256), 0, s, n, P, B, nb, partial,
+                     need, cols1, P2, B2, nb2, partial + (int64_t)NV * nb * kRedBlocks);
+}
+
+static void launch_dots_any(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
+                            hipStream_t s, const int* need, const double* P2, const double* B2, int nb2) {
+  static_assert(kMgsMulti == 8, "one instantiation per vector count");
+  switch (nv) {
+    case 1: launch_dots_nv<1>(n, P, B, nb, partial, s, need, P2, B2, nb2); break;
+    case 2: launch_dots_nv<2>(n, P, B, nb, partial, s, need, P2, B2, nb2); break;
+    case 3: launch_dots_nv<3>(n, P, B, nb, partial, s, need, P2, B2, nb2); break;
+    case 4: launch_dots_nv<4>(n, P, B, nb, partial, s, need, P2, B2, nb2); break;
+    case 5: launch_dots_nv<5>(n, P, B, nb, partial, s, need, P2, B2, nb2); break;
+    case 6: launch_dots_nv<6>(n, P, B, nb, partial, s, need, P2, B2, nb2); break;
+    case 7: launch_dots_nv<7>(n, P, B, nb, partial, s, need, P2, B2, nb2); break;
+    default: launch_dots_nv<8>(n, P, B, nb, partial, s, need, P2, B2, nb2); break;
+  }
 }
 
 hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
                              double* out, hipStream_t s, const int* need) {
   if (nv <= 0 || nb <= 0) return hipSuccess;
   if (nv > kMgsMulti) return hipErrorInvalidValue;
-  static_assert(kMgsMulti == 8, "one instantiation per vector count");
-  switch (nv) {
-    case 1: launch_dots_nv<1>(n, P, B, nb, partial, s, need); break;
-    case 2: launch_dots_nv<2>(n, P, B, nb, partial, s, need); break;
-    case 3: launch_dots_nv<3>(n, P, B, nb, partial, s, need); break;
-    case 4: launch_dots_nv<4>(n, P, B, nb, partial, s, need); break;
-    case 5: launch_dots_nv<5>(n, P, B, nb, partial, s, need); break;
-    case 6: launch_dots_nv<6>(n, P, B, nb, partial, s, need); break;
-    case 7: launch_dots_nv<7>(n, P, B, nb, partial, s, need); break;
-    default: launch_dots_nv<8>(n, P, B, nb, partial, s, need); break;
-  }
+  launch_dots_any(n, P, nv, B, nb, partial, s, need, nullptr, nullptr, 0);
   hipLaunchKernelGGL(k_reduce_rows, dim3(nv * nb), dim3(256), 0, s, partial, kRedBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_dots_pair(int64_t n, int nv, const double* P1, const double* B1, int nb1, const double* P2,
+                            const double* B2, int nb2, double* partial, double* out, hipStream_t s) {
+  if (nv <= 0 || nv > kMgsMulti || nb1 < 0 || nb2 < 0) return hipErrorInvalidValue;
+  if (nb1 + nb2 == 0) return hipSuccess;
+  launch_dots_any(n, P1, nv, B1, nb1, partial, s, nullptr, P2, B2, nb2);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nv * (nb1 + nb2)), dim3(256), 0, s, partial, kRedBlocks, out);
   return hipGetLastError();
 }
 

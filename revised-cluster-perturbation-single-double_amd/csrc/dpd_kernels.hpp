@@ -363,6 +363,11 @@ constexpr int kMgsMulti = 8;
 hipError_t launch_dots_multi(int64_t n, const double* P, int nv, const double* B, int nb, double* partial,
                              double* out, hipStream_t s,
                              const int* need = nullptr);
+// two such sets in one launch and one reduction: out[j + k*nv] = P1_j . B1_k
+// (k < nb1), then out[nv*nb1 + j + k*nv] = P2_j . B2_k (k < nb2); partial:
+// kRedBlocks * nv * (nb1 + nb2)
+hipError_t launch_dots_pair(int64_t n, int nv, const double* P1, const double* B1, int nb1, const double* P2,
+                            const double* B2, int nb2, double* partial, double* out, hipStream_t s);
 // the root of each of up to kMgsMulti directions (an index into per-root arrays)
 struct MgsSrc {
   int32_t src[kMgsMulti];
