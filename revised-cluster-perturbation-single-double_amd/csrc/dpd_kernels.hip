@@ -1673,12 +1673,111 @@ __global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __r
   }
 }
 
+typedef double dots_d4_t __attribute__((ext_vector_type(4)));
+
+// The same dots on the FP64 matrix cores: a block column covers kDotsMB = 32
+// basis vectors (two 16-row MFMA tiles), so the P vectors are read once per 32
+// basis vectors instead of once per KW.  Per step a block stages a 64-element
+// slice of its 32 basis vectors and of the NV P vectors in LDS (loaded a row
+// per instruction, 512 contiguous bytes), and each wave multiplies 16 of the 64
+// elements with v_mfma_f64_16x16x4f64: A = basis (row m = lane & 15, k =
+// lane >> 4), B = P (k = lane >> 4, column n = lane & 15, zero for n >= NV), the
+// 16 x 16 result in 4 registers a lane (row (lane >> 4) + 4 r, column lane & 15).
+// LDS rows are 66 doubles apart, which makes the fragment reads and the row
+// writes free of bank conflicts; the staging is double buffered with one barrier
+// per step.  The four waves' results are added in a fixed order; the partials
+// go out as k_dots_multi writes them.  Grid (kRedBlocks, columns of set 1 + set 2).
+constexpr int kDotsMB = 32;
+constexpr int kDotsTE = 64;
+constexpr int kDotsLd = kDotsTE + 2;
+template <int NV>
+__global__ __launch_bounds__(256) void k_dots_mfma(int64_t n, const double* __restrict__ P,
+                                                   const double* __restrict__ B, int nb, double* partial,
+                                                   const int* __restrict__ need, int cols1,
+                                                   const double* __restrict__ P2, const double* __restrict__ B2,
+                                                   int nb2, double* partial2) {
+  static_assert(NV >= 1 && NV <= 8, "up to 8 vectors (rows 0-7 of the B fragment)");
+  __shared__ double bs[2][kDotsMB * kDotsLd];
+  __shared__ double ps[2][NV * kDotsLd];
+  if (need && *need == 0) return;
+  int by = blockIdx.y;
+  if (by >= cols1) {
+    P = P2;
+    B = B2;
+    nb = nb2;
+    partial = partial2;
+    by -= cols1;
+  }
+  const int m0 = by * kDotsMB;
+  const int nm = min(kDotsMB, nb - m0);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = lane & 15, g = lane >> 4;
+  // staging: wave w loads basis rows w + 4 i (i < 8) and P rows w, w + 4
+  double rb[kDotsMB / 4], rp[2];
+  const int64_t ntiles = (n + kDotsTE - 1) / kDotsTE;
+  auto load = [&](int64_t tile) {
+    const int64_t e = tile * kDotsTE + lane;
+    const bool in = e < n;
+#pragma unroll
+    for (int i = 0; i < kDotsMB / 4; i++) {
+      const int m = wave + 4 * i;
+      rb[i] = (in && m < nm) ? B[(int64_t)(m0 + m) * n + e] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int j = wave + 4 * i;
+      rp[i] = (in && j < NV) ? P[(int64_t)j * n + e] : 0.0;
+    }
+  };
+  dots_d4_t acc[2];
+#pragma unroll
+  for (int t = 0; t < 2; t++) acc[t] = dots_d4_t{0.0, 0.0, 0.0, 0.0};
+  int64_t tile = blockIdx.x;
+  if (tile < ntiles) load(tile);
+  for (int buf = 0; tile < ntiles; tile += gridDim.x, buf ^= 1) {
+#pragma unroll
+    for (int i = 0; i < kDotsMB / 4; i++) bs[buf][(wave + 4 * i) * kDotsLd + lane] = rb[i];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+      if (wave + 4 * i < NV) ps[buf][(wave + 4 * i) * kDotsLd + lane] = rp[i];
+    __syncthreads();
+    if (tile + gridDim.x < ntiles) load(tile + gridDim.x);
+    // this wave's 16 elements: 4 k-steps of 4
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+      const int k = 16 * wave + 4 * s4 + g;
+      const double b = c < NV ? ps[buf][c * kDotsLd + k] : 0.0;
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const double a = bs[buf][(16 * t + c) * kDotsLd + k];
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  // the four waves' 32 x NV results, added in a fixed order (staging buffers reused)
+  __syncthreads();
+  double* red = &bs[0][0];  // [wave][t][r][lane]: 4 x 2 x 4 x 64 doubles
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) red[((wave * 2 + t) * 4 + r) * 64 + lane] = acc[t][r];
+  __syncthreads();
+  for (int q = threadIdx.x; q < 2 * 4 * 64; q += blockDim.x) {
+    const int t = q / 256, r = (q / 64) % 4, l = q % 64;
+    const int m = 16 * t + (l >> 4) + 4 * r, j = l & 15;
+    if (j < NV && m < nm) {
+      const double v = (red[((0 * 2 + t) * 4 + r) * 64 + l] + red[((1 * 2 + t) * 4 + r) * 64 + l]) +
+                       (red[((2 * 2 + t) * 4 + r) * 64 + l] + red[((3 * 2 + t) * 4 + r) * 64 + l]);
+      partial[((int64_t)j + (int64_t)(m0 + m) * NV) * kRedBlocks + blockIdx.x] = v;
+    }
+  }
+}
+
 template <int NV>
 static void launch_dots_nv(int64_t n, const double* P, const double* B, int nb, double* partial, hipStream_t s,
                            const int* need, const double* P2, const double* B2, int nb2) {
-  constexpr int KW = NV <= 5 ? 8 : 4;
-  const int cols1 = (nb + KW - 1) / KW, cols2 = (nb2 + KW - 1) / KW;
-  hipLaunchKernelGGL((k_dots_multi<NV, KW>), dim3(kRedBlocks, cols1 + cols2), dim3(256), 0, s, n, P, B, nb, partial,
+  const int cols1 = (nb + kDotsMB - 1) / kDotsMB, cols2 = (nb2 + kDotsMB - 1) / kDotsMB;
+  hipLaunchKernelGGL((k_dots_mfma<NV>), dim3(kRedBlocks, cols1 + cols2), dim3(256), 0, s, n, P, B, nb, partial,
                      need, cols1, P2, B2, nb2, partial + (int64_t)NV * nb * kRedBlocks);
 }
 
