@@ -1613,71 +1613,14 @@ hipError_t launch_mgs_step(int64_t n, double* p, const double* bprev, const doub
   return hipGetLastError();
 }
 
-// C[j + k*nv] = P_j . B_k for NV (<= kMgsMulti) vectors P (stride n) against KW
-// basis vectors B_k0.. per block (stride n); grid (kRedBlocks, ceil(nb/KW)).  Each
-// basis vector is read once; the P vectors are re-read by every block column,
-// so KW is as wide as the registers of NV accumulator rows allow.  Partials
-// are reduced in a fixed order.  Block columns from cols1 on compute a second,
-// independent set of the same kind (P2, B2, nb2 into partial2): two sets in
-// one launch.
-template <int NV, int KW>
-__global__ __launch_bounds__(256) void k_dots_multi(int64_t n, const double* __restrict__ P,
-                                                    const double* __restrict__ B, int nb, double* partial,
-                                                    const int* __restrict__ need, int cols1,
-                                                    const double* __restrict__ P2, const double* __restrict__ B2,
-                                                    int nb2, double* partial2) {
-  __shared__ double red[4][NV * KW];
-  if (need && *need == 0) return;
-  int by = blockIdx.y;
-  if (by >= cols1) {
-    P = P2;
-    B = B2;
-    nb = nb2;
-    partial = partial2;
-    by -= cols1;
-  }
-  const int k0 = by * KW;
-  const int nk = min(KW, nb - k0);
-  double acc[NV][KW];
-#pragma unroll
-  for (int j = 0; j < NV; j++)
-#pragma unroll
-    for (int q = 0; q < KW; q++) acc[j][q] = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double b[KW];
-#pragma unroll
-    for (int q = 0; q < KW; q++) b[q] = q < nk ? B[(int64_t)(k0 + q) * n + i] : 0.0;
-#pragma unroll
-    for (int j = 0; j < NV; j++) {
-      const double x = P[(int64_t)j * n + i];
-#pragma unroll
-      for (int q = 0; q < KW; q++) acc[j][q] += x * b[q];
-    }
-  }
-  // every wave reduces its sums, then one barrier; wave sums added in a fixed order
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int j = 0; j < NV; j++)
-#pragma unroll
-    for (int q = 0; q < KW; q++)
-      if (q < nk) {
-        const double v = wave_sum(acc[j][q]);
-        if (lane == 0) red[wave][j * KW + q] = v;
-      }
-  __syncthreads();
-  for (int t = threadIdx.x; t < NV * KW; t += blockDim.x) {
-    const int j = t / KW, q = t % KW;
-    if (q < nk)
-      partial[((int64_t)j + (int64_t)(k0 + q) * NV) * kRedBlocks + blockIdx.x] =
-          (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-  }
-}
-
 typedef double dots_d4_t __attribute__((ext_vector_type(4)));
 
-// The same dots on the FP64 matrix cores: a block column covers kDotsMB = 32
-// basis vectors (two 16-row MFMA tiles), so the P vectors are read once per 32
-// basis vectors instead of once per KW.  Per step a block stages a 64-element
+// Block dot products C[j + k*nv] = P_j . B_k of NV (<= 8) vectors P (stride n)
+// against nb basis vectors B (stride n) on the FP64 matrix cores.  A block
+// column covers kDotsMB = 32 basis vectors (two 16-row MFMA tiles): each basis
+// vector is read once and the P vectors once per 32 basis vectors (a VALU
+// version holding NV x 8 sums a lane read them once per 8, 0.6x the speed at
+// 5 vectors and 35 basis vectors).  Per step a block stages a 64-element
 // slice of its 32 basis vectors and of the NV P vectors in LDS (loaded a row
 // per instruction, 512 contiguous bytes), and each wave multiplies 16 of the 64
 // elements with v_mfma_f64_16x16x4f64: A = basis (row m = lane & 15, k =
@@ -1685,8 +1628,11 @@ typedef double dots_d4_t __attribute__((ext_vector_type(4)));
 // 16 x 16 result in 4 registers a lane (row (lane >> 4) + 4 r, column lane & 15).
 // LDS rows are 66 doubles apart, which makes the fragment reads and the row
 // writes free of bank conflicts; the staging is double buffered with one barrier
-// per step.  The four waves' results are added in a fixed order; the partials
-// go out as k_dots_multi writes them.  Grid (kRedBlocks, columns of set 1 + set 2).
+// per step.  The four waves' results are added in a fixed order.  Partials:
+// row j + k*NV, kRedBlocks a row (block x of the grid in slot x).  Block
+// columns from cols1 on compute a second, independent set of the same kind
+// (P2, B2, nb2 into partial2): two sets in one launch.  Grid (kRedBlocks,
+// columns of set 1 + set 2).
 constexpr int kDotsMB = 32;
 constexpr int kDotsTE = 64;
 constexpr int kDotsLd = kDotsTE + 2;
@@ -1821,71 +1767,65 @@ hipError_t launch_dots_pair(int64_t n, int nv, const double* P1, const double* B
 // the partials of the Gram matrix of the new P_j (rows j + k*nv, j <= k, as
 // k_tri_apply writes them).  need given and *need == 0: nothing is done.
 // Grid: kRedBlocks blocks of 256.
+template <int NV>
 __global__ __launch_bounds__(256) void k_cgs_update(int64_t n, const double* __restrict__ B, int nb,
-                                                    const double* __restrict__ C, int nv, double* __restrict__ P,
+                                                    const double* __restrict__ C, double* __restrict__ P,
                                                     const int* __restrict__ need, const double* pn2, MgsSrc src,
                                                     double* __restrict__ gram) {
-  __shared__ double red[4][kMgsMulti * kMgsMulti];
+  constexpr int KU = 8;  // basis vectors loaded together (in flight) per element
+  __shared__ double red[4][NV * NV];
   if (need && *need == 0) return;
-  double f[kMgsMulti];
-  int jc[kMgsMulti];
+  double f[NV];
 #pragma unroll
-  for (int j = 0; j < kMgsMulti; j++) {
-    f[j] = (pn2 && j < nv) ? 1.0 / sqrt(pn2[src.src[j]]) : 1.0;
-    jc[j] = min(j, nv - 1);  // rows past nv repeat the last one (computed, never stored)
-  }
-  double g[kMgsMulti][kMgsMulti];
+  for (int j = 0; j < NV; j++) f[j] = pn2 ? 1.0 / sqrt(pn2[src.src[j]]) : 1.0;
+  double g[NV][NV];
 #pragma unroll
-  for (int j = 0; j < kMgsMulti; j++)
+  for (int j = 0; j < NV; j++)
 #pragma unroll
-    for (int k = 0; k < kMgsMulti; k++) g[j][k] = 0.0;
+    for (int k = 0; k < NV; k++) g[j][k] = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double s[kMgsMulti];
+    double s[NV];
 #pragma unroll
-    for (int j = 0; j < kMgsMulti; j++) s[j] = 0.0;
+    for (int j = 0; j < NV; j++) s[j] = 0.0;
     int k = 0;
-    for (; k + 4 <= nb; k += 4) {
-      double b[4];
+    for (; k + KU <= nb; k += KU) {
+      double b[KU];
 #pragma unroll
-      for (int u = 0; u < 4; u++) b[u] = B[(int64_t)(k + u) * n + i];
+      for (int u = 0; u < KU; u++) b[u] = B[(int64_t)(k + u) * n + i];
 #pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int u = 0; u < KU; u++)
 #pragma unroll
-        for (int j = 0; j < kMgsMulti; j++) s[j] += C[jc[j] + (int64_t)(k + u) * nv] * b[u];
+        for (int j = 0; j < NV; j++) s[j] += C[j + (int64_t)(k + u) * NV] * b[u];
     }
     for (; k < nb; k++) {
       const double b = B[(int64_t)k * n + i];
 #pragma unroll
-      for (int j = 0; j < kMgsMulti; j++) s[j] += C[jc[j] + (int64_t)k * nv] * b;
+      for (int j = 0; j < NV; j++) s[j] += C[j + (int64_t)k * NV] * b;
     }
-    double x[kMgsMulti];
+    double x[NV];
 #pragma unroll
-    for (int j = 0; j < kMgsMulti; j++) {
-      x[j] = 0.0;
-      if (j < nv) {
-        x[j] = (P[(int64_t)j * n + i] - s[j]) * f[j];
-        P[(int64_t)j * n + i] = x[j];
-      }
+    for (int j = 0; j < NV; j++) {
+      x[j] = (P[(int64_t)j * n + i] - s[j]) * f[j];
+      P[(int64_t)j * n + i] = x[j];
     }
 #pragma unroll
-    for (int k2 = 0; k2 < kMgsMulti; k2++)
+    for (int k2 = 0; k2 < NV; k2++)
 #pragma unroll
       for (int j = 0; j <= k2; j++) g[j][k2] += x[j] * x[k2];
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-  for (int k = 0; k < kMgsMulti; k++)
+  for (int k = 0; k < NV; k++)
 #pragma unroll
-    for (int j = 0; j <= k; j++)
-      if (k < nv) {
-        const double v = wave_sum(g[j][k]);
-        if (lane == 0) red[wave][j + k * kMgsMulti] = v;
-      }
+    for (int j = 0; j <= k; j++) {
+      const double v = wave_sum(g[j][k]);
+      if (lane == 0) red[wave][j + k * NV] = v;
+    }
   __syncthreads();
-  for (int t = threadIdx.x; t < kMgsMulti * kMgsMulti; t += blockDim.x) {
-    const int j = t % kMgsMulti, k = t / kMgsMulti;
-    if (j <= k && k < nv)
-      gram[((int64_t)j + (int64_t)k * nv) * kRedBlocks + blockIdx.x] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+  for (int t = threadIdx.x; t < NV * NV; t += blockDim.x) {
+    const int j = t % NV, k = t / NV;
+    if (j <= k)
+      gram[((int64_t)j + (int64_t)k * NV) * kRedBlocks + blockIdx.x] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
   }
 }
 
@@ -1894,7 +1834,21 @@ hipError_t launch_cgs_update(int64_t n, const double* B, int nb, const double* C
   if (nv <= 0 || nv > kMgsMulti || nb < 0 || (nb > 0 && !C) || !gram) return hipErrorInvalidValue;
   for (int j = 0; pn2 && j < nv; j++)
     if (src.src[j] < 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_cgs_update, dim3(kRedBlocks), dim3(256), 0, s, n, B, nb, C, nv, P, need, pn2, src, gram);
+  static_assert(kMgsMulti == 8, "one instantiation per vector count");
+#define DPD_CGS(NV_)                                                                                               \
+  hipLaunchKernelGGL(k_cgs_update<NV_>, dim3(kRedBlocks), dim3(256), 0, s, n, B, nb, C, P, need, pn2, src, gram); \
+  break;
+  switch (nv) {
+    case 1: DPD_CGS(1)
+    case 2: DPD_CGS(2)
+    case 3: DPD_CGS(3)
+    case 4: DPD_CGS(4)
+    case 5: DPD_CGS(5)
+    case 6: DPD_CGS(6)
+    case 7: DPD_CGS(7)
+    default: DPD_CGS(8)
+  }
+#undef DPD_CGS
   return hipGetLastError();
 }
 
@@ -1936,8 +1890,8 @@ hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream
 // Y_j = sum_{i<=j} T[i + j*nv] X_i for j < nv <= kMgsMulti (T == nullptr: Y = X;
 // Y == nullptr: nothing stored; Y may be X: each thread reads its elements of
 // every X_i before it writes); with partial given, also the partials of the
-// Gram matrix Y_j . Y_k, j <= k, in rows j + k*nv of kRedBlocks as
-// k_dots_multi writes them.  Grid: kRedBlocks blocks of 256.
+// Gram matrix Y_j . Y_k, j <= k, in rows j + k*nv of kRedBlocks (block x in
+// slot x).  Grid: kRedBlocks blocks of 256.
 __global__ __launch_bounds__(256) void k_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y,
                                                    double* partial) {
   __shared__ double red[4][kMgsMulti * kMgsMulti];
