@@ -438,7 +438,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       (rc = grow(W.amaxi, sizeof(int64_t) * nroot)) || (rc = grow(W.wdev, sizeof(double) * nroot)) ||
       (rc = grow(W.idx, sizeof(int32_t) * nroot)) || (rc = grow(W.cgs, 256)) ||
       (rc = grow(W.tri, sizeof(double) * (2 * kMgsMulti * kMgsMulti + kMgsMulti))) ||
-      (rc = grow(W.gram, sizeof(double) * kRedBlocks * kMgsMulti * kMgsMulti)) ||
+      (rc = grow(W.gram, sizeof(double) * 2 * kRedBlocks * kMgsMulti * kMgsMulti)) ||
       (rc = grow(W.qpart, sizeof(double) * kRedBlocks * nroot)) ||
       (rc = grow(W.ppart, sizeof(double) * kRedBlocks * nroot)) || (rc = grow(W.rstat, sizeof(double) * 3 * nroot)))
     return rc;
@@ -626,10 +626,11 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
     // for the unconverged ones, and |q|^2 and |p|^2.  The renormalisation of q
     // and Aq (:378-386, by 1/|q| when ||q| - 1| > 1e-10) is applied afterwards:
     // e and max|e| scale with it (the host scales max|e|), the directions are
-    // normalised anyway, and q (the eigenvector output R) is rescaled on exit
+    // normalised anyway, and q (the eigenvector output R) is rescaled on exit.
+    // e itself is not stored: the rare redo below forms it again from q and Aq
     RootVals wv;
     for (int v = 0; v < nroot; v++) wv.w[v] = eigenVal[v];
-    DPD_HIP_TRY(launch_ritz_resid_part(n, wv, diag, R, AQ, E, PV, nroot, (double*)W.amaxp.p, (int64_t*)W.amaxpi.p,
+    DPD_HIP_TRY(launch_ritz_resid_part(n, wv, diag, R, AQ, PV, nroot, (double*)W.amaxp.p, (int64_t*)W.amaxpi.p,
                                        (double*)W.qpart.p, (double*)W.ppart.p, rstat, (int64_t*)W.amaxi.p, st));
     bool roots_on_device = false;  // eigenvalues and candidates uploaded (the rare redo path needs them)
     if ((rc = fetch(mv.data(), rstat, sizeof(double) * 2 * nroot))) return rc;
@@ -734,8 +735,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
       // among themselves by Cholesky QR applied twice -- in exact arithmetic the
       // vectors and remaining norms of the sequential Gram-Schmidt
       double* T1 = (double*)W.tri.p;
-      double* T2 = T1 + kMgsMulti * kMgsMulti;
-      double* piv = T2 + kMgsMulti * kMgsMulti;
+      double* piv = T1 + 2 * kMgsMulti * kMgsMulti;
       for (size_t c0 = 0; c0 < nopt; c0 += kMgsMulti) {
         const int nv = (int)std::min<size_t>(kMgsMulti, nopt - c0);
         double* P0 = PV + c0 * n;
@@ -766,15 +766,26 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
         // norms (a direction below 1e-6 zeroes itself and the ones after it).
         // A second pass on Q restores the orthogonality the first loses to
         // the conditioning of G; its squared pivots times the first ones' are
-        // the norms the host checks (dots[c0 + j])
+        // the norms the host checks (dots[c0 + j]).  Each factor is taken by
+        // the launch that applies it (every block factors the small Gram
+        // matrix itself); Q1 = P T1 is formed only for its Gram matrix, and
+        // the new basis vectors are P (T1 T2)
+        TriChol c1;
+        c1.gpart = gram;
         if (nv == 1) {
-          DPD_HIP_TRY(launch_chol_tri(gram, nv, nullptr, T1, nullptr, dots + c0, st));
-          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T1, bv(dimIter + (int)c0), nullptr, st));
+          c1.nrm2_out = dots + c0;
+          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, nullptr, bv(dimIter + (int)c0), nullptr, st, c1));
         } else {
-          DPD_HIP_TRY(launch_chol_tri(gram, nv, nullptr, T1, piv, nullptr, st));
-          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T1, P0, gram, st));
-          DPD_HIP_TRY(launch_chol_tri(gram, nv, piv, T2, nullptr, dots + c0, st));
-          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, T2, bv(dimIter + (int)c0), nullptr, st));
+          double* gram2 = gram + (size_t)kRedBlocks * kMgsMulti * kMgsMulti;
+          c1.piv_out = piv;
+          c1.T_out = T1;
+          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, nullptr, nullptr, gram2, st, c1));
+          TriChol c2;
+          c2.gpart = gram2;
+          c2.piv_in = piv;
+          c2.T_pre = T1;
+          c2.nrm2_out = dots + c0;
+          DPD_HIP_TRY(launch_tri_apply(n, P0, nv, nullptr, bv(dimIter + (int)c0), nullptr, st, c2));
         }
       }
       spec = nopt;
@@ -809,7 +820,7 @@ static int davidson_run(dpd_problem* P, dpd_davidson_opts* o, double* R, double*
               return rc;
             roots_on_device = true;
           }
-          DPD_HIP_TRY(launch_precond_batched(n, wdev, (const int32_t*)W.idx.p + k, 1, diag, E, PV + k * n, st));
+          DPD_HIP_TRY(launch_precond_ritz(n, wdev, (const int32_t*)W.idx.p + k, 1, diag, AQ, R, PV + k * n, st));
           if ((rc = schmidt(k, nNewBase, dots))) return rc;
           double t2;
           if ((rc = fetch(&t2, dots, sizeof(double)))) return rc;

@@ -1494,15 +1494,15 @@ __global__ __launch_bounds__(256) void k_resid_amax_partial(int64_t n, const dou
 }
 
 // Residual and preconditioned direction of every root in one pass
-// (lib_cc/mod_davidson.f90:396-406, :417-428): e = Aq - w q into E,
+// (lib_cc/mod_davidson.f90:396-406, :417-428): e = Aq - w q (not stored),
 // p = e / (w - d) (|w - d| clamped to 1e-4) into P, the largest |e| per root,
 // and the partial sums of |q|^2 and |p|^2 (kRedBlocks per root).  q and Aq
 // are taken as they are: their renormalisation (:378-386) is applied by the
 // caller afterwards (e, p and max|e| are linear in it).
 __global__ __launch_bounds__(256) void k_ritz_resid(int64_t n, RootVals w, const double* __restrict__ d,
                                                     const double* __restrict__ q, const double* __restrict__ aq,
-                                                    double* __restrict__ e, double* __restrict__ p, double* pv,
-                                                    int64_t* pi, double* qpart, double* ppart) {
+                                                    double* __restrict__ p, double* pv, int64_t* pi, double* qpart,
+                                                    double* ppart) {
   __shared__ double sv[256];
   __shared__ int64_t si[256];
   __shared__ double red[4];
@@ -1510,14 +1510,12 @@ __global__ __launch_bounds__(256) void k_ritz_resid(int64_t n, RootVals w, const
   const double wv = w.w[v];
   const double* qv = q + (int64_t)v * n;
   const double* aqv = aq + (int64_t)v * n;
-  double* ev = e + (int64_t)v * n;
   double* pvv = p + (int64_t)v * n;
   double best = 0.0, sq = 0.0, sp = 0.0;
   int64_t bi = INT64_MAX;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const double x = qv[i], y = aqv[i];
     const double r = y + (-wv) * x;
-    ev[i] = r;
     double tt = wv - d[i];
     if (fabs(tt) < 1.0e-4) tt = (tt < 0.0) ? -1.0e-4 : 1.0e-4;
     const double pp = r / tt;
@@ -1570,13 +1568,12 @@ __global__ __launch_bounds__(256) void k_ritz_final(const double* pv, const int6
 }
 
 hipError_t launch_ritz_resid_part(int64_t n, const RootVals& w, const double* d, const double* q, const double* aq,
-                                  double* e, double* p, int nv, double* pv, int64_t* pi, double* qpart,
-                                  double* ppart, double* stat, int64_t* oi, hipStream_t s) {
+                                  double* p, int nv, double* pv, int64_t* pi, double* qpart, double* ppart,
+                                  double* stat, int64_t* oi, hipStream_t s) {
   if (nv == 0) return hipSuccess;
   if (nv > kRootVals) return hipErrorInvalidValue;
   static_assert(kRedBlocks == 512, "k_ritz_final sums two partials per thread");
-  hipLaunchKernelGGL(k_ritz_resid, dim3(kRedBlocks, nv), dim3(256), 0, s, n, w, d, q, aq, e, p, pv, pi, qpart,
-                     ppart);
+  hipLaunchKernelGGL(k_ritz_resid, dim3(kRedBlocks, nv), dim3(256), 0, s, n, w, d, q, aq, p, pv, pi, qpart, ppart);
   hipLaunchKernelGGL(k_ritz_final, dim3(nv), dim3(256), 0, s, pv, pi, qpart, ppart, stat, oi);
   return hipGetLastError();
 }
@@ -1887,72 +1884,6 @@ hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream
   return hipGetLastError();
 }
 
-// Y_j = sum_{i<=j} T[i + j*nv] X_i for j < nv <= kMgsMulti (T == nullptr: Y = X;
-// Y == nullptr: nothing stored; Y may be X: each thread reads its elements of
-// every X_i before it writes); with partial given, also the partials of the
-// Gram matrix Y_j . Y_k, j <= k, in rows j + k*nv of kRedBlocks (block x in
-// slot x).  Grid: kRedBlocks blocks of 256.
-__global__ __launch_bounds__(256) void k_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y,
-                                                   double* partial) {
-  __shared__ double red[4][kMgsMulti * kMgsMulti];
-  double t[kMgsMulti][kMgsMulti];
-#pragma unroll
-  for (int j = 0; j < kMgsMulti; j++)
-#pragma unroll
-    for (int i = 0; i < kMgsMulti; i++) t[i][j] = (i <= j && j < nv) ? (T ? T[i + j * nv] : (i == j ? 1.0 : 0.0)) : 0.0;
-  double g[kMgsMulti][kMgsMulti];
-#pragma unroll
-  for (int j = 0; j < kMgsMulti; j++)
-#pragma unroll
-    for (int k = 0; k < kMgsMulti; k++) g[j][k] = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    double x[kMgsMulti], y[kMgsMulti];
-#pragma unroll
-    for (int i = 0; i < kMgsMulti; i++) x[i] = i < nv ? X[(int64_t)i * n + e] : 0.0;
-#pragma unroll
-    for (int j = 0; j < kMgsMulti; j++) {
-      double s = 0.0;
-#pragma unroll
-      for (int i = 0; i <= j; i++) s += t[i][j] * x[i];
-      y[j] = s;
-    }
-    if (Y) {
-#pragma unroll
-      for (int j = 0; j < kMgsMulti; j++)
-        if (j < nv) Y[(int64_t)j * n + e] = y[j];
-    }
-    if (partial) {
-#pragma unroll
-      for (int k = 0; k < kMgsMulti; k++)
-#pragma unroll
-        for (int j = 0; j <= k; j++) g[j][k] += y[j] * y[k];
-    }
-  }
-  if (!partial) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 0; k < kMgsMulti; k++)
-#pragma unroll
-    for (int j = 0; j <= k; j++)
-      if (k < nv) {
-        const double v = wave_sum(g[j][k]);
-        if (lane == 0) red[wave][j + k * kMgsMulti] = v;
-      }
-  __syncthreads();
-  for (int t = threadIdx.x; t < kMgsMulti * kMgsMulti; t += blockDim.x) {
-    const int j = t % kMgsMulti, k = t / kMgsMulti;
-    if (j <= k && k < nv)
-      partial[((int64_t)j + (int64_t)k * nv) * kRedBlocks + blockIdx.x] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-  }
-}
-
-hipError_t launch_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y, double* partial,
-                            hipStream_t s) {
-  if (nv <= 0 || nv > kMgsMulti) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_tri_apply, dim3(kRedBlocks), dim3(256), 0, s, n, X, nv, T, Y, partial);
-  return hipGetLastError();
-}
-
 // Cholesky QR step for nv <= kMgsMulti vectors from the partials of their Gram
 // matrix G (rows j + k*nv, j <= k): G = L L^T and T = L^-T (upper triangular,
 // [i + j*nv]), so that Q = P T has orthonormal columns and L_jj is the norm of
@@ -1961,9 +1892,10 @@ hipError_t launch_tri_apply(int64_t n, const double* X, int nv, const double* T,
 // first j whose remaining norm, times piv_in[j] when given (the pivot of an
 // earlier pass), is below 1e-6 or not a number; columns j.. of T are zero.
 // piv_out[j] (if given) = that combined norm, nrm2_out[j] (if given) = its
-// square, both 0 from the stopping column on.  One block of 256.
-__global__ __launch_bounds__(256) void k_chol_tri(const double* partial, int nv, const double* piv_in, double* T,
-                                                  double* piv_out, double* nrm2_out) {
+// square, both 0 from the stopping column on.  Run by all 256 threads of a
+// block; T ([i + j*nv]) lands in Ts (LDS, visible to all threads on return).
+__device__ inline void chol_tri_block(const double* partial, int nv, const double* piv_in, double* Ts,
+                                      double* piv_out, double* nrm2_out) {
   // the nv(nv+1)/2 Gram rows are dealt round-robin to the 4 waves; every load
   // of a wave is issued before the first sum (one memory latency, not one per row)
   constexpr int kRowsPerWave = (kMgsMulti * (kMgsMulti + 1) / 2 + 3) / 4;
@@ -1999,7 +1931,7 @@ __global__ __launch_bounds__(256) void k_chol_tri(const double* partial, int nv,
     }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x == 0) {
   double G[kMgsMulti][kMgsMulti];
 #pragma unroll
   for (int i = 0; i < kMgsMulti; i++)
@@ -2064,20 +1996,105 @@ __global__ __launch_bounds__(256) void k_chol_tri(const double* partial, int nv,
     if (j < nv) {
 #pragma unroll
       for (int i = 0; i < kMgsMulti; i++)
-        if (i < nv) T[i + j * nv] = (j < nvalid && i <= j) ? Li[j][i] : 0.0;
+        if (i < nv) Ts[i + j * nv] = (j < nvalid && i <= j) ? Li[j][i] : 0.0;
       const double pv = j < nvalid ? L[j][j] * f[j] : 0.0;
       if (piv_out) piv_out[j] = pv;
       if (nrm2_out) nrm2_out[j] = pv * pv;
     }
   }
+  }
+  __syncthreads();
 }
 
-hipError_t launch_chol_tri(const double* partial, int nv, const double* piv_in, double* T, double* piv_out,
-                           double* nrm2_out, hipStream_t s) {
+// Y_j = sum_{i<=j} T[i + j*nv] X_i for j < nv <= kMgsMulti (T == nullptr: Y = X;
+// Y == nullptr: nothing stored; Y may be X: each thread reads its elements of
+// every X_i before it writes); with partial given, also the partials of the
+// Gram matrix Y_j . Y_k, j <= k, in rows j + k*nv of kRedBlocks (block x in
+// slot x).  With ch.gpart given, T is not read: every block first takes the
+// Cholesky QR step of chol_tri_block on those Gram partials (with ch.piv_in),
+// and T is that factor, times ch.T_pre on the left when given (a product of
+// upper triangular factors); block 0 writes the factor alone to ch.T_out and
+// the norms to ch.piv_out / ch.nrm2_out.  Grid: kRedBlocks blocks of 256.
+__global__ __launch_bounds__(256) void k_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y,
+                                                   double* partial, TriChol ch) {
+  __shared__ double red[4][kMgsMulti * kMgsMulti];
+  __shared__ double Ts[kMgsMulti * kMgsMulti], Te[kMgsMulti * kMgsMulti];
+  if (ch.gpart) {
+    const bool b0 = blockIdx.x == 0;
+    chol_tri_block(ch.gpart, nv, ch.piv_in, Ts, b0 ? ch.piv_out : nullptr, b0 ? ch.nrm2_out : nullptr);
+    const int q = threadIdx.x;
+    if (q < nv * nv) {
+      const int i = q % nv, j = q / nv;
+      double v = 0.0;
+      if (!ch.T_pre) {
+        v = Ts[q];
+      } else if (i <= j) {
+        for (int k = i; k <= j; k++) v += ch.T_pre[i + k * nv] * Ts[k + j * nv];
+      }
+      Te[q] = v;
+      if (b0 && ch.T_out) ch.T_out[q] = Ts[q];
+    }
+    __syncthreads();
+    T = Te;
+  }
+  double t[kMgsMulti][kMgsMulti];
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++)
+#pragma unroll
+    for (int i = 0; i < kMgsMulti; i++) t[i][j] = (i <= j && j < nv) ? (T ? T[i + j * nv] : (i == j ? 1.0 : 0.0)) : 0.0;
+  double g[kMgsMulti][kMgsMulti];
+#pragma unroll
+  for (int j = 0; j < kMgsMulti; j++)
+#pragma unroll
+    for (int k = 0; k < kMgsMulti; k++) g[j][k] = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double x[kMgsMulti], y[kMgsMulti];
+#pragma unroll
+    for (int i = 0; i < kMgsMulti; i++) x[i] = i < nv ? X[(int64_t)i * n + e] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kMgsMulti; j++) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i <= j; i++) s += t[i][j] * x[i];
+      y[j] = s;
+    }
+    if (Y) {
+#pragma unroll
+      for (int j = 0; j < kMgsMulti; j++)
+        if (j < nv) Y[(int64_t)j * n + e] = y[j];
+    }
+    if (partial) {
+#pragma unroll
+      for (int k = 0; k < kMgsMulti; k++)
+#pragma unroll
+        for (int j = 0; j <= k; j++) g[j][k] += y[j] * y[k];
+    }
+  }
+  if (!partial) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < kMgsMulti; k++)
+#pragma unroll
+    for (int j = 0; j <= k; j++)
+      if (k < nv) {
+        const double v = wave_sum(g[j][k]);
+        if (lane == 0) red[wave][j + k * kMgsMulti] = v;
+      }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kMgsMulti * kMgsMulti; t += blockDim.x) {
+    const int j = t % kMgsMulti, k = t / kMgsMulti;
+    if (j <= k && k < nv)
+      partial[((int64_t)j + (int64_t)k * nv) * kRedBlocks + blockIdx.x] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+  }
+}
+
+hipError_t launch_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y, double* partial,
+                            hipStream_t s, const TriChol& ch) {
   if (nv <= 0 || nv > kMgsMulti) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_chol_tri, dim3(1), dim3(256), 0, s, partial, nv, piv_in, T, piv_out, nrm2_out);
+  hipLaunchKernelGGL(k_tri_apply, dim3(kRedBlocks), dim3(256), 0, s, n, X, nv, T, Y, partial, ch);
   return hipGetLastError();
 }
+
 
 __global__ __launch_bounds__(256) void k_scale_by_norm(int64_t n, const double* __restrict__ p, const double* part,
                                                        double* __restrict__ dst) {
@@ -2104,6 +2121,29 @@ __global__ void k_precond_batched(int64_t n, const double* w, const int32_t* idx
     if (fabs(t) < 1.0e-4) t = (t < 0.0) ? -1.0e-4 : 1.0e-4;
     pv[i] = ev[i] / t;
   }
+}
+
+// p = (aq_v - w_v q_v) / clamp(w_v - d) for the root v = idx[k] of each output k
+__global__ void k_precond_ritz(int64_t n, const double* w, const int32_t* idx, const double* __restrict__ d,
+                               const double* __restrict__ aq, const double* __restrict__ q, double* __restrict__ p) {
+  const int v = idx[blockIdx.y];
+  const double wv = w[v];
+  const double* aqv = aq + (int64_t)v * n;
+  const double* qv = q + (int64_t)v * n;
+  double* pv = p + (int64_t)blockIdx.y * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double r = aqv[i] + (-wv) * qv[i];
+    double t = wv - d[i];
+    if (fabs(t) < 1.0e-4) t = (t < 0.0) ? -1.0e-4 : 1.0e-4;
+    pv[i] = r / t;
+  }
+}
+
+hipError_t launch_precond_ritz(int64_t n, const double* w, const int32_t* idx, int nv, const double* d,
+                               const double* aq, const double* q, double* p, hipStream_t s) {
+  if (nv == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_precond_ritz, dim3(grid_for(n), nv), dim3(256), 0, s, n, w, idx, d, aq, q, p);
+  return hipGetLastError();
 }
 
 hipError_t launch_precond_batched(int64_t n, const double* w, const int32_t* idx, int nv, const double* d,

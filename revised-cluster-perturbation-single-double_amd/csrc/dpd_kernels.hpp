@@ -329,15 +329,15 @@ struct RowList {
   int32_t src[kRootVals];
   int n;
 };
-// residuals e_v = Aq_v - w_v q_v (E) and preconditioned directions (P) of every
-// root in one pass (nv <= kRootVals), and in one more launch stat[v] = max|e_v|
+// preconditioned directions P of the residuals e_v = Aq_v - w_v q_v (e not
+// stored) of every root in one pass (nv <= kRootVals), and in one more launch stat[v] = max|e_v|
 // (first index oi[v]), stat[nv + v] = |q_v|^2 and stat[2 nv + v] = |p_v|^2, each
 // summed in a fixed order (qpart, ppart: kRedBlocks * nv partials).  q and Aq
 // are used as they are: e, p and max|e| scale with 1/|q| when the caller
 // renormalises q.
 hipError_t launch_ritz_resid_part(int64_t n, const RootVals& w, const double* d, const double* q, const double* aq,
-                                  double* e, double* p, int nv, double* pv, int64_t* pi, double* qpart,
-                                  double* ppart, double* stat, int64_t* oi, hipStream_t s);
+                                  double* p, int nv, double* pv, int64_t* pi, double* qpart, double* ppart,
+                                  double* stat, int64_t* oi, hipStream_t s);
 // launch_lincomb of two sets (X -> out and X2 -> out2) with the same coefficients
 hipError_t launch_lincomb2(int64_t n, const double* X, const double* X2, int64_t xs, int nk, const double* coef,
                            int ldc, int nout, double* out, double* out2, hipStream_t s);
@@ -384,18 +384,31 @@ hipError_t launch_cgs_update(int64_t n, const double* B, int nb, const double* C
 // pn2 given: the vectors P_j are unnormalised, |P_j|^2 = pn2[src[j]]
 hipError_t launch_cgs_need(const double* C, int nv, int nb, int* need, hipStream_t s,
                            const double* pn2 = nullptr, const MgsSrc& src = MgsSrc{});
+// Cholesky QR step taken inside launch_tri_apply: from the Gram partials gpart
+// (rows j + k*nv, j <= k) of nv vectors, G = L L^T and T = L^-T, stopping at
+// the first remaining norm (times piv_in[j] when given) below 1e-6 (columns
+// from there on zero); piv_out / nrm2_out: the norms and their squares (0 from
+// the stopping column on); T_out: T; T_pre given: the vectors are multiplied
+// by T_pre T instead of T
+struct TriChol {
+  const double* gpart = nullptr;
+  const double* piv_in = nullptr;
+  const double* T_pre = nullptr;
+  double* piv_out = nullptr;
+  double* nrm2_out = nullptr;
+  double* T_out = nullptr;
+};
 // Y_j = sum_{i<=j} T[i + j*nv] X_i (T null: identity; Y null: not stored; Y may
-// be X) and, with partial given, the Gram partials of Y (rows j + k*nv, j <= k)
+// be X) and, with partial given, the Gram partials of Y (rows j + k*nv, j <= k);
+// with ch.gpart given T is the factor of that Cholesky QR step instead
 hipError_t launch_tri_apply(int64_t n, const double* X, int nv, const double* T, double* Y, double* partial,
-                            hipStream_t s);
-// Cholesky QR step from those Gram partials: T = L^-T, stopping at the first
-// remaining norm (times piv_in) below 1e-6; piv_out / nrm2_out: the norms and
-// their squares (0 from the stopping column on)
-hipError_t launch_chol_tri(const double* partial, int nv, const double* piv_in, double* T, double* piv_out,
-                           double* nrm2_out, hipStream_t s);
+                            hipStream_t s, const TriChol& ch = TriChol{});
 // dst = p * (1/sqrt(sum(part))) ; returns nothing (norm read by the host separately)
 hipError_t launch_scale_by_norm(int64_t n, const double* p, const double* part, double* dst, hipStream_t s);
 // p = e / clamp(w - d) for the roots listed in idx (w on device)
 hipError_t launch_precond_batched(int64_t n, const double* w, const int32_t* idx, int nv, const double* d,
                                   const double* e, double* p, hipStream_t s);
+// the same from the Ritz pair: e = aq_v - w_v q_v formed on the fly
+hipError_t launch_precond_ritz(int64_t n, const double* w, const int32_t* idx, int nv, const double* d,
+                               const double* aq, const double* q, double* p, hipStream_t s);
 }  // namespace dpd
