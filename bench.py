@@ -940,7 +940,7 @@ KERNEL_CLASSES = [
     ("operand packing", ("k_pack",)),
     ("gathers (sorts, expansions, epilogues)", ("k_gather", "k_epi_fused", "k_sort")),
     ("Davidson vector algebra", ("k_multidot", "k_reduce_rows", "k_lincomb", "k_dots_", "k_cgs_need",
-                                 "k_tri_apply", "k_chol_tri", "k_mgs", "k_pair_dots", "k_renorm_pair",
+                                 "k_tri_apply", "k_mgs", "k_pair_dots", "k_renorm_pair",
                                  "k_resid", "k_amax", "k_precond_batched", "k_scale_by_norm", "k_block_dots",
                                  "k_davidson", "k_ritz_resid", "k_ritz_final", "k_cgs_update",
                                  "k_compact_rows")),
