@@ -241,10 +241,15 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
 }
 
 // panels from precomputed source codes: one thread per element of the hi (or
-// only) plane, panels back to back (panel pid at pid * pan_elems * planes)
+// only) plane, panels back to back (panel pid at pid * pan_elems * planes).
+// With dm != 0 (total < 2^31) the panel of element f is found by multiplying
+// with a reciprocal, q = (umulhi(f, dm) + f) >> dl (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", 1994), instead of a
+// 64-bit division.
 template <int MODE, typename TIN>
 __global__ __launch_bounds__(256) void k_pack_code(int64_t total, int64_t pan_elems, const int32_t* __restrict__ code,
-                                                   const TIN* __restrict__ src, void* __restrict__ dst) {
+                                                   const TIN* __restrict__ src, void* __restrict__ dst, uint32_t dm,
+                                                   uint32_t dl) {
   for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
     const int32_t cd = code[f];
     double v = 0.0;
@@ -255,7 +260,14 @@ __global__ __launch_bounds__(256) void k_pack_code(int64_t total, int64_t pan_el
     } else if (MODE == PREC_F32) {
       ((float*)dst)[f] = (float)v;
     } else {
-      const int64_t pid = f / pan_elems, e = f - pid * pan_elems;
+      int64_t pid;
+      if (dm) {
+        const uint32_t f32 = (uint32_t)f;
+        pid = (__umulhi(f32, dm) + f32) >> dl;
+      } else {
+        pid = f / pan_elems;
+      }
+      const int64_t e = f - pid * pan_elems;
       const int64_t base = pid * 2 * pan_elems + e;
       const float fv = (float)v;
       const uint16_t h = bf16_bits(fv);
@@ -273,11 +285,20 @@ hipError_t launch_pack_code(int mode, int src_f32, int64_t npanel, int64_t pan_e
   int64_t b = (total + 255) / 256;
   if (b > 16384) b = 16384;
   const dim3 grid((unsigned)b), blk(256);
-#define PKC(M)                                                                                              \
-  if (src_f32)                                                                                              \
-    hipLaunchKernelGGL((k_pack_code<M, float>), grid, blk, 0, s, total, pan_elems, code, (const float*)src, dst); \
-  else                                                                                                      \
-    hipLaunchKernelGGL((k_pack_code<M, double>), grid, blk, 0, s, total, pan_elems, code, (const double*)src, dst);
+  // the reciprocal of pan_elems (valid for dividends below 2^31)
+  uint32_t dm = 0, dl = 0;
+  if (total < ((int64_t)1 << 31) && pan_elems >= 1 && pan_elems < ((int64_t)1 << 31)) {
+    const uint64_t d = (uint64_t)pan_elems;
+    while (((uint64_t)1 << dl) < d) dl++;
+    dm = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << dl) - d)) / d + 1);
+  }
+#define PKC(M)                                                                                                   \
+  if (src_f32)                                                                                                   \
+    hipLaunchKernelGGL((k_pack_code<M, float>), grid, blk, 0, s, total, pan_elems, code, (const float*)src, dst,  \
+                       dm, dl);                                                                                  \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_pack_code<M, double>), grid, blk, 0, s, total, pan_elems, code, (const double*)src, dst, \
+                       dm, dl);
   switch (mode) {
     case PREC_F64: PKC(PREC_F64) break;
     case PREC_F32: PKC(PREC_F32) break;
